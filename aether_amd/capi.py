@@ -1,0 +1,460 @@
+"""ctypes view of the C ABI declared in include/aether_fdtd.h.
+
+The engine library is ``aether_amd/libaether_fdtd.so`` (built by
+``aether_amd/csrc/Makefile``).  :func:`engine` loads it and fails loudly when it
+is missing: there is no fallback path.  The same struct definitions and the
+:class:`Grid` wrapper also serve the test-side oracle library, whose functions
+carry the ``aeoracle_`` prefix instead of ``aefdtd_``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ENGINE_SO = os.path.join(HERE, "libaether_fdtd.so")
+
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED = 0, 1, 2, 3, 4, 5
+MODE_NORM, MODE_CUSTOM = 0, 5
+FP64, FP32 = 0, 1
+NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM = range(6)
+EX, EY, EZ, HX, HY, HZ = range(6)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class CMaterial(C.Structure):
+    _fields_ = [("eps_inf", C.c_double),
+                ("n_debye", C.c_int), ("debye", _dp),
+                ("n_drude", C.c_int), ("drude", _dp),
+                ("n_lorentz", C.c_int), ("lorentz", _dp),
+                ("n_critpoint", C.c_int), ("critpoint", _dp)]
+
+
+class CPulse(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("lambda_", "omega", "k", "kx", "ky", "kz", "n_ind", "dz", "C1",
+                 "Ex0", "Ey0", "Ez0", "Hx0", "Hy0", "Hz0")]
+
+
+class CMode(C.Structure):
+    _fields_ = [("Nt", C.c_int), ("Dx", C.c_double), ("Dy", C.c_double), ("Dz", C.c_double),
+                ("time_mod", C.c_double), ("tapering", C.c_int),
+                ("pml", C.c_int * 6), ("pad", C.c_int * 6),
+                ("kappa", C.c_double * 6), ("sigma", C.c_double * 6), ("alpha", C.c_double * 6),
+                ("Nl", C.c_int), ("lambda_min", C.c_double), ("lambda_max", C.c_double),
+                ("polarization", C.c_int), ("polar_angle", C.c_double),
+                ("n_disable", C.c_int), ("disable_fields", _ip),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("matsgrid", C.POINTER(C.c_uint32)),
+                ("n_materials", C.c_int), ("materials", C.POINTER(CMaterial)),
+                ("precision", C.c_int)]
+
+
+class CSensorReq(C.Structure):
+    _fields_ = [("box", C.c_int), ("orientation", C.c_int),
+                ("x1", C.c_int), ("x2", C.c_int), ("y1", C.c_int), ("y2", C.c_int),
+                ("z1", C.c_int), ("z2", C.c_int), ("disable", C.c_int * 6),
+                ("Nl", C.c_int), ("lambda_min", C.c_double), ("lambda_max", C.c_double)]
+
+
+class CNormalResult(C.Structure):
+    _fields_ = [("lambda_", _dp), ("R", _dp), ("T", _dp),
+                ("refx", _dp), ("refy", _dp), ("transx", _dp), ("transy", _dp),
+                ("index_sup", C.c_double), ("index_sub", C.c_double), ("Nt_run", C.c_int)]
+
+
+def _sig(lib, p):
+    """Declares argument/return types of every `p`-prefixed function."""
+    vp = C.c_void_p
+    d = C.c_double
+    i = C.c_int
+    table = {
+        "create": (i, [C.POINTER(vp), i, i, i, i, d, d, d, d, i, _ip, _ip, i]),
+        "destroy": (i, [vp]),
+        "last_error": (C.c_char_p, [vp]),
+        "layout": (i, [vp, _ip]),
+        "set_pml": (i, [vp, i, d, d, d]),
+        "set_tapering": (i, [vp, i]),
+        "set_matsgrid": (i, [vp, C.POINTER(C.c_uint32), i, i, i]),
+        "set_material": (i, [vp, C.c_uint, C.POINTER(CMaterial)]),
+        "disable_fields": (i, [vp, _ip, i]),
+        "bootstrap": (i, [vp]),
+        "update_E": (i, [vp]),
+        "update_H": (i, [vp]),
+        "reset_fields": (i, [vp]),
+        "read_field": (i, [vp, i, _dp]),
+        "write_field": (i, [vp, i, _dp]),
+        "set_plane_injection": (i, [vp, i, i, _dp, _dp, _dp, _dp]),
+        "set_tfsf": (i, [vp, vp, i, i, i, i, i, i, i, i]),
+        "add_spect_poynting": (i, [vp, i, i, i, i, i, i, i, i, _dp, _ip]),
+        "feed_sensors": (i, [vp]),
+        "sensor_poynting": (i, [vp, i, _dp]),
+        "sensor_spectra": (i, [vp, i, _dp, _ip]),
+        "add_plane_sum": (i, [vp, i, C.POINTER(CPulse), d, i, _dp, _ip]),
+        "read_plane_sum": (i, [vp, i, i, _dp]),
+        "pulse_init": (i, [C.POINTER(CPulse), d, d, d, d, d, d, d, d, d]),
+        "pulse_eval": (d, [C.POINTER(CPulse), i, d, d, d, d]),
+        "run_normal_incidence": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq),
+                                     C.POINTER(CNormalResult), _dp, _dp]),
+        "run_single_particle": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
+        # engine only
+        "profile_enable": (i, [vp, i]),
+        "profile_read": (i, [vp, C.c_char_p, _dp, C.POINTER(C.c_long)]),
+        "device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
+        "cancel": (None, []),
+        # oracle only
+        "step": (i, [vp, i]),
+        "set_threads": (None, [i]),
+        "get_threads": (i, []),
+        "material_coeffs": (i, [C.POINTER(CMaterial), d, d, d, d, _dp, _ip]),
+        "material_eval": (i, [C.POINTER(CMaterial), d, _dp, _dp]),
+        "pml_profile": (i, [i, i, i, d, d, _dp, _dp, _dp, d, _dp]),
+        "s_curve": (d, [d, d, d]),
+        "nearest_integer": (i, [d]),
+    }
+    for name, (res, args) in table.items():
+        f = getattr(lib, p + name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_ENGINE = None
+
+
+def engine():
+    """Loads the HIP engine library (raises if it has not been built)."""
+    global _ENGINE
+    if _ENGINE is None:
+        if not os.path.exists(ENGINE_SO):
+            raise RuntimeError(f"HIP engine library missing: {ENGINE_SO} (run __graft_entry__.build())")
+        _ENGINE = Backend(C.CDLL(ENGINE_SO), "aefdtd_")
+    return _ENGINE
+
+
+class FDTDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Backend:
+    """A loaded library plus its function prefix."""
+
+    def __init__(self, lib, prefix):
+        self.lib = _sig(lib, prefix)
+        self.prefix = prefix
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def has(self, name):
+        return hasattr(self.lib, self.prefix + name)
+
+    def check(self, rc, handle=None):
+        if rc != OK:
+            msg = self.fn("last_error")(handle)
+            raise FDTDError(rc, msg.decode() if msg else "")
+
+
+# --------------------------------------------------------------------------
+# Python-side descriptions of materials, modes and sensors
+# --------------------------------------------------------------------------
+@dataclass
+class Material:
+    """eps_inf plus pole lists; the FDTD-compatible part of Aether's Material."""
+    eps_inf: float = 1.0
+    debye: list = field(default_factory=list)      # (ds, t0)
+    drude: list = field(default_factory=list)      # (wd, g)
+    lorentz: list = field(default_factory=list)    # (A, O, G)
+    critpoint: list = field(default_factory=list)  # (A, O, P, G)
+
+    def to_c(self, keep):
+        def arr(v, w):
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1, w) if len(v) else np.zeros((0, w)))
+            keep.append(a)
+            return len(a), a.ctypes.data_as(_dp)
+        m = CMaterial()
+        m.eps_inf = self.eps_inf
+        m.n_debye, m.debye = arr(self.debye, 2)
+        m.n_drude, m.drude = arr(self.drude, 2)
+        m.n_lorentz, m.lorentz = arr(self.lorentz, 3)
+        m.n_critpoint, m.critpoint = arr(self.critpoint, 4)
+        return m
+
+
+@dataclass
+class SensorReq:
+    box: bool = True
+    orientation: int = NORMAL_Z
+    x1: int = 0
+    x2: int = 0
+    y1: int = 0
+    y2: int = 0
+    z1: int = 0
+    z2: int = 0
+    disable: tuple = (0, 0, 0, 0, 0, 0)
+    Nl: int = 64
+    lambda_min: float = 400e-9
+    lambda_max: float = 800e-9
+
+    def to_c(self):
+        s = CSensorReq()
+        s.box = int(self.box)
+        s.orientation = self.orientation
+        s.x1, s.x2, s.y1, s.y2, s.z1, s.z2 = self.x1, self.x2, self.y1, self.y2, self.z1, self.z2
+        for f in range(6):
+            s.disable[f] = int(self.disable[f])
+        s.Nl, s.lambda_min, s.lambda_max = self.Nl, self.lambda_min, self.lambda_max
+        return s
+
+    def out_len(self):
+        return 7 * self.Nl if self.box else self.Nl
+
+
+@dataclass
+class ModeSpec:
+    """The FDTD_Mode fields Aether's drivers read, plus the discretised structure."""
+    matsgrid: np.ndarray                      # (nz, ny, nx) uint32, x fastest
+    materials: list
+    Nt: int = 5000
+    Dx: float = 5e-9
+    Dy: float = 5e-9
+    Dz: float = 5e-9
+    time_mod: float = 1.0
+    tapering: int = 0
+    pml: tuple = (0, 0, 0, 0, 0, 0)
+    pad: tuple = (0, 0, 0, 0, 0, 0)
+    kappa: tuple = (25.0,) * 6
+    sigma: tuple = (1.0,) * 6
+    alpha: tuple = (0.2,) * 6
+    Nl: int = 481
+    lambda_min: float = 370e-9
+    lambda_max: float = 850e-9
+    polarization: int = 0                     # 0 TE, 1 TM, 2 mix
+    polar_angle: float = 0.0
+    disable_fields: tuple = ()
+    precision: int = FP64
+
+    def to_c(self, keep):
+        mg = np.ascontiguousarray(self.matsgrid, dtype=np.uint32)
+        keep.append(mg)
+        mats = (CMaterial * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i] = m.to_c(keep)
+        keep.append(mats)
+        dis = np.ascontiguousarray(np.asarray(self.disable_fields, dtype=np.int32).reshape(-1))
+        keep.append(dis)
+        c = CMode()
+        c.Nt = self.Nt
+        c.Dx, c.Dy, c.Dz = self.Dx, self.Dy, self.Dz
+        c.time_mod = self.time_mod
+        c.tapering = self.tapering
+        for f in range(6):
+            c.pml[f], c.pad[f] = self.pml[f], self.pad[f]
+            c.kappa[f], c.sigma[f], c.alpha[f] = self.kappa[f], self.sigma[f], self.alpha[f]
+        c.Nl, c.lambda_min, c.lambda_max = self.Nl, self.lambda_min, self.lambda_max
+        c.polarization, c.polar_angle = self.polarization, self.polar_angle
+        c.n_disable = len(dis)
+        c.disable_fields = dis.ctypes.data_as(_ip)
+        c.nz, c.ny, c.nx = mg.shape
+        c.matsgrid = mg.ctypes.data_as(C.POINTER(C.c_uint32))
+        c.n_materials = len(self.materials)
+        c.materials = mats
+        c.precision = self.precision
+        return c
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def run_normal_incidence(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
+    """Runs the normal-incidence driver; returns dict of spectra (and fields)."""
+    keep = []
+    cm = spec.to_c(keep)
+    Nl = spec.Nl
+    out = {k: np.zeros(Nl) for k in ("lambda", "R", "T")}
+    cplx = {k: np.zeros(2 * Nl) for k in ("refx", "refy", "transx", "transy")}
+    res = CNormalResult()
+    res.lambda_, res.R, res.T = _dptr(out["lambda"]), _dptr(out["R"]), _dptr(out["T"])
+    res.refx, res.refy = _dptr(cplx["refx"]), _dptr(cplx["refy"])
+    res.transx, res.transy = _dptr(cplx["transx"]), _dptr(cplx["transy"])
+    sreq = (CSensorReq * max(1, len(sensors)))(*[s.to_c() for s in sensors])
+    nsout = sum(s.out_len() for s in sensors)
+    sout = np.zeros(max(1, nsout))
+    fields = None
+    if want_fields:
+        fields = _normal_fields_buffer(spec)
+    rc = be.fn("run_normal_incidence")(C.byref(cm), len(sensors), sreq, C.byref(res), _dptr(sout),
+                                        _dptr(fields) if fields is not None else None)
+    if rc not in (OK, ECANCELLED):
+        be.check(rc, None)
+    for k, v in cplx.items():
+        out[k] = v[0::2] + 1j * v[1::2]
+    out["index_sup"], out["index_sub"], out["Nt_run"] = res.index_sup, res.index_sub, res.Nt_run
+    out["sensors"] = sout[:nsout]
+    if fields is not None:
+        out["fields"] = fields
+    return out
+
+
+def _normal_fields_buffer(spec):
+    nz, ny, nx = spec.matsgrid.shape
+    pzm, pzp = max(5, spec.pad[4]), max(5, spec.pad[5])
+    if spec.pml[4] or spec.pml[5]:
+        pzm, pzp = max(5, pzm), max(5, pzp)
+    Nz = nz + spec.pml[4] + spec.pml[5] + pzm + pzp
+    return np.zeros((6, Nz, ny, nx))
+
+
+def single_particle_dims(spec):
+    nz, ny, nx = spec.matsgrid.shape
+    pad = [max(10, p) for p in spec.pad]
+    if max(spec.pml[0], spec.pml[1]) <= 0:
+        pad[0] = pad[1] = 0
+    if max(spec.pml[2], spec.pml[3]) <= 0:
+        pad[2] = pad[3] = 0
+    for a in range(3):
+        if spec.pml[2 * a] or spec.pml[2 * a + 1]:
+            pad[2 * a], pad[2 * a + 1] = max(5, pad[2 * a]), max(5, pad[2 * a + 1])
+    Nx = nx + spec.pml[0] + spec.pml[1] + pad[0] + pad[1]
+    Ny = ny + spec.pml[2] + spec.pml[3] + pad[2] + pad[3]
+    Nz = nz + spec.pml[4] + spec.pml[5] + pad[4] + pad[5]
+    return Nx, Ny, Nz
+
+
+def run_single_particle(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
+    keep = []
+    cm = spec.to_c(keep)
+    sreq = (CSensorReq * max(1, len(sensors)))(*[s.to_c() for s in sensors])
+    nsout = sum(s.out_len() for s in sensors)
+    sout = np.zeros(max(1, nsout))
+    fields = None
+    if want_fields:
+        Nx, Ny, Nz = single_particle_dims(spec)
+        fields = np.zeros((6, Nz, Ny, Nx))
+    rc = be.fn("run_single_particle")(C.byref(cm), len(sensors), sreq, _dptr(sout),
+                                       _dptr(fields) if fields is not None else None)
+    if rc not in (OK, ECANCELLED):
+        be.check(rc, None)
+    out = {"sensors": sout[:nsout]}
+    if fields is not None:
+        out["fields"] = fields
+    return out
+
+
+class Grid:
+    """Thin object wrapper over one grid handle of a backend (engine or oracle)."""
+
+    def __init__(self, be: Backend, N, Nt, D, Dt, mode=MODE_CUSTOM, pml=(0,) * 6, pad=(0,) * 6, precision=FP64):
+        self.be = be
+        h = C.c_void_p()
+        p6 = (C.c_int * 6)(*pml)
+        q6 = (C.c_int * 6)(*pad)
+        rc = be.fn("create")(C.byref(h), N[0], N[1], N[2], Nt, D[0], D[1], D[2], Dt, mode, p6, q6, precision)
+        be.check(rc, None)
+        self.h = h
+        lay = (C.c_int * 16)()
+        be.fn("layout")(h, lay)
+        self.layout = list(lay)
+        self.Nx, self.Ny, self.Nz = self.layout[:3]
+        self._keep = []
+
+    def _c(self, rc):
+        self.be.check(rc, self.h)
+
+    def close(self):
+        if self.h:
+            self.be.fn("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pml(self, face, kappa=25.0, sigma=1.0, alpha=0.2):
+        self._c(self.be.fn("set_pml")(self.h, face, kappa, sigma, alpha))
+
+    def set_tapering(self, n):
+        self._c(self.be.fn("set_tapering")(self.h, n))
+
+    def set_matsgrid(self, mg):
+        mg = np.ascontiguousarray(mg, dtype=np.uint32)
+        nz, ny, nx = mg.shape
+        self._c(self.be.fn("set_matsgrid")(self.h, mg.ctypes.data_as(C.POINTER(C.c_uint32)), nx, ny, nz))
+
+    def set_material(self, idx, m: Material):
+        keep = []
+        cm = m.to_c(keep)
+        self._c(self.be.fn("set_material")(self.h, idx, C.byref(cm)))
+
+    def disable_fields(self, fields):
+        a = (C.c_int * max(1, len(fields)))(*fields)
+        self._c(self.be.fn("disable_fields")(self.h, a, len(fields)))
+
+    def bootstrap(self):
+        self._c(self.be.fn("bootstrap")(self.h))
+
+    def update_E(self):
+        self._c(self.be.fn("update_E")(self.h))
+
+    def update_H(self):
+        self._c(self.be.fn("update_H")(self.h))
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.update_E()
+            self.update_H()
+
+    def read(self, f):
+        out = np.zeros((self.Nz, self.Ny, self.Nx))
+        self._c(self.be.fn("read_field")(self.h, f, _dptr(out)))
+        return out
+
+    def write(self, f, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.Nz, self.Ny, self.Nx)
+        self._c(self.be.fn("write_field")(self.h, f, _dptr(a)))
+
+    def fields(self):
+        return np.stack([self.read(f) for f in range(6)])
+
+    def set_plane_injection(self, k, dEx, dEy, dHx, dHy):
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (dEx, dEy, dHx, dHy)]
+        self._keep += arrs
+        self._c(self.be.fn("set_plane_injection")(self.h, k, len(arrs[0]), *[_dptr(a) for a in arrs]))
+
+    def set_tfsf(self, aux, box, periodic_x=False, periodic_y=False):
+        self._c(self.be.fn("set_tfsf")(self.h, aux.h, *box, int(periodic_x), int(periodic_y)))
+
+    def add_spect_poynting(self, orientation, box, lambdas):
+        lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+        i = C.c_int()
+        self._c(self.be.fn("add_spect_poynting")(self.h, orientation, *box, len(lam), _dptr(lam), C.byref(i)))
+        return i.value
+
+    def feed_sensors(self):
+        self._c(self.be.fn("feed_sensors")(self.h))
+
+    def sensor_poynting(self, sid, Nl):
+        out = np.zeros(Nl)
+        self._c(self.be.fn("sensor_poynting")(self.h, sid, _dptr(out)))
+        return out
+
+    def sensor_spectra(self, sid):
+        dims = (C.c_int * 3)()
+        self._c(self.be.fn("sensor_spectra")(self.h, sid, None, dims))
+        s1, s2, nl = dims
+        out = np.zeros(6 * nl * s1 * s2 * 2)
+        self._c(self.be.fn("sensor_spectra")(self.h, sid, _dptr(out), dims))
+        v = out[0::2] + 1j * out[1::2]
+        return v.reshape(6, nl, s2, s1)

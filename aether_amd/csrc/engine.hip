@@ -1,0 +1,1424 @@
+// engine.hip -- gfx950 kernels and the grid-level C ABI of the FDTD engine.
+//
+// One fused pass per half step keeps the leapfrog at its HBM floor:
+//   E pass : read E(3) H(3) mat -> RC-ante (FP64 poles) -> E stencil -> TFSF-E
+//            -> RC-apply -> CPML-E (slab psi) -> PEC -> plane injection -> write E(3)
+//   H pass : read H(3) E(3)     -> H stencil -> CPML-H -> PEC -> plane injection
+//            -> TFSF-H -> write H(3)
+// This is the phase order of Aether's threaded_process_E/H (core/fdtd_threads.
+// cpp:32-235) and of its drivers (modes/normal_incidence.cpp:330-357,
+// modes/single_particle.cpp:256-378); every phase is cell-local, so fusing them
+// keeps each cell's sequence of floating-point updates.
+//
+// Thread mapping: a workgroup is 64 x 4 lanes over (x, y); x is the contiguous
+// axis, so one wavefront reads 64 consecutive values of each field. Each lane
+// marches KZ planes in z and keeps the previous plane's H (E pass) / next
+// plane's E (H pass) in registers, so the z-neighbour is never re-read.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+using namespace aefdtd;
+
+namespace {
+
+constexpr int BX = 64, BY = 4, KZ = 8;
+constexpr int KS = 32;                  // DFT staging depth (steps per accumulate)
+
+template <typename T>
+struct MatDev {
+    T C1, C2x, C2y, C2z, C4;
+    double C3;
+    int rc, np, pole0, pad_;
+};
+
+// 1D profile arrays per axis a (length N_a): ikE, ikH (inverse kappa with the
+// stencil's gating), bE, cE (c * 1/D), bH, cH.
+template <typename T>
+struct Prof {
+    const T *ikE[3], *ikH[3], *bE[3], *cE[3], *bH[3], *cH[3];
+};
+
+template <typename T>
+struct EArgs {
+    int Nx, Ny, Nz;
+    T *Ex, *Ey, *Ez;
+    const T *Hx, *Hy, *Hz;
+    const uint8_t *mat;
+    const MatDev<T> *mt;
+    const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
+    const int32_t *slot;
+    double2 *psi;
+    long ndisp;
+    Prof<T> pf;
+    int pml[6];
+    T *pEyx, *pEzx, *pExy, *pEzy, *pExz, *pEyz;   // slab psi, x:(pxm+pxp,Ny,Nz) y:(Nx,pym+pyp,Nz) z:(Nx,Ny,pzm+pzp)
+    int en;                    // bit c set: component c enabled
+    int inj_k;
+    T injx, injy;
+    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
+    const T *aHx, *aHy, *aHz;  // aux 1D grid H columns
+};
+
+template <typename T>
+struct HArgs {
+    int Nx, Ny, Nz;
+    T *Hx, *Hy, *Hz;
+    const T *Ex, *Ey, *Ez;
+    Prof<T> pf;
+    int pml[6];
+    T *pHyx, *pHzx, *pHxy, *pHzy, *pHxz, *pHyz;
+    int en;
+    T dtm, dtdmx, dtdmy, dtdmz;
+    int inj_k;
+    T injx, injy;
+    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
+    const T *aEx, *aEy, *aEz;
+};
+
+// ---------------------------------------------------------------- E pass
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_E(const EArgs<T> a) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.Nx || j >= a.Ny) return;
+    const int k0 = blockIdx.z * KZ;
+    const int k1 = min(k0 + KZ, a.Nz);
+    const size_t NxNy = (size_t)a.Nx * a.Ny;
+    const size_t base = (size_t)i + (size_t)a.Nx * j;
+    const long oim = (i == 0) ? (long)(a.Nx - 1) : -1L;                    // (i-1) with wrap
+    const long ojm = (j == 0) ? (long)(a.Ny - 1) * a.Nx : -(long)a.Nx;     // (j-1) with wrap
+    const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
+    T hx_km = a.Hx[base + km0 * NxNy];
+    T hy_km = a.Hy[base + km0 * NxNy];
+    const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
+    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
+    // CPML slab index along x / y (E loop bounds: near 1..pm-1, far N-pp+1..N-1)
+    int nx_ = -1, ny_ = -1;
+    if (px) {
+        if (i >= 1 && i < a.pml[0]) nx_ = i;
+        else if (i >= a.Nx - a.pml[1] + 1) nx_ = i - (a.Nx - a.pml[1]) + a.pml[0];
+    }
+    if (py) {
+        if (j >= 1 && j < a.pml[2]) ny_ = j;
+        else if (j >= a.Ny - a.pml[3] + 1) ny_ = j - (a.Ny - a.pml[3]) + a.pml[2];
+    }
+    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
+    for (int k = k0; k < k1; k++) {
+        const size_t id = base + (size_t)k * NxNy;
+        const T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
+        const T dHz_y = hz - a.Hz[id + ojm];
+        const T dHz_x = hz - a.Hz[id + oim];
+        const T dHy_x = hy - a.Hy[id + oim];
+        const T dHx_y = hx - a.Hx[id + ojm];
+        const T dHy_z = hy - hy_km;
+        const T dHx_z = hx - hx_km;
+        hx_km = hx;
+        hy_km = hy;
+        const T ex0 = a.Ex[id], ey0 = a.Ey[id], ez0 = a.Ez[id];
+        const MatDev<T> M = a.mt[a.mat[id]];
+        // RC-ante: psi = Crec psi + Dchi E^n (core/mats_RC.cpp:21-36), FP64
+        double sx = 0, sy = 0, sz = 0;
+        if (M.rc) {
+            const long s = a.slot[id];
+            for (int p = 0; p < M.np; p++) {
+                const double2 cr = a.poles[2 * (M.pole0 + p)], dc = a.poles[2 * (M.pole0 + p) + 1];
+                double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
+                const double e3[3] = {(double)ex0, (double)ey0, (double)ez0};
+                double acc[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    double2 v = q[(size_t)c * a.ndisp];
+                    double2 r;
+                    r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
+                    r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
+                    q[(size_t)c * a.ndisp] = r;
+                    acc[c] = r.x;
+                }
+                sx += acc[0]; sy += acc[1]; sz += acc[2];
+            }
+        }
+        const T ikz = a.pf.ikE[2][k];
+        // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
+        T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
+        T ey = (a.en & 2) ? M.C1 * ey0 + M.C2z * ikz * dHx_z - M.C2x * ikx * dHz_x : ey0;
+        T ez = (a.en & 4) ? M.C1 * ez0 + M.C2x * ikx * dHy_x - M.C2y * iky * dHx_y : ez0;
+        // TFSF-E (modes/single_particle.cpp:286-332): z faces, then y, then x
+        if (a.tfsf) {
+            const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
+            if (in_i && in_j) {
+                if (k == a.tb4) { ex += M.C2z * a.aHy[a.tb4 - 1]; ey -= M.C2z * a.aHx[a.tb4 - 1]; }
+                if (k == a.tb5) { ex -= M.C2z * a.aHy[a.tb5 - 1]; ey += M.C2z * a.aHx[a.tb5 - 1]; }
+            }
+            if (!a.tper_y && in_i && in_k) {
+                if (j == a.tb2) { ex -= M.C2y * a.aHz[k]; ez += M.C2y * a.aHx[k]; }
+                if (j == a.tb3) { ex += M.C2y * a.aHz[k]; ez -= M.C2y * a.aHx[k]; }
+            }
+            if (!a.tper_x && in_k && in_j) {
+                if (i == a.tb0) { ey += M.C2x * a.aHz[k]; ez -= M.C2x * a.aHy[k]; }
+                if (i == a.tb1) { ey -= M.C2x * a.aHz[k]; ez += M.C2x * a.aHy[k]; }
+            }
+        }
+        // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50)
+        if (M.rc) {
+            ex += (T)(M.C3 * sx);
+            ey += (T)(M.C3 * sy);
+            ez += (T)(M.C3 * sz);
+        }
+        // CPML-E + PEC walls (core/fdtd_pml.cpp:504-735)
+        int nz_ = -1;
+        if (pz) {
+            if (k >= 1 && k < a.pml[4]) nz_ = k;
+            else if (k >= a.Nz - a.pml[5] + 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
+        }
+        if (a.en & 1) {
+            if (py) {
+                if (ny_ >= 0) {
+                    T &q = a.pExy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
+                    q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHz_y;
+                    ex += M.C4 * q;
+                }
+                if (j == 0) ex = 0;
+            }
+            if (pz) {
+                if (nz_ >= 0) {
+                    T &q = a.pExz[base + NxNy * nz_];
+                    q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHy_z;
+                    ex -= M.C4 * q;
+                }
+                if (k == 0) ex = 0;
+            }
+        }
+        if (a.en & 2) {
+            if (px) {
+                if (nx_ >= 0) {
+                    T &q = a.pEyx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
+                    q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHz_x;
+                    ey -= M.C4 * q;
+                }
+                if (i == 0) ey = 0;
+            }
+            if (pz) {
+                if (nz_ >= 0) {
+                    T &q = a.pEyz[base + NxNy * nz_];
+                    q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHx_z;
+                    ey += M.C4 * q;
+                }
+                if (k == 0) ey = 0;
+            }
+        }
+        if (a.en & 4) {
+            if (px) {
+                if (nx_ >= 0) {
+                    T &q = a.pEzx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
+                    q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHy_x;
+                    ez += M.C4 * q;
+                }
+                if (i == 0) ez = 0;
+            }
+            if (py) {
+                if (ny_ >= 0) {
+                    T &q = a.pEzy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
+                    q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHx_y;
+                    ez -= M.C4 * q;
+                }
+                if (j == 0) ez = 0;
+            }
+        }
+        // plane injection after the whole E update (modes/normal_incidence.cpp:338-342)
+        if (k == a.inj_k) { ex += a.injx; ey += a.injy; }
+        a.Ex[id] = ex;
+        a.Ey[id] = ey;
+        a.Ez[id] = ez;
+    }
+}
+
+// ---------------------------------------------------------------- H pass
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_H(const HArgs<T> a) {
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    if (i >= a.Nx || j >= a.Ny) return;
+    const int k0 = blockIdx.z * KZ;
+    const int k1 = min(k0 + KZ, a.Nz);
+    const size_t NxNy = (size_t)a.Nx * a.Ny;
+    const size_t base = (size_t)i + (size_t)a.Nx * j;
+    const long oip = (i == a.Nx - 1) ? -(long)(a.Nx - 1) : 1L;
+    const long ojp = (j == a.Ny - 1) ? -(long)(a.Ny - 1) * a.Nx : (long)a.Nx;
+    T ex_k = a.Ex[base + (size_t)k0 * NxNy];
+    T ey_k = a.Ey[base + (size_t)k0 * NxNy];
+    const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
+    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
+    // CPML slab index (H loop bounds: near 0..pm-1, far N-pp..N-2)
+    int nx_ = -1, ny_ = -1;
+    if (px) {
+        if (i < a.pml[0]) nx_ = i;
+        else if (i >= a.Nx - a.pml[1] && i < a.Nx - 1) nx_ = i - (a.Nx - a.pml[1]) + a.pml[0];
+    }
+    if (py) {
+        if (j < a.pml[2]) ny_ = j;
+        else if (j >= a.Ny - a.pml[3] && j < a.Ny - 1) ny_ = j - (a.Ny - a.pml[3]) + a.pml[2];
+    }
+    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
+    for (int k = k0; k < k1; k++) {
+        const size_t id = base + (size_t)k * NxNy;
+        const int kp = (k == a.Nz - 1) ? 0 : k + 1;
+        const T ex = ex_k, ey = ey_k;
+        const T ex_kp = a.Ex[base + (size_t)kp * NxNy];
+        const T ey_kp = a.Ey[base + (size_t)kp * NxNy];
+        ex_k = ex_kp;
+        ey_k = ey_kp;
+        const T ez = a.Ez[id];
+        const T dEy_z = ey_kp - ey;
+        const T dEx_z = ex_kp - ex;
+        const T dEz_y = a.Ez[id + ojp] - ez;
+        const T dEz_x = a.Ez[id + oip] - ez;
+        const T dEx_y = a.Ex[id + ojp] - ex;
+        const T dEy_x = a.Ey[id + oip] - ey;
+        const T ikz = a.pf.ikH[2][k];
+        T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
+        // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
+        if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
+        if (a.en & 16) hy += a.dtdmx * ikx * dEz_x - a.dtdmz * ikz * dEx_z;
+        if (a.en & 32) hz += a.dtdmy * iky * dEx_y - a.dtdmx * ikx * dEy_x;
+        // CPML-H + PEC walls (core/fdtd_pml.cpp:737-929)
+        int nz_ = -1;
+        if (pz) {
+            if (k < a.pml[4]) nz_ = k;
+            else if (k >= a.Nz - a.pml[5] && k < a.Nz - 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
+        }
+        if (a.en & 8) {
+            if (py) {
+                if (ny_ >= 0) {
+                    T &q = a.pHxy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
+                    q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEz_y;
+                    hx -= a.dtm * q;
+                }
+                if (j == a.Ny - 1) hx = 0;
+            }
+            if (pz) {
+                if (nz_ >= 0) {
+                    T &q = a.pHxz[base + NxNy * nz_];
+                    q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEy_z;
+                    hx += a.dtm * q;
+                }
+                if (k == a.Nz - 1) hx = 0;
+            }
+        }
+        if (a.en & 16) {
+            if (px) {
+                if (nx_ >= 0) {
+                    T &q = a.pHyx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
+                    q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEz_x;
+                    hy += a.dtm * q;
+                }
+                if (i == a.Nx - 1) hy = 0;
+            }
+            if (pz) {
+                if (nz_ >= 0) {
+                    T &q = a.pHyz[base + NxNy * nz_];
+                    q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEx_z;
+                    hy -= a.dtm * q;
+                }
+            }
+        }
+        // top z wall of Hy: zeroed by the Hx pass and by the Hy pass (fdtd_pml.cpp:799,862-865)
+        if (pz && k == a.Nz - 1 && (a.en & (8 | 16))) hy = 0;
+        if (a.en & 32) {
+            if (px) {
+                if (nx_ >= 0) {
+                    T &q = a.pHzx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
+                    q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEy_x;
+                    hz -= a.dtm * q;
+                }
+                if (i == a.Nx - 1) hz = 0;
+            }
+            if (py) {
+                if (ny_ >= 0) {
+                    T &q = a.pHzy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
+                    q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEx_y;
+                    hz += a.dtm * q;
+                }
+                if (j == a.Ny - 1) hz = 0;
+            }
+        }
+        // plane injection (modes/normal_incidence.cpp:353-357)
+        if (k == a.inj_k) { hx += a.injx; hy += a.injy; }
+        // TFSF-H (modes/single_particle.cpp:351-378): z faces, then y, then x
+        if (a.tfsf) {
+            const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
+            if (in_i && in_j) {
+                if (k == a.tb4 - 1) { hx -= a.dtdmz * a.aEy[a.tb4]; hy += a.dtdmz * a.aEx[a.tb4]; }
+                if (k == a.tb5 - 1) { hx += a.dtdmz * a.aEy[a.tb5]; hy -= a.dtdmz * a.aEx[a.tb5]; }
+            }
+            if (!a.tper_y && in_i && in_k) {
+                if (j == a.tb2 - 1) { hx += a.dtdmy * a.aEz[k]; hz -= a.dtdmy * a.aEx[k]; }
+                if (j == a.tb3 - 1) { hx -= a.dtdmy * a.aEz[k]; hz += a.dtdmy * a.aEx[k]; }
+            }
+            if (!a.tper_x && in_k && in_j) {
+                if (i == a.tb0 - 1) { hy -= a.dtdmx * a.aEz[k]; hz += a.dtdmx * a.aEy[k]; }
+                if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
+            }
+        }
+        a.Hx[id] = hx;
+        a.Hy[id] = hy;
+        a.Hz[id] = hz;
+    }
+}
+
+}  // namespace
+
+// ================================================================ sensors
+namespace aefdtd {
+
+struct Sensor {                   // Spect_Poynting with interpolated samples
+    int type, x1, x2, y1, y2, z1, z2, span1, span2, Nl;
+    long P, off;                  // cells, offset in the stage/accumulator pools
+    std::vector<double> lambda;
+    double2 *d_acc = nullptr;     // [6][Nl][P]
+    double2 *d_coef = nullptr;    // [ncoef steps][2][Nl]: E then H coefficient
+    int ncoef = 0;
+};
+
+struct PlaneSum {
+    int k;
+    bool has_inc, subtract;
+    aefdtd_pulse inc;
+    double zpos;
+    double *d_tap = nullptr;      // [Nt]
+    double *d_out = nullptr;      // [cap][6]
+    int cap = 0, n = 0;
+};
+
+}  // namespace aefdtd
+
+namespace {
+
+struct SensDesc {
+    int type, x1, y1, z1, span1, span2;
+    long P, off;
+};
+
+// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) on one sensor plane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, const T *Ex, const T *Ey,
+                                                const T *Ez, const T *Hx, const T *Hy, const T *Hz, int Nx, int Ny,
+                                                int Nz, T *stage) {
+    const SensDesc d = sd[blockIdx.y];
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int a = (int)(p % d.span1), b = (int)(p / d.span1);
+    int i, j, k;
+    if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
+    else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) { i = d.x1 + a; j = d.y1; k = d.z1 + b; }
+    else { i = d.x1 + a; j = d.y1 + b; k = d.z1; }
+    const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
+    auto at = [&](const T *f, int x, int y, int z) { return f[(size_t)x + (size_t)Nx * ((size_t)y + (size_t)Ny * z)]; };
+    T v[6];
+    v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
+    v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
+    v[2] = (at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0;
+    v[3] = (at(Hx, i, j, k) + at(Hx, ip, j, k)) / (T)2.0;
+    v[4] = (at(Hy, i, j, k) + at(Hy, i, jp, k)) / (T)2.0;
+    v[5] = (at(Hz, i, j, k) + at(Hz, i, j, kp)) / (T)2.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) stage[((size_t)slot * 6 + c) * Ptot + d.off + p] = v[c];
+}
+
+// Running DFT over a block of nfill staged steps (sensors/sensors.cpp:456-479):
+// acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p], FP64 accumulators.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
+                                             const double2 *coef, int s0, int nfill, double2 *acc) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (p >= P) return;
+    double x[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) x[s] = (s < nfill) ? (double)stage[((size_t)s * 6 + c) * Ptot + off + p] : 0.0;
+    const int h = c >= 3 ? 1 : 0;
+    for (int l = 0; l < Nl; l++) {
+        double re = 0, im = 0;
+        for (int s = 0; s < nfill; s++) {
+            const double2 w = coef[((size_t)(s0 + s) * 2 + h) * Nl + l];
+            re += w.x * x[s];
+            im += w.y * x[s];
+        }
+        double2 &A = acc[((size_t)c * Nl + l) * P + p];
+        A.x += re;
+        A.y += im;
+    }
+}
+
+// Spect_Poynting::treat (sensors/s_spec_power.cpp:61-111): one workgroup per
+// wavelength, fixed-order tree reduction so the result is deterministic.
+__global__ __launch_bounds__(256) void k_poynting(const double2 *acc, long P, int Nl, double f, int type, double *rsp) {
+    const int l = blockIdx.x;
+    __shared__ double sh[3][256];
+    double S[3] = {0, 0, 0};
+    for (long p = threadIdx.x; p < P; p += blockDim.x) {
+        double2 e[3], h[3];
+        for (int c = 0; c < 3; c++) {
+            e[c] = acc[((size_t)c * Nl + l) * P + p];
+            h[c] = acc[((size_t)(3 + c) * Nl + l) * P + p];
+        }
+        // Re(a conj(b)) = a.x b.x + a.y b.y
+        S[0] += (e[1].x * h[2].x + e[1].y * h[2].y) - (e[2].x * h[1].x + e[2].y * h[1].y);
+        S[1] += (e[2].x * h[0].x + e[2].y * h[0].y) - (e[0].x * h[2].x + e[0].y * h[2].y);
+        S[2] += (e[0].x * h[1].x + e[0].y * h[1].y) - (e[1].x * h[0].x + e[1].y * h[0].y);
+    }
+    for (int c = 0; c < 3; c++) sh[c][threadIdx.x] = S[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 3; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double Sx = sh[0][0] * f, Sy = sh[1][0] * f, Sz = sh[2][0] * f;
+        double r;
+        switch (type) {
+            case AEFDTD_NORMAL_X: r = Sx; break;
+            case AEFDTD_NORMAL_Y: r = Sy; break;
+            case AEFDTD_NORMAL_Z: r = Sz; break;
+            case AEFDTD_NORMAL_XM: r = -Sx; break;
+            case AEFDTD_NORMAL_YM: r = -Sy; break;
+            default: r = -Sz; break;
+        }
+        rsp[l] = r;
+    }
+}
+
+// Analytic pulse on the device (core/chpin.cpp:44-90).
+__device__ double d_pulse(const aefdtd_pulse &p, double amp, double x, double y, double z, double t) {
+    const double ph = p.kx * x + p.ky * y + p.kz * z;
+    const double a = ph - p.omega * t;
+    const double b = (1.0 / p.k * ph - kC * t / p.n_ind + p.C1) / p.dz;
+    return amp * exp(-b * b) * cos(a);
+}
+
+// R/T plane sums of modes/normal_incidence.cpp:384-415 for step t; one
+// workgroup, fixed-order reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, const T *Ez, int Nx, int Ny, int kpl,
+                                                   aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
+                                                   double Dz, double zpos, const double *tap, int t, double Dt,
+                                                   double *out) {
+    __shared__ double sh[6][256];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double tb = (t + 0.5) * Dt;
+    const double tp = tap[t];
+    const size_t NxNy = (size_t)Nx * Ny;
+    for (long q = threadIdx.x; q < (long)NxNy; q += blockDim.x) {
+        const int i = (int)(q % Nx), j = (int)(q / Nx);
+        const double x = i * Dx, y = j * Dy;
+        double in[3] = {0, 0, 0};
+        if (has_inc) {
+            in[0] = d_pulse(inc, inc.Ex0, x + Dx / 2.0, y, zpos, tb);
+            in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
+            in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
+        }
+        const size_t id = (size_t)q + NxNy * kpl;
+        const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
+        for (int c = 0; c < 3; c++) {
+            acc[3 + c] += in[c] * tp;
+            acc[c] += (subtract ? (f[c] - in[c]) : f[c]) * tp;
+        }
+    }
+    for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[(size_t)t * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// --------------------------------------------------------------- helpers
+int fail(aefdtd_grid *g, int code, const std::string &msg) {
+    if (g) g->err = msg;
+    return code;
+}
+
+std::string g_create_err;
+std::mutex g_create_mu;
+
+#define HIPOK(g, call)                                                                                  \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return fail((g), e_ == hipErrorOutOfMemory ? AEFDTD_ENOMEM : AEFDTD_EDEVICE, \
+                                          std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+int dalloc(aefdtd_grid *g, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return AEFDTD_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(g, AEFDTD_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    g->dev_bytes += bytes;
+    return AEFDTD_OK;
+}
+
+void dfree(void *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename P>
+void dfree_t(P *&p) {
+    void *v = (void *)p;
+    dfree(v);
+    p = nullptr;
+}
+
+bool is_fp64(const aefdtd_grid *g) { return g->precision == AEFDTD_FP64; }
+
+// Host copy of a double array into a device buffer of T.
+int upload_T(aefdtd_grid *g, void *dst, const double *src, size_t n) {
+    if (n == 0) return AEFDTD_OK;
+    if (is_fp64(g)) {
+        HIPOK(g, hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, g->stream));
+    } else {
+        std::vector<float> t(n);
+        for (size_t i = 0; i < n; i++) t[i] = (float)src[i];
+        HIPOK(g, hipMemcpyAsync(dst, t.data(), n * 4, hipMemcpyHostToDevice, g->stream));
+        HIPOK(g, hipStreamSynchronize(g->stream));
+    }
+    return AEFDTD_OK;
+}
+
+void pml_coeff_calc(aefdtd_grid *g) {          // core/fdtd_pml.cpp:305-343
+    int N[3] = {g->Nx, g->Ny, g->Nz};
+    for (int a = 0; a < 3; a++) {
+        g->kE[a].assign(N[a], 1.0); g->kH[a].assign(N[a], 1.0);
+        g->bE[a].assign(N[a], 0.0); g->bH[a].assign(N[a], 0.0);
+        g->cE[a].assign(N[a], 0.0); g->cH[a].assign(N[a], 0.0);
+        if (g->pml[2 * a] || g->pml[2 * a + 1]) {
+            const double kap[2] = {g->pml_kappa[2 * a], g->pml_kappa[2 * a + 1]};
+            const double sig[2] = {g->pml_sigma[2 * a], g->pml_sigma[2 * a + 1]};
+            const double alp[2] = {g->pml_alpha[2 * a], g->pml_alpha[2 * a + 1]};
+            for (int i = 0; i < N[a]; i++) {
+                pml_profile(i, N[a], g->pml[2 * a], g->pml[2 * a + 1], kap, sig, alp, g->Dt, g->kE[a][i], g->bE[a][i], g->cE[a][i]);
+                pml_profile(i + 0.5, N[a], g->pml[2 * a], g->pml[2 * a + 1], kap, sig, alp, g->Dt, g->kH[a][i], g->bH[a][i], g->cH[a][i]);
+            }
+        }
+    }
+}
+
+// Uploads the 1D profile tables: inverse kappa as the stencils use it (x axis
+// gated by pml_xm||pml_xp, y axis by pml_ym||pml_xp as core/fdtd_core.cpp:399,
+// z by pml_zm||pml_zp), b, and c pre-multiplied by 1/D (fdtd_pml.cpp:523 etc.).
+int upload_profiles(aefdtd_grid *g) {
+    pml_coeff_calc(g);
+    int N[3] = {g->Nx, g->Ny, g->Nz};
+    const bool gate[3] = {g->pml[0] || g->pml[1], g->pml[2] || g->pml[1], g->pml[4] || g->pml[5]};
+    const double invD[3] = {1.0 / g->Dx, 1.0 / g->Dy, 1.0 / g->Dz};
+    size_t tot = 6 * (size_t)(N[0] + N[1] + N[2]);
+    std::vector<double> h(tot);
+    size_t o = 0;
+    for (int a = 0; a < 3; a++) {
+        for (int i = 0; i < N[a]; i++) h[o + i] = 1.0 / (gate[a] ? g->kE[a][i] : 1.0);
+        o += N[a];
+        for (int i = 0; i < N[a]; i++) h[o + i] = 1.0 / (gate[a] ? g->kH[a][i] : 1.0);
+        o += N[a];
+        for (int i = 0; i < N[a]; i++) h[o + i] = g->bE[a][i];
+        o += N[a];
+        for (int i = 0; i < N[a]; i++) h[o + i] = g->cE[a][i] * invD[a];
+        o += N[a];
+        for (int i = 0; i < N[a]; i++) h[o + i] = g->bH[a][i];
+        o += N[a];
+        for (int i = 0; i < N[a]; i++) h[o + i] = g->cH[a][i] * invD[a];
+        o += N[a];
+    }
+    if (!g->d_prof) {
+        int rc = dalloc(g, &g->d_prof, tot * g->tsize());
+        if (rc) return rc;
+    }
+    return upload_T(g, g->d_prof, h.data(), tot);
+}
+
+template <typename T>
+Prof<T> prof_of(const aefdtd_grid *g) {
+    Prof<T> p;
+    const T *b = (const T *)g->d_prof;
+    int N[3] = {g->Nx, g->Ny, g->Nz};
+    for (int a = 0; a < 3; a++) {
+        p.ikE[a] = b; b += N[a];
+        p.ikH[a] = b; b += N[a];
+        p.bE[a] = b; b += N[a];
+        p.cE[a] = b; b += N[a];
+        p.bH[a] = b; b += N[a];
+        p.cH[a] = b; b += N[a];
+    }
+    return p;
+}
+
+// CPML slab state (core/fdtd_pml.cpp:26-85): order EYX,EZX,HYX,HZX (x slabs),
+// EXY,EZY,HXY,HZY (y slabs), EXZ,EYZ,HXZ,HYZ (z slabs).
+int alloc_pml_state(aefdtd_grid *g) {
+    for (int s = 0; s < 12; s++) {
+        dfree(g->d_psipml[s]);
+        g->psipml_n[s] = 0;
+    }
+    size_t n[3] = {0, 0, 0};
+    if (g->pml[0] || g->pml[1]) n[0] = (size_t)(g->pml[0] + g->pml[1]) * g->Ny * g->Nz;
+    if (g->pml[2] || g->pml[3]) n[1] = (size_t)g->Nx * (g->pml[2] + g->pml[3]) * g->Nz;
+    if (g->pml[4] || g->pml[5]) n[2] = (size_t)g->Nx * g->Ny * (g->pml[4] + g->pml[5]);
+    for (int s = 0; s < 12; s++) {
+        size_t m = n[s / 4];
+        if (!m) continue;
+        int rc = dalloc(g, &g->d_psipml[s], m * g->tsize());
+        if (rc) return rc;
+        HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, m * g->tsize(), g->stream));
+        g->psipml_n[s] = m;
+    }
+    return AEFDTD_OK;
+}
+
+template <typename T>
+int upload_materials_T(aefdtd_grid *g) {
+    std::vector<MatDev<T>> tab(g->Nmat);
+    std::vector<double2> poles;
+    int npmax = 0;
+    for (unsigned m = 0; m < g->Nmat; m++) {
+        const HostMat &M = g->mats[m];
+        MatDev<T> d;
+        d.C1 = (T)M.C1; d.C2x = (T)M.C2x; d.C2y = (T)M.C2y; d.C2z = (T)M.C2z; d.C4 = (T)M.C4;
+        d.C3 = M.C3;
+        d.rc = M.rc ? 1 : 0;
+        d.np = (int)M.Crec.size();
+        d.pole0 = (int)(poles.size() / 2);
+        d.pad_ = 0;
+        for (int p = 0; p < d.np; p++) {
+            poles.push_back(make_double2(M.Crec[p].real(), M.Crec[p].imag()));
+            poles.push_back(make_double2(M.Dchi[p].real(), M.Dchi[p].imag()));
+        }
+        if (M.rc) npmax = std::max(npmax, d.np);
+        tab[m] = d;
+    }
+    dfree(g->d_mtab);
+    dfree_t(g->d_poles);
+    int rc = dalloc(g, &g->d_mtab, tab.size() * sizeof(MatDev<T>));
+    if (rc) return rc;
+    HIPOK(g, hipMemcpyAsync(g->d_mtab, tab.data(), tab.size() * sizeof(MatDev<T>), hipMemcpyHostToDevice, g->stream));
+    if (!poles.empty()) {
+        void *p;
+        if ((rc = dalloc(g, &p, poles.size() * sizeof(double2)))) return rc;
+        g->d_poles = (double2 *)p;
+        HIPOK(g, hipMemcpyAsync(g->d_poles, poles.data(), poles.size() * sizeof(double2), hipMemcpyHostToDevice, g->stream));
+    }
+    g->np_max = npmax;
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    return AEFDTD_OK;
+}
+
+// Material ids (uint8), dispersive slots and RC accumulators.  The reference
+// keeps one psi block per material over its bounding box (core/mats.cpp:394-496);
+// here only dispersive cells get a slot, numbered in x-fastest order so that
+// neighbouring lanes touch neighbouring slots.
+int upload_grid(aefdtd_grid *g) {
+    if (g->Nmat > 256) return fail(g, AEFDTD_EINVAL, "more than 256 material indices");
+    size_t N = g->ncell();
+    std::vector<uint8_t> m8(N);
+    for (size_t c = 0; c < N; c++) m8[c] = (uint8_t)g->matsgrid[c];
+    if (!g->d_mat) {
+        void *p;
+        int rc = dalloc(g, &p, N);
+        if (rc) return rc;
+        g->d_mat = (uint8_t *)p;
+    }
+    HIPOK(g, hipMemcpyAsync(g->d_mat, m8.data(), N, hipMemcpyHostToDevice, g->stream));
+    int rc = is_fp64(g) ? upload_materials_T<double>(g) : upload_materials_T<float>(g);
+    if (rc) return rc;
+    long nd = 0;
+    std::vector<int32_t> slot;
+    bool any = false;
+    for (unsigned m = 0; m < g->Nmat; m++) any = any || g->mats[m].rc;
+    dfree_t(g->d_slot);
+    dfree_t(g->d_psi);
+    g->n_disp = 0;
+    if (any) {
+        slot.assign(N, -1);
+        for (size_t c = 0; c < N; c++)
+            if (g->mats[g->matsgrid[c]].rc) slot[c] = (int32_t)nd++;
+        void *p;
+        if ((rc = dalloc(g, &p, N * sizeof(int32_t)))) return rc;
+        g->d_slot = (int32_t *)p;
+        HIPOK(g, hipMemcpyAsync(g->d_slot, slot.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        size_t np = (size_t)3 * g->np_max * nd;
+        if ((rc = dalloc(g, &p, np * sizeof(double2)))) return rc;
+        g->d_psi = (double2 *)p;
+        HIPOK(g, hipMemsetAsync(g->d_psi, 0, np * sizeof(double2), g->stream));
+        g->n_disp = nd;
+    }
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    return AEFDTD_OK;
+}
+
+int ensure_ready(aefdtd_grid *g) {
+    if (g->ready) return AEFDTD_OK;
+    int rc = upload_grid(g);
+    if (rc) return rc;
+    if ((rc = upload_profiles(g))) return rc;
+    g->pml_dirty = false;
+    g->ready = true;
+    return AEFDTD_OK;
+}
+
+void prof_begin(aefdtd_grid *g, int kind, hipEvent_t *b) {
+    *b = nullptr;
+    if (!g->prof_on) return;
+    hipEvent_t a;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(b) != hipSuccess) { *b = nullptr; return; }
+    (void)hipEventRecord(a, g->stream);
+    g->events.push_back({kind, a, *b});
+}
+
+void prof_end(aefdtd_grid *g, hipEvent_t b) {
+    if (b) (void)hipEventRecord(b, g->stream);
+}
+
+void prof_collect(aefdtd_grid *g) {
+    if (g->events.empty()) return;
+    (void)hipStreamSynchronize(g->stream);
+    for (auto &e : g->events) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            g->prof_ms[e.kind] += ms;
+            g->prof_n[e.kind] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    g->events.clear();
+}
+
+dim3 field_grid(const aefdtd_grid *g) {
+    return dim3((g->Nx + BX - 1) / BX, (g->Ny + BY - 1) / BY, (g->Nz + KZ - 1) / KZ);
+}
+
+template <typename T>
+int launch_E(aefdtd_grid *g) {
+    EArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+    a.Ex = (T *)g->F[0]; a.Ey = (T *)g->F[1]; a.Ez = (T *)g->F[2];
+    a.Hx = (const T *)g->F[3]; a.Hy = (const T *)g->F[4]; a.Hz = (const T *)g->F[5];
+    a.mat = g->d_mat;
+    a.mt = (const MatDev<T> *)g->d_mtab;
+    a.poles = g->d_poles;
+    a.slot = g->d_slot;
+    a.psi = g->d_psi;
+    a.ndisp = g->n_disp;
+    a.pf = prof_of<T>(g);
+    for (int f = 0; f < 6; f++) a.pml[f] = g->pml[f];
+    a.pEyx = (T *)g->d_psipml[0]; a.pEzx = (T *)g->d_psipml[1];
+    a.pExy = (T *)g->d_psipml[4]; a.pEzy = (T *)g->d_psipml[5];
+    a.pExz = (T *)g->d_psipml[8]; a.pEyz = (T *)g->d_psipml[9];
+    a.en = (g->enable[0] ? 1 : 0) | (g->enable[1] ? 2 : 0) | (g->enable[2] ? 4 : 0);
+    a.inj_k = -1;
+    if (g->inj_k >= 0 && g->tstep < (int)g->inj[0].size()) {
+        a.inj_k = g->inj_k;
+        a.injx = (T)g->inj[0][g->tstep];
+        a.injy = (T)g->inj[1][g->tstep];
+    }
+    if (g->tfsf_aux) {
+        a.tfsf = 1;
+        a.tb0 = g->tb[0]; a.tb1 = g->tb[1]; a.tb2 = g->tb[2]; a.tb3 = g->tb[3]; a.tb4 = g->tb[4]; a.tb5 = g->tb[5];
+        a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
+        a.aHx = (const T *)g->tfsf_aux->F[3]; a.aHy = (const T *)g->tfsf_aux->F[4]; a.aHz = (const T *)g->tfsf_aux->F[5];
+    }
+    hipEvent_t ev;
+    prof_begin(g, 0, &ev);
+    hipLaunchKernelGGL(k_update_E<T>, field_grid(g), dim3(BX, BY, 1), 0, g->stream, a);
+    prof_end(g, ev);
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+template <typename T>
+int launch_H(aefdtd_grid *g) {
+    HArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+    a.Hx = (T *)g->F[3]; a.Hy = (T *)g->F[4]; a.Hz = (T *)g->F[5];
+    a.Ex = (const T *)g->F[0]; a.Ey = (const T *)g->F[1]; a.Ez = (const T *)g->F[2];
+    a.pf = prof_of<T>(g);
+    for (int f = 0; f < 6; f++) a.pml[f] = g->pml[f];
+    a.pHyx = (T *)g->d_psipml[2]; a.pHzx = (T *)g->d_psipml[3];
+    a.pHxy = (T *)g->d_psipml[6]; a.pHzy = (T *)g->d_psipml[7];
+    a.pHxz = (T *)g->d_psipml[10]; a.pHyz = (T *)g->d_psipml[11];
+    a.en = (g->enable[3] ? 8 : 0) | (g->enable[4] ? 16 : 0) | (g->enable[5] ? 32 : 0);
+    a.dtm = (T)g->dtm; a.dtdmx = (T)g->dtdmx; a.dtdmy = (T)g->dtdmy; a.dtdmz = (T)g->dtdmz;
+    a.inj_k = -1;
+    if (g->inj_k >= 0 && g->tstep < (int)g->inj[2].size()) {
+        a.inj_k = g->inj_k;
+        a.injx = (T)g->inj[2][g->tstep];
+        a.injy = (T)g->inj[3][g->tstep];
+    }
+    if (g->tfsf_aux) {
+        a.tfsf = 1;
+        a.tb0 = g->tb[0]; a.tb1 = g->tb[1]; a.tb2 = g->tb[2]; a.tb3 = g->tb[3]; a.tb4 = g->tb[4]; a.tb5 = g->tb[5];
+        a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
+        a.aEx = (const T *)g->tfsf_aux->F[0]; a.aEy = (const T *)g->tfsf_aux->F[1]; a.aEz = (const T *)g->tfsf_aux->F[2];
+    }
+    hipEvent_t ev;
+    prof_begin(g, 1, &ev);
+    hipLaunchKernelGGL(k_update_H<T>, field_grid(g), dim3(BX, BY, 1), 0, g->stream, a);
+    prof_end(g, ev);
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+// DFT coefficients of a sensor for steps [0, n): tapering_E/H * e^{i w step dt}
+// with tapering taken from the previous feed (sensors/sensors.cpp:232-243,464-466).
+void sensor_coefs(const aefdtd_grid *g, const Sensor &s, int n, std::vector<double2> &out) {
+    out.resize((size_t)n * 2 * s.Nl);
+    const std::complex<double> I(0.0, 1.0);
+    double tapE = 1.0, tapH = 1.0;
+    for (int st = 0; st < n; st++) {
+        if (st > 0 && st - 1 >= g->Nt - g->Ntap) {
+            tapE = s_curve(st - 1, g->Nt - 1.0, g->Nt - g->Ntap);
+            tapH = s_curve(st - 1 + 0.5, g->Nt - 1.0, g->Nt - g->Ntap);
+        }
+        for (int l = 0; l < s.Nl; l++) {
+            double w = 2.0 * kPi * kC / s.lambda[l];
+            std::complex<double> cE = tapE * std::exp(w * st * g->Dt * I);
+            std::complex<double> cH = tapH * std::exp(w * (st + 0.5) * g->Dt * I);
+            out[((size_t)st * 2 + 0) * s.Nl + l] = make_double2(cE.real(), cE.imag());
+            out[((size_t)st * 2 + 1) * s.Nl + l] = make_double2(cH.real(), cH.imag());
+        }
+    }
+}
+
+int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
+    if (upto <= s.ncoef) return AEFDTD_OK;
+    int n = std::max(upto, std::max(g->Nt, 2 * s.ncoef));
+    std::vector<double2> h;
+    sensor_coefs(g, s, n, h);
+    void *p;
+    int rc = dalloc(g, &p, h.size() * sizeof(double2));
+    if (rc) return rc;
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    dfree_t(s.d_coef);
+    s.d_coef = (double2 *)p;
+    HIPOK(g, hipMemcpy(s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    s.ncoef = n;
+    return AEFDTD_OK;
+}
+
+template <typename T>
+int flush_dft(aefdtd_grid *g) {
+    if (g->stage_fill == 0) return AEFDTD_OK;
+    int s0 = g->sensor_step - g->stage_fill;
+    for (Sensor *s : g->sensors) {
+        int rc = ensure_coefs(g, *s, g->sensor_step);
+        if (rc) return rc;
+        dim3 grid((unsigned)((s->P + 255) / 256), 6, 1);
+        hipEvent_t ev;
+        prof_begin(g, 3, &ev);
+        hipLaunchKernelGGL(k_dft<T>, grid, dim3(256), 0, g->stream, (const T *)g->d_stage, g->stage_P, s->off, s->P,
+                           s->Nl, (const double2 *)s->d_coef, s0, g->stage_fill, s->d_acc);
+        prof_end(g, ev);
+        HIPOK(g, hipGetLastError());
+    }
+    g->stage_fill = 0;
+    return AEFDTD_OK;
+}
+
+template <typename T>
+int feed_T(aefdtd_grid *g) {
+    if (!g->sensors.empty()) {
+        int rc;
+        if (!g->d_stage) {
+            // first feed: lay the sensors out in one stage pool and upload their descriptors
+            long Ptot = 0;
+            std::vector<SensDesc> sd;
+            for (Sensor *s : g->sensors) {
+                s->off = Ptot;
+                Ptot += s->P;
+                sd.push_back({s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off});
+                g->stage_maxP = std::max(g->stage_maxP, s->P);
+            }
+            g->stage_P = Ptot;
+            if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
+            if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
+            HIPOK(g, hipMemcpy(g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
+        }
+        dim3 grid((unsigned)((g->stage_maxP + 255) / 256), (unsigned)g->sensors.size(), 1);
+        hipEvent_t ev;
+        prof_begin(g, 2, &ev);
+        hipLaunchKernelGGL(k_sample<T>, grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc, g->stage_fill,
+                           g->stage_P, (const T *)g->F[0], (const T *)g->F[1], (const T *)g->F[2], (const T *)g->F[3],
+                           (const T *)g->F[4], (const T *)g->F[5], g->Nx, g->Ny, g->Nz, (T *)g->d_stage);
+        prof_end(g, ev);
+        HIPOK(g, hipGetLastError());
+        g->stage_fill++;
+        g->sensor_step++;
+        if (g->stage_fill == KS) {
+            if ((rc = flush_dft<T>(g))) return rc;
+        }
+    }
+    for (PlaneSum *s : g->sums) {
+        int t = g->tstep - 1;
+        if (t < 0 || t >= g->Nt) continue;
+        hipLaunchKernelGGL(k_plane_sum<T>, dim3(1), dim3(256), 0, g->stream, (const T *)g->F[0], (const T *)g->F[1],
+                           (const T *)g->F[2], g->Nx, g->Ny, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
+                           g->Dy, g->Dz, s->zpos, (const double *)s->d_tap, t, g->Dt, s->d_out);
+        HIPOK(g, hipGetLastError());
+        s->n = std::max(s->n, t + 1);
+    }
+    return AEFDTD_OK;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+extern "C" {
+
+const char *aefdtd_last_error(const aefdtd_grid *g) {
+    if (g) return g->err.c_str();
+    return g_create_err.c_str();
+}
+
+int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, double Dy, double Dz, double Dt,
+                  int mode, const int pml[6], const int pad[6], int precision) {
+    auto cfail = [&](int code, const std::string &m) {
+        std::lock_guard<std::mutex> lk(g_create_mu);
+        g_create_err = m;
+        return code;
+    };
+    if (!out) return cfail(AEFDTD_EINVAL, "null output handle");
+    *out = nullptr;
+    if (Nx < 1 || Ny < 1 || Nz < 1 || Nt < 0) return cfail(AEFDTD_EINVAL, "bad grid dimensions");
+    if (!(Dx > 0) || !(Dy > 0) || !(Dz > 0) || !(Dt > 0)) return cfail(AEFDTD_EINVAL, "bad discretisation");
+    if (precision != AEFDTD_FP64 && precision != AEFDTD_FP32) return cfail(AEFDTD_EINVAL, "bad precision");
+    if (mode != AEFDTD_MODE_NORM && mode != AEFDTD_MODE_CUSTOM) return cfail(AEFDTD_EINVAL, "unsupported grid mode");
+    for (int f = 0; f < 6; f++)
+        if (pml[f] < 0 || pad[f] < 0) return cfail(AEFDTD_EINVAL, "negative pml/pad");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return cfail(AEFDTD_EDEVICE, "no HIP device visible");
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return cfail(AEFDTD_EDEVICE, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return cfail(AEFDTD_EDEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
+    auto *g = new aefdtd_grid();
+    g->precision = precision;
+    g->mode = mode;
+    g->Nt = Nt;
+    g->Dx = Dx; g->Dy = Dy; g->Dz = Dz; g->Dt = Dt;
+    for (int f = 0; f < 6; f++) {
+        g->pml[f] = pml[f]; g->pad[f] = pad[f];
+        g->pml_kappa[f] = 1.0; g->pml_sigma[f] = 0.0; g->pml_alpha[f] = 0.0;
+    }
+    // padding rules of core/fdtd_core.cpp:212-243 and extension of :245-251
+    for (int a = 0; a < 3; a++)
+        if (pml[2 * a] || pml[2 * a + 1]) {
+            g->pad[2 * a] = std::max(5, g->pad[2 * a]);
+            g->pad[2 * a + 1] = std::max(5, g->pad[2 * a + 1]);
+        }
+    g->Nx_s = Nx; g->Ny_s = Ny; g->Nz_s = Nz;
+    g->Nx = Nx + pml[0] + g->pad[0] + g->pad[1] + pml[1];
+    g->Ny = Ny + pml[2] + g->pad[2] + g->pad[3] + pml[3];
+    g->Nz = Nz + pml[4] + g->pad[4] + g->pad[5] + pml[5];
+    g->xs_s = pml[0] + g->pad[0]; g->xs_e = g->xs_s + Nx;
+    g->ys_s = pml[2] + g->pad[2]; g->ys_e = g->ys_s + Ny;
+    g->zs_s = pml[4] + g->pad[4]; g->zs_e = g->zs_s + Nz;
+    if ((size_t)g->Nx * g->Ny * g->Nz > ((size_t)1 << 34)) {
+        delete g;
+        return cfail(AEFDTD_EINVAL, "grid too large");
+    }
+    // basic_differentials_compute (core/fdtd_core.cpp:1137-1148)
+    g->dtm = Dt / kMu0;
+    g->dtdex = Dt / (Dx * kE0); g->dtdey = Dt / (Dy * kE0); g->dtdez = Dt / (Dz * kE0);
+    g->dtdmx = Dt / (Dx * kMu0); g->dtdmy = Dt / (Dy * kMu0); g->dtdmz = Dt / (Dz * kMu0);
+    size_t N = g->ncell();
+    g->matsgrid.assign(N, 0);
+    g->mats.assign(1, HostMat());
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return cfail(AEFDTD_EDEVICE, "hipStreamCreate failed");
+    }
+    for (int c = 0; c < 6; c++) {
+        if (dalloc(g, &g->F[c], N * g->tsize()) || hipMemsetAsync(g->F[c], 0, N * g->tsize(), g->stream) != hipSuccess) {
+            std::string e = g->err.empty() ? "field allocation failed" : g->err;
+            aefdtd_destroy(g);
+            return cfail(AEFDTD_ENOMEM, e);
+        }
+    }
+    if (alloc_pml_state(g)) {
+        std::string e = g->err;
+        aefdtd_destroy(g);
+        return cfail(AEFDTD_ENOMEM, e);
+    }
+    *out = g;
+    return AEFDTD_OK;
+}
+
+int aefdtd_destroy(aefdtd_grid *g) {
+    if (!g) return AEFDTD_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    prof_collect(g);
+    for (int c = 0; c < 6; c++) dfree(g->F[c]);
+    dfree_t(g->d_mat);
+    dfree(g->d_mtab);
+    dfree_t(g->d_poles);
+    dfree_t(g->d_slot);
+    dfree_t(g->d_psi);
+    dfree(g->d_prof);
+    for (int s = 0; s < 12; s++) dfree(g->d_psipml[s]);
+    dfree(g->d_stage);
+    dfree(g->d_sdesc);
+    for (Sensor *s : g->sensors) {
+        dfree_t(s->d_acc);
+        dfree_t(s->d_coef);
+        delete s;
+    }
+    for (PlaneSum *s : g->sums) {
+        dfree_t(s->d_tap);
+        dfree_t(s->d_out);
+        delete s;
+    }
+    if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return AEFDTD_OK;
+}
+
+int aefdtd_layout(const aefdtd_grid *g, int out[16]) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    int v[16] = {g->Nx, g->Ny, g->Nz, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e,
+                 g->pad[0], g->pad[1], g->pad[2], g->pad[3], g->pad[4], g->pad[5], g->tstep};
+    std::memcpy(out, v, sizeof(v));
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_pml(aefdtd_grid *g, int face, double kap, double sig, double alp) {   // fdtd_pml.cpp:457-497
+    if (!g) return AEFDTD_EINVAL;
+    if (face < 0 || face > 5) return fail(g, AEFDTD_EINVAL, "bad face");
+    double D = face < 2 ? g->Dx : (face < 4 ? g->Dy : g->Dz);
+    g->pml_kappa[face] = kap;
+    g->pml_sigma[face] = pml_sigma_scale(D) * sig;
+    g->pml_alpha[face] = alp;
+    g->pml_dirty = true;
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_tapering(aefdtd_grid *g, int Ntap) {
+    if (!g) return AEFDTD_EINVAL;
+    if (Ntap < 0) return fail(g, AEFDTD_EINVAL, "negative tapering");
+    g->Ntap = Ntap;
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_matsgrid(aefdtd_grid *g, const uint32_t *ids, int nx, int ny, int nz) {   // fdtd_utils.cpp:572-796
+    if (!g || !ids) return AEFDTD_EINVAL;
+    if (nx != g->Nx_s || ny != g->Ny_s || nz != g->Nz_s) return fail(g, AEFDTD_EINVAL, "matsgrid size mismatch");
+    aefdtd_grid &G = *g;
+    auto M = [&](int i, int j, int k) -> uint32_t & {
+        return G.matsgrid[(size_t)i + (size_t)G.Nx * ((size_t)j + (size_t)G.Ny * k)];
+    };
+    for (int k = G.zs_s; k < G.zs_e; k++)
+        for (int j = G.ys_s; j < G.ys_e; j++)
+            for (int i = G.xs_s; i < G.xs_e; i++)
+                M(i, j, k) = ids[(size_t)(i - G.xs_s) + (size_t)nx * ((size_t)(j - G.ys_s) + (size_t)ny * (k - G.zs_s))];
+    for (int k = G.zs_s; k < G.zs_e; k++)
+        for (int j = G.ys_s; j < G.ys_e; j++) {
+            for (int i = 0; i < G.xs_s; i++) M(i, j, k) = M(G.xs_s, j, k);
+            for (int i = G.xs_e; i < G.Nx; i++) M(i, j, k) = M(G.xs_e - 1, j, k);
+        }
+    for (int k = G.zs_s; k < G.zs_e; k++) {
+        for (int j = 0; j < G.ys_s; j++)
+            for (int i = 0; i < G.Nx; i++) M(i, j, k) = M(i, G.ys_s, k);
+        for (int j = G.ys_e; j < G.Ny; j++)
+            for (int i = 0; i < G.Nx; i++) M(i, j, k) = M(i, G.ys_e - 1, k);
+    }
+    for (int k = 0; k < G.zs_s; k++)
+        for (int j = 0; j < G.Ny; j++)
+            for (int i = 0; i < G.Nx; i++) M(i, j, k) = M(i, j, G.zs_s);
+    for (int k = G.zs_e; k < G.Nz; k++)
+        for (int j = 0; j < G.Ny; j++)
+            for (int i = 0; i < G.Nx; i++) M(i, j, k) = M(i, j, G.zs_e - 1);
+    unsigned nmat = G.Nmat;
+    for (uint32_t v : G.matsgrid) nmat = std::max(nmat, v + 1);
+    if (nmat > 256) return fail(g, AEFDTD_EINVAL, "more than 256 material indices");
+    G.Nmat = nmat;
+    G.mats.assign(nmat, HostMat());
+    G.ready = false;
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_material(aefdtd_grid *g, unsigned int index, const aefdtd_material *m) {   // fdtd_utils.cpp:810-820
+    if (!g || !m) return AEFDTD_EINVAL;
+    if (index >= g->Nmat) return AEFDTD_OK;
+    bool present = false;
+    for (uint32_t v : g->matsgrid)
+        if (v == index) { present = true; break; }
+    if (!present) return AEFDTD_OK;
+    material_coefficients(*m, g->Dx, g->Dy, g->Dz, g->Dt, g->mats[index]);
+    g->ready = false;
+    return AEFDTD_OK;
+}
+
+int aefdtd_disable_fields(aefdtd_grid *g, const int *fields, int n) {
+    if (!g) return AEFDTD_EINVAL;
+    for (int i = 0; i < n; i++)
+        if (fields[i] >= 0 && fields[i] < 6) g->enable[fields[i]] = false;
+    return AEFDTD_OK;
+}
+
+int aefdtd_bootstrap(aefdtd_grid *g) {     // core/fdtd_core.cpp:1095-1135
+    if (!g) return AEFDTD_EINVAL;
+    int rc = alloc_pml_state(g);
+    if (rc) return rc;
+    g->ready = false;
+    return ensure_ready(g);
+}
+
+int aefdtd_update_E(aefdtd_grid *g) {
+    if (!g) return AEFDTD_EINVAL;
+    int rc = ensure_ready(g);
+    if (rc) return rc;
+    if (g->tstep == 0 || g->pml_dirty) {     // pml_coeff_calc at tstep==0 (core/fdtd_core.cpp:1165)
+        if ((rc = upload_profiles(g))) return rc;
+        g->pml_dirty = false;
+    }
+    if (g->tfsf_aux && !g->tfsf_aux->ready) return fail(g, AEFDTD_ESTATE, "TFSF auxiliary grid not set up");
+    return is_fp64(g) ? launch_E<double>(g) : launch_E<float>(g);
+}
+
+int aefdtd_update_H(aefdtd_grid *g) {
+    if (!g) return AEFDTD_EINVAL;
+    int rc = ensure_ready(g);
+    if (rc) return rc;
+    rc = is_fp64(g) ? launch_H<double>(g) : launch_H<float>(g);
+    if (rc) return rc;
+    g->tstep += 1;
+    return AEFDTD_OK;
+}
+
+int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
+    if (!g) return AEFDTD_EINVAL;
+    size_t N = g->ncell();
+    for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->tsize(), g->stream));
+    for (int s = 0; s < 12; s++)
+        if (g->d_psipml[s]) HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, g->psipml_n[s] * g->tsize(), g->stream));
+    if (g->d_psi) HIPOK(g, hipMemsetAsync(g->d_psi, 0, (size_t)3 * g->np_max * g->n_disp * sizeof(double2), g->stream));
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    return AEFDTD_OK;
+}
+
+int aefdtd_read_field(aefdtd_grid *g, int f, double *out) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
+    size_t N = g->ncell();
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    if (is_fp64(g)) {
+        HIPOK(g, hipMemcpy(out, g->F[f], N * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(N);
+        HIPOK(g, hipMemcpy(t.data(), g->F[f], N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) out[i] = t[i];
+    }
+    return AEFDTD_OK;
+}
+
+int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
+    if (!g || !in) return AEFDTD_EINVAL;
+    if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    int rc = upload_T(g, g->F[f], in, g->ncell());
+    if (rc) return rc;
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_plane_injection(aefdtd_grid *g, int k, int n, const double *dEx, const double *dEy, const double *dHx,
+                               const double *dHy) {
+    if (!g) return AEFDTD_EINVAL;
+    if (k < 0 || k >= g->Nz || n < 0 || (n > 0 && (!dEx || !dEy || !dHx || !dHy)))
+        return fail(g, AEFDTD_EINVAL, "bad injection plane");
+    g->inj_k = k;
+    g->inj[0].assign(dEx, dEx + n); g->inj[1].assign(dEy, dEy + n);
+    g->inj[2].assign(dHx, dHx + n); g->inj[3].assign(dHy, dHy + n);
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_tfsf(aefdtd_grid *g, const aefdtd_grid *aux_c, int i1, int i2, int j1, int j2, int k1, int k2, int px,
+                    int py) {
+    if (!g || !aux_c) return AEFDTD_EINVAL;
+    aefdtd_grid *aux = const_cast<aefdtd_grid *>(aux_c);
+    if ((!px && (i1 < 1 || i2 >= g->Nx)) || (!py && (j1 < 1 || j2 >= g->Ny)) || i1 < 0 || j1 < 0 || i2 > g->Nx ||
+        j2 > g->Ny || i1 >= i2 || j1 >= j2 || k1 < 1 || k2 >= g->Nz || k1 >= k2)
+        return fail(g, AEFDTD_EINVAL, "bad TFSF box");
+    if (aux->Nx != 1 || aux->Ny != 1 || aux->Nz != g->Nz || aux->precision != g->precision)
+        return fail(g, AEFDTD_EINVAL, "TFSF auxiliary grid must be 1x1xNz with the same precision");
+    // the auxiliary grid is stepped on the main grid's stream so that both
+    // follow one order of launches
+    if (aux->stream != g->stream) {
+        HIPOK(g, hipStreamSynchronize(aux->stream));
+        if (aux->own_stream) (void)hipStreamDestroy(aux->stream);
+        aux->stream = g->stream;
+        aux->own_stream = false;
+    }
+    g->tfsf_aux = aux;
+    int b[6] = {i1, i2, j1, j2, k1, k2};
+    std::memcpy(g->tb, b, sizeof(b));
+    g->tf_per_x = px != 0;
+    g->tf_per_y = py != 0;
+    return AEFDTD_OK;
+}
+
+int aefdtd_add_spect_poynting(aefdtd_grid *g, int type, int x1, int x2, int y1, int y2, int z1, int z2, int Nl,
+                              const double *lambda, int *id) {
+    if (!g || !lambda || !id) return AEFDTD_EINVAL;
+    if (type < 0 || type > 5) return fail(g, AEFDTD_EINVAL, "bad orientation");
+    if (g->sensor_step > 0 || g->d_stage) return fail(g, AEFDTD_ESTATE, "sensors must be added before the first feed");
+    auto *s = new Sensor();
+    s->type = type;
+    s->x1 = std::clamp(x1 + g->xs_s, 0, g->Nx); s->x2 = std::clamp(x2 + g->xs_s, 0, g->Nx);
+    s->y1 = std::clamp(y1 + g->ys_s, 0, g->Ny); s->y2 = std::clamp(y2 + g->ys_s, 0, g->Ny);
+    s->z1 = std::clamp(z1 + g->zs_s, 0, g->Nz); s->z2 = std::clamp(z2 + g->zs_s, 0, g->Nz);
+    if (type == AEFDTD_NORMAL_X || type == AEFDTD_NORMAL_XM) { s->span1 = s->y2 - s->y1; s->span2 = s->z2 - s->z1; }
+    else if (type == AEFDTD_NORMAL_Y || type == AEFDTD_NORMAL_YM) { s->span1 = s->x2 - s->x1; s->span2 = s->z2 - s->z1; }
+    else { s->span1 = s->x2 - s->x1; s->span2 = s->y2 - s->y1; }
+    bool plane_ok = (type <= 1) ? s->x1 < g->Nx : (type <= 3 ? s->y1 < g->Ny : s->z1 < g->Nz);
+    if (s->span1 <= 0 || s->span2 <= 0 || Nl < 1 || !plane_ok) {
+        delete s;
+        return fail(g, AEFDTD_EINVAL, "empty sensor");
+    }
+    s->Nl = Nl;
+    s->lambda.assign(lambda, lambda + Nl);
+    s->P = (long)s->span1 * s->span2;
+    void *p;
+    int rc = dalloc(g, &p, (size_t)6 * Nl * s->P * sizeof(double2));
+    if (rc) { delete s; return rc; }
+    s->d_acc = (double2 *)p;
+    HIPOK(g, hipMemsetAsync(s->d_acc, 0, (size_t)6 * Nl * s->P * sizeof(double2), g->stream));
+    g->sensors.push_back(s);
+    *id = (int)g->sensors.size() - 1;
+    return AEFDTD_OK;
+}
+
+int aefdtd_feed_sensors(aefdtd_grid *g) {
+    if (!g) return AEFDTD_EINVAL;
+    int rc = ensure_ready(g);
+    if (rc) return rc;
+    return is_fp64(g) ? feed_T<double>(g) : feed_T<float>(g);
+}
+
+int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
+    if (!g || !rsp) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");
+    int rc = is_fp64(g) ? flush_dft<double>(g) : flush_dft<float>(g);
+    if (rc) return rc;
+    Sensor &s = *g->sensors[id];
+    double f;
+    if (s.type == AEFDTD_NORMAL_X || s.type == AEFDTD_NORMAL_XM) f = g->Dy * g->Dz;
+    else if (s.type == AEFDTD_NORMAL_Y || s.type == AEFDTD_NORMAL_YM) f = g->Dx * g->Dz;
+    else f = g->Dx * g->Dy;
+    double *d;
+    if ((rc = dalloc(g, (void **)&d, s.Nl * sizeof(double)))) return rc;
+    hipLaunchKernelGGL(k_poynting, dim3(s.Nl), dim3(256), 0, g->stream, (const double2 *)s.d_acc, s.P, s.Nl, f, s.type, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e == hipSuccess) e = hipMemcpy(rsp, d, s.Nl * sizeof(double), hipMemcpyDeviceToHost);
+    void *v = d;
+    g->dev_bytes -= s.Nl * sizeof(double);
+    dfree(v);
+    if (e != hipSuccess) return fail(g, AEFDTD_EDEVICE, std::string("poynting: ") + hipGetErrorString(e));
+    return AEFDTD_OK;
+}
+
+int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]) {
+    if (!g || !dims) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");
+    int rc = is_fp64(g) ? flush_dft<double>(g) : flush_dft<float>(g);
+    if (rc) return rc;
+    Sensor &s = *g->sensors[id];
+    dims[0] = s.span1; dims[1] = s.span2; dims[2] = s.Nl;
+    if (!out) return AEFDTD_OK;
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    HIPOK(g, hipMemcpy(out, s.d_acc, (size_t)6 * s.Nl * s.P * sizeof(double2), hipMemcpyDeviceToHost));
+    return AEFDTD_OK;
+}
+
+int aefdtd_add_plane_sum(aefdtd_grid *g, int k, const aefdtd_pulse *inc, double zpos, int subtract, const double *tap,
+                         int *id) {
+    if (!g || !tap || !id) return AEFDTD_EINVAL;
+    if (k < 0 || k >= g->Nz) return fail(g, AEFDTD_EINVAL, "bad plane");
+    if (g->Nt < 1) return fail(g, AEFDTD_EINVAL, "plane sums need Nt >= 1");
+    auto *s = new PlaneSum();
+    s->k = k;
+    s->has_inc = inc != nullptr;
+    s->subtract = subtract != 0 && inc != nullptr;
+    if (inc) s->inc = *inc;
+    else std::memset(&s->inc, 0, sizeof(s->inc));
+    s->zpos = zpos;
+    s->cap = g->Nt;
+    void *p;
+    int rc = dalloc(g, &p, g->Nt * sizeof(double));
+    if (rc) { delete s; return rc; }
+    s->d_tap = (double *)p;
+    if ((rc = dalloc(g, &p, (size_t)g->Nt * 6 * sizeof(double)))) { dfree_t(s->d_tap); delete s; return rc; }
+    s->d_out = (double *)p;
+    HIPOK(g, hipMemcpy(s->d_tap, tap, g->Nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(g, hipMemset(s->d_out, 0, (size_t)g->Nt * 6 * sizeof(double)));
+    g->sums.push_back(s);
+    *id = (int)g->sums.size() - 1;
+    return AEFDTD_OK;
+}
+
+int aefdtd_read_plane_sum(aefdtd_grid *g, int id, int n, double *out) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->sums.size()) return fail(g, AEFDTD_EINVAL, "bad sum id");
+    PlaneSum &s = *g->sums[id];
+    int m = std::min(n, s.cap);
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    HIPOK(g, hipMemcpy(out, s.d_out, (size_t)m * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    return AEFDTD_OK;
+}
+
+int aefdtd_profile_enable(aefdtd_grid *g, int on) {
+    if (!g) return AEFDTD_EINVAL;
+    prof_collect(g);
+    g->prof_on = on != 0;
+    if (on) {
+        for (int k = 0; k < 4; k++) { g->prof_ms[k] = 0; g->prof_n[k] = 0; }
+    }
+    return AEFDTD_OK;
+}
+
+int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches) {
+    if (!g || !kernel) return AEFDTD_EINVAL;
+    prof_collect(g);
+    int k = -1;
+    if (!std::strcmp(kernel, "E")) k = 0;
+    else if (!std::strcmp(kernel, "H")) k = 1;
+    else if (!std::strcmp(kernel, "sensor")) k = 2;
+    else if (!std::strcmp(kernel, "dft")) k = 3;
+    if (k < 0) return fail(g, AEFDTD_EINVAL, "unknown kernel name");
+    if (total_ms) *total_ms = g->prof_ms[k];
+    if (launches) *launches = g->prof_n[k];
+    return AEFDTD_OK;
+}
+
+int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes) {
+    if (!g || !bytes) return AEFDTD_EINVAL;
+    *bytes = g->dev_bytes;
+    return AEFDTD_OK;
+}
+
+}  // extern "C"

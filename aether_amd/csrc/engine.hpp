@@ -1,0 +1,113 @@
+// engine.hpp -- host-side state of one FDTD grid on the device, shared by the
+// HIP translation unit (kernels + grid C ABI) and the host mode drivers.
+#pragma once
+
+#include "../../include/aether_fdtd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace aefdtd {
+
+using cplx = std::complex<double>;
+
+// SI constants of Aether (src/common/phys_constants.h:19-21) and Pi (mathUT.h:34).
+constexpr double kE0 = 8.854187817e-12;
+constexpr double kMu0 = 12.566370614e-7;
+constexpr double kC = 299792458.0;
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// Update coefficients of one material index (FDTD_Material, core/fdtd_material.h:38-268).
+struct HostMat {
+    bool set = false;      // set_material was applied (else Aether's default object)
+    bool rc = false;       // recursive-convolution poles present (m_type == MAT_RC)
+    double C1 = 1, C2x = 0, C2y = 0, C2z = 0, C3 = 0, C4 = 0, ei = 1, sig = 0;
+    std::vector<cplx> Crec, chi, Dchi;
+};
+
+// Host side of the pole conversion and the per-material constants; defined in
+// host_setup.cpp.
+void material_coefficients(const aefdtd_material &m, double Dx, double Dy, double Dz,
+                           double Dt, HostMat &out);
+void pml_profile(double ind, int N, int pm, int pp, const double kap[2], const double sig[2],
+                 const double alp[2], double Dt, double &kappa, double &b, double &c);
+double pml_sigma_scale(double D);
+void pulse_init(aefdtd_pulse &p, double li, double lf, double inc, double pol, double dir,
+                double sx, double sy, double sz, double n_ind);
+double pulse_eval(const aefdtd_pulse &p, int comp, double x, double y, double z, double t);
+double s_curve(double x, double xmin, double xmax);
+
+struct Sensor;      // planar spectral Poynting sensor (engine.hip)
+struct PlaneSum;    // normal-incidence R/T plane sum (engine.hip)
+
+}  // namespace aefdtd
+
+struct aefdtd_grid {
+    // ---- parameters (as the members of Aether's FDTD, core/fdtd_core.h:33-77)
+    int precision = AEFDTD_FP64;
+    int mode = AEFDTD_MODE_CUSTOM;
+    int tstep = 0;
+    int Nx = 0, Ny = 0, Nz = 0, Nt = 0, Ntap = 0;
+    int Nx_s = 0, Ny_s = 0, Nz_s = 0;
+    int xs_s = 0, xs_e = 0, ys_s = 0, ys_e = 0, zs_s = 0, zs_e = 0;
+    int pad[6] = {0, 0, 0, 0, 0, 0};
+    int pml[6] = {0, 0, 0, 0, 0, 0};
+    double Dx = 0, Dy = 0, Dz = 0, Dt = 0;
+    double dtm = 0, dtdex = 0, dtdey = 0, dtdez = 0, dtdmx = 0, dtdmy = 0, dtdmz = 0;
+    double pml_kappa[6], pml_sigma[6], pml_alpha[6];
+    bool enable[6] = {true, true, true, true, true, true};
+
+    // ---- host copies
+    std::vector<uint32_t> matsgrid;
+    unsigned Nmat = 1;
+    std::vector<aefdtd::HostMat> mats;
+    std::vector<double> kE[3], kH[3], bE[3], bH[3], cE[3], cH[3];
+
+    // ---- device state
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    void *F[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // T* fields
+    uint8_t *d_mat = nullptr;
+    void *d_mtab = nullptr;           // per-material coefficient table (MatDev<T>)
+    double2 *d_poles = nullptr;       // per pole: Crec, Dchi
+    int32_t *d_slot = nullptr;        // dispersive slot per cell, -1 elsewhere
+    double2 *d_psi = nullptr;         // RC accumulators [3*Npmax][Ndisp]
+    long n_disp = 0;
+    int np_max = 0;
+    void *d_prof = nullptr;           // 1D profile arrays (T): ikE/ikH/bE/bH/cE/cH per axis
+    void *d_psipml[12] = {};          // CPML slab state, T
+    size_t psipml_n[12] = {};
+    // driver actions
+    int inj_k = -1;
+    std::vector<double> inj[4];
+    aefdtd_grid *tfsf_aux = nullptr;
+    int tb[6] = {0, 0, 0, 0, 0, 0};
+    bool tf_per_x = false, tf_per_y = false;
+    // sensors
+    std::vector<aefdtd::Sensor *> sensors;
+    std::vector<aefdtd::PlaneSum *> sums;
+    int sensor_step = 0;              // Sensor::step shared by every registered sensor
+    int stage_fill = 0;
+    void *d_stage = nullptr;          // [KS][6][Ptot] T
+    void *d_sdesc = nullptr;          // sensor descriptors for the sampling kernel
+    long stage_P = 0, stage_maxP = 0;
+    // bookkeeping
+    bool ready = false;               // device copies up to date
+    bool pml_dirty = true;
+    size_t dev_bytes = 0;
+    std::string err;
+    // profiling
+    bool prof_on = false;
+    struct Ev { int kind; hipEvent_t a, b; };
+    std::vector<Ev> events;
+    double prof_ms[4] = {0, 0, 0, 0};
+    long prof_n[4] = {0, 0, 0, 0};
+
+    size_t ncell() const { return (size_t)Nx * Ny * Nz; }
+    size_t tsize() const { return precision == AEFDTD_FP64 ? 8 : 4; }
+};

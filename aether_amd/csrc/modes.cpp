@@ -1,0 +1,318 @@
+// modes.cpp -- host mode drivers over the grid C ABI.  They replace Aether's
+// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) and
+// FDTD_single_particle (modes/single_particle.cpp:29-415): same setup, same
+// per-step sequence, same post-processing, with the fields resident on the
+// device and every per-step field access done by device passes.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace aefdtd;
+
+namespace {
+
+std::atomic<bool> g_cancel{false};
+
+struct Handle {
+    aefdtd_grid *g = nullptr;
+    ~Handle() { aefdtd_destroy(g); }
+};
+
+void lambda_grid(int Nl, double lmin, double lmax, std::vector<double> &L) {   // normal_incidence.cpp:235
+    L.resize(Nl);
+    for (int l = 0; l < Nl; l++) L[l] = lmin + (lmax - lmin) * l / (Nl - 1.0);
+}
+
+// Box_Spect_Poynting (sensors/s_spec_power.cpp:125-232) becomes six planar
+// sensors; a planar request one sensor.
+int add_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, std::vector<std::vector<int>> &ids) {
+    for (int r = 0; r < n; r++) {
+        const aefdtd_sensor_req &q = req[r];
+        if (q.Nl < 1) return AEFDTD_EINVAL;
+        std::vector<double> L;
+        lambda_grid(q.Nl, q.lambda_min, q.lambda_max, L);
+        std::vector<int> v;
+        if (q.box) {
+            const int t[6] = {AEFDTD_NORMAL_XM, AEFDTD_NORMAL_X, AEFDTD_NORMAL_YM, AEFDTD_NORMAL_Y, AEFDTD_NORMAL_ZM, AEFDTD_NORMAL_Z};
+            const int b[6][6] = {{q.x1, q.x1 + 1, q.y1, q.y2, q.z1, q.z2}, {q.x2 - 1, q.x2, q.y1, q.y2, q.z1, q.z2},
+                                 {q.x1, q.x2, q.y1, q.y1 + 1, q.z1, q.z2}, {q.x1, q.x2, q.y2 - 1, q.y2, q.z1, q.z2},
+                                 {q.x1, q.x2, q.y1, q.y2, q.z1, q.z1 + 1}, {q.x1, q.x2, q.y1, q.y2, q.z2 - 1, q.z2}};
+            for (int f = 0; f < 6; f++) {
+                int id = -1;
+                if (!q.disable[f]) {
+                    int rc = aefdtd_add_spect_poynting(g, t[f], b[f][0], b[f][1], b[f][2], b[f][3], b[f][4], b[f][5], q.Nl, L.data(), &id);
+                    if (rc) return rc;
+                }
+                v.push_back(id);
+            }
+        } else {
+            int id = -1;
+            int rc = aefdtd_add_spect_poynting(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.Nl, L.data(), &id);
+            if (rc) return rc;
+            v.push_back(id);
+        }
+        ids.push_back(v);
+    }
+    return AEFDTD_OK;
+}
+
+// Spect_Poynting::treat per plane; Box_Spect_Poynting::treat sums the faces
+// in the order xm, xp, ym, yp, zm, zp (s_spec_power.cpp:199-207).
+int collect_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, const std::vector<std::vector<int>> &ids,
+                     double *out) {
+    size_t off = 0;
+    for (int r = 0; r < n; r++) {
+        int Nl = req[r].Nl;
+        std::vector<double> tmp(Nl);
+        if (req[r].box) {
+            std::vector<double> tot(Nl, 0.0), faces(6 * (size_t)Nl, 0.0);
+            for (int f = 0; f < 6; f++) {
+                if (ids[r][f] < 0) continue;
+                int rc = aefdtd_sensor_poynting(g, ids[r][f], tmp.data());
+                if (rc) return rc;
+                std::memcpy(&faces[f * (size_t)Nl], tmp.data(), Nl * sizeof(double));
+            }
+            for (int l = 0; l < Nl; l++)
+                for (int f = 0; f < 6; f++)
+                    if (ids[r][f] >= 0) tot[l] += faces[f * (size_t)Nl + l];
+            if (out) {
+                std::memcpy(out + off, tot.data(), Nl * sizeof(double));
+                std::memcpy(out + off + Nl, faces.data(), 6 * (size_t)Nl * sizeof(double));
+            }
+            off += 7 * (size_t)Nl;
+        } else {
+            int rc = aefdtd_sensor_poynting(g, ids[r][0], tmp.data());
+            if (rc) return rc;
+            if (out) std::memcpy(out + off, tmp.data(), Nl * sizeof(double));
+            off += Nl;
+        }
+    }
+    return AEFDTD_OK;
+}
+
+int copy_fields(aefdtd_grid *g, double *out) {
+    if (!out) return AEFDTD_OK;
+    size_t N = g->ncell();
+    for (int c = 0; c < 6; c++) {
+        int rc = aefdtd_read_field(g, c, out + c * N);
+        if (rc) return rc;
+    }
+    return AEFDTD_OK;
+}
+
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+void aefdtd_cancel(void) { g_cancel = true; }
+
+// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607)
+int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                                aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
+    if (!m || !res || !m->matsgrid || m->Nt < 2 || m->Nl < 2) return AEFDTD_EINVAL;
+    g_cancel = false;
+    const int Nt = m->Nt;
+    const double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
+    int pad_zm = std::max(5, m->pad[4]), pad_zp = std::max(5, m->pad[5]);
+    double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * kC) * 0.99 * m->time_mod;
+    const int pml[6] = {0, 0, 0, 0, m->pml[4], m->pml[5]};
+    const int pad[6] = {0, 0, 0, 0, pad_zm, pad_zp};
+    Handle H;
+    TRY(aefdtd_create(&H.g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_NORM, pml, pad, m->precision));
+    aefdtd_grid *g = H.g;
+    TRY(aefdtd_set_pml(g, 4, m->kappa[4], m->sigma[4], m->alpha[4]));
+    TRY(aefdtd_set_pml(g, 5, m->kappa[5], m->sigma[5], m->alpha[5]));
+    TRY(aefdtd_set_tapering(g, m->tapering));
+    TRY(aefdtd_set_matsgrid(g, m->matsgrid, m->nx, m->ny, m->nz));
+    for (int i = 0; i < m->n_materials; i++) TRY(aefdtd_set_material(g, i, &m->materials[i]));
+    TRY(aefdtd_disable_fields(g, m->disable_fields, m->n_disable));
+    const int zs_s = g->zs_s, zs_e = g->zs_e;
+    const int Nl = m->Nl;
+    std::vector<double> Lambda, w(Nl);
+    lambda_grid(Nl, m->lambda_min, m->lambda_max, Lambda);
+    for (int l = 0; l < Nl; l++) w[l] = 2.0 * kPi * kC / Lambda[l];
+    double pol = (m->polarization == 1) ? 90 : 0;
+    auto mg = [&](int i, int j, int k) { return g->matsgrid[(size_t)i + (size_t)g->Nx * ((size_t)j + (size_t)g->Ny * k)]; };
+    double eps_sub = g->mats[mg(0, 0, zs_s)].ei, eps_sup = g->mats[mg(0, 0, zs_e)].ei;
+    double index_sub = std::sqrt(eps_sub), index_sup = std::sqrt(eps_sup);
+    aefdtd_pulse chp;
+    pulse_init(chp, m->lambda_min, m->lambda_max, 0, pol, 0, 0, 0, (zs_e - 1) * Dz, index_sup);
+    TRY(aefdtd_bootstrap(g));
+    // per-step injection values (normal_incidence.cpp:334-357) and tapering (:384-385)
+    std::vector<double> dEx(Nt), dEy(Nt), dHx(Nt), dHy(Nt), tap(Nt);
+    for (int t = 0; t < Nt; t++) {
+        double tb = t * Dt;
+        double iHx = pulse_eval(chp, 3, 0, 0, (zs_e - 1 + 0.5) * Dz, tb);
+        double iHy = pulse_eval(chp, 4, 0, 0, (zs_e - 1 + 0.5) * Dz, tb);
+        dEx[t] = -(g->dtdez * iHy / eps_sup);
+        dEy[t] = g->dtdez * iHx / eps_sup;
+        tb = (t + 0.5) * Dt;
+        double iEx = pulse_eval(chp, 0, 0, 0, (zs_e - 1) * Dz, tb);
+        double iEy = pulse_eval(chp, 1, 0, 0, (zs_e - 1) * Dz, tb);
+        dHx[t] = g->dtdmz * iEy;
+        dHy[t] = -(g->dtdmz * iEx);
+        tap[t] = (t >= Nt - m->tapering) ? s_curve(t, Nt - 1.0, Nt - m->tapering) : 1.0;
+    }
+    TRY(aefdtd_set_plane_injection(g, zs_e - 1, Nt, dEx.data(), dEy.data(), dHx.data(), dHy.data()));
+    std::vector<std::vector<int>> ids;
+    TRY(add_requests(g, n_sensors, sensors, ids));
+    int idR, idT;
+    TRY(aefdtd_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &idR));
+    TRY(aefdtd_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT));
+    int t;
+    for (t = 0; t < Nt; t++) {
+        TRY(aefdtd_update_E(g));
+        TRY(aefdtd_update_H(g));
+        TRY(aefdtd_feed_sensors(g));
+        if (g_cancel) break;     // end_computation poll (normal_incidence.cpp:431-435): Nt=t
+    }
+    const int Nrun = t;
+    std::vector<double> R(6 * (size_t)Nt), T(6 * (size_t)Nt);
+    TRY(aefdtd_read_plane_sum(g, idR, Nt, R.data()));
+    TRY(aefdtd_read_plane_sum(g, idT, Nt, T.data()));
+    // find_slab (core/fdtd_utils.cpp:347-392)
+    int Nsub = 0, Nsup = 0;
+    long IDsup = 0, IDsub = 0, IDmoy;
+    for (int i = g->xs_s; i < g->xs_e; i++)
+        for (int j = g->ys_s; j < g->ys_e; j++) { IDsup += mg(i, j, zs_e); IDsub += mg(i, j, zs_s); }
+    IDmoy = IDsub;
+    int k = zs_s;
+    while (IDmoy == IDsub && k < zs_e - 1) {
+        IDmoy = 0;
+        for (int i = g->xs_s; i < g->xs_e; i++)
+            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg(i, j, k + 1);
+        Nsub++; k++;
+    }
+    IDmoy = IDsup;
+    k = zs_e - 1;
+    while (IDmoy == IDsup && k >= zs_s) {
+        IDmoy = 0;
+        for (int i = g->xs_s; i < g->xs_e; i++)
+            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg(i, j, k - 1);
+        Nsup++; k--;
+    }
+    double hsup = (Nsup - 1) * Dz, hsub = Nsub * Dz, hstruc = (g->Nz_s - Nsup - Nsub) * Dz;
+    hsup -= Dz / 2.0;   // "phase calibration" of normal_incidence.cpp:448-449
+    hsub -= Dz / 2.0;
+    // post-run DFT of the plane sums (normal_incidence.cpp:455-488)
+    const std::complex<double> I(0.0, 1.0);
+    for (int l = 0; l < Nl; l++) {
+        double k0 = -w[l] / kC, k1 = k0 * index_sup, k3 = k0 * index_sub;
+        std::complex<double> tmp_sup = std::exp(2.0 * k1 * hsup * I);
+        std::complex<double> tmp_sub = std::exp((k3 * hsub - k0 * (hstruc + hsub)) * I);
+        std::complex<double> r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+        for (int tt = 0; tt < Nrun; tt++) {
+            std::complex<double> tmp1 = std::exp(tt * Dt * w[l] * I) / (Nrun - 1.0);
+            std::complex<double> tmp2 = tmp1 * tmp_sup, tmp3 = tmp1 * tmp_sub;
+            for (int c = 0; c < 3; c++) {
+                r1[c] += R[6 * (size_t)tt + 3 + c] * tmp1;
+                r2[c] += R[6 * (size_t)tt + c] * tmp2;
+                t1[c] += T[6 * (size_t)tt + 3 + c] * tmp1;
+                t2[c] += T[6 * (size_t)tt + c] * tmp3;
+            }
+        }
+        double a = std::sqrt(std::norm(r2[0]) + std::norm(r2[1]) + std::norm(r2[2]));
+        double b = std::sqrt(std::norm(r1[0]) + std::norm(r1[1]) + std::norm(r1[2]));
+        if (res->lambda) res->lambda[l] = Lambda[l];
+        if (res->R) res->R[l] = a / b;
+        a = std::sqrt(std::norm(t2[0]) + std::norm(t2[1]) + std::norm(t2[2]));
+        b = std::sqrt(std::norm(t1[0]) + std::norm(t1[1]) + std::norm(t1[2]));
+        if (res->T) res->T[l] = a / b;
+        std::complex<double> ref0 = r1[1], trans0 = t1[1];
+        if (m->polarization == 1) { ref0 = r1[0]; trans0 = t1[0]; }
+        std::complex<double> v[4] = {r2[0] / ref0, r2[1] / ref0, t2[0] / trans0, t2[1] / trans0};
+        double *dst[4] = {res->refx, res->refy, res->transx, res->transy};
+        for (int q = 0; q < 4; q++)
+            if (dst[q]) { dst[q][2 * l] = v[q].real(); dst[q][2 * l + 1] = v[q].imag(); }
+    }
+    res->index_sup = index_sup;
+    res->index_sub = index_sub;
+    res->Nt_run = Nrun;
+    TRY(collect_requests(g, n_sensors, sensors, ids, sensor_out));
+    TRY(copy_fields(g, fields_out));
+    return g_cancel ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
+
+// FDTD_single_particle (modes/single_particle.cpp:29-415)
+int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                               double *sensor_out, double *fields_out) {
+    if (!m || !m->matsgrid || m->Nt < 1) return AEFDTD_EINVAL;
+    g_cancel = false;
+    const int Nt = m->Nt;
+    const double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
+    int pml_x = std::max(m->pml[0], m->pml[1]), pml_y = std::max(m->pml[2], m->pml[3]);
+    bool periodic_x = pml_x <= 0, periodic_y = pml_y <= 0;
+    double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * kC) * 0.99 * m->time_mod;
+    int pad[6];
+    for (int f = 0; f < 6; f++) pad[f] = std::max(10, m->pad[f]);
+    if (pml_x <= 0) pad[0] = pad[1] = 0;
+    if (pml_y <= 0) pad[2] = pad[3] = 0;
+    Handle G, A;
+    TRY(aefdtd_create(&G.g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, pad, m->precision));
+    const int apml[6] = {0, 0, 0, 0, m->pml[4], m->pml[5]};
+    const int apad[6] = {0, 0, 0, 0, pad[4], pad[5]};
+    TRY(aefdtd_create(&A.g, 1, 1, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, apml, apad, m->precision));
+    aefdtd_grid *g = G.g, *a = A.g;
+    for (int f = 0; f < 6; f++) TRY(aefdtd_set_pml(g, f, m->kappa[f], m->sigma[f], m->alpha[f]));
+    TRY(aefdtd_set_pml(a, 4, m->kappa[4], m->sigma[4], m->alpha[4]));
+    TRY(aefdtd_set_pml(a, 5, m->kappa[5], m->sigma[5], m->alpha[5]));
+    TRY(aefdtd_set_matsgrid(g, m->matsgrid, m->nx, m->ny, m->nz));
+    std::vector<uint32_t> col(m->nz);
+    for (int k = 0; k < m->nz; k++) col[k] = m->matsgrid[(size_t)m->nx * m->ny * k];   // aux_grid(0,0,k)=matsgrid(0,0,k)
+    TRY(aefdtd_set_matsgrid(a, col.data(), 1, 1, m->nz));
+    for (int i = 0; i < m->n_materials; i++) {
+        TRY(aefdtd_set_material(g, i, &m->materials[i]));
+        TRY(aefdtd_set_material(a, i, &m->materials[i]));
+    }
+    TRY(aefdtd_disable_fields(g, m->disable_fields, m->n_disable));
+    const int zs_e = g->zs_e;
+    double pol = 0;
+    if (m->polarization == 1) pol = 90;
+    else if (m->polarization == 2) pol = m->polar_angle;
+    double eps_sup = g->mats[g->matsgrid[(size_t)g->Nx * g->Ny * zs_e]].ei;
+    aefdtd_pulse chp;
+    pulse_init(chp, m->lambda_min, m->lambda_max, 0, pol, 0, 0, 0, (zs_e - 1 + 5) * Dz, 1.0);
+    TRY(aefdtd_bootstrap(g));
+    TRY(aefdtd_bootstrap(a));
+    // auxiliary-grid injection values (single_particle.cpp:270-275,343-348)
+    std::vector<double> dEx(Nt), dEy(Nt), dHx(Nt), dHy(Nt);
+    for (int t = 0; t < Nt; t++) {
+        double tb = t * Dt;
+        double iHx = pulse_eval(chp, 3, 0, 0, (zs_e - 1 + 5.5) * Dz, tb);
+        double iHy = pulse_eval(chp, 4, 0, 0, (zs_e - 1 + 5.5) * Dz, tb);
+        dEx[t] = -(g->dtdez * iHy / eps_sup);
+        dEy[t] = g->dtdez * iHx / eps_sup;
+        tb = (t + 0.5) * Dt;
+        double iEx = pulse_eval(chp, 0, 0, 0, (zs_e - 1 + 5) * Dz, tb);
+        double iEy = pulse_eval(chp, 1, 0, 0, (zs_e - 1 + 5) * Dz, tb);
+        dHx[t] = g->dtdmz * iEy;
+        dHy[t] = -(g->dtdmz * iEx);
+    }
+    TRY(aefdtd_set_plane_injection(a, zs_e - 1 + 5, Nt, dEx.data(), dEy.data(), dHx.data(), dHy.data()));
+    TRY(aefdtd_set_tfsf(g, a, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e, periodic_x, periodic_y));
+    std::vector<std::vector<int>> ids;
+    TRY(add_requests(g, n_sensors, sensors, ids));
+    for (int t = 0; t < Nt; t++) {
+        TRY(aefdtd_update_E(a));
+        TRY(aefdtd_update_E(g));
+        TRY(aefdtd_update_H(a));
+        TRY(aefdtd_update_H(g));
+        TRY(aefdtd_feed_sensors(g));
+        if (g_cancel) break;
+    }
+    TRY(collect_requests(g, n_sensors, sensors, ids, sensor_out));
+    TRY(copy_fields(g, fields_out));
+    return g_cancel ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
+
+}  // extern "C"

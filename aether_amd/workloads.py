@@ -1,0 +1,131 @@
+"""Synthetic inputs of the benchmark configurations.
+
+Structures are discretised the way Aether's Structure::discretize does it
+(src/structure/structure.cpp:94-113): cell (i,j,k) takes the material of the
+last structure operation that contains the corner point (i*Dx, j*Dy, k*Dz),
+else the default material.  Spheres follow Add_Sphere::index
+(src/structure/structure_primitives.cpp:490-500, closed ball) and layers
+Add_Layer::index (:323-338, half-open z1 <= z < z2).
+
+Random numbers come from a 64-bit Mersenne twister (mt19937_64) mapped to
+doubles as (x >> 11) * 2**-53, which is portable across languages.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .capi import FP32, FP64, Material, ModeSpec, SensorReq
+
+
+class MT19937_64:
+    """Standard 64-bit Mersenne twister (Matsumoto & Nishimura, 2000)."""
+    _N, _M = 312, 156
+    _A = 0xB5026F5AA96619E9
+    _UM, _LM = 0xFFFFFFFF80000000, 0x7FFFFFFF
+    _MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.mt = [0] * self._N
+        self.mt[0] = seed & self._MASK
+        for i in range(1, self._N):
+            prev = self.mt[i - 1]
+            self.mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & self._MASK
+        self.idx = self._N
+
+    def next_u64(self):
+        N, M = self._N, self._M
+        if self.idx >= N:
+            mt = self.mt
+            for i in range(N):
+                x = (mt[i] & self._UM) | (mt[(i + 1) % N] & self._LM)
+                xa = x >> 1
+                if x & 1:
+                    xa ^= self._A
+                mt[i] = mt[(i + M) % N] ^ xa
+            self.idx = 0
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEB6A0000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self._MASK
+
+    def uniform(self, a=0.0, b=1.0):
+        return a + (b - a) * ((self.next_u64() >> 11) * 2.0 ** -53)
+
+
+# ---------------------------------------------------------------- materials
+# Gold-like Drude pole of the 256^3 configuration plus the two critical-point
+# poles of mat_lib/Au_400n_1000n_Vial.lua:8-9, eps_inf 1.1.
+GOLD = Material(eps_inf=1.1, drude=[(1.37e16, 1.07e14)],
+                critpoint=[(0.86822, 4.0812e15, -0.60756, 7.3277e14),
+                           (1.3700, 6.4269e15, -0.087341, 6.7371e14)])
+
+
+def voxelize_spheres(n, D, spheres, background=0):
+    """spheres: list of (x0, y0, z0, r, mat) in metres; later ones win."""
+    nx, ny, nz = n
+    g = np.full((nz, ny, nx), background, dtype=np.uint32)
+    x = np.arange(nx) * D[0]
+    y = np.arange(ny) * D[1]
+    z = np.arange(nz) * D[2]
+    for (x0, y0, z0, r, mat) in spheres:
+        i0, i1 = max(0, int(math.floor((x0 - r) / D[0])) - 1), min(nx, int(math.ceil((x0 + r) / D[0])) + 2)
+        j0, j1 = max(0, int(math.floor((y0 - r) / D[1])) - 1), min(ny, int(math.ceil((y0 + r) / D[1])) + 2)
+        k0, k1 = max(0, int(math.floor((z0 - r) / D[2])) - 1), min(nz, int(math.ceil((z0 + r) / D[2])) + 2)
+        if i0 >= i1 or j0 >= j1 or k0 >= k1:
+            continue
+        dx = x[i0:i1] - x0
+        dy = y[j0:j1] - y0
+        dz = z[k0:k1] - z0
+        inside = (dx[None, None, :] * dx[None, None, :] + dy[None, :, None] * dy[None, :, None]
+                  + dz[:, None, None] * dz[:, None, None]) <= r * r
+        g[k0:k1, j0:j1, i0:i1][inside] = mat
+    return g
+
+
+def random_spheres(seed, count, L, rmin, rmax, mat=1, max_tries=200000):
+    """Non-overlapping spheres, radius U[rmin,rmax], centre U[r, L-r]^3, by rejection."""
+    rng = MT19937_64(seed)
+    out = []
+    tries = 0
+    while len(out) < count and tries < max_tries:
+        tries += 1
+        r = rng.uniform(rmin, rmax)
+        c = [rng.uniform(r, L[a] - r) for a in range(3)]
+        if all((c[0] - s[0]) ** 2 + (c[1] - s[1]) ** 2 + (c[2] - s[2]) ** 2 > (r + s[3]) ** 2 for s in out):
+            out.append((c[0], c[1], c[2], r, mat))
+    return out
+
+
+# ---------------------------------------------------------------- configs
+def slab_normal_incidence(n=(64, 64, 64), D=5e-9, slab=(22, 42), eps=2.25, Nt=2000, Nl=50,
+                          pml_z=8, precision=FP64):
+    """Plain 3D normal-incidence run: dielectric slab between z cells slab[0]..slab[1]
+    (Add_Layer corner sampling), 8-cell CPML on +-z, periodic x/y, x-polarised
+    pulse, Courant factor 0.95 of the 3D limit (time_mod = 0.95/0.99)."""
+    nx, ny, nz = n
+    g = np.zeros((nz, ny, nx), dtype=np.uint32)
+    z = np.arange(nz) * D
+    g[(z >= slab[0] * D) & (z < slab[1] * D)] = 1
+    return ModeSpec(matsgrid=g, materials=[Material(1.0), Material(eps)], Nt=Nt, Dx=D, Dy=D, Dz=D,
+                    time_mod=0.95 / 0.99, pml=(0, 0, 0, 0, pml_z, pml_z), Nl=Nl,
+                    lambda_min=400e-9, lambda_max=800e-9, polarization=1, precision=precision)
+
+
+def spheres_single_particle(n=256, D=2e-9, count=40, rmin=8e-9, rmax=20e-9, seed=2, Nt=10000, Nl=64,
+                            pml=12, eps_bg=1.77, precision=FP32, time_mod=1.0):
+    """Scatterer run: gold-like Drude + 2 critical-point spheres in a eps=1.77
+    background, CPML on all six faces, TFSF plane wave, box spectral-Poynting
+    sensor on the structure boundary."""
+    L = (n * D,) * 3
+    sph = random_spheres(seed, count, L, rmin, rmax)
+    g = voxelize_spheres((n, n, n), (D, D, D), sph)
+    spec = ModeSpec(matsgrid=g, materials=[Material(eps_bg), GOLD], Nt=Nt, Dx=D, Dy=D, Dz=D,
+                    time_mod=time_mod, pml=(pml,) * 6, Nl=Nl, lambda_min=400e-9, lambda_max=800e-9,
+                    polarization=1, precision=precision)
+    sensor = SensorReq(box=True, x1=0, x2=n, y1=0, y2=n, z1=0, z2=n, Nl=Nl, lambda_min=400e-9, lambda_max=800e-9)
+    return spec, [sensor], sph

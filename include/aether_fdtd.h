@@ -1,0 +1,260 @@
+/*
+ * aether_fdtd.h -- C ABI of the MI355X (gfx950) FDTD engine.
+ *
+ * The engine is a drop-in for Aether's 3D/2D Yee solver
+ * (src/finite_differences/FDTD).  Every entry point below replaces one
+ * member of Aether's C++ `class FDTD` (src/finite_differences/FDTD/core/
+ * fdtd_core.h:26-285), one per-step action that Aether's mode drivers perform
+ * by touching `fdtd.Ex(i,j,k)` directly between `update_E()` and `update_H()`
+ * (those actions move onto the device so that the fields stay resident in
+ * HBM), or one mode driver (modes/lua_fdtd.h:91-108).
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross the ABI.  Every function returns
+ *    an int status (AEFDTD_OK == 0).  aefdtd_last_error() gives the text of
+ *    the last failure on a handle (or of the last failed create when called
+ *    with NULL).
+ *  - Grids are x-fastest: value (i,j,k) lives at i + j*Nx + k*Nx*Ny, as in
+ *    Aether's Grid3 (src/common/grid.h:868-877).
+ *  - Indices of faces: 0 xm, 1 xp, 2 ym, 3 yp, 4 zm, 5 zp.
+ *  - Field ids: 0 Ex, 1 Ey, 2 Ez, 3 Hx, 4 Hy, 5 Hz (Aether's EX_FIELD..HZ_FIELD
+ *    order, src/common/enum_constants.h).
+ *  - A handle is single-threaded: the caller serialises calls on one handle.
+ *    Distinct handles may be driven from distinct host threads.
+ *  - The engine owns all device memory.  Inputs are copied; outputs are
+ *    written to caller buffers.
+ */
+#ifndef AETHER_FDTD_H
+#define AETHER_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ----------------------------------------------------- */
+#define AEFDTD_OK          0
+#define AEFDTD_EINVAL      1   /* bad argument or dimension                   */
+#define AEFDTD_ENOMEM      2   /* device allocation failed / over HBM budget  */
+#define AEFDTD_EDEVICE     3   /* HIP runtime error, or no usable gfx950 GPU  */
+#define AEFDTD_ESTATE      4   /* call out of order (e.g. step before setup)  */
+#define AEFDTD_ECANCELLED  5   /* run stopped by aefdtd_cancel                */
+
+/* ---- grid modes (the `smod` string of FDTD::FDTD, fdtd_core.cpp:198-204) */
+#define AEFDTD_MODE_NORM    0  /* "NORM"   : normal-incidence slab driver     */
+#define AEFDTD_MODE_CUSTOM  5  /* "CUSTOM" : single-particle / custom drivers */
+
+/* ---- storage precision of the fields ------------------------------------ */
+#define AEFDTD_FP64  0     /* FP64 fields and arithmetic (parity form)         */
+#define AEFDTD_FP32  1     /* FP32 fields/arithmetic; FP64 RC poles and DFT    */
+
+/* ---- sensor orientations (Aether's NORMAL_X.. enum order) --------------- */
+#define AEFDTD_NORMAL_X   0
+#define AEFDTD_NORMAL_XM  1
+#define AEFDTD_NORMAL_Y   2
+#define AEFDTD_NORMAL_YM  3
+#define AEFDTD_NORMAL_Z   4
+#define AEFDTD_NORMAL_ZM  5
+
+/* A material as Aether's FDTD sees it: eps_inf plus pole lists.
+ * Mirrors the FDTD-compatible part of `Material` (src/materials/material.h)
+ * and the setters of Dielec_model (src/materials/dielec_models.cpp:233,297,
+ * 365,430).  Arrays hold n_* consecutive tuples. */
+typedef struct aefdtd_material {
+    double        eps_inf;
+    int           n_debye;     const double *debye;      /* (ds, t0)       */
+    int           n_drude;     const double *drude;      /* (wd, g)        */
+    int           n_lorentz;   const double *lorentz;    /* (A, O, G)      */
+    int           n_critpoint; const double *critpoint;  /* (A, O, P, G)   */
+} aefdtd_material;
+
+/* Analytic Gaussian plane-wave pulse; the state of Aether's ChpIn after
+ * init_pulse (core/chpin.cpp:479-545). Filled by aefdtd_pulse_init. */
+typedef struct aefdtd_pulse {
+    double lambda, omega, k, kx, ky, kz, n_ind, dz, C1;
+    double Ex0, Ey0, Ez0, Hx0, Hy0, Hz0;
+} aefdtd_pulse;
+
+typedef struct aefdtd_grid aefdtd_grid;   /* opaque handle == one `FDTD` */
+
+/* ======================================================================== */
+/*  Grid object: replaces `class FDTD` (core/fdtd_core.h:26-285)            */
+/* ======================================================================== */
+
+/* FDTD::FDTD(Nx,..,pad_zp) -- core/fdtd_core.cpp:158-285.  Applies Aether's
+ * padding rules (:212-243) and grid extension (:245-251).  pml[6]/pad[6] in
+ * face order.  `precision` is AEFDTD_FP64 or AEFDTD_FP32. */
+int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt,
+                  double Dx, double Dy, double Dz, double Dt, int mode,
+                  const int pml[6], const int pad[6], int precision);
+int aefdtd_destroy(aefdtd_grid *g);
+const char *aefdtd_last_error(const aefdtd_grid *g);
+
+/* Layout after extension: out[0..2]=Nx,Ny,Nz (full grid); out[3..8]=xs_s,xs_e,
+ * ys_s,ys_e,zs_s,zs_e; out[9..14]=pad_xm..pad_zp; out[15]=tstep.
+ * Mirrors the public members of FDTD (fdtd_core.h:35-44,75-77). */
+int aefdtd_layout(const aefdtd_grid *g, int out[16]);
+
+/* FDTD::set_pml_xm..zp -- core/fdtd_pml.cpp:457-497 (sigma scaled there). */
+int aefdtd_set_pml(aefdtd_grid *g, int face, double kappa, double sigma, double alpha);
+/* FDTD::set_tapering -- core/fdtd_utils.cpp:915-918. */
+int aefdtd_set_tapering(aefdtd_grid *g, int Ntap);
+/* FDTD::set_matsgrid -- core/fdtd_utils.cpp:572-796: copies the structure
+ * grid (nx,ny,nz == the Nx,Ny,Nz given to create) and extends its edge
+ * planes into pads and PML. */
+int aefdtd_set_matsgrid(aefdtd_grid *g, const uint32_t *ids, int nx, int ny, int nz);
+/* FDTD::set_material -- core/fdtd_utils.cpp:810-820 (no-op for an index
+ * absent from the grid, as mats_in_grid). Derives C1..C4 and the recursive-
+ * convolution pole constants (core/mats_RC.cpp:126-184, mats_const.cpp:18-80).*/
+int aefdtd_set_material(aefdtd_grid *g, unsigned int index, const aefdtd_material *m);
+/* FDTD::disable_fields -- core/fdtd_core.cpp:1150-1161. */
+int aefdtd_disable_fields(aefdtd_grid *g, const int *fields, int n);
+/* FDTD::bootstrap -- core/fdtd_core.cpp:1095-1135 (reallocates CPML state,
+ * recomputes profiles). Also uploads everything set so far to the device. */
+int aefdtd_bootstrap(aefdtd_grid *g);
+/* FDTD::update_E / update_H -- core/fdtd_core.cpp:1163-1218, 1316-1335.
+ * One fused device pass each: RC-ante, E stencil, [TFSF-E], RC-apply, CPML-E,
+ * PEC, [plane injection]  /  H stencil, CPML-H, PEC, [plane injection],
+ * [TFSF-H].  Bracketed actions run only when registered below, in the place
+ * where Aether's drivers perform them. update_H advances tstep. */
+int aefdtd_update_E(aefdtd_grid *g);
+int aefdtd_update_H(aefdtd_grid *g);
+/* FDTD::reset_fields -- core/fdtd_utils.cpp:855-881. */
+int aefdtd_reset_fields(aefdtd_grid *g);
+
+/* Host access to the public Grid3<double> Ex..Hz (fdtd_core.h:62). Full
+ * extended grid, x-fastest, converted to/from double. Synchronous. */
+int aefdtd_read_field(aefdtd_grid *g, int field, double *out);
+int aefdtd_write_field(aefdtd_grid *g, int field, const double *in);
+
+/* ---- per-step actions of the mode drivers, moved onto the device ------- */
+
+/* Plane injection of normal_incidence.cpp:334-357 / single_particle.cpp:
+ * 270-275,343-348: after each update_E, Ex(:,:,k)+=dE[0][t], Ey(:,:,k)+=dE[1][t];
+ * after each update_H, Hx(:,:,k)+=dH[0][t], Hy(:,:,k)+=dH[1][t], t = tstep.
+ * Tables hold nsteps values each and are copied to the device. */
+int aefdtd_set_plane_injection(aefdtd_grid *g, int k_plane, int nsteps,
+                               const double *dEx, const double *dEy,
+                               const double *dHx, const double *dHy);
+
+/* Total-field/scattered-field box of single_particle.cpp:277-332 (E) and
+ * :350-378 (H), fed by the 1D auxiliary grid `aux` advanced in lockstep
+ * (the caller steps aux.update_E before g.update_E and aux.update_H before
+ * g.update_H, as the reference does). Box in full-grid indices. */
+int aefdtd_set_tfsf(aefdtd_grid *g, const aefdtd_grid *aux,
+                    int i1, int i2, int j1, int j2, int k1, int k2,
+                    int periodic_x, int periodic_y);
+
+/* Planar spectral Poynting sensor, Spect_Poynting (sensors/s_spec_power.cpp:
+ * 32-119) over SensorFieldHolder with interpolation (sensors/sensors.cpp:
+ * 423-644). Location in structure cells, offset and clamped as Sensor::link
+ * (sensors.cpp:196-230). lambda[Nl] copied. Writes the sensor id. */
+int aefdtd_add_spect_poynting(aefdtd_grid *g, int orientation,
+                              int x1, int x2, int y1, int y2, int z1, int z2,
+                              int Nl, const double *lambda, int *id);
+/* Sensor::feed for every registered sensor (sensors.cpp:232-243): one
+ * interpolated plane sample + running DFT, tapering as Aether. */
+int aefdtd_feed_sensors(aefdtd_grid *g);
+/* Spect_Poynting::treat -- rsp_result[l] for l < Nl (s_spec_power.cpp:44-119).*/
+int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp);
+/* Raw DFT accumulators sp_Ex..sp_Hz of one sensor: out[c][l][j][i] complex
+ * as (re,im), c in Ex,Ey,Ez,Hx,Hy,Hz; span1 x span2 written to dims[0..1]. */
+int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]);
+
+/* Reflection/transmission plane sums of normal_incidence.cpp:378-415: each
+ * feed appends sum_ij (F(i,j,k)-s*inc)*tap and sum_ij inc*tap for F=Ex,Ey,Ez,
+ * where inc is the analytic pulse at (i*Dx(+Dx/2), j*Dy(+Dy/2), z_pos(+Dz/2))
+ * and time (tstep-1+0.5)*Dt, s = subtract (1 for the reflection plane, 0 for
+ * the transmission plane). tap[Nt] is copied. */
+int aefdtd_add_plane_sum(aefdtd_grid *g, int k_plane, const aefdtd_pulse *inc,
+                         double z_pos, int subtract, const double *tap, int *id);
+/* Reads nsteps x 6 doubles: (sumEx-s*inc, sumEy-s*inc, sumEz-s*inc, incX, incY, incZ). */
+int aefdtd_read_plane_sum(aefdtd_grid *g, int id, int nsteps, double *out);
+
+/* ChpIn::init_pulse + setfields (core/chpin.cpp:479-545). Host only. */
+int aefdtd_pulse_init(aefdtd_pulse *p, double lambda_min, double lambda_max,
+                      double inc_deg, double pol_deg, double dir_deg,
+                      double startx, double starty, double startz, double n_ind);
+/* ChpIn::PulseEx..Hz (core/chpin.cpp:44-90); comp 0..5 = Ex..Hz. Host only. */
+double aefdtd_pulse_eval(const aefdtd_pulse *p, int comp,
+                         double x, double y, double z, double t);
+
+/* ======================================================================== */
+/*  Mode drivers: replace `void FDTD_normal_incidence(FDTD_Mode const&,..)` */
+/*  and `FDTD_single_particle` (modes/lua_fdtd.h:91-108)                    */
+/* ======================================================================== */
+
+/* The FDTD_Mode fields the drivers read (modes/lua_fdtd.h:22-89,
+ * core/lua_fd.h:51-150) plus the discretised structure. */
+typedef struct aefdtd_mode {
+    int     Nt;
+    double  Dx, Dy, Dz;
+    double  time_mod;
+    int     tapering;
+    int     pml[6];              /* after FDTD_Mode::finalize              */
+    int     pad[6];
+    double  kappa[6], sigma[6], alpha[6];
+    int     Nl;
+    double  lambda_min, lambda_max;
+    int     polarization;        /* 0 "TE", 1 "TM", 2 "mix"                */
+    double  polar_angle;         /* degrees, used by "mix"                 */
+    int     n_disable;  const int *disable_fields;
+    int     nx, ny, nz;  const uint32_t *matsgrid;   /* Structure::discretize */
+    int     n_materials; const aefdtd_material *materials;
+    int     precision;           /* AEFDTD_FP64 / AEFDTD_FP32              */
+} aefdtd_mode;
+
+/* A box or planar spectral Poynting sensor request (Sensor_generator,
+ * sensors/sensors.cpp:27-161, already to_discrete'd: structure cells). */
+typedef struct aefdtd_sensor_req {
+    int    box;                  /* 1: box_spectral_poynting, 0: planar   */
+    int    orientation;          /* planar only                           */
+    int    x1, x2, y1, y2, z1, z2;
+    int    disable[6];           /* box faces to skip                     */
+    int    Nl;
+    double lambda_min, lambda_max;
+} aefdtd_sensor_req;
+
+/* Results of the normal-incidence driver (normal_incidence.cpp:442-570):
+ * arrays of Nl. R,T are the `spectdata_norm` columns; refx..transy the
+ * complex amplitudes written to `spectdata_norm2` (as re,im pairs). */
+typedef struct aefdtd_normal_result {
+    double *lambda, *R, *T;
+    double *refx, *refy, *transx, *transy;   /* 2*Nl each */
+    double  index_sup, index_sub;
+    int     Nt_run;
+} aefdtd_normal_result;
+
+/* FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) on the device.
+ * sensor_out: per sensor Nl doubles of rsp_result (box: total), may be NULL.
+ * fields_out: if non-NULL, 6 x (full grid) doubles of the final fields. */
+int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors,
+                                const aefdtd_sensor_req *sensors,
+                                aefdtd_normal_result *res, double *sensor_out,
+                                double *fields_out);
+
+/* FDTD_single_particle (modes/single_particle.cpp:29-415) on the device:
+ * TFSF box fed by the 1D auxiliary grid. sensor_out as above; for a box
+ * sensor it holds Nl totals followed by 6*Nl per-face values (xm..zp). */
+int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors,
+                               const aefdtd_sensor_req *sensors,
+                               double *sensor_out, double *fields_out);
+
+/* Cancels a running mode driver between steps (the end_computation poll of
+ * normal_incidence.cpp:431-435). Safe to call from another host thread. */
+void aefdtd_cancel(void);
+
+/* ---- introspection used by the benchmark -------------------------------- */
+/* Per-kernel device time accumulated on the handle's stream with HIP events
+ * while profiling is enabled: names "E","H","sensor","dft".  Returns count. */
+int aefdtd_profile_enable(aefdtd_grid *g, int on);
+int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches);
+/* Bytes of device memory held by the handle. */
+int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AETHER_FDTD_H */

@@ -1,0 +1,43 @@
+"""Test-side checker: the CPU FP64 restatement of Aether's FDTD (liboracle.so)
+and, where the reference tree was present at build time, Aether's own compiled
+material/math layer (_ref/libaeref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libaeref.so")
+
+_OR = None
+
+
+def load():
+    """Backend over liboracle.so (prefix aeoracle_)."""
+    global _OR
+    if _OR is None:
+        from aether_amd.capi import Backend
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run make -C oracle)")
+        _OR = Backend(C.CDLL(ORACLE_SO), "aeoracle_")
+    return _OR
+
+
+def load_ref():
+    """Aether's compiled dielectric-model layer, or None when it was not built."""
+    if not os.path.exists(REF_SO):
+        return None
+    lib = C.CDLL(REF_SO)
+    dp = C.POINTER(C.c_double)
+    lib.aeref_time_exp.argtypes = [C.c_int, dp, dp]
+    lib.aeref_time_exp.restype = C.c_int
+    lib.aeref_eval.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
+    lib.aeref_eval.restype = C.c_int
+    lib.aeref_s_curve.argtypes = [C.c_double] * 3
+    lib.aeref_s_curve.restype = C.c_double
+    lib.aeref_nearest_integer.argtypes = [C.c_double]
+    lib.aeref_nearest_integer.restype = C.c_int
+    return lib

@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP engine")
+    config.addinivalue_line("markers", "slow: long CPU-side run")
+
+
+@pytest.fixture(scope="session")
+def oracle_be():
+    import oracle
+    return oracle.load()
+
+
+@pytest.fixture(scope="session")
+def engine_be():
+    from aether_amd import capi
+    return capi.engine()
