@@ -48,7 +48,7 @@ class MT19937_64:
         y = self.mt[self.idx]
         self.idx += 1
         y ^= (y >> 29) & 0x5555555555555555
-        y ^= (y << 17) & 0x71D67FFFEB6A0000
+        y ^= (y << 17) & 0x71D67FFFEDA60000
         y ^= (y << 37) & 0xFFF7EEE000000000
         y ^= y >> 43
         return y & self._MASK
