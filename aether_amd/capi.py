@@ -103,9 +103,18 @@ def _sig(lib, p):
                                      C.POINTER(CNormalResult), _dp, _dp]),
         "run_single_particle": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
         # engine only
+        "run_create": (i, [C.POINTER(vp), i, C.POINTER(CMode), i, C.POINTER(CSensorReq)]),
+        "run_step": (i, [vp, i]),
+        "run_progress": (i, [vp, _ip, _ip]),
+        "run_grid": (vp, [vp]),
+        "run_error": (C.c_char_p, [vp]),
+        "run_finish": (i, [vp, C.POINTER(CNormalResult), _dp, _dp]),
+        "run_destroy": (i, [vp]),
+        "synchronize": (i, [vp]),
         "profile_enable": (i, [vp, i]),
         "profile_read": (i, [vp, C.c_char_p, _dp, C.POINTER(C.c_long)]),
         "device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
+        "stats": (i, [vp, C.POINTER(C.c_long)]),
         "cancel": (None, []),
         # oracle only
         "step": (i, [vp, i]),
@@ -349,6 +358,96 @@ def run_single_particle(be: Backend, spec: ModeSpec, sensors=(), want_fields=Fal
     if fields is not None:
         out["fields"] = fields
     return out
+
+
+class Run:
+    """A mode driver split into create / step / finish (engine only)."""
+
+    def __init__(self, be: Backend, kind, spec: ModeSpec, sensors=()):
+        self.be, self.kind, self.spec, self.sensors = be, kind, spec, list(sensors)
+        self._keep = []
+        self._cm = spec.to_c(self._keep)
+        self._sreq = (CSensorReq * max(1, len(self.sensors)))(*[s.to_c() for s in self.sensors])
+        h = C.c_void_p()
+        rc = be.fn("run_create")(C.byref(h), kind, C.byref(self._cm), len(self.sensors), self._sreq)
+        be.check(rc, None)
+        self.h = h
+
+    def step(self, n):
+        rc = self.be.fn("run_step")(self.h, n)
+        if rc != OK:
+            raise FDTDError(rc, (self.be.fn("run_error")(self.h) or b"").decode())
+
+    def progress(self):
+        t, nt = C.c_int(), C.c_int()
+        self.be.fn("run_progress")(self.h, C.byref(t), C.byref(nt))
+        return t.value, nt.value
+
+    def grid(self):
+        return self.be.fn("run_grid")(self.h)
+
+    def synchronize(self):
+        self.be.check(self.be.fn("synchronize")(self.grid()), self.grid())
+
+    def stats(self):
+        out = (C.c_long * 4)()
+        self.be.fn("stats")(self.grid(), out)
+        return list(out)
+
+    def layout(self):
+        lay = (C.c_int * 16)()
+        self.be.fn("layout")(self.grid(), lay)
+        return list(lay)
+
+    def profile(self, on):
+        self.be.fn("profile_enable")(self.grid(), int(on))
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_long()
+        self.be.fn("profile_read")(self.grid(), kernel.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def finish(self, want_fields=False):
+        nsout = sum(s.out_len() for s in self.sensors)
+        sout = np.zeros(max(1, nsout))
+        out = {}
+        res = None
+        if self.kind == 0:
+            Nl = self.spec.Nl
+            for k in ("lambda", "R", "T"):
+                out[k] = np.zeros(Nl)
+            cplx = {k: np.zeros(2 * Nl) for k in ("refx", "refy", "transx", "transy")}
+            res = CNormalResult()
+            res.lambda_, res.R, res.T = _dptr(out["lambda"]), _dptr(out["R"]), _dptr(out["T"])
+            res.refx, res.refy = _dptr(cplx["refx"]), _dptr(cplx["refy"])
+            res.transx, res.transy = _dptr(cplx["transx"]), _dptr(cplx["transy"])
+        fields = None
+        if want_fields:
+            lay = self.layout()
+            fields = np.zeros((6, lay[2], lay[1], lay[0]))
+        rc = self.be.fn("run_finish")(self.h, C.byref(res) if res is not None else None, _dptr(sout),
+                                       _dptr(fields) if fields is not None else None)
+        if rc != OK:
+            raise FDTDError(rc, (self.be.fn("run_error")(self.h) or b"").decode())
+        if res is not None:
+            for k, v in cplx.items():
+                out[k] = v[0::2] + 1j * v[1::2]
+            out["index_sup"], out["index_sub"], out["Nt_run"] = res.index_sup, res.index_sub, res.Nt_run
+        out["sensors"] = sout[:nsout]
+        if fields is not None:
+            out["fields"] = fields
+        return out
+
+    def close(self):
+        if self.h:
+            self.be.fn("run_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Grid:

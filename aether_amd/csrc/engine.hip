@@ -547,6 +547,15 @@ int fail(aefdtd_grid *g, int code, const std::string &msg) {
 std::string g_create_err;
 std::mutex g_create_mu;
 
+}  // namespace
+
+void aefdtd::set_global_error(const std::string &msg) {
+    std::lock_guard<std::mutex> lk(g_create_mu);
+    g_create_err = msg;
+}
+
+namespace {
+
 #define HIPOK(g, call)                                                                                  \
     do {                                                                                                \
         hipError_t e_ = (call);                                                                         \
@@ -1412,6 +1421,15 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
     if (k < 0) return fail(g, AEFDTD_EINVAL, "unknown kernel name");
     if (total_ms) *total_ms = g->prof_ms[k];
     if (launches) *launches = g->prof_n[k];
+    return AEFDTD_OK;
+}
+
+int aefdtd_stats(const aefdtd_grid *g, long out[4]) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    out[0] = (long)g->ncell();
+    out[1] = g->n_disp;
+    out[2] = g->np_max;
+    out[3] = (long)g->tsize();
     return AEFDTD_OK;
 }
 

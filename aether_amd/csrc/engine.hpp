@@ -41,6 +41,8 @@ void pulse_init(aefdtd_pulse &p, double li, double lf, double inc, double pol, d
                 double sx, double sy, double sz, double n_ind);
 double pulse_eval(const aefdtd_pulse &p, int comp, double x, double y, double z, double t);
 double s_curve(double x, double xmin, double xmax);
+// Text returned by aefdtd_last_error(NULL) (engine.hip).
+void set_global_error(const std::string &msg);
 
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct PlaneSum;    // normal-incidence R/T plane sum (engine.hip)

@@ -3,6 +3,11 @@
 // FDTD_single_particle (modes/single_particle.cpp:29-415): same setup, same
 // per-step sequence, same post-processing, with the fields resident on the
 // device and every per-step field access done by device passes.
+//
+// A driver is split into a run object (create / step n / finish) so that a
+// caller can interleave many runs (the serving loop) or time the stepping
+// alone (the benchmark); aefdtd_run_normal_incidence/_single_particle are the
+// one-call forms.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -17,21 +22,49 @@ namespace {
 
 std::atomic<bool> g_cancel{false};
 
-struct Handle {
-    aefdtd_grid *g = nullptr;
-    ~Handle() { aefdtd_destroy(g); }
-};
-
 void lambda_grid(int Nl, double lmin, double lmax, std::vector<double> &L) {   // normal_incidence.cpp:235
     L.resize(Nl);
     for (int l = 0; l < Nl; l++) L[l] = lmin + (lmax - lmin) * l / (Nl - 1.0);
 }
 
+}  // namespace
+
+struct aefdtd_run {
+    int kind = 0;                     // 0 normal incidence, 1 single particle
+    int Nt = 0, t = 0;
+    int polarization = 0, Nl = 0;
+    double lambda_min = 0, lambda_max = 0, Dt = 0;
+    aefdtd_grid *g = nullptr, *a = nullptr;
+    std::vector<aefdtd_sensor_req> reqs;
+    std::vector<std::vector<int>> ids;
+    int idR = -1, idT = -1;
+    double index_sup = 1, index_sub = 1;
+    std::string err;
+    ~aefdtd_run() {
+        aefdtd_destroy(g);
+        aefdtd_destroy(a);
+    }
+};
+
+namespace {
+
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
+    } while (0)
+
+int rfail(aefdtd_run *r, int rc) {
+    if (r && r->g && !r->g->err.empty()) r->err = r->g->err;
+    else if (r && r->a && !r->a->err.empty()) r->err = r->a->err;
+    else if (r) r->err = aefdtd_last_error(nullptr);
+    return rc;
+}
+
 // Box_Spect_Poynting (sensors/s_spec_power.cpp:125-232) becomes six planar
 // sensors; a planar request one sensor.
-int add_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, std::vector<std::vector<int>> &ids) {
-    for (int r = 0; r < n; r++) {
-        const aefdtd_sensor_req &q = req[r];
+int add_requests(aefdtd_run *r) {
+    for (const aefdtd_sensor_req &q : r->reqs) {
         if (q.Nl < 1) return AEFDTD_EINVAL;
         std::vector<double> L;
         lambda_grid(q.Nl, q.lambda_min, q.lambda_max, L);
@@ -43,50 +76,44 @@ int add_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, std::vecto
                                  {q.x1, q.x2, q.y1, q.y2, q.z1, q.z1 + 1}, {q.x1, q.x2, q.y1, q.y2, q.z2 - 1, q.z2}};
             for (int f = 0; f < 6; f++) {
                 int id = -1;
-                if (!q.disable[f]) {
-                    int rc = aefdtd_add_spect_poynting(g, t[f], b[f][0], b[f][1], b[f][2], b[f][3], b[f][4], b[f][5], q.Nl, L.data(), &id);
-                    if (rc) return rc;
-                }
+                if (!q.disable[f])
+                    TRY(aefdtd_add_spect_poynting(r->g, t[f], b[f][0], b[f][1], b[f][2], b[f][3], b[f][4], b[f][5], q.Nl, L.data(), &id));
                 v.push_back(id);
             }
         } else {
             int id = -1;
-            int rc = aefdtd_add_spect_poynting(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.Nl, L.data(), &id);
-            if (rc) return rc;
+            TRY(aefdtd_add_spect_poynting(r->g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.Nl, L.data(), &id));
             v.push_back(id);
         }
-        ids.push_back(v);
+        r->ids.push_back(v);
     }
     return AEFDTD_OK;
 }
 
 // Spect_Poynting::treat per plane; Box_Spect_Poynting::treat sums the faces
 // in the order xm, xp, ym, yp, zm, zp (s_spec_power.cpp:199-207).
-int collect_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, const std::vector<std::vector<int>> &ids,
-                     double *out) {
+int collect_requests(aefdtd_run *r, double *out) {
     size_t off = 0;
-    for (int r = 0; r < n; r++) {
-        int Nl = req[r].Nl;
+    for (size_t q = 0; q < r->reqs.size(); q++) {
+        int Nl = r->reqs[q].Nl;
         std::vector<double> tmp(Nl);
-        if (req[r].box) {
+        if (r->reqs[q].box) {
             std::vector<double> tot(Nl, 0.0), faces(6 * (size_t)Nl, 0.0);
             for (int f = 0; f < 6; f++) {
-                if (ids[r][f] < 0) continue;
-                int rc = aefdtd_sensor_poynting(g, ids[r][f], tmp.data());
-                if (rc) return rc;
+                if (r->ids[q][f] < 0) continue;
+                TRY(aefdtd_sensor_poynting(r->g, r->ids[q][f], tmp.data()));
                 std::memcpy(&faces[f * (size_t)Nl], tmp.data(), Nl * sizeof(double));
             }
             for (int l = 0; l < Nl; l++)
                 for (int f = 0; f < 6; f++)
-                    if (ids[r][f] >= 0) tot[l] += faces[f * (size_t)Nl + l];
+                    if (r->ids[q][f] >= 0) tot[l] += faces[f * (size_t)Nl + l];
             if (out) {
                 std::memcpy(out + off, tot.data(), Nl * sizeof(double));
                 std::memcpy(out + off + Nl, faces.data(), 6 * (size_t)Nl * sizeof(double));
             }
             off += 7 * (size_t)Nl;
         } else {
-            int rc = aefdtd_sensor_poynting(g, ids[r][0], tmp.data());
-            if (rc) return rc;
+            TRY(aefdtd_sensor_poynting(r->g, r->ids[q][0], tmp.data()));
             if (out) std::memcpy(out + off, tmp.data(), Nl * sizeof(double));
             off += Nl;
         }
@@ -97,39 +124,25 @@ int collect_requests(aefdtd_grid *g, int n, const aefdtd_sensor_req *req, const 
 int copy_fields(aefdtd_grid *g, double *out) {
     if (!out) return AEFDTD_OK;
     size_t N = g->ncell();
-    for (int c = 0; c < 6; c++) {
-        int rc = aefdtd_read_field(g, c, out + c * N);
-        if (rc) return rc;
-    }
+    for (int c = 0; c < 6; c++) TRY(aefdtd_read_field(g, c, out + c * N));
     return AEFDTD_OK;
 }
 
-#define TRY(x)                   \
-    do {                         \
-        int rc_ = (x);           \
-        if (rc_) return rc_;     \
-    } while (0)
+uint32_t mg_at(const aefdtd_grid *g, int i, int j, int k) {
+    return g->matsgrid[(size_t)i + (size_t)g->Nx * ((size_t)j + (size_t)g->Ny * k)];
+}
 
-}  // namespace
-
-extern "C" {
-
-void aefdtd_cancel(void) { g_cancel = true; }
-
-// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607)
-int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                                aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
-    if (!m || !res || !m->matsgrid || m->Nt < 2 || m->Nl < 2) return AEFDTD_EINVAL;
-    g_cancel = false;
+// Setup of FDTD_normal_incidence up to the time loop (normal_incidence.cpp:161-324).
+int setup_normal(aefdtd_run *r, const aefdtd_mode *m) {
     const int Nt = m->Nt;
     const double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
     int pad_zm = std::max(5, m->pad[4]), pad_zp = std::max(5, m->pad[5]);
     double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * kC) * 0.99 * m->time_mod;
+    r->Dt = Dt;
     const int pml[6] = {0, 0, 0, 0, m->pml[4], m->pml[5]};
     const int pad[6] = {0, 0, 0, 0, pad_zm, pad_zp};
-    Handle H;
-    TRY(aefdtd_create(&H.g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_NORM, pml, pad, m->precision));
-    aefdtd_grid *g = H.g;
+    TRY(aefdtd_create(&r->g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_NORM, pml, pad, m->precision));
+    aefdtd_grid *g = r->g;
     TRY(aefdtd_set_pml(g, 4, m->kappa[4], m->sigma[4], m->alpha[4]));
     TRY(aefdtd_set_pml(g, 5, m->kappa[5], m->sigma[5], m->alpha[5]));
     TRY(aefdtd_set_tapering(g, m->tapering));
@@ -137,16 +150,12 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
     for (int i = 0; i < m->n_materials; i++) TRY(aefdtd_set_material(g, i, &m->materials[i]));
     TRY(aefdtd_disable_fields(g, m->disable_fields, m->n_disable));
     const int zs_s = g->zs_s, zs_e = g->zs_e;
-    const int Nl = m->Nl;
-    std::vector<double> Lambda, w(Nl);
-    lambda_grid(Nl, m->lambda_min, m->lambda_max, Lambda);
-    for (int l = 0; l < Nl; l++) w[l] = 2.0 * kPi * kC / Lambda[l];
     double pol = (m->polarization == 1) ? 90 : 0;
-    auto mg = [&](int i, int j, int k) { return g->matsgrid[(size_t)i + (size_t)g->Nx * ((size_t)j + (size_t)g->Ny * k)]; };
-    double eps_sub = g->mats[mg(0, 0, zs_s)].ei, eps_sup = g->mats[mg(0, 0, zs_e)].ei;
-    double index_sub = std::sqrt(eps_sub), index_sup = std::sqrt(eps_sup);
+    double eps_sub = g->mats[mg_at(g, 0, 0, zs_s)].ei, eps_sup = g->mats[mg_at(g, 0, 0, zs_e)].ei;
+    r->index_sub = std::sqrt(eps_sub);
+    r->index_sup = std::sqrt(eps_sup);
     aefdtd_pulse chp;
-    pulse_init(chp, m->lambda_min, m->lambda_max, 0, pol, 0, 0, 0, (zs_e - 1) * Dz, index_sup);
+    pulse_init(chp, m->lambda_min, m->lambda_max, 0, pol, 0, 0, 0, (zs_e - 1) * Dz, r->index_sup);
     TRY(aefdtd_bootstrap(g));
     // per-step injection values (normal_incidence.cpp:334-357) and tapering (:384-385)
     std::vector<double> dEx(Nt), dEy(Nt), dHx(Nt), dHy(Nt), tap(Nt);
@@ -164,33 +173,35 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
         tap[t] = (t >= Nt - m->tapering) ? s_curve(t, Nt - 1.0, Nt - m->tapering) : 1.0;
     }
     TRY(aefdtd_set_plane_injection(g, zs_e - 1, Nt, dEx.data(), dEy.data(), dHx.data(), dHy.data()));
-    std::vector<std::vector<int>> ids;
-    TRY(add_requests(g, n_sensors, sensors, ids));
-    int idR, idT;
-    TRY(aefdtd_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &idR));
-    TRY(aefdtd_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT));
-    int t;
-    for (t = 0; t < Nt; t++) {
-        TRY(aefdtd_update_E(g));
-        TRY(aefdtd_update_H(g));
-        TRY(aefdtd_feed_sensors(g));
-        if (g_cancel) break;     // end_computation poll (normal_incidence.cpp:431-435): Nt=t
-    }
-    const int Nrun = t;
+    TRY(add_requests(r));
+    TRY(aefdtd_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &r->idR));
+    TRY(aefdtd_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &r->idT));
+    return AEFDTD_OK;
+}
+
+// Post-processing of FDTD_normal_incidence (normal_incidence.cpp:442-570).
+int finish_normal(aefdtd_run *r, aefdtd_normal_result *res) {
+    aefdtd_grid *g = r->g;
+    const int Nt = r->Nt, Nrun = r->t, Nl = r->Nl;
+    const double Dz = g->Dz, Dt = r->Dt;
+    const int zs_s = g->zs_s, zs_e = g->zs_e;
+    std::vector<double> Lambda, w(Nl);
+    lambda_grid(Nl, r->lambda_min, r->lambda_max, Lambda);
+    for (int l = 0; l < Nl; l++) w[l] = 2.0 * kPi * kC / Lambda[l];
     std::vector<double> R(6 * (size_t)Nt), T(6 * (size_t)Nt);
-    TRY(aefdtd_read_plane_sum(g, idR, Nt, R.data()));
-    TRY(aefdtd_read_plane_sum(g, idT, Nt, T.data()));
+    TRY(aefdtd_read_plane_sum(g, r->idR, Nt, R.data()));
+    TRY(aefdtd_read_plane_sum(g, r->idT, Nt, T.data()));
     // find_slab (core/fdtd_utils.cpp:347-392)
     int Nsub = 0, Nsup = 0;
     long IDsup = 0, IDsub = 0, IDmoy;
     for (int i = g->xs_s; i < g->xs_e; i++)
-        for (int j = g->ys_s; j < g->ys_e; j++) { IDsup += mg(i, j, zs_e); IDsub += mg(i, j, zs_s); }
+        for (int j = g->ys_s; j < g->ys_e; j++) { IDsup += mg_at(g, i, j, zs_e); IDsub += mg_at(g, i, j, zs_s); }
     IDmoy = IDsub;
     int k = zs_s;
     while (IDmoy == IDsub && k < zs_e - 1) {
         IDmoy = 0;
         for (int i = g->xs_s; i < g->xs_e; i++)
-            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg(i, j, k + 1);
+            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg_at(g, i, j, k + 1);
         Nsub++; k++;
     }
     IDmoy = IDsup;
@@ -198,7 +209,7 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
     while (IDmoy == IDsup && k >= zs_s) {
         IDmoy = 0;
         for (int i = g->xs_s; i < g->xs_e; i++)
-            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg(i, j, k - 1);
+            for (int j = g->ys_s; j < g->ys_e; j++) IDmoy += mg_at(g, i, j, k - 1);
         Nsup++; k--;
     }
     double hsup = (Nsup - 1) * Dz, hsub = Nsub * Dz, hstruc = (g->Nz_s - Nsup - Nsub) * Dz;
@@ -207,7 +218,7 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
     // post-run DFT of the plane sums (normal_incidence.cpp:455-488)
     const std::complex<double> I(0.0, 1.0);
     for (int l = 0; l < Nl; l++) {
-        double k0 = -w[l] / kC, k1 = k0 * index_sup, k3 = k0 * index_sub;
+        double k0 = -w[l] / kC, k1 = k0 * r->index_sup, k3 = k0 * r->index_sub;
         std::complex<double> tmp_sup = std::exp(2.0 * k1 * hsup * I);
         std::complex<double> tmp_sub = std::exp((k3 * hsub - k0 * (hstruc + hsub)) * I);
         std::complex<double> r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0}, t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
@@ -229,40 +240,35 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
         b = std::sqrt(std::norm(t1[0]) + std::norm(t1[1]) + std::norm(t1[2]));
         if (res->T) res->T[l] = a / b;
         std::complex<double> ref0 = r1[1], trans0 = t1[1];
-        if (m->polarization == 1) { ref0 = r1[0]; trans0 = t1[0]; }
+        if (r->polarization == 1) { ref0 = r1[0]; trans0 = t1[0]; }
         std::complex<double> v[4] = {r2[0] / ref0, r2[1] / ref0, t2[0] / trans0, t2[1] / trans0};
         double *dst[4] = {res->refx, res->refy, res->transx, res->transy};
         for (int q = 0; q < 4; q++)
             if (dst[q]) { dst[q][2 * l] = v[q].real(); dst[q][2 * l + 1] = v[q].imag(); }
     }
-    res->index_sup = index_sup;
-    res->index_sub = index_sub;
+    res->index_sup = r->index_sup;
+    res->index_sub = r->index_sub;
     res->Nt_run = Nrun;
-    TRY(collect_requests(g, n_sensors, sensors, ids, sensor_out));
-    TRY(copy_fields(g, fields_out));
-    return g_cancel ? AEFDTD_ECANCELLED : AEFDTD_OK;
+    return AEFDTD_OK;
 }
 
-// FDTD_single_particle (modes/single_particle.cpp:29-415)
-int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                               double *sensor_out, double *fields_out) {
-    if (!m || !m->matsgrid || m->Nt < 1) return AEFDTD_EINVAL;
-    g_cancel = false;
+// Setup of FDTD_single_particle up to the time loop (single_particle.cpp:29-255).
+int setup_single(aefdtd_run *r, const aefdtd_mode *m) {
     const int Nt = m->Nt;
     const double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
     int pml_x = std::max(m->pml[0], m->pml[1]), pml_y = std::max(m->pml[2], m->pml[3]);
     bool periodic_x = pml_x <= 0, periodic_y = pml_y <= 0;
     double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * kC) * 0.99 * m->time_mod;
+    r->Dt = Dt;
     int pad[6];
     for (int f = 0; f < 6; f++) pad[f] = std::max(10, m->pad[f]);
     if (pml_x <= 0) pad[0] = pad[1] = 0;
     if (pml_y <= 0) pad[2] = pad[3] = 0;
-    Handle G, A;
-    TRY(aefdtd_create(&G.g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, pad, m->precision));
+    TRY(aefdtd_create(&r->g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, pad, m->precision));
     const int apml[6] = {0, 0, 0, 0, m->pml[4], m->pml[5]};
     const int apad[6] = {0, 0, 0, 0, pad[4], pad[5]};
-    TRY(aefdtd_create(&A.g, 1, 1, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, apml, apad, m->precision));
-    aefdtd_grid *g = G.g, *a = A.g;
+    TRY(aefdtd_create(&r->a, 1, 1, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, apml, apad, m->precision));
+    aefdtd_grid *g = r->g, *a = r->a;
     for (int f = 0; f < 6; f++) TRY(aefdtd_set_pml(g, f, m->kappa[f], m->sigma[f], m->alpha[f]));
     TRY(aefdtd_set_pml(a, 4, m->kappa[4], m->sigma[4], m->alpha[4]));
     TRY(aefdtd_set_pml(a, 5, m->kappa[5], m->sigma[5], m->alpha[5]));
@@ -279,7 +285,7 @@ int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd
     double pol = 0;
     if (m->polarization == 1) pol = 90;
     else if (m->polarization == 2) pol = m->polar_angle;
-    double eps_sup = g->mats[g->matsgrid[(size_t)g->Nx * g->Ny * zs_e]].ei;
+    double eps_sup = g->mats[mg_at(g, 0, 0, zs_e)].ei;
     aefdtd_pulse chp;
     pulse_init(chp, m->lambda_min, m->lambda_max, 0, pol, 0, 0, 0, (zs_e - 1 + 5) * Dz, 1.0);
     TRY(aefdtd_bootstrap(g));
@@ -300,19 +306,128 @@ int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd
     }
     TRY(aefdtd_set_plane_injection(a, zs_e - 1 + 5, Nt, dEx.data(), dEy.data(), dHx.data(), dHy.data()));
     TRY(aefdtd_set_tfsf(g, a, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e, periodic_x, periodic_y));
-    std::vector<std::vector<int>> ids;
-    TRY(add_requests(g, n_sensors, sensors, ids));
-    for (int t = 0; t < Nt; t++) {
-        TRY(aefdtd_update_E(a));
-        TRY(aefdtd_update_E(g));
-        TRY(aefdtd_update_H(a));
-        TRY(aefdtd_update_H(g));
-        TRY(aefdtd_feed_sensors(g));
-        if (g_cancel) break;
+    TRY(add_requests(r));
+    return AEFDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void aefdtd_cancel(void) { g_cancel = true; }
+
+int aefdtd_synchronize(aefdtd_grid *g) {
+    if (!g) return AEFDTD_EINVAL;
+    if (hipStreamSynchronize(g->stream) != hipSuccess) {
+        g->err = "hipStreamSynchronize failed";
+        return AEFDTD_EDEVICE;
     }
-    TRY(collect_requests(g, n_sensors, sensors, ids, sensor_out));
-    TRY(copy_fields(g, fields_out));
-    return g_cancel ? AEFDTD_ECANCELLED : AEFDTD_OK;
+    return AEFDTD_OK;
+}
+
+int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_sensors,
+                      const aefdtd_sensor_req *sensors) {
+    if (!out) return AEFDTD_EINVAL;
+    *out = nullptr;
+    if (!m || !m->matsgrid || (kind != 0 && kind != 1) || m->Nt < 1 || (n_sensors > 0 && !sensors)) return AEFDTD_EINVAL;
+    if (kind == 0 && (m->Nt < 2 || m->Nl < 2)) return AEFDTD_EINVAL;
+    auto *r = new aefdtd_run();
+    r->kind = kind;
+    r->Nt = m->Nt;
+    r->polarization = m->polarization;
+    r->Nl = m->Nl;
+    r->lambda_min = m->lambda_min;
+    r->lambda_max = m->lambda_max;
+    r->reqs.assign(sensors, sensors + std::max(0, n_sensors));
+    int rc = kind == 0 ? setup_normal(r, m) : setup_single(r, m);
+    if (rc) {
+        // keep the failure text reachable through aefdtd_last_error(NULL)
+        std::string e = r->g && !r->g->err.empty() ? r->g->err : (r->a && !r->a->err.empty() ? r->a->err : "");
+        if (!e.empty()) set_global_error(e);
+        delete r;
+        return rc;
+    }
+    *out = r;
+    return AEFDTD_OK;
+}
+
+int aefdtd_run_step(aefdtd_run *r, int n) {
+    if (!r) return AEFDTD_EINVAL;
+    for (int s = 0; s < n && r->t < r->Nt; s++) {
+        int rc;
+        if (r->kind == 1) {
+            // single_particle.cpp:256-381: aux E, main E (+TFSF-E), aux H, main H (+TFSF-H), sensors
+            if ((rc = aefdtd_update_E(r->a)) || (rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->a)) ||
+                (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
+                return rfail(r, rc);
+        } else {
+            // normal_incidence.cpp:325-415: E (+injection), H (+injection), sensors + R/T sums
+            if ((rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
+                return rfail(r, rc);
+        }
+        r->t++;
+        if (g_cancel) return AEFDTD_ECANCELLED;
+    }
+    return AEFDTD_OK;
+}
+
+int aefdtd_run_progress(const aefdtd_run *r, int *t, int *Nt) {
+    if (!r) return AEFDTD_EINVAL;
+    if (t) *t = r->t;
+    if (Nt) *Nt = r->Nt;
+    return AEFDTD_OK;
+}
+
+aefdtd_grid *aefdtd_run_grid(aefdtd_run *r) { return r ? r->g : nullptr; }
+
+const char *aefdtd_run_error(const aefdtd_run *r) { return r ? r->err.c_str() : ""; }
+
+int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
+    if (!r) return AEFDTD_EINVAL;
+    int rc;
+    if (r->kind == 0 && res && (rc = finish_normal(r, res))) return rfail(r, rc);
+    if ((rc = collect_requests(r, sensor_out))) return rfail(r, rc);
+    if ((rc = copy_fields(r->g, fields_out))) return rfail(r, rc);
+    return AEFDTD_OK;
+}
+
+int aefdtd_run_destroy(aefdtd_run *r) {
+    delete r;
+    return AEFDTD_OK;
+}
+
+// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) in one call.
+int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                                aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
+    if (!res) return AEFDTD_EINVAL;
+    g_cancel = false;
+    aefdtd_run *r = nullptr;
+    TRY(aefdtd_run_create(&r, 0, m, n_sensors, sensors));
+    int rc = AEFDTD_OK;
+    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    // a cancelled run keeps the steps done so far (Nt=t, normal_incidence.cpp:431-435)
+    bool cancelled = rc == AEFDTD_ECANCELLED;
+    if (cancelled) { r->t -= 1; rc = AEFDTD_OK; }
+    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, res, sensor_out, fields_out);
+    aefdtd_run_destroy(r);
+    if (rc) return rc;
+    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
+
+// FDTD_single_particle (modes/single_particle.cpp:29-415) in one call.
+int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                               double *sensor_out, double *fields_out) {
+    g_cancel = false;
+    aefdtd_run *r = nullptr;
+    TRY(aefdtd_run_create(&r, 1, m, n_sensors, sensors));
+    int rc = AEFDTD_OK;
+    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    bool cancelled = rc == AEFDTD_ECANCELLED;
+    if (cancelled) rc = AEFDTD_OK;
+    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);
+    aefdtd_run_destroy(r);
+    if (rc) return rc;
+    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,188 @@
+"""Benchmark of the FDTD hot path on one MI355X.
+
+Workload (the headline configuration): a 256^3-cell structure, 2 nm cells, 40
+non-overlapping gold-like spheres (Drude + 2 critical-point poles, radii
+8-20 nm, seed 2) in an eps=1.77 background, 12-cell CPML on all six faces and
+10-cell pads (the single-particle driver's extension) -> a 300^3 full grid,
+TFSF plane wave fed by the 1D auxiliary grid, box spectral-Poynting sensor on
+the structure boundary at 64 wavelengths.  FP32 field storage and arithmetic,
+FP64 recursive-convolution poles and DFT accumulators.  Synthetic structure,
+no files.
+
+A "step" is one full time step of the single-particle driver loop: auxiliary
+E, main E (stencil + TFSF + RC + CPML), auxiliary H, main H, sensor sample and
+(every 32 steps) the staged DFT.  value = full-grid cells x steps / seconds.
+
+Also reported: the roofline of the dominant kernel (E pass) from HIP events
+over the timed region, and the CPU baseline: the oracle (this project's C++
+FP64 restatement of Aether's CPU solver, threaded as Aether) timed on this
+host on a bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
+       [--no-cpu] [--cpu-steps S]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+
+
+def e_pass_bytes(lay, pml, stats):
+    """Algorithmic HBM bytes of one E-pass launch: read E,H (6 values) and the
+    uint8 material id, write E (3 values); CPML slab cells read+write 2 psi per
+    PML axis (E ranges: 1..pm-1 and N-pp+1..N-1); dispersive cells read+write
+    3 complex FP64 accumulators per pole and read a 4-byte slot index."""
+    ncell, ndisp, npmax, s = stats
+    Nx, Ny, Nz = lay[:3]
+    b = ncell * (9 * s + 1)
+    cross = (Ny * Nz, Nx * Nz, Nx * Ny)
+    for a in range(3):
+        pm, pp = pml[2 * a], pml[2 * a + 1]
+        if pm or pp:
+            planes = max(0, pm - 1) + max(0, pp - 1)
+            b += planes * cross[a] * 2 * 2 * s
+    b += ndisp * (npmax * 96 + 4)
+    return b
+
+
+def h_pass_bytes(lay, pml, stats):
+    """H pass: read H,E, write H (no material id); CPML H ranges 0..pm-1 and N-pp..N-2."""
+    ncell, _, _, s = stats
+    Nx, Ny, Nz = lay[:3]
+    b = ncell * 9 * s
+    cross = (Ny * Nz, Nx * Nz, Nx * Ny)
+    for a in range(3):
+        pm, pp = pml[2 * a], pml[2 * a + 1]
+        if pm or pp:
+            planes = pm + max(0, pp - 1)
+            b += planes * cross[a] * 2 * 2 * s
+    return b
+
+
+def cpu_baseline(spec, sensors, steps_lo, steps_hi, threads):
+    """Times the oracle's single-particle driver at two run lengths on the same
+    workload in FP64; the difference is the stepping cost (setup cancels)."""
+    import oracle
+    from aether_amd import capi
+    be = oracle.load()
+    be.fn("set_threads")(threads)
+    import copy
+    times = []
+    for n in (steps_lo, steps_hi):
+        s = copy.copy(spec)
+        s.Nt = n
+        s.precision = capi.FP64
+        t0 = time.perf_counter()
+        capi.run_single_particle(be, s, sensors)
+        times.append(time.perf_counter() - t0)
+    dt = (times[1] - times[0]) / (steps_hi - steps_lo)
+    return dt, times
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--n", type=int, default=256, help="structure cells per side")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a plain device copy of this many GiB (rocprof counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the engine runs one simulation per GPU with no collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibration > 0:
+        import torch
+        n = int(args.copy_calibration * (1 << 30)) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
+        return
+
+    from aether_amd import capi, workloads
+    prec = capi.FP32 if args.precision == "fp32" else capi.FP64
+    nsteps = args.warmup + args.steps
+    t_setup = time.perf_counter()
+    spec, sensors, spheres = workloads.spheres_single_particle(n=args.n, Nt=nsteps, precision=prec)
+    run = capi.Run(capi.engine(), 1, spec, sensors)
+    setup_s = time.perf_counter() - t_setup
+    lay = run.layout()
+    stats = run.stats()
+    ncell = stats[0]
+
+    run.step(args.warmup)
+    run.synchronize()
+    run.profile(True)
+    t0 = time.perf_counter()
+    run.step(args.steps)
+    run.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    e_ms, e_n = run.profile_read("E")
+    h_ms, h_n = run.profile_read("H")
+    s_ms, s_n = run.profile_read("sensor")
+    d_ms, d_n = run.profile_read("dft")
+    run.profile(False)
+    run.close()
+
+    value = ncell * args.steps / wall / 1e6
+    e_bytes = e_pass_bytes(lay, spec.pml, stats)
+    h_bytes = h_pass_bytes(lay, spec.pml, stats)
+    e_avg = e_ms / max(1, e_n) / 1e3
+    h_avg = h_ms / max(1, h_n) / 1e3
+    achieved = e_bytes / e_avg / 1e9
+    step_bytes = e_bytes + h_bytes
+    rec = {
+        "metric": "Mcell-updates/s (256^3 Drude+CP spheres, CPML, TFSF, 64-wavelength box DFT)",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == capi.FP32 else "f64",
+        "data": "synthetic (seeded random sphere packing, no files)",
+        "config": {"workload": f"single-particle {args.n}^3 structure -> {lay[0]}x{lay[1]}x{lay[2]} grid, "
+                               f"{len(spheres)} Drude+2CP spheres, 12-cell CPML x6, 64-lambda box DFT",
+                   "cells": ncell, "dispersive_cells": stats[1], "poles": stats[2],
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_update_E", "bytes_per_launch": e_bytes, "avg_launch_ms": round(e_avg * 1e3, 5)},
+        "kernels_ms_avg": {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5),
+                           "sensor": round(s_ms / max(1, s_n), 5), "dft": round(d_ms / max(1, d_n), 5)},
+        "step_hbm_fraction": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
+        "setup_s": round(setup_s, 2),
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        lo, hi = 2, 2 + args.cpu_steps
+        dt, times = cpu_baseline(spec, sensors, lo, hi, threads)
+        rec["cpu_baseline"] = {"value": round(ncell / dt / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle FP64 single-particle driver on the same {lay[0]}^3 workload, "
+                                         f"{hi}-step run minus {lo}-step run ({times[1]:.1f}s vs {times[0]:.1f}s)"}
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

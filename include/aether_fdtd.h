@@ -246,6 +246,24 @@ int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors,
  * normal_incidence.cpp:431-435). Safe to call from another host thread. */
 void aefdtd_cancel(void);
 
+/* The same drivers split into create / step / finish, so that a caller can
+ * interleave several runs (serving) or time the stepping alone. kind 0 =
+ * normal incidence, 1 = single particle. run_step advances up to n steps
+ * (never past Nt); run_finish performs the driver's post-processing (res is
+ * used by kind 0 only and may be NULL for kind 1). */
+typedef struct aefdtd_run aefdtd_run;
+int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m,
+                      int n_sensors, const aefdtd_sensor_req *sensors);
+int aefdtd_run_step(aefdtd_run *r, int n);
+int aefdtd_run_progress(const aefdtd_run *r, int *t, int *Nt);
+aefdtd_grid *aefdtd_run_grid(aefdtd_run *r);
+const char *aefdtd_run_error(const aefdtd_run *r);
+int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res,
+                      double *sensor_out, double *fields_out);
+int aefdtd_run_destroy(aefdtd_run *r);
+/* Waits for all work queued on the grid's stream. */
+int aefdtd_synchronize(aefdtd_grid *g);
+
 /* ---- introspection used by the benchmark -------------------------------- */
 /* Per-kernel device time accumulated on the handle's stream with HIP events
  * while profiling is enabled: names "E","H","sensor","dft".  Returns count. */
@@ -253,6 +271,9 @@ int aefdtd_profile_enable(aefdtd_grid *g, int on);
 int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches);
 /* Bytes of device memory held by the handle. */
 int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
+/* out[0]=cells of the full grid, out[1]=dispersive (RC) cells, out[2]=max
+ * poles per dispersive material, out[3]=bytes per stored field value. */
+int aefdtd_stats(const aefdtd_grid *g, long out[4]);
 
 #ifdef __cplusplus
 }

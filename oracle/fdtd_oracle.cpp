@@ -925,8 +925,8 @@ int set_err(aeoracle_grid *g, int code, const std::string &msg) {
 }
 
 // Sensor::feed + SensorFieldHolder::deep_feed/update_t_interp/FT_comp
-// (sensors.cpp:232-243,456-494,601-644); threads over lambda are a
-// partition of independent accumulators, so the serial loop is exact.
+// (sensors.cpp:232-243,456-494,601-644): the interpolated plane is gathered on
+// the calling thread, the running DFT is split over wavelengths.
 void sensor_feed(aeoracle_grid &g, SpectSensor &s) {
     for (int b = 0; b < s.span2; b++)
         for (int a = 0; a < s.span1; a++) {
@@ -936,16 +936,24 @@ void sensor_feed(aeoracle_grid &g, SpectSensor &s) {
             else { i = s.x1 + a; j = s.y1 + b; k = s.z1; }
             for (int c = 0; c < 6; c++) s.t[c][a + (size_t)s.span1 * b] = g.local(c, i, j, k);
         }
-    for (int l = 0; l < s.Nl; l++) {
-        double w = 2.0 * Pi * c_light / s.lambda[l];
-        cplx cE = s.tapE * std::exp(w * s.step * s.Dt * Im);
-        cplx cH = s.tapH * std::exp(w * (s.step + 0.5) * s.Dt * Im);
+    // FT_comp over a slice of wavelengths per worker (sensors.cpp:456-479,534-550)
+    auto ft = [&](int l1, int l2) {
         size_t P = (size_t)s.span1 * s.span2;
-        for (size_t p = 0; p < P; p++) {
-            for (int c = 0; c < 3; c++) s.sp[c][p + P * l] += cE * s.t[c][p];
-            for (int c = 3; c < 6; c++) s.sp[c][p + P * l] += cH * s.t[c][p];
+        for (int l = l1; l < l2; l++) {
+            double w = 2.0 * Pi * c_light / s.lambda[l];
+            cplx cE = s.tapE * std::exp(w * s.step * s.Dt * Im);
+            cplx cH = s.tapH * std::exp(w * (s.step + 0.5) * s.Dt * Im);
+            for (size_t p = 0; p < P; p++) {
+                for (int c = 0; c < 3; c++) s.sp[c][p + P * l] += cE * s.t[c][p];
+                for (int c = 3; c < 6; c++) s.sp[c][p + P * l] += cH * s.t[c][p];
+            }
         }
-    }
+    };
+    const int NT = g.pool->size();
+    g.pool->run([&](int ID) {
+        if (s.Nl > NT) ft((ID * s.Nl) / NT, ((ID + 1) * s.Nl) / NT);
+        else if (ID < s.Nl) ft(ID, ID + 1);
+    });
     if (s.step >= s.Nt - s.Ntap) {
         s.tapE = s_curve(s.step, s.Nt - 1.0, s.Nt - s.Ntap);
         s.tapH = s_curve(s.step + 0.5, s.Nt - 1.0, s.Nt - s.Ntap);
