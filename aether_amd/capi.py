@@ -115,6 +115,8 @@ def _sig(lib, p):
         "profile_read": (i, [vp, C.c_char_p, _dp, C.POINTER(C.c_long)]),
         "device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
         "stats": (i, [vp, C.POINTER(C.c_long)]),
+        "autotune": (i, [vp, i, _ip]),
+        "set_tiles": (i, [vp, i, i]),
         "cancel": (None, []),
         # oracle only
         "step": (i, [vp, i]),
@@ -401,6 +403,11 @@ class Run:
 
     def profile(self, on):
         self.be.fn("profile_enable")(self.grid(), int(on))
+
+    def autotune(self, reps=3):
+        out = (C.c_int * 2)()
+        self.be.check(self.be.fn("autotune")(self.grid(), reps, out), self.grid())
+        return list(out)
 
     def profile_read(self, kernel):
         ms, n = C.c_double(), C.c_long()

@@ -1,377 +1,16 @@
-// engine.hip -- gfx950 kernels and the grid-level C ABI of the FDTD engine.
-//
-// One fused pass per half step keeps the leapfrog at its HBM floor:
-//   E pass : read E(3) H(3) mat -> RC-ante (FP64 poles) -> E stencil -> TFSF-E
-//            -> RC-apply -> CPML-E (slab psi) -> PEC -> plane injection -> write E(3)
-//   H pass : read H(3) E(3)     -> H stencil -> CPML-H -> PEC -> plane injection
-//            -> TFSF-H -> write H(3)
-// This is the phase order of Aether's threaded_process_E/H (core/fdtd_threads.
-// cpp:32-235) and of its drivers (modes/normal_incidence.cpp:330-357,
-// modes/single_particle.cpp:256-378); every phase is cell-local, so fusing them
-// keeps each cell's sequence of floating-point updates.
-//
-// Thread mapping: a workgroup is 64 x 4 lanes over (x, y); x is the contiguous
-// axis, so one wavefront reads 64 consecutive values of each field. Each lane
-// marches KZ planes in z and keeps the previous plane's H (E pass) / next
-// plane's E (H pass) in registers, so the z-neighbour is never re-read.
-#include "engine.hpp"
+// engine.hip -- grid-level C ABI of the FDTD engine: device state, uploads,
+// launches, sensors and the tile autotuner.  The kernels are in kernels.hpp.
+#include "kernels.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 
 using namespace aefdtd;
+using namespace aefdtd::dev;
 
-namespace {
-
-constexpr int BX = 64, BY = 4, KZ = 8;
-constexpr int KS = 32;                  // DFT staging depth (steps per accumulate)
-
-template <typename T>
-struct MatDev {
-    T C1, C2x, C2y, C2z, C4;
-    double C3;
-    int rc, np, pole0, pad_;
-};
-
-// 1D profile arrays per axis a (length N_a): ikE, ikH (inverse kappa with the
-// stencil's gating), bE, cE (c * 1/D), bH, cH.
-template <typename T>
-struct Prof {
-    const T *ikE[3], *ikH[3], *bE[3], *cE[3], *bH[3], *cH[3];
-};
-
-template <typename T>
-struct EArgs {
-    int Nx, Ny, Nz;
-    T *Ex, *Ey, *Ez;
-    const T *Hx, *Hy, *Hz;
-    const uint8_t *mat;
-    const MatDev<T> *mt;
-    const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
-    const int32_t *slot;
-    double2 *psi;
-    long ndisp;
-    Prof<T> pf;
-    int pml[6];
-    T *pEyx, *pEzx, *pExy, *pEzy, *pExz, *pEyz;   // slab psi, x:(pxm+pxp,Ny,Nz) y:(Nx,pym+pyp,Nz) z:(Nx,Ny,pzm+pzp)
-    int en;                    // bit c set: component c enabled
-    int inj_k;
-    T injx, injy;
-    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
-    const T *aHx, *aHy, *aHz;  // aux 1D grid H columns
-};
-
-template <typename T>
-struct HArgs {
-    int Nx, Ny, Nz;
-    T *Hx, *Hy, *Hz;
-    const T *Ex, *Ey, *Ez;
-    Prof<T> pf;
-    int pml[6];
-    T *pHyx, *pHzx, *pHxy, *pHzy, *pHxz, *pHyz;
-    int en;
-    T dtm, dtdmx, dtdmy, dtdmz;
-    int inj_k;
-    T injx, injy;
-    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
-    const T *aEx, *aEy, *aEz;
-};
-
-// ---------------------------------------------------------------- E pass
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_E(const EArgs<T> a) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.Nx || j >= a.Ny) return;
-    const int k0 = blockIdx.z * KZ;
-    const int k1 = min(k0 + KZ, a.Nz);
-    const size_t NxNy = (size_t)a.Nx * a.Ny;
-    const size_t base = (size_t)i + (size_t)a.Nx * j;
-    const long oim = (i == 0) ? (long)(a.Nx - 1) : -1L;                    // (i-1) with wrap
-    const long ojm = (j == 0) ? (long)(a.Ny - 1) * a.Nx : -(long)a.Nx;     // (j-1) with wrap
-    const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
-    T hx_km = a.Hx[base + km0 * NxNy];
-    T hy_km = a.Hy[base + km0 * NxNy];
-    const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
-    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
-    // CPML slab index along x / y (E loop bounds: near 1..pm-1, far N-pp+1..N-1)
-    int nx_ = -1, ny_ = -1;
-    if (px) {
-        if (i >= 1 && i < a.pml[0]) nx_ = i;
-        else if (i >= a.Nx - a.pml[1] + 1) nx_ = i - (a.Nx - a.pml[1]) + a.pml[0];
-    }
-    if (py) {
-        if (j >= 1 && j < a.pml[2]) ny_ = j;
-        else if (j >= a.Ny - a.pml[3] + 1) ny_ = j - (a.Ny - a.pml[3]) + a.pml[2];
-    }
-    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
-    for (int k = k0; k < k1; k++) {
-        const size_t id = base + (size_t)k * NxNy;
-        const T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
-        const T dHz_y = hz - a.Hz[id + ojm];
-        const T dHz_x = hz - a.Hz[id + oim];
-        const T dHy_x = hy - a.Hy[id + oim];
-        const T dHx_y = hx - a.Hx[id + ojm];
-        const T dHy_z = hy - hy_km;
-        const T dHx_z = hx - hx_km;
-        hx_km = hx;
-        hy_km = hy;
-        const T ex0 = a.Ex[id], ey0 = a.Ey[id], ez0 = a.Ez[id];
-        const MatDev<T> M = a.mt[a.mat[id]];
-        // RC-ante: psi = Crec psi + Dchi E^n (core/mats_RC.cpp:21-36), FP64
-        double sx = 0, sy = 0, sz = 0;
-        if (M.rc) {
-            const long s = a.slot[id];
-            for (int p = 0; p < M.np; p++) {
-                const double2 cr = a.poles[2 * (M.pole0 + p)], dc = a.poles[2 * (M.pole0 + p) + 1];
-                double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
-                const double e3[3] = {(double)ex0, (double)ey0, (double)ez0};
-                double acc[3];
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    double2 v = q[(size_t)c * a.ndisp];
-                    double2 r;
-                    r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
-                    r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
-                    q[(size_t)c * a.ndisp] = r;
-                    acc[c] = r.x;
-                }
-                sx += acc[0]; sy += acc[1]; sz += acc[2];
-            }
-        }
-        const T ikz = a.pf.ikE[2][k];
-        // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
-        T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
-        T ey = (a.en & 2) ? M.C1 * ey0 + M.C2z * ikz * dHx_z - M.C2x * ikx * dHz_x : ey0;
-        T ez = (a.en & 4) ? M.C1 * ez0 + M.C2x * ikx * dHy_x - M.C2y * iky * dHx_y : ez0;
-        // TFSF-E (modes/single_particle.cpp:286-332): z faces, then y, then x
-        if (a.tfsf) {
-            const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
-            if (in_i && in_j) {
-                if (k == a.tb4) { ex += M.C2z * a.aHy[a.tb4 - 1]; ey -= M.C2z * a.aHx[a.tb4 - 1]; }
-                if (k == a.tb5) { ex -= M.C2z * a.aHy[a.tb5 - 1]; ey += M.C2z * a.aHx[a.tb5 - 1]; }
-            }
-            if (!a.tper_y && in_i && in_k) {
-                if (j == a.tb2) { ex -= M.C2y * a.aHz[k]; ez += M.C2y * a.aHx[k]; }
-                if (j == a.tb3) { ex += M.C2y * a.aHz[k]; ez -= M.C2y * a.aHx[k]; }
-            }
-            if (!a.tper_x && in_k && in_j) {
-                if (i == a.tb0) { ey += M.C2x * a.aHz[k]; ez -= M.C2x * a.aHy[k]; }
-                if (i == a.tb1) { ey -= M.C2x * a.aHz[k]; ez += M.C2x * a.aHy[k]; }
-            }
-        }
-        // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50)
-        if (M.rc) {
-            ex += (T)(M.C3 * sx);
-            ey += (T)(M.C3 * sy);
-            ez += (T)(M.C3 * sz);
-        }
-        // CPML-E + PEC walls (core/fdtd_pml.cpp:504-735)
-        int nz_ = -1;
-        if (pz) {
-            if (k >= 1 && k < a.pml[4]) nz_ = k;
-            else if (k >= a.Nz - a.pml[5] + 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
-        }
-        if (a.en & 1) {
-            if (py) {
-                if (ny_ >= 0) {
-                    T &q = a.pExy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
-                    q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHz_y;
-                    ex += M.C4 * q;
-                }
-                if (j == 0) ex = 0;
-            }
-            if (pz) {
-                if (nz_ >= 0) {
-                    T &q = a.pExz[base + NxNy * nz_];
-                    q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHy_z;
-                    ex -= M.C4 * q;
-                }
-                if (k == 0) ex = 0;
-            }
-        }
-        if (a.en & 2) {
-            if (px) {
-                if (nx_ >= 0) {
-                    T &q = a.pEyx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
-                    q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHz_x;
-                    ey -= M.C4 * q;
-                }
-                if (i == 0) ey = 0;
-            }
-            if (pz) {
-                if (nz_ >= 0) {
-                    T &q = a.pEyz[base + NxNy * nz_];
-                    q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHx_z;
-                    ey += M.C4 * q;
-                }
-                if (k == 0) ey = 0;
-            }
-        }
-        if (a.en & 4) {
-            if (px) {
-                if (nx_ >= 0) {
-                    T &q = a.pEzx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
-                    q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHy_x;
-                    ez += M.C4 * q;
-                }
-                if (i == 0) ez = 0;
-            }
-            if (py) {
-                if (ny_ >= 0) {
-                    T &q = a.pEzy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
-                    q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHx_y;
-                    ez -= M.C4 * q;
-                }
-                if (j == 0) ez = 0;
-            }
-        }
-        // plane injection after the whole E update (modes/normal_incidence.cpp:338-342)
-        if (k == a.inj_k) { ex += a.injx; ey += a.injy; }
-        a.Ex[id] = ex;
-        a.Ey[id] = ey;
-        a.Ez[id] = ez;
-    }
-}
-
-// ---------------------------------------------------------------- H pass
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_H(const HArgs<T> a) {
-    const int i = blockIdx.x * BX + threadIdx.x;
-    const int j = blockIdx.y * BY + threadIdx.y;
-    if (i >= a.Nx || j >= a.Ny) return;
-    const int k0 = blockIdx.z * KZ;
-    const int k1 = min(k0 + KZ, a.Nz);
-    const size_t NxNy = (size_t)a.Nx * a.Ny;
-    const size_t base = (size_t)i + (size_t)a.Nx * j;
-    const long oip = (i == a.Nx - 1) ? -(long)(a.Nx - 1) : 1L;
-    const long ojp = (j == a.Ny - 1) ? -(long)(a.Ny - 1) * a.Nx : (long)a.Nx;
-    T ex_k = a.Ex[base + (size_t)k0 * NxNy];
-    T ey_k = a.Ey[base + (size_t)k0 * NxNy];
-    const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
-    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
-    // CPML slab index (H loop bounds: near 0..pm-1, far N-pp..N-2)
-    int nx_ = -1, ny_ = -1;
-    if (px) {
-        if (i < a.pml[0]) nx_ = i;
-        else if (i >= a.Nx - a.pml[1] && i < a.Nx - 1) nx_ = i - (a.Nx - a.pml[1]) + a.pml[0];
-    }
-    if (py) {
-        if (j < a.pml[2]) ny_ = j;
-        else if (j >= a.Ny - a.pml[3] && j < a.Ny - 1) ny_ = j - (a.Ny - a.pml[3]) + a.pml[2];
-    }
-    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
-    for (int k = k0; k < k1; k++) {
-        const size_t id = base + (size_t)k * NxNy;
-        const int kp = (k == a.Nz - 1) ? 0 : k + 1;
-        const T ex = ex_k, ey = ey_k;
-        const T ex_kp = a.Ex[base + (size_t)kp * NxNy];
-        const T ey_kp = a.Ey[base + (size_t)kp * NxNy];
-        ex_k = ex_kp;
-        ey_k = ey_kp;
-        const T ez = a.Ez[id];
-        const T dEy_z = ey_kp - ey;
-        const T dEx_z = ex_kp - ex;
-        const T dEz_y = a.Ez[id + ojp] - ez;
-        const T dEz_x = a.Ez[id + oip] - ez;
-        const T dEx_y = a.Ex[id + ojp] - ex;
-        const T dEy_x = a.Ey[id + oip] - ey;
-        const T ikz = a.pf.ikH[2][k];
-        T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
-        // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
-        if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
-        if (a.en & 16) hy += a.dtdmx * ikx * dEz_x - a.dtdmz * ikz * dEx_z;
-        if (a.en & 32) hz += a.dtdmy * iky * dEx_y - a.dtdmx * ikx * dEy_x;
-        // CPML-H + PEC walls (core/fdtd_pml.cpp:737-929)
-        int nz_ = -1;
-        if (pz) {
-            if (k < a.pml[4]) nz_ = k;
-            else if (k >= a.Nz - a.pml[5] && k < a.Nz - 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
-        }
-        if (a.en & 8) {
-            if (py) {
-                if (ny_ >= 0) {
-                    T &q = a.pHxy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
-                    q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEz_y;
-                    hx -= a.dtm * q;
-                }
-                if (j == a.Ny - 1) hx = 0;
-            }
-            if (pz) {
-                if (nz_ >= 0) {
-                    T &q = a.pHxz[base + NxNy * nz_];
-                    q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEy_z;
-                    hx += a.dtm * q;
-                }
-                if (k == a.Nz - 1) hx = 0;
-            }
-        }
-        if (a.en & 16) {
-            if (px) {
-                if (nx_ >= 0) {
-                    T &q = a.pHyx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
-                    q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEz_x;
-                    hy += a.dtm * q;
-                }
-                if (i == a.Nx - 1) hy = 0;
-            }
-            if (pz) {
-                if (nz_ >= 0) {
-                    T &q = a.pHyz[base + NxNy * nz_];
-                    q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEx_z;
-                    hy -= a.dtm * q;
-                }
-            }
-        }
-        // top z wall of Hy: zeroed by the Hx pass and by the Hy pass (fdtd_pml.cpp:799,862-865)
-        if (pz && k == a.Nz - 1 && (a.en & (8 | 16))) hy = 0;
-        if (a.en & 32) {
-            if (px) {
-                if (nx_ >= 0) {
-                    T &q = a.pHzx[(size_t)nx_ + (size_t)wx * (j + (size_t)a.Ny * k)];
-                    q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEy_x;
-                    hz -= a.dtm * q;
-                }
-                if (i == a.Nx - 1) hz = 0;
-            }
-            if (py) {
-                if (ny_ >= 0) {
-                    T &q = a.pHzy[(size_t)i + (size_t)a.Nx * (ny_ + (size_t)wy * k)];
-                    q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEx_y;
-                    hz += a.dtm * q;
-                }
-                if (j == a.Ny - 1) hz = 0;
-            }
-        }
-        // plane injection (modes/normal_incidence.cpp:353-357)
-        if (k == a.inj_k) { hx += a.injx; hy += a.injy; }
-        // TFSF-H (modes/single_particle.cpp:351-378): z faces, then y, then x
-        if (a.tfsf) {
-            const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
-            if (in_i && in_j) {
-                if (k == a.tb4 - 1) { hx -= a.dtdmz * a.aEy[a.tb4]; hy += a.dtdmz * a.aEx[a.tb4]; }
-                if (k == a.tb5 - 1) { hx += a.dtdmz * a.aEy[a.tb5]; hy -= a.dtdmz * a.aEx[a.tb5]; }
-            }
-            if (!a.tper_y && in_i && in_k) {
-                if (j == a.tb2 - 1) { hx += a.dtdmy * a.aEz[k]; hz -= a.dtdmy * a.aEx[k]; }
-                if (j == a.tb3 - 1) { hx -= a.dtdmy * a.aEz[k]; hz += a.dtdmy * a.aEx[k]; }
-            }
-            if (!a.tper_x && in_k && in_j) {
-                if (i == a.tb0 - 1) { hy -= a.dtdmx * a.aEz[k]; hz += a.dtdmx * a.aEy[k]; }
-                if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
-            }
-        }
-        a.Hx[id] = hx;
-        a.Hy[id] = hy;
-        a.Hz[id] = hz;
-    }
-}
-
-}  // namespace
-
-// ================================================================ sensors
 namespace aefdtd {
 
 struct Sensor {                   // Spect_Poynting with interpolated samples
@@ -396,147 +35,6 @@ struct PlaneSum {
 }  // namespace aefdtd
 
 namespace {
-
-struct SensDesc {
-    int type, x1, y1, z1, span1, span2;
-    long P, off;
-};
-
-// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) on one sensor plane.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, const T *Ex, const T *Ey,
-                                                const T *Ez, const T *Hx, const T *Hy, const T *Hz, int Nx, int Ny,
-                                                int Nz, T *stage) {
-    const SensDesc d = sd[blockIdx.y];
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.P) return;
-    const int a = (int)(p % d.span1), b = (int)(p / d.span1);
-    int i, j, k;
-    if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
-    else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) { i = d.x1 + a; j = d.y1; k = d.z1 + b; }
-    else { i = d.x1 + a; j = d.y1 + b; k = d.z1; }
-    const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
-    auto at = [&](const T *f, int x, int y, int z) { return f[(size_t)x + (size_t)Nx * ((size_t)y + (size_t)Ny * z)]; };
-    T v[6];
-    v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
-    v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
-    v[2] = (at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0;
-    v[3] = (at(Hx, i, j, k) + at(Hx, ip, j, k)) / (T)2.0;
-    v[4] = (at(Hy, i, j, k) + at(Hy, i, jp, k)) / (T)2.0;
-    v[5] = (at(Hz, i, j, k) + at(Hz, i, j, kp)) / (T)2.0;
-#pragma unroll
-    for (int c = 0; c < 6; c++) stage[((size_t)slot * 6 + c) * Ptot + d.off + p] = v[c];
-}
-
-// Running DFT over a block of nfill staged steps (sensors/sensors.cpp:456-479):
-// acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p], FP64 accumulators.
-template <typename T>
-__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
-                                             const double2 *coef, int s0, int nfill, double2 *acc) {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (p >= P) return;
-    double x[KS];
-#pragma unroll
-    for (int s = 0; s < KS; s++) x[s] = (s < nfill) ? (double)stage[((size_t)s * 6 + c) * Ptot + off + p] : 0.0;
-    const int h = c >= 3 ? 1 : 0;
-    for (int l = 0; l < Nl; l++) {
-        double re = 0, im = 0;
-        for (int s = 0; s < nfill; s++) {
-            const double2 w = coef[((size_t)(s0 + s) * 2 + h) * Nl + l];
-            re += w.x * x[s];
-            im += w.y * x[s];
-        }
-        double2 &A = acc[((size_t)c * Nl + l) * P + p];
-        A.x += re;
-        A.y += im;
-    }
-}
-
-// Spect_Poynting::treat (sensors/s_spec_power.cpp:61-111): one workgroup per
-// wavelength, fixed-order tree reduction so the result is deterministic.
-__global__ __launch_bounds__(256) void k_poynting(const double2 *acc, long P, int Nl, double f, int type, double *rsp) {
-    const int l = blockIdx.x;
-    __shared__ double sh[3][256];
-    double S[3] = {0, 0, 0};
-    for (long p = threadIdx.x; p < P; p += blockDim.x) {
-        double2 e[3], h[3];
-        for (int c = 0; c < 3; c++) {
-            e[c] = acc[((size_t)c * Nl + l) * P + p];
-            h[c] = acc[((size_t)(3 + c) * Nl + l) * P + p];
-        }
-        // Re(a conj(b)) = a.x b.x + a.y b.y
-        S[0] += (e[1].x * h[2].x + e[1].y * h[2].y) - (e[2].x * h[1].x + e[2].y * h[1].y);
-        S[1] += (e[2].x * h[0].x + e[2].y * h[0].y) - (e[0].x * h[2].x + e[0].y * h[2].y);
-        S[2] += (e[0].x * h[1].x + e[0].y * h[1].y) - (e[1].x * h[0].x + e[1].y * h[0].y);
-    }
-    for (int c = 0; c < 3; c++) sh[c][threadIdx.x] = S[c];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int c = 0; c < 3; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double Sx = sh[0][0] * f, Sy = sh[1][0] * f, Sz = sh[2][0] * f;
-        double r;
-        switch (type) {
-            case AEFDTD_NORMAL_X: r = Sx; break;
-            case AEFDTD_NORMAL_Y: r = Sy; break;
-            case AEFDTD_NORMAL_Z: r = Sz; break;
-            case AEFDTD_NORMAL_XM: r = -Sx; break;
-            case AEFDTD_NORMAL_YM: r = -Sy; break;
-            default: r = -Sz; break;
-        }
-        rsp[l] = r;
-    }
-}
-
-// Analytic pulse on the device (core/chpin.cpp:44-90).
-__device__ double d_pulse(const aefdtd_pulse &p, double amp, double x, double y, double z, double t) {
-    const double ph = p.kx * x + p.ky * y + p.kz * z;
-    const double a = ph - p.omega * t;
-    const double b = (1.0 / p.k * ph - kC * t / p.n_ind + p.C1) / p.dz;
-    return amp * exp(-b * b) * cos(a);
-}
-
-// R/T plane sums of modes/normal_incidence.cpp:384-415 for step t; one
-// workgroup, fixed-order reduction.
-template <typename T>
-__global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, const T *Ez, int Nx, int Ny, int kpl,
-                                                   aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
-                                                   double Dz, double zpos, const double *tap, int t, double Dt,
-                                                   double *out) {
-    __shared__ double sh[6][256];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    const double tb = (t + 0.5) * Dt;
-    const double tp = tap[t];
-    const size_t NxNy = (size_t)Nx * Ny;
-    for (long q = threadIdx.x; q < (long)NxNy; q += blockDim.x) {
-        const int i = (int)(q % Nx), j = (int)(q / Nx);
-        const double x = i * Dx, y = j * Dy;
-        double in[3] = {0, 0, 0};
-        if (has_inc) {
-            in[0] = d_pulse(inc, inc.Ex0, x + Dx / 2.0, y, zpos, tb);
-            in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
-            in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
-        }
-        const size_t id = (size_t)q + NxNy * kpl;
-        const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
-        for (int c = 0; c < 3; c++) {
-            acc[3 + c] += in[c] * tp;
-            acc[c] += (subtract ? (f[c] - in[c]) : f[c]) * tp;
-        }
-    }
-    for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int c = 0; c < 6; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) out[(size_t)t * 6 + threadIdx.x] = sh[threadIdx.x][0];
-}
 
 // --------------------------------------------------------------- helpers
 int fail(aefdtd_grid *g, int code, const std::string &msg) {
@@ -705,7 +203,6 @@ int upload_materials_T(aefdtd_grid *g) {
         d.rc = M.rc ? 1 : 0;
         d.np = (int)M.Crec.size();
         d.pole0 = (int)(poles.size() / 2);
-        d.pad_ = 0;
         for (int p = 0; p < d.np; p++) {
             poles.push_back(make_double2(M.Crec[p].real(), M.Crec[p].imag()));
             poles.push_back(make_double2(M.Dchi[p].real(), M.Dchi[p].imag()));
@@ -810,15 +307,62 @@ void prof_collect(aefdtd_grid *g) {
     g->events.clear();
 }
 
-dim3 field_grid(const aefdtd_grid *g) {
-    return dim3((g->Nx + BX - 1) / BX, (g->Ny + BY - 1) / BY, (g->Nz + KZ - 1) / KZ);
+// ---------------------------------------------------------------- launches
+// Tile shapes compiled for the 3D passes: (BY, KZ) with BY in {1,2,4,8} and
+// KZ in {4,8,16,32}; tile_cfg packs (BY index, KZ index, remap).
+constexpr int kBY[4] = {1, 2, 4, 8};
+constexpr int kKZ[4] = {4, 8, 16, 32};
+
+template <typename T, int BY, int KZ>
+void launch_E_cfg(const EArgs<T> &a, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_update_E<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+}
+template <typename T, int BY, int KZ>
+void launch_H_cfg(const HArgs<T> &a, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_update_H<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+}
+
+template <typename T, int BY>
+void launch_E_by(const EArgs<T> &a, int kzi, int nb, hipStream_t st) {
+    switch (kzi) {
+        case 0: launch_E_cfg<T, BY, 4>(a, nb, st); break;
+        case 1: launch_E_cfg<T, BY, 8>(a, nb, st); break;
+        case 2: launch_E_cfg<T, BY, 16>(a, nb, st); break;
+        default: launch_E_cfg<T, BY, 32>(a, nb, st); break;
+    }
+}
+template <typename T, int BY>
+void launch_H_by(const HArgs<T> &a, int kzi, int nb, hipStream_t st) {
+    switch (kzi) {
+        case 0: launch_H_cfg<T, BY, 4>(a, nb, st); break;
+        case 1: launch_H_cfg<T, BY, 8>(a, nb, st); break;
+        case 2: launch_H_cfg<T, BY, 16>(a, nb, st); break;
+        default: launch_H_cfg<T, BY, 32>(a, nb, st); break;
+    }
+}
+
+// Fills the tile geometry of an argument block for configuration cfg and
+// returns the number of workgroups.
+template <typename A>
+int tile_geometry(A &a, int cfg) {
+    const int by = kBY[(cfg >> 0) & 3], kz = kKZ[(cfg >> 2) & 3];
+    a.nbx = (a.Nx + BX - 1) / BX;
+    a.nby = (a.Ny + by - 1) / by;
+    a.nbz = (a.Nz + kz - 1) / kz;
+    a.remap = (cfg >> 4) & 1;
+    return a.nbx * a.nby * a.nbz;
+}
+
+bool is_column(const aefdtd_grid *g) {
+    return g->Nx == 1 && g->Ny == 1 && !(g->pml[0] || g->pml[1] || g->pml[2] || g->pml[3]);
 }
 
 template <typename T>
-int launch_E(aefdtd_grid *g) {
+EArgs<T> e_args(aefdtd_grid *g) {
     EArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+    a.nmat = (int)g->Nmat;
     a.Ex = (T *)g->F[0]; a.Ey = (T *)g->F[1]; a.Ez = (T *)g->F[2];
     a.Hx = (const T *)g->F[3]; a.Hy = (const T *)g->F[4]; a.Hz = (const T *)g->F[5];
     a.mat = g->d_mat;
@@ -826,7 +370,7 @@ int launch_E(aefdtd_grid *g) {
     a.poles = g->d_poles;
     a.slot = g->d_slot;
     a.psi = g->d_psi;
-    a.ndisp = g->n_disp;
+    a.ndisp = (int)g->n_disp;
     a.pf = prof_of<T>(g);
     for (int f = 0; f < 6; f++) a.pml[f] = g->pml[f];
     a.pEyx = (T *)g->d_psipml[0]; a.pEzx = (T *)g->d_psipml[1];
@@ -845,16 +389,11 @@ int launch_E(aefdtd_grid *g) {
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
         a.aHx = (const T *)g->tfsf_aux->F[3]; a.aHy = (const T *)g->tfsf_aux->F[4]; a.aHz = (const T *)g->tfsf_aux->F[5];
     }
-    hipEvent_t ev;
-    prof_begin(g, 0, &ev);
-    hipLaunchKernelGGL(k_update_E<T>, field_grid(g), dim3(BX, BY, 1), 0, g->stream, a);
-    prof_end(g, ev);
-    HIPOK(g, hipGetLastError());
-    return AEFDTD_OK;
+    return a;
 }
 
 template <typename T>
-int launch_H(aefdtd_grid *g) {
+HArgs<T> h_args(aefdtd_grid *g) {
     HArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
@@ -879,11 +418,126 @@ int launch_H(aefdtd_grid *g) {
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
         a.aEx = (const T *)g->tfsf_aux->F[0]; a.aEy = (const T *)g->tfsf_aux->F[1]; a.aEz = (const T *)g->tfsf_aux->F[2];
     }
+    return a;
+}
+
+template <typename T>
+void dispatch_E(aefdtd_grid *g, EArgs<T> &a, int cfg) {
+    if (is_column(g)) {
+        hipLaunchKernelGGL(k_update_E_col<T>, dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
+        return;
+    }
+    const int nb = tile_geometry(a, cfg);
+    switch (cfg & 3) {
+        case 0: launch_E_by<T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 1: launch_E_by<T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 2: launch_E_by<T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        default: launch_E_by<T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+    }
+}
+
+template <typename T>
+void dispatch_H(aefdtd_grid *g, HArgs<T> &a, int cfg) {
+    if (is_column(g)) {
+        hipLaunchKernelGGL(k_update_H_col<T>, dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
+        return;
+    }
+    const int nb = tile_geometry(a, cfg);
+    switch (cfg & 3) {
+        case 0: launch_H_by<T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 1: launch_H_by<T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 2: launch_H_by<T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        default: launch_H_by<T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+    }
+}
+
+template <typename T>
+int launch_E(aefdtd_grid *g) {
+    EArgs<T> a = e_args<T>(g);
     hipEvent_t ev;
-    prof_begin(g, 1, &ev);
-    hipLaunchKernelGGL(k_update_H<T>, field_grid(g), dim3(BX, BY, 1), 0, g->stream, a);
+    prof_begin(g, 0, &ev);
+    dispatch_E<T>(g, a, g->tile_e);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+template <typename T>
+int launch_H(aefdtd_grid *g) {
+    HArgs<T> a = h_args<T>(g);
+    hipEvent_t ev;
+    prof_begin(g, 1, &ev);
+    dispatch_H<T>(g, a, g->tile_h);
+    prof_end(g, ev);
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+// Times one tile configuration of the E or H pass: reps launches between events.
+template <typename T>
+float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hipEvent_t eb) {
+    (void)hipEventRecord(ea, g->stream);
+    for (int r = 0; r < reps; r++) {
+        if (e_pass) {
+            EArgs<T> a = e_args<T>(g);
+            dispatch_E<T>(g, a, cfg);
+        } else {
+            HArgs<T> a = h_args<T>(g);
+            dispatch_H<T>(g, a, cfg);
+        }
+    }
+    (void)hipEventRecord(eb, g->stream);
+    (void)hipEventSynchronize(eb);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ea, eb);
+    return ms;
+}
+
+// Project autotuner: runs every compiled (BY, KZ, remap) tile of the E and H
+// passes on the grid itself, keeps the fastest, and restores the grid's state
+// (fields, CPML and RC accumulators) from a device snapshot afterwards.
+template <typename T>
+int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
+    std::vector<std::pair<void *, size_t>> state;
+    for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->ncell() * g->tsize()});
+    for (int s = 0; s < 12; s++)
+        if (g->d_psipml[s]) state.push_back({g->d_psipml[s], g->psipml_n[s] * g->tsize()});
+    if (g->d_psi) state.push_back({g->d_psi, (size_t)3 * g->np_max * g->n_disp * sizeof(double2)});
+    std::vector<void *> snap(state.size(), nullptr);
+    for (size_t q = 0; q < state.size(); q++) {
+        int rc = dalloc(g, &snap[q], state[q].second);
+        if (rc) {
+            for (void *&p : snap) dfree(p);
+            return rc;
+        }
+        HIPOK(g, hipMemcpyAsync(snap[q], state[q].first, state[q].second, hipMemcpyDeviceToDevice, g->stream));
+    }
+    hipEvent_t ea, eb;
+    HIPOK(g, hipEventCreate(&ea));
+    HIPOK(g, hipEventCreate(&eb));
+    int best[2] = {g->tile_e, g->tile_h};
+    for (int pass = 0; pass < 2; pass++) {
+        float bt = 1e30f;
+        for (int cfg = 0; cfg < 32; cfg++) {
+            time_cfg<T>(g, pass == 0, cfg, 1, ea, eb);          // warm the instruction cache
+            float ms = time_cfg<T>(g, pass == 0, cfg, reps, ea, eb);
+            if (ms < bt) { bt = ms; best[pass] = cfg; }
+        }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    for (size_t q = 0; q < state.size(); q++) {
+        HIPOK(g, hipMemcpyAsync(state[q].first, snap[q], state[q].second, hipMemcpyDeviceToDevice, g->stream));
+    }
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    for (size_t q = 0; q < state.size(); q++) {
+        g->dev_bytes -= state[q].second;
+        dfree(snap[q]);
+    }
+    HIPOK(g, hipGetLastError());
+    g->tile_e = best[0];
+    g->tile_h = best[1];
+    if (out) { out[0] = best[0]; out[1] = best[1]; }
     return AEFDTD_OK;
 }
 
@@ -1046,9 +700,9 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
     g->xs_s = pml[0] + g->pad[0]; g->xs_e = g->xs_s + Nx;
     g->ys_s = pml[2] + g->pad[2]; g->ys_e = g->ys_s + Ny;
     g->zs_s = pml[4] + g->pad[4]; g->zs_e = g->zs_s + Nz;
-    if ((size_t)g->Nx * g->Ny * g->Nz > ((size_t)1 << 34)) {
+    if ((size_t)g->Nx * g->Ny * g->Nz >= ((size_t)1 << 31)) {     // kernels index cells with 32-bit offsets
         delete g;
-        return cfail(AEFDTD_EINVAL, "grid too large");
+        return cfail(AEFDTD_EINVAL, "grid too large (2^31 cells or more)");
     }
     // basic_differentials_compute (core/fdtd_core.cpp:1137-1148)
     g->dtm = Dt / kMu0;
@@ -1397,6 +1051,29 @@ int aefdtd_read_plane_sum(aefdtd_grid *g, int id, int n, double *out) {
     int m = std::min(n, s.cap);
     HIPOK(g, hipStreamSynchronize(g->stream));
     HIPOK(g, hipMemcpy(out, s.d_out, (size_t)m * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    return AEFDTD_OK;
+}
+
+int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]) {
+    if (!g) return AEFDTD_EINVAL;
+    int rc = ensure_ready(g);
+    if (rc) return rc;
+    if (g->pml_dirty || g->tstep == 0) {
+        if ((rc = upload_profiles(g))) return rc;
+        g->pml_dirty = false;
+    }
+    if (is_column(g)) {
+        if (out) { out[0] = g->tile_e; out[1] = g->tile_h; }
+        return AEFDTD_OK;
+    }
+    return is_fp64(g) ? autotune_T<double>(g, std::max(1, reps), out) : autotune_T<float>(g, std::max(1, reps), out);
+}
+
+int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h) {
+    if (!g) return AEFDTD_EINVAL;
+    if (tile_e < 0 || tile_e > 31 || tile_h < 0 || tile_h > 31) return fail(g, AEFDTD_EINVAL, "tile configuration out of range");
+    g->tile_e = tile_e;
+    g->tile_h = tile_h;
     return AEFDTD_OK;
 }
 

@@ -1,0 +1,608 @@
+// kernels.hpp -- gfx950 device code of the FDTD engine (included by engine.hip).
+//
+// One fused pass per half step keeps the leapfrog at its HBM floor:
+//   E pass : read E(3) H(3) mat -> RC-ante (FP64 poles) -> E stencil -> TFSF-E
+//            -> RC-apply -> CPML-E (slab psi) -> PEC -> plane injection -> write E(3)
+//   H pass : read H(3) E(3)     -> H stencil -> CPML-H -> PEC -> plane injection
+//            -> TFSF-H -> write H(3)
+// This is the phase order of Aether's threaded_process_E/H (core/fdtd_threads.
+// cpp:32-235) and of its drivers (modes/normal_incidence.cpp:330-357,
+// modes/single_particle.cpp:256-378); every phase is cell-local, so fusing them
+// keeps each cell's sequence of floating-point updates.
+//
+// Thread mapping of the 3D passes: a workgroup is 64 x BY lanes over (x, y);
+// x is the contiguous axis, so one wavefront reads 64 consecutive values of a
+// field.  Each lane marches KZ planes in z and carries the previous plane's H
+// (E pass) / next plane's E (H pass) in registers.  Workgroups are launched
+// as a 1D grid; with `remap` set, the hardware workgroup id is folded so that
+// each of the 8 XCDs owns one contiguous range of tiles (x fastest, then y,
+// then z), which keeps the halo rows of neighbouring tiles in one L2.
+// The per-material coefficients are staged once per workgroup in LDS.
+#pragma once
+
+#include "engine.hpp"
+
+namespace aefdtd {
+namespace dev {
+
+constexpr int BX = 64;
+constexpr int KS = 64;             // DFT staging depth (steps per accumulate)
+constexpr int DFT_LCH = 32;        // wavelengths per LDS chunk in the DFT pass
+constexpr int MAX_MATS = 256;
+
+// Per-material constants (FDTD_Material, core/fdtd_material.h:46-47).
+template <typename T>
+struct MatDev {
+    T C1, C2x, C2y, C2z, C4;
+    int rc;          // recursive-convolution poles present
+    double C3;
+    int np, pole0;
+};
+
+// 1D profile arrays per axis a (length N_a): ikE, ikH (inverse kappa with the
+// stencil's gating), bE, cE (c * 1/D), bH, cH.
+template <typename T>
+struct Prof {
+    const T *ikE[3], *ikH[3], *bE[3], *cE[3], *bH[3], *cH[3];
+};
+
+template <typename T>
+struct EArgs {
+    int Nx, Ny, Nz, nmat;
+    int nbx, nby, nbz, remap;
+    T *Ex, *Ey, *Ez;
+    const T *Hx, *Hy, *Hz;
+    const uint8_t *mat;
+    const MatDev<T> *mt;
+    const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
+    const int32_t *slot;
+    double2 *psi;
+    int ndisp;
+    Prof<T> pf;
+    int pml[6];
+    T *pEyx, *pEzx, *pExy, *pEzy, *pExz, *pEyz;   // slab psi, x:(pxm+pxp,Ny,Nz) y:(Nx,pym+pyp,Nz) z:(Nx,Ny,pzm+pzp)
+    int en;                    // bit c set: component c enabled
+    int inj_k;
+    T injx, injy;
+    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
+    const T *aHx, *aHy, *aHz;  // aux 1D grid H columns
+};
+
+template <typename T>
+struct HArgs {
+    int Nx, Ny, Nz;
+    int nbx, nby, nbz, remap;
+    T *Hx, *Hy, *Hz;
+    const T *Ex, *Ey, *Ez;
+    Prof<T> pf;
+    int pml[6];
+    T *pHyx, *pHzx, *pHxy, *pHzy, *pHxz, *pHyz;
+    int en;
+    T dtm, dtdmx, dtdmy, dtdmz;
+    int inj_k;
+    T injx, injy;
+    int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
+    const T *aEx, *aEy, *aEz;
+};
+
+// Hardware workgroup id -> logical tile (bx, by, bz).  Workgroups are dealt to
+// the 8 XCDs round-robin; the fold gives XCD x the x-th contiguous range.
+__device__ __forceinline__ void tile_of(int nbx, int nby, int nbz, int remap, int &bx, int &by, int &bz) {
+    int w = blockIdx.x;
+    if (remap) {
+        const int B = nbx * nby * nbz;
+        const int q = B / 8, r = B % 8;
+        const int x = w % 8, loc = w / 8;
+        w = (x < r) ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+    }
+    bx = w % nbx;
+    const int t = w / nbx;
+    by = t % nby;
+    bz = t / nby;
+}
+
+template <typename T>
+struct MatL {
+    T C1, C2x, C2y, C2z, C4;
+    int rc;
+};
+
+// ---------------------------------------------------------------- E cell
+// Everything after the loads of one E cell; dH* are the curl differences.
+template <typename T>
+__device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
+                                       T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y, T ikx, T iky, int nx_,
+                                       int ny_) {
+    const T ex0 = a.Ex[id], ey0 = a.Ey[id], ez0 = a.Ez[id];
+    // RC-ante: psi = Crec psi + Dchi E^n (core/mats_RC.cpp:21-36), FP64
+    double sx = 0, sy = 0, sz = 0;
+    double C3 = 0;
+    if (M.rc) {
+        const MatDev<T> &G = a.mt[m];
+        C3 = G.C3;
+        const int s = a.slot[id];
+        const double e3[3] = {(double)ex0, (double)ey0, (double)ez0};
+        for (int p = 0; p < G.np; p++) {
+            const double2 cr = a.poles[2 * (G.pole0 + p)], dc = a.poles[2 * (G.pole0 + p) + 1];
+            double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
+            double acc[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double2 v = q[(size_t)c * a.ndisp];
+                double2 r;
+                r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
+                r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
+                q[(size_t)c * a.ndisp] = r;
+                acc[c] = r.x;
+            }
+            sx += acc[0]; sy += acc[1]; sz += acc[2];
+        }
+    }
+    const T ikz = a.pf.ikE[2][k];
+    // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
+    T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
+    T ey = (a.en & 2) ? M.C1 * ey0 + M.C2z * ikz * dHx_z - M.C2x * ikx * dHz_x : ey0;
+    T ez = (a.en & 4) ? M.C1 * ez0 + M.C2x * ikx * dHy_x - M.C2y * iky * dHx_y : ez0;
+    // TFSF-E (modes/single_particle.cpp:286-332): z faces, then y, then x
+    if (a.tfsf) {
+        const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
+        if (in_i && in_j) {
+            if (k == a.tb4) { ex += M.C2z * a.aHy[a.tb4 - 1]; ey -= M.C2z * a.aHx[a.tb4 - 1]; }
+            if (k == a.tb5) { ex -= M.C2z * a.aHy[a.tb5 - 1]; ey += M.C2z * a.aHx[a.tb5 - 1]; }
+        }
+        if (!a.tper_y && in_i && in_k) {
+            if (j == a.tb2) { ex -= M.C2y * a.aHz[k]; ez += M.C2y * a.aHx[k]; }
+            if (j == a.tb3) { ex += M.C2y * a.aHz[k]; ez -= M.C2y * a.aHx[k]; }
+        }
+        if (!a.tper_x && in_k && in_j) {
+            if (i == a.tb0) { ey += M.C2x * a.aHz[k]; ez -= M.C2x * a.aHy[k]; }
+            if (i == a.tb1) { ey -= M.C2x * a.aHz[k]; ez += M.C2x * a.aHy[k]; }
+        }
+    }
+    // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50)
+    if (M.rc) {
+        ex += (T)(C3 * sx);
+        ey += (T)(C3 * sy);
+        ez += (T)(C3 * sz);
+    }
+    // CPML-E + PEC walls (core/fdtd_pml.cpp:504-735)
+    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
+    int nz_ = -1;
+    if (pz) {
+        if (k >= 1 && k < a.pml[4]) nz_ = k;
+        else if (k >= a.Nz - a.pml[5] + 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
+    }
+    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
+    const int NxNy = a.Nx * a.Ny;
+    if (a.en & 1) {
+        if (py) {
+            if (ny_ >= 0) {
+                T &q = a.pExy[i + a.Nx * (ny_ + wy * k)];
+                q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHz_y;
+                ex += M.C4 * q;
+            }
+            if (j == 0) ex = 0;
+        }
+        if (pz) {
+            if (nz_ >= 0) {
+                T &q = a.pExz[i + a.Nx * j + NxNy * nz_];
+                q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHy_z;
+                ex -= M.C4 * q;
+            }
+            if (k == 0) ex = 0;
+        }
+    }
+    if (a.en & 2) {
+        if (px) {
+            if (nx_ >= 0) {
+                T &q = a.pEyx[nx_ + wx * (j + a.Ny * k)];
+                q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHz_x;
+                ey -= M.C4 * q;
+            }
+            if (i == 0) ey = 0;
+        }
+        if (pz) {
+            if (nz_ >= 0) {
+                T &q = a.pEyz[i + a.Nx * j + NxNy * nz_];
+                q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHx_z;
+                ey += M.C4 * q;
+            }
+            if (k == 0) ey = 0;
+        }
+    }
+    if (a.en & 4) {
+        if (px) {
+            if (nx_ >= 0) {
+                T &q = a.pEzx[nx_ + wx * (j + a.Ny * k)];
+                q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHy_x;
+                ez += M.C4 * q;
+            }
+            if (i == 0) ez = 0;
+        }
+        if (py) {
+            if (ny_ >= 0) {
+                T &q = a.pEzy[i + a.Nx * (ny_ + wy * k)];
+                q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHx_y;
+                ez -= M.C4 * q;
+            }
+            if (j == 0) ez = 0;
+        }
+    }
+    // plane injection after the whole E update (modes/normal_incidence.cpp:338-342)
+    if (k == a.inj_k) { ex += a.injx; ey += a.injy; }
+    a.Ex[id] = ex;
+    a.Ey[id] = ey;
+    a.Ez[id] = ez;
+}
+
+// CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
+__device__ __forceinline__ int slabE(int v, int N, int pm, int pp) {
+    if (!(pm || pp)) return -1;
+    if (v >= 1 && v < pm) return v;
+    if (v >= N - pp + 1) return v - (N - pp) + pm;
+    return -1;
+}
+// ... of an H cell (near 0..pm-1, far N-pp..N-2).
+__device__ __forceinline__ int slabH(int v, int N, int pm, int pp) {
+    if (!(pm || pp)) return -1;
+    if (v < pm) return v;
+    if (v >= N - pp && v < N - 1) return v - (N - pp) + pm;
+    return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void stage_mats(const EArgs<T> &a, MatL<T> *sm) {
+    for (int m = threadIdx.x + blockDim.x * threadIdx.y; m < a.nmat; m += blockDim.x * blockDim.y) {
+        const MatDev<T> &G = a.mt[m];
+        sm[m] = MatL<T>{G.C1, G.C2x, G.C2y, G.C2z, G.C4, G.rc};
+    }
+    __syncthreads();
+}
+
+// 3D E pass.
+template <typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
+    __shared__ MatL<T> sm[MAX_MATS];
+    stage_mats(a, sm);
+    int bx, by, bz;
+    tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    const int i = bx * BX + threadIdx.x;
+    const int j = by * BY + threadIdx.y;
+    if (i >= a.Nx || j >= a.Ny) return;
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.Nz);
+    const int NxNy = a.Nx * a.Ny;
+    const int base = i + a.Nx * j;
+    const int oim = (i == 0) ? a.Nx - 1 : -1;                    // (i-1) with wrap
+    const int ojm = (j == 0) ? (a.Ny - 1) * a.Nx : -a.Nx;        // (j-1) with wrap
+    const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
+    T hx_km = a.Hx[base + km0 * NxNy];
+    T hy_km = a.Hy[base + km0 * NxNy];
+    const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
+    const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
+    const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+    for (int k = k0; k < k1; k++) {
+        const int id = base + k * NxNy;
+        const T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
+        const T dHz_y = hz - a.Hz[id + ojm];
+        const T dHz_x = hz - a.Hz[id + oim];
+        const T dHy_x = hy - a.Hy[id + oim];
+        const T dHx_y = hx - a.Hx[id + ojm];
+        const T dHy_z = hy - hy_km;
+        const T dHx_z = hx - hx_km;
+        hx_km = hx;
+        hy_km = hy;
+        const int m = a.mat[id];
+        e_cell(a, sm[m], m, i, j, k, id, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky, nx_, ny_);
+    }
+}
+
+// E pass of a 1 x 1 x Nz column (the auxiliary grid of the TFSF source):
+// one lane per cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_E_col(const EArgs<T> a) {
+    __shared__ MatL<T> sm[MAX_MATS];
+    stage_mats(a, sm);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.Nz) return;
+    const int km = (k == 0) ? a.Nz - 1 : k - 1;
+    const T hx = a.Hx[k], hy = a.Hy[k];
+    const int m = a.mat[k];
+    const T zero = 0;
+    e_cell(a, sm[m], m, 0, 0, k, k, zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero, zero, a.pf.ikE[0][0],
+           a.pf.ikE[1][0], -1, -1);
+}
+
+// ---------------------------------------------------------------- H cell
+template <typename T>
+__device__ __forceinline__ void h_cell(const HArgs<T> &a, int i, int j, int k, int id, T dEy_z, T dEx_z, T dEz_y,
+                                       T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
+    const T ikz = a.pf.ikH[2][k];
+    T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
+    // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
+    if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
+    if (a.en & 16) hy += a.dtdmx * ikx * dEz_x - a.dtdmz * ikz * dEx_z;
+    if (a.en & 32) hz += a.dtdmy * iky * dEx_y - a.dtdmx * ikx * dEy_x;
+    // CPML-H + PEC walls (core/fdtd_pml.cpp:737-929)
+    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
+    const int nz_ = slabH(k, a.Nz, a.pml[4], a.pml[5]);
+    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
+    const int NxNy = a.Nx * a.Ny;
+    if (a.en & 8) {
+        if (py) {
+            if (ny_ >= 0) {
+                T &q = a.pHxy[i + a.Nx * (ny_ + wy * k)];
+                q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEz_y;
+                hx -= a.dtm * q;
+            }
+            if (j == a.Ny - 1) hx = 0;
+        }
+        if (pz) {
+            if (nz_ >= 0) {
+                T &q = a.pHxz[i + a.Nx * j + NxNy * nz_];
+                q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEy_z;
+                hx += a.dtm * q;
+            }
+            if (k == a.Nz - 1) hx = 0;
+        }
+    }
+    if (a.en & 16) {
+        if (px) {
+            if (nx_ >= 0) {
+                T &q = a.pHyx[nx_ + wx * (j + a.Ny * k)];
+                q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEz_x;
+                hy += a.dtm * q;
+            }
+            if (i == a.Nx - 1) hy = 0;
+        }
+        if (pz && nz_ >= 0) {
+            T &q = a.pHyz[i + a.Nx * j + NxNy * nz_];
+            q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEx_z;
+            hy -= a.dtm * q;
+        }
+    }
+    // top z wall of Hy: zeroed by the Hx pass and by the Hy pass (fdtd_pml.cpp:799,862-865)
+    if (pz && k == a.Nz - 1 && (a.en & (8 | 16))) hy = 0;
+    if (a.en & 32) {
+        if (px) {
+            if (nx_ >= 0) {
+                T &q = a.pHzx[nx_ + wx * (j + a.Ny * k)];
+                q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEy_x;
+                hz -= a.dtm * q;
+            }
+            if (i == a.Nx - 1) hz = 0;
+        }
+        if (py) {
+            if (ny_ >= 0) {
+                T &q = a.pHzy[i + a.Nx * (ny_ + wy * k)];
+                q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEx_y;
+                hz += a.dtm * q;
+            }
+            if (j == a.Ny - 1) hz = 0;
+        }
+    }
+    // plane injection (modes/normal_incidence.cpp:353-357)
+    if (k == a.inj_k) { hx += a.injx; hy += a.injy; }
+    // TFSF-H (modes/single_particle.cpp:351-378): z faces, then y, then x
+    if (a.tfsf) {
+        const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
+        if (in_i && in_j) {
+            if (k == a.tb4 - 1) { hx -= a.dtdmz * a.aEy[a.tb4]; hy += a.dtdmz * a.aEx[a.tb4]; }
+            if (k == a.tb5 - 1) { hx += a.dtdmz * a.aEy[a.tb5]; hy -= a.dtdmz * a.aEx[a.tb5]; }
+        }
+        if (!a.tper_y && in_i && in_k) {
+            if (j == a.tb2 - 1) { hx += a.dtdmy * a.aEz[k]; hz -= a.dtdmy * a.aEx[k]; }
+            if (j == a.tb3 - 1) { hx -= a.dtdmy * a.aEz[k]; hz += a.dtdmy * a.aEx[k]; }
+        }
+        if (!a.tper_x && in_k && in_j) {
+            if (i == a.tb0 - 1) { hy -= a.dtdmx * a.aEz[k]; hz += a.dtdmx * a.aEy[k]; }
+            if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
+        }
+    }
+    a.Hx[id] = hx;
+    a.Hy[id] = hy;
+    a.Hz[id] = hz;
+}
+
+template <typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<T> a) {
+    int bx, by, bz;
+    tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    const int i = bx * BX + threadIdx.x;
+    const int j = by * BY + threadIdx.y;
+    if (i >= a.Nx || j >= a.Ny) return;
+    const int k0 = bz * KZ;
+    const int k1 = min(k0 + KZ, a.Nz);
+    const int NxNy = a.Nx * a.Ny;
+    const int base = i + a.Nx * j;
+    const int oip = (i == a.Nx - 1) ? -(a.Nx - 1) : 1;
+    const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.Nx : a.Nx;
+    T ex_k = a.Ex[base + k0 * NxNy];
+    T ey_k = a.Ey[base + k0 * NxNy];
+    const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
+    const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
+    const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
+    for (int k = k0; k < k1; k++) {
+        const int id = base + k * NxNy;
+        const int kp = (k == a.Nz - 1) ? 0 : k + 1;
+        const T ex = ex_k, ey = ey_k;
+        const T ex_kp = a.Ex[base + kp * NxNy];
+        const T ey_kp = a.Ey[base + kp * NxNy];
+        ex_k = ex_kp;
+        ey_k = ey_kp;
+        const T ez = a.Ez[id];
+        h_cell(a, i, j, k, id, ey_kp - ey, ex_kp - ex, a.Ez[id + ojp] - ez, a.Ez[id + oip] - ez,
+               a.Ex[id + ojp] - ex, a.Ey[id + oip] - ey, ikx, iky, nx_, ny_);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_update_H_col(const HArgs<T> a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.Nz) return;
+    const int kp = (k == a.Nz - 1) ? 0 : k + 1;
+    const T zero = 0;
+    h_cell(a, 0, 0, k, k, a.Ey[kp] - a.Ey[k], a.Ex[kp] - a.Ex[k], zero, zero, zero, zero, a.pf.ikH[0][0],
+           a.pf.ikH[1][0], -1, -1);
+}
+
+// ================================================================ sensors
+struct SensDesc {
+    int type, x1, y1, z1, span1, span2;
+    long P, off;
+};
+
+// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) on one sensor plane.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, const T *Ex, const T *Ey,
+                                                const T *Ez, const T *Hx, const T *Hy, const T *Hz, int Nx, int Ny,
+                                                int Nz, T *stage) {
+    const SensDesc d = sd[blockIdx.y];
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    const int a = (int)(p % d.span1), b = (int)(p / d.span1);
+    int i, j, k;
+    if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
+    else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) { i = d.x1 + a; j = d.y1; k = d.z1 + b; }
+    else { i = d.x1 + a; j = d.y1 + b; k = d.z1; }
+    const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
+    auto at = [&](const T *f, int x, int y, int z) { return f[(size_t)x + (size_t)Nx * ((size_t)y + (size_t)Ny * z)]; };
+    T v[6];
+    v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
+    v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
+    v[2] = (at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0;
+    v[3] = (at(Hx, i, j, k) + at(Hx, ip, j, k)) / (T)2.0;
+    v[4] = (at(Hy, i, j, k) + at(Hy, i, jp, k)) / (T)2.0;
+    v[5] = (at(Hz, i, j, k) + at(Hz, i, j, kp)) / (T)2.0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) stage[((size_t)slot * 6 + c) * Ptot + d.off + p] = v[c];
+}
+
+// Running DFT over a block of nfill <= KS staged steps (sensors/sensors.cpp:
+// 456-479): acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p], FP64.
+// One lane per (cell, component) keeps its KS samples in registers; the
+// coefficients of DFT_LCH wavelengths at a time are staged in LDS and read as
+// broadcasts, so each accumulator is read and written once per KS steps.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
+                                             const double2 *coef, int s0, int nfill, double2 *acc) {
+    __shared__ double2 W[KS * DFT_LCH];
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    const int h = c >= 3 ? 1 : 0;
+    const bool live = p < P;
+    double x[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) x[s] = (live && s < nfill) ? (double)stage[((size_t)s * 6 + c) * Ptot + off + p] : 0.0;
+    for (int l0 = 0; l0 < Nl; l0 += DFT_LCH) {
+        const int nl = min(DFT_LCH, Nl - l0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < KS * DFT_LCH; q += blockDim.x) {
+            const int s = q / DFT_LCH, l = q % DFT_LCH;
+            W[q] = (s < nfill && l < nl) ? coef[((size_t)(s0 + s) * 2 + h) * Nl + l0 + l] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (live) {
+            for (int l = 0; l < nl; l++) {
+                double re = 0, im = 0;
+#pragma unroll
+                for (int s = 0; s < KS; s++) {
+                    const double2 w = W[s * DFT_LCH + l];
+                    re += w.x * x[s];
+                    im += w.y * x[s];
+                }
+                double2 &A = acc[((size_t)c * Nl + l0 + l) * P + p];
+                A.x += re;
+                A.y += im;
+            }
+        }
+    }
+}
+
+// Spect_Poynting::treat (sensors/s_spec_power.cpp:61-111): one workgroup per
+// wavelength, fixed-order tree reduction so the result is deterministic.
+__global__ __launch_bounds__(256) void k_poynting(const double2 *acc, long P, int Nl, double f, int type, double *rsp) {
+    const int l = blockIdx.x;
+    __shared__ double sh[3][256];
+    double S[3] = {0, 0, 0};
+    for (long p = threadIdx.x; p < P; p += blockDim.x) {
+        double2 e[3], h[3];
+        for (int c = 0; c < 3; c++) {
+            e[c] = acc[((size_t)c * Nl + l) * P + p];
+            h[c] = acc[((size_t)(3 + c) * Nl + l) * P + p];
+        }
+        // Re(a conj(b)) = a.x b.x + a.y b.y
+        S[0] += (e[1].x * h[2].x + e[1].y * h[2].y) - (e[2].x * h[1].x + e[2].y * h[1].y);
+        S[1] += (e[2].x * h[0].x + e[2].y * h[0].y) - (e[0].x * h[2].x + e[0].y * h[2].y);
+        S[2] += (e[0].x * h[1].x + e[0].y * h[1].y) - (e[1].x * h[0].x + e[1].y * h[0].y);
+    }
+    for (int c = 0; c < 3; c++) sh[c][threadIdx.x] = S[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 3; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double Sx = sh[0][0] * f, Sy = sh[1][0] * f, Sz = sh[2][0] * f;
+        double r;
+        switch (type) {
+            case AEFDTD_NORMAL_X: r = Sx; break;
+            case AEFDTD_NORMAL_Y: r = Sy; break;
+            case AEFDTD_NORMAL_Z: r = Sz; break;
+            case AEFDTD_NORMAL_XM: r = -Sx; break;
+            case AEFDTD_NORMAL_YM: r = -Sy; break;
+            default: r = -Sz; break;
+        }
+        rsp[l] = r;
+    }
+}
+
+// Analytic pulse on the device (core/chpin.cpp:44-90).
+__device__ inline double d_pulse(const aefdtd_pulse &p, double amp, double x, double y, double z, double t) {
+    const double ph = p.kx * x + p.ky * y + p.kz * z;
+    const double a = ph - p.omega * t;
+    const double b = (1.0 / p.k * ph - kC * t / p.n_ind + p.C1) / p.dz;
+    return amp * exp(-b * b) * cos(a);
+}
+
+// R/T plane sums of modes/normal_incidence.cpp:384-415 for step t; one
+// workgroup, fixed-order reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, const T *Ez, int Nx, int Ny, int kpl,
+                                                   aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
+                                                   double Dz, double zpos, const double *tap, int t, double Dt,
+                                                   double *out) {
+    __shared__ double sh[6][256];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double tb = (t + 0.5) * Dt;
+    const double tp = tap[t];
+    const size_t NxNy = (size_t)Nx * Ny;
+    for (long q = threadIdx.x; q < (long)NxNy; q += blockDim.x) {
+        const int i = (int)(q % Nx), j = (int)(q / Nx);
+        const double x = i * Dx, y = j * Dy;
+        double in[3] = {0, 0, 0};
+        if (has_inc) {
+            in[0] = d_pulse(inc, inc.Ex0, x + Dx / 2.0, y, zpos, tb);
+            in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
+            in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
+        }
+        const size_t id = (size_t)q + NxNy * kpl;
+        const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
+        for (int c = 0; c < 3; c++) {
+            acc[3 + c] += in[c] * tp;
+            acc[c] += (subtract ? (f[c] - in[c]) : f[c]) * tp;
+        }
+    }
+    for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[(size_t)t * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+}  // namespace dev
+}  // namespace aefdtd

@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--n", type=int, default=256, help="structure cells per side")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (rocprof counter calibration)")
@@ -126,6 +127,7 @@ def main():
     ncell = stats[0]
 
     run.step(args.warmup)
+    tiles = run.autotune(3) if not args.no_tune else None
     run.synchronize()
     run.profile(True)
     t0 = time.perf_counter()
@@ -172,6 +174,7 @@ def main():
         "step_hbm_fraction": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
         "setup_s": round(setup_s, 2),
+        "tiles": tiles,
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

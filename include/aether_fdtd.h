@@ -269,6 +269,12 @@ int aefdtd_synchronize(aefdtd_grid *g);
  * while profiling is enabled: names "E","H","sensor","dft".  Returns count. */
 int aefdtd_profile_enable(aefdtd_grid *g, int on);
 int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches);
+/* Tile autotuner: times every compiled tile shape of the E and H passes on
+ * this grid (reps launches each), keeps the fastest and restores the grid's
+ * state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
+ * XCD-remap<<4).  No counterpart in the reference (a CPU thread-count choice). */
+int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
+int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
 /* Bytes of device memory held by the handle. */
 int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
 /* out[0]=cells of the full grid, out[1]=dispersive (RC) cells, out[2]=max

@@ -157,3 +157,35 @@ def test_single_particle_fp32_spectra(engine_be, oracle_be):
     err = np.max(np.abs(g["sensors"] - r["sensors"])) / scale
     print("fp32 relative spectral error", err)
     assert err <= 1e-4
+
+
+@pytest.mark.parametrize("cfg", range(32))
+def test_every_tile_shape_matches_oracle(engine_be, oracle_be, cfg):
+    """Each compiled (BY, KZ, XCD-remap) tile of the E/H passes updates every cell."""
+    import ctypes as C
+    rng = np.random.default_rng(cfg)
+    n = (70, 13, 37)
+    mg = rng.integers(0, 3, size=(n[2], n[1], n[0])).astype(np.uint32)
+    e, o = build_pair(engine_be, oracle_be, n, (3, 4, 2, 3, 5, 2), (0,) * 6, [Material(1.5), GOLD, LORENTZ], mg, 8)
+    assert engine_be.fn("set_tiles")(e.h, cfg, cfg) == 0
+    for _ in range(4):
+        e.update_E(); o.update_E()
+        e.update_H(); o.update_H()
+    compare(e, o)
+
+
+def test_autotune_keeps_state(engine_be, oracle_be):
+    """The autotuner runs the passes on the grid itself and must restore its state."""
+    import ctypes as C
+    rng = np.random.default_rng(12)
+    n = (40, 21, 30)
+    mg = rng.integers(0, 2, size=(n[2], n[1], n[0])).astype(np.uint32)
+    e, o = build_pair(engine_be, oracle_be, n, (4, 4, 4, 4, 4, 4), (0,) * 6, [Material(2.0), GOLD], mg, 3)
+    for _ in range(3):
+        e.step(); o.step()
+    out = (C.c_int * 2)()
+    assert engine_be.fn("autotune")(e.h, 2, out) == 0
+    assert 0 <= out[0] < 32 and 0 <= out[1] < 32
+    for _ in range(3):
+        e.step(); o.step()
+    compare(e, o)
