@@ -111,9 +111,8 @@ struct MatL {
 // Everything after the loads of one E cell; dH* are the curl differences.
 template <typename T>
 __device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
-                                       T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y, T ikx, T iky, int nx_,
-                                       int ny_) {
-    const T ex0 = a.Ex[id], ey0 = a.Ey[id], ez0 = a.Ez[id];
+                                       T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
+                                       T ikx, T iky, int nx_, int ny_) {
     // RC-ante: psi = Crec psi + Dchi E^n (core/mats_RC.cpp:21-36), FP64
     double sx = 0, sy = 0, sz = 0;
     double C3 = 0;
@@ -281,19 +280,37 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
     const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
     const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+    // software pipeline: the loads of plane k+1 are issued before plane k is
+    // computed and stored, so each lane keeps two planes of loads in flight
+    struct Ld {
+        T hx, hy, hz, hz_jm, hz_im, hy_im, hx_jm, ex, ey, ez;
+        int m;
+    };
+    auto load = [&](int id) {
+        Ld v;
+        v.hx = a.Hx[id]; v.hy = a.Hy[id]; v.hz = a.Hz[id];
+        v.hz_jm = a.Hz[id + ojm]; v.hz_im = a.Hz[id + oim];
+        v.hy_im = a.Hy[id + oim]; v.hx_jm = a.Hx[id + ojm];
+        v.ex = a.Ex[id]; v.ey = a.Ey[id]; v.ez = a.Ez[id];
+        v.m = a.mat[id];
+        return v;
+    };
+    Ld cur = load(base + k0 * NxNy);
     for (int k = k0; k < k1; k++) {
         const int id = base + k * NxNy;
-        const T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
-        const T dHz_y = hz - a.Hz[id + ojm];
-        const T dHz_x = hz - a.Hz[id + oim];
-        const T dHy_x = hy - a.Hy[id + oim];
-        const T dHx_y = hx - a.Hx[id + ojm];
-        const T dHy_z = hy - hy_km;
-        const T dHx_z = hx - hx_km;
-        hx_km = hx;
-        hy_km = hy;
-        const int m = a.mat[id];
-        e_cell(a, sm[m], m, i, j, k, id, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky, nx_, ny_);
+        Ld nxt = cur;
+        if (k + 1 < k1) nxt = load(id + NxNy);
+        const T dHz_y = cur.hz - cur.hz_jm;
+        const T dHz_x = cur.hz - cur.hz_im;
+        const T dHy_x = cur.hy - cur.hy_im;
+        const T dHx_y = cur.hx - cur.hx_jm;
+        const T dHy_z = cur.hy - hy_km;
+        const T dHx_z = cur.hx - hx_km;
+        hx_km = cur.hx;
+        hy_km = cur.hy;
+        e_cell(a, sm[cur.m], cur.m, i, j, k, id, cur.ex, cur.ey, cur.ez, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y,
+               ikx, iky, nx_, ny_);
+        cur = nxt;
     }
 }
 
@@ -309,16 +326,15 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<T> a) {
     const T hx = a.Hx[k], hy = a.Hy[k];
     const int m = a.mat[k];
     const T zero = 0;
-    e_cell(a, sm[m], m, 0, 0, k, k, zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero, zero, a.pf.ikE[0][0],
-           a.pf.ikE[1][0], -1, -1);
+    e_cell(a, sm[m], m, 0, 0, k, k, a.Ex[k], a.Ey[k], a.Ez[k], zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero,
+           zero, a.pf.ikE[0][0], a.pf.ikE[1][0], -1, -1);
 }
 
 // ---------------------------------------------------------------- H cell
 template <typename T>
-__device__ __forceinline__ void h_cell(const HArgs<T> &a, int i, int j, int k, int id, T dEy_z, T dEx_z, T dEz_y,
-                                       T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
+__device__ __forceinline__ void h_cell(const HArgs<T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
+                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
     const T ikz = a.pf.ikH[2][k];
-    T hx = a.Hx[id], hy = a.Hy[id], hz = a.Hz[id];
     // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
     if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
     if (a.en & 16) hy += a.dtdmx * ikx * dEz_x - a.dtdmz * ikz * dEx_z;
@@ -417,22 +433,39 @@ __global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<T> a) {
     const int base = i + a.Nx * j;
     const int oip = (i == a.Nx - 1) ? -(a.Nx - 1) : 1;
     const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.Nx : a.Nx;
-    T ex_k = a.Ex[base + k0 * NxNy];
-    T ey_k = a.Ey[base + k0 * NxNy];
     const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
+    // software pipeline as in the E pass; E(k+1) of plane k is E(k) of plane k+1
+    struct Ld {
+        T hx, hy, hz, ex, ey, ez, ez_jp, ez_ip, ex_jp, ey_ip;
+    };
+    auto load = [&](int id) {
+        Ld v;
+        v.hx = a.Hx[id]; v.hy = a.Hy[id]; v.hz = a.Hz[id];
+        v.ex = a.Ex[id]; v.ey = a.Ey[id]; v.ez = a.Ez[id];
+        v.ez_jp = a.Ez[id + ojp]; v.ez_ip = a.Ez[id + oip];
+        v.ex_jp = a.Ex[id + ojp]; v.ey_ip = a.Ey[id + oip];
+        return v;
+    };
+    Ld cur = load(base + k0 * NxNy);
     for (int k = k0; k < k1; k++) {
         const int id = base + k * NxNy;
         const int kp = (k == a.Nz - 1) ? 0 : k + 1;
-        const T ex = ex_k, ey = ey_k;
-        const T ex_kp = a.Ex[base + kp * NxNy];
-        const T ey_kp = a.Ey[base + kp * NxNy];
-        ex_k = ex_kp;
-        ey_k = ey_kp;
-        const T ez = a.Ez[id];
-        h_cell(a, i, j, k, id, ey_kp - ey, ex_kp - ex, a.Ez[id + ojp] - ez, a.Ez[id + oip] - ez,
-               a.Ex[id + ojp] - ex, a.Ey[id + oip] - ey, ikx, iky, nx_, ny_);
+        Ld nxt;
+        T ex_kp, ey_kp;
+        if (k + 1 < k1) {
+            nxt = load(id + NxNy);
+            ex_kp = nxt.ex;
+            ey_kp = nxt.ey;
+        } else {
+            nxt = cur;
+            ex_kp = a.Ex[base + kp * NxNy];
+            ey_kp = a.Ey[base + kp * NxNy];
+        }
+        h_cell(a, i, j, k, id, cur.hx, cur.hy, cur.hz, ey_kp - cur.ey, ex_kp - cur.ex, cur.ez_jp - cur.ez,
+               cur.ez_ip - cur.ez, cur.ex_jp - cur.ex, cur.ey_ip - cur.ey, ikx, iky, nx_, ny_);
+        cur = nxt;
     }
 }
 
@@ -442,8 +475,8 @@ __global__ __launch_bounds__(256) void k_update_H_col(const HArgs<T> a) {
     if (k >= a.Nz) return;
     const int kp = (k == a.Nz - 1) ? 0 : k + 1;
     const T zero = 0;
-    h_cell(a, 0, 0, k, k, a.Ey[kp] - a.Ey[k], a.Ex[kp] - a.Ex[k], zero, zero, zero, zero, a.pf.ikH[0][0],
-           a.pf.ikH[1][0], -1, -1);
+    h_cell(a, 0, 0, k, k, a.Hx[k], a.Hy[k], a.Hz[k], a.Ey[kp] - a.Ey[k], a.Ex[kp] - a.Ex[k], zero, zero, zero,
+           zero, a.pf.ikH[0][0], a.pf.ikH[1][0], -1, -1);
 }
 
 // ================================================================ sensors
