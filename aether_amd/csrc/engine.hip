@@ -250,15 +250,25 @@ int upload_grid(aefdtd_grid *g) {
     for (unsigned m = 0; m < g->Nmat; m++) any = any || g->mats[m].rc;
     dfree_t(g->d_slot);
     dfree_t(g->d_psi);
+    dfree_t(g->d_cells);
+    dfree(g->d_corr);
     g->n_disp = 0;
     if (any) {
         slot.assign(N, -1);
+        std::vector<int32_t> cells;
         for (size_t c = 0; c < N; c++)
-            if (g->mats[g->matsgrid[c]].rc) slot[c] = (int32_t)nd++;
+            if (g->mats[g->matsgrid[c]].rc) {
+                slot[c] = (int32_t)nd++;
+                cells.push_back((int32_t)c);
+            }
         void *p;
         if ((rc = dalloc(g, &p, N * sizeof(int32_t)))) return rc;
         g->d_slot = (int32_t *)p;
         HIPOK(g, hipMemcpyAsync(g->d_slot, slot.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        if ((rc = dalloc(g, &p, cells.size() * sizeof(int32_t)))) return rc;
+        g->d_cells = (int32_t *)p;
+        HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        if ((rc = dalloc(g, &g->d_corr, (size_t)3 * nd * g->tsize()))) return rc;
         size_t np = (size_t)3 * g->np_max * nd;
         if ((rc = dalloc(g, &p, np * sizeof(double2)))) return rc;
         g->d_psi = (double2 *)p;
@@ -315,7 +325,7 @@ constexpr int kKZ[4] = {4, 8, 16, 32};
 
 template <typename T, int BY, int KZ>
 void launch_E_cfg(const EArgs<T> &a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_update_E<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+    hipLaunchKernelGGL((k_update_E<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
 }
 template <typename T, int BY, int KZ>
 void launch_H_cfg(const HArgs<T> &a, int nblocks, hipStream_t st) {
@@ -369,8 +379,7 @@ EArgs<T> e_args(aefdtd_grid *g) {
     a.mt = (const MatDev<T> *)g->d_mtab;
     a.poles = g->d_poles;
     a.slot = g->d_slot;
-    a.psi = g->d_psi;
-    a.ndisp = (int)g->n_disp;
+    a.corr = (const T *)g->d_corr;
     a.pf = prof_of<T>(g);
     for (int f = 0; f < 6; f++) a.pml[f] = g->pml[f];
     a.pEyx = (T *)g->d_psipml[0]; a.pEzx = (T *)g->d_psipml[1];
@@ -424,7 +433,8 @@ HArgs<T> h_args(aefdtd_grid *g) {
 template <typename T>
 void dispatch_E(aefdtd_grid *g, EArgs<T> &a, int cfg) {
     if (is_column(g)) {
-        hipLaunchKernelGGL(k_update_E_col<T>, dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
+        hipLaunchKernelGGL(k_update_E_col<T>, dim3((g->Nz + 255) / 256), dim3(256), a.nmat * sizeof(MatL<T>),
+                           g->stream, a);
         return;
     }
     const int nb = tile_geometry(a, cfg);
@@ -452,7 +462,29 @@ void dispatch_H(aefdtd_grid *g, HArgs<T> &a, int cfg) {
 }
 
 template <typename T>
+int launch_rc(aefdtd_grid *g) {
+    if (g->n_disp == 0) return AEFDTD_OK;
+    RcArgs<T> r;
+    r.ndisp = (int)g->n_disp;
+    r.cells = g->d_cells;
+    r.mat = g->d_mat;
+    r.mt = (const MatDev<T> *)g->d_mtab;
+    r.poles = g->d_poles;
+    r.psi = g->d_psi;
+    r.Ex = (const T *)g->F[0]; r.Ey = (const T *)g->F[1]; r.Ez = (const T *)g->F[2];
+    r.corr = (T *)g->d_corr;
+    hipEvent_t ev;
+    prof_begin(g, 4, &ev);
+    hipLaunchKernelGGL(k_rc_ante<T>, dim3((unsigned)((g->n_disp + 255) / 256)), dim3(256), 0, g->stream, r);
+    prof_end(g, ev);
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+template <typename T>
 int launch_E(aefdtd_grid *g) {
+    int rc = launch_rc<T>(g);
+    if (rc) return rc;
     EArgs<T> a = e_args<T>(g);
     hipEvent_t ev;
     prof_begin(g, 0, &ev);
@@ -741,6 +773,8 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree_t(g->d_poles);
     dfree_t(g->d_slot);
     dfree_t(g->d_psi);
+    dfree_t(g->d_cells);
+    dfree(g->d_corr);
     dfree(g->d_prof);
     for (int s = 0; s < 12; s++) dfree(g->d_psipml[s]);
     dfree(g->d_stage);
@@ -1082,7 +1116,7 @@ int aefdtd_profile_enable(aefdtd_grid *g, int on) {
     prof_collect(g);
     g->prof_on = on != 0;
     if (on) {
-        for (int k = 0; k < 4; k++) { g->prof_ms[k] = 0; g->prof_n[k] = 0; }
+        for (int k = 0; k < 5; k++) { g->prof_ms[k] = 0; g->prof_n[k] = 0; }
     }
     return AEFDTD_OK;
 }
@@ -1095,6 +1129,7 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
     else if (!std::strcmp(kernel, "H")) k = 1;
     else if (!std::strcmp(kernel, "sensor")) k = 2;
     else if (!std::strcmp(kernel, "dft")) k = 3;
+    else if (!std::strcmp(kernel, "rc")) k = 4;
     if (k < 0) return fail(g, AEFDTD_EINVAL, "unknown kernel name");
     if (total_ms) *total_ms = g->prof_ms[k];
     if (launches) *launches = g->prof_n[k];

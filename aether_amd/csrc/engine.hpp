@@ -78,6 +78,8 @@ struct aefdtd_grid {
     void *d_mtab = nullptr;           // per-material coefficient table (MatDev<T>)
     double2 *d_poles = nullptr;       // per pole: Crec, Dchi
     int32_t *d_slot = nullptr;        // dispersive slot per cell, -1 elsewhere
+    int32_t *d_cells = nullptr;       // cell of each dispersive slot
+    void *d_corr = nullptr;           // per slot: RC increment of the current E update (T[3])
     double2 *d_psi = nullptr;         // RC accumulators [3*Npmax][Ndisp]
     long n_disp = 0;
     int np_max = 0;
@@ -109,8 +111,8 @@ struct aefdtd_grid {
     bool prof_on = false;
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> events;
-    double prof_ms[4] = {0, 0, 0, 0};
-    long prof_n[4] = {0, 0, 0, 0};
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    long prof_n[5] = {0, 0, 0, 0, 0};
 
     size_t ncell() const { return (size_t)Nx * Ny * Nz; }
     size_t tsize() const { return precision == AEFDTD_FP64 ? 8 : 4; }

@@ -56,8 +56,7 @@ struct EArgs {
     const MatDev<T> *mt;
     const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
     const int32_t *slot;
-    double2 *psi;
-    int ndisp;
+    const T *corr;              // per dispersive slot: (T)(C3 sum_p Re psi) for x, y, z
     Prof<T> pf;
     int pml[6];
     T *pEyx, *pEzx, *pExy, *pEzy, *pExz, *pEyz;   // slab psi, x:(pxm+pxp,Ny,Nz) y:(Nx,pym+pyp,Nz) z:(Nx,Ny,pzm+pzp)
@@ -113,30 +112,6 @@ template <typename T>
 __device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
                                        T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
                                        T ikx, T iky, int nx_, int ny_) {
-    // RC-ante: psi = Crec psi + Dchi E^n (core/mats_RC.cpp:21-36), FP64
-    double sx = 0, sy = 0, sz = 0;
-    double C3 = 0;
-    if (M.rc) {
-        const MatDev<T> &G = a.mt[m];
-        C3 = G.C3;
-        const int s = a.slot[id];
-        const double e3[3] = {(double)ex0, (double)ey0, (double)ez0};
-        for (int p = 0; p < G.np; p++) {
-            const double2 cr = a.poles[2 * (G.pole0 + p)], dc = a.poles[2 * (G.pole0 + p) + 1];
-            double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
-            double acc[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const double2 v = q[(size_t)c * a.ndisp];
-                double2 r;
-                r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
-                r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
-                q[(size_t)c * a.ndisp] = r;
-                acc[c] = r.x;
-            }
-            sx += acc[0]; sy += acc[1]; sz += acc[2];
-        }
-    }
     const T ikz = a.pf.ikE[2][k];
     // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
     T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
@@ -158,11 +133,13 @@ __device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int 
             if (i == a.tb1) { ey -= M.C2x * a.aHz[k]; ez += M.C2x * a.aHy[k]; }
         }
     }
-    // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50)
+    // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50); the sum was
+    // formed from E^n by the RC pre-pass (k_rc_ante) of this step
     if (M.rc) {
-        ex += (T)(C3 * sx);
-        ey += (T)(C3 * sy);
-        ez += (T)(C3 * sz);
+        const T *cr = a.corr + 3 * a.slot[id];
+        ex += cr[0];
+        ey += cr[1];
+        ez += cr[2];
     }
     // CPML-E + PEC walls (core/fdtd_pml.cpp:504-735)
     const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
@@ -249,6 +226,48 @@ __device__ __forceinline__ int slabH(int v, int N, int pm, int pp) {
     return -1;
 }
 
+// RC pre-pass over the dispersive cells (slot order): psi = Crec psi + Dchi E^n
+// (core/mats_RC.cpp:21-36) in FP64, then the E increment (T)(C3 sum_p Re psi)
+// of RC_apply_E (:38-50), which the E pass adds after its stencil and TFSF.
+template <typename T>
+struct RcArgs {
+    int ndisp;
+    const int32_t *cells;
+    const uint8_t *mat;
+    const MatDev<T> *mt;
+    const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
+    double2 *psi;               // [3 * pole + component][slot]
+    const T *Ex, *Ey, *Ez;
+    T *corr;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<T> a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.ndisp) return;
+    const int id = a.cells[s];
+    const MatDev<T> &G = a.mt[a.mat[id]];
+    const double e3[3] = {(double)a.Ex[id], (double)a.Ey[id], (double)a.Ez[id]};
+    double sum[3] = {0, 0, 0};
+    for (int p = 0; p < G.np; p++) {
+        const double2 cr = a.poles[2 * (G.pole0 + p)], dc = a.poles[2 * (G.pole0 + p) + 1];
+        double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
+        double acc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double2 v = q[(size_t)c * a.ndisp];
+            double2 r;
+            r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
+            r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
+            q[(size_t)c * a.ndisp] = r;
+            acc[c] = r.x;
+        }
+        sum[0] += acc[0]; sum[1] += acc[1]; sum[2] += acc[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) a.corr[3 * s + c] = (T)(G.C3 * sum[c]);
+}
+
 template <typename T>
 __device__ __forceinline__ void stage_mats(const EArgs<T> &a, MatL<T> *sm) {
     for (int m = threadIdx.x + blockDim.x * threadIdx.y; m < a.nmat; m += blockDim.x * blockDim.y) {
@@ -261,7 +280,8 @@ __device__ __forceinline__ void stage_mats(const EArgs<T> &a, MatL<T> *sm) {
 // 3D E pass.
 template <typename T, int BY, int KZ>
 __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
-    __shared__ MatL<T> sm[MAX_MATS];
+    extern __shared__ char lds_raw[];          // nmat entries, sized at launch
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
     stage_mats(a, sm);
     int bx, by, bz;
     tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
@@ -318,7 +338,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
 // one lane per cell.
 template <typename T>
 __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<T> a) {
-    __shared__ MatL<T> sm[MAX_MATS];
+    extern __shared__ char lds_raw[];
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
     stage_mats(a, sm);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.Nz) return;

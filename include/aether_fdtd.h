@@ -266,7 +266,8 @@ int aefdtd_synchronize(aefdtd_grid *g);
 
 /* ---- introspection used by the benchmark -------------------------------- */
 /* Per-kernel device time accumulated on the handle's stream with HIP events
- * while profiling is enabled: names "E","H","sensor","dft".  Returns count. */
+ * while profiling is enabled: names "E","H","sensor","dft","rc" (the RC pole
+ * pre-pass of the E update). */
 int aefdtd_profile_enable(aefdtd_grid *g, int on);
 int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches);
 /* Tile autotuner: times every compiled tile shape of the E and H passes on
