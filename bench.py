@@ -34,10 +34,10 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 
 
 def e_pass_bytes(lay, pml, stats):
-    """Algorithmic HBM bytes of one E-pass launch: read E,H (6 values) and the
-    uint8 material id, write E (3 values); CPML slab cells read+write 2 psi per
-    PML axis (E ranges: 1..pm-1 and N-pp+1..N-1); dispersive cells read+write
-    3 complex FP64 accumulators per pole and read a 4-byte slot index."""
+    """Algorithmic HBM bytes of one E-pass launch (k_update_E): read E,H (6
+    values) and the uint8 material id, write E (3 values); CPML slab cells
+    read+write 2 psi per PML axis (E ranges: 1..pm-1 and N-pp+1..N-1);
+    dispersive cells read a 4-byte slot index and the 3-value RC increment."""
     ncell, ndisp, npmax, s = stats
     Nx, Ny, Nz = lay[:3]
     b = ncell * (9 * s + 1)
@@ -47,8 +47,15 @@ def e_pass_bytes(lay, pml, stats):
         if pm or pp:
             planes = max(0, pm - 1) + max(0, pp - 1)
             b += planes * cross[a] * 2 * 2 * s
-    b += ndisp * (npmax * 96 + 4)
+    b += ndisp * (4 + 3 * s)
     return b
+
+
+def rc_pass_bytes(stats):
+    """RC pre-pass (k_rc_ante): per dispersive cell read cell index, material id
+    and E (3), read+write 3 complex FP64 accumulators per pole, write 3 values."""
+    ncell, ndisp, npmax, s = stats
+    return ndisp * (4 + 1 + 3 * s + npmax * 96 + 3 * s)
 
 
 def h_pass_bytes(lay, pml, stats):
@@ -139,6 +146,7 @@ def main():
     h_ms, h_n = run.profile_read("H")
     s_ms, s_n = run.profile_read("sensor")
     d_ms, d_n = run.profile_read("dft")
+    r_ms, r_n = run.profile_read("rc")
     run.profile(False)
     run.close()
 
@@ -148,7 +156,7 @@ def main():
     e_avg = e_ms / max(1, e_n) / 1e3
     h_avg = h_ms / max(1, h_n) / 1e3
     achieved = e_bytes / e_avg / 1e9
-    step_bytes = e_bytes + h_bytes
+    step_bytes = e_bytes + h_bytes + rc_pass_bytes(stats)
     rec = {
         "metric": "Mcell-updates/s (256^3 Drude+CP spheres, CPML, TFSF, 64-wavelength box DFT)",
         "value": round(value, 1),
@@ -170,7 +178,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_update_E", "bytes_per_launch": e_bytes, "avg_launch_ms": round(e_avg * 1e3, 5)},
         "kernels_ms_avg": {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5),
-                           "sensor": round(s_ms / max(1, s_n), 5), "dft": round(d_ms / max(1, d_n), 5)},
+                           "sensor": round(s_ms / max(1, s_n), 5), "dft": round(d_ms / max(1, d_n), 5),
+                           "rc": round(r_ms / max(1, r_n), 5)},
         "step_hbm_fraction": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
         "setup_s": round(setup_s, 2),
