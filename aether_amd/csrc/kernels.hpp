@@ -457,36 +457,20 @@ __global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<T> a) {
     const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
-    // software pipeline as in the E pass; E(k+1) of plane k is E(k) of plane k+1
-    struct Ld {
-        T hx, hy, hz, ex, ey, ez, ez_jp, ez_ip, ex_jp, ey_ip;
-    };
-    auto load = [&](int id) {
-        Ld v;
-        v.hx = a.Hx[id]; v.hy = a.Hy[id]; v.hz = a.Hz[id];
-        v.ex = a.Ex[id]; v.ey = a.Ey[id]; v.ez = a.Ez[id];
-        v.ez_jp = a.Ez[id + ojp]; v.ez_ip = a.Ez[id + oip];
-        v.ex_jp = a.Ex[id + ojp]; v.ey_ip = a.Ey[id + oip];
-        return v;
-    };
-    Ld cur = load(base + k0 * NxNy);
+    // E(k+1) of plane k is E(k) of plane k+1: carried in registers
+    T ex_k = a.Ex[base + k0 * NxNy];
+    T ey_k = a.Ey[base + k0 * NxNy];
     for (int k = k0; k < k1; k++) {
         const int id = base + k * NxNy;
         const int kp = (k == a.Nz - 1) ? 0 : k + 1;
-        Ld nxt;
-        T ex_kp, ey_kp;
-        if (k + 1 < k1) {
-            nxt = load(id + NxNy);
-            ex_kp = nxt.ex;
-            ey_kp = nxt.ey;
-        } else {
-            nxt = cur;
-            ex_kp = a.Ex[base + kp * NxNy];
-            ey_kp = a.Ey[base + kp * NxNy];
-        }
-        h_cell(a, i, j, k, id, cur.hx, cur.hy, cur.hz, ey_kp - cur.ey, ex_kp - cur.ex, cur.ez_jp - cur.ez,
-               cur.ez_ip - cur.ez, cur.ex_jp - cur.ex, cur.ey_ip - cur.ey, ikx, iky, nx_, ny_);
-        cur = nxt;
+        const T ex = ex_k, ey = ey_k;
+        const T ex_kp = a.Ex[base + kp * NxNy];
+        const T ey_kp = a.Ey[base + kp * NxNy];
+        ex_k = ex_kp;
+        ey_k = ey_kp;
+        const T ez = a.Ez[id];
+        h_cell(a, i, j, k, id, a.Hx[id], a.Hy[id], a.Hz[id], ey_kp - ey, ex_kp - ex, a.Ez[id + ojp] - ez,
+               a.Ez[id + oip] - ez, a.Ex[id + ojp] - ex, a.Ey[id + oip] - ey, ikx, iky, nx_, ny_);
     }
 }
 
