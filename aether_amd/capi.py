@@ -19,7 +19,7 @@ ENGINE_SO = os.path.join(HERE, "libaether_fdtd.so")
 
 OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED = 0, 1, 2, 3, 4, 5
 MODE_NORM, MODE_CUSTOM = 0, 5
-FP64, FP32 = 0, 1
+FP64, FP32, FP16 = 0, 1, 2
 NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM = range(6)
 EX, EY, EZ, HX, HY, HZ = range(6)
 
@@ -116,6 +116,7 @@ def _sig(lib, p):
         "device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
         "stats": (i, [vp, C.POINTER(C.c_long)]),
         "autotune": (i, [vp, i, _ip]),
+        "set_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
         "cancel": (None, []),
         # oracle only

@@ -88,7 +88,23 @@ void dfree_t(P *&p) {
 
 bool is_fp64(const aefdtd_grid *g) { return g->precision == AEFDTD_FP64; }
 
-// Host copy of a double array into a device buffer of T.
+// Calls f.template operator()<S, T>() with the storage and arithmetic types of
+// the grid: (double, double), (float, float) or (_Float16, float).
+template <class F>
+int with_types(const aefdtd_grid *g, F &&f) {
+    if (g->precision == AEFDTD_FP64) return f.template operator()<double, double>();
+    if (g->precision == AEFDTD_FP32) return f.template operator()<float, float>();
+    return f.template operator()<_Float16, float>();
+}
+
+// Calls f.template operator()<T>() with the arithmetic type of the grid.
+template <class F>
+int with_ctype(const aefdtd_grid *g, F &&f) {
+    if (g->precision == AEFDTD_FP64) return f.template operator()<double>();
+    return f.template operator()<float>();
+}
+
+// Host copy of a double array into a device buffer of the arithmetic type.
 int upload_T(aefdtd_grid *g, void *dst, const double *src, size_t n) {
     if (n == 0) return AEFDTD_OK;
     if (is_fp64(g)) {
@@ -146,7 +162,7 @@ int upload_profiles(aefdtd_grid *g) {
         o += N[a];
     }
     if (!g->d_prof) {
-        int rc = dalloc(g, &g->d_prof, tot * g->tsize());
+        int rc = dalloc(g, &g->d_prof, tot * g->csize());
         if (rc) return rc;
     }
     return upload_T(g, g->d_prof, h.data(), tot);
@@ -182,9 +198,9 @@ int alloc_pml_state(aefdtd_grid *g) {
     for (int s = 0; s < 12; s++) {
         size_t m = n[s / 4];
         if (!m) continue;
-        int rc = dalloc(g, &g->d_psipml[s], m * g->tsize());
+        int rc = dalloc(g, &g->d_psipml[s], m * g->csize());
         if (rc) return rc;
-        HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, m * g->tsize(), g->stream));
+        HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, m * g->csize(), g->stream));
         g->psipml_n[s] = m;
     }
     return AEFDTD_OK;
@@ -242,7 +258,7 @@ int upload_grid(aefdtd_grid *g) {
         g->d_mat = (uint8_t *)p;
     }
     HIPOK(g, hipMemcpyAsync(g->d_mat, m8.data(), N, hipMemcpyHostToDevice, g->stream));
-    int rc = is_fp64(g) ? upload_materials_T<double>(g) : upload_materials_T<float>(g);
+    int rc = with_ctype(g, [&]<typename T>() { return upload_materials_T<T>(g); });
     if (rc) return rc;
     long nd = 0;
     std::vector<int32_t> slot;
@@ -268,7 +284,7 @@ int upload_grid(aefdtd_grid *g) {
         if ((rc = dalloc(g, &p, cells.size() * sizeof(int32_t)))) return rc;
         g->d_cells = (int32_t *)p;
         HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
-        if ((rc = dalloc(g, &g->d_corr, (size_t)3 * nd * g->tsize()))) return rc;
+        if ((rc = dalloc(g, &g->d_corr, (size_t)3 * nd * g->csize()))) return rc;
         size_t np = (size_t)3 * g->np_max * nd;
         if ((rc = dalloc(g, &p, np * sizeof(double2)))) return rc;
         g->d_psi = (double2 *)p;
@@ -323,31 +339,31 @@ void prof_collect(aefdtd_grid *g) {
 constexpr int kBY[4] = {1, 2, 4, 8};
 constexpr int kKZ[4] = {4, 8, 16, 32};
 
-template <typename T, int BY, int KZ>
-void launch_E_cfg(const EArgs<T> &a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_update_E<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
+template <typename S, typename T, int BY, int KZ>
+void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
 }
-template <typename T, int BY, int KZ>
-void launch_H_cfg(const HArgs<T> &a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_update_H<T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+template <typename S, typename T, int BY, int KZ>
+void launch_H_cfg(const HArgs<S, T> &a, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_update_H<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
 }
 
-template <typename T, int BY>
-void launch_E_by(const EArgs<T> &a, int kzi, int nb, hipStream_t st) {
+template <typename S, typename T, int BY>
+void launch_E_by(const EArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
     switch (kzi) {
-        case 0: launch_E_cfg<T, BY, 4>(a, nb, st); break;
-        case 1: launch_E_cfg<T, BY, 8>(a, nb, st); break;
-        case 2: launch_E_cfg<T, BY, 16>(a, nb, st); break;
-        default: launch_E_cfg<T, BY, 32>(a, nb, st); break;
+        case 0: launch_E_cfg<S, T, BY, 4>(a, nb, st); break;
+        case 1: launch_E_cfg<S, T, BY, 8>(a, nb, st); break;
+        case 2: launch_E_cfg<S, T, BY, 16>(a, nb, st); break;
+        default: launch_E_cfg<S, T, BY, 32>(a, nb, st); break;
     }
 }
-template <typename T, int BY>
-void launch_H_by(const HArgs<T> &a, int kzi, int nb, hipStream_t st) {
+template <typename S, typename T, int BY>
+void launch_H_by(const HArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
     switch (kzi) {
-        case 0: launch_H_cfg<T, BY, 4>(a, nb, st); break;
-        case 1: launch_H_cfg<T, BY, 8>(a, nb, st); break;
-        case 2: launch_H_cfg<T, BY, 16>(a, nb, st); break;
-        default: launch_H_cfg<T, BY, 32>(a, nb, st); break;
+        case 0: launch_H_cfg<S, T, BY, 4>(a, nb, st); break;
+        case 1: launch_H_cfg<S, T, BY, 8>(a, nb, st); break;
+        case 2: launch_H_cfg<S, T, BY, 16>(a, nb, st); break;
+        default: launch_H_cfg<S, T, BY, 32>(a, nb, st); break;
     }
 }
 
@@ -367,17 +383,26 @@ bool is_column(const aefdtd_grid *g) {
     return g->Nx == 1 && g->Ny == 1 && !(g->pml[0] || g->pml[1] || g->pml[2] || g->pml[3]);
 }
 
-template <typename T>
-EArgs<T> e_args(aefdtd_grid *g) {
-    EArgs<T> a;
+// Field component c of grid g as seen by the kernels.
+template <typename S, typename T>
+Fld<S, T> fld(const aefdtd_grid *g, int c) {
+    Fld<S, T> f;
+    f.p = (S *)g->F[c];
+    f.ld = (T)std::ldexp(1.0, -g->scale_exp[c]);
+    f.st = (T)std::ldexp(1.0, g->scale_exp[c]);
+    return f;
+}
+
+template <typename S, typename T>
+EArgs<S, T> e_args(aefdtd_grid *g) {
+    EArgs<S, T> a;
     std::memset(&a, 0, sizeof(a));
     a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
     a.nmat = (int)g->Nmat;
-    a.Ex = (T *)g->F[0]; a.Ey = (T *)g->F[1]; a.Ez = (T *)g->F[2];
-    a.Hx = (const T *)g->F[3]; a.Hy = (const T *)g->F[4]; a.Hz = (const T *)g->F[5];
+    a.Ex = fld<S, T>(g, 0); a.Ey = fld<S, T>(g, 1); a.Ez = fld<S, T>(g, 2);
+    a.Hx = fld<S, T>(g, 3); a.Hy = fld<S, T>(g, 4); a.Hz = fld<S, T>(g, 5);
     a.mat = g->d_mat;
     a.mt = (const MatDev<T> *)g->d_mtab;
-    a.poles = g->d_poles;
     a.slot = g->d_slot;
     a.corr = (const T *)g->d_corr;
     a.pf = prof_of<T>(g);
@@ -396,18 +421,18 @@ EArgs<T> e_args(aefdtd_grid *g) {
         a.tfsf = 1;
         a.tb0 = g->tb[0]; a.tb1 = g->tb[1]; a.tb2 = g->tb[2]; a.tb3 = g->tb[3]; a.tb4 = g->tb[4]; a.tb5 = g->tb[5];
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
-        a.aHx = (const T *)g->tfsf_aux->F[3]; a.aHy = (const T *)g->tfsf_aux->F[4]; a.aHz = (const T *)g->tfsf_aux->F[5];
+        a.aHx = fld<S, T>(g->tfsf_aux, 3); a.aHy = fld<S, T>(g->tfsf_aux, 4); a.aHz = fld<S, T>(g->tfsf_aux, 5);
     }
     return a;
 }
 
-template <typename T>
-HArgs<T> h_args(aefdtd_grid *g) {
-    HArgs<T> a;
+template <typename S, typename T>
+HArgs<S, T> h_args(aefdtd_grid *g) {
+    HArgs<S, T> a;
     std::memset(&a, 0, sizeof(a));
     a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
-    a.Hx = (T *)g->F[3]; a.Hy = (T *)g->F[4]; a.Hz = (T *)g->F[5];
-    a.Ex = (const T *)g->F[0]; a.Ey = (const T *)g->F[1]; a.Ez = (const T *)g->F[2];
+    a.Ex = fld<S, T>(g, 0); a.Ey = fld<S, T>(g, 1); a.Ez = fld<S, T>(g, 2);
+    a.Hx = fld<S, T>(g, 3); a.Hy = fld<S, T>(g, 4); a.Hz = fld<S, T>(g, 5);
     a.pf = prof_of<T>(g);
     for (int f = 0; f < 6; f++) a.pml[f] = g->pml[f];
     a.pHyx = (T *)g->d_psipml[2]; a.pHzx = (T *)g->d_psipml[3];
@@ -425,97 +450,97 @@ HArgs<T> h_args(aefdtd_grid *g) {
         a.tfsf = 1;
         a.tb0 = g->tb[0]; a.tb1 = g->tb[1]; a.tb2 = g->tb[2]; a.tb3 = g->tb[3]; a.tb4 = g->tb[4]; a.tb5 = g->tb[5];
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
-        a.aEx = (const T *)g->tfsf_aux->F[0]; a.aEy = (const T *)g->tfsf_aux->F[1]; a.aEz = (const T *)g->tfsf_aux->F[2];
+        a.aEx = fld<S, T>(g->tfsf_aux, 0); a.aEy = fld<S, T>(g->tfsf_aux, 1); a.aEz = fld<S, T>(g->tfsf_aux, 2);
     }
     return a;
 }
 
-template <typename T>
-void dispatch_E(aefdtd_grid *g, EArgs<T> &a, int cfg) {
+template <typename S, typename T>
+void dispatch_E(aefdtd_grid *g, EArgs<S, T> &a, int cfg) {
     if (is_column(g)) {
-        hipLaunchKernelGGL(k_update_E_col<T>, dim3((g->Nz + 255) / 256), dim3(256), a.nmat * sizeof(MatL<T>),
+        hipLaunchKernelGGL((k_update_E_col<S, T>), dim3((g->Nz + 255) / 256), dim3(256), a.nmat * sizeof(MatL<T>),
                            g->stream, a);
         return;
     }
     const int nb = tile_geometry(a, cfg);
     switch (cfg & 3) {
-        case 0: launch_E_by<T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 1: launch_E_by<T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 2: launch_E_by<T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        default: launch_E_by<T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 0: launch_E_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 1: launch_E_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 2: launch_E_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        default: launch_E_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
     }
 }
 
-template <typename T>
-void dispatch_H(aefdtd_grid *g, HArgs<T> &a, int cfg) {
+template <typename S, typename T>
+void dispatch_H(aefdtd_grid *g, HArgs<S, T> &a, int cfg) {
     if (is_column(g)) {
-        hipLaunchKernelGGL(k_update_H_col<T>, dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
+        hipLaunchKernelGGL((k_update_H_col<S, T>), dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
         return;
     }
     const int nb = tile_geometry(a, cfg);
     switch (cfg & 3) {
-        case 0: launch_H_by<T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 1: launch_H_by<T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 2: launch_H_by<T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        default: launch_H_by<T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 0: launch_H_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 1: launch_H_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 2: launch_H_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        default: launch_H_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
     }
 }
 
-template <typename T>
+template <typename S, typename T>
 int launch_rc(aefdtd_grid *g) {
     if (g->n_disp == 0) return AEFDTD_OK;
-    RcArgs<T> r;
+    RcArgs<S, T> r;
     r.ndisp = (int)g->n_disp;
     r.cells = g->d_cells;
     r.mat = g->d_mat;
     r.mt = (const MatDev<T> *)g->d_mtab;
     r.poles = g->d_poles;
     r.psi = g->d_psi;
-    r.Ex = (const T *)g->F[0]; r.Ey = (const T *)g->F[1]; r.Ez = (const T *)g->F[2];
+    r.Ex = fld<S, T>(g, 0); r.Ey = fld<S, T>(g, 1); r.Ez = fld<S, T>(g, 2);
     r.corr = (T *)g->d_corr;
     hipEvent_t ev;
     prof_begin(g, 4, &ev);
-    hipLaunchKernelGGL(k_rc_ante<T>, dim3((unsigned)((g->n_disp + 255) / 256)), dim3(256), 0, g->stream, r);
+    hipLaunchKernelGGL((k_rc_ante<S, T>), dim3((unsigned)((g->n_disp + 255) / 256)), dim3(256), 0, g->stream, r);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;
 }
 
-template <typename T>
+template <typename S, typename T>
 int launch_E(aefdtd_grid *g) {
-    int rc = launch_rc<T>(g);
+    int rc = launch_rc<S, T>(g);
     if (rc) return rc;
-    EArgs<T> a = e_args<T>(g);
+    EArgs<S, T> a = e_args<S, T>(g);
     hipEvent_t ev;
     prof_begin(g, 0, &ev);
-    dispatch_E<T>(g, a, g->tile_e);
+    dispatch_E<S, T>(g, a, g->tile_e);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;
 }
 
-template <typename T>
+template <typename S, typename T>
 int launch_H(aefdtd_grid *g) {
-    HArgs<T> a = h_args<T>(g);
+    HArgs<S, T> a = h_args<S, T>(g);
     hipEvent_t ev;
     prof_begin(g, 1, &ev);
-    dispatch_H<T>(g, a, g->tile_h);
+    dispatch_H<S, T>(g, a, g->tile_h);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;
 }
 
 // Times one tile configuration of the E or H pass: reps launches between events.
-template <typename T>
+template <typename S, typename T>
 float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hipEvent_t eb) {
     (void)hipEventRecord(ea, g->stream);
     for (int r = 0; r < reps; r++) {
         if (e_pass) {
-            EArgs<T> a = e_args<T>(g);
-            dispatch_E<T>(g, a, cfg);
+            EArgs<S, T> a = e_args<S, T>(g);
+            dispatch_E<S, T>(g, a, cfg);
         } else {
-            HArgs<T> a = h_args<T>(g);
-            dispatch_H<T>(g, a, cfg);
+            HArgs<S, T> a = h_args<S, T>(g);
+            dispatch_H<S, T>(g, a, cfg);
         }
     }
     (void)hipEventRecord(eb, g->stream);
@@ -528,12 +553,12 @@ float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hi
 // Project autotuner: runs every compiled (BY, KZ, remap) tile of the E and H
 // passes on the grid itself, keeps the fastest, and restores the grid's state
 // (fields, CPML and RC accumulators) from a device snapshot afterwards.
-template <typename T>
+template <typename S, typename T>
 int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     std::vector<std::pair<void *, size_t>> state;
-    for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->ncell() * g->tsize()});
+    for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->ncell() * g->ssize()});
     for (int s = 0; s < 12; s++)
-        if (g->d_psipml[s]) state.push_back({g->d_psipml[s], g->psipml_n[s] * g->tsize()});
+        if (g->d_psipml[s]) state.push_back({g->d_psipml[s], g->psipml_n[s] * g->csize()});
     if (g->d_psi) state.push_back({g->d_psi, (size_t)3 * g->np_max * g->n_disp * sizeof(double2)});
     std::vector<void *> snap(state.size(), nullptr);
     for (size_t q = 0; q < state.size(); q++) {
@@ -551,8 +576,8 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     for (int pass = 0; pass < 2; pass++) {
         float bt = 1e30f;
         for (int cfg = 0; cfg < 32; cfg++) {
-            time_cfg<T>(g, pass == 0, cfg, 1, ea, eb);          // warm the instruction cache
-            float ms = time_cfg<T>(g, pass == 0, cfg, reps, ea, eb);
+            time_cfg<S, T>(g, pass == 0, cfg, 1, ea, eb);          // warm the instruction cache
+            float ms = time_cfg<S, T>(g, pass == 0, cfg, reps, ea, eb);
             if (ms < bt) { bt = ms; best[pass] = cfg; }
         }
     }
@@ -629,7 +654,7 @@ int flush_dft(aefdtd_grid *g) {
     return AEFDTD_OK;
 }
 
-template <typename T>
+template <typename S, typename T>
 int feed_T(aefdtd_grid *g) {
     if (!g->sensors.empty()) {
         int rc;
@@ -651,9 +676,9 @@ int feed_T(aefdtd_grid *g) {
         dim3 grid((unsigned)((g->stage_maxP + 255) / 256), (unsigned)g->sensors.size(), 1);
         hipEvent_t ev;
         prof_begin(g, 2, &ev);
-        hipLaunchKernelGGL(k_sample<T>, grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc, g->stage_fill,
-                           g->stage_P, (const T *)g->F[0], (const T *)g->F[1], (const T *)g->F[2], (const T *)g->F[3],
-                           (const T *)g->F[4], (const T *)g->F[5], g->Nx, g->Ny, g->Nz, (T *)g->d_stage);
+        hipLaunchKernelGGL((k_sample<S, T>), grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc,
+                           g->stage_fill, g->stage_P, fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2),
+                           fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, (T *)g->d_stage);
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
         g->stage_fill++;
@@ -665,8 +690,8 @@ int feed_T(aefdtd_grid *g) {
     for (PlaneSum *s : g->sums) {
         int t = g->tstep - 1;
         if (t < 0 || t >= g->Nt) continue;
-        hipLaunchKernelGGL(k_plane_sum<T>, dim3(1), dim3(256), 0, g->stream, (const T *)g->F[0], (const T *)g->F[1],
-                           (const T *)g->F[2], g->Nx, g->Ny, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
+        hipLaunchKernelGGL((k_plane_sum<S, T>), dim3(PS_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0), fld<S, T>(g, 1),
+                           fld<S, T>(g, 2), g->Nx, g->Ny, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
                            g->Dy, g->Dz, s->zpos, (const double *)s->d_tap, t, g->Dt, s->d_out);
         HIPOK(g, hipGetLastError());
         s->n = std::max(s->n, t + 1);
@@ -695,7 +720,8 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
     *out = nullptr;
     if (Nx < 1 || Ny < 1 || Nz < 1 || Nt < 0) return cfail(AEFDTD_EINVAL, "bad grid dimensions");
     if (!(Dx > 0) || !(Dy > 0) || !(Dz > 0) || !(Dt > 0)) return cfail(AEFDTD_EINVAL, "bad discretisation");
-    if (precision != AEFDTD_FP64 && precision != AEFDTD_FP32) return cfail(AEFDTD_EINVAL, "bad precision");
+    if (precision != AEFDTD_FP64 && precision != AEFDTD_FP32 && precision != AEFDTD_FP16)
+        return cfail(AEFDTD_EINVAL, "bad precision");
     if (mode != AEFDTD_MODE_NORM && mode != AEFDTD_MODE_CUSTOM) return cfail(AEFDTD_EINVAL, "unsupported grid mode");
     for (int f = 0; f < 6; f++)
         if (pml[f] < 0 || pad[f] < 0) return cfail(AEFDTD_EINVAL, "negative pml/pad");
@@ -712,6 +738,8 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
         return cfail(AEFDTD_EDEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
     auto *g = new aefdtd_grid();
     g->precision = precision;
+    if (precision == AEFDTD_FP16)
+        for (int c = 0; c < 6; c++) g->scale_exp[c] = c < 3 ? 8 : 16;
     g->mode = mode;
     g->Nt = Nt;
     g->Dx = Dx; g->Dy = Dy; g->Dz = Dz; g->Dt = Dt;
@@ -748,7 +776,7 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
         return cfail(AEFDTD_EDEVICE, "hipStreamCreate failed");
     }
     for (int c = 0; c < 6; c++) {
-        if (dalloc(g, &g->F[c], N * g->tsize()) || hipMemsetAsync(g->F[c], 0, N * g->tsize(), g->stream) != hipSuccess) {
+        if (dalloc(g, &g->F[c], N * g->ssize()) || hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream) != hipSuccess) {
             std::string e = g->err.empty() ? "field allocation failed" : g->err;
             aefdtd_destroy(g);
             return cfail(AEFDTD_ENOMEM, e);
@@ -893,14 +921,14 @@ int aefdtd_update_E(aefdtd_grid *g) {
         g->pml_dirty = false;
     }
     if (g->tfsf_aux && !g->tfsf_aux->ready) return fail(g, AEFDTD_ESTATE, "TFSF auxiliary grid not set up");
-    return is_fp64(g) ? launch_E<double>(g) : launch_E<float>(g);
+    return with_types(g, [&]<typename S, typename T>() { return launch_E<S, T>(g); });
 }
 
 int aefdtd_update_H(aefdtd_grid *g) {
     if (!g) return AEFDTD_EINVAL;
     int rc = ensure_ready(g);
     if (rc) return rc;
-    rc = is_fp64(g) ? launch_H<double>(g) : launch_H<float>(g);
+    rc = with_types(g, [&]<typename S, typename T>() { return launch_H<S, T>(g); });
     if (rc) return rc;
     g->tstep += 1;
     return AEFDTD_OK;
@@ -909,9 +937,9 @@ int aefdtd_update_H(aefdtd_grid *g) {
 int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;
     size_t N = g->ncell();
-    for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->tsize(), g->stream));
+    for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream));
     for (int s = 0; s < 12; s++)
-        if (g->d_psipml[s]) HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, g->psipml_n[s] * g->tsize(), g->stream));
+        if (g->d_psipml[s]) HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, g->psipml_n[s] * g->csize(), g->stream));
     if (g->d_psi) HIPOK(g, hipMemsetAsync(g->d_psi, 0, (size_t)3 * g->np_max * g->n_disp * sizeof(double2), g->stream));
     HIPOK(g, hipStreamSynchronize(g->stream));
     return AEFDTD_OK;
@@ -922,12 +950,17 @@ int aefdtd_read_field(aefdtd_grid *g, int f, double *out) {
     if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
     size_t N = g->ncell();
     HIPOK(g, hipStreamSynchronize(g->stream));
-    if (is_fp64(g)) {
+    if (g->precision == AEFDTD_FP64) {
         HIPOK(g, hipMemcpy(out, g->F[f], N * 8, hipMemcpyDeviceToHost));
-    } else {
+    } else if (g->precision == AEFDTD_FP32) {
         std::vector<float> t(N);
         HIPOK(g, hipMemcpy(t.data(), g->F[f], N * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < N; i++) out[i] = t[i];
+    } else {
+        std::vector<_Float16> t(N);
+        HIPOK(g, hipMemcpy(t.data(), g->F[f], N * 2, hipMemcpyDeviceToHost));
+        const double inv = std::ldexp(1.0, -g->scale_exp[f]);
+        for (size_t i = 0; i < N; i++) out[i] = (double)(float)t[i] * inv;
     }
     return AEFDTD_OK;
 }
@@ -936,9 +969,27 @@ int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
     if (!g || !in) return AEFDTD_EINVAL;
     if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
     HIPOK(g, hipStreamSynchronize(g->stream));
+    if (g->precision == AEFDTD_FP16) {
+        size_t N = g->ncell();
+        std::vector<_Float16> t(N);
+        const double sc = std::ldexp(1.0, g->scale_exp[f]);
+        for (size_t i = 0; i < N; i++) t[i] = (_Float16)(float)(in[i] * sc);
+        HIPOK(g, hipMemcpy(g->F[f], t.data(), N * 2, hipMemcpyHostToDevice));
+        return AEFDTD_OK;
+    }
     int rc = upload_T(g, g->F[f], in, g->ncell());
     if (rc) return rc;
     HIPOK(g, hipStreamSynchronize(g->stream));
+    return AEFDTD_OK;
+}
+
+int aefdtd_set_storage_scale(aefdtd_grid *g, const int exps[6]) {
+    if (!g || !exps) return AEFDTD_EINVAL;
+    if (g->precision != AEFDTD_FP16) return fail(g, AEFDTD_EINVAL, "storage scales apply to FP16 storage only");
+    if (g->tstep != 0) return fail(g, AEFDTD_ESTATE, "storage scales must be set before the first update");
+    for (int c = 0; c < 6; c++)
+        if (exps[c] < -24 || exps[c] > 24) return fail(g, AEFDTD_EINVAL, "storage scale exponent out of range");
+    for (int c = 0; c < 6; c++) g->scale_exp[c] = exps[c];
     return AEFDTD_OK;
 }
 
@@ -1013,13 +1064,13 @@ int aefdtd_feed_sensors(aefdtd_grid *g) {
     if (!g) return AEFDTD_EINVAL;
     int rc = ensure_ready(g);
     if (rc) return rc;
-    return is_fp64(g) ? feed_T<double>(g) : feed_T<float>(g);
+    return with_types(g, [&]<typename S, typename T>() { return feed_T<S, T>(g); });
 }
 
 int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
     if (!g || !rsp) return AEFDTD_EINVAL;
     if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");
-    int rc = is_fp64(g) ? flush_dft<double>(g) : flush_dft<float>(g);
+    int rc = with_ctype(g, [&]<typename T>() { return flush_dft<T>(g); });
     if (rc) return rc;
     Sensor &s = *g->sensors[id];
     double f;
@@ -1042,7 +1093,7 @@ int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
 int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]) {
     if (!g || !dims) return AEFDTD_EINVAL;
     if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");
-    int rc = is_fp64(g) ? flush_dft<double>(g) : flush_dft<float>(g);
+    int rc = with_ctype(g, [&]<typename T>() { return flush_dft<T>(g); });
     if (rc) return rc;
     Sensor &s = *g->sensors[id];
     dims[0] = s.span1; dims[1] = s.span2; dims[2] = s.Nl;
@@ -1069,10 +1120,10 @@ int aefdtd_add_plane_sum(aefdtd_grid *g, int k, const aefdtd_pulse *inc, double 
     int rc = dalloc(g, &p, g->Nt * sizeof(double));
     if (rc) { delete s; return rc; }
     s->d_tap = (double *)p;
-    if ((rc = dalloc(g, &p, (size_t)g->Nt * 6 * sizeof(double)))) { dfree_t(s->d_tap); delete s; return rc; }
+    if ((rc = dalloc(g, &p, (size_t)g->Nt * PS_BLOCKS * 6 * sizeof(double)))) { dfree_t(s->d_tap); delete s; return rc; }
     s->d_out = (double *)p;
     HIPOK(g, hipMemcpy(s->d_tap, tap, g->Nt * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(g, hipMemset(s->d_out, 0, (size_t)g->Nt * 6 * sizeof(double)));
+    HIPOK(g, hipMemset(s->d_out, 0, (size_t)g->Nt * PS_BLOCKS * 6 * sizeof(double)));
     g->sums.push_back(s);
     *id = (int)g->sums.size() - 1;
     return AEFDTD_OK;
@@ -1084,7 +1135,14 @@ int aefdtd_read_plane_sum(aefdtd_grid *g, int id, int n, double *out) {
     PlaneSum &s = *g->sums[id];
     int m = std::min(n, s.cap);
     HIPOK(g, hipStreamSynchronize(g->stream));
-    HIPOK(g, hipMemcpy(out, s.d_out, (size_t)m * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> part((size_t)m * PS_BLOCKS * 6);
+    HIPOK(g, hipMemcpy(part.data(), s.d_out, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int t = 0; t < m; t++)
+        for (int c = 0; c < 6; c++) {
+            double v = 0;
+            for (int b = 0; b < PS_BLOCKS; b++) v += part[((size_t)t * PS_BLOCKS + b) * 6 + c];
+            out[(size_t)t * 6 + c] = v;
+        }
     return AEFDTD_OK;
 }
 
@@ -1100,7 +1158,7 @@ int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]) {
         if (out) { out[0] = g->tile_e; out[1] = g->tile_h; }
         return AEFDTD_OK;
     }
-    return is_fp64(g) ? autotune_T<double>(g, std::max(1, reps), out) : autotune_T<float>(g, std::max(1, reps), out);
+    return with_types(g, [&]<typename S, typename T>() { return autotune_T<S, T>(g, std::max(1, reps), out); });
 }
 
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h) {
@@ -1141,7 +1199,7 @@ int aefdtd_stats(const aefdtd_grid *g, long out[4]) {
     out[0] = (long)g->ncell();
     out[1] = g->n_disp;
     out[2] = g->np_max;
-    out[3] = (long)g->tsize();
+    out[3] = (long)g->ssize();
     return AEFDTD_OK;
 }
 

@@ -114,6 +114,10 @@ struct aefdtd_grid {
     double prof_ms[5] = {0, 0, 0, 0, 0};
     long prof_n[5] = {0, 0, 0, 0, 0};
 
+    // power-of-two storage scale per field component (FP16 storage only)
+    int scale_exp[6] = {0, 0, 0, 0, 0, 0};
+
     size_t ncell() const { return (size_t)Nx * Ny * Nz; }
-    size_t tsize() const { return precision == AEFDTD_FP64 ? 8 : 4; }
+    size_t ssize() const { return precision == AEFDTD_FP64 ? 8 : (precision == AEFDTD_FP32 ? 4 : 2); }   // field storage
+    size_t csize() const { return precision == AEFDTD_FP64 ? 8 : 4; }   // arithmetic, CPML psi, tables
 };

@@ -22,6 +22,8 @@
 
 #include "engine.hpp"
 
+#include <type_traits>
+
 namespace aefdtd {
 namespace dev {
 
@@ -29,6 +31,24 @@ constexpr int BX = 64;
 constexpr int KS = 64;             // DFT staging depth (steps per accumulate)
 constexpr int DFT_LCH = 32;        // wavelengths per LDS chunk in the DFT pass
 constexpr int MAX_MATS = 256;
+
+// A field array in storage type S read and written in compute type T.  With
+// S == T this is a plain array; with FP16 storage each component carries a
+// power-of-two scale (stored = value * 2^e) so that E (~1) and H (~E/377) both
+// sit well inside the FP16 range.
+template <typename S, typename T>
+struct Fld {
+    S *p;
+    T ld, st;        // 2^-e and 2^e
+    __device__ __forceinline__ T operator[](int i) const {
+        if constexpr (std::is_same_v<S, T>) return p[i];
+        else return (T)p[i] * ld;
+    }
+    __device__ __forceinline__ void set(int i, T v) const {
+        if constexpr (std::is_same_v<S, T>) p[i] = v;
+        else p[i] = (S)(v * st);
+    }
+};
 
 // Per-material constants (FDTD_Material, core/fdtd_material.h:46-47).
 template <typename T>
@@ -46,12 +66,11 @@ struct Prof {
     const T *ikE[3], *ikH[3], *bE[3], *cE[3], *bH[3], *cH[3];
 };
 
-template <typename T>
+template <typename S, typename T>
 struct EArgs {
     int Nx, Ny, Nz, nmat;
     int nbx, nby, nbz, remap;
-    T *Ex, *Ey, *Ez;
-    const T *Hx, *Hy, *Hz;
+    Fld<S, T> Ex, Ey, Ez, Hx, Hy, Hz;
     const uint8_t *mat;
     const MatDev<T> *mt;
     const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
@@ -64,15 +83,14 @@ struct EArgs {
     int inj_k;
     T injx, injy;
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
-    const T *aHx, *aHy, *aHz;  // aux 1D grid H columns
+    Fld<S, T> aHx, aHy, aHz;   // aux 1D grid H columns
 };
 
-template <typename T>
+template <typename S, typename T>
 struct HArgs {
     int Nx, Ny, Nz;
     int nbx, nby, nbz, remap;
-    T *Hx, *Hy, *Hz;
-    const T *Ex, *Ey, *Ez;
+    Fld<S, T> Hx, Hy, Hz, Ex, Ey, Ez;
     Prof<T> pf;
     int pml[6];
     T *pHyx, *pHzx, *pHxy, *pHzy, *pHxz, *pHyz;
@@ -81,7 +99,7 @@ struct HArgs {
     int inj_k;
     T injx, injy;
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
-    const T *aEx, *aEy, *aEz;
+    Fld<S, T> aEx, aEy, aEz;
 };
 
 // Hardware workgroup id -> logical tile (bx, by, bz).  Workgroups are dealt to
@@ -108,8 +126,8 @@ struct MatL {
 
 // ---------------------------------------------------------------- E cell
 // Everything after the loads of one E cell; dH* are the curl differences.
-template <typename T>
-__device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
+template <typename S, typename T>
+__device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
                                        T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
                                        T ikx, T iky, int nx_, int ny_) {
     const T ikz = a.pf.ikE[2][k];
@@ -206,9 +224,9 @@ __device__ __forceinline__ void e_cell(const EArgs<T> &a, const MatL<T> &M, int 
     }
     // plane injection after the whole E update (modes/normal_incidence.cpp:338-342)
     if (k == a.inj_k) { ex += a.injx; ey += a.injy; }
-    a.Ex[id] = ex;
-    a.Ey[id] = ey;
-    a.Ez[id] = ez;
+    a.Ex.set(id, ex);
+    a.Ey.set(id, ey);
+    a.Ez.set(id, ez);
 }
 
 // CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
@@ -229,7 +247,7 @@ __device__ __forceinline__ int slabH(int v, int N, int pm, int pp) {
 // RC pre-pass over the dispersive cells (slot order): psi = Crec psi + Dchi E^n
 // (core/mats_RC.cpp:21-36) in FP64, then the E increment (T)(C3 sum_p Re psi)
 // of RC_apply_E (:38-50), which the E pass adds after its stencil and TFSF.
-template <typename T>
+template <typename S, typename T>
 struct RcArgs {
     int ndisp;
     const int32_t *cells;
@@ -237,12 +255,12 @@ struct RcArgs {
     const MatDev<T> *mt;
     const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
     double2 *psi;               // [3 * pole + component][slot]
-    const T *Ex, *Ey, *Ez;
+    Fld<S, T> Ex, Ey, Ez;
     T *corr;
 };
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<T> a) {
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<S, T> a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.ndisp) return;
     const int id = a.cells[s];
@@ -268,8 +286,8 @@ __global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<T> a) {
     for (int c = 0; c < 3; c++) a.corr[3 * s + c] = (T)(G.C3 * sum[c]);
 }
 
-template <typename T>
-__device__ __forceinline__ void stage_mats(const EArgs<T> &a, MatL<T> *sm) {
+template <typename S, typename T>
+__device__ __forceinline__ void stage_mats(const EArgs<S, T> &a, MatL<T> *sm) {
     for (int m = threadIdx.x + blockDim.x * threadIdx.y; m < a.nmat; m += blockDim.x * blockDim.y) {
         const MatDev<T> &G = a.mt[m];
         sm[m] = MatL<T>{G.C1, G.C2x, G.C2y, G.C2z, G.C4, G.rc};
@@ -278,8 +296,8 @@ __device__ __forceinline__ void stage_mats(const EArgs<T> &a, MatL<T> *sm) {
 }
 
 // 3D E pass.
-template <typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
     extern __shared__ char lds_raw[];          // nmat entries, sized at launch
     MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
     stage_mats(a, sm);
@@ -336,8 +354,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<T> a) {
 
 // E pass of a 1 x 1 x Nz column (the auxiliary grid of the TFSF source):
 // one lane per cell.
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_E_col(const EArgs<T> a) {
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
     extern __shared__ char lds_raw[];
     MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
     stage_mats(a, sm);
@@ -352,8 +370,8 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<T> a) {
 }
 
 // ---------------------------------------------------------------- H cell
-template <typename T>
-__device__ __forceinline__ void h_cell(const HArgs<T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
+template <typename S, typename T>
+__device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
                                        T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
     const T ikz = a.pf.ikH[2][k];
     // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
@@ -436,13 +454,13 @@ __device__ __forceinline__ void h_cell(const HArgs<T> &a, int i, int j, int k, i
             if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
         }
     }
-    a.Hx[id] = hx;
-    a.Hy[id] = hy;
-    a.Hz[id] = hz;
+    a.Hx.set(id, hx);
+    a.Hy.set(id, hy);
+    a.Hz.set(id, hz);
 }
 
-template <typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<T> a) {
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
     int bx, by, bz;
     tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
     const int i = bx * BX + threadIdx.x;
@@ -474,8 +492,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<T> a) {
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_update_H_col(const HArgs<T> a) {
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_update_H_col(const HArgs<S, T> a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.Nz) return;
     const int kp = (k == a.Nz - 1) ? 0 : k + 1;
@@ -491,9 +509,9 @@ struct SensDesc {
 };
 
 // local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) on one sensor plane.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, const T *Ex, const T *Ey,
-                                                const T *Ez, const T *Hx, const T *Hy, const T *Hz, int Nx, int Ny,
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
+                                                Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
                                                 int Nz, T *stage) {
     const SensDesc d = sd[blockIdx.y];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -504,7 +522,7 @@ __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, lo
     else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) { i = d.x1 + a; j = d.y1; k = d.z1 + b; }
     else { i = d.x1 + a; j = d.y1 + b; k = d.z1; }
     const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
-    auto at = [&](const T *f, int x, int y, int z) { return f[(size_t)x + (size_t)Nx * ((size_t)y + (size_t)Ny * z)]; };
+    auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + Nx * (y + Ny * z)]; };
     T v[6];
     v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
     v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
@@ -604,10 +622,14 @@ __device__ inline double d_pulse(const aefdtd_pulse &p, double amp, double x, do
     return amp * exp(-b * b) * cos(a);
 }
 
-// R/T plane sums of modes/normal_incidence.cpp:384-415 for step t; one
-// workgroup, fixed-order reduction.
-template <typename T>
-__global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, const T *Ez, int Nx, int Ny, int kpl,
+// R/T plane sums of modes/normal_incidence.cpp:384-415 for step t.  Each of
+// the PS_BLOCKS workgroups reduces a fixed strided subset of the plane in a
+// fixed order and writes its partial; the partials are summed in block order
+// when they are read back, so the result does not depend on scheduling.
+constexpr int PS_BLOCKS = 64;
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int Ny, int kpl,
                                                    aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
                                                    double Dz, double zpos, const double *tap, int t, double Dt,
                                                    double *out) {
@@ -615,8 +637,9 @@ __global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, con
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const double tb = (t + 0.5) * Dt;
     const double tp = tap[t];
-    const size_t NxNy = (size_t)Nx * Ny;
-    for (long q = threadIdx.x; q < (long)NxNy; q += blockDim.x) {
+    const long NxNy = (long)Nx * Ny;
+    const long stride = (long)blockDim.x * gridDim.x;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < NxNy; q += stride) {
         const int i = (int)(q % Nx), j = (int)(q / Nx);
         const double x = i * Dx, y = j * Dy;
         double in[3] = {0, 0, 0};
@@ -625,7 +648,7 @@ __global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, con
             in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
             in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
         }
-        const size_t id = (size_t)q + NxNy * kpl;
+        const int id = (int)(q + NxNy * kpl);
         const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
         for (int c = 0; c < 3; c++) {
             acc[3 + c] += in[c] * tp;
@@ -639,7 +662,7 @@ __global__ __launch_bounds__(256) void k_plane_sum(const T *Ex, const T *Ey, con
             for (int c = 0; c < 6; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 6) out[(size_t)t * 6 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 6) out[((size_t)t * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 }  // namespace dev

@@ -129,3 +129,44 @@ def spheres_single_particle(n=256, D=2e-9, count=40, rmin=8e-9, rmax=20e-9, seed
                     polarization=1, precision=precision)
     sensor = SensorReq(box=True, x1=0, x2=n, y1=0, y2=n, z1=0, z2=n, Nl=Nl, lambda_min=400e-9, lambda_max=800e-9)
     return spec, [sensor], sph
+
+
+LORENTZ_LAYER = Material(eps_inf=1.0, lorentz=[(1.5, 3e15, 1e14)])
+
+
+def multilayer_normal_incidence(n=(512, 512, 384), D=4e-9, seed=3, nlayers=12, margin=16, Nt=8000, Nl=128,
+                                pml_z=16, precision=FP32):
+    """Precision-form stack: nlayers dielectric layers (thickness U[10,60] cells,
+    eps U[1.5,6.0]) plus one Lorentz-pole layer (d_eps=1.5, w0=3e15 rad/s,
+    g=1e14 1/s) at a seeded position, thicknesses rescaled to fit the structure
+    height minus a vacuum margin on each side; 16-cell CPML on +-z, periodic x/y."""
+    rng = MT19937_64(seed)
+    nx, ny, nz = n
+    thick = [rng.uniform(10, 60) for _ in range(nlayers)]
+    eps = [rng.uniform(1.5, 6.0) for _ in range(nlayers)]
+    lor_pos = int(rng.uniform(0, nlayers + 1))
+    lor_thick = rng.uniform(10, 60)
+    thick.insert(lor_pos, lor_thick)
+    avail = nz - 2 * margin
+    scale = avail / sum(thick)
+    edges = [margin]
+    acc = 0.0
+    for t in thick:
+        acc += t * scale
+        edges.append(margin + int(round(acc)))
+    mats = [Material(1.0)]
+    ids = []
+    li = 0
+    for q in range(len(thick)):
+        if q == lor_pos:
+            mats.append(LORENTZ_LAYER)
+        else:
+            mats.append(Material(eps[li]))
+            li += 1
+        ids.append(len(mats) - 1)
+    g = np.zeros((nz, ny, nx), dtype=np.uint32)
+    for q in range(len(thick)):
+        g[edges[q]:edges[q + 1]] = ids[q]
+    spec = ModeSpec(matsgrid=g, materials=mats, Nt=Nt, Dx=D, Dy=D, Dz=D, pml=(0, 0, 0, 0, pml_z, pml_z), Nl=Nl,
+                    lambda_min=400e-9, lambda_max=800e-9, polarization=1, precision=precision)
+    return spec, {"thickness_scale": scale, "edges": edges, "lorentz_layer": lor_pos}

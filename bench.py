@@ -92,6 +92,39 @@ def cpu_baseline(spec, sensors, steps_lo, steps_hi, threads):
     return dt, times
 
 
+def precision_form(args):
+    """The precision form: one multilayer normal-incidence stack (12 random layers +
+    a Lorentz layer, 512x512x384 structure, 16-cell CPML, 128 wavelengths) run with
+    FP64, FP32 and FP16 field storage; |r| and |t| spectra of FP32/FP16 are compared
+    with the FP64 run (which matches the CPU oracle to <=1e-9, see the GPU tests)."""
+    import numpy as np
+    from aether_amd import capi, workloads
+    out = {}
+    for name, prec in (("f64", capi.FP64), ("f32", capi.FP32), ("f16", capi.FP16)):
+        spec, info = workloads.multilayer_normal_incidence(Nt=args.nt, precision=prec)
+        run = capi.Run(capi.engine(), 0, spec)
+        ncell = run.stats()[0]
+        run.synchronize()
+        t0 = time.perf_counter()
+        run.step(args.nt)
+        run.synchronize()
+        wall = time.perf_counter() - t0
+        res = run.finish()
+        run.close()
+        out[name] = {"R": res["R"], "T": res["T"], "wall": wall, "mcells": ncell * args.nt / wall / 1e6}
+        print(f"precision form {name}: {wall:.2f}s, {out[name]['mcells']:.0f} Mcell/s", file=sys.stderr, flush=True)
+    rec = {"form": "precision", "workload": f"multilayer 512x512x384 (13 layers incl. Lorentz), "
+                                            f"{ncell} cells, {args.nt} steps, 128 wavelengths",
+           "thickness_scale": info["thickness_scale"]}
+    for name in ("f64", "f32", "f16"):
+        e = {"Mcell_updates_per_s": round(out[name]["mcells"], 1), "wall_s": round(out[name]["wall"], 2)}
+        if name != "f64":
+            e["max_abs_err_R"] = float(np.max(np.abs(out[name]["R"] - out["f64"]["R"])))
+            e["max_abs_err_T"] = float(np.max(np.abs(out[name]["T"] - out["f64"]["T"])))
+        rec[name] = e
+    print(json.dumps(rec))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +135,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--form", default="plain", choices=["plain", "precision"],
+                    help="plain: the headline single-particle run; precision: the 512x512x384 multilayer "
+                         "stack in FP64, FP32 and FP16 storage with spectral errors against FP64")
+    ap.add_argument("--nt", type=int, default=8000, help="time steps of the precision form")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (rocprof counter calibration)")
     args = ap.parse_args()
@@ -123,6 +160,8 @@ def main():
         return
 
     from aether_amd import capi, workloads
+    if args.form == "precision":
+        return precision_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
     nsteps = args.warmup + args.steps
     t_setup = time.perf_counter()

@@ -49,6 +49,8 @@ extern "C" {
 /* ---- storage precision of the fields ------------------------------------ */
 #define AEFDTD_FP64  0     /* FP64 fields and arithmetic (parity form)         */
 #define AEFDTD_FP32  1     /* FP32 fields/arithmetic; FP64 RC poles and DFT    */
+#define AEFDTD_FP16  2     /* FP16 field storage with per-component 2^e scale, */
+                           /* FP32 arithmetic and CPML state, FP64 poles/DFT  */
 
 /* ---- sensor orientations (Aether's NORMAL_X.. enum order) --------------- */
 #define AEFDTD_NORMAL_X   0
@@ -85,7 +87,7 @@ typedef struct aefdtd_grid aefdtd_grid;   /* opaque handle == one `FDTD` */
 
 /* FDTD::FDTD(Nx,..,pad_zp) -- core/fdtd_core.cpp:158-285.  Applies Aether's
  * padding rules (:212-243) and grid extension (:245-251).  pml[6]/pad[6] in
- * face order.  `precision` is AEFDTD_FP64 or AEFDTD_FP32. */
+ * face order.  `precision` is AEFDTD_FP64, AEFDTD_FP32 or AEFDTD_FP16. */
 int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt,
                   double Dx, double Dy, double Dz, double Dt, int mode,
                   const int pml[6], const int pad[6], int precision);
@@ -276,6 +278,10 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
  * XCD-remap<<4).  No counterpart in the reference (a CPU thread-count choice). */
 int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
+/* Power-of-two storage scales of the six field components (stored value =
+ * field * 2^exps[c]); FP16 storage only (defaults E: 2^8, H: 2^16, so that
+ * both sit near the middle of the FP16 range). Before the first update. */
+int aefdtd_set_storage_scale(aefdtd_grid *g, const int exps[6]);
 /* Bytes of device memory held by the handle. */
 int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
 /* out[0]=cells of the full grid, out[1]=dispersive (RC) cells, out[2]=max

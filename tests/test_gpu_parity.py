@@ -189,3 +189,46 @@ def test_autotune_keeps_state(engine_be, oracle_be):
     for _ in range(3):
         e.step(); o.step()
     compare(e, o)
+
+
+def test_fp16_storage_round_trip(engine_be):
+    """FP16 storage with per-component 2^e scale: write/read keeps 2^-11 relative precision."""
+    D = (5e-9,) * 3
+    Dt = 5e-9 / (np.sqrt(3.0) * 299792458.0) * 0.99
+    g = Grid(engine_be, (24, 20, 16), 10, D, Dt, capi.MODE_CUSTOM, (0,) * 6, (0,) * 6, capi.FP16)
+    g.set_matsgrid(np.zeros((16, 20, 24), dtype=np.uint32))
+    g.set_material(0, Material(1.0))
+    g.bootstrap()
+    rng = np.random.default_rng(0)
+    for f in range(6):
+        a = rng.uniform(0.5, 1.0, size=(16, 20, 24)) * (1.0 if f < 3 else 1 / 377.0)
+        g.write(f, a)
+        b = g.read(f)
+        assert np.max(np.abs(b - a) / np.abs(a)) <= 2.0 ** -11
+
+
+def test_fp16_spectra_error_reported(engine_be, oracle_be):
+    """FP16 storage, FP32 arithmetic: spectral error against the FP64 oracle (reported;
+    guarded at 5e-2 of the spectrum's peak)."""
+    spec, sens, _ = workloads.spheres_single_particle(n=20, D=2e-9, count=3, rmin=3e-9, rmax=6e-9, seed=5,
+                                                      Nt=300, Nl=10, pml=5, precision=capi.FP16)
+    g = capi.run_single_particle(engine_be, spec, sens)
+    spec.precision = capi.FP64
+    r = capi.run_single_particle(oracle_be, spec, sens)
+    err = np.max(np.abs(g["sensors"] - r["sensors"])) / np.max(np.abs(r["sensors"]))
+    print("fp16 relative spectral error", err)
+    assert err <= 5e-2
+
+
+def test_fp16_normal_incidence_spectra(engine_be, oracle_be):
+    spec = workloads.slab_normal_incidence(n=(8, 6, 40), slab=(12, 24), Nt=600, Nl=12, pml_z=8)
+    r = capi.run_normal_incidence(oracle_be, spec)
+    spec.precision = capi.FP16
+    g = capi.run_normal_incidence(engine_be, spec)
+    spec.precision = capi.FP32
+    f = capi.run_normal_incidence(engine_be, spec)
+    e16 = max(np.max(np.abs(g["R"] - r["R"])), np.max(np.abs(g["T"] - r["T"])))
+    e32 = max(np.max(np.abs(f["R"] - r["R"])), np.max(np.abs(f["T"] - r["T"])))
+    print("normal incidence |R|,|T| error: fp32", e32, "fp16", e16)
+    assert e32 <= 1e-4
+    assert e16 <= 5e-2

@@ -26,7 +26,8 @@ def test_oracle_mirrors_the_grid_abi():
     import oracle
     lib = C.CDLL(oracle.ORACLE_SO)
     engine_only = {"aefdtd_profile_enable", "aefdtd_profile_read", "aefdtd_device_bytes", "aefdtd_stats",
-                   "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize"}
+                   "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
+                   "aefdtd_set_storage_scale"}
     missing = [s for s in declared_symbols()
                if s not in engine_only and not s.startswith("aefdtd_run_") or s in
                ("aefdtd_run_normal_incidence", "aefdtd_run_single_particle")]
