@@ -20,7 +20,8 @@ ENGINE_SO = os.path.join(HERE, "libaether_fdtd.so")
 OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED = 0, 1, 2, 3, 4, 5
 MODE_NORM, MODE_CUSTOM = 0, 5
 FP64, FP32, FP16 = 0, 1, 2
-NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM = range(6)
+NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM, NORMAL_UNDEF = range(7)
+KIND_NORMAL, KIND_SINGLE, KIND_CUSTOM = 0, 1, 2
 EX, EY, EZ, HX, HY, HZ = range(6)
 
 _dp = C.POINTER(C.c_double)
@@ -41,6 +42,13 @@ class CPulse(C.Structure):
                  "Ex0", "Ey0", "Ez0", "Hx0", "Hy0", "Hz0")]
 
 
+class CSourceReq(C.Structure):
+    _fields_ = [("type", C.c_int), ("orientation", C.c_int),
+                ("x1", C.c_int), ("x2", C.c_int), ("y1", C.c_int), ("y2", C.c_int),
+                ("z1", C.c_int), ("z2", C.c_int),
+                ("lambda_min", C.c_double), ("lambda_max", C.c_double)]
+
+
 class CMode(C.Structure):
     _fields_ = [("Nt", C.c_int), ("Dx", C.c_double), ("Dy", C.c_double), ("Dz", C.c_double),
                 ("time_mod", C.c_double), ("tapering", C.c_int),
@@ -52,7 +60,8 @@ class CMode(C.Structure):
                 ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("matsgrid", C.POINTER(C.c_uint32)),
                 ("n_materials", C.c_int), ("materials", C.POINTER(CMaterial)),
-                ("precision", C.c_int)]
+                ("precision", C.c_int),
+                ("n_sources", C.c_int), ("sources", C.POINTER(CSourceReq))]
 
 
 class CSensorReq(C.Structure):
@@ -102,6 +111,9 @@ def _sig(lib, p):
         "run_normal_incidence": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq),
                                      C.POINTER(CNormalResult), _dp, _dp]),
         "run_single_particle": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
+        "run_custom": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
+        "add_oscillator": (i, [vp, i, i, i, i, i, i, i, d, d, _ip]),
+        "mode_finalize": (i, [C.POINTER(CMode), i, i, C.POINTER(CSensorReq)]),
         # engine only
         "run_create": (i, [C.POINTER(vp), i, C.POINTER(CMode), i, C.POINTER(CSensorReq)]),
         "run_step": (i, [vp, i]),
@@ -119,6 +131,17 @@ def _sig(lib, p):
         "set_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
         "cancel": (None, []),
+        "batch_create": (i, [C.POINTER(vp), i]),
+        "batch_destroy": (i, [vp]),
+        "batch_error": (C.c_char_p, [vp]),
+        "batch_add": (i, [vp, vp]),
+        "batch_remove": (i, [vp, vp]),
+        "batch_step": (i, [vp, i]),
+        "batch_running": (i, [vp]),
+        "batch_synchronize": (i, [vp]),
+        "batch_set_tiles": (i, [vp, i, i]),
+        "batch_autotune": (i, [vp, _ip]),
+        "batch_profile": (i, [vp, i, _dp, C.POINTER(C.c_long)]),
         # oracle only
         "step": (i, [vp, i]),
         "set_threads": (None, [i]),
@@ -232,6 +255,28 @@ class SensorReq:
 
 
 @dataclass
+class SourceReq:
+    """An oscillator source (Source_generator with SOURCE_GEN_OSCILLATOR), in structure cells."""
+    orientation: int = NORMAL_Z
+    x1: int = 0
+    x2: int = 0
+    y1: int = 0
+    y2: int = 0
+    z1: int = 0
+    z2: int = 0
+    lambda_min: float = 400e-9
+    lambda_max: float = 800e-9
+    type: int = 0
+
+    def to_c(self):
+        s = CSourceReq()
+        s.type, s.orientation = self.type, self.orientation
+        s.x1, s.x2, s.y1, s.y2, s.z1, s.z2 = self.x1, self.x2, self.y1, self.y2, self.z1, self.z2
+        s.lambda_min, s.lambda_max = self.lambda_min, self.lambda_max
+        return s
+
+
+@dataclass
 class ModeSpec:
     """The FDTD_Mode fields Aether's drivers read, plus the discretised structure."""
     matsgrid: np.ndarray                      # (nz, ny, nx) uint32, x fastest
@@ -254,6 +299,7 @@ class ModeSpec:
     polar_angle: float = 0.0
     disable_fields: tuple = ()
     precision: int = FP64
+    sources: list = field(default_factory=list)   # SourceReq, custom mode
 
     def to_c(self, keep):
         mg = np.ascontiguousarray(self.matsgrid, dtype=np.uint32)
@@ -281,7 +327,53 @@ class ModeSpec:
         c.n_materials = len(self.materials)
         c.materials = mats
         c.precision = self.precision
+        srcs = (CSourceReq * max(1, len(self.sources)))(*[q.to_c() for q in self.sources])
+        keep.append(srcs)
+        c.n_sources = len(self.sources)
+        c.sources = srcs
         return c
+
+
+def mode_finalize(be: Backend, spec: ModeSpec, kind, sensors=()):
+    """FDTD_Mode::finalize's grid sizing: returns a copy of spec with pml/pad updated."""
+    import copy
+    keep = []
+    cm = spec.to_c(keep)
+    sreq = (CSensorReq * max(1, len(sensors)))(*[q.to_c() for q in sensors])
+    be.check(be.fn("mode_finalize")(C.byref(cm), kind, len(sensors), sreq), None)
+    out = copy.copy(spec)
+    out.pml = tuple(cm.pml[f] for f in range(6))
+    out.pad = tuple(cm.pad[f] for f in range(6))
+    return out
+
+
+def run_custom(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
+    """Runs the custom-mode driver (spec already finalized); sensor outputs and fields."""
+    keep = []
+    cm = spec.to_c(keep)
+    sreq = (CSensorReq * max(1, len(sensors)))(*[q.to_c() for q in sensors])
+    nsout = sum(q.out_len() for q in sensors)
+    sout = np.zeros(max(1, nsout))
+    fields = None
+    if want_fields:
+        fields = np.zeros((6,) + custom_dims(spec)[::-1])
+    rc = be.fn("run_custom")(C.byref(cm), len(sensors), sreq, _dptr(sout),
+                             _dptr(fields) if fields is not None else None)
+    if rc not in (OK, ECANCELLED):
+        be.check(rc, None)
+    out = {"sensors": sout[:nsout]}
+    if fields is not None:
+        out["fields"] = fields
+    return out
+
+
+def custom_dims(spec):
+    """Full-grid (Nx, Ny, Nz) of a finalized custom-mode spec (FDTD ctor extension)."""
+    nz, ny, nx = spec.matsgrid.shape
+    n = [nx, ny, nz]
+    for a in range(3):
+        n[a] += spec.pml[2 * a] + spec.pml[2 * a + 1] + spec.pad[2 * a] + spec.pad[2 * a + 1]
+    return tuple(n)
 
 
 def _dptr(a):
@@ -410,6 +502,9 @@ class Run:
         self.be.check(self.be.fn("autotune")(self.grid(), reps, out), self.grid())
         return list(out)
 
+    def set_tiles(self, tile_e, tile_h):
+        self.be.check(self.be.fn("set_tiles")(self.grid(), tile_e, tile_h), self.grid())
+
     def profile_read(self, kernel):
         ms, n = C.c_double(), C.c_long()
         self.be.fn("profile_read")(self.grid(), kernel.encode(), C.byref(ms), C.byref(n))
@@ -449,6 +544,63 @@ class Run:
     def close(self):
         if self.h:
             self.be.fn("run_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Custom-mode runs stepped together by the engine (one launch per half step
+    for all members, 64-step chunks replayed from a HIP graph)."""
+
+    def __init__(self, be: Backend, precision=FP32):
+        self.be = be
+        h = C.c_void_p()
+        be.check(be.fn("batch_create")(C.byref(h), precision), None)
+        self.h = h
+        self.runs = []
+
+    def _c(self, rc):
+        if rc != OK:
+            raise FDTDError(rc, (self.be.fn("batch_error")(self.h) or b"").decode())
+
+    def add(self, run: "Run"):
+        self._c(self.be.fn("batch_add")(self.h, run.h))
+        self.runs.append(run)
+
+    def remove(self, run: "Run"):
+        self._c(self.be.fn("batch_remove")(self.h, run.h))
+        self.runs.remove(run)
+
+    def step(self, chunks=1):
+        self._c(self.be.fn("batch_step")(self.h, chunks))
+
+    def running(self):
+        return self.be.fn("batch_running")(self.h)
+
+    def synchronize(self):
+        self._c(self.be.fn("batch_synchronize")(self.h))
+
+    def set_tiles(self, tile_e, tile_h):
+        self._c(self.be.fn("batch_set_tiles")(self.h, tile_e, tile_h))
+
+    def autotune(self):
+        out = (C.c_int * 2)()
+        self._c(self.be.fn("batch_autotune")(self.h, out))
+        return list(out)
+
+    def profile(self, on):
+        ms, n = C.c_double(), C.c_long()
+        self._c(self.be.fn("batch_profile")(self.h, on, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            self.be.fn("batch_destroy")(self.h)
             self.h = None
 
     def __del__(self):
@@ -547,6 +699,11 @@ class Grid:
         lam = np.ascontiguousarray(lambdas, dtype=np.float64)
         i = C.c_int()
         self._c(self.be.fn("add_spect_poynting")(self.h, orientation, *box, len(lam), _dptr(lam), C.byref(i)))
+        return i.value
+
+    def add_oscillator(self, orientation, box, lambda_min, lambda_max):
+        i = C.c_int()
+        self._c(self.be.fn("add_oscillator")(self.h, orientation, *box, lambda_min, lambda_max, C.byref(i)))
         return i.value
 
     def feed_sensors(self):

@@ -393,6 +393,66 @@ Fld<S, T> fld(const aefdtd_grid *g, int c) {
     return f;
 }
 
+// Value tables of the oscillators up to grid step `upto` (exclusive), grown
+// by doubling; the descriptors are re-uploaded when a table moves.
+int ensure_osc_tables(aefdtd_grid *g, int upto) {
+    bool moved = false;
+    for (size_t q = 0; q < g->osc.size(); q++) {
+        OscDev &o = g->osc[q];
+        const int need = upto - o.step0;
+        if (need <= o.nval) continue;
+        const int n = std::max(need, 2 * o.nval);
+        std::vector<double> v(n);
+        for (int st = 0; st < n; st++) v[st] = oscillator_value(g->osc_par[q], st);
+        void *p;
+        int rc = dalloc(g, &p, n * sizeof(double));
+        if (rc) return rc;
+        HIPOK(g, hipStreamSynchronize(g->stream));
+        HIPOK(g, hipMemcpy(p, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        if (o.val) {
+            g->dev_bytes -= (size_t)o.nval * sizeof(double);
+            void *old = (void *)o.val;
+            dfree(old);
+        }
+        o.val = (const double *)p;
+        o.nval = n;
+        moved = true;
+    }
+    if (moved) HIPOK(g, hipMemcpy(g->d_osc, g->osc.data(), g->osc.size() * sizeof(OscDev), hipMemcpyHostToDevice));
+    return AEFDTD_OK;
+}
+
+// Load masks of the passes.  An E component is loaded and stored when it is
+// updated (enable_*, core/fdtd_threads.cpp:60-82) or written by an injection;
+// an H component is loaded when an updated E component takes a difference of
+// it along an axis longer than one cell (along a one-cell axis the periodic
+// difference is exactly zero, core/fdtd_core.cpp:376-379).  The H pass mirrors
+// this with E and H exchanged.  Fields outside the masks stay zero.
+int load_mask_E(const aefdtd_grid *g) {
+    const bool ex = g->enable[0], ey = g->enable[1], ez = g->enable[2];
+    const bool lx = g->Nx > 1, ly = g->Ny > 1, lz = g->Nz > 1;
+    int m = (ex ? 1 : 0) | (ey ? 2 : 0) | (ez ? 4 : 0);
+    if (g->inj_k >= 0) m |= 1 | 2;
+    if (g->tfsf_aux) m |= 7;
+    for (const OscDev &o : g->osc) m |= 1 << o.comp;
+    if ((ey && lz) || (ez && ly)) m |= 8;     // Hx: dHx/dz in Ey, dHx/dy in Ez
+    if ((ex && lz) || (ez && lx)) m |= 16;    // Hy: dHy/dz in Ex, dHy/dx in Ez
+    if ((ex && ly) || (ey && lx)) m |= 32;    // Hz: dHz/dy in Ex, dHz/dx in Ey
+    return m;
+}
+
+int load_mask_H(const aefdtd_grid *g) {
+    const bool hx = g->enable[3], hy = g->enable[4], hz = g->enable[5];
+    const bool lx = g->Nx > 1, ly = g->Ny > 1, lz = g->Nz > 1;
+    int m = (hx ? 8 : 0) | (hy ? 16 : 0) | (hz ? 32 : 0);
+    if (g->inj_k >= 0) m |= 8 | 16;
+    if (g->tfsf_aux) m |= 8 | 16 | 32;
+    if ((hy && lz) || (hz && ly)) m |= 1;     // Ex: dEx/dz in Hy, dEx/dy in Hz
+    if ((hx && lz) || (hz && lx)) m |= 2;     // Ey: dEy/dz in Hx, dEy/dx in Hz
+    if ((hx && ly) || (hy && lx)) m |= 4;     // Ez: dEz/dy in Hx, dEz/dx in Hy
+    return m;
+}
+
 template <typename S, typename T>
 EArgs<S, T> e_args(aefdtd_grid *g) {
     EArgs<S, T> a;
@@ -423,6 +483,10 @@ EArgs<S, T> e_args(aefdtd_grid *g) {
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
         a.aHx = fld<S, T>(g->tfsf_aux, 3); a.aHy = fld<S, T>(g->tfsf_aux, 4); a.aHz = fld<S, T>(g->tfsf_aux, 5);
     }
+    a.lm = load_mask_E(g);
+    a.nosc = (int)g->osc.size();
+    a.osc = (const OscDev *)g->d_osc;
+    a.ostep = g->tstep;
     return a;
 }
 
@@ -452,6 +516,7 @@ HArgs<S, T> h_args(aefdtd_grid *g) {
         a.tper_x = g->tf_per_x; a.tper_y = g->tf_per_y;
         a.aEx = fld<S, T>(g->tfsf_aux, 0); a.aEy = fld<S, T>(g->tfsf_aux, 1); a.aEz = fld<S, T>(g->tfsf_aux, 2);
     }
+    a.lm = load_mask_H(g);
     return a;
 }
 
@@ -508,7 +573,9 @@ int launch_rc(aefdtd_grid *g) {
 
 template <typename S, typename T>
 int launch_E(aefdtd_grid *g) {
-    int rc = launch_rc<S, T>(g);
+    int rc = ensure_osc_tables(g, g->tstep + 1);
+    if (rc) return rc;
+    rc = launch_rc<S, T>(g);
     if (rc) return rc;
     EArgs<S, T> a = e_args<S, T>(g);
     hipEvent_t ev;
@@ -555,6 +622,8 @@ float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hi
 // (fields, CPML and RC accumulators) from a device snapshot afterwards.
 template <typename S, typename T>
 int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
+    int rc0 = ensure_osc_tables(g, g->tstep + 1);
+    if (rc0) return rc0;
     std::vector<std::pair<void *, size_t>> state;
     for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->ncell() * g->ssize()});
     for (int s = 0; s < 12; s++)
@@ -654,25 +723,31 @@ int flush_dft(aefdtd_grid *g) {
     return AEFDTD_OK;
 }
 
+// First feed: lays the sensors out in one stage pool and uploads their descriptors.
+template <typename T>
+int prepare_stage(aefdtd_grid *g) {
+    if (g->d_stage || g->sensors.empty()) return AEFDTD_OK;
+    long Ptot = 0;
+    std::vector<SensDesc> sd;
+    for (Sensor *s : g->sensors) {
+        s->off = Ptot;
+        Ptot += s->P;
+        sd.push_back({s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off});
+        g->stage_maxP = std::max(g->stage_maxP, s->P);
+    }
+    g->stage_P = Ptot;
+    int rc;
+    if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
+    if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
+    HIPOK(g, hipMemcpy(g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
+    return AEFDTD_OK;
+}
+
 template <typename S, typename T>
 int feed_T(aefdtd_grid *g) {
     if (!g->sensors.empty()) {
         int rc;
-        if (!g->d_stage) {
-            // first feed: lay the sensors out in one stage pool and upload their descriptors
-            long Ptot = 0;
-            std::vector<SensDesc> sd;
-            for (Sensor *s : g->sensors) {
-                s->off = Ptot;
-                Ptot += s->P;
-                sd.push_back({s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off});
-                g->stage_maxP = std::max(g->stage_maxP, s->P);
-            }
-            g->stage_P = Ptot;
-            if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
-            if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
-            HIPOK(g, hipMemcpy(g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
-        }
+        if ((rc = prepare_stage<T>(g))) return rc;
         dim3 grid((unsigned)((g->stage_maxP + 255) / 256), (unsigned)g->sensors.size(), 1);
         hipEvent_t ev;
         prof_begin(g, 2, &ev);
@@ -699,6 +774,335 @@ int feed_T(aefdtd_grid *g) {
     return AEFDTD_OK;
 }
 
+
+// ================================================================ batches
+// Many independent grids advanced together (the batching form): one launch
+// per half step covers every member grid, the per-step sensor samples and
+// the end-of-chunk DFT of all members are one launch each, and a chunk of KS
+// steps is captured once in a HIP graph and replayed while the membership is
+// unchanged.  Members join with an empty DFT stage (a multiple of KS steps
+// done) and leave between chunks; a member whose step count ends inside a
+// chunk idles for the rest of it.
+
+#define BHIPOK(b, call)                                                                                 \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);                               \
+            return e_ == hipErrorOutOfMemory ? AEFDTD_ENOMEM : AEFDTD_EDEVICE;                          \
+        }                                                                                               \
+    } while (0)
+
+template <typename S, typename T, int BY, int KZ>
+void launch_Eb_cfg(int nwg, size_t lds, hipStream_t st, const void *d, const int *wg0, int n, const int *tc,
+                   const int *te, int s) {
+    hipLaunchKernelGGL((k_update_E_batch<S, T, BY, KZ>), dim3(nwg), dim3(BX, BY, 1), lds, st,
+                       (const EArgs<S, T> *)d, wg0, n, tc, te, s);
+}
+template <typename S, typename T, int BY, int KZ>
+void launch_Hb_cfg(int nwg, hipStream_t st, const void *d, const int *wg0, int n, const int *tc, const int *te,
+                   int s) {
+    hipLaunchKernelGGL((k_update_H_batch<S, T, BY, KZ>), dim3(nwg), dim3(BX, BY, 1), 0, st, (const HArgs<S, T> *)d,
+                       wg0, n, tc, te, s);
+}
+
+template <typename S, typename T, int BY>
+void launch_Eb_by(int kzi, int nwg, size_t lds, hipStream_t st, const void *d, const int *wg0, int n, const int *tc,
+                  const int *te, int s) {
+    switch (kzi) {
+        case 0: launch_Eb_cfg<S, T, BY, 4>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        case 1: launch_Eb_cfg<S, T, BY, 8>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        case 2: launch_Eb_cfg<S, T, BY, 16>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        default: launch_Eb_cfg<S, T, BY, 32>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+    }
+}
+template <typename S, typename T, int BY>
+void launch_Hb_by(int kzi, int nwg, hipStream_t st, const void *d, const int *wg0, int n, const int *tc,
+                  const int *te, int s) {
+    switch (kzi) {
+        case 0: launch_Hb_cfg<S, T, BY, 4>(nwg, st, d, wg0, n, tc, te, s); break;
+        case 1: launch_Hb_cfg<S, T, BY, 8>(nwg, st, d, wg0, n, tc, te, s); break;
+        case 2: launch_Hb_cfg<S, T, BY, 16>(nwg, st, d, wg0, n, tc, te, s); break;
+        default: launch_Hb_cfg<S, T, BY, 32>(nwg, st, d, wg0, n, tc, te, s); break;
+    }
+}
+
+}  // namespace
+
+struct aefdtd::Batch {
+    int precision = AEFDTD_FP32;
+    hipStream_t stream = nullptr;
+    std::vector<aefdtd_grid *> members;
+    int tile_e = 2 | (1 << 2), tile_h = 2 | (1 << 2);
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // device tables of the current membership
+    void *d_ea = nullptr, *d_ha = nullptr, *d_samp = nullptr, *d_dft = nullptr;
+    int *d_wgE = nullptr, *d_wgH = nullptr, *d_tcur = nullptr, *d_tend = nullptr;
+    int nwgE = 0, nwgH = 0, nsamp = 0, ndft = 0;
+    long maxP = 0;
+    size_t ldsE = 0;
+    std::vector<int> tcur;                 // host mirror of d_tcur
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool prof_on = false;
+    double prof_ms = 0;
+    long prof_chunks = 0;
+    std::string err;
+};
+
+namespace {
+
+void batch_free_tables(Batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    for (void **p : {&b->d_ea, &b->d_ha, &b->d_samp, &b->d_dft, (void **)&b->d_wgE, (void **)&b->d_wgH,
+                     (void **)&b->d_tcur, (void **)&b->d_tend})
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+}
+
+template <typename V>
+int batch_upload(Batch *b, void **dst, const std::vector<V> &v) {
+    if (v.empty()) return AEFDTD_OK;
+    BHIPOK(b, hipMalloc(dst, v.size() * sizeof(V)));
+    BHIPOK(b, hipMemcpy(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    return AEFDTD_OK;
+}
+
+// Lays out the device tables of the membership and captures one chunk.
+template <typename S, typename T>
+int batch_build(Batch *b) {
+    BHIPOK(b, hipStreamSynchronize(b->stream));      // the previous graph may still read the tables
+    batch_free_tables(b);
+    const int n = (int)b->members.size();
+    b->tcur.assign(n, 0);
+    if (n == 0) {
+        b->dirty = false;
+        return AEFDTD_OK;
+    }
+    std::vector<EArgs<S, T>> ea;
+    std::vector<HArgs<S, T>> ha;
+    std::vector<int> wgE, wgH, tend(n);
+    std::vector<SampB<S, T>> samp;
+    std::vector<DftB<T>> dft;
+    int totE = 0, totH = 0, maxmat = 1;
+    b->maxP = 0;
+    for (int r = 0; r < n; r++) {
+        aefdtd_grid *g = b->members[r];
+        int rc;
+        if ((rc = ensure_ready(g)) || (rc = ensure_osc_tables(g, g->Nt)) || (rc = prepare_stage<T>(g))) {
+            b->err = g->err;
+            return rc;
+        }
+        for (Sensor *s : g->sensors)
+            if ((rc = ensure_coefs(g, *s, g->Nt))) { b->err = g->err; return rc; }
+        EArgs<S, T> a = e_args<S, T>(g);
+        const int nb = tile_geometry(a, b->tile_e);
+        wgE.push_back(totE);
+        totE += (nb + 7) / 8 * 8;
+        maxmat = std::max(maxmat, a.nmat);
+        ea.push_back(a);
+        HArgs<S, T> h = h_args<S, T>(g);
+        const int nh = tile_geometry(h, b->tile_h);
+        wgH.push_back(totH);
+        totH += (nh + 7) / 8 * 8;
+        ha.push_back(h);
+        b->tcur[r] = g->tstep;
+        tend[r] = g->Nt;
+        for (Sensor *s : g->sensors) {
+            SampB<S, T> q;
+            q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off};
+            for (int c = 0; c < 6; c++) q.F[c] = fld<S, T>(g, c);
+            q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.member = r;
+            q.Ptot = g->stage_P;
+            q.stage = (T *)g->d_stage;
+            samp.push_back(q);
+            dft.push_back(DftB<T>{(const T *)g->d_stage, g->stage_P, s->off, s->P, s->Nl, r,
+                                  (const double2 *)s->d_coef, s->d_acc});
+            b->maxP = std::max(b->maxP, s->P);
+        }
+    }
+    wgE.push_back(totE);
+    wgH.push_back(totH);
+    b->nwgE = totE;
+    b->nwgH = totH;
+    b->nsamp = (int)samp.size();
+    b->ndft = (int)dft.size();
+    b->ldsE = maxmat * sizeof(MatL<T>);
+    int rc;
+    if ((rc = batch_upload(b, &b->d_ea, ea)) || (rc = batch_upload(b, &b->d_ha, ha)) ||
+        (rc = batch_upload(b, (void **)&b->d_wgE, wgE)) || (rc = batch_upload(b, (void **)&b->d_wgH, wgH)) ||
+        (rc = batch_upload(b, (void **)&b->d_tcur, b->tcur)) || (rc = batch_upload(b, (void **)&b->d_tend, tend)) ||
+        (rc = batch_upload(b, &b->d_samp, samp)) || (rc = batch_upload(b, &b->d_dft, dft)))
+        return rc;
+    // one chunk: KS x (E, H, samples), the DFT flush, the step advance
+    BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int ebi = b->tile_e & 3, ekz = (b->tile_e >> 2) & 3, hbi = b->tile_h & 3, hkz = (b->tile_h >> 2) & 3;
+    for (int s = 0; s < KS; s++) {
+        switch (ebi) {
+            case 0: launch_Eb_by<S, T, 1>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
+            case 1: launch_Eb_by<S, T, 2>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
+            case 2: launch_Eb_by<S, T, 4>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
+            default: launch_Eb_by<S, T, 8>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
+        }
+        switch (hbi) {
+            case 0: launch_Hb_by<S, T, 1>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
+            case 1: launch_Hb_by<S, T, 2>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
+            case 2: launch_Hb_by<S, T, 4>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
+            default: launch_Hb_by<S, T, 8>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
+        }
+        if (b->nsamp)
+            hipLaunchKernelGGL((k_sample_batch<S, T>), dim3((unsigned)((b->maxP + 255) / 256), (unsigned)b->nsamp),
+                               dim3(256), 0, b->stream, (const SampB<S, T> *)b->d_samp, b->d_tcur, b->d_tend, s);
+    }
+    if (b->ndft)
+        hipLaunchKernelGGL((k_dft_batch<T>), dim3((unsigned)((b->maxP + 255) / 256), 6, (unsigned)b->ndft), dim3(256),
+                           0, b->stream, (const DftB<T> *)b->d_dft, b->d_tcur, b->d_tend);
+    hipLaunchKernelGGL(k_batch_advance, dim3((n + 255) / 256), dim3(256), 0, b->stream, b->d_tcur, b->d_tend, n);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+    BHIPOK(b, le);
+    BHIPOK(b, ce);
+    BHIPOK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->dirty = false;
+    return AEFDTD_OK;
+}
+
+template <class F>
+int with_types_p(int precision, F &&f) {
+    if (precision == AEFDTD_FP64) return f.template operator()<double, double>();
+    if (precision == AEFDTD_FP32) return f.template operator()<float, float>();
+    return f.template operator()<_Float16, float>();
+}
+
+}  // namespace
+
+namespace aefdtd {
+
+int batch_create(Batch **out, int precision) {
+    *out = nullptr;
+    if (precision < AEFDTD_FP64 || precision > AEFDTD_FP16) return AEFDTD_EINVAL;
+    auto *b = new Batch();
+    b->precision = precision;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        set_global_error("batch stream/event creation failed");
+        batch_destroy(b);
+        return AEFDTD_EDEVICE;
+    }
+    *out = b;
+    return AEFDTD_OK;
+}
+
+void batch_destroy(Batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free_tables(b);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char *batch_error(const Batch *b) { return b->err.c_str(); }
+
+int batch_add(Batch *b, aefdtd_grid *g) {
+    if (!b || !g) return AEFDTD_EINVAL;
+    auto bad = [&](const char *why) {
+        b->err = why;
+        return AEFDTD_ESTATE;
+    };
+    if (g->precision != b->precision) return bad("grid precision differs from the batch's");
+    if (std::find(b->members.begin(), b->members.end(), g) != b->members.end()) return bad("grid already in the batch");
+    if (g->tfsf_aux || g->inj_k >= 0 || !g->sums.empty()) return bad("TFSF, plane injection and plane sums are not batched");
+    if (is_column(g)) return bad("1x1xN column grids are not batched");
+    if (g->tstep >= g->Nt) return bad("grid has no steps left");
+    if (g->tstep % KS != 0 || g->stage_fill != 0) return bad("a grid joins a batch after a multiple of 64 steps");
+    int rc = ensure_ready(g);
+    if (rc) { b->err = g->err; return rc; }
+    if (g->n_disp) return bad("grids with dispersive (RC) materials are not batched");
+    BHIPOK(b, hipStreamSynchronize(g->stream));
+    BHIPOK(b, hipStreamSynchronize(b->stream));
+    b->members.push_back(g);
+    b->dirty = true;
+    return AEFDTD_OK;
+}
+
+int batch_remove(Batch *b, aefdtd_grid *g) {
+    if (!b || !g) return AEFDTD_EINVAL;
+    auto it = std::find(b->members.begin(), b->members.end(), g);
+    if (it == b->members.end()) {
+        b->err = "grid is not in the batch";
+        return AEFDTD_EINVAL;
+    }
+    BHIPOK(b, hipStreamSynchronize(b->stream));
+    b->members.erase(it);
+    b->dirty = true;
+    return AEFDTD_OK;
+}
+
+int batch_members(const Batch *b) { return (int)b->members.size(); }
+
+int batch_step(Batch *b, int chunks) {
+    if (!b || chunks < 0) return AEFDTD_EINVAL;
+    if (b->members.empty() || chunks == 0) return AEFDTD_OK;
+    if (b->dirty) {
+        int rc = with_types_p(b->precision, [&]<typename S, typename T>() { return batch_build<S, T>(b); });
+        if (rc) return rc;
+    }
+    if (b->prof_on) BHIPOK(b, hipEventRecord(b->ev0, b->stream));
+    for (int c = 0; c < chunks; c++) {
+        BHIPOK(b, hipGraphLaunch(b->exec, b->stream));
+        for (size_t r = 0; r < b->members.size(); r++) {
+            aefdtd_grid *g = b->members[r];
+            b->tcur[r] = std::min(b->tcur[r] + KS, g->Nt);
+        }
+    }
+    if (b->prof_on) {
+        BHIPOK(b, hipEventRecord(b->ev1, b->stream));
+        BHIPOK(b, hipEventSynchronize(b->ev1));
+        float ms = 0;
+        BHIPOK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->prof_ms += ms;
+        b->prof_chunks += chunks;
+    }
+    // host state of the members: steps, feeds and an empty stage
+    for (size_t r = 0; r < b->members.size(); r++) {
+        aefdtd_grid *g = b->members[r];
+        g->tstep = b->tcur[r];
+        g->sensor_step = b->tcur[r];
+        g->stage_fill = 0;
+    }
+    return AEFDTD_OK;
+}
+
+int batch_synchronize(Batch *b) {
+    BHIPOK(b, hipStreamSynchronize(b->stream));
+    return AEFDTD_OK;
+}
+
+int batch_set_tiles(Batch *b, int tile_e, int tile_h) {
+    if (tile_e < 0 || tile_e > 31 || tile_h < 0 || tile_h > 31) return AEFDTD_EINVAL;
+    b->tile_e = tile_e;
+    b->tile_h = tile_h;
+    b->dirty = true;
+    return AEFDTD_OK;
+}
+
+void batch_profile(Batch *b, int on, double *ms, long *chunks) {
+    if (ms) *ms = b->prof_ms;
+    if (chunks) *chunks = b->prof_chunks;
+    if (on >= 0) {
+        b->prof_on = on != 0;
+        if (on) { b->prof_ms = 0; b->prof_chunks = 0; }
+    }
+}
+
+}  // namespace aefdtd
+
+namespace {
 }  // namespace
 
 // ================================================================= C ABI
@@ -807,6 +1211,11 @@ int aefdtd_destroy(aefdtd_grid *g) {
     for (int s = 0; s < 12; s++) dfree(g->d_psipml[s]);
     dfree(g->d_stage);
     dfree(g->d_sdesc);
+    dfree(g->d_osc);
+    for (OscDev &o : g->osc) {
+        void *p = (void *)o.val;
+        dfree(p);
+    }
     for (Sensor *s : g->sensors) {
         dfree_t(s->d_acc);
         dfree_t(s->d_coef);
@@ -1026,6 +1435,37 @@ int aefdtd_set_tfsf(aefdtd_grid *g, const aefdtd_grid *aux_c, int i1, int i2, in
     std::memcpy(g->tb, b, sizeof(b));
     g->tf_per_x = px != 0;
     g->tf_per_y = py != 0;
+    return AEFDTD_OK;
+}
+
+int aefdtd_add_oscillator(aefdtd_grid *g, int type, int x1, int x2, int y1, int y2, int z1, int z2,
+                          double lambda_min, double lambda_max, int *id) {
+    if (!g || !id) return AEFDTD_EINVAL;
+    *id = -1;
+    if (type < 0 || type > 6) return fail(g, AEFDTD_EINVAL, "bad oscillator orientation");
+    if (!(lambda_min > 0) || !(lambda_max > 0) || lambda_min == lambda_max)
+        return fail(g, AEFDTD_EINVAL, "bad oscillator spectrum");
+    if ((int)g->osc.size() >= MAX_OSC) return fail(g, AEFDTD_EINVAL, "too many oscillators");
+    if (type == 6) return AEFDTD_OK;    // NORMAL_UNDEF injects nothing (sc_oscillator.cpp:63-65)
+    OscDev o;
+    o.comp = type / 2;                  // NORMAL_XM -> NORMAL_X etc. (sc_oscillator.cpp:72-74)
+    o.x1 = std::clamp(x1 + g->xs_s, 0, g->Nx); o.x2 = std::clamp(x2 + g->xs_s, 0, g->Nx);   // Source::link
+    o.y1 = std::clamp(y1 + g->ys_s, 0, g->Ny); o.y2 = std::clamp(y2 + g->ys_s, 0, g->Ny);
+    o.z1 = std::clamp(z1 + g->zs_s, 0, g->Nz); o.z2 = std::clamp(z2 + g->zs_s, 0, g->Nz);
+    o.step0 = g->tstep;
+    o.nval = 0;
+    o.val = nullptr;
+    OscParams par;
+    oscillator_init(lambda_min, lambda_max, g->Dt, par);
+    if (!g->d_osc) {
+        int rc = dalloc(g, &g->d_osc, MAX_OSC * sizeof(OscDev));
+        if (rc) return rc;
+    }
+    g->osc.push_back(o);
+    g->osc_par.push_back(par);
+    int rc = ensure_osc_tables(g, g->tstep + std::max(g->Nt, 1));
+    if (rc) return rc;
+    *id = (int)g->osc.size() - 1;
     return AEFDTD_OK;
 }
 

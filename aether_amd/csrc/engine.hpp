@@ -44,6 +44,43 @@ double s_curve(double x, double xmin, double xmax);
 // Text returned by aefdtd_last_error(NULL) (engine.hip).
 void set_global_error(const std::string &msg);
 
+// Oscillator source (sources/sc_oscillator.cpp:34-84): after every E update
+// comp += val[step] over the box [x1,x2) x [y1,y2) x [z1,z2) (full-grid
+// indices, after Source::link), step = grid step - step0 counting the
+// source's own time steps.  val is a device table of the injected values,
+// filled on the host by oscillator_value for steps [0, nval).
+struct OscDev {
+    int comp;                  // 0 Ex, 1 Ey, 2 Ez
+    int x1, x2, y1, y2, z1, z2;
+    int step0, nval;
+    const double *val;
+};
+constexpr int MAX_OSC = 8;
+// Oscillator::initialize (sources/sc_oscillator.cpp:70-84) for time step Dt.
+struct OscParams {
+    double w0 = 0, dw = 0, Dt = 0;
+    int tshift = 0;            // an int in Aether (core/sources.h:222)
+};
+void oscillator_init(double lambda_min, double lambda_max, double Dt, OscParams &o);
+// Re(e^{-i w0 step Dt} e^{-(dw/2 (step-tshift) Dt)^2}) * 50 (sc_oscillator.cpp:37-38).
+double oscillator_value(const OscParams &o, int step);
+
+// Batches of grids stepped together (engine.hip): one launch per half step
+// over every member, chunks of 64 steps captured in a HIP graph.
+struct Batch;
+int batch_create(Batch **out, int precision);
+void batch_destroy(Batch *b);
+const char *batch_error(const Batch *b);
+int batch_add(Batch *b, aefdtd_grid *g);
+int batch_remove(Batch *b, aefdtd_grid *g);
+int batch_members(const Batch *b);
+int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
+int batch_synchronize(Batch *b);
+int batch_set_tiles(Batch *b, int tile_e, int tile_h);
+// Device time of batch_step calls while on (HIP events): on=1 resets and
+// starts, 0 stops, -1 only reads.
+void batch_profile(Batch *b, int on, double *ms, long *chunks);
+
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct PlaneSum;    // normal-incidence R/T plane sum (engine.hip)
 
@@ -89,6 +126,9 @@ struct aefdtd_grid {
     // driver actions
     int inj_k = -1;
     std::vector<double> inj[4];
+    std::vector<aefdtd::OscDev> osc;  // oscillator sources, registration order
+    std::vector<aefdtd::OscParams> osc_par;
+    void *d_osc = nullptr;            // OscDev[MAX_OSC]
     aefdtd_grid *tfsf_aux = nullptr;
     int tb[6] = {0, 0, 0, 0, 0, 0};
     bool tf_per_x = false, tf_per_y = false;

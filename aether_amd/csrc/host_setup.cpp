@@ -155,6 +155,27 @@ double s_curve(double x, double xmin, double xmax) {
     return u * u * (3.0 - 2.0 * u);
 }
 
+// Oscillator::initialize (sources/sc_oscillator.cpp:70-84): centre w0 of the
+// [lambda_min, lambda_max] band in rad/s, width dw from the sqrt(0.1) edge
+// level, delay tshift (truncated to whole steps) from the 1e-7 onset level.
+void oscillator_init(double lambda_min, double lambda_max, double Dt, OscParams &o) {
+    const double alp = std::sqrt(0.1), bet = 1e-7;
+    const double w1 = 2.0 * kPi * kC / lambda_max, w2 = 2.0 * kPi * kC / lambda_min;
+    o.w0 = (w1 + w2) / 2.0;
+    o.dw = std::abs(w2 - w1) / (2.0 * std::sqrt(-std::log(alp)));
+    o.tshift = (int)(std::sqrt(-4.0 / (o.dw * o.dw) * std::log(bet)) / Dt);
+    o.Dt = Dt;
+}
+
+// Oscillator::deep_inject_E's value (sc_oscillator.cpp:37-38): the complex
+// carrier e^{-i w0 step Dt} times the real Gaussian envelope, real part, x50.
+double oscillator_value(const OscParams &o, int step) {
+    const std::complex<double> I(0.0, 1.0);
+    const std::complex<double> v = std::exp(-o.w0 * step * o.Dt * I) *
+                                   std::exp(-std::pow(o.dw / 2.0 * ((step - o.tshift) * o.Dt), 2.0));
+    return std::real(v) * 50;
+}
+
 }  // namespace aefdtd
 
 extern "C" {

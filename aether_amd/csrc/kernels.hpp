@@ -84,6 +84,9 @@ struct EArgs {
     T injx, injy;
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
     Fld<S, T> aHx, aHy, aHz;   // aux 1D grid H columns
+    int lm;                    // bits 0-2: E components loaded and stored, bits 3-5: H components loaded
+    int nosc, ostep;           // oscillators and the step their value is taken at
+    const OscDev *osc;
 };
 
 template <typename S, typename T>
@@ -100,12 +103,12 @@ struct HArgs {
     T injx, injy;
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
     Fld<S, T> aEx, aEy, aEz;
+    int lm;                    // bits 0-2: E components loaded, bits 3-5: H components loaded and stored
 };
 
 // Hardware workgroup id -> logical tile (bx, by, bz).  Workgroups are dealt to
 // the 8 XCDs round-robin; the fold gives XCD x the x-th contiguous range.
-__device__ __forceinline__ void tile_of(int nbx, int nby, int nbz, int remap, int &bx, int &by, int &bz) {
-    int w = blockIdx.x;
+__device__ __forceinline__ void tile_of(int w, int nbx, int nby, int nbz, int remap, int &bx, int &by, int &bz) {
     if (remap) {
         const int B = nbx * nby * nbz;
         const int q = B / 8, r = B % 8;
@@ -129,7 +132,7 @@ struct MatL {
 template <typename S, typename T>
 __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
                                        T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
-                                       T ikx, T iky, int nx_, int ny_) {
+                                       T ikx, T iky, int nx_, int ny_, int ostep) {
     const T ikz = a.pf.ikE[2][k];
     // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
     T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
@@ -224,9 +227,19 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
     }
     // plane injection after the whole E update (modes/normal_incidence.cpp:338-342)
     if (k == a.inj_k) { ex += a.injx; ey += a.injy; }
-    a.Ex.set(id, ex);
-    a.Ey.set(id, ey);
-    a.Ez.set(id, ez);
+    // oscillator sources, in registration order (modes/default_fdtd.cpp:158-159)
+    for (int q = 0; q < a.nosc; q++) {
+        const OscDev &o = a.osc[q];
+        if (i >= o.x1 && i < o.x2 && j >= o.y1 && j < o.y2 && k >= o.z1 && k < o.z2) {
+            const T v = (T)o.val[ostep - o.step0];
+            if (o.comp == 0) ex += v;
+            else if (o.comp == 1) ey += v;
+            else ez += v;
+        }
+    }
+    if (a.lm & 1) a.Ex.set(id, ex);
+    if (a.lm & 2) a.Ey.set(id, ey);
+    if (a.lm & 4) a.Ez.set(id, ez);
 }
 
 // CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
@@ -295,14 +308,11 @@ __device__ __forceinline__ void stage_mats(const EArgs<S, T> &a, MatL<T> *sm) {
     __syncthreads();
 }
 
-// 3D E pass.
+// 3D E pass over tile w of the grid.
 template <typename S, typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
-    extern __shared__ char lds_raw[];          // nmat entries, sized at launch
-    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
-    stage_mats(a, sm);
+__device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, int w, int ostep) {
     int bx, by, bz;
-    tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    tile_of(w, a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
     const int i = bx * BX + threadIdx.x;
     const int j = by * BY + threadIdx.y;
     if (i >= a.Nx || j >= a.Ny) return;
@@ -313,8 +323,12 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
     const int oim = (i == 0) ? a.Nx - 1 : -1;                    // (i-1) with wrap
     const int ojm = (j == 0) ? (a.Ny - 1) * a.Nx : -a.Nx;        // (j-1) with wrap
     const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
-    T hx_km = a.Hx[base + km0 * NxNy];
-    T hy_km = a.Hy[base + km0 * NxNy];
+    // components outside the load mask are identically zero (disabled fields,
+    // or differences along a one-cell axis) and are neither loaded nor stored
+    const int lm = a.lm;
+    const T zero = 0;
+    T hx_km = (lm & 8) ? a.Hx[base + km0 * NxNy] : zero;
+    T hy_km = (lm & 16) ? a.Hy[base + km0 * NxNy] : zero;
     const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
     const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
@@ -326,10 +340,14 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
     };
     auto load = [&](int id) {
         Ld v;
-        v.hx = a.Hx[id]; v.hy = a.Hy[id]; v.hz = a.Hz[id];
-        v.hz_jm = a.Hz[id + ojm]; v.hz_im = a.Hz[id + oim];
-        v.hy_im = a.Hy[id + oim]; v.hx_jm = a.Hx[id + ojm];
-        v.ex = a.Ex[id]; v.ey = a.Ey[id]; v.ez = a.Ez[id];
+        v.hx = zero; v.hx_jm = zero; v.hy = zero; v.hy_im = zero; v.hz = zero; v.hz_jm = zero; v.hz_im = zero;
+        v.ex = zero; v.ey = zero; v.ez = zero;
+        if (lm & 8) { v.hx = a.Hx[id]; v.hx_jm = a.Hx[id + ojm]; }
+        if (lm & 16) { v.hy = a.Hy[id]; v.hy_im = a.Hy[id + oim]; }
+        if (lm & 32) { v.hz = a.Hz[id]; v.hz_jm = a.Hz[id + ojm]; v.hz_im = a.Hz[id + oim]; }
+        if (lm & 1) v.ex = a.Ex[id];
+        if (lm & 2) v.ey = a.Ey[id];
+        if (lm & 4) v.ez = a.Ez[id];
         v.m = a.mat[id];
         return v;
     };
@@ -347,9 +365,50 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
         hx_km = cur.hx;
         hy_km = cur.hy;
         e_cell(a, sm[cur.m], cur.m, i, j, k, id, cur.ex, cur.ey, cur.ez, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y,
-               ikx, iky, nx_, ny_);
+               ikx, iky, nx_, ny_, ostep);
         cur = nxt;
     }
+}
+
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
+    extern __shared__ char lds_raw[];          // nmat entries, sized at launch
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    stage_mats(a, sm);
+    e_tile<S, T, BY, KZ>(a, sm, blockIdx.x, a.ostep);
+}
+
+// Workgroup -> grid of a batch: grid r owns workgroups [wg0[r], wg0[r+1]),
+// each range starting at a multiple of 8 so that the XCD fold of tile_of
+// holds inside every grid.
+__device__ __forceinline__ int batch_member(const int *wg0, int n) {
+    const int w = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wg0[mid] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// E pass of many independent grids in one launch (the batching form): step s
+// of a chunk that started at step tcur[r] of grid r; grids already at their
+// last step (tend) do nothing.
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_E_batch(const EArgs<S, T> *__restrict__ d,
+                                                           const int *__restrict__ wg0, int n,
+                                                           const int *__restrict__ tcur,
+                                                           const int *__restrict__ tend, int s) {
+    extern __shared__ char lds_raw[];
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    const int r = batch_member(wg0, n);
+    const EArgs<S, T> &a = d[r];
+    const int t = tcur[r] + s;
+    const int w = blockIdx.x - wg0[r];
+    if (t >= tend[r] || w >= a.nbx * a.nby * a.nbz) return;
+    stage_mats(a, sm);
+    e_tile<S, T, BY, KZ>(a, sm, w, t);
 }
 
 // E pass of a 1 x 1 x Nz column (the auxiliary grid of the TFSF source):
@@ -366,7 +425,7 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
     const int m = a.mat[k];
     const T zero = 0;
     e_cell(a, sm[m], m, 0, 0, k, k, a.Ex[k], a.Ey[k], a.Ez[k], zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero,
-           zero, a.pf.ikE[0][0], a.pf.ikE[1][0], -1, -1);
+           zero, a.pf.ikE[0][0], a.pf.ikE[1][0], -1, -1, a.ostep);
 }
 
 // ---------------------------------------------------------------- H cell
@@ -454,15 +513,15 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
             if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
         }
     }
-    a.Hx.set(id, hx);
-    a.Hy.set(id, hy);
-    a.Hz.set(id, hz);
+    if (a.lm & 8) a.Hx.set(id, hx);
+    if (a.lm & 16) a.Hy.set(id, hy);
+    if (a.lm & 32) a.Hz.set(id, hz);
 }
 
 template <typename S, typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
+__device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     int bx, by, bz;
-    tile_of(a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    tile_of(w, a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
     const int i = bx * BX + threadIdx.x;
     const int j = by * BY + threadIdx.y;
     if (i >= a.Nx || j >= a.Ny) return;
@@ -475,21 +534,44 @@ __global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
     const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
+    const int lm = a.lm;
+    const T zero = 0;
     // E(k+1) of plane k is E(k) of plane k+1: carried in registers
-    T ex_k = a.Ex[base + k0 * NxNy];
-    T ey_k = a.Ey[base + k0 * NxNy];
+    T ex_k = (lm & 1) ? a.Ex[base + k0 * NxNy] : zero;
+    T ey_k = (lm & 2) ? a.Ey[base + k0 * NxNy] : zero;
     for (int k = k0; k < k1; k++) {
         const int id = base + k * NxNy;
         const int kp = (k == a.Nz - 1) ? 0 : k + 1;
         const T ex = ex_k, ey = ey_k;
-        const T ex_kp = a.Ex[base + kp * NxNy];
-        const T ey_kp = a.Ey[base + kp * NxNy];
+        T ex_kp = zero, ey_kp = zero, ez = zero, ez_jp = zero, ez_ip = zero, ex_jp = zero, ey_ip = zero;
+        if (lm & 1) { ex_kp = a.Ex[base + kp * NxNy]; ex_jp = a.Ex[id + ojp]; }
+        if (lm & 2) { ey_kp = a.Ey[base + kp * NxNy]; ey_ip = a.Ey[id + oip]; }
+        if (lm & 4) { ez = a.Ez[id]; ez_jp = a.Ez[id + ojp]; ez_ip = a.Ez[id + oip]; }
         ex_k = ex_kp;
         ey_k = ey_kp;
-        const T ez = a.Ez[id];
-        h_cell(a, i, j, k, id, a.Hx[id], a.Hy[id], a.Hz[id], ey_kp - ey, ex_kp - ex, a.Ez[id + ojp] - ez,
-               a.Ez[id + oip] - ez, a.Ex[id + ojp] - ex, a.Ey[id + oip] - ey, ikx, iky, nx_, ny_);
+        const T hx = (lm & 8) ? a.Hx[id] : zero;
+        const T hy = (lm & 16) ? a.Hy[id] : zero;
+        const T hz = (lm & 32) ? a.Hz[id] : zero;
+        h_cell(a, i, j, k, id, hx, hy, hz, ey_kp - ey, ex_kp - ex, ez_jp - ez, ez_ip - ez, ex_jp - ex, ey_ip - ey, ikx,
+               iky, nx_, ny_);
     }
+}
+
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
+    h_tile<S, T, BY, KZ>(a, blockIdx.x);
+}
+
+template <typename S, typename T, int BY, int KZ>
+__global__ __launch_bounds__(BX *BY) void k_update_H_batch(const HArgs<S, T> *__restrict__ d,
+                                                           const int *__restrict__ wg0, int n,
+                                                           const int *__restrict__ tcur,
+                                                           const int *__restrict__ tend, int s) {
+    const int r = batch_member(wg0, n);
+    const HArgs<S, T> &a = d[r];
+    const int w = blockIdx.x - wg0[r];
+    if (tcur[r] + s >= tend[r] || w >= a.nbx * a.nby * a.nbz) return;
+    h_tile<S, T, BY, KZ>(a, w);
 }
 
 template <typename S, typename T>
@@ -508,14 +590,13 @@ struct SensDesc {
     long P, off;
 };
 
-// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) on one sensor plane.
+// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) at plane cell p of a
+// sensor, written to stage slot `slot`.
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
-                                                Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
-                                                int Nz, T *stage) {
-    const SensDesc d = sd[blockIdx.y];
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= d.P) return;
+__device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot, long Ptot, const Fld<S, T> &Ex,
+                                             const Fld<S, T> &Ey, const Fld<S, T> &Ez, const Fld<S, T> &Hx,
+                                             const Fld<S, T> &Hy, const Fld<S, T> &Hz, int Nx, int Ny, int Nz,
+                                             T *stage) {
     const int a = (int)(p % d.span1), b = (int)(p / d.span1);
     int i, j, k;
     if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
@@ -534,17 +615,49 @@ __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, lo
     for (int c = 0; c < 6; c++) stage[((size_t)slot * 6 + c) * Ptot + d.off + p] = v[c];
 }
 
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
+                                                Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
+                                                int Nz, T *stage) {
+    const SensDesc d = sd[blockIdx.y];
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.P) return;
+    sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, stage);
+}
+
+// One sensor of one grid of a batch: the grid's fields, its stage pool and
+// the batch member index (for the step window).
+template <typename S, typename T>
+struct SampB {
+    SensDesc d;
+    Fld<S, T> F[6];
+    int Nx, Ny, Nz, member;
+    long Ptot;
+    T *stage;
+};
+
+// Sensor sampling of every grid of a batch at chunk step s (stage slot s:
+// members join a batch with an empty stage at a multiple of KS steps).
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_sample_batch(const SampB<S, T> *__restrict__ e, const int *__restrict__ tcur,
+                                                      const int *__restrict__ tend, int s) {
+    const SampB<S, T> &q = e[blockIdx.y];
+    if (tcur[q.member] + s >= tend[q.member]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= q.d.P) return;
+    sample_point(q.d, p, s, q.Ptot, q.F[0], q.F[1], q.F[2], q.F[3], q.F[4], q.F[5], q.Nx, q.Ny, q.Nz, q.stage);
+}
+
 // Running DFT over a block of nfill <= KS staged steps (sensors/sensors.cpp:
 // 456-479): acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p], FP64.
 // One lane per (cell, component) keeps its KS samples in registers; the
 // coefficients of DFT_LCH wavelengths at a time are staged in LDS and read as
 // broadcasts, so each accumulator is read and written once per KS steps.
 template <typename T>
-__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
-                                             const double2 *coef, int s0, int nfill, double2 *acc) {
+__device__ __forceinline__ void dft_block(const T *stage, long Ptot, long off, long P, int Nl, const double2 *coef,
+                                          int s0, int nfill, double2 *acc, int c) {
     __shared__ double2 W[KS * DFT_LCH];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
     const int h = c >= 3 ? 1 : 0;
     const bool live = p < P;
     double x[KS];
@@ -573,6 +686,40 @@ __global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off
             }
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
+                                             const double2 *coef, int s0, int nfill, double2 *acc) {
+    dft_block(stage, Ptot, off, P, Nl, coef, s0, nfill, acc, blockIdx.y);
+}
+
+// One sensor of one grid of a batch for the DFT flush at the end of a chunk.
+template <typename T>
+struct DftB {
+    const T *stage;
+    long Ptot, off, P;
+    int Nl, member;
+    const double2 *coef;
+    double2 *acc;
+};
+
+// DFT flush of every sensor of a batch after a KS-step chunk that started at
+// tcur (each member staged min(KS, tend - tcur) steps).
+template <typename T>
+__global__ __launch_bounds__(256) void k_dft_batch(const DftB<T> *__restrict__ e, const int *__restrict__ tcur,
+                                                   const int *__restrict__ tend) {
+    const DftB<T> &q = e[blockIdx.z];
+    const int t0 = tcur[q.member];
+    const int nfill = min(KS, tend[q.member] - t0);
+    if (nfill <= 0 || (long)blockIdx.x * blockDim.x >= q.P) return;
+    dft_block(q.stage, q.Ptot, q.off, q.P, q.Nl, q.coef, t0, nfill, q.acc, blockIdx.y);
+}
+
+// End of a batch chunk: every member advances KS steps, stopping at its end.
+__global__ void k_batch_advance(int *tcur, const int *tend, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) tcur[r] = min(tcur[r] + KS, tend[r]);
 }
 
 // Spect_Poynting::treat (sensors/s_spec_power.cpp:61-111): one workgroup per

@@ -1,6 +1,7 @@
 // modes.cpp -- host mode drivers over the grid C ABI.  They replace Aether's
-// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) and
-// FDTD_single_particle (modes/single_particle.cpp:29-415): same setup, same
+// FDTD_normal_incidence (modes/normal_incidence.cpp:161-607),
+// FDTD_single_particle (modes/single_particle.cpp:29-415) and the custom mode
+// mode_default_fdtd (modes/default_fdtd.cpp:24-210): same setup, same
 // per-step sequence, same post-processing, with the fields resident on the
 // device and every per-step field access done by device passes.
 //
@@ -30,20 +31,28 @@ void lambda_grid(int Nl, double lmin, double lmax, std::vector<double> &L) {   /
 }  // namespace
 
 struct aefdtd_run {
-    int kind = 0;                     // 0 normal incidence, 1 single particle
+    int kind = 0;                     // 0 normal incidence, 1 single particle, 2 custom
     int Nt = 0, t = 0;
     int polarization = 0, Nl = 0;
     double lambda_min = 0, lambda_max = 0, Dt = 0;
     aefdtd_grid *g = nullptr, *a = nullptr;
     std::vector<aefdtd_sensor_req> reqs;
+    std::vector<aefdtd_source_req> srcs;
     std::vector<std::vector<int>> ids;
     int idR = -1, idT = -1;
     double index_sup = 1, index_sub = 1;
     std::string err;
+    struct aefdtd_batch *batch = nullptr;   // set while a member of a batch
     ~aefdtd_run() {
         aefdtd_destroy(g);
         aefdtd_destroy(a);
     }
+};
+
+struct aefdtd_batch {
+    Batch *b = nullptr;
+    std::vector<aefdtd_run *> runs;
+    std::string err;
 };
 
 namespace {
@@ -310,11 +319,74 @@ int setup_single(aefdtd_run *r, const aefdtd_mode *m) {
     return AEFDTD_OK;
 }
 
+// Setup of mode_default_fdtd up to the time loop (default_fdtd.cpp:24-98):
+// the grid as given by the finalized mode, then sensors and sources.
+int setup_custom(aefdtd_run *r, const aefdtd_mode *m) {
+    const double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
+    double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * kC) * 0.99 * m->time_mod;
+    r->Dt = Dt;
+    TRY(aefdtd_create(&r->g, m->nx, m->ny, m->nz, m->Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, m->pad,
+                      m->precision));
+    aefdtd_grid *g = r->g;
+    for (int f = 0; f < 6; f++) TRY(aefdtd_set_pml(g, f, m->kappa[f], m->sigma[f], m->alpha[f]));
+    TRY(aefdtd_set_tapering(g, m->tapering));
+    TRY(aefdtd_set_matsgrid(g, m->matsgrid, m->nx, m->ny, m->nz));
+    for (int i = 0; i < m->n_materials; i++) TRY(aefdtd_set_material(g, i, &m->materials[i]));
+    TRY(aefdtd_disable_fields(g, m->disable_fields, m->n_disable));
+    TRY(aefdtd_bootstrap(g));
+    TRY(add_requests(r));
+    for (const aefdtd_source_req &q : r->srcs) {
+        if (q.type != 0) {
+            g->err = "only oscillator sources are supported";
+            return AEFDTD_EINVAL;
+        }
+        int id;
+        TRY(aefdtd_add_oscillator(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, q.lambda_max,
+                                  &id));
+    }
+    return AEFDTD_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 void aefdtd_cancel(void) { g_cancel = true; }
+
+int aefdtd_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors) {
+    if (!m || kind < 0 || kind > 2 || (n_sensors > 0 && !sensors) || (m->n_sources > 0 && !m->sources))
+        return AEFDTD_EINVAL;
+    if (kind == 0) {                       // lua_fdtd.cpp:90-97
+        m->pml[0] = m->pml[1] = m->pml[2] = m->pml[3] = 0;
+        if (m->pml[4] <= 0) m->pml[4] = 25;
+        if (m->pml[5] <= 0) m->pml[5] = 25;
+    }
+    const int N[3] = {m->nx, m->ny, m->nz};
+    for (int a = 0; a < 3; a++) {          // lua_fdtd.cpp:99-202
+        if (N[a] <= 1) continue;
+        for (int side = 0; side < 2; side++) {
+            const int f = 2 * a + side;
+            if (!m->pml[f]) {
+                m->pad[f] = 0;
+                continue;
+            }
+            int p = std::max(m->pad[f], 5);
+            for (int i = 0; i < n_sensors; i++) {
+                const int lo[3] = {sensors[i].x1, sensors[i].y1, sensors[i].z1};
+                const int hi[3] = {sensors[i].x2, sensors[i].y2, sensors[i].z2};
+                p = std::max(p, side == 0 ? 5 - lo[a] : 5 + hi[a] - N[a]);
+            }
+            for (int i = 0; i < m->n_sources; i++) {
+                const aefdtd_source_req &q = m->sources[i];
+                const int lo[3] = {q.x1, q.y1, q.z1}, hi[3] = {q.x2, q.y2, q.z2};
+                const int margin = q.type == 0 ? 50 : 5;
+                p = std::max(p, side == 0 ? margin - lo[a] : margin + hi[a] - N[a]);
+            }
+            m->pad[f] = p;
+        }
+    }
+    return AEFDTD_OK;
+}
 
 int aefdtd_synchronize(aefdtd_grid *g) {
     if (!g) return AEFDTD_EINVAL;
@@ -329,7 +401,9 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
                       const aefdtd_sensor_req *sensors) {
     if (!out) return AEFDTD_EINVAL;
     *out = nullptr;
-    if (!m || !m->matsgrid || (kind != 0 && kind != 1) || m->Nt < 1 || (n_sensors > 0 && !sensors)) return AEFDTD_EINVAL;
+    if (!m || !m->matsgrid || kind < 0 || kind > 2 || m->Nt < 1 || (n_sensors > 0 && !sensors) ||
+        (m->n_sources > 0 && !m->sources))
+        return AEFDTD_EINVAL;
     if (kind == 0 && (m->Nt < 2 || m->Nl < 2)) return AEFDTD_EINVAL;
     auto *r = new aefdtd_run();
     r->kind = kind;
@@ -339,7 +413,8 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
     r->lambda_min = m->lambda_min;
     r->lambda_max = m->lambda_max;
     r->reqs.assign(sensors, sensors + std::max(0, n_sensors));
-    int rc = kind == 0 ? setup_normal(r, m) : setup_single(r, m);
+    if (kind == 2 && m->n_sources > 0) r->srcs.assign(m->sources, m->sources + m->n_sources);
+    int rc = kind == 0 ? setup_normal(r, m) : (kind == 1 ? setup_single(r, m) : setup_custom(r, m));
     if (rc) {
         // keep the failure text reachable through aefdtd_last_error(NULL)
         std::string e = r->g && !r->g->err.empty() ? r->g->err : (r->a && !r->a->err.empty() ? r->a->err : "");
@@ -353,6 +428,10 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
 
 int aefdtd_run_step(aefdtd_run *r, int n) {
     if (!r) return AEFDTD_EINVAL;
+    if (r->batch) {
+        r->err = "run is stepped by a batch";
+        return AEFDTD_ESTATE;
+    }
     for (int s = 0; s < n && r->t < r->Nt; s++) {
         int rc;
         if (r->kind == 1) {
@@ -361,7 +440,9 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
                 (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
                 return rfail(r, rc);
         } else {
-            // normal_incidence.cpp:325-415: E (+injection), H (+injection), sensors + R/T sums
+            // normal_incidence.cpp:325-415: E (+injection), H (+injection), sensors + R/T sums;
+            // default_fdtd.cpp:148-165: E (+sources' inject_E), H, sensors (inject_H of an
+            // oscillator only counts its step, which update_H does)
             if ((rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
                 return rfail(r, rc);
         }
@@ -384,6 +465,10 @@ const char *aefdtd_run_error(const aefdtd_run *r) { return r ? r->err.c_str() : 
 
 int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
     if (!r) return AEFDTD_EINVAL;
+    if (r->batch) {
+        r->err = "run is in a batch: remove it first";
+        return AEFDTD_ESTATE;
+    }
     int rc;
     if (r->kind == 0 && res && (rc = finish_normal(r, res))) return rfail(r, rc);
     if ((rc = collect_requests(r, sensor_out))) return rfail(r, rc);
@@ -392,6 +477,7 @@ int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res, double *sensor_o
 }
 
 int aefdtd_run_destroy(aefdtd_run *r) {
+    if (r && r->batch) aefdtd_batch_remove(r->batch, r);
     delete r;
     return AEFDTD_OK;
 }
@@ -409,6 +495,138 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
     bool cancelled = rc == AEFDTD_ECANCELLED;
     if (cancelled) { r->t -= 1; rc = AEFDTD_OK; }
     if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, res, sensor_out, fields_out);
+    aefdtd_run_destroy(r);
+    if (rc) return rc;
+    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
+
+// ---- batches of custom-mode runs
+int aefdtd_batch_create(aefdtd_batch **out, int precision) {
+    if (!out) return AEFDTD_EINVAL;
+    *out = nullptr;
+    Batch *b = nullptr;
+    TRY(batch_create(&b, precision));
+    *out = new aefdtd_batch();
+    (*out)->b = b;
+    return AEFDTD_OK;
+}
+
+int aefdtd_batch_destroy(aefdtd_batch *b) {
+    if (!b) return AEFDTD_OK;
+    for (aefdtd_run *r : b->runs) r->batch = nullptr;
+    batch_destroy(b->b);
+    delete b;
+    return AEFDTD_OK;
+}
+
+const char *aefdtd_batch_error(const aefdtd_batch *b) {
+    if (!b) return "";
+    return b->err.empty() ? batch_error(b->b) : b->err.c_str();
+}
+
+int aefdtd_batch_add(aefdtd_batch *b, aefdtd_run *r) {
+    if (!b || !r) return AEFDTD_EINVAL;
+    b->err.clear();
+    if (r->kind != 2) {
+        b->err = "only custom-mode runs are batched";
+        return AEFDTD_ESTATE;
+    }
+    if (r->batch) {
+        b->err = "run is already in a batch";
+        return AEFDTD_ESTATE;
+    }
+    TRY(batch_add(b->b, r->g));
+    b->runs.push_back(r);
+    r->batch = b;
+    return AEFDTD_OK;
+}
+
+int aefdtd_batch_remove(aefdtd_batch *b, aefdtd_run *r) {
+    if (!b || !r) return AEFDTD_EINVAL;
+    b->err.clear();
+    auto it = std::find(b->runs.begin(), b->runs.end(), r);
+    if (it == b->runs.end()) {
+        b->err = "run is not in the batch";
+        return AEFDTD_EINVAL;
+    }
+    TRY(batch_remove(b->b, r->g));
+    b->runs.erase(it);
+    r->batch = nullptr;
+    return AEFDTD_OK;
+}
+
+int aefdtd_batch_step(aefdtd_batch *b, int chunks) {
+    if (!b) return AEFDTD_EINVAL;
+    b->err.clear();
+    TRY(batch_step(b->b, chunks));
+    for (aefdtd_run *r : b->runs) r->t = r->g->tstep;
+    return AEFDTD_OK;
+}
+
+int aefdtd_batch_running(const aefdtd_batch *b) {
+    if (!b) return 0;
+    int n = 0;
+    for (const aefdtd_run *r : b->runs) n += r->t < r->Nt;
+    return n;
+}
+
+int aefdtd_batch_synchronize(aefdtd_batch *b) {
+    if (!b) return AEFDTD_EINVAL;
+    return batch_synchronize(b->b);
+}
+
+int aefdtd_batch_set_tiles(aefdtd_batch *b, int tile_e, int tile_h) {
+    if (!b) return AEFDTD_EINVAL;
+    return batch_set_tiles(b->b, tile_e, tile_h);
+}
+
+// Candidates: every (BY, remap) with the smallest KZ when all members are
+// one cell thick in z (KZ is then irrelevant), else all 32 codes; E and H
+// share a code.  Each candidate times one chunk of the real run.
+int aefdtd_batch_autotune(aefdtd_batch *b, int out[2]) {
+    if (!b) return AEFDTD_EINVAL;
+    bool flat = true;
+    for (aefdtd_run *r : b->runs) flat = flat && r->g->Nz == 1;
+    std::vector<int> cand;
+    for (int c = 0; c < 32; c++)
+        if (!flat || ((c >> 2) & 3) == 0) cand.push_back(c);
+    double best = 1e30;
+    int best_c = -1;
+    for (int c : cand) {
+        if (aefdtd_batch_running(b) == 0) break;
+        TRY(batch_set_tiles(b->b, c, c));
+        TRY(aefdtd_batch_step(b, 1));      // captures the graph; its first replay also uploads it
+        if (aefdtd_batch_running(b) == 0) break;
+        double ms = 0;
+        long n = 0;
+        batch_profile(b->b, 1, nullptr, nullptr);
+        TRY(aefdtd_batch_step(b, 1));
+        batch_profile(b->b, 0, &ms, &n);
+        if (n > 0 && ms < best) { best = ms; best_c = c; }
+    }
+    if (best_c < 0) best_c = 2 | (1 << 2);
+    TRY(batch_set_tiles(b->b, best_c, best_c));
+    if (out) { out[0] = best_c; out[1] = best_c; }
+    return AEFDTD_OK;
+}
+
+int aefdtd_batch_profile(aefdtd_batch *b, int on, double *ms, long *chunks) {
+    if (!b) return AEFDTD_EINVAL;
+    batch_profile(b->b, on, ms, chunks);
+    return AEFDTD_OK;
+}
+
+// mode_default_fdtd (modes/default_fdtd.cpp:24-210) in one call.
+int aefdtd_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors, double *sensor_out,
+                      double *fields_out) {
+    g_cancel = false;
+    aefdtd_run *r = nullptr;
+    TRY(aefdtd_run_create(&r, 2, m, n_sensors, sensors));
+    int rc = AEFDTD_OK;
+    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    bool cancelled = rc == AEFDTD_ECANCELLED;
+    if (cancelled) rc = AEFDTD_OK;
+    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);
     aefdtd_run_destroy(r);
     if (rc) return rc;
     return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;

@@ -16,7 +16,8 @@ import math
 
 import numpy as np
 
-from .capi import FP32, FP64, Material, ModeSpec, SensorReq
+from .capi import (EX, EY, EZ, FP32, FP64, HX, HY, HZ, NORMAL_X, NORMAL_Y, NORMAL_Z, Material, ModeSpec, SensorReq,
+                   SourceReq)
 
 
 class MT19937_64:
@@ -170,3 +171,81 @@ def multilayer_normal_incidence(n=(512, 512, 384), D=4e-9, seed=3, nlayers=12, m
     spec = ModeSpec(matsgrid=g, materials=mats, Nt=Nt, Dx=D, Dy=D, Dz=D, pml=(0, 0, 0, 0, pml_z, pml_z), Nl=Nl,
                     lambda_min=400e-9, lambda_max=800e-9, polarization=1, precision=precision)
     return spec, {"thickness_scale": scale, "edges": edges, "lorentz_layer": lor_pos}
+
+
+# ---------------------------------------------------------------- 2D sweeps
+def random_cylinders(rng, n, fill, rmin=3.0, rmax=15.0, max_tries_factor=40):
+    """Rejection-packed disks (cylinders along z) on an n x n cell grid: radius
+    U[rmin, rmax] cells, centre U[r, n-r]^2, accepted when the disk grown by
+    one cell covers no cell of an earlier disk; stops at the fill fraction or
+    after max_tries_factor times the expected disk count of draws.  Returns
+    the (n, n) boolean mask (corner sampling: cell (i,j) is inside when
+    (i-x0)^2+(j-y0)^2 <= r^2) and the disk list."""
+    occ = np.zeros((n, n), dtype=bool)
+    target = fill * n * n
+    mean_area = math.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3.0
+    max_tries = int(max_tries_factor * target / mean_area) + 100
+    area = 0
+    disks = []
+    for _ in range(max_tries):
+        if area >= target:
+            break
+        r = rng.uniform(rmin, rmax)
+        x0 = rng.uniform(r, n - r)
+        y0 = rng.uniform(r, n - r)
+        R = r + 1.0
+        i0, i1 = max(0, int(math.floor(x0 - R))), min(n, int(math.ceil(x0 + R)) + 1)
+        j0, j1 = max(0, int(math.floor(y0 - R))), min(n, int(math.ceil(y0 + R)) + 1)
+        dx = np.arange(i0, i1) - x0
+        dy = np.arange(j0, j1) - y0
+        d2 = dy[:, None] ** 2 + dx[None, :] ** 2
+        if occ[j0:j1, i0:i1][d2 <= R * R].any():
+            continue
+        inside = d2 <= r * r
+        occ[j0:j1, i0:i1] |= inside
+        area += int(inside.sum())
+        disks.append((x0, y0, r))
+    return occ, disks
+
+
+def sweep_steps(side, lo=256, hi=2048):
+    """Time steps of a sweep run: 4000 at side 256 rising linearly to 12000 at 2048."""
+    return int(round(4000 + 8000 * (side - lo) / (hi - lo)))
+
+
+def sweep_run(rng, index, D=10e-9, pml=16, Nl=64, lambda_min=500e-9, lambda_max=1000e-9, precision=FP32,
+              side=None, steps=None):
+    """One run of the 2D sweep: an n x n x 1 structure (n log-uniform in
+    [256, 2048]), dielectric cylinders (eps U[2,12] per run, fill U[20%,50%]),
+    TE (Ex,Ey,Hz) or TM (Ez,Hx,Hy) chosen at random, 16-cell CPML on the four
+    in-plane faces, an oscillator line source along x at the bottom edge of
+    the structure and a planar spectral-Poynting line sensor along x at its
+    top edge.  The spec is not yet finalized (pads come from mode_finalize)."""
+    n = side if side is not None else int(round(math.exp(rng.uniform(math.log(256), math.log(2048)))))
+    eps = rng.uniform(2.0, 12.0)
+    fill = rng.uniform(0.2, 0.5)
+    tm = rng.uniform() < 0.5
+    occ, disks = random_cylinders(rng, n, fill)
+    g = occ.astype(np.uint32).reshape(1, n, n)
+    nt = steps if steps is not None else sweep_steps(n)
+    if tm:
+        disable, orient = (EX, EY, HZ), NORMAL_Z
+    else:
+        disable, orient = (EZ, HX, HY), NORMAL_X
+    src = SourceReq(orientation=orient, x1=0, x2=n, y1=0, y2=1, z1=0, z2=1,
+                    lambda_min=lambda_min, lambda_max=lambda_max)
+    sens = SensorReq(box=False, orientation=NORMAL_Y, x1=0, x2=n, y1=n - 1, y2=n, z1=0, z2=1, Nl=Nl,
+                     lambda_min=lambda_min, lambda_max=lambda_max)
+    spec = ModeSpec(matsgrid=g, materials=[Material(1.0), Material(eps)], Nt=nt, Dx=D, Dy=D, Dz=D,
+                    pml=(pml, pml, pml, pml, 0, 0), pad=(0,) * 6, Nl=Nl, lambda_min=lambda_min,
+                    lambda_max=lambda_max, disable_fields=disable, precision=precision, sources=[src])
+    info = {"index": index, "side": n, "eps": eps, "fill_target": fill, "fill": float(occ.mean()),
+            "cylinders": len(disks), "pol": "TM" if tm else "TE", "steps": nt}
+    return spec, [sens], info
+
+
+def sweep_runs(count=64, seed=4, precision=FP32, **kw):
+    """The batching workload: `count` independent 2D runs drawn from one
+    mt19937_64 stream (seed 4)."""
+    rng = MT19937_64(seed)
+    return [sweep_run(rng, q, precision=precision, **kw) for q in range(count)]

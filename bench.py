@@ -11,7 +11,7 @@ no files.
 
 A "step" is one full time step of the single-particle driver loop: auxiliary
 E, main E (stencil + TFSF + RC + CPML), auxiliary H, main H, sensor sample and
-(every 32 steps) the staged DFT.  value = full-grid cells x steps / seconds.
+(every 64 steps) the staged DFT.  value = full-grid cells x steps / seconds.
 
 Also reported: the roofline of the dominant kernel (E pass) from HIP events
 over the timed region, and the CPU baseline: the oracle (this project's C++
@@ -72,6 +72,18 @@ def h_pass_bytes(lay, pml, stats):
     return b
 
 
+def pmc_traffic(prec, n):
+    """HBM bytes per k_update_E launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE runs, FETCH scaled by the copy-kernel calibration factor) on this
+    workload; read from the committed counter summary, None when there is none."""
+    path = os.path.join(ROOT, "profiles", "k_update_E_single_particle_256_fp32_pmc.txt")
+    if prec == 1 and n == 256 and os.path.exists(path):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":")[1]))
+    return None
+
+
 def cpu_baseline(spec, sensors, steps_lo, steps_hi, threads):
     """Times the oracle's single-particle driver at two run lengths on the same
     workload in FP64; the difference is the stepping cost (setup cancels)."""
@@ -125,6 +137,91 @@ def precision_form(args):
     print(json.dumps(rec))
 
 
+def batching_form(args):
+    """The batching form: 64 independent 2D TE/TM runs (seed 4; sides
+    log-uniform in 256..2048, dielectric cylinders, 16-cell CPML, oscillator
+    line source, 4000-12000 steps, 64-wavelength transmission line sensor).
+    Timed: (a) all runs in one FP32 batch (one launch per half step for every
+    run, 64-step chunks replayed from a HIP graph, tiles chosen by the online
+    tuner inside the timed region); (b) the same runs one at a time in FP64
+    and (c) one at a time in FP32, each on the same GPU.  Device buffers are
+    allocated and inputs uploaded before each timed region; the timed region
+    covers stepping and the spectra.  runs/s = runs / wall seconds."""
+    import numpy as np
+    from aether_amd import capi, workloads
+    be = capi.engine()
+    t0 = time.perf_counter()
+    runs = workloads.sweep_runs(args.runs, seed=4, precision=capi.FP32)
+    specs = [(capi.mode_finalize(be, s, capi.KIND_CUSTOM, q), q, info) for s, q, info in runs]
+    gen_s = time.perf_counter() - t0
+    print(f"batching form: {len(specs)} runs generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    cell_steps = 0
+    for spec, _, _ in specs:
+        Nx, Ny, Nz = capi.custom_dims(spec)
+        cell_steps += Nx * Ny * Nz * spec.Nt
+
+    def with_prec(spec, prec):
+        return capi.ModeSpec(**{**spec.__dict__, "precision": prec})
+
+    out = {}
+    # (a) one FP32 batch
+    objs = [capi.Run(be, capi.KIND_CUSTOM, with_prec(sp, capi.FP32), q) for sp, q, _ in specs]
+    b = capi.Batch(be, capi.FP32)
+    for r in objs:
+        b.add(r)
+    b.synchronize()
+    t0 = time.perf_counter()
+    tiles = b.autotune()
+    while b.running():
+        b.step(8)
+    b.synchronize()
+    res_b = []
+    for r in objs:
+        b.remove(r)
+        res_b.append(r.finish()["sensors"])
+    wall = time.perf_counter() - t0
+    for r in objs:
+        r.close()
+    b.close()
+    out["batched_fp32"] = {"wall_s": round(wall, 3), "runs_per_s": round(len(specs) / wall, 3),
+                           "Mcell_updates_per_s": round(cell_steps / wall / 1e6, 1), "tiles": tiles}
+    print(f"batched fp32: {wall:.2f}s", file=sys.stderr, flush=True)
+
+    def one_at_a_time(prec):
+        tot = 0.0
+        res = []
+        for sp, q, _ in specs:
+            r = capi.Run(be, capi.KIND_CUSTOM, with_prec(sp, prec), q)
+            r.synchronize()
+            t1 = time.perf_counter()
+            r.step(sp.Nt)
+            res.append(r.finish()["sensors"])
+            tot += time.perf_counter() - t1
+            r.close()
+        return tot, res
+
+    for name, prec in (("alone_fp64", capi.FP64), ("alone_fp32", capi.FP32)):
+        wall_a, res = one_at_a_time(prec)
+        out[name] = {"wall_s": round(wall_a, 3), "runs_per_s": round(len(specs) / wall_a, 3),
+                     "Mcell_updates_per_s": round(cell_steps / wall_a / 1e6, 1)}
+        if prec == capi.FP64:
+            res64 = res
+        else:
+            res32 = res
+        print(f"{name}: {wall_a:.2f}s", file=sys.stderr, flush=True)
+    err_b = [float(np.max(np.abs(a - r)) / np.max(np.abs(r))) for a, r in zip(res_b, res64)]
+    same = [float(np.max(np.abs(a - r)) / np.max(np.abs(r))) for a, r in zip(res_b, res32)]
+    rec = {"form": "batching",
+           "workload": f"{len(specs)} 2D TE/TM runs (seed 4), sides 256-2048 log-uniform, cylinders, "
+                       f"16-cell CPML, oscillator line source, 4000-12000 steps, 64-lambda line sensor",
+           "cell_steps": cell_steps, "setup_s": round(gen_s, 1), **out,
+           "speedup_vs_alone_fp64": round(out["alone_fp64"]["wall_s"] / out["batched_fp32"]["wall_s"], 3),
+           "speedup_vs_alone_fp32": round(out["alone_fp32"]["wall_s"] / out["batched_fp32"]["wall_s"], 3),
+           "spectral_err_vs_fp64": {"max": max(err_b), "median": float(np.median(err_b))},
+           "batched_vs_alone_fp32_max_rel_diff": max(same)}
+    print(json.dumps(rec))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,10 +231,15 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="structure cells per side")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
-    ap.add_argument("--cpu-steps", type=int, default=16)
-    ap.add_argument("--form", default="plain", choices=["plain", "precision"],
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="CPU baseline sample: (2 + S)-step run minus 2-step run of the oracle")
+    ap.add_argument("--tiles", type=int, nargs=2, default=None, metavar=("E", "H"),
+                    help="fixed tile configurations (0..31) instead of the autotuner's")
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching"],
                     help="plain: the headline single-particle run; precision: the 512x512x384 multilayer "
-                         "stack in FP64, FP32 and FP16 storage with spectral errors against FP64")
+                         "stack in FP64, FP32 and FP16 storage with spectral errors against FP64; "
+                         "batching: 64 2D runs batched in FP32 against one-at-a-time FP64 and FP32")
+    ap.add_argument("--runs", type=int, default=64, help="runs of the batching form")
     ap.add_argument("--nt", type=int, default=8000, help="time steps of the precision form")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (rocprof counter calibration)")
@@ -162,6 +264,8 @@ def main():
     from aether_amd import capi, workloads
     if args.form == "precision":
         return precision_form(args)
+    if args.form == "batching":
+        return batching_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
     nsteps = args.warmup + args.steps
     t_setup = time.perf_counter()
@@ -173,7 +277,11 @@ def main():
     ncell = stats[0]
 
     run.step(args.warmup)
-    tiles = run.autotune(3) if not args.no_tune else None
+    if args.tiles is not None:
+        run.set_tiles(*args.tiles)
+        tiles = list(args.tiles)
+    else:
+        tiles = run.autotune(3) if not args.no_tune else None
     run.synchronize()
     run.profile(True)
     t0 = time.perf_counter()
@@ -214,7 +322,7 @@ def main():
                    "cells": ncell, "dispersive_cells": stats[1], "poles": stats[2],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(prec, args.n),
                      "kernel": "k_update_E", "bytes_per_launch": e_bytes, "avg_launch_ms": round(e_avg * 1e3, 5)},
         "kernels_ms_avg": {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5),
                            "sensor": round(s_ms / max(1, s_n), 5), "dft": round(d_ms / max(1, d_n), 5),

@@ -59,6 +59,7 @@ extern "C" {
 #define AEFDTD_NORMAL_YM  3
 #define AEFDTD_NORMAL_Z   4
 #define AEFDTD_NORMAL_ZM  5
+#define AEFDTD_NORMAL_UNDEF 6
 
 /* A material as Aether's FDTD sees it: eps_inf plus pole lists.
  * Mirrors the FDTD-compatible part of `Material` (src/materials/material.h)
@@ -149,6 +150,18 @@ int aefdtd_set_tfsf(aefdtd_grid *g, const aefdtd_grid *aux,
                     int i1, int i2, int j1, int j2, int k1, int k2,
                     int periodic_x, int periodic_y);
 
+/* Oscillator source: the object generate_fdtd_source builds for
+ * SOURCE_GEN_OSCILLATOR (sources/sources.cpp:323-348), Oscillator
+ * (sources/sc_oscillator.cpp:20-84).  After every update_E, the field
+ * normal to `orientation` (NORMAL_X/XM -> Ex, Y -> Ey, Z -> Ez; NORMAL_UNDEF
+ * -> nothing) gets Re(e^{-i w0 s Dt} e^{-(dw/2 (s-tshift) Dt)^2})*50 added on
+ * [x1,x2)x[y1,y2)x[z1,z2), s = update_H calls since this call (Source::step,
+ * sources.cpp:240-249).  Location in structure cells, offset and clamped as
+ * Source::link (sources.cpp:251-283); w0, dw, tshift from lambda_min/max as
+ * Oscillator::initialize.  At most 8 per grid.  Writes the source id. */
+int aefdtd_add_oscillator(aefdtd_grid *g, int orientation, int x1, int x2, int y1, int y2,
+                          int z1, int z2, double lambda_min, double lambda_max, int *id);
+
 /* Planar spectral Poynting sensor, Spect_Poynting (sensors/s_spec_power.cpp:
  * 32-119) over SensorFieldHolder with interpolation (sensors/sensors.cpp:
  * 423-644). Location in structure cells, offset and clamped as Sensor::link
@@ -205,8 +218,19 @@ typedef struct aefdtd_mode {
     int     n_disable;  const int *disable_fields;
     int     nx, ny, nz;  const uint32_t *matsgrid;   /* Structure::discretize */
     int     n_materials; const aefdtd_material *materials;
-    int     precision;           /* AEFDTD_FP64 / AEFDTD_FP32              */
+    int     precision;           /* AEFDTD_FP64 / AEFDTD_FP32 / AEFDTD_FP16 */
+    int     n_sources;  const struct aefdtd_source_req *sources;   /* custom mode */
 } aefdtd_mode;
+
+/* A source request (Source_generator, core/sources.h:29-59, already
+ * to_discrete'd: structure cells).  type 0 = oscillator. */
+typedef struct aefdtd_source_req {
+    int    type;
+    int    orientation;
+    int    x1, x2, y1, y2, z1, z2;
+    double lambda_min, lambda_max;
+} aefdtd_source_req;
+
 
 /* A box or planar spectral Poynting sensor request (Sensor_generator,
  * sensors/sensors.cpp:27-161, already to_discrete'd: structure cells). */
@@ -218,6 +242,14 @@ typedef struct aefdtd_sensor_req {
     int    Nl;
     double lambda_min, lambda_max;
 } aefdtd_sensor_req;
+
+/* The sizing part of FDTD_Mode::finalize (modes/lua_fdtd.cpp:74-203) for
+ * mode type `kind` (0 normal incidence, 1 single particle, 2 custom): normal
+ * incidence drops the x/y PML and defaults the z PML to 25 cells; on every
+ * axis longer than one cell a PML face gets a pad of at least 5 cells, more
+ * when a sensor (5) or an oscillator source (50) comes near it, and a face
+ * without PML gets no pad.  Updates m->pml and m->pad in place. */
+int aefdtd_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors);
 
 /* Results of the normal-incidence driver (normal_incidence.cpp:442-570):
  * arrays of Nl. R,T are the `spectdata_norm` columns; refx..transy the
@@ -244,13 +276,20 @@ int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors,
                                const aefdtd_sensor_req *sensors,
                                double *sensor_out, double *fields_out);
 
+/* mode_default_fdtd (modes/default_fdtd.cpp:24-210), the custom mode, on the
+ * device: m->sources (oscillators) and the sensor requests, loop update_E,
+ * sources' inject_E, update_H, sensors' feed, sources' inject_H for Nt steps
+ * (fixed-time runs).  sensor_out as above. */
+int aefdtd_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                      double *sensor_out, double *fields_out);
+
 /* Cancels a running mode driver between steps (the end_computation poll of
  * normal_incidence.cpp:431-435). Safe to call from another host thread. */
 void aefdtd_cancel(void);
 
 /* The same drivers split into create / step / finish, so that a caller can
  * interleave several runs (serving) or time the stepping alone. kind 0 =
- * normal incidence, 1 = single particle. run_step advances up to n steps
+ * normal incidence, 1 = single particle, 2 = custom. run_step advances up to n steps
  * (never past Nt); run_finish performs the driver's post-processing (res is
  * used by kind 0 only and may be NULL for kind 1). */
 typedef struct aefdtd_run aefdtd_run;
@@ -265,6 +304,38 @@ int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res,
 int aefdtd_run_destroy(aefdtd_run *r);
 /* Waits for all work queued on the grid's stream. */
 int aefdtd_synchronize(aefdtd_grid *g);
+
+/* ---- batches: many independent custom-mode runs stepped together -------- */
+/* No counterpart in the reference, which steps one FDTD object at a time (its
+ * only multi-grid loop is the lockstep pair of modes/oblique_biphase.cpp:
+ * 251-292).  Each member advances exactly as aefdtd_run_step would advance
+ * it alone; the batch issues one launch per half step for all members and
+ * replays chunks of 64 steps from a captured HIP graph.
+ *  - Members are kind-2 (custom) runs of the batch's precision without
+ *    dispersive materials, added at step 0 (or any multiple of 64).
+ *  - batch_step(b, c) advances every member by up to 64*c steps (never past
+ *    its Nt); runs that reach Nt idle until removed.  A member must not be
+ *    stepped on its own while in the batch; remove it, then finish it.
+ *  - Membership changes take effect at the next batch_step (the graph is
+ *    captured again). */
+typedef struct aefdtd_batch aefdtd_batch;
+int aefdtd_batch_create(aefdtd_batch **out, int precision);
+int aefdtd_batch_destroy(aefdtd_batch *b);
+const char *aefdtd_batch_error(const aefdtd_batch *b);
+int aefdtd_batch_add(aefdtd_batch *b, aefdtd_run *r);
+int aefdtd_batch_remove(aefdtd_batch *b, aefdtd_run *r);
+int aefdtd_batch_step(aefdtd_batch *b, int chunks);
+/* Members with steps left. */
+int aefdtd_batch_running(const aefdtd_batch *b);
+int aefdtd_batch_synchronize(aefdtd_batch *b);
+/* Tile configurations (codes as aefdtd_set_tiles) of the batched passes. */
+int aefdtd_batch_set_tiles(aefdtd_batch *b, int tile_e, int tile_h);
+/* Online tuner: advances the batch one chunk per candidate tile shape (all
+ * shapes compute the same steps), keeps the fastest; out[2] = chosen codes. */
+int aefdtd_batch_autotune(aefdtd_batch *b, int out[2]);
+/* Device time of aefdtd_batch_step (HIP events on the batch stream): on=1
+ * resets and starts, 0 stops, -1 only reads. */
+int aefdtd_batch_profile(aefdtd_batch *b, int on, double *ms, long *chunks);
 
 /* ---- introspection used by the benchmark -------------------------------- */
 /* Per-kernel device time accumulated on the handle's stream with HIP events

@@ -306,6 +306,15 @@ struct SpectSensor {               // Spect_Poynting over SensorFieldHolder (int
     std::vector<cplx> sp[6];       // (span1, span2, Nl) x-fastest
 };
 
+// Oscillator (sources/sc_oscillator.cpp:20-84) with the Source base members
+// it uses (core/sources.h:219-233: tshift is an int).
+struct Oscillator {
+    int type;                      // field index 0..2 after initialize's XM->X folding
+    int x1, x2, y1, y2, z1, z2;
+    int step = 0, tshift = 0;
+    double w0 = 0, dw = 0, Dt = 0;
+};
+
 struct PlaneSum {
     int k;
     bool has_inc, subtract;
@@ -343,6 +352,7 @@ struct aeoracle_grid {
     bool tf_per_x = false, tf_per_y = false;
     std::vector<SpectSensor *> sensors;
     std::vector<PlaneSum *> sums;
+    std::vector<Oscillator> oscs;
     Pool *pool = nullptr;
     std::string err;
 
@@ -856,6 +866,14 @@ void aeoracle_grid::update_E() {
                 Ey(i, j, inj_k) += dy;
             }
     }
+    // sources' inject_E (default_fdtd.cpp:158-159): Oscillator::deep_inject_E (sc_oscillator.cpp:34-67)
+    for (Oscillator &o : oscs) {
+        cplx im_val = std::exp(-o.w0 * o.step * o.Dt * Im) * std::exp(-std::pow(o.dw / 2.0 * ((o.step - o.tshift) * o.Dt), 2.0));
+        double val = std::real(im_val) * 50;
+        for (int i = o.x1; i < o.x2; i++)
+            for (int j = o.y1; j < o.y2; j++)
+                for (int k = o.z1; k < o.z2; k++) F[o.type][id(i, j, k)] += val;
+    }
 }
 
 // update_H: threaded_process_H (fdtd_threads.cpp:165-235), then the driver's
@@ -902,6 +920,9 @@ void aeoracle_grid::update_H() {
     }
     if (tfsf_aux) tfsf_H();
     tstep += 1;
+    // sources' inject_H: an oscillator only counts its step (sources.cpp:240-249);
+    // the sensors' feed in between does not read it
+    for (Oscillator &o : oscs) o.step += 1;
 }
 
 // local_Ex..local_Hz (fdtd_utils.cpp:457-507); c = 0..5
@@ -1247,6 +1268,32 @@ int aeoracle_add_spect_poynting(aeoracle_grid *g, int type, int x1, int x2, int 
     return AEFDTD_OK;
 }
 
+// generate_fdtd_source for SOURCE_GEN_OSCILLATOR (sources/sources.cpp:323-348):
+// Oscillator ctor, set_spectrum, Source::link (offset by xs_s.., clamp, :251-283)
+// and Oscillator::initialize (sc_oscillator.cpp:70-84).
+int aeoracle_add_oscillator(aeoracle_grid *g, int type, int x1, int x2, int y1, int y2, int z1, int z2,
+                            double lambda_min, double lambda_max, int *id) {
+    *id = -1;
+    if (type < 0 || type > 6) return set_err(g, AEFDTD_EINVAL, "bad oscillator orientation");
+    if (type == 6) return AEFDTD_OK;
+    Oscillator o;
+    o.type = type / 2;
+    o.x1 = std::clamp(x1 + g->xs_s, 0, g->Nx); o.x2 = std::clamp(x2 + g->xs_s, 0, g->Nx);
+    o.y1 = std::clamp(y1 + g->ys_s, 0, g->Ny); o.y2 = std::clamp(y2 + g->ys_s, 0, g->Ny);
+    o.z1 = std::clamp(z1 + g->zs_s, 0, g->Nz); o.z2 = std::clamp(z2 + g->zs_s, 0, g->Nz);
+    o.Dt = g->Dt;
+    double alp = std::sqrt(0.1);
+    double bet = 1e-7;
+    double w1 = 2.0 * Pi * c_light / lambda_max;    // m_to_rad_Hz (common/phys_tools.cpp:159)
+    double w2 = 2.0 * Pi * c_light / lambda_min;
+    o.w0 = (w1 + w2) / 2.0;
+    o.dw = std::abs(w2 - w1) / (2.0 * std::sqrt(-std::log(alp)));
+    o.tshift = std::sqrt(-4.0 / (o.dw * o.dw) * std::log(bet)) / o.Dt;
+    g->oscs.push_back(o);
+    *id = (int)g->oscs.size() - 1;
+    return AEFDTD_OK;
+}
+
 int aeoracle_feed_sensors(aeoracle_grid *g) {
     for (auto *s : g->sensors) sensor_feed(*g, *s);
     for (auto *s : g->sums) plane_sum_feed(*g, *s);
@@ -1583,6 +1630,85 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
 }
 
 // FDTD_single_particle, modes/single_particle.cpp:29-415
+// FDTD_Mode::finalize (modes/lua_fdtd.cpp:74-203), the part that sizes the
+// grid, written out axis by axis as the reference does.
+int aeoracle_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors) {
+    if (!m) return AEFDTD_EINVAL;
+    const int Nx = m->nx, Ny = m->ny, Nz = m->nz;
+    if (kind == 0) {
+        m->pml[0] = m->pml[1] = 0;
+        m->pml[2] = m->pml[3] = 0;
+        if (m->pml[4] <= 0) m->pml[4] = 25;
+        if (m->pml[5] <= 0) m->pml[5] = 25;
+    }
+    auto near = [&](int &pad, int pml, auto lo_of, auto src_lo_of) {
+        if (pml) {
+            pad = std::max(pad, 5);
+            for (int i = 0; i < n_sensors; i++) pad = std::max(pad, 5 - lo_of(sensors[i]));
+            for (int i = 0; i < m->n_sources; i++) {
+                if (m->sources[i].type == 0) pad = std::max(pad, 50 - src_lo_of(m->sources[i]));
+                else pad = std::max(pad, 5 - src_lo_of(m->sources[i]));
+            }
+        } else pad = 0;
+    };
+    auto far = [&](int &pad, int pml, int N, auto hi_of, auto src_hi_of) {
+        if (pml) {
+            pad = std::max(pad, 5);
+            for (int i = 0; i < n_sensors; i++) pad = std::max(pad, 5 + hi_of(sensors[i]) - N);
+            for (int i = 0; i < m->n_sources; i++) {
+                if (m->sources[i].type == 0) pad = std::max(pad, 50 + src_hi_of(m->sources[i]) - N);
+                else pad = std::max(pad, 5 + src_hi_of(m->sources[i]) - N);
+            }
+        } else pad = 0;
+    };
+    if (Nx > 1) {
+        near(m->pad[0], m->pml[0], [](const aefdtd_sensor_req &q) { return q.x1; }, [](const aefdtd_source_req &q) { return q.x1; });
+        far(m->pad[1], m->pml[1], Nx, [](const aefdtd_sensor_req &q) { return q.x2; }, [](const aefdtd_source_req &q) { return q.x2; });
+    }
+    if (Ny > 1) {
+        near(m->pad[2], m->pml[2], [](const aefdtd_sensor_req &q) { return q.y1; }, [](const aefdtd_source_req &q) { return q.y1; });
+        far(m->pad[3], m->pml[3], Ny, [](const aefdtd_sensor_req &q) { return q.y2; }, [](const aefdtd_source_req &q) { return q.y2; });
+    }
+    if (Nz > 1) {
+        near(m->pad[4], m->pml[4], [](const aefdtd_sensor_req &q) { return q.z1; }, [](const aefdtd_source_req &q) { return q.z1; });
+        far(m->pad[5], m->pml[5], Nz, [](const aefdtd_sensor_req &q) { return q.z2; }, [](const aefdtd_source_req &q) { return q.z2; });
+    }
+    return AEFDTD_OK;
+}
+
+// mode_default_fdtd, modes/default_fdtd.cpp:24-210 (fixed step count).
+int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors, double *sensor_out,
+                        double *fields_out) {
+    g_cancel = false;
+    int Nt = m->Nt;
+    double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
+    double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * c_light) * 0.99 * m->time_mod;
+    GridGuard G;
+    int rc = aeoracle_create(&G.g, m->nx, m->ny, m->nz, Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, m->pad, AEFDTD_FP64);
+    if (rc) return rc;
+    aeoracle_grid *g = G.g;
+    if ((rc = setup_grid(g, m))) return rc;
+    aeoracle_bootstrap(g);
+    SensorSet S;
+    if ((rc = add_requests(g, n_sensors, sensors, S))) return rc;
+    for (int i = 0; i < m->n_sources; i++) {
+        const aefdtd_source_req &q = m->sources[i];
+        if (q.type != 0) return set_err(g, AEFDTD_EINVAL, "only oscillator sources are supported");
+        int id;
+        if ((rc = aeoracle_add_oscillator(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, q.lambda_max, &id)))
+            return rc;
+    }
+    for (int t = 0; t < Nt; t++) {
+        g->update_E();       // + sources' inject_E
+        g->update_H();       // + sources' inject_H (step count)
+        aeoracle_feed_sensors(g);
+        if (g_cancel) break;
+    }
+    collect_requests(g, n_sensors, sensors, S, sensor_out);
+    copy_fields(g, fields_out);
+    return AEFDTD_OK;
+}
+
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
                                  double *sensor_out, double *fields_out) {
     g_cancel = false;

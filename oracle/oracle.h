@@ -52,6 +52,8 @@ int aeoracle_set_tfsf(aeoracle_grid *g, const aeoracle_grid *aux,
 int aeoracle_add_spect_poynting(aeoracle_grid *g, int orientation,
                                 int x1, int x2, int y1, int y2, int z1, int z2,
                                 int Nl, const double *lambda, int *id);
+int aeoracle_add_oscillator(aeoracle_grid *g, int orientation, int x1, int x2, int y1, int y2,
+                            int z1, int z2, double lambda_min, double lambda_max, int *id);
 int aeoracle_feed_sensors(aeoracle_grid *g);
 int aeoracle_sensor_poynting(aeoracle_grid *g, int id, double *rsp);
 int aeoracle_sensor_spectra(aeoracle_grid *g, int id, double *out, int dims[3]);
@@ -84,6 +86,9 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors,
                                   const aefdtd_sensor_req *sensors,
                                   aefdtd_normal_result *res, double *sensor_out,
                                   double *fields_out);
+int aeoracle_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors);
+int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                        double *sensor_out, double *fields_out);
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors,
                                  const aefdtd_sensor_req *sensors,
                                  double *sensor_out, double *fields_out);

@@ -29,8 +29,8 @@ def test_oracle_mirrors_the_grid_abi():
                    "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
                    "aefdtd_set_storage_scale"}
     missing = [s for s in declared_symbols()
-               if s not in engine_only and not s.startswith("aefdtd_run_") or s in
-               ("aefdtd_run_normal_incidence", "aefdtd_run_single_particle")]
+               if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_")) or s in
+               ("aefdtd_run_normal_incidence", "aefdtd_run_single_particle", "aefdtd_run_custom")]
     missing = [s for s in missing if not hasattr(lib, s.replace("aefdtd_", "aeoracle_"))]
     assert not missing, missing
 
@@ -80,3 +80,62 @@ def test_engine_refuses_without_gpu_or_bad_args():
     rc = lib.fn("create")(C.byref(h), 0, 4, 4, 10, 1e-9, 1e-9, 1e-9, 1e-18, capi.MODE_CUSTOM, p6, p6, capi.FP64)
     assert rc == capi.EINVAL
     assert b"dimension" in lib.fn("last_error")(None)
+
+
+def _finalized(be, spec, kind, sensors):
+    return capi.mode_finalize(be, spec, kind, sensors)
+
+
+@pytest.mark.parametrize("backend", ["engine", "oracle"])
+def test_mode_finalize_pads(backend):
+    """FDTD_Mode::finalize (modes/lua_fdtd.cpp:74-203): a PML face gets a pad of
+    at least 5, 5 beyond a sensor edge and 50 beyond an oscillator edge; a face
+    without PML gets none; a one-cell axis is left alone; normal incidence
+    drops the x/y PML and defaults the z PML to 25."""
+    import oracle
+    be = capi.engine() if backend == "engine" else oracle.load()
+    n = 100
+    g = np.zeros((1, n, n), dtype=np.uint32)
+    src = capi.SourceReq(orientation=capi.NORMAL_Z, x1=0, x2=n, y1=0, y2=1, z1=0, z2=1)
+    sens = capi.SensorReq(box=False, orientation=capi.NORMAL_Y, x1=0, x2=n, y1=n - 1, y2=n, z1=0, z2=1)
+    spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(1.0)], pml=(16, 16, 16, 16, 0, 0),
+                         pad=(0, 0, 0, 0, 7, 7), sources=[src])
+    out = _finalized(be, spec, capi.KIND_CUSTOM, [sens])
+    # x: oscillator spans [0, n): 50 - 0 and 50 + n - n; y: source at 0 -> 50,
+    # sensor top edge n -> 5 + n - n = 5; z: one cell, pads untouched
+    assert out.pad == (50, 50, 50, 5, 7, 7)
+    assert out.pml == (16, 16, 16, 16, 0, 0)
+    # a sensor 3 cells inside the lower x face needs 5 - 3 = 2 < 5: the minimum holds;
+    # one at -4 (outside the structure) needs 9
+    g3 = np.zeros((40, 30, 20), dtype=np.uint32)
+    s2 = capi.SensorReq(box=True, x1=-4, x2=20, y1=3, y2=33, z1=0, z2=40)
+    spec3 = capi.ModeSpec(matsgrid=g3, materials=[capi.Material(1.0)], pml=(8, 0, 8, 8, 8, 8), pad=(1, 9, 0, 0, 0, 6))
+    out3 = _finalized(be, spec3, capi.KIND_SINGLE, [s2])
+    assert out3.pad == (9, 0, 5, 8, 5, 6)
+    # normal incidence: x/y PML removed (their pads follow), z PML defaulted
+    spec4 = capi.ModeSpec(matsgrid=g3, materials=[capi.Material(1.0)], pml=(8, 8, 8, 8, 0, 12), pad=(3,) * 6)
+    out4 = _finalized(be, spec4, capi.KIND_NORMAL, [])
+    assert out4.pml == (0, 0, 0, 0, 25, 12)
+    assert out4.pad == (0, 0, 0, 0, 5, 5)
+
+
+def test_cylinder_packing():
+    rng = workloads.MT19937_64(11)
+    occ, disks = workloads.random_cylinders(rng, 200, 0.35)
+    assert occ.mean() >= 0.35 * 0.97
+    # disks never touch: centre distance > r1 + r2 (one clear cell between them)
+    for a in range(len(disks)):
+        xa, ya, ra = disks[a]
+        assert 3.0 <= ra <= 15.0 and ra <= xa <= 200 - ra and ra <= ya <= 200 - ra
+        for b in range(a):
+            xb, yb, rb = disks[b]
+            assert (xa - xb) ** 2 + (ya - yb) ** 2 > (ra + rb) ** 2
+
+
+def test_sweep_steps_and_sizes():
+    assert workloads.sweep_steps(256) == 4000 and workloads.sweep_steps(2048) == 12000
+    assert workloads.sweep_steps(1152) == 8000
+    rng = workloads.MT19937_64(4)
+    spec, sens, info = workloads.sweep_run(rng, 0, side=300)
+    assert spec.matsgrid.shape == (1, 300, 300) and info["steps"] == workloads.sweep_steps(300)
+    assert set(spec.disable_fields) in ({capi.EX, capi.EY, capi.HZ}, {capi.EZ, capi.HX, capi.HY})

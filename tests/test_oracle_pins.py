@@ -10,7 +10,11 @@ term, a wrong sign, an off-by-half index, a transposed operand) fails it:
     kernel), the CPML grading against its closed form at both sides;
   * physics: exact discrete-energy conservation of the Yee scheme, the exact
     two-step recurrence of a single Fourier mode at the numerical-dispersion
-    frequency, CPML absorption, and slab R/T against the Airy formula.
+    frequency (3D, and 2D TE/TM with disabled fields on a one-cell axis),
+    CPML absorption, and slab R/T against the Airy formula;
+  * the oscillator source: its injected series against the design levels of
+    Oscillator::initialize (peak 50 at tshift, 1e-7 onset, spectrum centred
+    on the band with sqrt(0.1) at both band edges).
 """
 import ctypes as C
 import json
@@ -308,3 +312,112 @@ def test_mt19937_64_known_answer():
     for _ in range(9999):
         g.next_u64()
     assert g.next_u64() == 9981545732273789042
+
+
+@pytest.mark.parametrize("pol", ["TM", "TE"])
+def test_2d_te_tm_dispersion(oracle_be, pol):
+    """A one-cell z axis makes every z difference vanish (periodic wrap onto the
+    same cell): with TE or TM fields disabled, an in-plane Fourier mode obeys
+    the 2D two-step recurrence and the disabled components stay zero."""
+    n = (12, 10, 1)
+    D = (5e-9, 4e-9, 6e-9)
+    Dt = min(D) / (np.sqrt(3.0) * C0) * 0.99
+    g = Grid(oracle_be, n, 10, D, Dt)
+    g.set_matsgrid(np.zeros(n[::-1], dtype=np.uint32))
+    g.set_material(0, Material(1.0))
+    off = capi.EX, capi.EY, capi.HZ
+    if pol == "TE":
+        off = capi.EZ, capi.HX, capi.HY
+    g.disable_fields(off)
+    g.bootstrap()
+    k = [2 * np.pi * 1 / (n[0] * D[0]), 2 * np.pi * 2 / (n[1] * D[1])]
+    x = np.arange(n[0]) * D[0]
+    y = np.arange(n[1]) * D[1]
+    if pol == "TM":
+        comps, pol_v, offs = [capi.EZ], [1.0], [(0.0, 0.0)]
+    else:
+        kv = np.array([np.sin(k[a] * D[a] / 2) / D[a] for a in range(2)])
+        pv = np.array([-kv[1], kv[0]]) / np.linalg.norm(kv)
+        comps, pol_v, offs = [capi.EX, capi.EY], list(pv), [(0.5, 0.0), (0.0, 0.5)]
+    shapes = []
+    for c, pv_c, (ox, oy) in zip(comps, pol_v, offs):
+        sh = np.cos(k[0] * (x[None, :] + ox * D[0]) + k[1] * (y[:, None] + oy * D[1]))[None]
+        shapes.append(sh)
+        g.write(c, pv_c * sh)
+    s2 = sum(np.sin(k[a] * D[a] / 2) ** 2 / D[a] ** 2 for a in range(2))
+    cw = 1 - 2 * (C0 * Dt) ** 2 * s2
+    F = []
+    for _ in range(60):
+        F.append(sum(pv_c * np.sum(g.read(c) * sh) for c, pv_c, sh in zip(comps, pol_v, shapes)))
+        g.step(1)
+    F = np.array(F)
+    res = F[2:] - 2 * cw * F[1:-1] + F[:-2]
+    assert np.max(np.abs(res)) <= 1e-10 * np.max(np.abs(F))
+    for c in off:
+        assert not np.any(g.read(c))
+
+
+def _oscillator_series(be, lmin, lmax, nsteps, D=10e-9):
+    """Values the oscillator adds to its cell at steps 0..nsteps-1 (a 1x1x1 grid
+    with every field disabled, so E only changes by the injection)."""
+    Dt = D / (np.sqrt(3.0) * C0) * 0.99
+    g = Grid(be, (1, 1, 1), nsteps, (D,) * 3, Dt)
+    g.set_matsgrid(np.zeros((1, 1, 1), dtype=np.uint32))
+    g.set_material(0, Material(1.0))
+    g.disable_fields(range(6))
+    g.bootstrap()
+    g.add_oscillator(capi.NORMAL_ZM, (0, 1, 0, 1, 0, 1), lmin, lmax)
+    prev, vals = 0.0, []
+    for _ in range(nsteps):
+        g.update_E()
+        v = float(g.read(capi.EZ)[0, 0, 0])
+        vals.append(v - prev)
+        prev = v
+        g.update_H()
+    return np.array(vals), Dt
+
+
+def test_oscillator_design_levels(oracle_be):
+    lmin, lmax = 500e-9, 1000e-9
+    vals, Dt = _oscillator_series(oracle_be, lmin, lmax, 1100)
+    w1, w2 = 2 * np.pi * C0 / lmax, 2 * np.pi * C0 / lmin
+    w0 = 0.5 * (w1 + w2)
+    # amplitude 50 at the envelope peak; onset level 1e-7 of it at step 0
+    # (the delay is truncated to whole steps, so the onset is a little above)
+    assert 49.0 < np.max(np.abs(vals)) <= 50.0
+    assert 1e-7 <= vals[0] / 50.0 < 2e-6
+    assert abs(vals[-1]) < 50.0 * 1e-6
+    # spectrum: a Gaussian centred on w0 whose level at both band edges is
+    # sqrt(0.1) of the peak
+    t = np.arange(len(vals)) * Dt
+    def amp(w):
+        return np.abs(np.sum(vals * np.exp(1j * w * t)))
+    ws = np.linspace(0.8 * w0, 1.2 * w0, 801)
+    a = np.array([amp(w) for w in ws])
+    assert abs(ws[np.argmax(a)] / w0 - 1) < 2e-3
+    peak = amp(w0)
+    assert abs(amp(w1) / peak - np.sqrt(0.1)) < 5e-3
+    assert abs(amp(w2) / peak - np.sqrt(0.1)) < 5e-3
+
+
+def test_oscillator_orientation_and_step_origin(oracle_be):
+    """NORMAL_XM drives Ex, NORMAL_Y drives Ey; a source added after s steps
+    starts its own step count at 0; NORMAL_UNDEF injects nothing."""
+    D = 10e-9
+    Dt = D / (np.sqrt(3.0) * C0) * 0.99
+    g = Grid(oracle_be, (2, 2, 1), 100, (D,) * 3, Dt)
+    g.set_matsgrid(np.zeros((1, 2, 2), dtype=np.uint32))
+    g.set_material(0, Material(1.0))
+    g.disable_fields(range(6))
+    g.bootstrap()
+    for _ in range(5):
+        g.update_E(); g.update_H()
+    g.add_oscillator(capi.NORMAL_XM, (0, 1, 0, 1, 0, 1), 500e-9, 1000e-9)
+    g.add_oscillator(capi.NORMAL_Y, (1, 2, 0, 2, 0, 1), 500e-9, 1000e-9)
+    g.add_oscillator(capi.NORMAL_UNDEF, (0, 2, 0, 2, 0, 1), 500e-9, 1000e-9)
+    g.update_E()
+    ref, _ = _oscillator_series(oracle_be, 500e-9, 1000e-9, 1)
+    ex, ey, ez = g.read(capi.EX), g.read(capi.EY), g.read(capi.EZ)
+    assert ex[0, 0, 0] == ref[0] and ex[0, 0, 1] == 0 and ex[0, 1, 0] == 0
+    assert ey[0, 0, 1] == ref[0] and ey[0, 1, 1] == ref[0] and ey[0, 0, 0] == 0
+    assert not np.any(ez)
