@@ -142,6 +142,16 @@ def _sig(lib, p):
         "batch_set_tiles": (i, [vp, i, i]),
         "batch_autotune": (i, [vp, _ip]),
         "batch_profile": (i, [vp, i, _dp, C.POINTER(C.c_long)]),
+        "server_create": (i, [C.POINTER(vp), C.c_size_t]),
+        "server_destroy": (i, [vp]),
+        "server_submit": (i, [vp, i, C.POINTER(CMode), i, C.POINTER(CSensorReq), C.POINTER(C.c_long)]),
+        "server_cancel": (i, [vp, C.c_long]),
+        "server_wait": (i, [vp, C.c_long, d, _ip]),
+        "server_result": (i, [vp, C.c_long, _dp, C.c_size_t, _dp]),
+        "server_error": (C.c_char_p, [vp, C.c_long]),
+        "server_release": (i, [vp, C.c_long]),
+        "server_stats": (i, [vp, _dp]),
+        "estimate_bytes": (i, [i, C.POINTER(CMode), i, C.POINTER(CSensorReq), C.POINTER(C.c_size_t)]),
         # oracle only
         "step": (i, [vp, i]),
         "set_threads": (None, [i]),
@@ -601,6 +611,86 @@ class Batch:
     def close(self):
         if self.h:
             self.be.fn("batch_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
+
+def estimate_bytes(be: Backend, kind, spec: ModeSpec, sensors=()):
+    keep = []
+    cm = spec.to_c(keep)
+    sreq = (CSensorReq * max(1, len(sensors)))(*[q.to_c() for q in sensors])
+    out = C.c_size_t()
+    be.check(be.fn("estimate_bytes")(kind, C.byref(cm), len(sensors), sreq, C.byref(out)), None)
+    return out.value
+
+
+class Server:
+    """The engine's request server: FIFO admission under a device-memory budget,
+    resident runs stepped together in 64-step rounds (continuous batching)."""
+
+    def __init__(self, be: Backend, budget_bytes):
+        self.be = be
+        h = C.c_void_p()
+        be.check(be.fn("server_create")(C.byref(h), int(budget_bytes)), None)
+        self.h = h
+        self.out_len = {}
+
+    def submit(self, kind, spec: ModeSpec, sensors=()):
+        keep = []
+        cm = spec.to_c(keep)
+        sreq = (CSensorReq * max(1, len(sensors)))(*[q.to_c() for q in sensors])
+        rid = C.c_long()
+        rc = self.be.fn("server_submit")(self.h, kind, C.byref(cm), len(sensors), sreq, C.byref(rid))
+        if rc != OK:
+            raise FDTDError(rc, "submit refused")
+        n = sum(q.out_len() for q in sensors) + (2 * spec.Nl if kind == KIND_NORMAL else 0)
+        self.out_len[rid.value] = n
+        return rid.value
+
+    def cancel(self, rid):
+        return self.be.fn("server_cancel")(self.h, rid)
+
+    def wait(self, rid, timeout=-1.0):
+        st = C.c_int()
+        rc = self.be.fn("server_wait")(self.h, rid, timeout, C.byref(st))
+        if rc != OK:
+            raise FDTDError(rc, "unknown request")
+        return st.value
+
+    def result(self, rid):
+        """(outputs, times) of a finished request; times = (queue wait, admission to
+        spectrum, submit to spectrum, resident bytes).  Raises for a failed one."""
+        n = self.out_len.get(rid, 0)
+        out = np.zeros(max(1, n))
+        t = np.zeros(4)
+        rc = self.be.fn("server_result")(self.h, rid, _dptr(out), n, _dptr(t))
+        if rc != OK:
+            raise FDTDError(rc, (self.be.fn("server_error")(self.h, rid) or b"").decode())
+        return out[:n], t
+
+    def error(self, rid):
+        return (self.be.fn("server_error")(self.h, rid) or b"").decode()
+
+    def release(self, rid):
+        return self.be.fn("server_release")(self.h, rid)
+
+    def stats(self):
+        out = np.zeros(8)
+        self.be.check(self.be.fn("server_stats")(self.h, _dptr(out)), None)
+        keys = ("queued", "resident", "done", "failed", "bytes_resident", "peak_bytes", "budget", "rounds")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if self.h:
+            self.be.fn("server_destroy")(self.h)
             self.h = None
 
     def __del__(self):

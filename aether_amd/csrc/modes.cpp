@@ -9,7 +9,7 @@
 // caller can interleave many runs (the serving loop) or time the stepping
 // alone (the benchmark); aefdtd_run_normal_incidence/_single_particle are the
 // one-call forms.
-#include "engine.hpp"
+#include "runs.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -29,31 +29,6 @@ void lambda_grid(int Nl, double lmin, double lmax, std::vector<double> &L) {   /
 }
 
 }  // namespace
-
-struct aefdtd_run {
-    int kind = 0;                     // 0 normal incidence, 1 single particle, 2 custom
-    int Nt = 0, t = 0;
-    int polarization = 0, Nl = 0;
-    double lambda_min = 0, lambda_max = 0, Dt = 0;
-    aefdtd_grid *g = nullptr, *a = nullptr;
-    std::vector<aefdtd_sensor_req> reqs;
-    std::vector<aefdtd_source_req> srcs;
-    std::vector<std::vector<int>> ids;
-    int idR = -1, idT = -1;
-    double index_sup = 1, index_sub = 1;
-    std::string err;
-    struct aefdtd_batch *batch = nullptr;   // set while a member of a batch
-    ~aefdtd_run() {
-        aefdtd_destroy(g);
-        aefdtd_destroy(a);
-    }
-};
-
-struct aefdtd_batch {
-    Batch *b = nullptr;
-    std::vector<aefdtd_run *> runs;
-    std::string err;
-};
 
 namespace {
 
@@ -348,6 +323,31 @@ int setup_custom(aefdtd_run *r, const aefdtd_mode *m) {
 }
 
 }  // namespace
+
+// Grid sizes of the drivers' setups (setup_normal/_single/_custom) without
+// building them, for admission control.
+void aefdtd::run_dims(int kind, const aefdtd_mode *m, int dims[3], int pads[6]) {
+    int pml[6], pad[6];
+    for (int f = 0; f < 6; f++) { pml[f] = m->pml[f]; pad[f] = m->pad[f]; }
+    if (kind == 0) {
+        for (int f = 0; f < 4; f++) pml[f] = pad[f] = 0;
+        pad[4] = std::max(5, m->pad[4]);
+        pad[5] = std::max(5, m->pad[5]);
+    } else if (kind == 1) {
+        for (int f = 0; f < 6; f++) pad[f] = std::max(10, m->pad[f]);
+        if (std::max(pml[0], pml[1]) <= 0) pad[0] = pad[1] = 0;
+        if (std::max(pml[2], pml[3]) <= 0) pad[2] = pad[3] = 0;
+    }
+    for (int a = 0; a < 3; a++)                 // FDTD::FDTD (core/fdtd_core.cpp:212-243)
+        if (pml[2 * a] || pml[2 * a + 1]) {
+            pad[2 * a] = std::max(5, pad[2 * a]);
+            pad[2 * a + 1] = std::max(5, pad[2 * a + 1]);
+        }
+    const int n[3] = {m->nx, m->ny, m->nz};
+    for (int a = 0; a < 3; a++) dims[a] = n[a] + pml[2 * a] + pml[2 * a + 1] + pad[2 * a] + pad[2 * a + 1];
+    if (pads)
+        for (int f = 0; f < 6; f++) pads[f] = pad[f];
+}
 
 extern "C" {
 

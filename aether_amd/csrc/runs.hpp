@@ -1,0 +1,40 @@
+// runs.hpp -- host objects of the mode drivers (modes.cpp) shared with the
+// request server (server.cpp).
+#pragma once
+
+#include "engine.hpp"
+
+#include <string>
+#include <vector>
+
+// One mode driver between create and finish (modes.cpp).
+struct aefdtd_run {
+    int kind = 0;                     // 0 normal incidence, 1 single particle, 2 custom
+    int Nt = 0, t = 0;
+    int polarization = 0, Nl = 0;
+    double lambda_min = 0, lambda_max = 0, Dt = 0;
+    aefdtd_grid *g = nullptr, *a = nullptr;
+    std::vector<aefdtd_sensor_req> reqs;
+    std::vector<aefdtd_source_req> srcs;
+    std::vector<std::vector<int>> ids;
+    int idR = -1, idT = -1;
+    double index_sup = 1, index_sub = 1;
+    std::string err;
+    struct aefdtd_batch *batch = nullptr;   // set while a member of a batch
+    ~aefdtd_run() {
+        aefdtd_destroy(g);
+        aefdtd_destroy(a);
+    }
+};
+
+struct aefdtd_batch {
+    aefdtd::Batch *b = nullptr;
+    std::vector<aefdtd_run *> runs;
+    std::string err;
+};
+
+namespace aefdtd {
+// Full-grid dimensions a mode driver of `kind` builds for mode m (the padding
+// rules of the drivers and of FDTD::FDTD), and the auxiliary grid's cells.
+void run_dims(int kind, const aefdtd_mode *m, int dims[3], int pads[6]);
+}  // namespace aefdtd

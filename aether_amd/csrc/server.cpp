@@ -1,0 +1,484 @@
+// server.cpp -- the request server of the serving form: a worker thread that
+// admits FDTD requests under a device-memory budget and keeps every admitted
+// run resident, stepping all of them together in rounds of 64 steps until
+// their spectra are out (continuous batching).  The reference has no such
+// layer: it runs one mode driver per call (modes/lua_fdtd.cpp:296-320).
+#include "runs.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+using namespace aefdtd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kRound = 64;     // steps per round (one batch chunk)
+
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+// A request with its own copy of every input the driver reads.
+struct Request {
+    long id = 0;
+    int kind = 0;
+    int state = AEFDTD_REQ_QUEUED;
+    int rc = AEFDTD_OK;
+    std::string err;
+    aefdtd_mode m{};
+    std::vector<uint32_t> mg;
+    std::vector<aefdtd_material> mats;
+    std::vector<std::vector<double>> poles;
+    std::vector<int> disable;
+    std::vector<aefdtd_source_req> srcs;
+    std::vector<aefdtd_sensor_req> sens;
+    size_t est = 0, bytes = 0;
+    aefdtd_run *run = nullptr;
+    aefdtd_batch *batch = nullptr;   // set when stepped by a batch
+    std::vector<double> out;
+    Clock::time_point t_submit, t_admit, t_done;
+    bool cancel = false;
+};
+
+void copy_inputs(Request &q, const aefdtd_mode *m, int ns, const aefdtd_sensor_req *sens) {
+    q.m = *m;
+    q.mg.assign(m->matsgrid, m->matsgrid + (size_t)m->nx * m->ny * m->nz);
+    q.m.matsgrid = q.mg.data();
+    q.mats.assign(m->materials, m->materials + std::max(0, m->n_materials));
+    q.poles.reserve(4 * q.mats.size());
+    for (aefdtd_material &a : q.mats) {
+        auto keep = [&](const double *p, int n, int w) -> const double * {
+            if (n <= 0 || !p) return nullptr;
+            q.poles.emplace_back(p, p + (size_t)n * w);
+            return q.poles.back().data();
+        };
+        a.debye = keep(a.debye, a.n_debye, 2);
+        a.drude = keep(a.drude, a.n_drude, 2);
+        a.lorentz = keep(a.lorentz, a.n_lorentz, 3);
+        a.critpoint = keep(a.critpoint, a.n_critpoint, 4);
+    }
+    q.m.materials = q.mats.data();
+    if (m->n_disable > 0) q.disable.assign(m->disable_fields, m->disable_fields + m->n_disable);
+    q.m.disable_fields = q.disable.data();
+    if (m->n_sources > 0) q.srcs.assign(m->sources, m->sources + m->n_sources);
+    q.m.sources = q.srcs.data();
+    if (ns > 0) q.sens.assign(sens, sens + ns);
+}
+
+size_t out_len(const Request &q) {
+    size_t n = 0;
+    for (const aefdtd_sensor_req &s : q.sens) n += s.box ? 7 * (size_t)s.Nl : (size_t)s.Nl;
+    if (q.kind == 0) n += 2 * (size_t)q.m.Nl;
+    return n;
+}
+
+bool has_poles(const aefdtd_material &a) { return a.n_debye + a.n_drude + a.n_lorentz + a.n_critpoint > 0; }
+
+}  // namespace
+
+struct aefdtd_server {
+    size_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request *> queue;
+    std::vector<Request *> resident;
+    std::map<long, std::unique_ptr<Request>> all;
+    long next_id = 1;
+    bool stop = false;
+    size_t used = 0, peak = 0;
+    long rounds = 0, n_done = 0, n_failed = 0;
+    aefdtd_batch *batch[3] = {nullptr, nullptr, nullptr};   // per precision
+    std::thread worker;
+    std::string err;
+};
+
+namespace {
+
+// Resident bytes of a run: its grid and the auxiliary grid of a single-particle run.
+size_t run_bytes(aefdtd_run *r) {
+    size_t a = 0, b = 0;
+    if (r->g) aefdtd_device_bytes(r->g, &a);
+    if (r->a) aefdtd_device_bytes(r->a, &b);
+    return a + b;
+}
+
+void finish_request(aefdtd_server *S, Request *q, int state, int rc, const std::string &err) {
+    // called with S->mu held
+    q->state = state;
+    q->rc = rc;
+    if (!err.empty()) q->err = err;
+    q->t_done = Clock::now();
+    if (state == AEFDTD_REQ_DONE) S->n_done++;
+    else if (state == AEFDTD_REQ_FAILED) S->n_failed++;
+    S->cv.notify_all();
+}
+
+// Builds the run of a request (worker thread, lock not held).
+int admit(aefdtd_server *S, Request *q) {
+    int rc = aefdtd_run_create(&q->run, q->kind, &q->m, (int)q->sens.size(), q->sens.data());
+    if (rc) {
+        q->err = aefdtd_last_error(nullptr);
+        return rc;
+    }
+    q->bytes = run_bytes(q->run);
+    bool batchable = q->kind == 2 && !q->run->g->n_disp;
+    if (batchable) {
+        int p = q->m.precision;
+        if (!S->batch[p] && (rc = aefdtd_batch_create(&S->batch[p], p))) {
+            q->err = "batch creation failed";
+            return rc;
+        }
+        if ((rc = aefdtd_batch_add(S->batch[p], q->run)) == AEFDTD_OK) q->batch = S->batch[p];
+        else q->err = aefdtd_batch_error(S->batch[p]);
+        // a run the batch refuses still runs on its own
+        rc = AEFDTD_OK;
+    }
+    return rc;
+}
+
+// Spectra of a finished run, then its device memory is released (worker thread).
+int retire(Request *q) {
+    int rc = AEFDTD_OK;
+    q->bytes = std::max(q->bytes, run_bytes(q->run));   // stage and DFT tables grow after admission
+    if (q->batch) {
+        rc = aefdtd_batch_remove(q->batch, q->run);
+        q->batch = nullptr;
+    }
+    if (!rc) {
+        q->out.assign(out_len(*q), 0.0);
+        size_t nsens = q->out.size() - (q->kind == 0 ? 2 * (size_t)q->m.Nl : 0);
+        aefdtd_normal_result res{};
+        std::vector<double> lam(q->m.Nl), cx(4 * 2 * (size_t)std::max(1, q->m.Nl));
+        if (q->kind == 0) {
+            res.lambda = lam.data();
+            res.R = q->out.data() + nsens;
+            res.T = q->out.data() + nsens + q->m.Nl;
+            res.refx = cx.data(); res.refy = cx.data() + 2 * q->m.Nl;
+            res.transx = cx.data() + 4 * q->m.Nl; res.transy = cx.data() + 6 * q->m.Nl;
+        }
+        rc = aefdtd_run_finish(q->run, q->kind == 0 ? &res : nullptr, q->out.data(), nullptr);
+        if (rc) q->err = aefdtd_run_error(q->run);
+    }
+    aefdtd_run_destroy(q->run);
+    q->run = nullptr;
+    return rc;
+}
+
+void drop(Request *q) {
+    if (q->batch) aefdtd_batch_remove(q->batch, q->run);
+    q->batch = nullptr;
+    aefdtd_run_destroy(q->run);
+    q->run = nullptr;
+}
+
+void worker_loop(aefdtd_server *S) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        // ---- cancelled requests leave the queue wherever they are
+        for (auto it = S->queue.begin(); it != S->queue.end();) {
+            if ((*it)->cancel) {
+                finish_request(S, *it, AEFDTD_REQ_CANCELLED, AEFDTD_ECANCELLED, "cancelled");
+                it = S->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        // ---- admission, FIFO, under the budget
+        while (!S->queue.empty()) {
+            Request *q = S->queue.front();
+            if (q->cancel) {
+                S->queue.pop_front();
+                finish_request(S, q, AEFDTD_REQ_CANCELLED, AEFDTD_ECANCELLED, "cancelled");
+                continue;
+            }
+            if (q->est > S->budget) {
+                S->queue.pop_front();
+                finish_request(S, q, AEFDTD_REQ_FAILED, AEFDTD_ENOMEM, "request exceeds the device-memory budget");
+                continue;
+            }
+            if (S->used + q->est > S->budget) break;
+            S->queue.pop_front();
+            S->used += q->est;          // held while the run is built
+            lk.unlock();
+            int rc = admit(S, q);
+            lk.lock();
+            S->used -= q->est;
+            if (rc) {
+                if (q->run) drop(q);
+                finish_request(S, q, AEFDTD_REQ_FAILED, rc, q->err.empty() ? "admission failed" : q->err);
+                continue;
+            }
+            S->used += std::max(q->bytes, q->est);
+            S->peak = std::max(S->peak, S->used);
+            q->state = AEFDTD_REQ_RUNNING;
+            q->t_admit = Clock::now();
+            S->resident.push_back(q);
+        }
+        if (S->resident.empty()) {
+            if (S->stop) break;
+            S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            continue;
+        }
+        std::vector<Request *> res = S->resident;
+        lk.unlock();
+        // ---- one round: every resident run advances up to 64 steps
+        std::vector<std::pair<Request *, int>> failed;
+        for (aefdtd_batch *b : S->batch)
+            if (b && aefdtd_batch_running(b) > 0) {
+                int rc = aefdtd_batch_step(b, 1);
+                if (rc)
+                    for (Request *q : res)
+                        if (q->batch == b) failed.push_back({q, rc});
+            }
+        for (Request *q : res) {
+            if (q->batch) continue;
+            int rc = aefdtd_run_step(q->run, kRound);
+            if (rc) failed.push_back({q, rc});
+        }
+        for (aefdtd_batch *b : S->batch)
+            if (b) aefdtd_batch_synchronize(b);
+        for (Request *q : res)
+            if (!q->batch && q->run) aefdtd_synchronize(aefdtd_run_grid(q->run));
+        // ---- retire finished, failed and cancelled runs
+        std::vector<std::pair<Request *, int>> ended;   // request, final state
+        for (Request *q : res) {
+            bool f = std::any_of(failed.begin(), failed.end(), [&](auto &p) { return p.first == q; });
+            int t = 0, nt = 0;
+            aefdtd_run_progress(q->run, &t, &nt);
+            if (f || q->cancel || t >= nt) ended.push_back({q, f ? AEFDTD_REQ_FAILED : (q->cancel ? AEFDTD_REQ_CANCELLED : AEFDTD_REQ_DONE)});
+        }
+        std::vector<int> rcs;
+        for (auto &e : ended) {
+            Request *q = e.first;
+            int rc = AEFDTD_OK;
+            if (e.second == AEFDTD_REQ_DONE) {
+                rc = retire(q);
+            } else {
+                if (e.second == AEFDTD_REQ_FAILED) {
+                    for (auto &p : failed)
+                        if (p.first == q) rc = p.second;
+                    if (q->err.empty()) q->err = aefdtd_run_error(q->run);
+                }
+                drop(q);
+            }
+            rcs.push_back(rc);
+        }
+        lk.lock();
+        S->rounds++;
+        for (size_t i = 0; i < ended.size(); i++) {
+            Request *q = ended[i].first;
+            S->resident.erase(std::find(S->resident.begin(), S->resident.end(), q));
+            S->used -= std::max(q->bytes, q->est);
+            int st = ended[i].second;
+            if (st == AEFDTD_REQ_DONE && rcs[i]) st = AEFDTD_REQ_FAILED;
+            int rc = st == AEFDTD_REQ_CANCELLED ? AEFDTD_ECANCELLED : (st == AEFDTD_REQ_FAILED ? (rcs[i] ? rcs[i] : AEFDTD_EDEVICE) : AEFDTD_OK);
+            finish_request(S, q, st, rc, "");
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_sensor_req *sens, size_t *bytes) {
+    if (!m || !bytes || kind < 0 || kind > 2 || (ns > 0 && !sens) || !m->matsgrid) return AEFDTD_EINVAL;
+    int d[3], pads[6];
+    run_dims(kind, m, d, pads);
+    const size_t N = (size_t)d[0] * d[1] * d[2];
+    const int p = m->precision;
+    const size_t ss = p == AEFDTD_FP64 ? 8 : (p == AEFDTD_FP32 ? 4 : 2), cs = p == AEFDTD_FP64 ? 8 : 4;
+    size_t b = 6 * N * ss + N;                              // fields, material ids
+    // dispersive cells of the extended grid: edge planes repeat into pads and PML
+    int npmax = 0;
+    bool any = false;
+    std::vector<char> rc(std::max(1, m->n_materials), 0);
+    for (int i = 0; i < m->n_materials; i++)
+        if (has_poles(m->materials[i])) {
+            rc[i] = 1;
+            any = true;
+            const aefdtd_material &a = m->materials[i];
+            npmax = std::max(npmax, a.n_debye + a.n_drude + a.n_lorentz + a.n_critpoint);
+        }
+    if (any) {
+        int pml[6];
+        for (int f = 0; f < 6; f++) pml[f] = kind == 0 && f < 4 ? 0 : m->pml[f];
+        const int lo[3] = {pml[0] + pads[0], pml[2] + pads[2], pml[4] + pads[4]};
+        const int nn[3] = {m->nx, m->ny, m->nz};
+        auto mult = [&](int a, int v) {
+            size_t w = 1;
+            if (v == 0) w += lo[a];
+            if (v == nn[a] - 1) w += d[a] - lo[a] - nn[a];
+            return w;
+        };
+        size_t nd = 0;
+        for (int k = 0; k < m->nz; k++)
+            for (int j = 0; j < m->ny; j++) {
+                const size_t wjk = mult(1, j) * mult(2, k);
+                const uint32_t *row = m->matsgrid + ((size_t)k * m->ny + j) * m->nx;
+                for (int i = 0; i < m->nx; i++)
+                    if (row[i] < (uint32_t)m->n_materials && rc[row[i]]) nd += wjk * mult(0, i);
+            }
+        b += 4 * N + nd * (4 + 3 * cs + 3 * (size_t)npmax * 16);
+    }
+    const size_t cross[3] = {(size_t)d[1] * d[2], (size_t)d[0] * d[2], (size_t)d[0] * d[1]};
+    for (int a = 0; a < 3; a++) {
+        int pm = kind == 0 && a < 2 ? 0 : m->pml[2 * a], pp = kind == 0 && a < 2 ? 0 : m->pml[2 * a + 1];
+        if (pm || pp) b += 4 * (size_t)(pm + pp) * cross[a] * cs;
+    }
+    b += 6 * (size_t)(d[0] + d[1] + d[2]) * cs;              // CPML profiles
+    size_t Ptot = 0;
+    for (int i = 0; i < ns; i++) {
+        const aefdtd_sensor_req &q = sens[i];
+        const size_t sx = std::max(0, q.x2 - q.x1), sy = std::max(0, q.y2 - q.y1), sz = std::max(0, q.z2 - q.z1);
+        size_t P[6] = {0, 0, 0, 0, 0, 0};
+        int nf = 0;
+        if (q.box) {
+            P[0] = P[1] = sy * sz; P[2] = P[3] = sx * sz; P[4] = P[5] = sx * sy;
+            nf = 6;
+        } else {
+            P[0] = (q.orientation <= 1) ? sy * sz : (q.orientation <= 3 ? sx * sz : sx * sy);
+            nf = 1;
+        }
+        for (int f = 0; f < nf; f++) {
+            if (q.box && q.disable[f]) continue;
+            b += 6 * (size_t)q.Nl * P[f] * 16 + (size_t)std::max(m->Nt, 1) * 2 * q.Nl * 16;
+            Ptot += P[f];
+        }
+    }
+    b += (size_t)kRound * 6 * Ptot * cs;                    // DFT stage
+    if (kind == 2) b += (size_t)std::max(0, m->n_sources) * std::max(m->Nt, 1) * 8;
+    if (kind == 0) b += 2 * (size_t)std::max(m->Nt, 1) * 64 * 6 * 8 + 2 * (size_t)m->Nt * 8;   // R/T plane sums
+    if (kind == 1) b += (size_t)d[2] * (6 * ss + 1 + 8 * cs) + (size_t)d[2] * 4 * 4;           // auxiliary column
+    *bytes = b + b / 50 + (8u << 20);
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_create(aefdtd_server **out, size_t budget) {
+    if (!out || budget == 0) return AEFDTD_EINVAL;
+    auto *S = new aefdtd_server();
+    S->budget = budget;
+    S->worker = std::thread(worker_loop, S);
+    *out = S;
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_destroy(aefdtd_server *S) {
+    if (!S) return AEFDTD_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto &kv : S->all) kv.second->cancel = true;
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    S->worker.join();
+    for (aefdtd_batch *b : S->batch) aefdtd_batch_destroy(b);
+    delete S;
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_submit(aefdtd_server *S, int kind, const aefdtd_mode *m, int ns, const aefdtd_sensor_req *sens,
+                         long *id) {
+    if (!S || !m || !id || kind < 0 || kind > 2 || !m->matsgrid || (ns > 0 && !sens)) return AEFDTD_EINVAL;
+    if (m->precision < AEFDTD_FP64 || m->precision > AEFDTD_FP16) return AEFDTD_EINVAL;
+    auto q = std::make_unique<Request>();
+    q->kind = kind;
+    copy_inputs(*q, m, ns, sens);
+    int rc = aefdtd_estimate_bytes(kind, &q->m, ns, q->sens.data(), &q->est);
+    if (rc) return rc;
+    q->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return AEFDTD_ESTATE;
+        q->id = S->next_id++;
+        *id = q->id;
+        S->queue.push_back(q.get());
+        S->all[q->id] = std::move(q);
+    }
+    S->cv.notify_all();
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_cancel(aefdtd_server *S, long id) {
+    if (!S) return AEFDTD_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->all.find(id);
+        if (it == S->all.end()) return AEFDTD_EINVAL;
+        it->second->cancel = true;
+    }
+    S->cv.notify_all();
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_wait(aefdtd_server *S, long id, double timeout_s, int *state) {
+    if (!S) return AEFDTD_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return AEFDTD_EINVAL;
+    Request *q = it->second.get();
+    auto ended = [&] { return q->state >= AEFDTD_REQ_DONE; };
+    if (timeout_s < 0) S->cv.wait(lk, ended);
+    else S->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), ended);
+    if (state) *state = q->state;
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_result(aefdtd_server *S, long id, double *out, size_t n, double times[4]) {
+    if (!S) return AEFDTD_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return AEFDTD_EINVAL;
+    Request *q = it->second.get();
+    if (q->state < AEFDTD_REQ_DONE) return AEFDTD_ESTATE;
+    if (q->state != AEFDTD_REQ_DONE) return q->rc ? q->rc : AEFDTD_ESTATE;
+    if (out) std::memcpy(out, q->out.data(), std::min(n, q->out.size()) * sizeof(double));
+    if (times) {
+        times[0] = secs(q->t_submit, q->t_admit);
+        times[1] = secs(q->t_admit, q->t_done);
+        times[2] = secs(q->t_submit, q->t_done);
+        times[3] = (double)q->bytes;
+    }
+    return AEFDTD_OK;
+}
+
+const char *aefdtd_server_error(aefdtd_server *S, long id) {
+    if (!S) return "";
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return "unknown request";
+    return it->second->err.c_str();
+}
+
+int aefdtd_server_release(aefdtd_server *S, long id) {
+    if (!S) return AEFDTD_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return AEFDTD_EINVAL;
+    if (it->second->state < AEFDTD_REQ_DONE) return AEFDTD_ESTATE;
+    S->all.erase(it);
+    return AEFDTD_OK;
+}
+
+int aefdtd_server_stats(aefdtd_server *S, double out[8]) {
+    if (!S || !out) return AEFDTD_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    out[0] = (double)S->queue.size();
+    out[1] = (double)S->resident.size();
+    out[2] = (double)S->n_done;
+    out[3] = (double)S->n_failed;
+    out[4] = (double)S->used;
+    out[5] = (double)S->peak;
+    out[6] = (double)S->budget;
+    out[7] = (double)S->rounds;
+    return AEFDTD_OK;
+}
+
+}  // extern "C"

@@ -21,38 +21,53 @@ from .capi import (EX, EY, EZ, FP32, FP64, HX, HY, HZ, NORMAL_X, NORMAL_Y, NORMA
 
 
 class MT19937_64:
-    """Standard 64-bit Mersenne twister (Matsumoto & Nishimura, 2000)."""
+    """Standard 64-bit Mersenne twister (Matsumoto & Nishimura, 2000).  The
+    312-word state is regenerated with numpy in three vector pieces (words
+    whose recurrence partners are all old, then those that read words already
+    renewed, then the last one), so draws cost little in Python."""
     _N, _M = 312, 156
-    _A = 0xB5026F5AA96619E9
-    _UM, _LM = 0xFFFFFFFF80000000, 0x7FFFFFFF
+    _A = np.uint64(0xB5026F5AA96619E9)
+    _UM, _LM = np.uint64(0xFFFFFFFF80000000), np.uint64(0x7FFFFFFF)
     _MASK = (1 << 64) - 1
 
     def __init__(self, seed):
-        self.mt = [0] * self._N
-        self.mt[0] = seed & self._MASK
+        mt = [0] * self._N
+        mt[0] = seed & self._MASK
         for i in range(1, self._N):
-            prev = self.mt[i - 1]
-            self.mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & self._MASK
-        self.idx = self._N
+            prev = mt[i - 1]
+            mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & self._MASK
+        self.mt = np.array(mt, dtype=np.uint64)
+        self.out = []
+        self.idx = 0
+
+    def _twist(self):
+        N, M = self._N, self._M
+        mt = self.mt
+        one = np.uint64(1)
+
+        def step(i0, i1, nxt, far):
+            x = (mt[i0:i1] & self._UM) | (nxt & self._LM)
+            xa = x >> one
+            xa = np.where((x & one) == one, xa ^ self._A, xa)
+            mt[i0:i1] = far ^ xa
+
+        step(0, N - M, mt[1:N - M + 1], mt[M:N])                   # partners i+1, i+M not yet renewed
+        step(N - M, N - 1, mt[N - M + 1:N], mt[0:M - 1].copy())     # partner i+M-N already renewed
+        step(N - 1, N, mt[0:1], mt[M - 1:M])                       # wraps onto the renewed word 0
+        y = mt.copy()
+        y ^= (y >> np.uint64(29)) & np.uint64(0x5555555555555555)
+        y ^= (y << np.uint64(17)) & np.uint64(0x71D67FFFEDA60000)
+        y ^= (y << np.uint64(37)) & np.uint64(0xFFF7EEE000000000)
+        y ^= y >> np.uint64(43)
+        self.out = y.tolist()
+        self.idx = 0
 
     def next_u64(self):
-        N, M = self._N, self._M
-        if self.idx >= N:
-            mt = self.mt
-            for i in range(N):
-                x = (mt[i] & self._UM) | (mt[(i + 1) % N] & self._LM)
-                xa = x >> 1
-                if x & 1:
-                    xa ^= self._A
-                mt[i] = mt[(i + M) % N] ^ xa
-            self.idx = 0
-        y = self.mt[self.idx]
+        if self.idx >= len(self.out):
+            self._twist()
+        v = self.out[self.idx]
         self.idx += 1
-        y ^= (y >> 29) & 0x5555555555555555
-        y ^= (y << 17) & 0x71D67FFFEDA60000
-        y ^= (y << 37) & 0xFFF7EEE000000000
-        y ^= y >> 43
-        return y & self._MASK
+        return v
 
     def uniform(self, a=0.0, b=1.0):
         return a + (b - a) * ((self.next_u64() >> 11) * 2.0 ** -53)
@@ -174,18 +189,22 @@ def multilayer_normal_incidence(n=(512, 512, 384), D=4e-9, seed=3, nlayers=12, m
 
 
 # ---------------------------------------------------------------- 2D sweeps
-def random_cylinders(rng, n, fill, rmin=3.0, rmax=15.0, max_tries_factor=40):
+def random_cylinders(rng, n, fill, rmin=3.0, rmax=15.0, max_tries_factor=15):
     """Rejection-packed disks (cylinders along z) on an n x n cell grid: radius
-    U[rmin, rmax] cells, centre U[r, n-r]^2, accepted when the disk grown by
-    one cell covers no cell of an earlier disk; stops at the fill fraction or
-    after max_tries_factor times the expected disk count of draws.  Returns
-    the (n, n) boolean mask (corner sampling: cell (i,j) is inside when
-    (i-x0)^2+(j-y0)^2 <= r^2) and the disk list."""
+    U[rmin, rmax] cells, centre U[r, n-r]^2, accepted when its centre lies
+    more than r + r' + 1 cells from every earlier disk (one clear cell between
+    disks); stops at the fill fraction or after max_tries_factor times the
+    expected disk count of draws.  Returns the (n, n) boolean mask (corner
+    sampling: cell (i,j) is inside when (i-x0)^2+(j-y0)^2 <= r^2) and the
+    disk list."""
     occ = np.zeros((n, n), dtype=bool)
     target = fill * n * n
     mean_area = math.pi * (rmin * rmin + rmin * rmax + rmax * rmax) / 3.0
     max_tries = int(max_tries_factor * target / mean_area) + 100
-    area = 0
+    cell = 2.0 * rmax + 1.0                  # bucket side: neighbours lie in the 3x3 buckets around
+    nb = int(n // cell) + 1
+    buckets = {}
+    area = 0.0
     disks = []
     for _ in range(max_tries):
         if area >= target:
@@ -193,18 +212,30 @@ def random_cylinders(rng, n, fill, rmin=3.0, rmax=15.0, max_tries_factor=40):
         r = rng.uniform(rmin, rmax)
         x0 = rng.uniform(r, n - r)
         y0 = rng.uniform(r, n - r)
-        R = r + 1.0
-        i0, i1 = max(0, int(math.floor(x0 - R))), min(n, int(math.ceil(x0 + R)) + 1)
-        j0, j1 = max(0, int(math.floor(y0 - R))), min(n, int(math.ceil(y0 + R)) + 1)
+        bx, by = int(x0 // cell), int(y0 // cell)
+        ok = True
+        for qx in (bx - 1, bx, bx + 1):
+            for qy in (by - 1, by, by + 1):
+                for (xa, ya, ra) in buckets.get(qx * nb + qy, ()):
+                    d = r + ra + 1.0
+                    if (x0 - xa) ** 2 + (y0 - ya) ** 2 <= d * d:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        buckets.setdefault(bx * nb + by, []).append((x0, y0, r))
+        disks.append((x0, y0, r))
+        area += math.pi * r * r
+    for (x0, y0, r) in disks:
+        i0, i1 = max(0, int(math.floor(x0 - r))), min(n, int(math.ceil(x0 + r)) + 1)
+        j0, j1 = max(0, int(math.floor(y0 - r))), min(n, int(math.ceil(y0 + r)) + 1)
         dx = np.arange(i0, i1) - x0
         dy = np.arange(j0, j1) - y0
-        d2 = dy[:, None] ** 2 + dx[None, :] ** 2
-        if occ[j0:j1, i0:i1][d2 <= R * R].any():
-            continue
-        inside = d2 <= r * r
-        occ[j0:j1, i0:i1] |= inside
-        area += int(inside.sum())
-        disks.append((x0, y0, r))
+        occ[j0:j1, i0:i1] |= (dy[:, None] ** 2 + dx[None, :] ** 2) <= r * r
     return occ, disks
 
 
@@ -249,3 +280,49 @@ def sweep_runs(count=64, seed=4, precision=FP32, **kw):
     mt19937_64 stream (seed 4)."""
     rng = MT19937_64(seed)
     return [sweep_run(rng, q, precision=precision, **kw) for q in range(count)]
+
+
+# ---------------------------------------------------------------- serving
+LORENTZ_SCATTERER = Material(eps_inf=2.0, lorentz=[(1.5, 3e15, 1e14)])
+
+
+def serving_requests(count=24, seed=5, rate=4.0, precision=FP32):
+    """The serving stream: Poisson arrivals at `rate` per second (seed 5) of
+    mixed requests, each (arrival_s, kind, spec, sensors, info):
+      * 3D single-particle runs, structure side U[96,384] cells of 2 nm, as the
+        256^3 scatterer run (12-cell CPML on all faces, eps 1.77 background,
+        box sensor at 64 wavelengths) with round(40 (side/256)^3) spheres of
+        radius 8-20 nm, either gold-like (Drude + 2 critical points) or a
+        Lorentz scatterer;
+      * 2D custom runs as the sweep (cylinders, TE or TM, oscillator line
+        source, line sensor) with side U[512,4096];
+    step counts U[2000,10000]; FP32 fields, FP64 sensor accumulators."""
+    from .capi import KIND_CUSTOM, KIND_SINGLE
+    rng = MT19937_64(seed)
+    out = []
+    t = 0.0
+    for q in range(count):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        steps = int(round(rng.uniform(2000, 10000)))
+        if rng.uniform() < 0.5:
+            side = int(round(rng.uniform(96, 384)))
+            nsph = max(1, int(round(40 * (side / 256.0) ** 3)))
+            gold = rng.uniform() < 0.5
+            D = 2e-9
+            L = (side * D,) * 3
+            sph = random_spheres(int(rng.next_u64() & 0x7FFFFFFF), nsph, L, 8e-9, 20e-9)
+            g = voxelize_spheres((side,) * 3, (D,) * 3, sph)
+            spec = ModeSpec(matsgrid=g, materials=[Material(1.77), GOLD if gold else LORENTZ_SCATTERER], Nt=steps,
+                            Dx=D, Dy=D, Dz=D, pml=(12,) * 6, Nl=64, lambda_min=400e-9, lambda_max=800e-9,
+                            polarization=1, precision=precision)
+            sens = [SensorReq(box=True, x1=0, x2=side, y1=0, y2=side, z1=0, z2=side, Nl=64,
+                              lambda_min=400e-9, lambda_max=800e-9)]
+            info = {"index": q, "kind": "3D", "side": side, "spheres": len(sph), "steps": steps,
+                    "material": "gold" if gold else "lorentz"}
+            out.append((t, KIND_SINGLE, spec, sens, info))
+        else:
+            side = int(round(rng.uniform(512, 4096)))
+            spec, sens, info = sweep_run(rng, q, side=side, steps=steps, precision=precision)
+            info["kind"] = "2D"
+            out.append((t, KIND_CUSTOM, spec, sens, info))
+    return out

@@ -172,13 +172,14 @@ def batching_form(args):
     b.synchronize()
     t0 = time.perf_counter()
     tiles = b.autotune()
-    while b.running():
-        b.step(8)
-    b.synchronize()
-    res_b = []
-    for r in objs:
-        b.remove(r)
-        res_b.append(r.finish()["sensors"])
+    res_b = [None] * len(objs)
+    live = list(range(len(objs)))
+    while live:
+        b.step(1)
+        for i in [i for i in live if objs[i].progress()[0] >= objs[i].progress()[1]]:
+            b.remove(objs[i])               # finished runs leave at once: no idle workgroups
+            res_b[i] = objs[i].finish()["sensors"]
+            live.remove(i)
     wall = time.perf_counter() - t0
     for r in objs:
         r.close()
@@ -222,6 +223,100 @@ def batching_form(args):
     print(json.dumps(rec))
 
 
+def serving_form(args):
+    """The serving form: a Poisson stream (4 requests/s, seed 5) of mixed 3D
+    single-particle and 2D custom requests (FP32 fields, FP64 sensors) sent to
+    the engine's request server with a 200 GB device-memory budget; reports
+    time-to-spectrum p50/p99 (submit to spectra out, measured by the server)
+    and runs/s (requests / (last spectrum - first arrival)).  Then the same
+    requests one at a time in FP64 (the baseline: create, step, spectra per
+    request, back to back), and two requests served alone to show that a
+    result does not depend on the mix."""
+    import numpy as np
+    from aether_amd import capi, workloads
+    be = capi.engine()
+    t0 = time.perf_counter()
+    reqs = workloads.serving_requests(args.requests, seed=5)
+    reqs = [(t, k, capi.mode_finalize(be, sp, k, q), q, info) for t, k, sp, q, info in reqs]
+    gen_s = time.perf_counter() - t0
+    print(f"serving form: {len(reqs)} requests generated in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    cell_steps = 0
+    for _, k, sp, q, info in reqs:
+        Nx, Ny, Nz = (capi.single_particle_dims(sp) if k == capi.KIND_SINGLE else capi.custom_dims(sp))
+        cell_steps += Nx * Ny * Nz * sp.Nt
+    budget = int(args.budget_gb * 1e9)
+    srv = capi.Server(be, budget)
+    ids = []
+    start = time.perf_counter()
+    for t, k, sp, q, info in reqs:
+        now = time.perf_counter() - start
+        if t > now:
+            time.sleep(t - now)
+        ids.append(srv.submit(k, sp, q))
+    outs, lat, waits = [], [], []
+    for rid in ids:
+        st = srv.wait(rid)
+        if st != capi.REQ_DONE:
+            raise RuntimeError(f"request {rid} ended in state {st}: {srv.error(rid)}")
+        o, tm = srv.result(rid)
+        outs.append(o)
+        lat.append(tm[2])
+        waits.append(tm[0])
+    end = time.perf_counter()
+    stats = srv.stats()
+    srv.close()
+    makespan = end - start
+    print(f"served: {makespan:.2f}s", file=sys.stderr, flush=True)
+    # one at a time, FP64
+    base = None
+    if not args.no_baseline:
+        tb = 0.0
+        for _, k, sp, q, info in reqs:
+            s64 = capi.ModeSpec(**{**sp.__dict__, "precision": capi.FP64})
+            r = capi.Run(be, k, s64, q)
+            r.synchronize()
+            t1 = time.perf_counter()
+            r.step(s64.Nt)
+            r.finish()
+            tb += time.perf_counter() - t1
+            r.close()
+        base = {"wall_s": round(tb, 3), "runs_per_s": round(len(reqs) / tb, 4)}
+        print(f"one at a time fp64: {tb:.2f}s", file=sys.stderr, flush=True)
+    # alone == mixed: the smallest 3D and the smallest 2D request, each on a fresh server
+    picks = []
+    for kind in (capi.KIND_SINGLE, capi.KIND_CUSTOM):
+        cand = [(i, r) for i, r in enumerate(reqs) if r[1] == kind]
+        if cand:
+            picks.append(min(cand, key=lambda c: c[1][4]["side"])[0])
+    diff = 0.0
+    for i in picks:
+        _, k, sp, q, info = reqs[i]
+        s1 = capi.Server(be, budget)
+        rid = s1.submit(k, sp, q)
+        s1.wait(rid)
+        o, _ = s1.result(rid)
+        s1.close()
+        diff = max(diff, float(np.max(np.abs(o - outs[i])) / max(1e-300, np.max(np.abs(outs[i])))))
+    lat = np.array(lat)
+    rec = {"form": "serving",
+           "workload": f"{len(reqs)} requests, Poisson 4/s (seed 5): 3D single-particle side 96-384 "
+                       f"(Drude+CP or Lorentz spheres), 2D custom side 512-4096 (cylinders), 2000-10000 steps, FP32",
+           "requests": len(reqs), "n_3d": sum(1 for r in reqs if r[1] == capi.KIND_SINGLE),
+           "cell_steps": cell_steps, "budget_GB": args.budget_gb,
+           "peak_resident_GB": round(stats["peak_bytes"] / 1e9, 2), "rounds": int(stats["rounds"]),
+           "time_to_spectrum_s": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                  "p99": round(float(np.percentile(lat, 99)), 3),
+                                  "max": round(float(lat.max()), 3)},
+           "queue_wait_s_p50": round(float(np.percentile(waits, 50)), 3),
+           "runs_per_s": round(len(reqs) / makespan, 4),
+           "Mcell_updates_per_s": round(cell_steps / makespan / 1e6, 1),
+           "one_at_a_time_fp64": base,
+           "speedup_vs_one_at_a_time_fp64": round(base["wall_s"] / makespan, 3) if base else None,
+           "alone_vs_mixed_max_rel_diff": diff,
+           "setup_s": round(gen_s, 1)}
+    print(json.dumps(rec))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,11 +330,15 @@ def main():
                     help="CPU baseline sample: (2 + S)-step run minus 2-step run of the oracle")
     ap.add_argument("--tiles", type=int, nargs=2, default=None, metavar=("E", "H"),
                     help="fixed tile configurations (0..31) instead of the autotuner's")
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving"],
                     help="plain: the headline single-particle run; precision: the 512x512x384 multilayer "
                          "stack in FP64, FP32 and FP16 storage with spectral errors against FP64; "
-                         "batching: 64 2D runs batched in FP32 against one-at-a-time FP64 and FP32")
+                         "batching: 64 2D runs batched in FP32 against one-at-a-time FP64 and FP32; "
+                         "serving: a Poisson request stream through the request server")
     ap.add_argument("--runs", type=int, default=64, help="runs of the batching form")
+    ap.add_argument("--requests", type=int, default=24, help="requests of the serving form")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="device-memory budget of the serving form")
+    ap.add_argument("--no-baseline", action="store_true", help="serving form without the FP64 one-at-a-time leg")
     ap.add_argument("--nt", type=int, default=8000, help="time steps of the precision form")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (rocprof counter calibration)")
@@ -266,6 +365,8 @@ def main():
         return precision_form(args)
     if args.form == "batching":
         return batching_form(args)
+    if args.form == "serving":
+        return serving_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
     nsteps = args.warmup + args.steps
     t_setup = time.perf_counter()

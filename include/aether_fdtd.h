@@ -337,6 +337,52 @@ int aefdtd_batch_autotune(aefdtd_batch *b, int out[2]);
  * resets and starts, 0 stops, -1 only reads. */
 int aefdtd_batch_profile(aefdtd_batch *b, int on, double *ms, long *chunks);
 
+/* ---- request server: continuous batching under an HBM budget ------------ */
+/* No counterpart in the reference (it runs one mode per process call,
+ * modes/lua_fdtd.cpp:296-320).  A worker thread owns the GPU work: requests
+ * (a mode driver kind + its FDTD_Mode fields + sensor requests, all copied at
+ * submit) wait in FIFO order until their estimated device bytes fit in the
+ * budget next to the runs already resident, then stay resident until their
+ * spectra are out.  Every round advances each resident run by 64 steps:
+ * custom-mode runs without dispersive materials share one batch per
+ * precision (aefdtd_batch_*), the others step on their own streams, so the
+ * work of all resident runs overlaps on the device.  Requests that arrive
+ * while others run join at the next round.  A request is served with the
+ * same arithmetic alone or in a mix, so its result does not depend on the
+ * mix. */
+typedef struct aefdtd_server aefdtd_server;
+#define AEFDTD_REQ_QUEUED    0
+#define AEFDTD_REQ_RUNNING   1
+#define AEFDTD_REQ_DONE      2
+#define AEFDTD_REQ_FAILED    3
+#define AEFDTD_REQ_CANCELLED 4
+int aefdtd_server_create(aefdtd_server **out, size_t budget_bytes);
+/* Cancels what is queued or resident, then stops the worker. */
+int aefdtd_server_destroy(aefdtd_server *s);
+/* kind as aefdtd_run_create (mode already finalized). Thread-safe. */
+int aefdtd_server_submit(aefdtd_server *s, int kind, const aefdtd_mode *m, int n_sensors,
+                         const aefdtd_sensor_req *sensors, long *id);
+int aefdtd_server_cancel(aefdtd_server *s, long id);
+/* Waits up to timeout_s (<0: no limit) for the request to end; *state gets
+ * its AEFDTD_REQ_* state either way. */
+int aefdtd_server_wait(aefdtd_server *s, long id, double timeout_s, int *state);
+/* Output of a finished request: the sensor outputs as aefdtd_run_finish
+ * writes them (n doubles at most), then for kind 0 the R and T spectra (Nl
+ * each).  times (may be NULL): [0] queue wait, [1] admission to spectrum,
+ * [2] submit to spectrum (time-to-spectrum), [3] resident device bytes, all
+ * in seconds / bytes.  A failed request returns its error code. */
+int aefdtd_server_result(aefdtd_server *s, long id, double *out, size_t n, double times[4]);
+const char *aefdtd_server_error(aefdtd_server *s, long id);
+/* Forgets an ended request. */
+int aefdtd_server_release(aefdtd_server *s, long id);
+/* out[0..7]: queued, resident, done, failed, bytes resident, peak bytes
+ * resident, budget, rounds. */
+int aefdtd_server_stats(aefdtd_server *s, double out[8]);
+/* Device bytes a driver run of `kind` will hold for mode m (the admission
+ * estimate; an upper bound of aefdtd_device_bytes over its grids). */
+int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int n_sensors,
+                          const aefdtd_sensor_req *sensors, size_t *bytes);
+
 /* ---- introspection used by the benchmark -------------------------------- */
 /* Per-kernel device time accumulated on the handle's stream with HIP events
  * while profiling is enabled: names "E","H","sensor","dft","rc" (the RC pole

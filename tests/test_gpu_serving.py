@@ -1,0 +1,106 @@
+"""GPU tests of the request server: a request's result is the same served
+alone, in a mix, or run directly; admission respects the device-memory
+budget; the admission estimate bounds the bytes a run holds; cancellation
+releases a resident run."""
+import time
+
+import numpy as np
+import pytest
+
+from aether_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _custom(seed, side, steps, precision=capi.FP32):
+    rng = workloads.MT19937_64(seed)
+    spec, sens, _ = workloads.sweep_run(rng, seed, side=side, steps=steps, Nl=8, precision=precision)
+    return capi.KIND_CUSTOM, capi.mode_finalize(capi.engine(), spec, capi.KIND_CUSTOM, sens), sens
+
+
+def _single(seed, side, steps, precision=capi.FP32):
+    spec, sens, _ = workloads.spheres_single_particle(n=side, D=2e-9, count=2, rmin=3e-9, rmax=5e-9, seed=seed,
+                                                      Nt=steps, Nl=8, pml=5, precision=precision)
+    return capi.KIND_SINGLE, capi.mode_finalize(capi.engine(), spec, capi.KIND_SINGLE, sens), sens
+
+
+def _mixed():
+    return [_custom(1, 40, 300), _single(2, 16, 200), _custom(3, 30, 170, capi.FP64), _custom(4, 52, 230),
+            _single(5, 14, 130, capi.FP64)]
+
+
+def _direct(kind, spec, sens):
+    r = capi.Run(capi.engine(), kind, spec, sens)
+    r.step(spec.Nt)
+    out = r.finish()["sensors"]
+    r.close()
+    return out
+
+
+def test_server_alone_equals_mixed():
+    reqs = _mixed()
+    be = capi.engine()
+    srv = capi.Server(be, 200 << 30)
+    ids = [srv.submit(k, sp, q) for k, sp, q in reqs]
+    mixed = []
+    for rid in ids:
+        assert srv.wait(rid, 120.0) == capi.REQ_DONE, srv.error(rid)
+        mixed.append(srv.result(rid)[0])
+    st = srv.stats()
+    srv.close()
+    assert st["done"] == len(reqs) and st["bytes_resident"] == 0
+    for (k, sp, q), m in zip(reqs, mixed):
+        s1 = capi.Server(be, 200 << 30)
+        rid = s1.submit(k, sp, q)
+        assert s1.wait(rid, 120.0) == capi.REQ_DONE
+        alone = s1.result(rid)[0]
+        s1.close()
+        assert np.array_equal(alone, m)
+        d = _direct(k, sp, q)
+        assert np.max(np.abs(d - m)) <= 1e-6 * np.max(np.abs(d))
+
+
+def test_server_budget_admission_and_estimate():
+    """With room for one request at a time the server serves them in FIFO
+    order, one resident at a time, and every run's bytes stay within its
+    admission estimate."""
+    reqs = [_custom(11, 40, 200), _custom(12, 44, 150), _single(13, 16, 120)]
+    be = capi.engine()
+    est = [capi.estimate_bytes(be, k, sp, q) for k, sp, q in reqs]
+    budget = int(max(est) * 1.5)
+    assert budget < sum(est[:2])
+    srv = capi.Server(be, budget)
+    ids = [srv.submit(k, sp, q) for k, sp, q in reqs]
+    times = []
+    for rid in ids:
+        assert srv.wait(rid, 120.0) == capi.REQ_DONE, srv.error(rid)
+        times.append(srv.result(rid)[1])
+    st = srv.stats()
+    srv.close()
+    assert st["peak_bytes"] <= budget
+    for t, e in zip(times, est):
+        assert 0 < t[3] <= e
+    # FIFO, one at a time: each later request waited for the previous ones
+    assert times[1][0] >= times[0][1] * 0.5 and times[2][0] >= times[1][1] * 0.5
+
+
+def test_server_cancel_releases_memory():
+    be = capi.engine()
+    srv = capi.Server(be, 200 << 30)
+    k, sp, q = _custom(21, 60, 200000)
+    rid = srv.submit(k, sp, q)
+    t0 = time.time()
+    while srv.stats()["resident"] < 1 and time.time() - t0 < 60:
+        time.sleep(0.01)
+    assert srv.cancel(rid) == capi.OK
+    assert srv.wait(rid, 60.0) == capi.REQ_CANCELLED
+    with pytest.raises(capi.FDTDError) as e:
+        srv.result(rid)
+    assert e.value.code == capi.ECANCELLED
+    st = srv.stats()
+    assert st["resident"] == 0 and st["bytes_resident"] == 0
+    # the server keeps serving afterwards
+    k2, sp2, q2 = _custom(22, 30, 100)
+    r2 = srv.submit(k2, sp2, q2)
+    assert srv.wait(r2, 60.0) == capi.REQ_DONE
+    srv.close()

@@ -67,7 +67,8 @@ def test_oscillators_on_grid_fp64(engine_be, oracle_be):
         g.add_oscillator(capi.NORMAL_Z, (2, 9, 4, 5, 3, 4), 400e-9, 800e-9)
         g.add_oscillator(capi.NORMAL_XM, (0, 18, 0, 15, 6, 7), 300e-9, 600e-9)
         grids.append(g)
-    fields = [rng.standard_normal(n[::-1]) * (1.0 if f < 3 else 1 / 377.0) for f in range(6)]
+    full = (grids[0].Nz, grids[0].Ny, grids[0].Nx)
+    fields = [rng.standard_normal(full) * (1.0 if f < 3 else 1 / 377.0) for f in range(6)]
     for g in grids:
         for f in range(6):
             g.write(f, fields[f])

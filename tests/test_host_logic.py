@@ -29,7 +29,8 @@ def test_oracle_mirrors_the_grid_abi():
                    "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
                    "aefdtd_set_storage_scale"}
     missing = [s for s in declared_symbols()
-               if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_")) or s in
+               if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_", "aefdtd_server_"))
+               and s != "aefdtd_estimate_bytes" or s in
                ("aefdtd_run_normal_incidence", "aefdtd_run_single_particle", "aefdtd_run_custom")]
     missing = [s for s in missing if not hasattr(lib, s.replace("aefdtd_", "aeoracle_"))]
     assert not missing, missing
@@ -139,3 +140,85 @@ def test_sweep_steps_and_sizes():
     spec, sens, info = workloads.sweep_run(rng, 0, side=300)
     assert spec.matsgrid.shape == (1, 300, 300) and info["steps"] == workloads.sweep_steps(300)
     assert set(spec.disable_fields) in ({capi.EX, capi.EY, capi.HZ}, {capi.EZ, capi.HX, capi.HY})
+
+
+def _no_gpu():
+    lib = capi.engine()
+    h = C.c_void_p()
+    p6 = (C.c_int * 6)(*([0] * 6))
+    rc = lib.fn("create")(C.byref(h), 4, 4, 4, 10, 1e-9, 1e-9, 1e-9, 1e-18, capi.MODE_CUSTOM, p6, p6, capi.FP64)
+    if rc == capi.OK:
+        lib.fn("destroy")(h)
+    return rc == capi.EDEVICE
+
+
+def _tiny_custom(n=8, Nt=20, precision=capi.FP32):
+    g = np.zeros((1, n, n), dtype=np.uint32)
+    src = capi.SourceReq(orientation=capi.NORMAL_Z, x1=0, x2=n, y1=0, y2=1, z1=0, z2=1)
+    sens = [capi.SensorReq(box=False, orientation=capi.NORMAL_Y, x1=0, x2=n, y1=n - 1, y2=n, z1=0, z2=1, Nl=4)]
+    spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(1.0)], Nt=Nt, pml=(4, 4, 4, 4, 0, 0),
+                         disable_fields=(capi.EX, capi.EY, capi.HZ), sources=[src], precision=precision)
+    return capi.mode_finalize(capi.engine(), spec, capi.KIND_CUSTOM, sens), sens
+
+
+def test_estimate_bytes_closed_form():
+    """Fields, material ids and CPML profiles of a grid without PML, poles or
+    sensors, plus the 2% + 8 MiB margin of the estimate."""
+    be = capi.engine()
+    g = np.zeros((6, 5, 7), dtype=np.uint32)
+    for prec, ss, cs in ((capi.FP64, 8, 8), (capi.FP32, 4, 4), (capi.FP16, 2, 4)):
+        spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(2.0)], Nt=10, precision=prec)
+        N = 7 * 5 * 6
+        b = 6 * N * ss + N + 6 * (7 + 5 + 6) * cs
+        assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
+    # a dispersive material adds the slot map and the per-cell pole state
+    spec = capi.ModeSpec(matsgrid=np.ones((6, 5, 7), dtype=np.uint32), materials=[capi.Material(1.0), workloads.GOLD],
+                         Nt=10, precision=capi.FP32)
+    N = 7 * 5 * 6
+    b = 6 * N * 4 + N + 6 * 18 * 4 + 4 * N + N * (4 + 3 * 4 + 3 * 3 * 16)
+    assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
+
+
+def test_server_refuses_request_over_budget():
+    """A request larger than the whole budget fails at admission with ENOMEM
+    (no device work), and the server keeps serving."""
+    spec, sens = _tiny_custom()
+    srv = capi.Server(capi.engine(), 1 << 20)
+    rid = srv.submit(capi.KIND_CUSTOM, spec, sens)
+    assert srv.wait(rid, 30.0) == capi.REQ_FAILED
+    with pytest.raises(capi.FDTDError) as e:
+        srv.result(rid)
+    assert e.value.code == capi.ENOMEM and "budget" in srv.error(rid)
+    st = srv.stats()
+    assert st["failed"] == 1 and st["resident"] == 0 and st["bytes_resident"] == 0
+    assert srv.release(rid) == capi.OK
+    srv.close()
+
+
+def test_server_reports_device_failure_without_gpu():
+    """Without a usable GPU, admission fails with EDEVICE and the error text."""
+    if not _no_gpu():
+        pytest.skip("a GPU is visible: the device-failure path is not reachable")
+    spec, sens = _tiny_custom()
+    srv = capi.Server(capi.engine(), 1 << 34)
+    ids = [srv.submit(capi.KIND_CUSTOM, spec, sens) for _ in range(3)]
+    for rid in ids:
+        assert srv.wait(rid, 30.0) == capi.REQ_FAILED
+        with pytest.raises(capi.FDTDError) as e:
+            srv.result(rid)
+        assert e.value.code == capi.EDEVICE
+    assert srv.stats()["failed"] == 3
+    srv.close()
+
+
+def test_server_rejects_bad_submissions():
+    spec, sens = _tiny_custom()
+    srv = capi.Server(capi.engine(), 1 << 30)
+    with pytest.raises(capi.FDTDError):
+        srv.submit(7, spec, sens)
+    bad = capi.ModeSpec(**{**spec.__dict__, "precision": 9})
+    with pytest.raises(capi.FDTDError):
+        srv.submit(capi.KIND_CUSTOM, bad, sens)
+    with pytest.raises(capi.FDTDError):
+        srv.wait(12345, 0.0)
+    srv.close()
