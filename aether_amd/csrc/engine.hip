@@ -341,11 +341,17 @@ constexpr int kKZ[4] = {4, 8, 16, 32};
 
 template <typename S, typename T, int BY, int KZ>
 void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
+    if (a.Nz == 1)
+        hipLaunchKernelGGL((k_update_E2<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
+    else
+        hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
 }
 template <typename S, typename T, int BY, int KZ>
 void launch_H_cfg(const HArgs<S, T> &a, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_update_H<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+    if (a.Nz == 1)
+        hipLaunchKernelGGL((k_update_H2<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_update_H<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
 }
 
 template <typename S, typename T, int BY>
@@ -369,12 +375,19 @@ void launch_H_by(const HArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
 
 // Fills the tile geometry of an argument block for configuration cfg and
 // returns the number of workgroups.
+// A grid one cell thick in z runs the 2D passes, which march the KZ depth
+// along y instead.
 template <typename A>
 int tile_geometry(A &a, int cfg) {
     const int by = kBY[(cfg >> 0) & 3], kz = kKZ[(cfg >> 2) & 3];
     a.nbx = (a.Nx + BX - 1) / BX;
-    a.nby = (a.Ny + by - 1) / by;
-    a.nbz = (a.Nz + kz - 1) / kz;
+    if (a.Nz == 1) {
+        a.nby = (a.Ny + by * kz - 1) / (by * kz);
+        a.nbz = 1;
+    } else {
+        a.nby = (a.Ny + by - 1) / by;
+        a.nbz = (a.Nz + kz - 1) / kz;
+    }
     a.remap = (cfg >> 4) & 1;
     return a.nbx * a.nby * a.nbz;
 }

@@ -128,11 +128,32 @@ struct MatL {
 };
 
 // ---------------------------------------------------------------- E cell
-// Everything after the loads of one E cell; dH* are the curl differences.
+// CPML psi of an E cell in the grid's slab arrays (x slab: (pxm+pxp, Ny, Nz),
+// y slab: (Nx, pym+pyp, Nz), z slab: (Nx, Ny, pzm+pzp)); the argument is the
+// cell's slab index along the PML axis.
 template <typename S, typename T>
-__device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
-                                       T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
-                                       T ikx, T iky, int nx_, int ny_, int ostep) {
+struct GlobalPsiE {
+    const EArgs<S, T> &a;
+    int i, j, k;
+    __device__ __forceinline__ T &Exy(int n) const { return a.pExy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+    __device__ __forceinline__ T &Exz(int n) const { return a.pExz[i + a.Nx * j + a.Nx * a.Ny * n]; }
+    __device__ __forceinline__ T &Eyx(int n) const { return a.pEyx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
+    __device__ __forceinline__ T &Eyz(int n) const { return a.pEyz[i + a.Nx * j + a.Nx * a.Ny * n]; }
+    __device__ __forceinline__ T &Ezx(int n) const { return a.pEzx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
+    __device__ __forceinline__ T &Ezy(int n) const { return a.pEzy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+};
+
+template <typename T>
+struct V3 {
+    T x, y, z;
+};
+
+// The new E of one cell from its old E and the curl differences dH*; the
+// CPML psi live where P says.
+template <typename S, typename T, class P>
+__device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T> &M, int i, int j, int k, int id,
+                                             T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x,
+                                             T dHx_y, T ikx, T iky, int nx_, int ny_, int ostep, P ps) {
     const T ikz = a.pf.ikE[2][k];
     // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
     T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
@@ -169,12 +190,10 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
         if (k >= 1 && k < a.pml[4]) nz_ = k;
         else if (k >= a.Nz - a.pml[5] + 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
     }
-    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
-    const int NxNy = a.Nx * a.Ny;
     if (a.en & 1) {
         if (py) {
             if (ny_ >= 0) {
-                T &q = a.pExy[i + a.Nx * (ny_ + wy * k)];
+                T &q = ps.Exy(ny_);
                 q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHz_y;
                 ex += M.C4 * q;
             }
@@ -182,7 +201,7 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = a.pExz[i + a.Nx * j + NxNy * nz_];
+                T &q = ps.Exz(nz_);
                 q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHy_z;
                 ex -= M.C4 * q;
             }
@@ -192,7 +211,7 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
     if (a.en & 2) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = a.pEyx[nx_ + wx * (j + a.Ny * k)];
+                T &q = ps.Eyx(nx_);
                 q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHz_x;
                 ey -= M.C4 * q;
             }
@@ -200,7 +219,7 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = a.pEyz[i + a.Nx * j + NxNy * nz_];
+                T &q = ps.Eyz(nz_);
                 q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHx_z;
                 ey += M.C4 * q;
             }
@@ -210,7 +229,7 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
     if (a.en & 4) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = a.pEzx[nx_ + wx * (j + a.Ny * k)];
+                T &q = ps.Ezx(nx_);
                 q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHy_x;
                 ez += M.C4 * q;
             }
@@ -218,7 +237,7 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
         }
         if (py) {
             if (ny_ >= 0) {
-                T &q = a.pEzy[i + a.Nx * (ny_ + wy * k)];
+                T &q = ps.Ezy(ny_);
                 q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHx_y;
                 ez -= M.C4 * q;
             }
@@ -237,9 +256,19 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
             else ez += v;
         }
     }
-    if (a.lm & 1) a.Ex.set(id, ex);
-    if (a.lm & 2) a.Ey.set(id, ey);
-    if (a.lm & 4) a.Ez.set(id, ez);
+    return V3<T>{ex, ey, ez};
+}
+
+// Everything after the loads of one E cell, with the grid's CPML state.
+template <typename S, typename T>
+__device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
+                                       T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
+                                       T ikx, T iky, int nx_, int ny_, int ostep) {
+    const V3<T> e = e_cell_vals(a, M, i, j, k, id, ex0, ey0, ez0, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky,
+                                nx_, ny_, ostep, GlobalPsiE<S, T>{a, i, j, k});
+    if (a.lm & 1) a.Ex.set(id, e.x);
+    if (a.lm & 2) a.Ey.set(id, e.y);
+    if (a.lm & 4) a.Ez.set(id, e.z);
 }
 
 // CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
@@ -370,6 +399,66 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
     }
 }
 
+// 2D E pass (Nz == 1) over tile w: each lane marches KY consecutive rows in y
+// and carries the previous row's Hx, Hz in registers, so the per-lane set-up
+// is paid once per KY cells and every row load is a contiguous 64-wide read.
+// Differences along the one-cell z axis are exactly zero.
+template <typename S, typename T, int BY, int KY>
+__device__ __forceinline__ void e_tile2d(const EArgs<S, T> &a, const MatL<T> *sm, int w, int ostep) {
+    int bx, by, bz;
+    tile_of(w, a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    const int i = bx * BX + threadIdx.x;
+    const int j0 = (by * BY + threadIdx.y) * KY;
+    if (i >= a.Nx || j0 >= a.Ny) return;
+    const int j1 = min(j0 + KY, a.Ny);
+    const int oim = (i == 0) ? a.Nx - 1 : -1;
+    const int lm = a.lm;
+    const T zero = 0;
+    const int jm0 = (j0 == 0) ? a.Ny - 1 : j0 - 1;
+    T hx_jm = (lm & 8) ? a.Hx[i + a.Nx * jm0] : zero;
+    T hz_jm = (lm & 32) ? a.Hz[i + a.Nx * jm0] : zero;
+    const T ikx = a.pf.ikE[0][i];
+    const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
+    struct Ld {
+        T hx, hy, hz, hz_im, hy_im, ex, ey, ez;
+        int m;
+    };
+    auto load = [&](int id) {
+        Ld v;
+        v.hx = zero; v.hy = zero; v.hy_im = zero; v.hz = zero; v.hz_im = zero;
+        v.ex = zero; v.ey = zero; v.ez = zero;
+        if (lm & 8) v.hx = a.Hx[id];
+        if (lm & 16) { v.hy = a.Hy[id]; v.hy_im = a.Hy[id + oim]; }
+        if (lm & 32) { v.hz = a.Hz[id]; v.hz_im = a.Hz[id + oim]; }
+        if (lm & 1) v.ex = a.Ex[id];
+        if (lm & 2) v.ey = a.Ey[id];
+        if (lm & 4) v.ez = a.Ez[id];
+        v.m = a.mat[id];
+        return v;
+    };
+    Ld cur = load(i + a.Nx * j0);
+    for (int j = j0; j < j1; j++) {
+        const int id = i + a.Nx * j;
+        Ld nxt = cur;
+        if (j + 1 < j1) nxt = load(id + a.Nx);
+        const T iky = a.pf.ikE[1][j];
+        const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+        e_cell(a, sm[cur.m], cur.m, i, j, 0, id, cur.ex, cur.ey, cur.ez, cur.hz - hz_jm, zero, zero, cur.hz - cur.hz_im,
+               cur.hy - cur.hy_im, cur.hx - hx_jm, ikx, iky, nx_, ny_, ostep);
+        hx_jm = cur.hx;
+        hz_jm = cur.hz;
+        cur = nxt;
+    }
+}
+
+template <typename S, typename T, int BY, int KY>
+__global__ __launch_bounds__(BX *BY) void k_update_E2(const EArgs<S, T> a) {
+    extern __shared__ char lds_raw[];
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    stage_mats(a, sm);
+    e_tile2d<S, T, BY, KY>(a, sm, blockIdx.x, a.ostep);
+}
+
 template <typename S, typename T, int BY, int KZ>
 __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
     extern __shared__ char lds_raw[];          // nmat entries, sized at launch
@@ -408,7 +497,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_E_batch(const EArgs<S, T> *__
     const int w = blockIdx.x - wg0[r];
     if (t >= tend[r] || w >= a.nbx * a.nby * a.nbz) return;
     stage_mats(a, sm);
-    e_tile<S, T, BY, KZ>(a, sm, w, t);
+    if (a.Nz == 1) e_tile2d<S, T, BY, KZ>(a, sm, w, t);
+    else e_tile<S, T, BY, KZ>(a, sm, w, t);
 }
 
 // E pass of a 1 x 1 x Nz column (the auxiliary grid of the TFSF source):
@@ -430,8 +520,22 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
 
 // ---------------------------------------------------------------- H cell
 template <typename S, typename T>
-__device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
-                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
+struct GlobalPsiH {
+    const HArgs<S, T> &a;
+    int i, j, k;
+    __device__ __forceinline__ T &Hxy(int n) const { return a.pHxy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+    __device__ __forceinline__ T &Hxz(int n) const { return a.pHxz[i + a.Nx * j + a.Nx * a.Ny * n]; }
+    __device__ __forceinline__ T &Hyx(int n) const { return a.pHyx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
+    __device__ __forceinline__ T &Hyz(int n) const { return a.pHyz[i + a.Nx * j + a.Nx * a.Ny * n]; }
+    __device__ __forceinline__ T &Hzx(int n) const { return a.pHzx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
+    __device__ __forceinline__ T &Hzy(int n) const { return a.pHzy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+};
+
+// The new H of one cell (in place in hx, hy, hz) from the curl differences dE*.
+template <typename S, typename T, class P>
+__device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, int k, T &hx, T &hy, T &hz, T dEy_z,
+                                            T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_,
+                                            P ps) {
     const T ikz = a.pf.ikH[2][k];
     // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
     if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
@@ -440,12 +544,10 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
     // CPML-H + PEC walls (core/fdtd_pml.cpp:737-929)
     const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
     const int nz_ = slabH(k, a.Nz, a.pml[4], a.pml[5]);
-    const int wx = a.pml[0] + a.pml[1], wy = a.pml[2] + a.pml[3];
-    const int NxNy = a.Nx * a.Ny;
     if (a.en & 8) {
         if (py) {
             if (ny_ >= 0) {
-                T &q = a.pHxy[i + a.Nx * (ny_ + wy * k)];
+                T &q = ps.Hxy(ny_);
                 q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEz_y;
                 hx -= a.dtm * q;
             }
@@ -453,7 +555,7 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = a.pHxz[i + a.Nx * j + NxNy * nz_];
+                T &q = ps.Hxz(nz_);
                 q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEy_z;
                 hx += a.dtm * q;
             }
@@ -463,14 +565,14 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
     if (a.en & 16) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = a.pHyx[nx_ + wx * (j + a.Ny * k)];
+                T &q = ps.Hyx(nx_);
                 q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEz_x;
                 hy += a.dtm * q;
             }
             if (i == a.Nx - 1) hy = 0;
         }
         if (pz && nz_ >= 0) {
-            T &q = a.pHyz[i + a.Nx * j + NxNy * nz_];
+            T &q = ps.Hyz(nz_);
             q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEx_z;
             hy -= a.dtm * q;
         }
@@ -480,7 +582,7 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
     if (a.en & 32) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = a.pHzx[nx_ + wx * (j + a.Ny * k)];
+                T &q = ps.Hzx(nx_);
                 q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEy_x;
                 hz -= a.dtm * q;
             }
@@ -488,7 +590,7 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
         }
         if (py) {
             if (ny_ >= 0) {
-                T &q = a.pHzy[i + a.Nx * (ny_ + wy * k)];
+                T &q = ps.Hzy(ny_);
                 q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEx_y;
                 hz += a.dtm * q;
             }
@@ -513,6 +615,13 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
             if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
         }
     }
+}
+
+template <typename S, typename T>
+__device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
+                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
+    h_cell_vals(a, i, j, k, hx, hy, hz, dEy_z, dEx_z, dEz_y, dEz_x, dEx_y, dEy_x, ikx, iky, nx_, ny_,
+                GlobalPsiH<S, T>{a, i, j, k});
     if (a.lm & 8) a.Hx.set(id, hx);
     if (a.lm & 16) a.Hy.set(id, hy);
     if (a.lm & 32) a.Hz.set(id, hz);
@@ -557,6 +666,46 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     }
 }
 
+// 2D H pass (Nz == 1): rows ascending; E(j+1) of row j is E(j) of row j+1.
+template <typename S, typename T, int BY, int KY>
+__device__ __forceinline__ void h_tile2d(const HArgs<S, T> &a, int w) {
+    int bx, by, bz;
+    tile_of(w, a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
+    const int i = bx * BX + threadIdx.x;
+    const int j0 = (by * BY + threadIdx.y) * KY;
+    if (i >= a.Nx || j0 >= a.Ny) return;
+    const int j1 = min(j0 + KY, a.Ny);
+    const int oip = (i == a.Nx - 1) ? -(a.Nx - 1) : 1;
+    const T ikx = a.pf.ikH[0][i];
+    const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
+    const int lm = a.lm;
+    const T zero = 0;
+    T ex_j = (lm & 1) ? a.Ex[i + a.Nx * j0] : zero;
+    T ez_j = (lm & 4) ? a.Ez[i + a.Nx * j0] : zero;
+    for (int j = j0; j < j1; j++) {
+        const int id = i + a.Nx * j;
+        const int idp = (j == a.Ny - 1) ? i : id + a.Nx;      // (j+1) with wrap
+        T ex_jp = zero, ez_jp = zero, ey = zero, ey_ip = zero, ez_ip = zero;
+        if (lm & 1) ex_jp = a.Ex[idp];
+        if (lm & 2) { ey = a.Ey[id]; ey_ip = a.Ey[id + oip]; }
+        if (lm & 4) { ez_jp = a.Ez[idp]; ez_ip = a.Ez[id + oip]; }
+        const T hx = (lm & 8) ? a.Hx[id] : zero;
+        const T hy = (lm & 16) ? a.Hy[id] : zero;
+        const T hz = (lm & 32) ? a.Hz[id] : zero;
+        const T iky = a.pf.ikH[1][j];
+        const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
+        h_cell(a, i, j, 0, id, hx, hy, hz, zero, zero, ez_jp - ez_j, ez_ip - ez_j, ex_jp - ex_j, ey_ip - ey, ikx, iky,
+               nx_, ny_);
+        ex_j = ex_jp;
+        ez_j = ez_jp;
+    }
+}
+
+template <typename S, typename T, int BY, int KY>
+__global__ __launch_bounds__(BX *BY) void k_update_H2(const HArgs<S, T> a) {
+    h_tile2d<S, T, BY, KY>(a, blockIdx.x);
+}
+
 template <typename S, typename T, int BY, int KZ>
 __global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
     h_tile<S, T, BY, KZ>(a, blockIdx.x);
@@ -571,7 +720,8 @@ __global__ __launch_bounds__(BX *BY) void k_update_H_batch(const HArgs<S, T> *__
     const HArgs<S, T> &a = d[r];
     const int w = blockIdx.x - wg0[r];
     if (tcur[r] + s >= tend[r] || w >= a.nbx * a.nby * a.nbz) return;
-    h_tile<S, T, BY, KZ>(a, w);
+    if (a.Nz == 1) h_tile2d<S, T, BY, KZ>(a, w);
+    else h_tile<S, T, BY, KZ>(a, w);
 }
 
 template <typename S, typename T>

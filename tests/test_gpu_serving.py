@@ -67,8 +67,8 @@ def test_server_budget_admission_and_estimate():
     reqs = [_custom(11, 40, 200), _custom(12, 44, 150), _single(13, 16, 120)]
     be = capi.engine()
     est = [capi.estimate_bytes(be, k, sp, q) for k, sp, q in reqs]
-    budget = int(max(est) * 1.5)
-    assert budget < sum(est[:2])
+    budget = max(est)
+    assert all(budget < est[a] + est[b] for a in range(3) for b in range(a))
     srv = capi.Server(be, budget)
     ids = [srv.submit(k, sp, q) for k, sp, q in reqs]
     times = []
