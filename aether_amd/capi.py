@@ -141,6 +141,8 @@ def _sig(lib, p):
         "batch_synchronize": (i, [vp]),
         "batch_set_tiles": (i, [vp, i, i]),
         "batch_autotune": (i, [vp, _ip]),
+        "batch_set_skew": (i, [vp, i]),
+        "batch_info": (i, [vp, _ip]),
         "batch_profile": (i, [vp, i, _dp, C.POINTER(C.c_long)]),
         "server_create": (i, [C.POINTER(vp), C.c_size_t]),
         "server_destroy": (i, [vp]),
@@ -152,6 +154,7 @@ def _sig(lib, p):
         "server_release": (i, [vp, C.c_long]),
         "server_stats": (i, [vp, _dp]),
         "estimate_bytes": (i, [i, C.POINTER(CMode), i, C.POINTER(CSensorReq), C.POINTER(C.c_size_t)]),
+        "stream_bandwidth": (i, [C.c_size_t, i, _dp]),
         # oracle only
         "step": (i, [vp, i]),
         "set_threads": (None, [i]),
@@ -599,9 +602,17 @@ class Batch:
         self._c(self.be.fn("batch_set_tiles")(self.h, tile_e, tile_h))
 
     def autotune(self):
-        out = (C.c_int * 2)()
+        out = (C.c_int * 3)()
         self._c(self.be.fn("batch_autotune")(self.h, out))
         return list(out)
+
+    def set_skew(self, code):
+        self._c(self.be.fn("batch_set_skew")(self.h, code))
+
+    def info(self):
+        out = (C.c_int * 4)()
+        self._c(self.be.fn("batch_info")(self.h, out))
+        return dict(zip(("members", "running", "skewed", "skew"), list(out)))
 
     def profile(self, on):
         ms, n = C.c_double(), C.c_long()

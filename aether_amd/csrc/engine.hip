@@ -840,6 +840,67 @@ void launch_Hb_by(int kzi, int nwg, hipStream_t st, const void *d, const int *wg
     }
 }
 
+// Time-skewed 2D passes: tile heights (small, large) per arithmetic type and
+// depths 2, 4, 8 steps.
+template <typename T>
+constexpr int tb_ly(int lyi) {
+    return sizeof(T) == 8 ? (lyi ? 32 : 16) : (lyi ? 48 : 32);
+}
+constexpr int kTS[3] = {2, 4, 8};
+
+bool tb_code_valid(int precision, int code) {
+    if (code < 0) return true;
+    const int lyi = code & 3, tsi = (code >> 2) & 3;
+    if (lyi > 1 || tsi > 2) return false;
+    const int ly = precision == AEFDTD_FP64 ? (lyi ? 32 : 16) : (lyi ? 48 : 32);
+    return ly - 2 * kTS[tsi] - 1 >= 4;
+}
+
+// 0 = TM, 1 = TE, -1 = the grid is stepped by the per-step passes.
+int tb_polarisation(const aefdtd_grid *g) {
+    if (g->Nz != 1 || g->n_disp || g->tfsf_aux || g->inj_k >= 0 || g->precision == AEFDTD_FP16) return -1;
+    const bool *e = g->enable;
+    int pol = -1;
+    if (!e[0] && !e[1] && e[2] && e[3] && e[4] && !e[5]) pol = 0;
+    else if (e[0] && e[1] && !e[2] && !e[3] && !e[4] && e[5]) pol = 1;
+    if (pol < 0) return -1;
+    for (const OscDev &o : g->osc)
+        if ((pol == 0) != (o.comp == 2)) return -1;
+    return pol;
+}
+
+template <typename S, typename T, int POL, int LY, int TS>
+void launch_tb_k(int nwg, size_t lds, hipStream_t st, const void *ea, const void *ha, const int *mem, const int *wg0,
+                 int nm, const int *tc, const int *te, const void *samp, const int *sbeg, int s0) {
+    hipLaunchKernelGGL((k_tb2d_batch<S, T, POL, LY, TS>), dim3(nwg), dim3(64, TB_WAVES, 1), lds, st,
+                       (const EArgs<S, T> *)ea, (const HArgs<S, T> *)ha, mem, wg0, nm, tc, te,
+                       (const SampB<S, T> *)samp, sbeg, s0);
+}
+
+template <typename S, typename T, int POL, int LY, int TS>
+hipError_t tb_attr(size_t lds) {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2d_batch<S, T, POL, LY, TS>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+// Calls f.template operator()<POL, LY, TS>() for a polarisation and a tile code.
+template <typename T, class F>
+void with_tb(int pol, int code, F &&f) {
+    constexpr int L0 = tb_ly<T>(0), L1 = tb_ly<T>(1);
+    const int lyi = code & 3, tsi = (code >> 2) & 3;
+    auto by_ts = [&]<int P, int L>() {
+        if (tsi == 0) f.template operator()<P, L, 2>();
+        else if (tsi == 1) f.template operator()<P, L, 4>();
+        else if constexpr (L - 2 * 8 - 1 >= 4) f.template operator()<P, L, 8>();
+    };
+    auto by_ly = [&]<int P>() {
+        if (lyi) by_ts.template operator()<P, L1>();
+        else by_ts.template operator()<P, L0>();
+    };
+    if (pol == 0) by_ly.template operator()<0>();
+    else by_ly.template operator()<1>();
+}
+
 }  // namespace
 
 struct aefdtd::Batch {
@@ -856,6 +917,13 @@ struct aefdtd::Batch {
     int nwgE = 0, nwgH = 0, nsamp = 0, ndft = 0;
     long maxP = 0;
     size_t ldsE = 0;
+    // time-skewed 2D TE/TM members: tile code (small/large tile height | depth
+    // index << 2), -1 = off; member lists and workgroup prefixes per polarisation
+    int tb_code = 1 | (1 << 2);
+    int *d_mem[2] = {nullptr, nullptr}, *d_wgT[2] = {nullptr, nullptr}, *d_sbeg = nullptr;
+    int nmem[2] = {0, 0}, nwgT[2] = {0, 0}, nstep = 0;  // nstep: members stepped by the per-step passes
+    int nsamp_step = 0;                    // leading sample entries that belong to those members
+    size_t ldsT = 0;
     std::vector<int> tcur;                 // host mirror of d_tcur
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool prof_on = false;
@@ -872,7 +940,8 @@ void batch_free_tables(Batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
     for (void **p : {&b->d_ea, &b->d_ha, &b->d_samp, &b->d_dft, (void **)&b->d_wgE, (void **)&b->d_wgH,
-                     (void **)&b->d_tcur, (void **)&b->d_tend})
+                     (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_mem[0], (void **)&b->d_mem[1],
+                     (void **)&b->d_wgT[0], (void **)&b->d_wgT[1], (void **)&b->d_sbeg})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
 
@@ -895,12 +964,15 @@ int batch_build(Batch *b) {
         b->dirty = false;
         return AEFDTD_OK;
     }
+    const bool tb_on = b->tb_code >= 0 && std::is_same_v<S, T>;
+    const int ly = tb_ly<T>(b->tb_code & 3), TS = kTS[(b->tb_code >> 2) & 3];
+    const int IX = TBX - 2 * TS - 1, IY = ly - 2 * TS - 1;
     std::vector<EArgs<S, T>> ea;
     std::vector<HArgs<S, T>> ha;
-    std::vector<int> wgE, wgH, tend(n);
+    std::vector<int> wgE, wgH, tend(n), pol(n, -1), mem[2], wgT[2], sbeg(2 * n, 0);
     std::vector<SampB<S, T>> samp;
     std::vector<DftB<T>> dft;
-    int totE = 0, totH = 0, maxmat = 1;
+    int totE = 0, totH = 0, totT[2] = {0, 0}, maxmat = 1;
     b->maxP = 0;
     for (int r = 0; r < n; r++) {
         aefdtd_grid *g = b->members[r];
@@ -911,49 +983,93 @@ int batch_build(Batch *b) {
         }
         for (Sensor *s : g->sensors)
             if ((rc = ensure_coefs(g, *s, g->Nt))) { b->err = g->err; return rc; }
+        if (tb_on) pol[r] = tb_polarisation(g);
         EArgs<S, T> a = e_args<S, T>(g);
-        const int nb = tile_geometry(a, b->tile_e);
+        HArgs<S, T> h = h_args<S, T>(g);
+        const int nb = tile_geometry(a, b->tile_e), nh = tile_geometry(h, b->tile_h);
         wgE.push_back(totE);
-        totE += (nb + 7) / 8 * 8;
+        wgH.push_back(totH);
+        if (pol[r] < 0) {
+            totE += (nb + 7) / 8 * 8;
+            totH += (nh + 7) / 8 * 8;
+        } else {
+            const int p = pol[r];
+            mem[p].push_back(r);
+            wgT[p].push_back(totT[p]);
+            totT[p] += ((g->Nx + IX - 1) / IX) * ((g->Ny + IY - 1) / IY);
+        }
         maxmat = std::max(maxmat, a.nmat);
         ea.push_back(a);
-        HArgs<S, T> h = h_args<S, T>(g);
-        const int nh = tile_geometry(h, b->tile_h);
-        wgH.push_back(totH);
-        totH += (nh + 7) / 8 * 8;
         ha.push_back(h);
         b->tcur[r] = g->tstep;
         tend[r] = g->Nt;
-        for (Sensor *s : g->sensors) {
-            SampB<S, T> q;
-            q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off};
-            for (int c = 0; c < 6; c++) q.F[c] = fld<S, T>(g, c);
-            q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.member = r;
-            q.Ptot = g->stage_P;
-            q.stage = (T *)g->d_stage;
-            samp.push_back(q);
-            dft.push_back(DftB<T>{(const T *)g->d_stage, g->stage_P, s->off, s->P, s->Nl, r,
-                                  (const double2 *)s->d_coef, s->d_acc});
-            b->maxP = std::max(b->maxP, s->P);
-        }
     }
+    // sample entries: members of the per-step passes first, then the time-skewed ones
+    for (int pass = 0; pass < 2; pass++)
+        for (int r = 0; r < n; r++) {
+            if ((pass == 0) != (pol[r] < 0)) continue;
+            aefdtd_grid *g = b->members[r];
+            sbeg[2 * r] = (int)samp.size();
+            for (Sensor *s : g->sensors) {
+                SampB<S, T> q;
+                q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off};
+                for (int c = 0; c < 6; c++) q.F[c] = fld<S, T>(g, c);
+                q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.member = r;
+                q.Ptot = g->stage_P;
+                q.stage = (T *)g->d_stage;
+                samp.push_back(q);
+                dft.push_back(DftB<T>{(const T *)g->d_stage, g->stage_P, s->off, s->P, s->Nl, r,
+                                      (const double2 *)s->d_coef, s->d_acc});
+                b->maxP = std::max(b->maxP, s->P);
+            }
+            sbeg[2 * r + 1] = (int)samp.size();
+            if (pass == 0) b->nsamp_step = (int)samp.size();
+        }
     wgE.push_back(totE);
     wgH.push_back(totH);
+    for (int p = 0; p < 2; p++) {
+        wgT[p].push_back(totT[p]);
+        b->nmem[p] = (int)mem[p].size();
+        b->nwgT[p] = totT[p];
+    }
     b->nwgE = totE;
     b->nwgH = totH;
     b->nsamp = (int)samp.size();
     b->ndft = (int)dft.size();
     b->ldsE = maxmat * sizeof(MatL<T>);
+    b->ldsT = ((maxmat * sizeof(MatL<T>) + 15) & ~(size_t)15) + 3 * (size_t)TBX * ly * sizeof(T);
     int rc;
     if ((rc = batch_upload(b, &b->d_ea, ea)) || (rc = batch_upload(b, &b->d_ha, ha)) ||
         (rc = batch_upload(b, (void **)&b->d_wgE, wgE)) || (rc = batch_upload(b, (void **)&b->d_wgH, wgH)) ||
         (rc = batch_upload(b, (void **)&b->d_tcur, b->tcur)) || (rc = batch_upload(b, (void **)&b->d_tend, tend)) ||
-        (rc = batch_upload(b, &b->d_samp, samp)) || (rc = batch_upload(b, &b->d_dft, dft)))
+        (rc = batch_upload(b, &b->d_samp, samp)) || (rc = batch_upload(b, &b->d_dft, dft)) ||
+        (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)))
         return rc;
-    // one chunk: KS x (E, H, samples), the DFT flush, the step advance
+    for (int p = 0; p < 2; p++)
+        if (b->nmem[p]) {
+            if ((rc = batch_upload(b, (void **)&b->d_mem[p], mem[p])) ||
+                (rc = batch_upload(b, (void **)&b->d_wgT[p], wgT[p])))
+                return rc;
+            hipError_t ae = hipSuccess;
+            if constexpr (std::is_same_v<S, T>)
+                with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() { ae = tb_attr<S, T, P, L, TSK>(b->ldsT); });
+            BHIPOK(b, ae);
+        }
+    // one chunk: the time-skewed blocks, KS x (E, H, samples) of the other
+    // members, the DFT flush, the step advance
     BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    if constexpr (std::is_same_v<S, T>) {
+        for (int s0 = 0; s0 < KS; s0 += TS)
+            for (int p = 0; p < 2; p++)
+                if (b->nmem[p])
+                    with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() {
+                        launch_tb_k<S, T, P, L, TSK>(b->nwgT[p], b->ldsT, b->stream, b->d_ea, b->d_ha, b->d_mem[p],
+                                                     b->d_wgT[p], b->nmem[p], b->d_tcur, b->d_tend, b->d_samp,
+                                                     b->d_sbeg, s0);
+                    });
+    }
     const int ebi = b->tile_e & 3, ekz = (b->tile_e >> 2) & 3, hbi = b->tile_h & 3, hkz = (b->tile_h >> 2) & 3;
-    for (int s = 0; s < KS; s++) {
+    for (int s = 0; s < KS && totE > 0; s++) {
         switch (ebi) {
             case 0: launch_Eb_by<S, T, 1>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
             case 1: launch_Eb_by<S, T, 2>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
@@ -966,8 +1082,8 @@ int batch_build(Batch *b) {
             case 2: launch_Hb_by<S, T, 4>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
             default: launch_Hb_by<S, T, 8>(hkz, totH, b->stream, b->d_ha, b->d_wgH, n, b->d_tcur, b->d_tend, s); break;
         }
-        if (b->nsamp)
-            hipLaunchKernelGGL((k_sample_batch<S, T>), dim3((unsigned)((b->maxP + 255) / 256), (unsigned)b->nsamp),
+        if (b->nsamp_step)
+            hipLaunchKernelGGL((k_sample_batch<S, T>), dim3((unsigned)((b->maxP + 255) / 256), (unsigned)b->nsamp_step),
                                dim3(256), 0, b->stream, (const SampB<S, T> *)b->d_samp, b->d_tcur, b->d_tend, s);
     }
     if (b->ndft)
@@ -999,6 +1115,7 @@ int batch_create(Batch **out, int precision) {
     if (precision < AEFDTD_FP64 || precision > AEFDTD_FP16) return AEFDTD_EINVAL;
     auto *b = new Batch();
     b->precision = precision;
+    b->tb_code = precision == AEFDTD_FP16 ? -1 : (1 | (1 << 2));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         set_global_error("batch stream/event creation failed");
@@ -1102,6 +1219,21 @@ int batch_set_tiles(Batch *b, int tile_e, int tile_h) {
     b->tile_h = tile_h;
     b->dirty = true;
     return AEFDTD_OK;
+}
+
+int batch_set_skew(Batch *b, int code) {
+    if (!tb_code_valid(b->precision, code)) return AEFDTD_EINVAL;
+    b->tb_code = code;
+    b->dirty = true;
+    return AEFDTD_OK;
+}
+
+int batch_skew(const Batch *b) { return b->tb_code; }
+
+int batch_skewed_members(const Batch *b) {
+    int n = 0;
+    for (aefdtd_grid *g : b->members) n += tb_polarisation(g) >= 0;
+    return b->tb_code >= 0 && b->precision != AEFDTD_FP16 ? n : 0;
 }
 
 void batch_profile(Batch *b, int on, double *ms, long *chunks) {
@@ -1645,6 +1777,52 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
     if (total_ms) *total_ms = g->prof_ms[k];
     if (launches) *launches = g->prof_n[k];
     return AEFDTD_OK;
+}
+
+int aefdtd_stream_bandwidth(size_t bytes, int reps, double out[3]) {
+    if (!out || bytes < (1u << 20) || reps < 1) return AEFDTD_EINVAL;
+    const size_t n = bytes / sizeof(float4);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = AEFDTD_OK;
+    if (hipMalloc(&a, n * sizeof(float4)) != hipSuccess || hipMalloc(&b, n * sizeof(float4)) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = AEFDTD_EDEVICE;
+    } else {
+        (void)hipMemsetAsync(a, 0, n * sizeof(float4), st);
+        (void)hipMemsetAsync(b, 0, n * sizeof(float4), st);
+        double best = 0, best_nt = 0, best_ms = 0;
+        const unsigned grids[4] = {1024, 2048, 4096, 8192};
+        for (int nt = 0; nt < 2; nt++)
+            for (unsigned gdim : grids) {
+                for (int r = 0; r < reps + 1; r++) {
+                    (void)hipEventRecord(e0, st);
+                    if (nt) hipLaunchKernelGGL(k_stream_copy<true>, dim3(gdim), dim3(256), 0, st, (const f32x4 *)a, (f32x4 *)b, n);
+                    else hipLaunchKernelGGL(k_stream_copy<false>, dim3(gdim), dim3(256), 0, st, (const f32x4 *)a, (f32x4 *)b, n);
+                    (void)hipEventRecord(e1, st);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (r == 0 || ms <= 0) continue;            // first launch warms up
+                    const double gbs = 2.0 * n * sizeof(float4) / (ms * 1e-3) / 1e9;
+                    if (nt) best_nt = std::max(best_nt, gbs);
+                    else if (gbs > best) { best = gbs; best_ms = ms; }
+                }
+            }
+        if (hipGetLastError() != hipSuccess) rc = AEFDTD_EDEVICE;
+        out[0] = best;
+        out[1] = best_nt;
+        out[2] = best_ms;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
 }
 
 int aefdtd_stats(const aefdtd_grid *g, long out[4]) {

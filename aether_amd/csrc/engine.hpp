@@ -77,6 +77,11 @@ int batch_members(const Batch *b);
 int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
 int batch_synchronize(Batch *b);
 int batch_set_tiles(Batch *b, int tile_e, int tile_h);
+// Time-skewed passes of 2D TE/TM members: code = tile height index (0 small,
+// 1 large) | depth index (2, 4, 8 steps) << 2; -1 steps them one step at a time.
+int batch_set_skew(Batch *b, int code);
+int batch_skew(const Batch *b);
+int batch_skewed_members(const Batch *b);
 // Device time of batch_step calls while on (HIP events): on=1 resets and
 // starts, 0 stops, -1 only reads.
 void batch_profile(Batch *b, int on, double *ms, long *chunks);

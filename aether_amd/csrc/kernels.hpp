@@ -962,5 +962,256 @@ __global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, F
     if (threadIdx.x < 6) out[((size_t)t * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// ================================================================ 2D time skewing
+// Temporally blocked passes for 2D TE/TM grids of a batch: a workgroup loads
+// a tile of TBX x LY cells (three live field components into LDS, material
+// ids and CPML psi of its own cells into registers), advances it TS full
+// steps (E update, oscillators, H update, sensor samples) and writes back the
+// inner (TBX-2TS-1) x (LY-2TS-1) cells, whose dependency cones stay inside the
+// tile (overlapped tiling: the halo is recomputed by the neighbours).  Loads
+// wrap around the grid like the stencil does, so boundary tiles see exactly
+// the neighbours the per-step passes see.  Each cell's update is e_cell_vals /
+// h_cell_vals, the same arithmetic as the per-step passes.
+constexpr int TBX = 128;          // tile width: 64 lanes x 2 columns
+constexpr int TB_WAVES = 8;       // waves per workgroup
+
+// CPML psi of one 2D cell in registers: TM (Ez,Hx,Hy) uses Ezx, Ezy, Hxy, Hyx;
+// TE (Ex,Ey,Hz) uses Exy, Eyx, Hzx, Hzy; z-slab terms never occur in 2D.
+template <typename T>
+struct RegPsiE {
+    T &a, &b;
+    __device__ __forceinline__ T &Exy(int) const { return a; }
+    __device__ __forceinline__ T &Exz(int) const { return a; }
+    __device__ __forceinline__ T &Eyx(int) const { return b; }
+    __device__ __forceinline__ T &Eyz(int) const { return b; }
+    __device__ __forceinline__ T &Ezx(int) const { return a; }
+    __device__ __forceinline__ T &Ezy(int) const { return b; }
+};
+template <typename T>
+struct RegPsiH {
+    T &c, &d;
+    __device__ __forceinline__ T &Hxy(int) const { return c; }
+    __device__ __forceinline__ T &Hxz(int) const { return c; }
+    __device__ __forceinline__ T &Hyx(int) const { return d; }
+    __device__ __forceinline__ T &Hyz(int) const { return d; }
+    __device__ __forceinline__ T &Hzx(int) const { return c; }
+    __device__ __forceinline__ T &Hzy(int) const { return d; }
+};
+
+// POL 0 = TM (LDS planes: Ez, Hx, Hy), 1 = TE (Ex, Ey, Hz).
+template <typename S, typename T, int POL, int LY, int TS>
+__global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> *__restrict__ ead,
+                                                             const HArgs<S, T> *__restrict__ had,
+                                                             const int *__restrict__ members,
+                                                             const int *__restrict__ wg0, int nm,
+                                                             const int *__restrict__ tcur,
+                                                             const int *__restrict__ tend,
+                                                             const SampB<S, T> *__restrict__ samp,
+                                                             const int *__restrict__ sbeg, int s0) {
+    constexpr int CX = TBX / 64, RY = LY / TB_WAVES;
+    constexpr int IX = TBX - 2 * TS - 1, IY = LY - 2 * TS - 1;
+    extern __shared__ char lds_raw[];
+    const int q = batch_member(wg0, nm);
+    const int r = members[q];
+    const EArgs<S, T> &ea = ead[r];
+    const HArgs<S, T> &ha = had[r];
+    const int tbeg = tcur[r] + s0;
+    const int nsteps = min(TS, tend[r] - tbeg);
+    const int w = blockIdx.x - wg0[q];
+    const int Nx = ea.Nx, Ny = ea.Ny;
+    const int ntx = (Nx + IX - 1) / IX, nty = (Ny + IY - 1) / IY;
+    if (nsteps <= 0 || w >= ntx * nty) return;
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    T *F = reinterpret_cast<T *>(lds_raw + ((ea.nmat * sizeof(MatL<T>) + 15) & ~(size_t)15));
+    T *F0 = F, *F1 = F + TBX * LY, *F2 = F + 2 * TBX * LY;      // TM: Ez Hx Hy; TE: Ex Ey Hz
+    stage_mats(ea, sm);
+    const int bx = w % ntx, by = w / ntx;
+    const int ox = bx * IX, oy = by * IY;           // inner origin (global)
+    const int tx = threadIdx.x, wv = threadIdx.y;
+    const int wxs = ea.pml[0] + ea.pml[1];
+    const Fld<S, T> &G0 = POL == 0 ? ea.Ez : ea.Ex, &G1 = POL == 0 ? ea.Hx : ea.Ey, &G2 = POL == 0 ? ea.Hy : ea.Hz;
+    // per-column / per-row data of the lane's cells
+    int gi[CX], gj[RY], nxE[CX], nyE[RY], nxH[CX], nyH[RY];
+    T ikxE[CX], ikyE[RY], ikxH[CX], ikyH[RY];
+#pragma unroll
+    for (int cx = 0; cx < CX; cx++) {
+        int g = (ox - TS + tx + 64 * cx) % Nx;
+        if (g < 0) g += Nx;
+        gi[cx] = g;
+        nxE[cx] = slabE(g, Nx, ea.pml[0], ea.pml[1]);
+        nxH[cx] = slabH(g, Nx, ea.pml[0], ea.pml[1]);
+        ikxE[cx] = ea.pf.ikE[0][g];
+        ikxH[cx] = ha.pf.ikH[0][g];
+    }
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++) {
+        int g = (oy - TS + wv + TB_WAVES * ry) % Ny;
+        if (g < 0) g += Ny;
+        gj[ry] = g;
+        nyE[ry] = slabE(g, Ny, ea.pml[2], ea.pml[3]);
+        nyH[ry] = slabH(g, Ny, ea.pml[2], ea.pml[3]);
+        ikyE[ry] = ea.pf.ikE[1][g];
+        ikyH[ry] = ha.pf.ikH[1][g];
+    }
+    // load: fields into LDS, material ids and psi into registers
+    int mat[CX * RY];
+    T pa[CX * RY], pb[CX * RY], pc[CX * RY], pd[CX * RY];
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+        for (int cx = 0; cx < CX; cx++) {
+            const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+            const int id = gi[cx] + Nx * gj[ry];
+            F0[l] = G0[id];
+            F1[l] = G1[id];
+            F2[l] = G2[id];
+            mat[c] = ea.mat[id];
+            const T z = 0;
+            if (POL == 0) {
+                pa[c] = nxE[cx] >= 0 ? ea.pEzx[nxE[cx] + wxs * gj[ry]] : z;
+                pb[c] = nyE[ry] >= 0 ? ea.pEzy[gi[cx] + Nx * nyE[ry]] : z;
+                pc[c] = nyH[ry] >= 0 ? ha.pHxy[gi[cx] + Nx * nyH[ry]] : z;
+                pd[c] = nxH[cx] >= 0 ? ha.pHyx[nxH[cx] + wxs * gj[ry]] : z;
+            } else {
+                pa[c] = nyE[ry] >= 0 ? ea.pExy[gi[cx] + Nx * nyE[ry]] : z;
+                pb[c] = nxE[cx] >= 0 ? ea.pEyx[nxE[cx] + wxs * gj[ry]] : z;
+                pc[c] = nxH[cx] >= 0 ? ha.pHzx[nxH[cx] + wxs * gj[ry]] : z;
+                pd[c] = nyH[ry] >= 0 ? ha.pHzy[gi[cx] + Nx * nyH[ry]] : z;
+            }
+        }
+    // sensors of this grid
+    const int sb = sbeg[2 * r], se = sbeg[2 * r + 1];
+    __syncthreads();
+    const T zero = 0;
+    for (int st = 0; st < nsteps; st++) {
+        const int t = tbeg + st;
+        // E update (cells with an i-1 and a j-1 neighbour in the tile)
+#pragma unroll
+        for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+            for (int cx = 0; cx < CX; cx++) {
+                const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+                if (li < 1 || lj < 1) continue;
+                const int id = gi[cx] + Nx * gj[ry];
+                const MatL<T> &M = sm[mat[c]];
+                if (POL == 0) {
+                    const V3<T> e = e_cell_vals(ea, M, gi[cx], gj[ry], 0, id, zero, zero, F0[l], zero, zero, zero,
+                                                zero, F2[l] - F2[l - 1], F1[l] - F1[l - TBX], ikxE[cx], ikyE[ry],
+                                                nxE[cx], nyE[ry], t, RegPsiE<T>{pa[c], pb[c]});
+                    F0[l] = e.z;
+                } else {
+                    const V3<T> e = e_cell_vals(ea, M, gi[cx], gj[ry], 0, id, F0[l], F1[l], zero,
+                                                F2[l] - F2[l - TBX], zero, zero, F2[l] - F2[l - 1], zero, zero,
+                                                ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], t, RegPsiE<T>{pa[c], pb[c]});
+                    F0[l] = e.x;
+                    F1[l] = e.y;
+                }
+            }
+        __syncthreads();
+        // H update (cells with an i+1 and a j+1 neighbour in the tile)
+#pragma unroll
+        for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+            for (int cx = 0; cx < CX; cx++) {
+                const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+                if (li > TBX - 2 || lj > LY - 2) continue;
+                if (POL == 0) {
+                    T hx = F1[l], hy = F2[l], hz = zero;
+                    const T ez = F0[l];
+                    h_cell_vals(ha, gi[cx], gj[ry], 0, hx, hy, hz, zero, zero, F0[l + TBX] - ez, F0[l + 1] - ez, zero,
+                                zero, ikxH[cx], ikyH[ry], nxH[cx], nyH[ry], RegPsiH<T>{pc[c], pd[c]});
+                    F1[l] = hx;
+                    F2[l] = hy;
+                } else {
+                    T hx = zero, hy = zero, hz = F2[l];
+                    h_cell_vals(ha, gi[cx], gj[ry], 0, hx, hy, hz, zero, zero, zero, zero, F0[l + TBX] - F0[l],
+                                F1[l + 1] - F1[l], ikxH[cx], ikyH[ry], nxH[cx], nyH[ry], RegPsiH<T>{pc[c], pd[c]});
+                    F2[l] = hz;
+                }
+            }
+        __syncthreads();
+        // sensor samples of the inner cells (local_E/H, core/fdtd_utils.cpp:457-507)
+        if (sb < se) {
+#pragma unroll
+            for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+                for (int cx = 0; cx < CX; cx++) {
+                    const int li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+                    const int ui = li - TS, uj = lj - TS;
+                    if (ui < 0 || ui >= IX || uj < 0 || uj >= IY || ox + ui >= Nx || oy + uj >= Ny) continue;
+                    const int i = gi[cx], j = gj[ry];
+                    for (int e = sb; e < se; e++) {
+                        const SensDesc &d = samp[e].d;
+                        long p;
+                        if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) {
+                            if (i != d.x1 || j < d.y1 || j >= d.y1 + d.span1 || d.z1 != 0) continue;
+                            p = j - d.y1;
+                        } else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) {
+                            if (j != d.y1 || i < d.x1 || i >= d.x1 + d.span1 || d.z1 != 0) continue;
+                            p = i - d.x1;
+                        } else {
+                            if (d.z1 != 0 || i < d.x1 || i >= d.x1 + d.span1 || j < d.y1 || j >= d.y1 + d.span2)
+                                continue;
+                            p = (i - d.x1) + (long)d.span1 * (j - d.y1);
+                        }
+                        T v[6];
+                        if (POL == 0) {
+                            v[0] = zero; v[1] = zero; v[5] = zero;
+                            v[2] = (F0[l] + F0[l + 1] + F0[l + TBX] + F0[l + 1 + TBX]) / (T)4.0;
+                            v[3] = (F1[l] + F1[l + 1]) / (T)2.0;
+                            v[4] = (F2[l] + F2[l + TBX]) / (T)2.0;
+                        } else {
+                            v[2] = zero; v[3] = zero; v[4] = zero;
+                            v[0] = (F0[l] + F0[l + TBX] + F0[l] + F0[l + TBX]) / (T)4.0;
+                            v[1] = (F1[l] + F1[l + 1] + F1[l] + F1[l + 1]) / (T)4.0;
+                            v[5] = (F2[l] + F2[l]) / (T)2.0;
+                        }
+                        const SampB<S, T> &qs = samp[e];
+                        const int slot = s0 + st;
+#pragma unroll
+                        for (int cc = 0; cc < 6; cc++) qs.stage[((size_t)slot * 6 + cc) * qs.Ptot + d.off + p] = v[cc];
+                    }
+                }
+            __syncthreads();
+        }
+    }
+    // write back the inner cells
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+        for (int cx = 0; cx < CX; cx++) {
+            const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+            const int ui = li - TS, uj = lj - TS;
+            if (ui < 0 || ui >= IX || uj < 0 || uj >= IY || ox + ui >= Nx || oy + uj >= Ny) continue;
+            const int id = gi[cx] + Nx * gj[ry];
+            G0.set(id, F0[l]);
+            G1.set(id, F1[l]);
+            G2.set(id, F2[l]);
+            if (POL == 0) {
+                if (nxE[cx] >= 0) ea.pEzx[nxE[cx] + wxs * gj[ry]] = pa[c];
+                if (nyE[ry] >= 0) ea.pEzy[gi[cx] + Nx * nyE[ry]] = pb[c];
+                if (nyH[ry] >= 0) ha.pHxy[gi[cx] + Nx * nyH[ry]] = pc[c];
+                if (nxH[cx] >= 0) ha.pHyx[nxH[cx] + wxs * gj[ry]] = pd[c];
+            } else {
+                if (nyE[ry] >= 0) ea.pExy[gi[cx] + Nx * nyE[ry]] = pa[c];
+                if (nxE[cx] >= 0) ea.pEyx[nxE[cx] + wxs * gj[ry]] = pb[c];
+                if (nxH[cx] >= 0) ha.pHzx[nxH[cx] + wxs * gj[ry]] = pc[c];
+                if (nyH[ry] >= 0) ha.pHzy[gi[cx] + Nx * nyH[ry]] = pd[c];
+            }
+        }
+}
+
+// Streaming copy used to measure the HBM bandwidth a kernel can reach on
+// this device (16-byte accesses, grid-stride).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_copy(const f32x4 *__restrict__ a, f32x4 *__restrict__ b, size_t n) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        if constexpr (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(a + q), b + q);
+        else b[q] = a[q];
+    }
+}
+
 }  // namespace dev
 }  // namespace aefdtd

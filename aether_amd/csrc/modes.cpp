@@ -580,33 +580,79 @@ int aefdtd_batch_set_tiles(aefdtd_batch *b, int tile_e, int tile_h) {
     return batch_set_tiles(b->b, tile_e, tile_h);
 }
 
-// Candidates: every (BY, remap) with the smallest KZ when all members are
-// one cell thick in z (KZ is then irrelevant), else all 32 codes; E and H
-// share a code.  Each candidate times one chunk of the real run.
-int aefdtd_batch_autotune(aefdtd_batch *b, int out[2]) {
+int aefdtd_batch_set_skew(aefdtd_batch *b, int code) {
     if (!b) return AEFDTD_EINVAL;
+    return batch_set_skew(b->b, code);
+}
+
+int aefdtd_batch_info(const aefdtd_batch *b, int out[4]) {
+    if (!b || !out) return AEFDTD_EINVAL;
+    out[0] = batch_members(b->b);
+    out[1] = aefdtd_batch_running(b);
+    out[2] = batch_skewed_members(b->b);
+    out[3] = batch_skew(b->b);
+    return AEFDTD_OK;
+}
+
+namespace {
+
+// Times one chunk of the batch with its current configuration (the first
+// replay after a capture also uploads the graph, so the second one is timed);
+// returns false when the members ran out of steps first.
+bool time_chunk(aefdtd_batch *b, double &ms, int &rc) {
+    ms = 1e30;
+    if (aefdtd_batch_running(b) == 0) return false;
+    if ((rc = aefdtd_batch_step(b, 1))) return false;
+    if (aefdtd_batch_running(b) == 0) return false;
+    long n = 0;
+    batch_profile(b->b, 1, nullptr, nullptr);
+    if ((rc = aefdtd_batch_step(b, 1))) return false;
+    batch_profile(b->b, 0, &ms, &n);
+    return n > 0;
+}
+
+}  // namespace
+
+// Online tuner over the real run: every time-skewing code (tile height,
+// depth) when 2D TE/TM members are present, then the tile codes of the
+// per-step passes when other members are: every (BY, remap) with the
+// smallest KZ when all of them are one cell thick in z (KZ is then
+// irrelevant), else all 32; E and H share a code.  Every candidate computes
+// the same steps, so tuning advances the members like any other chunk.
+int aefdtd_batch_autotune(aefdtd_batch *b, int out[3]) {
+    if (!b) return AEFDTD_EINVAL;
+    int rc = AEFDTD_OK;
+    if (batch_skewed_members(b->b) > 0 || batch_skew(b->b) < 0) {
+        int best_s = batch_skew(b->b);
+        double best = 1e30;
+        for (int code = -1; code < 12; code++) {
+            if (batch_set_skew(b->b, code) != AEFDTD_OK) continue;
+            if (code >= 0 && batch_skewed_members(b->b) == 0) continue;
+            double ms;
+            if (!time_chunk(b, ms, rc)) break;
+            if (ms < best) { best = ms; best_s = code; }
+        }
+        if (rc) return rc;
+        TRY(batch_set_skew(b->b, best_s));
+    }
+    // members left to the per-step passes
+    const bool stepped = batch_skew(b->b) < 0 || batch_skewed_members(b->b) < (int)b->runs.size();
     bool flat = true;
     for (aefdtd_run *r : b->runs) flat = flat && r->g->Nz == 1;
-    std::vector<int> cand;
-    for (int c = 0; c < 32; c++)
-        if (!flat || ((c >> 2) & 3) == 0) cand.push_back(c);
-    double best = 1e30;
-    int best_c = -1;
-    for (int c : cand) {
-        if (aefdtd_batch_running(b) == 0) break;
-        TRY(batch_set_tiles(b->b, c, c));
-        TRY(aefdtd_batch_step(b, 1));      // captures the graph; its first replay also uploads it
-        if (aefdtd_batch_running(b) == 0) break;
-        double ms = 0;
-        long n = 0;
-        batch_profile(b->b, 1, nullptr, nullptr);
-        TRY(aefdtd_batch_step(b, 1));
-        batch_profile(b->b, 0, &ms, &n);
-        if (n > 0 && ms < best) { best = ms; best_c = c; }
+    int best_c = 2 | (1 << 2);
+    if (stepped) {
+        double best = 1e30;
+        for (int c = 0; c < 32; c++) {
+            if (flat && ((c >> 2) & 3) != 0) continue;
+            TRY(batch_set_tiles(b->b, c, c));
+            double ms;
+            if (!time_chunk(b, ms, rc)) break;
+            if (ms < best) { best = ms; best_c = c; }
+        }
+        if (rc) return rc;
     }
-    if (best_c < 0) best_c = 2 | (1 << 2);
     TRY(batch_set_tiles(b->b, best_c, best_c));
-    if (out) { out[0] = best_c; out[1] = best_c; }
+    if (out) { out[0] = best_c; out[1] = best_c; out[2] = batch_skew(b->b); }
     return AEFDTD_OK;
 }
 

@@ -360,6 +360,7 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 5}))
         return
 
+    import numpy as np
     from aether_amd import capi, workloads
     if args.form == "precision":
         return precision_form(args)
@@ -377,6 +378,8 @@ def main():
     stats = run.stats()
     ncell = stats[0]
 
+    bw = np.zeros(3)
+    capi.engine().fn("stream_bandwidth")(1 << 31, 3, bw.ctypes.data_as(capi._dp))
     run.step(args.warmup)
     if args.tiles is not None:
         run.set_tiles(*args.tiles)
@@ -430,6 +433,8 @@ def main():
                            "rc": round(r_ms / max(1, r_n), 5)},
         "step_hbm_fraction": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
+        "stream_copy_GBps": {"plain": round(float(bw[0]), 1), "nontemporal": round(float(bw[1]), 1),
+                             "note": "best 2 GiB device copy (read+write), the achievable-bandwidth reference"},
         "setup_s": round(setup_s, 2),
         "tiles": tiles,
     }

@@ -330,9 +330,21 @@ int aefdtd_batch_running(const aefdtd_batch *b);
 int aefdtd_batch_synchronize(aefdtd_batch *b);
 /* Tile configurations (codes as aefdtd_set_tiles) of the batched passes. */
 int aefdtd_batch_set_tiles(aefdtd_batch *b, int tile_e, int tile_h);
-/* Online tuner: advances the batch one chunk per candidate tile shape (all
- * shapes compute the same steps), keeps the fastest; out[2] = chosen codes. */
-int aefdtd_batch_autotune(aefdtd_batch *b, int out[2]);
+/* Time-skewed passes for 2D TE/TM members (one cell thick in z, no
+ * dispersive material, TFSF or plane injection, FP64/FP32 storage): a
+ * workgroup advances a tile `depth` steps in LDS before writing it back.
+ * code = tile height index (0: 32 rows FP32 / 16 FP64, 1: 48 / 32) | depth
+ * index (0: 2, 1: 4, 2: 8 steps) << 2; -1 steps every member one step at a
+ * time.  Same arithmetic per cell either way. */
+int aefdtd_batch_set_skew(aefdtd_batch *b, int code);
+/* out[0..3]: members, members with steps left, members stepped by the
+ * time-skewed passes, current skew code. */
+int aefdtd_batch_info(const aefdtd_batch *b, int out[4]);
+/* Online tuner: advances the batch by timed chunks, one per candidate
+ * time-skew code and then per tile shape of the per-step passes (all
+ * candidates compute the same steps), keeps the fastest; out[3] = chosen
+ * E tile, H tile and skew codes. */
+int aefdtd_batch_autotune(aefdtd_batch *b, int out[3]);
 /* Device time of aefdtd_batch_step (HIP events on the batch stream): on=1
  * resets and starts, 0 stops, -1 only reads. */
 int aefdtd_batch_profile(aefdtd_batch *b, int on, double *ms, long *chunks);
@@ -399,6 +411,12 @@ int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
  * field * 2^exps[c]); FP16 storage only (defaults E: 2^8, H: 2^16, so that
  * both sit near the middle of the FP16 range). Before the first update. */
 int aefdtd_set_storage_scale(aefdtd_grid *g, const int exps[6]);
+/* Reference point for the roofline, no counterpart in the reference: the
+ * best bandwidth of a streaming device copy of `bytes` (read + write counted)
+ * over grid sizes 1024..8192 x 256 lanes, 16-byte accesses; out[0] GB/s with
+ * plain accesses, out[1] GB/s with non-temporal ones, out[2] ms of the best
+ * plain copy. */
+int aefdtd_stream_bandwidth(size_t bytes, int reps, double out[3]);
 /* Bytes of device memory held by the handle. */
 int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
 /* out[0]=cells of the full grid, out[1]=dispersive (RC) cells, out[2]=max

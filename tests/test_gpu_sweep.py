@@ -221,3 +221,50 @@ def test_batch_refuses_what_it_cannot_step(engine_be):
     for x in (r64, rs, r):
         x.close()
     b.close()
+
+
+def _sensor_variety(seed, side, steps, pol):
+    """A 2D run with a line sensor along x, one along y and a z-normal plane
+    sensor, and a second oscillator on the other live E component for TE."""
+    spec, sens, info = small_run(seed, side, steps, pol=pol)
+    sens = sens + [capi.SensorReq(box=False, orientation=capi.NORMAL_XM, x1=side // 3, x2=side // 3 + 1, y1=2,
+                                  y2=side - 3, z1=0, z2=1, Nl=5),
+                   capi.SensorReq(box=False, orientation=capi.NORMAL_Z, x1=4, x2=side // 2, y1=side // 2,
+                                  y2=side // 2 + 5, z1=0, z2=1, Nl=3)]
+    if pol == "TE":
+        spec.sources.append(capi.SourceReq(orientation=capi.NORMAL_Y, x1=side // 2, x2=side // 2 + 1, y1=3,
+                                           y2=side - 3, z1=0, z2=1, lambda_min=450e-9, lambda_max=900e-9))
+    return finalized(spec, sens), sens
+
+
+@pytest.mark.parametrize("code", list(range(-1, 12)))
+def test_batch_time_skew_codes_fp64(engine_be, oracle_be, code):
+    """Every time-skew code (tile height, depth 2/4/8) and the per-step passes
+    give the oracle's fields and spectra on uneven 2D TM/TE runs, including
+    tiles larger than the grid and step counts that end inside a block."""
+    cases = [_sensor_variety(51, 36, 203, "TM"), _sensor_variety(52, 150, 131, "TE"),
+             _sensor_variety(53, 20, 77, "TE")]
+    be = capi.engine()
+    b = capi.Batch(be, capi.FP64)
+    try:
+        b.set_skew(code)
+    except capi.FDTDError:
+        b.close()
+        pytest.skip("code not valid for FP64 tiles")
+    runs = []
+    for spec, sens in cases:
+        r = capi.Run(be, capi.KIND_CUSTOM, spec, sens)
+        b.add(r)
+        runs.append(r)
+    info = b.info()
+    assert info["skewed"] == (0 if code < 0 else 3)
+    while b.running():
+        b.step(1)
+    for (spec, sens), run in zip(cases, runs):
+        b.remove(run)
+        g = run.finish(want_fields=True)
+        r = capi.run_custom(oracle_be, spec, sens, want_fields=True)
+        assert rel_l2(g["fields"], r["fields"]) <= FIELD_TOL
+        assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
+        run.close()
+    b.close()
