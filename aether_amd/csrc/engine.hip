@@ -871,10 +871,34 @@ int tb_polarisation(const aefdtd_grid *g) {
 
 template <typename S, typename T, int POL, int LY, int TS>
 void launch_tb_k(int nwg, size_t lds, hipStream_t st, const void *ea, const void *ha, const int *mem, const int *wg0,
-                 int nm, const int *tc, const int *te, const void *samp, const int *sbeg, int s0) {
+                 int nm, const int *tc, const int *te, const void *samp, const int *sbeg, const void *alt, int s0) {
     hipLaunchKernelGGL((k_tb2d_batch<S, T, POL, LY, TS>), dim3(nwg), dim3(64, TB_WAVES, 1), lds, st,
                        (const EArgs<S, T> *)ea, (const HArgs<S, T> *)ha, mem, wg0, nm, tc, te,
-                       (const SampB<S, T> *)samp, sbeg, s0);
+                       (const SampB<S, T> *)samp, sbeg, (const TbAlt<S, T> *)alt, s0, (s0 / TS) & 1);
+}
+
+// The second live-state set of a time-skewed grid (engine-owned, kept until
+// the grid is destroyed): field components and CPML psi arrays of `pol`.
+const int kTbPsi[2][4] = {{1, 5, 6, 2}, {4, 0, 3, 7}};  // TM: Ezx Ezy Hxy Hyx; TE: Exy Eyx Hzx Hzy
+
+int ensure_tb_alt(aefdtd_grid *g, int pol) {
+    if (g->tb_alt_pol == pol) return AEFDTD_OK;
+    for (void *&p : g->tb_alt) dfree(p);
+    g->dev_bytes -= g->tb_alt_bytes;
+    g->tb_alt_bytes = 0;
+    g->tb_alt_pol = -1;
+    const size_t before = g->dev_bytes;
+    for (int c = 0; c < 3; c++) {
+        int rc = dalloc(g, &g->tb_alt[c], g->ncell() * g->ssize());
+        if (rc) return rc;
+    }
+    for (int c = 0; c < 4; c++) {
+        int rc = dalloc(g, &g->tb_alt[3 + c], g->psipml_n[kTbPsi[pol][c]] * g->csize());
+        if (rc) return rc;
+    }
+    g->tb_alt_bytes = g->dev_bytes - before;
+    g->tb_alt_pol = pol;
+    return AEFDTD_OK;
 }
 
 template <typename S, typename T, int POL, int LY, int TS>
@@ -921,6 +945,7 @@ struct aefdtd::Batch {
     // index << 2), -1 = off; member lists and workgroup prefixes per polarisation
     int tb_code = 1 | (1 << 2);
     int *d_mem[2] = {nullptr, nullptr}, *d_wgT[2] = {nullptr, nullptr}, *d_sbeg = nullptr;
+    void *d_alt = nullptr;                 // TbAlt per member
     int nmem[2] = {0, 0}, nwgT[2] = {0, 0}, nstep = 0;  // nstep: members stepped by the per-step passes
     int nsamp_step = 0;                    // leading sample entries that belong to those members
     size_t ldsT = 0;
@@ -941,7 +966,7 @@ void batch_free_tables(Batch *b) {
     b->graph = nullptr;
     for (void **p : {&b->d_ea, &b->d_ha, &b->d_samp, &b->d_dft, (void **)&b->d_wgE, (void **)&b->d_wgH,
                      (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_mem[0], (void **)&b->d_mem[1],
-                     (void **)&b->d_wgT[0], (void **)&b->d_wgT[1], (void **)&b->d_sbeg})
+                     (void **)&b->d_wgT[0], (void **)&b->d_wgT[1], (void **)&b->d_sbeg, &b->d_alt})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
 }
 
@@ -972,6 +997,7 @@ int batch_build(Batch *b) {
     std::vector<int> wgE, wgH, tend(n), pol(n, -1), mem[2], wgT[2], sbeg(2 * n, 0);
     std::vector<SampB<S, T>> samp;
     std::vector<DftB<T>> dft;
+    std::vector<TbAlt<S, T>> alt(n);
     int totE = 0, totH = 0, totT[2] = {0, 0}, maxmat = 1;
     b->maxP = 0;
     for (int r = 0; r < n; r++) {
@@ -994,6 +1020,12 @@ int batch_build(Batch *b) {
             totH += (nh + 7) / 8 * 8;
         } else {
             const int p = pol[r];
+            if ((rc = ensure_tb_alt(g, p))) {
+                b->err = g->err;
+                return rc;
+            }
+            for (int c = 0; c < 3; c++) alt[r].f[c] = (S *)g->tb_alt[c];
+            for (int c = 0; c < 4; c++) alt[r].p[c] = (T *)g->tb_alt[3 + c];
             mem[p].push_back(r);
             wgT[p].push_back(totT[p]);
             totT[p] += ((g->Nx + IX - 1) / IX) * ((g->Ny + IY - 1) / IY);
@@ -1043,7 +1075,7 @@ int batch_build(Batch *b) {
         (rc = batch_upload(b, (void **)&b->d_wgE, wgE)) || (rc = batch_upload(b, (void **)&b->d_wgH, wgH)) ||
         (rc = batch_upload(b, (void **)&b->d_tcur, b->tcur)) || (rc = batch_upload(b, (void **)&b->d_tend, tend)) ||
         (rc = batch_upload(b, &b->d_samp, samp)) || (rc = batch_upload(b, &b->d_dft, dft)) ||
-        (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)))
+        (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)) || (rc = batch_upload(b, &b->d_alt, alt)))
         return rc;
     for (int p = 0; p < 2; p++)
         if (b->nmem[p]) {
@@ -1065,7 +1097,7 @@ int batch_build(Batch *b) {
                     with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() {
                         launch_tb_k<S, T, P, L, TSK>(b->nwgT[p], b->ldsT, b->stream, b->d_ea, b->d_ha, b->d_mem[p],
                                                      b->d_wgT[p], b->nmem[p], b->d_tcur, b->d_tend, b->d_samp,
-                                                     b->d_sbeg, s0);
+                                                     b->d_sbeg, b->d_alt, s0);
                     });
     }
     const int ebi = b->tile_e & 3, ekz = (b->tile_e >> 2) & 3, hbi = b->tile_h & 3, hkz = (b->tile_h >> 2) & 3;
@@ -1357,6 +1389,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree(g->d_stage);
     dfree(g->d_sdesc);
     dfree(g->d_osc);
+    for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
         void *p = (void *)o.val;
         dfree(p);

@@ -131,6 +131,9 @@ struct aefdtd_grid {
     // driver actions
     int inj_k = -1;
     std::vector<double> inj[4];
+    void *tb_alt[7] = {};             // second live-state set of the time-skewed batch passes
+    int tb_alt_pol = -1;              // (3 field components, 4 CPML psi), for this polarisation
+    size_t tb_alt_bytes = 0;
     std::vector<aefdtd::OscDev> osc;  // oscillator sources, registration order
     std::vector<aefdtd::OscParams> osc_par;
     void *d_osc = nullptr;            // OscDev[MAX_OSC]

@@ -998,6 +998,17 @@ struct RegPsiH {
     __device__ __forceinline__ T &Hzy(int) const { return d; }
 };
 
+// Second copy of a time-skewed grid's live state: the three live field
+// components (TM: Ez Hx Hy, TE: Ex Ey Hz) and its four CPML psi arrays (TM:
+// Ezx Ezy Hxy Hyx, TE: Exy Eyx Hzx Hzy).  Launches alternate between reading
+// the grid's own arrays and writing these and the reverse, so no workgroup
+// ever loads a halo cell that another one has already advanced.
+template <typename S, typename T>
+struct TbAlt {
+    S *f[3];
+    T *p[4];
+};
+
 // POL 0 = TM (LDS planes: Ez, Hx, Hy), 1 = TE (Ex, Ey, Hz).
 template <typename S, typename T, int POL, int LY, int TS>
 __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> *__restrict__ ead,
@@ -1007,7 +1018,8 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                                                              const int *__restrict__ tcur,
                                                              const int *__restrict__ tend,
                                                              const SampB<S, T> *__restrict__ samp,
-                                                             const int *__restrict__ sbeg, int s0) {
+                                                             const int *__restrict__ sbeg,
+                                                             const TbAlt<S, T> *__restrict__ altd, int s0, int odd) {
     constexpr int CX = TBX / 64, RY = LY / TB_WAVES;
     constexpr int IX = TBX - 2 * TS - 1, IY = LY - 2 * TS - 1;
     extern __shared__ char lds_raw[];
@@ -1016,11 +1028,12 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
     const EArgs<S, T> &ea = ead[r];
     const HArgs<S, T> &ha = had[r];
     const int tbeg = tcur[r] + s0;
-    const int nsteps = min(TS, tend[r] - tbeg);
+    // a grid at its end still passes its state to the other buffer set
+    const int nsteps = max(0, min(TS, tend[r] - tbeg));
     const int w = blockIdx.x - wg0[q];
     const int Nx = ea.Nx, Ny = ea.Ny;
     const int ntx = (Nx + IX - 1) / IX, nty = (Ny + IY - 1) / IY;
-    if (nsteps <= 0 || w >= ntx * nty) return;
+    if (w >= ntx * nty) return;
     MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
     T *F = reinterpret_cast<T *>(lds_raw + ((ea.nmat * sizeof(MatL<T>) + 15) & ~(size_t)15));
     T *F0 = F, *F1 = F + TBX * LY, *F2 = F + 2 * TBX * LY;      // TM: Ez Hx Hy; TE: Ex Ey Hz
@@ -1030,6 +1043,19 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
     const int tx = threadIdx.x, wv = threadIdx.y;
     const int wxs = ea.pml[0] + ea.pml[1];
     const Fld<S, T> &G0 = POL == 0 ? ea.Ez : ea.Ex, &G1 = POL == 0 ? ea.Hx : ea.Ey, &G2 = POL == 0 ? ea.Hy : ea.Hz;
+    // source and destination of this launch (the grid's own arrays or the second set)
+    const TbAlt<S, T> &alt = altd[r];
+    T *const own_p[4] = {POL == 0 ? ea.pEzx : ea.pExy, POL == 0 ? ea.pEzy : ea.pEyx,
+                         POL == 0 ? ha.pHxy : ha.pHzx, POL == 0 ? ha.pHyx : ha.pHzy};
+    Fld<S, T> src[3] = {G0, G1, G2}, dst[3] = {G0, G1, G2};
+    T *psrc[4], *pdst[4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) (odd ? src[c] : dst[c]).p = alt.f[c];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        psrc[c] = odd ? alt.p[c] : own_p[c];
+        pdst[c] = odd ? own_p[c] : alt.p[c];
+    }
     // per-column / per-row data of the lane's cells
     int gi[CX], gj[RY], nxE[CX], nyE[RY], nxH[CX], nyH[RY];
     T ikxE[CX], ikyE[RY], ikxH[CX], ikyH[RY];
@@ -1062,21 +1088,21 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         for (int cx = 0; cx < CX; cx++) {
             const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
             const int id = gi[cx] + Nx * gj[ry];
-            F0[l] = G0[id];
-            F1[l] = G1[id];
-            F2[l] = G2[id];
+            F0[l] = src[0][id];
+            F1[l] = src[1][id];
+            F2[l] = src[2][id];
             mat[c] = ea.mat[id];
             const T z = 0;
             if (POL == 0) {
-                pa[c] = nxE[cx] >= 0 ? ea.pEzx[nxE[cx] + wxs * gj[ry]] : z;
-                pb[c] = nyE[ry] >= 0 ? ea.pEzy[gi[cx] + Nx * nyE[ry]] : z;
-                pc[c] = nyH[ry] >= 0 ? ha.pHxy[gi[cx] + Nx * nyH[ry]] : z;
-                pd[c] = nxH[cx] >= 0 ? ha.pHyx[nxH[cx] + wxs * gj[ry]] : z;
+                pa[c] = nxE[cx] >= 0 ? psrc[0][nxE[cx] + wxs * gj[ry]] : z;
+                pb[c] = nyE[ry] >= 0 ? psrc[1][gi[cx] + Nx * nyE[ry]] : z;
+                pc[c] = nyH[ry] >= 0 ? psrc[2][gi[cx] + Nx * nyH[ry]] : z;
+                pd[c] = nxH[cx] >= 0 ? psrc[3][nxH[cx] + wxs * gj[ry]] : z;
             } else {
-                pa[c] = nyE[ry] >= 0 ? ea.pExy[gi[cx] + Nx * nyE[ry]] : z;
-                pb[c] = nxE[cx] >= 0 ? ea.pEyx[nxE[cx] + wxs * gj[ry]] : z;
-                pc[c] = nxH[cx] >= 0 ? ha.pHzx[nxH[cx] + wxs * gj[ry]] : z;
-                pd[c] = nyH[ry] >= 0 ? ha.pHzy[gi[cx] + Nx * nyH[ry]] : z;
+                pa[c] = nyE[ry] >= 0 ? psrc[0][gi[cx] + Nx * nyE[ry]] : z;
+                pb[c] = nxE[cx] >= 0 ? psrc[1][nxE[cx] + wxs * gj[ry]] : z;
+                pc[c] = nxH[cx] >= 0 ? psrc[2][nxH[cx] + wxs * gj[ry]] : z;
+                pd[c] = nyH[ry] >= 0 ? psrc[3][gi[cx] + Nx * nyH[ry]] : z;
             }
         }
     // sensors of this grid
@@ -1184,19 +1210,19 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             const int ui = li - TS, uj = lj - TS;
             if (ui < 0 || ui >= IX || uj < 0 || uj >= IY || ox + ui >= Nx || oy + uj >= Ny) continue;
             const int id = gi[cx] + Nx * gj[ry];
-            G0.set(id, F0[l]);
-            G1.set(id, F1[l]);
-            G2.set(id, F2[l]);
+            dst[0].set(id, F0[l]);
+            dst[1].set(id, F1[l]);
+            dst[2].set(id, F2[l]);
             if (POL == 0) {
-                if (nxE[cx] >= 0) ea.pEzx[nxE[cx] + wxs * gj[ry]] = pa[c];
-                if (nyE[ry] >= 0) ea.pEzy[gi[cx] + Nx * nyE[ry]] = pb[c];
-                if (nyH[ry] >= 0) ha.pHxy[gi[cx] + Nx * nyH[ry]] = pc[c];
-                if (nxH[cx] >= 0) ha.pHyx[nxH[cx] + wxs * gj[ry]] = pd[c];
+                if (nxE[cx] >= 0) pdst[0][nxE[cx] + wxs * gj[ry]] = pa[c];
+                if (nyE[ry] >= 0) pdst[1][gi[cx] + Nx * nyE[ry]] = pb[c];
+                if (nyH[ry] >= 0) pdst[2][gi[cx] + Nx * nyH[ry]] = pc[c];
+                if (nxH[cx] >= 0) pdst[3][nxH[cx] + wxs * gj[ry]] = pd[c];
             } else {
-                if (nyE[ry] >= 0) ea.pExy[gi[cx] + Nx * nyE[ry]] = pa[c];
-                if (nxE[cx] >= 0) ea.pEyx[nxE[cx] + wxs * gj[ry]] = pb[c];
-                if (nxH[cx] >= 0) ha.pHzx[nxH[cx] + wxs * gj[ry]] = pc[c];
-                if (nyH[ry] >= 0) ha.pHzy[gi[cx] + Nx * nyH[ry]] = pd[c];
+                if (nyE[ry] >= 0) pdst[0][gi[cx] + Nx * nyE[ry]] = pa[c];
+                if (nxE[cx] >= 0) pdst[1][nxE[cx] + wxs * gj[ry]] = pb[c];
+                if (nxH[cx] >= 0) pdst[2][nxH[cx] + wxs * gj[ry]] = pc[c];
+                if (nyH[ry] >= 0) pdst[3][gi[cx] + Nx * nyH[ry]] = pd[c];
             }
         }
 }

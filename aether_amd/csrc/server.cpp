@@ -329,10 +329,15 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
         b += 4 * N + nd * (4 + 3 * cs + 3 * (size_t)npmax * 16);
     }
     const size_t cross[3] = {(size_t)d[1] * d[2], (size_t)d[0] * d[2], (size_t)d[0] * d[1]};
+    size_t cpml = 0;
     for (int a = 0; a < 3; a++) {
         int pm = kind == 0 && a < 2 ? 0 : m->pml[2 * a], pp = kind == 0 && a < 2 ? 0 : m->pml[2 * a + 1];
-        if (pm || pp) b += 4 * (size_t)(pm + pp) * cross[a] * cs;
+        if (pm || pp) cpml += 4 * (size_t)(pm + pp) * cross[a] * cs;
     }
+    b += cpml;
+    // a 2D custom run stepped by the time-skewed batch passes keeps a second
+    // copy of its three live components and their CPML psi
+    if (kind == 2 && d[2] == 1 && !any && p != AEFDTD_FP16) b += 3 * N * ss + cpml / 2;
     b += 6 * (size_t)(d[0] + d[1] + d[2]) * cs;              // CPML profiles
     size_t Ptot = 0;
     for (int i = 0; i < ns; i++) {

@@ -268,3 +268,36 @@ def test_batch_time_skew_codes_fp64(engine_be, oracle_be, code):
         assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
         run.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-12), (capi.FP32, 1e-5)])
+def test_batch_time_skew_large_grid(engine_be, prec, tol):
+    """On grids with far more tiles than the GPU holds at once, the time-skewed
+    passes give the per-step passes' result (tiles never read a neighbour's
+    already-advanced halo)."""
+    outs = []
+    for code in (-1, None):
+        be = capi.engine()
+        b = capi.Batch(be, prec)
+        if code is not None:
+            b.set_skew(code)
+        runs = []
+        for seed, side, steps, pol in ((61, 1400, 300, "TM"), (62, 1100, 250, "TE")):
+            spec, sens, _ = small_run(seed, side, steps, prec, pol=pol)
+            spec = finalized(spec, sens)
+            r = capi.Run(be, capi.KIND_CUSTOM, spec, sens)
+            b.add(r)
+            runs.append(r)
+        assert b.info()["skewed"] == (0 if code == -1 else 2)
+        while b.running():
+            b.step(1)
+        res = []
+        for r in runs:
+            b.remove(r)
+            res.append(r.finish(want_fields=True))
+            r.close()
+        b.close()
+        outs.append(res)
+    for a, c in zip(*outs):
+        assert rel_l2(c["fields"], a["fields"]) <= tol
+        assert spec_err(c["sensors"], a["sensors"]) <= tol
