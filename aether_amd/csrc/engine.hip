@@ -335,9 +335,9 @@ void prof_collect(aefdtd_grid *g) {
 
 // ---------------------------------------------------------------- launches
 // Tile shapes compiled for the 3D passes: (BY, KZ) with BY in {1,2,4,8} and
-// KZ in {4,8,16,32}; tile_cfg packs (BY index, KZ index, remap).
+// KZ in {1,2,4,8}; tile_cfg packs (BY index, KZ index, remap).
 constexpr int kBY[4] = {1, 2, 4, 8};
-constexpr int kKZ[4] = {4, 8, 16, 32};
+constexpr int kKZ[4] = {1, 2, 4, 8};
 
 template <typename S, typename T, int BY, int KZ>
 void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st) {
@@ -357,19 +357,19 @@ void launch_H_cfg(const HArgs<S, T> &a, int nblocks, hipStream_t st) {
 template <typename S, typename T, int BY>
 void launch_E_by(const EArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
     switch (kzi) {
-        case 0: launch_E_cfg<S, T, BY, 4>(a, nb, st); break;
-        case 1: launch_E_cfg<S, T, BY, 8>(a, nb, st); break;
-        case 2: launch_E_cfg<S, T, BY, 16>(a, nb, st); break;
-        default: launch_E_cfg<S, T, BY, 32>(a, nb, st); break;
+        case 0: launch_E_cfg<S, T, BY, 1>(a, nb, st); break;
+        case 1: launch_E_cfg<S, T, BY, 2>(a, nb, st); break;
+        case 2: launch_E_cfg<S, T, BY, 4>(a, nb, st); break;
+        default: launch_E_cfg<S, T, BY, 8>(a, nb, st); break;
     }
 }
 template <typename S, typename T, int BY>
 void launch_H_by(const HArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
     switch (kzi) {
-        case 0: launch_H_cfg<S, T, BY, 4>(a, nb, st); break;
-        case 1: launch_H_cfg<S, T, BY, 8>(a, nb, st); break;
-        case 2: launch_H_cfg<S, T, BY, 16>(a, nb, st); break;
-        default: launch_H_cfg<S, T, BY, 32>(a, nb, st); break;
+        case 0: launch_H_cfg<S, T, BY, 1>(a, nb, st); break;
+        case 1: launch_H_cfg<S, T, BY, 2>(a, nb, st); break;
+        case 2: launch_H_cfg<S, T, BY, 4>(a, nb, st); break;
+        default: launch_H_cfg<S, T, BY, 8>(a, nb, st); break;
     }
 }
 
@@ -823,20 +823,20 @@ template <typename S, typename T, int BY>
 void launch_Eb_by(int kzi, int nwg, size_t lds, hipStream_t st, const void *d, const int *wg0, int n, const int *tc,
                   const int *te, int s) {
     switch (kzi) {
-        case 0: launch_Eb_cfg<S, T, BY, 4>(nwg, lds, st, d, wg0, n, tc, te, s); break;
-        case 1: launch_Eb_cfg<S, T, BY, 8>(nwg, lds, st, d, wg0, n, tc, te, s); break;
-        case 2: launch_Eb_cfg<S, T, BY, 16>(nwg, lds, st, d, wg0, n, tc, te, s); break;
-        default: launch_Eb_cfg<S, T, BY, 32>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        case 0: launch_Eb_cfg<S, T, BY, 1>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        case 1: launch_Eb_cfg<S, T, BY, 2>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        case 2: launch_Eb_cfg<S, T, BY, 4>(nwg, lds, st, d, wg0, n, tc, te, s); break;
+        default: launch_Eb_cfg<S, T, BY, 8>(nwg, lds, st, d, wg0, n, tc, te, s); break;
     }
 }
 template <typename S, typename T, int BY>
 void launch_Hb_by(int kzi, int nwg, hipStream_t st, const void *d, const int *wg0, int n, const int *tc,
                   const int *te, int s) {
     switch (kzi) {
-        case 0: launch_Hb_cfg<S, T, BY, 4>(nwg, st, d, wg0, n, tc, te, s); break;
-        case 1: launch_Hb_cfg<S, T, BY, 8>(nwg, st, d, wg0, n, tc, te, s); break;
-        case 2: launch_Hb_cfg<S, T, BY, 16>(nwg, st, d, wg0, n, tc, te, s); break;
-        default: launch_Hb_cfg<S, T, BY, 32>(nwg, st, d, wg0, n, tc, te, s); break;
+        case 0: launch_Hb_cfg<S, T, BY, 1>(nwg, st, d, wg0, n, tc, te, s); break;
+        case 1: launch_Hb_cfg<S, T, BY, 2>(nwg, st, d, wg0, n, tc, te, s); break;
+        case 2: launch_Hb_cfg<S, T, BY, 4>(nwg, st, d, wg0, n, tc, te, s); break;
+        default: launch_Hb_cfg<S, T, BY, 8>(nwg, st, d, wg0, n, tc, te, s); break;
     }
 }
 
@@ -931,7 +931,7 @@ struct aefdtd::Batch {
     int precision = AEFDTD_FP32;
     hipStream_t stream = nullptr;
     std::vector<aefdtd_grid *> members;
-    int tile_e = 2 | (1 << 2), tile_h = 2 | (1 << 2);
+    int tile_e = 2 | (2 << 2), tile_h = 2 | (2 << 2);
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

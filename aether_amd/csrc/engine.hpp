@@ -149,7 +149,7 @@ struct aefdtd_grid {
     void *d_sdesc = nullptr;          // sensor descriptors for the sampling kernel
     long stage_P = 0, stage_maxP = 0;
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
-    int tile_e = 2 | (1 << 2) | (1 << 4), tile_h = 2 | (1 << 2) | (1 << 4);
+    int tile_e = 2 | (3 << 2) | (1 << 4), tile_h = 2 | (3 << 2) | (1 << 4);
     // bookkeeping
     bool ready = false;               // device copies up to date
     bool pml_dirty = true;

@@ -613,19 +613,19 @@ bool time_chunk(aefdtd_batch *b, double &ms, int &rc) {
 
 }  // namespace
 
-// Online tuner over the real run: every time-skewing code (tile height,
-// depth) when 2D TE/TM members are present, then the tile codes of the
-// per-step passes when other members are: every (BY, remap) with the
-// smallest KZ when all of them are one cell thick in z (KZ is then
-// irrelevant), else all 32; E and H share a code.  Every candidate computes
-// the same steps, so tuning advances the members like any other chunk.
+// Online tuner over the real run.  Candidates: the per-step passes and the
+// large time-skewed tiles at depths 4 and 8 when 2D TE/TM members are
+// present; then, for the members left to the per-step passes, the tile codes
+// with the XCD fold and BY in {2,4,8}, KZ in {1,2,4,8} (E and H share a
+// code).  Every candidate computes the same steps, so tuning advances the
+// members like any other chunk.
 int aefdtd_batch_autotune(aefdtd_batch *b, int out[3]) {
     if (!b) return AEFDTD_EINVAL;
     int rc = AEFDTD_OK;
     if (batch_skewed_members(b->b) > 0 || batch_skew(b->b) < 0) {
         int best_s = batch_skew(b->b);
         double best = 1e30;
-        for (int code = -1; code < 12; code++) {
+        for (int code : {-1, 1 | (1 << 2), 1 | (2 << 2)}) {
             if (batch_set_skew(b->b, code) != AEFDTD_OK) continue;
             if (code >= 0 && batch_skewed_members(b->b) == 0) continue;
             double ms;
@@ -637,18 +637,17 @@ int aefdtd_batch_autotune(aefdtd_batch *b, int out[3]) {
     }
     // members left to the per-step passes
     const bool stepped = batch_skew(b->b) < 0 || batch_skewed_members(b->b) < (int)b->runs.size();
-    bool flat = true;
-    for (aefdtd_run *r : b->runs) flat = flat && r->g->Nz == 1;
-    int best_c = 2 | (1 << 2);
+    int best_c = 2 | (2 << 2) | (1 << 4);
     if (stepped) {
         double best = 1e30;
-        for (int c = 0; c < 32; c++) {
-            if (flat && ((c >> 2) & 3) != 0) continue;
-            TRY(batch_set_tiles(b->b, c, c));
-            double ms;
-            if (!time_chunk(b, ms, rc)) break;
-            if (ms < best) { best = ms; best_c = c; }
-        }
+        for (int by = 1; by < 4; by++)
+            for (int kz = 0; kz < 4; kz++) {
+                const int c = by | (kz << 2) | (1 << 4);
+                TRY(batch_set_tiles(b->b, c, c));
+                double ms;
+                if (!time_chunk(b, ms, rc)) break;
+                if (ms < best) { best = ms; best_c = c; }
+            }
         if (rc) return rc;
     }
     TRY(batch_set_tiles(b->b, best_c, best_c));

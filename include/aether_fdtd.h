@@ -341,9 +341,10 @@ int aefdtd_batch_set_skew(aefdtd_batch *b, int code);
  * time-skewed passes, current skew code. */
 int aefdtd_batch_info(const aefdtd_batch *b, int out[4]);
 /* Online tuner: advances the batch by timed chunks, one per candidate
- * time-skew code and then per tile shape of the per-step passes (all
- * candidates compute the same steps), keeps the fastest; out[3] = chosen
- * E tile, H tile and skew codes. */
+ * time-skew setting (off, large tiles at depth 4 and 8) and then per tile
+ * shape of the per-step passes (BY 2..8, KZ 1..8, XCD fold on); all
+ * candidates compute the same steps.  Keeps the fastest; out[3] = chosen E
+ * tile, H tile and skew codes. */
 int aefdtd_batch_autotune(aefdtd_batch *b, int out[3]);
 /* Device time of aefdtd_batch_step (HIP events on the batch stream): on=1
  * resets and starts, 0 stops, -1 only reads. */
@@ -404,7 +405,8 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
 /* Tile autotuner: times every compiled tile shape of the E and H passes on
  * this grid (reps launches each), keeps the fastest and restores the grid's
  * state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
- * XCD-remap<<4).  No counterpart in the reference (a CPU thread-count choice). */
+ * XCD-remap<<4; BY in 1,2,4,8 rows per workgroup, KZ in 1,2,4,8 planes per
+ * lane, or rows per lane on grids one cell thick in z).  No counterpart in the reference (a CPU thread-count choice). */
 int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
 /* Power-of-two storage scales of the six field components (stored value =
