@@ -972,7 +972,7 @@ __global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, F
 // wrap around the grid like the stencil does, so boundary tiles see exactly
 // the neighbours the per-step passes see.  Each cell's update is e_cell_vals /
 // h_cell_vals, the same arithmetic as the per-step passes.
-constexpr int TBX = 128;          // tile width: 64 lanes x 2 columns
+constexpr int TBX = 64;           // tile width: one column per lane
 constexpr int TB_WAVES = 8;       // waves per workgroup
 
 // CPML psi of one 2D cell in registers: TM (Ez,Hx,Hy) uses Ezx, Ezy, Hxy, Hyx;
@@ -997,6 +997,116 @@ struct RegPsiH {
     __device__ __forceinline__ T &Hzx(int) const { return c; }
     __device__ __forceinline__ T &Hzy(int) const { return d; }
 };
+
+// The cell updates of the time-skewed passes: e_cell_vals / h_cell_vals
+// restricted to what a time-skewed grid has (2D, TM or TE, CPML on x/y, no
+// TFSF, RC or plane injection, oscillators), the same operations in the same
+// order, with every per-grid constant held in registers for the whole launch.
+template <typename T>
+struct TbCtx {
+    bool px, py;
+    int Nx, Ny, nosc;
+    const OscDev *osc;
+    const T *bEx, *cEx, *bEy, *cEy, *bHx, *cHx, *bHy, *cHy;
+    T ikzE, ikzH, dtm, dtdmx, dtdmy, dtdmz;
+};
+
+template <int POL, typename T>
+__device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const MatL<T> &M, int i, int j, int t, T &e0, T &e1,
+                                          T dHa, T dHb, T ikx, T iky, int nx_, int ny_, T &pa, T &pb, int om) {
+    const T zero = 0;
+    if (POL == 0) {        // Ez from dHy_x = dHa, dHx_y = dHb
+        T ez = M.C1 * e0 + M.C2x * ikx * dHa - M.C2y * iky * dHb;
+        if (x.px) {
+            if (nx_ >= 0) {
+                pa = x.bEx[i] * pa + x.cEx[i] * dHa;
+                ez += M.C4 * pa;
+            }
+            if (i == 0) ez = 0;
+        }
+        if (x.py) {
+            if (ny_ >= 0) {
+                pb = x.bEy[j] * pb + x.cEy[j] * dHb;
+                ez -= M.C4 * pb;
+            }
+            if (j == 0) ez = 0;
+        }
+        // om: the oscillators whose box holds this cell, in registration order
+        for (int q = 0; om; q++, om >>= 1)
+            if (om & 1) ez += (T)x.osc[q].val[t - x.osc[q].step0];
+        e0 = ez;
+    } else {               // Ex, Ey from dHz_y = dHa, dHz_x = dHb
+        T ex = M.C1 * e0 + M.C2y * iky * dHa - M.C2z * x.ikzE * zero;
+        T ey = M.C1 * e1 + M.C2z * x.ikzE * zero - M.C2x * ikx * dHb;
+        if (x.py) {
+            if (ny_ >= 0) {
+                pa = x.bEy[j] * pa + x.cEy[j] * dHa;
+                ex += M.C4 * pa;
+            }
+            if (j == 0) ex = 0;
+        }
+        if (x.px) {
+            if (nx_ >= 0) {
+                pb = x.bEx[i] * pb + x.cEx[i] * dHb;
+                ey -= M.C4 * pb;
+            }
+            if (i == 0) ey = 0;
+        }
+        for (int q = 0; om; q++, om >>= 1)
+            if (om & 1) {
+                const T v = (T)x.osc[q].val[t - x.osc[q].step0];
+                if (x.osc[q].comp == 0) ex += v;
+                else ey += v;
+            }
+        e0 = ex;
+        e1 = ey;
+    }
+}
+
+template <int POL, typename T>
+__device__ __forceinline__ void tb_h_cell(const TbCtx<T> &x, int i, int j, T &h0, T &h1, T dEa, T dEb, T ikx, T iky,
+                                          int nx_, int ny_, T &pc, T &pd) {
+    const T zero = 0;
+    if (POL == 0) {        // Hx, Hy from dEz_y = dEa, dEz_x = dEb
+        T hx = h0, hy = h1;
+        hx += x.dtdmz * x.ikzH * zero - x.dtdmy * iky * dEa;
+        hy += x.dtdmx * ikx * dEb - x.dtdmz * x.ikzH * zero;
+        if (x.py) {
+            if (ny_ >= 0) {
+                pc = x.bHy[j] * pc + x.cHy[j] * dEa;
+                hx -= x.dtm * pc;
+            }
+            if (j == x.Ny - 1) hx = 0;
+        }
+        if (x.px) {
+            if (nx_ >= 0) {
+                pd = x.bHx[i] * pd + x.cHx[i] * dEb;
+                hy += x.dtm * pd;
+            }
+            if (i == x.Nx - 1) hy = 0;
+        }
+        h0 = hx;
+        h1 = hy;
+    } else {               // Hz from dEx_y = dEa, dEy_x = dEb
+        T hz = h0;
+        hz += x.dtdmy * iky * dEa - x.dtdmx * ikx * dEb;
+        if (x.px) {
+            if (nx_ >= 0) {
+                pc = x.bHx[i] * pc + x.cHx[i] * dEb;
+                hz -= x.dtm * pc;
+            }
+            if (i == x.Nx - 1) hz = 0;
+        }
+        if (x.py) {
+            if (ny_ >= 0) {
+                pd = x.bHy[j] * pd + x.cHy[j] * dEa;
+                hz += x.dtm * pd;
+            }
+            if (j == x.Ny - 1) hz = 0;
+        }
+        h0 = hz;
+    }
+}
 
 // Second copy of a time-skewed grid's live state: the three live field
 // components (TM: Ez Hx Hy, TE: Ex Ey Hz) and its four CPML psi arrays (TM:
@@ -1056,7 +1166,19 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         psrc[c] = odd ? alt.p[c] : own_p[c];
         pdst[c] = odd ? own_p[c] : alt.p[c];
     }
-    // per-column / per-row data of the lane's cells
+    TbCtx<T> x;
+    x.px = ea.pml[0] || ea.pml[1];
+    x.py = ea.pml[2] || ea.pml[3];
+    x.Nx = Nx;
+    x.Ny = Ny;
+    x.nosc = ea.nosc;
+    x.osc = ea.osc;
+    x.ikzE = ea.pf.ikE[2][0];
+    x.ikzH = ha.pf.ikH[2][0];
+    x.dtm = ha.dtm; x.dtdmx = ha.dtdmx; x.dtdmy = ha.dtdmy; x.dtdmz = ha.dtdmz;
+    x.bEx = ea.pf.bE[0]; x.cEx = ea.pf.cE[0]; x.bEy = ea.pf.bE[1]; x.cEy = ea.pf.cE[1];
+    x.bHx = ha.pf.bH[0]; x.cHx = ha.pf.cH[0]; x.bHy = ha.pf.bH[1]; x.cHy = ha.pf.cH[1];
+    // per-column / per-row data of the lane's cells (profiles b, c/D of E and H)
     int gi[CX], gj[RY], nxE[CX], nyE[RY], nxH[CX], nyH[RY];
     T ikxE[CX], ikyE[RY], ikxH[CX], ikyH[RY];
 #pragma unroll
@@ -1107,8 +1229,51 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         }
     // sensors of this grid
     const int sb = sbeg[2 * r], se = sbeg[2 * r + 1];
-    __syncthreads();
+    // per cell: the oscillators covering it (bits 0-7) and the sensors it is a
+    // sample point of (bits 8-31, sensors beyond the 24th are checked in full)
+    int hit[CX * RY];
+#pragma unroll
+    for (int c = 0; c < CX * RY; c++) hit[c] = 0;
+    for (int q = 0; q < x.nosc; q++) {
+        const int x1 = x.osc[q].x1, x2 = x.osc[q].x2, y1 = x.osc[q].y1, y2 = x.osc[q].y2;
+        const bool zin = x.osc[q].z1 <= 0 && x.osc[q].z2 > 0;
+#pragma unroll
+        for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+            for (int cx = 0; cx < CX; cx++)
+                if (zin && gi[cx] >= x1 && gi[cx] < x2 && gj[ry] >= y1 && gj[ry] < y2) hit[ry * CX + cx] |= 1 << q;
+    }
+    for (int e = sb; e < se; e++) {
+        const SensDesc d = samp[e].d;
+        const int bit = e - sb < 24 ? 1 << (8 + e - sb) : 1 << 31;
+#pragma unroll
+        for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+            for (int cx = 0; cx < CX; cx++) {
+                const int i = gi[cx], j = gj[ry];
+                bool on;
+                if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM)
+                    on = i == d.x1 && j >= d.y1 && j < d.y1 + d.span1 && d.z1 == 0;
+                else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM)
+                    on = j == d.y1 && i >= d.x1 && i < d.x1 + d.span1 && d.z1 == 0;
+                else
+                    on = d.z1 == 0 && i >= d.x1 && i < d.x1 + d.span1 && j >= d.y1 && j < d.y1 + d.span2;
+                if (on) hit[ry * CX + cx] |= bit;
+            }
+    }
+    // a tile none of whose inner cells is a sample point skips the sampling and its barrier
+    int any_sample = 0;
+#pragma unroll
+    for (int ry = 0; ry < RY; ry++)
+#pragma unroll
+        for (int cx = 0; cx < CX; cx++) {
+            const int ui = tx + 64 * cx - TS, uj = wv + TB_WAVES * ry - TS;
+            if (ui >= 0 && ui < IX && uj >= 0 && uj < IY && ox + ui < Nx && oy + uj < Ny)
+                any_sample |= hit[ry * CX + cx] >> 8;
+        }
+    any_sample = __syncthreads_or(any_sample);
     const T zero = 0;
+#pragma unroll 1
     for (int st = 0; st < nsteps; st++) {
         const int t = tbeg + st;
         // E update (cells with an i-1 and a j-1 neighbour in the tile)
@@ -1118,19 +1283,18 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             for (int cx = 0; cx < CX; cx++) {
                 const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
                 if (li < 1 || lj < 1) continue;
-                const int id = gi[cx] + Nx * gj[ry];
                 const MatL<T> &M = sm[mat[c]];
                 if (POL == 0) {
-                    const V3<T> e = e_cell_vals(ea, M, gi[cx], gj[ry], 0, id, zero, zero, F0[l], zero, zero, zero,
-                                                zero, F2[l] - F2[l - 1], F1[l] - F1[l - TBX], ikxE[cx], ikyE[ry],
-                                                nxE[cx], nyE[ry], t, RegPsiE<T>{pa[c], pb[c]});
-                    F0[l] = e.z;
+                    T e0 = F0[l], e1 = zero;
+                    tb_e_cell<0>(x, M, gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - 1], F1[l] - F1[l - TBX], ikxE[cx],
+                                 ikyE[ry], nxE[cx], nyE[ry], pa[c], pb[c], hit[c] & 0xff);
+                    F0[l] = e0;
                 } else {
-                    const V3<T> e = e_cell_vals(ea, M, gi[cx], gj[ry], 0, id, F0[l], F1[l], zero,
-                                                F2[l] - F2[l - TBX], zero, zero, F2[l] - F2[l - 1], zero, zero,
-                                                ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], t, RegPsiE<T>{pa[c], pb[c]});
-                    F0[l] = e.x;
-                    F1[l] = e.y;
+                    T e0 = F0[l], e1 = F1[l];
+                    tb_e_cell<1>(x, M, gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - TBX], F2[l] - F2[l - 1], ikxE[cx],
+                                 ikyE[ry], nxE[cx], nyE[ry], pa[c], pb[c], hit[c] & 0xff);
+                    F0[l] = e0;
+                    F1[l] = e1;
                 }
             }
         __syncthreads();
@@ -1142,31 +1306,36 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                 const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
                 if (li > TBX - 2 || lj > LY - 2) continue;
                 if (POL == 0) {
-                    T hx = F1[l], hy = F2[l], hz = zero;
+                    T h0 = F1[l], h1 = F2[l];
                     const T ez = F0[l];
-                    h_cell_vals(ha, gi[cx], gj[ry], 0, hx, hy, hz, zero, zero, F0[l + TBX] - ez, F0[l + 1] - ez, zero,
-                                zero, ikxH[cx], ikyH[ry], nxH[cx], nyH[ry], RegPsiH<T>{pc[c], pd[c]});
-                    F1[l] = hx;
-                    F2[l] = hy;
+                    tb_h_cell<0>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - ez, F0[l + 1] - ez, ikxH[cx], ikyH[ry],
+                                 nxH[cx], nyH[ry], pc[c], pd[c]);
+                    F1[l] = h0;
+                    F2[l] = h1;
                 } else {
-                    T hx = zero, hy = zero, hz = F2[l];
-                    h_cell_vals(ha, gi[cx], gj[ry], 0, hx, hy, hz, zero, zero, zero, zero, F0[l + TBX] - F0[l],
-                                F1[l + 1] - F1[l], ikxH[cx], ikyH[ry], nxH[cx], nyH[ry], RegPsiH<T>{pc[c], pd[c]});
-                    F2[l] = hz;
+                    T h0 = F2[l], h1 = zero;
+                    tb_h_cell<1>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - F0[l], F1[l + 1] - F1[l], ikxH[cx],
+                                 ikyH[ry], nxH[cx], nyH[ry], pc[c], pd[c]);
+                    F2[l] = h0;
                 }
             }
         __syncthreads();
         // sensor samples of the inner cells (local_E/H, core/fdtd_utils.cpp:457-507)
-        if (sb < se) {
+        if (any_sample) {
 #pragma unroll
             for (int ry = 0; ry < RY; ry++)
 #pragma unroll
                 for (int cx = 0; cx < CX; cx++) {
-                    const int li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
+                    const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
                     const int ui = li - TS, uj = lj - TS;
-                    if (ui < 0 || ui >= IX || uj < 0 || uj >= IY || ox + ui >= Nx || oy + uj >= Ny) continue;
+                    if ((hit[c] >> 8) == 0 || ui < 0 || ui >= IX || uj < 0 || uj >= IY || ox + ui >= Nx ||
+                        oy + uj >= Ny)
+                        continue;
                     const int i = gi[cx], j = gj[ry];
+#pragma unroll 1
                     for (int e = sb; e < se; e++) {
+                        const int bit = e - sb < 24 ? 1 << (8 + e - sb) : 1 << 31;
+                        if (!(hit[c] & bit)) continue;
                         const SensDesc &d = samp[e].d;
                         long p;
                         if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) {
