@@ -1069,7 +1069,7 @@ int batch_build(Batch *b) {
     b->nsamp = (int)samp.size();
     b->ndft = (int)dft.size();
     b->ldsE = maxmat * sizeof(MatL<T>);
-    b->ldsT = ((maxmat * sizeof(MatL<T>) + 15) & ~(size_t)15) + 3 * (size_t)TBX * ly * sizeof(T);
+    b->ldsT = 3 * (size_t)TBX * ly * sizeof(T);
     int rc;
     if ((rc = batch_upload(b, &b->d_ea, ea)) || (rc = batch_upload(b, &b->d_ha, ha)) ||
         (rc = batch_upload(b, (void **)&b->d_wgE, wgE)) || (rc = batch_upload(b, (void **)&b->d_wgH, wgH)) ||

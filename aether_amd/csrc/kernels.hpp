@@ -1011,22 +1011,35 @@ struct TbCtx {
     T ikzE, ikzH, dtm, dtdmx, dtdmy, dtdmz;
 };
 
+// Per-cell update coefficients held in registers (FDTD_Material C1, C2x, C2y, C4).
+template <typename T>
+struct TbCoef {
+    T C1, C2x, C2y, C4;
+};
+
+// CPML profile values of a cell's column (x) and row (y), E and H positions.
+template <typename T>
+struct TbProf {
+    T bE, cE, bH, cH;
+};
+
 template <int POL, typename T>
-__device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const MatL<T> &M, int i, int j, int t, T &e0, T &e1,
-                                          T dHa, T dHb, T ikx, T iky, int nx_, int ny_, T &pa, T &pb, int om) {
-    const T zero = 0;
+__device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const TbCoef<T> &M, int i, int j, int t, T &e0, T &e1,
+                                          T dHa, T dHb, T ikx, T iky, int nx_, int ny_, const TbProf<T> &PX,
+                                          const TbProf<T> &PY, T &pa, T &pb, int om) {
+    // terms along the one-cell z axis are products with an exact zero and are left out
     if (POL == 0) {        // Ez from dHy_x = dHa, dHx_y = dHb
         T ez = M.C1 * e0 + M.C2x * ikx * dHa - M.C2y * iky * dHb;
         if (x.px) {
             if (nx_ >= 0) {
-                pa = x.bEx[i] * pa + x.cEx[i] * dHa;
+                pa = PX.bE * pa + PX.cE * dHa;
                 ez += M.C4 * pa;
             }
             if (i == 0) ez = 0;
         }
         if (x.py) {
             if (ny_ >= 0) {
-                pb = x.bEy[j] * pb + x.cEy[j] * dHb;
+                pb = PY.bE * pb + PY.cE * dHb;
                 ez -= M.C4 * pb;
             }
             if (j == 0) ez = 0;
@@ -1036,18 +1049,18 @@ __device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const MatL<T> &M, i
             if (om & 1) ez += (T)x.osc[q].val[t - x.osc[q].step0];
         e0 = ez;
     } else {               // Ex, Ey from dHz_y = dHa, dHz_x = dHb
-        T ex = M.C1 * e0 + M.C2y * iky * dHa - M.C2z * x.ikzE * zero;
-        T ey = M.C1 * e1 + M.C2z * x.ikzE * zero - M.C2x * ikx * dHb;
+        T ex = M.C1 * e0 + M.C2y * iky * dHa;
+        T ey = M.C1 * e1 - M.C2x * ikx * dHb;
         if (x.py) {
             if (ny_ >= 0) {
-                pa = x.bEy[j] * pa + x.cEy[j] * dHa;
+                pa = PY.bE * pa + PY.cE * dHa;
                 ex += M.C4 * pa;
             }
             if (j == 0) ex = 0;
         }
         if (x.px) {
             if (nx_ >= 0) {
-                pb = x.bEx[i] * pb + x.cEx[i] * dHb;
+                pb = PX.bE * pb + PX.cE * dHb;
                 ey -= M.C4 * pb;
             }
             if (i == 0) ey = 0;
@@ -1065,22 +1078,21 @@ __device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const MatL<T> &M, i
 
 template <int POL, typename T>
 __device__ __forceinline__ void tb_h_cell(const TbCtx<T> &x, int i, int j, T &h0, T &h1, T dEa, T dEb, T ikx, T iky,
-                                          int nx_, int ny_, T &pc, T &pd) {
-    const T zero = 0;
+                                          int nx_, int ny_, const TbProf<T> &PX, const TbProf<T> &PY, T &pc, T &pd) {
     if (POL == 0) {        // Hx, Hy from dEz_y = dEa, dEz_x = dEb
         T hx = h0, hy = h1;
-        hx += x.dtdmz * x.ikzH * zero - x.dtdmy * iky * dEa;
-        hy += x.dtdmx * ikx * dEb - x.dtdmz * x.ikzH * zero;
+        hx += -(x.dtdmy * iky * dEa);
+        hy += x.dtdmx * ikx * dEb;
         if (x.py) {
             if (ny_ >= 0) {
-                pc = x.bHy[j] * pc + x.cHy[j] * dEa;
+                pc = PY.bH * pc + PY.cH * dEa;
                 hx -= x.dtm * pc;
             }
             if (j == x.Ny - 1) hx = 0;
         }
         if (x.px) {
             if (nx_ >= 0) {
-                pd = x.bHx[i] * pd + x.cHx[i] * dEb;
+                pd = PX.bH * pd + PX.cH * dEb;
                 hy += x.dtm * pd;
             }
             if (i == x.Nx - 1) hy = 0;
@@ -1092,14 +1104,14 @@ __device__ __forceinline__ void tb_h_cell(const TbCtx<T> &x, int i, int j, T &h0
         hz += x.dtdmy * iky * dEa - x.dtdmx * ikx * dEb;
         if (x.px) {
             if (nx_ >= 0) {
-                pc = x.bHx[i] * pc + x.cHx[i] * dEb;
+                pc = PX.bH * pc + PX.cH * dEb;
                 hz -= x.dtm * pc;
             }
             if (i == x.Nx - 1) hz = 0;
         }
         if (x.py) {
             if (ny_ >= 0) {
-                pd = x.bHy[j] * pd + x.cHy[j] * dEa;
+                pd = PY.bH * pd + PY.cH * dEa;
                 hz += x.dtm * pd;
             }
             if (j == x.Ny - 1) hz = 0;
@@ -1144,10 +1156,8 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
     const int Nx = ea.Nx, Ny = ea.Ny;
     const int ntx = (Nx + IX - 1) / IX, nty = (Ny + IY - 1) / IY;
     if (w >= ntx * nty) return;
-    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
-    T *F = reinterpret_cast<T *>(lds_raw + ((ea.nmat * sizeof(MatL<T>) + 15) & ~(size_t)15));
+    T *F = reinterpret_cast<T *>(lds_raw);
     T *F0 = F, *F1 = F + TBX * LY, *F2 = F + 2 * TBX * LY;      // TM: Ez Hx Hy; TE: Ex Ey Hz
-    stage_mats(ea, sm);
     const int bx = w % ntx, by = w / ntx;
     const int ox = bx * IX, oy = by * IY;           // inner origin (global)
     const int tx = threadIdx.x, wv = threadIdx.y;
@@ -1181,6 +1191,7 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
     // per-column / per-row data of the lane's cells (profiles b, c/D of E and H)
     int gi[CX], gj[RY], nxE[CX], nyE[RY], nxH[CX], nyH[RY];
     T ikxE[CX], ikyE[RY], ikxH[CX], ikyH[RY];
+    TbProf<T> PX[CX], PY[RY];
 #pragma unroll
     for (int cx = 0; cx < CX; cx++) {
         int g = (ox - TS + tx + 64 * cx) % Nx;
@@ -1190,6 +1201,8 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         nxH[cx] = slabH(g, Nx, ea.pml[0], ea.pml[1]);
         ikxE[cx] = ea.pf.ikE[0][g];
         ikxH[cx] = ha.pf.ikH[0][g];
+        const T z = 0;
+        PX[cx] = x.px ? TbProf<T>{x.bEx[g], x.cEx[g], x.bHx[g], x.cHx[g]} : TbProf<T>{z, z, z, z};
     }
 #pragma unroll
     for (int ry = 0; ry < RY; ry++) {
@@ -1200,9 +1213,11 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         nyH[ry] = slabH(g, Ny, ea.pml[2], ea.pml[3]);
         ikyE[ry] = ea.pf.ikE[1][g];
         ikyH[ry] = ha.pf.ikH[1][g];
+        const T z = 0;
+        PY[ry] = x.py ? TbProf<T>{x.bEy[g], x.cEy[g], x.bHy[g], x.cHy[g]} : TbProf<T>{z, z, z, z};
     }
     // load: fields into LDS, material ids and psi into registers
-    int mat[CX * RY];
+    TbCoef<T> cf[CX * RY];
     T pa[CX * RY], pb[CX * RY], pc[CX * RY], pd[CX * RY];
 #pragma unroll
     for (int ry = 0; ry < RY; ry++)
@@ -1213,7 +1228,8 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             F0[l] = src[0][id];
             F1[l] = src[1][id];
             F2[l] = src[2][id];
-            mat[c] = ea.mat[id];
+            const MatDev<T> &G = ea.mt[ea.mat[id]];
+            cf[c] = TbCoef<T>{G.C1, G.C2x, G.C2y, G.C4};
             const T z = 0;
             if (POL == 0) {
                 pa[c] = nxE[cx] >= 0 ? psrc[0][nxE[cx] + wxs * gj[ry]] : z;
@@ -1283,16 +1299,15 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             for (int cx = 0; cx < CX; cx++) {
                 const int c = ry * CX + cx, li = tx + 64 * cx, lj = wv + TB_WAVES * ry, l = li + TBX * lj;
                 if (li < 1 || lj < 1) continue;
-                const MatL<T> &M = sm[mat[c]];
                 if (POL == 0) {
                     T e0 = F0[l], e1 = zero;
-                    tb_e_cell<0>(x, M, gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - 1], F1[l] - F1[l - TBX], ikxE[cx],
-                                 ikyE[ry], nxE[cx], nyE[ry], pa[c], pb[c], hit[c] & 0xff);
+                    tb_e_cell<0>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - 1], F1[l] - F1[l - TBX],
+                                 ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], PX[cx], PY[ry], pa[c], pb[c], hit[c] & 0xff);
                     F0[l] = e0;
                 } else {
                     T e0 = F0[l], e1 = F1[l];
-                    tb_e_cell<1>(x, M, gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - TBX], F2[l] - F2[l - 1], ikxE[cx],
-                                 ikyE[ry], nxE[cx], nyE[ry], pa[c], pb[c], hit[c] & 0xff);
+                    tb_e_cell<1>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - TBX], F2[l] - F2[l - 1],
+                                 ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], PX[cx], PY[ry], pa[c], pb[c], hit[c] & 0xff);
                     F0[l] = e0;
                     F1[l] = e1;
                 }
@@ -1309,13 +1324,13 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                     T h0 = F1[l], h1 = F2[l];
                     const T ez = F0[l];
                     tb_h_cell<0>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - ez, F0[l + 1] - ez, ikxH[cx], ikyH[ry],
-                                 nxH[cx], nyH[ry], pc[c], pd[c]);
+                                 nxH[cx], nyH[ry], PX[cx], PY[ry], pc[c], pd[c]);
                     F1[l] = h0;
                     F2[l] = h1;
                 } else {
                     T h0 = F2[l], h1 = zero;
                     tb_h_cell<1>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - F0[l], F1[l + 1] - F1[l], ikxH[cx],
-                                 ikyH[ry], nxH[cx], nyH[ry], pc[c], pd[c]);
+                                 ikyH[ry], nxH[cx], nyH[ry], PX[cx], PY[ry], pc[c], pd[c]);
                     F2[l] = h0;
                 }
             }
