@@ -127,20 +127,144 @@ struct MatL {
     int rc;
 };
 
+// A load whose address is the same in every lane of the wavefront (a profile
+// value or an auxiliary-grid value of the current plane): through the
+// constant address space it becomes a scalar load, which neither occupies
+// vector registers nor counts against the vector-memory counter that the
+// software pipelines of the passes wait on.  Only for data that the kernel
+// itself does not write.
+template <typename T>
+__device__ __forceinline__ T uload(const T *p, int n) {
+    return ((const __attribute__((address_space(4))) T *)p)[n];
+}
+template <typename S, typename T>
+__device__ __forceinline__ T uload(const Fld<S, T> &f, int n) {
+    if constexpr (std::is_same_v<S, T>) return uload(f.p, n);
+    else return (T)uload(f.p, n) * f.ld;
+}
+
+// A CPML psi held in a register and written through to its slab array.
+template <typename T>
+struct PsiRef {
+    T &r;
+    T *g;
+    __device__ __forceinline__ operator T() const { return r; }
+    __device__ __forceinline__ void operator=(T v) const {
+        r = v;
+        *g = v;
+    }
+};
+
+// Which boundary treatments a set of cells can need (the side mask): CPML
+// slabs or PEC walls along x, y, z, and TFSF surface corrections.  The cell
+// updates test their per-cell conditions only under the bits set, so a tile
+// that meets no boundary runs the bare stencil.
+constexpr int SIDE_X = 1, SIDE_Y = 2, SIDE_Z = 4, SIDE_TFSF = 8;
+
+template <class A>
+__device__ __forceinline__ int side_all(const A &a) {
+    return ((a.pml[0] || a.pml[1]) ? SIDE_X : 0) | ((a.pml[2] || a.pml[3]) ? SIDE_Y : 0) |
+           ((a.pml[4] || a.pml[5]) ? SIDE_Z : 0) | (a.tfsf ? SIDE_TFSF : 0);
+}
+
+// [lo, hi) meets [p, q]
+__device__ __forceinline__ bool meets(int lo, int hi, int p, int q) { return lo <= q && hi > p; }
+
+// Side mask of the cell block [ilo,ihi) x [jlo,jhi) x [klo,khi) in the E pass
+// (h = 0) or the H pass (h = 1).  E cells in a CPML slab lie in 1..pm-1 and
+// N-pp+1..N-1 with the PEC wall at 0; H cells in 0..pm-1 and N-pp..N-2 with
+// the wall at N-1.  TFSF corrections touch E cells on the box faces and H
+// cells one cell outside them; the test here is a superset of both.
+template <class A>
+__device__ __forceinline__ int side_block(const A &a, int h, int ilo, int ihi, int jlo, int jhi, int klo, int khi) {
+    const int N[3] = {a.Nx, a.Ny, a.Nz};
+    const int lo[3] = {ilo, jlo, klo}, hi[3] = {ihi, jhi, khi};
+    int m = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int pm = a.pml[2 * d], pp = a.pml[2 * d + 1];
+        if (!(pm || pp)) continue;
+        const bool near = h ? lo[d] < pm : lo[d] < max(pm, 1);
+        const bool far = h ? hi[d] > N[d] - max(pp, 1) : hi[d] > N[d] - pp + 1;
+        if (near || far) m |= 1 << d;
+    }
+    if (a.tfsf && meets(ilo, ihi, a.tb0 - 1, a.tb1) && meets(jlo, jhi, a.tb2 - 1, a.tb3) &&
+        meets(klo, khi, a.tb4 - 1, a.tb5)) {
+        const bool fx = !a.tper_x && (meets(ilo, ihi, a.tb0 - 1, a.tb0) || meets(ilo, ihi, a.tb1 - 1, a.tb1));
+        const bool fy = !a.tper_y && (meets(jlo, jhi, a.tb2 - 1, a.tb2) || meets(jlo, jhi, a.tb3 - 1, a.tb3));
+        const bool fz = meets(klo, khi, a.tb4 - 1, a.tb4) || meets(klo, khi, a.tb5 - 1, a.tb5);
+        if (fx || fy || fz) m |= SIDE_TFSF;
+    }
+    return m;
+}
+
 // ---------------------------------------------------------------- E cell
 // CPML psi of an E cell in the grid's slab arrays (x slab: (pxm+pxp, Ny, Nz),
 // y slab: (Nx, pym+pyp, Nz), z slab: (Nx, Ny, pzm+pzp)); the argument is the
 // cell's slab index along the PML axis.
 template <typename S, typename T>
+struct PsiAddrE {
+    const EArgs<S, T> &a;
+    int i, j, k;
+    __device__ __forceinline__ T *xy(int n) const { return a.pExy + i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k); }
+    __device__ __forceinline__ T *xz(int n) const { return a.pExz + i + a.Nx * j + a.Nx * a.Ny * n; }
+    __device__ __forceinline__ T *yx(int n) const { return a.pEyx + n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k); }
+    __device__ __forceinline__ T *yz(int n) const { return a.pEyz + i + a.Nx * j + a.Nx * a.Ny * n; }
+    __device__ __forceinline__ T *zx(int n) const { return a.pEzx + n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k); }
+    __device__ __forceinline__ T *zy(int n) const { return a.pEzy + i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k); }
+};
+
+// CPML z-slab index of an E cell at plane k (near 1..pm-1, far N-pp+1..N-1).
+template <typename S, typename T>
+__device__ __forceinline__ int slabEz(const EArgs<S, T> &a, int k) {
+    if (k >= 1 && k < a.pml[4]) return k;
+    if (k >= a.Nz - a.pml[5] + 1) return k - (a.Nz - a.pml[5]) + a.pml[4];
+    return -1;
+}
+
+// Boundary data of an E cell read where they lie: psi in the slab arrays,
+// x and y CPML profile values in the profile arrays.
+template <typename S, typename T>
 struct GlobalPsiE {
     const EArgs<S, T> &a;
     int i, j, k;
-    __device__ __forceinline__ T &Exy(int n) const { return a.pExy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
-    __device__ __forceinline__ T &Exz(int n) const { return a.pExz[i + a.Nx * j + a.Nx * a.Ny * n]; }
-    __device__ __forceinline__ T &Eyx(int n) const { return a.pEyx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
-    __device__ __forceinline__ T &Eyz(int n) const { return a.pEyz[i + a.Nx * j + a.Nx * a.Ny * n]; }
-    __device__ __forceinline__ T &Ezx(int n) const { return a.pEzx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
-    __device__ __forceinline__ T &Ezy(int n) const { return a.pEzy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+    __device__ __forceinline__ T &Exy(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.xy(n); }
+    __device__ __forceinline__ T &Exz(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.xz(n); }
+    __device__ __forceinline__ T &Eyx(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.yx(n); }
+    __device__ __forceinline__ T &Eyz(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.yz(n); }
+    __device__ __forceinline__ T &Ezx(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.zx(n); }
+    __device__ __forceinline__ T &Ezy(int n) const { return *PsiAddrE<S, T>{a, i, j, k}.zy(n); }
+    __device__ __forceinline__ T bx() const { return a.pf.bE[0][i]; }
+    __device__ __forceinline__ T cx() const { return a.pf.cE[0][i]; }
+    __device__ __forceinline__ T by() const { return a.pf.bE[1][j]; }
+    __device__ __forceinline__ T cy() const { return a.pf.cE[1][j]; }
+};
+
+// The psi of one E cell loaded ahead of their use (only those of the slabs
+// the cell lies in are meaningful).
+template <typename T>
+struct PsiSetE {
+    T xy, xz, yx, yz, zx, zy;
+};
+
+// Boundary data of an E cell from registers: psi loaded ahead (written
+// through on update) and the lane's x and y profile values.
+template <typename S, typename T>
+struct PrefPsiE {
+    const EArgs<S, T> &a;
+    int i, j, k;
+    PsiSetE<T> &q;
+    T bx_, cx_, by_, cy_;
+    __device__ __forceinline__ PsiRef<T> Exy(int n) const { return {q.xy, PsiAddrE<S, T>{a, i, j, k}.xy(n)}; }
+    __device__ __forceinline__ PsiRef<T> Exz(int n) const { return {q.xz, PsiAddrE<S, T>{a, i, j, k}.xz(n)}; }
+    __device__ __forceinline__ PsiRef<T> Eyx(int n) const { return {q.yx, PsiAddrE<S, T>{a, i, j, k}.yx(n)}; }
+    __device__ __forceinline__ PsiRef<T> Eyz(int n) const { return {q.yz, PsiAddrE<S, T>{a, i, j, k}.yz(n)}; }
+    __device__ __forceinline__ PsiRef<T> Ezx(int n) const { return {q.zx, PsiAddrE<S, T>{a, i, j, k}.zx(n)}; }
+    __device__ __forceinline__ PsiRef<T> Ezy(int n) const { return {q.zy, PsiAddrE<S, T>{a, i, j, k}.zy(n)}; }
+    __device__ __forceinline__ T bx() const { return bx_; }
+    __device__ __forceinline__ T cx() const { return cx_; }
+    __device__ __forceinline__ T by() const { return by_; }
+    __device__ __forceinline__ T cy() const { return cy_; }
 };
 
 template <typename T>
@@ -153,26 +277,26 @@ struct V3 {
 template <typename S, typename T, class P>
 __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T> &M, int i, int j, int k, int id,
                                              T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x,
-                                             T dHx_y, T ikx, T iky, int nx_, int ny_, int ostep, P ps) {
-    const T ikz = a.pf.ikE[2][k];
+                                             T dHx_y, T ikx, T iky, T ikz, int nx_, int ny_, int ostep, int side,
+                                             P ps) {
     // stencil (core/fdtd_core.cpp:414-415, 528-529, 658-659)
     T ex = (a.en & 1) ? M.C1 * ex0 + M.C2y * iky * dHz_y - M.C2z * ikz * dHy_z : ex0;
     T ey = (a.en & 2) ? M.C1 * ey0 + M.C2z * ikz * dHx_z - M.C2x * ikx * dHz_x : ey0;
     T ez = (a.en & 4) ? M.C1 * ez0 + M.C2x * ikx * dHy_x - M.C2y * iky * dHx_y : ez0;
     // TFSF-E (modes/single_particle.cpp:286-332): z faces, then y, then x
-    if (a.tfsf) {
+    if (side & SIDE_TFSF) {
         const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
         if (in_i && in_j) {
-            if (k == a.tb4) { ex += M.C2z * a.aHy[a.tb4 - 1]; ey -= M.C2z * a.aHx[a.tb4 - 1]; }
-            if (k == a.tb5) { ex -= M.C2z * a.aHy[a.tb5 - 1]; ey += M.C2z * a.aHx[a.tb5 - 1]; }
+            if (k == a.tb4) { ex += M.C2z * uload(a.aHy, a.tb4 - 1); ey -= M.C2z * uload(a.aHx, a.tb4 - 1); }
+            if (k == a.tb5) { ex -= M.C2z * uload(a.aHy, a.tb5 - 1); ey += M.C2z * uload(a.aHx, a.tb5 - 1); }
         }
         if (!a.tper_y && in_i && in_k) {
-            if (j == a.tb2) { ex -= M.C2y * a.aHz[k]; ez += M.C2y * a.aHx[k]; }
-            if (j == a.tb3) { ex += M.C2y * a.aHz[k]; ez -= M.C2y * a.aHx[k]; }
+            if (j == a.tb2) { ex -= M.C2y * uload(a.aHz, k); ez += M.C2y * uload(a.aHx, k); }
+            if (j == a.tb3) { ex += M.C2y * uload(a.aHz, k); ez -= M.C2y * uload(a.aHx, k); }
         }
         if (!a.tper_x && in_k && in_j) {
-            if (i == a.tb0) { ey += M.C2x * a.aHz[k]; ez -= M.C2x * a.aHy[k]; }
-            if (i == a.tb1) { ey -= M.C2x * a.aHz[k]; ez += M.C2x * a.aHy[k]; }
+            if (i == a.tb0) { ey += M.C2x * uload(a.aHz, k); ez -= M.C2x * uload(a.aHy, k); }
+            if (i == a.tb1) { ey -= M.C2x * uload(a.aHz, k); ez += M.C2x * uload(a.aHy, k); }
         }
     }
     // RC-apply: E += C3 sum_p Re psi (core/mats_RC.cpp:38-50); the sum was
@@ -184,25 +308,21 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
         ez += cr[2];
     }
     // CPML-E + PEC walls (core/fdtd_pml.cpp:504-735)
-    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
-    int nz_ = -1;
-    if (pz) {
-        if (k >= 1 && k < a.pml[4]) nz_ = k;
-        else if (k >= a.Nz - a.pml[5] + 1) nz_ = k - (a.Nz - a.pml[5]) + a.pml[4];
-    }
+    const bool px = side & SIDE_X, py = side & SIDE_Y, pz = side & SIDE_Z;
+    const int nz_ = pz ? slabEz(a, k) : -1;
     if (a.en & 1) {
         if (py) {
             if (ny_ >= 0) {
-                T &q = ps.Exy(ny_);
-                q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHz_y;
+                auto &&q = ps.Exy(ny_);
+                q = ps.by() * q + ps.cy() * dHz_y;
                 ex += M.C4 * q;
             }
             if (j == 0) ex = 0;
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = ps.Exz(nz_);
-                q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHy_z;
+                auto &&q = ps.Exz(nz_);
+                q = uload(a.pf.bE[2], k) * q + uload(a.pf.cE[2], k) * dHy_z;
                 ex -= M.C4 * q;
             }
             if (k == 0) ex = 0;
@@ -211,16 +331,16 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
     if (a.en & 2) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = ps.Eyx(nx_);
-                q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHz_x;
+                auto &&q = ps.Eyx(nx_);
+                q = ps.bx() * q + ps.cx() * dHz_x;
                 ey -= M.C4 * q;
             }
             if (i == 0) ey = 0;
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = ps.Eyz(nz_);
-                q = a.pf.bE[2][k] * q + a.pf.cE[2][k] * dHx_z;
+                auto &&q = ps.Eyz(nz_);
+                q = uload(a.pf.bE[2], k) * q + uload(a.pf.cE[2], k) * dHx_z;
                 ey += M.C4 * q;
             }
             if (k == 0) ey = 0;
@@ -229,16 +349,16 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
     if (a.en & 4) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = ps.Ezx(nx_);
-                q = a.pf.bE[0][i] * q + a.pf.cE[0][i] * dHy_x;
+                auto &&q = ps.Ezx(nx_);
+                q = ps.bx() * q + ps.cx() * dHy_x;
                 ez += M.C4 * q;
             }
             if (i == 0) ez = 0;
         }
         if (py) {
             if (ny_ >= 0) {
-                T &q = ps.Ezy(ny_);
-                q = a.pf.bE[1][j] * q + a.pf.cE[1][j] * dHx_y;
+                auto &&q = ps.Ezy(ny_);
+                q = ps.by() * q + ps.cy() * dHx_y;
                 ez -= M.C4 * q;
             }
             if (j == 0) ez = 0;
@@ -250,7 +370,7 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
     for (int q = 0; q < a.nosc; q++) {
         const OscDev &o = a.osc[q];
         if (i >= o.x1 && i < o.x2 && j >= o.y1 && j < o.y2 && k >= o.z1 && k < o.z2) {
-            const T v = (T)o.val[ostep - o.step0];
+            const T v = (T)uload(o.val, ostep - o.step0);
             if (o.comp == 0) ex += v;
             else if (o.comp == 1) ey += v;
             else ez += v;
@@ -259,16 +379,17 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
     return V3<T>{ex, ey, ez};
 }
 
-// Everything after the loads of one E cell, with the grid's CPML state.
-template <typename S, typename T>
+// Everything after the loads of one E cell, with the grid's CPML state; the
+// components in store mask `smask` (bit c: component c) are written back.
+template <typename S, typename T, class P>
 __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
                                        T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
-                                       T ikx, T iky, int nx_, int ny_, int ostep) {
+                                       T ikx, T iky, T ikz, int nx_, int ny_, int ostep, int side, int smask, P ps) {
     const V3<T> e = e_cell_vals(a, M, i, j, k, id, ex0, ey0, ez0, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky,
-                                nx_, ny_, ostep, GlobalPsiE<S, T>{a, i, j, k});
-    if (a.lm & 1) a.Ex.set(id, e.x);
-    if (a.lm & 2) a.Ey.set(id, e.y);
-    if (a.lm & 4) a.Ez.set(id, e.z);
+                                ikz, nx_, ny_, ostep, side, ps);
+    if (smask & 1) a.Ex.set(id, e.x);
+    if (smask & 2) a.Ey.set(id, e.y);
+    if (smask & 4) a.Ez.set(id, e.z);
 }
 
 // CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
@@ -338,6 +459,15 @@ __device__ __forceinline__ void stage_mats(const EArgs<S, T> &a, MatL<T> *sm) {
 }
 
 // 3D E pass over tile w of the grid.
+// Every component is loaded and stored here, whatever the load mask says: a
+// component outside the mask is either never written (it keeps the value it
+// is loaded with) or only differenced along a one-cell axis, where the
+// periodic difference of a finite value is exactly zero, so the results are
+// those of the masked loads.  Unconditional loads keep the number of loads
+// per plane fixed, which lets the compiler wait for exactly one plane while
+// the next plane's loads stay in flight (software pipeline: plane k+1 is
+// requested before plane k is computed and stored; the last plane of the
+// tile is peeled off so that the loop body never skips its loads).
 template <typename S, typename T, int BY, int KZ>
 __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, int w, int ostep) {
     int bx, by, bz;
@@ -352,51 +482,76 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
     const int oim = (i == 0) ? a.Nx - 1 : -1;                    // (i-1) with wrap
     const int ojm = (j == 0) ? (a.Ny - 1) * a.Nx : -a.Nx;        // (j-1) with wrap
     const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
-    // components outside the load mask are identically zero (disabled fields,
-    // or differences along a one-cell axis) and are neither loaded nor stored
-    const int lm = a.lm;
-    const T zero = 0;
-    T hx_km = (lm & 8) ? a.Hx[base + km0 * NxNy] : zero;
-    T hy_km = (lm & 16) ? a.Hy[base + km0 * NxNy] : zero;
+    T hx_km = a.Hx[base + km0 * NxNy];
+    T hy_km = a.Hy[base + km0 * NxNy];
     const T ikx = a.pf.ikE[0][i], iky = a.pf.ikE[1][j];
     const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
-    // software pipeline: the loads of plane k+1 are issued before plane k is
-    // computed and stored, so each lane keeps two planes of loads in flight
+    const int side = side_block(a, 0, bx * BX, min(bx * BX + BX, a.Nx), by * BY, min(by * BY + BY, a.Ny), k0, k1);
+    // the lane's x and y CPML profile values (loop invariant)
+    T bx_ = 0, cx_ = 0, by_ = 0, cy_ = 0;
+    if ((side & SIDE_X) && nx_ >= 0) { bx_ = a.pf.bE[0][i]; cx_ = a.pf.cE[0][i]; }
+    if ((side & SIDE_Y) && ny_ >= 0) { by_ = a.pf.bE[1][j]; cy_ = a.pf.cE[1][j]; }
     struct Ld {
         T hx, hy, hz, hz_jm, hz_im, hy_im, hx_jm, ex, ey, ez;
         int m;
     };
-    auto load = [&](int id) {
+    auto load = [&](int k) {
+        const int id = base + k * NxNy;
         Ld v;
-        v.hx = zero; v.hx_jm = zero; v.hy = zero; v.hy_im = zero; v.hz = zero; v.hz_jm = zero; v.hz_im = zero;
-        v.ex = zero; v.ey = zero; v.ez = zero;
-        if (lm & 8) { v.hx = a.Hx[id]; v.hx_jm = a.Hx[id + ojm]; }
-        if (lm & 16) { v.hy = a.Hy[id]; v.hy_im = a.Hy[id + oim]; }
-        if (lm & 32) { v.hz = a.Hz[id]; v.hz_jm = a.Hz[id + ojm]; v.hz_im = a.Hz[id + oim]; }
-        if (lm & 1) v.ex = a.Ex[id];
-        if (lm & 2) v.ey = a.Ey[id];
-        if (lm & 4) v.ez = a.Ez[id];
+        v.hx = a.Hx[id];
+        v.hx_jm = a.Hx[id + ojm];
+        v.hy = a.Hy[id];
+        v.hy_im = a.Hy[id + oim];
+        v.hz = a.Hz[id];
+        v.hz_jm = a.Hz[id + ojm];
+        v.hz_im = a.Hz[id + oim];
+        v.ex = a.Ex[id];
+        v.ey = a.Ey[id];
+        v.ez = a.Ez[id];
         v.m = a.mat[id];
         return v;
     };
-    Ld cur = load(base + k0 * NxNy);
-    for (int k = k0; k < k1; k++) {
-        const int id = base + k * NxNy;
-        Ld nxt = cur;
-        if (k + 1 < k1) nxt = load(id + NxNy);
-        const T dHz_y = cur.hz - cur.hz_jm;
-        const T dHz_x = cur.hz - cur.hz_im;
-        const T dHy_x = cur.hy - cur.hy_im;
-        const T dHx_y = cur.hx - cur.hx_jm;
-        const T dHy_z = cur.hy - hy_km;
-        const T dHx_z = cur.hx - hx_km;
-        hx_km = cur.hx;
-        hy_km = cur.hy;
-        e_cell(a, sm[cur.m], cur.m, i, j, k, id, cur.ex, cur.ey, cur.ez, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y,
-               ikx, iky, nx_, ny_, ostep);
+    // The psi of plane k are requested before the loads of plane k+1, so
+    // that waiting for them leaves the next plane in flight.
+    auto load_psi = [&](int k) {
+        PsiSetE<T> q{};
+        const PsiAddrE<S, T> ad{a, i, j, k};
+        if ((side & SIDE_X) && nx_ >= 0) {
+            if (a.en & 2) q.yx = *ad.yx(nx_);
+            if (a.en & 4) q.zx = *ad.zx(nx_);
+        }
+        if ((side & SIDE_Y) && ny_ >= 0) {
+            if (a.en & 1) q.xy = *ad.xy(ny_);
+            if (a.en & 4) q.zy = *ad.zy(ny_);
+        }
+        if (side & SIDE_Z) {
+            const int nz_ = slabEz(a, k);
+            if (nz_ >= 0) {
+                if (a.en & 1) q.xz = *ad.xz(nz_);
+                if (a.en & 2) q.yz = *ad.yz(nz_);
+            }
+        }
+        return q;
+    };
+    auto cell = [&](const Ld &c, PsiSetE<T> &q, int k) {
+        const T dHy_z = c.hy - hy_km;
+        const T dHx_z = c.hx - hx_km;
+        hx_km = c.hx;
+        hy_km = c.hy;
+        e_cell(a, sm[c.m], c.m, i, j, k, base + k * NxNy, c.ex, c.ey, c.ez, c.hz - c.hz_jm, dHy_z, dHx_z,
+               c.hz - c.hz_im, c.hy - c.hy_im, c.hx - c.hx_jm, ikx, iky, uload(a.pf.ikE[2], k), nx_, ny_, ostep, side,
+               7, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+    };
+    Ld cur = load(k0);
+    for (int k = k0; k < k1 - 1; k++) {
+        PsiSetE<T> q = load_psi(k);
+        const Ld nxt = load(k + 1);
+        cell(cur, q, k);
         cur = nxt;
     }
+    PsiSetE<T> q = load_psi(k1 - 1);
+    cell(cur, q, k1 - 1);
 }
 
 // 2D E pass (Nz == 1) over tile w: each lane marches KY consecutive rows in y
@@ -419,6 +574,7 @@ __device__ __forceinline__ void e_tile2d(const EArgs<S, T> &a, const MatL<T> *sm
     T hz_jm = (lm & 32) ? a.Hz[i + a.Nx * jm0] : zero;
     const T ikx = a.pf.ikE[0][i];
     const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
+    const T ikz = a.pf.ikE[2][0];
     struct Ld {
         T hx, hy, hz, hz_im, hy_im, ex, ey, ez;
         int m;
@@ -444,7 +600,8 @@ __device__ __forceinline__ void e_tile2d(const EArgs<S, T> &a, const MatL<T> *sm
         const T iky = a.pf.ikE[1][j];
         const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
         e_cell(a, sm[cur.m], cur.m, i, j, 0, id, cur.ex, cur.ey, cur.ez, cur.hz - hz_jm, zero, zero, cur.hz - cur.hz_im,
-               cur.hy - cur.hy_im, cur.hx - hx_jm, ikx, iky, nx_, ny_, ostep);
+               cur.hy - cur.hy_im, cur.hx - hx_jm, ikx, iky, ikz, nx_, ny_, ostep, side_all(a), lm,
+               GlobalPsiE<S, T>{a, i, j, 0});
         hx_jm = cur.hx;
         hz_jm = cur.hz;
         cur = nxt;
@@ -515,48 +672,88 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
     const int m = a.mat[k];
     const T zero = 0;
     e_cell(a, sm[m], m, 0, 0, k, k, a.Ex[k], a.Ey[k], a.Ez[k], zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero,
-           zero, a.pf.ikE[0][0], a.pf.ikE[1][0], -1, -1, a.ostep);
+           zero, a.pf.ikE[0][0], a.pf.ikE[1][0], a.pf.ikE[2][k], -1, -1, a.ostep, side_all(a), a.lm,
+           GlobalPsiE<S, T>{a, 0, 0, k});
 }
 
 // ---------------------------------------------------------------- H cell
+// CPML psi of an H cell in the grid's slab arrays (layout as for E).
+template <typename S, typename T>
+struct PsiAddrH {
+    const HArgs<S, T> &a;
+    int i, j, k;
+    __device__ __forceinline__ T *xy(int n) const { return a.pHxy + i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k); }
+    __device__ __forceinline__ T *xz(int n) const { return a.pHxz + i + a.Nx * j + a.Nx * a.Ny * n; }
+    __device__ __forceinline__ T *yx(int n) const { return a.pHyx + n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k); }
+    __device__ __forceinline__ T *yz(int n) const { return a.pHyz + i + a.Nx * j + a.Nx * a.Ny * n; }
+    __device__ __forceinline__ T *zx(int n) const { return a.pHzx + n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k); }
+    __device__ __forceinline__ T *zy(int n) const { return a.pHzy + i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k); }
+};
+
 template <typename S, typename T>
 struct GlobalPsiH {
     const HArgs<S, T> &a;
     int i, j, k;
-    __device__ __forceinline__ T &Hxy(int n) const { return a.pHxy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
-    __device__ __forceinline__ T &Hxz(int n) const { return a.pHxz[i + a.Nx * j + a.Nx * a.Ny * n]; }
-    __device__ __forceinline__ T &Hyx(int n) const { return a.pHyx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
-    __device__ __forceinline__ T &Hyz(int n) const { return a.pHyz[i + a.Nx * j + a.Nx * a.Ny * n]; }
-    __device__ __forceinline__ T &Hzx(int n) const { return a.pHzx[n + (a.pml[0] + a.pml[1]) * (j + a.Ny * k)]; }
-    __device__ __forceinline__ T &Hzy(int n) const { return a.pHzy[i + a.Nx * (n + (a.pml[2] + a.pml[3]) * k)]; }
+    __device__ __forceinline__ T &Hxy(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.xy(n); }
+    __device__ __forceinline__ T &Hxz(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.xz(n); }
+    __device__ __forceinline__ T &Hyx(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.yx(n); }
+    __device__ __forceinline__ T &Hyz(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.yz(n); }
+    __device__ __forceinline__ T &Hzx(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.zx(n); }
+    __device__ __forceinline__ T &Hzy(int n) const { return *PsiAddrH<S, T>{a, i, j, k}.zy(n); }
+    __device__ __forceinline__ T bx() const { return a.pf.bH[0][i]; }
+    __device__ __forceinline__ T cx() const { return a.pf.cH[0][i]; }
+    __device__ __forceinline__ T by() const { return a.pf.bH[1][j]; }
+    __device__ __forceinline__ T cy() const { return a.pf.cH[1][j]; }
+};
+
+template <typename T>
+struct PsiSetH {
+    T xy, xz, yx, yz, zx, zy;
+};
+
+template <typename S, typename T>
+struct PrefPsiH {
+    const HArgs<S, T> &a;
+    int i, j, k;
+    PsiSetH<T> &q;
+    T bx_, cx_, by_, cy_;
+    __device__ __forceinline__ PsiRef<T> Hxy(int n) const { return {q.xy, PsiAddrH<S, T>{a, i, j, k}.xy(n)}; }
+    __device__ __forceinline__ PsiRef<T> Hxz(int n) const { return {q.xz, PsiAddrH<S, T>{a, i, j, k}.xz(n)}; }
+    __device__ __forceinline__ PsiRef<T> Hyx(int n) const { return {q.yx, PsiAddrH<S, T>{a, i, j, k}.yx(n)}; }
+    __device__ __forceinline__ PsiRef<T> Hyz(int n) const { return {q.yz, PsiAddrH<S, T>{a, i, j, k}.yz(n)}; }
+    __device__ __forceinline__ PsiRef<T> Hzx(int n) const { return {q.zx, PsiAddrH<S, T>{a, i, j, k}.zx(n)}; }
+    __device__ __forceinline__ PsiRef<T> Hzy(int n) const { return {q.zy, PsiAddrH<S, T>{a, i, j, k}.zy(n)}; }
+    __device__ __forceinline__ T bx() const { return bx_; }
+    __device__ __forceinline__ T cx() const { return cx_; }
+    __device__ __forceinline__ T by() const { return by_; }
+    __device__ __forceinline__ T cy() const { return cy_; }
 };
 
 // The new H of one cell (in place in hx, hy, hz) from the curl differences dE*.
 template <typename S, typename T, class P>
 __device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, int k, T &hx, T &hy, T &hz, T dEy_z,
-                                            T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_,
-                                            P ps) {
-    const T ikz = a.pf.ikH[2][k];
+                                            T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, T ikz, int nx_,
+                                            int ny_, int side, P ps) {
     // stencil (core/fdtd_core.cpp:756-757, 852-853, 964-965)
     if (a.en & 8) hx += a.dtdmz * ikz * dEy_z - a.dtdmy * iky * dEz_y;
     if (a.en & 16) hy += a.dtdmx * ikx * dEz_x - a.dtdmz * ikz * dEx_z;
     if (a.en & 32) hz += a.dtdmy * iky * dEx_y - a.dtdmx * ikx * dEy_x;
     // CPML-H + PEC walls (core/fdtd_pml.cpp:737-929)
-    const bool px = a.pml[0] || a.pml[1], py = a.pml[2] || a.pml[3], pz = a.pml[4] || a.pml[5];
-    const int nz_ = slabH(k, a.Nz, a.pml[4], a.pml[5]);
+    const bool px = side & SIDE_X, py = side & SIDE_Y, pz = side & SIDE_Z;
+    const int nz_ = pz ? slabH(k, a.Nz, a.pml[4], a.pml[5]) : -1;
     if (a.en & 8) {
         if (py) {
             if (ny_ >= 0) {
-                T &q = ps.Hxy(ny_);
-                q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEz_y;
+                auto &&q = ps.Hxy(ny_);
+                q = ps.by() * q + ps.cy() * dEz_y;
                 hx -= a.dtm * q;
             }
             if (j == a.Ny - 1) hx = 0;
         }
         if (pz) {
             if (nz_ >= 0) {
-                T &q = ps.Hxz(nz_);
-                q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEy_z;
+                auto &&q = ps.Hxz(nz_);
+                q = uload(a.pf.bH[2], k) * q + uload(a.pf.cH[2], k) * dEy_z;
                 hx += a.dtm * q;
             }
             if (k == a.Nz - 1) hx = 0;
@@ -565,15 +762,15 @@ __device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, 
     if (a.en & 16) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = ps.Hyx(nx_);
-                q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEz_x;
+                auto &&q = ps.Hyx(nx_);
+                q = ps.bx() * q + ps.cx() * dEz_x;
                 hy += a.dtm * q;
             }
             if (i == a.Nx - 1) hy = 0;
         }
         if (pz && nz_ >= 0) {
-            T &q = ps.Hyz(nz_);
-            q = a.pf.bH[2][k] * q + a.pf.cH[2][k] * dEx_z;
+            auto &&q = ps.Hyz(nz_);
+            q = uload(a.pf.bH[2], k) * q + uload(a.pf.cH[2], k) * dEx_z;
             hy -= a.dtm * q;
         }
     }
@@ -582,16 +779,16 @@ __device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, 
     if (a.en & 32) {
         if (px) {
             if (nx_ >= 0) {
-                T &q = ps.Hzx(nx_);
-                q = a.pf.bH[0][i] * q + a.pf.cH[0][i] * dEy_x;
+                auto &&q = ps.Hzx(nx_);
+                q = ps.bx() * q + ps.cx() * dEy_x;
                 hz -= a.dtm * q;
             }
             if (i == a.Nx - 1) hz = 0;
         }
         if (py) {
             if (ny_ >= 0) {
-                T &q = ps.Hzy(ny_);
-                q = a.pf.bH[1][j] * q + a.pf.cH[1][j] * dEx_y;
+                auto &&q = ps.Hzy(ny_);
+                q = ps.by() * q + ps.cy() * dEx_y;
                 hz += a.dtm * q;
             }
             if (j == a.Ny - 1) hz = 0;
@@ -600,33 +797,38 @@ __device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, 
     // plane injection (modes/normal_incidence.cpp:353-357)
     if (k == a.inj_k) { hx += a.injx; hy += a.injy; }
     // TFSF-H (modes/single_particle.cpp:351-378): z faces, then y, then x
-    if (a.tfsf) {
+    if (side & SIDE_TFSF) {
         const bool in_i = i >= a.tb0 && i < a.tb1, in_j = j >= a.tb2 && j < a.tb3, in_k = k >= a.tb4 && k < a.tb5;
         if (in_i && in_j) {
-            if (k == a.tb4 - 1) { hx -= a.dtdmz * a.aEy[a.tb4]; hy += a.dtdmz * a.aEx[a.tb4]; }
-            if (k == a.tb5 - 1) { hx += a.dtdmz * a.aEy[a.tb5]; hy -= a.dtdmz * a.aEx[a.tb5]; }
+            if (k == a.tb4 - 1) { hx -= a.dtdmz * uload(a.aEy, a.tb4); hy += a.dtdmz * uload(a.aEx, a.tb4); }
+            if (k == a.tb5 - 1) { hx += a.dtdmz * uload(a.aEy, a.tb5); hy -= a.dtdmz * uload(a.aEx, a.tb5); }
         }
         if (!a.tper_y && in_i && in_k) {
-            if (j == a.tb2 - 1) { hx += a.dtdmy * a.aEz[k]; hz -= a.dtdmy * a.aEx[k]; }
-            if (j == a.tb3 - 1) { hx -= a.dtdmy * a.aEz[k]; hz += a.dtdmy * a.aEx[k]; }
+            if (j == a.tb2 - 1) { hx += a.dtdmy * uload(a.aEz, k); hz -= a.dtdmy * uload(a.aEx, k); }
+            if (j == a.tb3 - 1) { hx -= a.dtdmy * uload(a.aEz, k); hz += a.dtdmy * uload(a.aEx, k); }
         }
         if (!a.tper_x && in_k && in_j) {
-            if (i == a.tb0 - 1) { hy -= a.dtdmx * a.aEz[k]; hz += a.dtdmx * a.aEy[k]; }
-            if (i == a.tb1 - 1) { hy += a.dtdmx * a.aEz[k]; hz -= a.dtdmx * a.aEy[k]; }
+            if (i == a.tb0 - 1) { hy -= a.dtdmx * uload(a.aEz, k); hz += a.dtdmx * uload(a.aEy, k); }
+            if (i == a.tb1 - 1) { hy += a.dtdmx * uload(a.aEz, k); hz -= a.dtdmx * uload(a.aEy, k); }
         }
     }
 }
 
-template <typename S, typename T>
+// The new H of one cell written back for the components in store mask
+// `smask` (bit 3 + c: component c).
+template <typename S, typename T, class P>
 __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
-                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, int nx_, int ny_) {
-    h_cell_vals(a, i, j, k, hx, hy, hz, dEy_z, dEx_z, dEz_y, dEz_x, dEx_y, dEy_x, ikx, iky, nx_, ny_,
-                GlobalPsiH<S, T>{a, i, j, k});
-    if (a.lm & 8) a.Hx.set(id, hx);
-    if (a.lm & 16) a.Hy.set(id, hy);
-    if (a.lm & 32) a.Hz.set(id, hz);
+                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, T ikz, int nx_,
+                                       int ny_, int side, int smask, P ps) {
+    h_cell_vals(a, i, j, k, hx, hy, hz, dEy_z, dEx_z, dEz_y, dEz_x, dEx_y, dEy_x, ikx, iky, ikz, nx_, ny_, side, ps);
+    if (smask & 8) a.Hx.set(id, hx);
+    if (smask & 16) a.Hy.set(id, hy);
+    if (smask & 32) a.Hz.set(id, hz);
 }
 
+// 3D H pass over tile w of the grid: the mirror of e_tile (every component
+// loaded and stored, plane k+1 requested before plane k is computed, last
+// plane peeled).  E(k+1) of plane k is E(k) of plane k+1 and is carried.
 template <typename S, typename T, int BY, int KZ>
 __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     int bx, by, bz;
@@ -643,27 +845,68 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
-    const int lm = a.lm;
-    const T zero = 0;
-    // E(k+1) of plane k is E(k) of plane k+1: carried in registers
-    T ex_k = (lm & 1) ? a.Ex[base + k0 * NxNy] : zero;
-    T ey_k = (lm & 2) ? a.Ey[base + k0 * NxNy] : zero;
-    for (int k = k0; k < k1; k++) {
+    const int side = side_block(a, 1, bx * BX, min(bx * BX + BX, a.Nx), by * BY, min(by * BY + BY, a.Ny), k0, k1);
+    T bx_ = 0, cx_ = 0, by_ = 0, cy_ = 0;
+    if ((side & SIDE_X) && nx_ >= 0) { bx_ = a.pf.bH[0][i]; cx_ = a.pf.cH[0][i]; }
+    if ((side & SIDE_Y) && ny_ >= 0) { by_ = a.pf.bH[1][j]; cy_ = a.pf.cH[1][j]; }
+    T ex_k = a.Ex[base + k0 * NxNy];
+    T ey_k = a.Ey[base + k0 * NxNy];
+    struct Ld {
+        T ex_kp, ey_kp, ex_jp, ey_ip, ez, ez_jp, ez_ip, hx, hy, hz;
+    };
+    auto load = [&](int k) {
         const int id = base + k * NxNy;
-        const int kp = (k == a.Nz - 1) ? 0 : k + 1;
+        const int idkp = base + ((k == a.Nz - 1) ? 0 : k + 1) * NxNy;
+        Ld v;
+        v.ex_kp = a.Ex[idkp];
+        v.ey_kp = a.Ey[idkp];
+        v.ex_jp = a.Ex[id + ojp];
+        v.ey_ip = a.Ey[id + oip];
+        v.ez = a.Ez[id];
+        v.ez_jp = a.Ez[id + ojp];
+        v.ez_ip = a.Ez[id + oip];
+        v.hx = a.Hx[id];
+        v.hy = a.Hy[id];
+        v.hz = a.Hz[id];
+        return v;
+    };
+    auto load_psi = [&](int k) {
+        PsiSetH<T> q{};
+        const PsiAddrH<S, T> ad{a, i, j, k};
+        if ((side & SIDE_X) && nx_ >= 0) {
+            if (a.en & 16) q.yx = *ad.yx(nx_);
+            if (a.en & 32) q.zx = *ad.zx(nx_);
+        }
+        if ((side & SIDE_Y) && ny_ >= 0) {
+            if (a.en & 8) q.xy = *ad.xy(ny_);
+            if (a.en & 32) q.zy = *ad.zy(ny_);
+        }
+        if (side & SIDE_Z) {
+            const int nz_ = slabH(k, a.Nz, a.pml[4], a.pml[5]);
+            if (nz_ >= 0) {
+                if (a.en & 8) q.xz = *ad.xz(nz_);
+                if (a.en & 16) q.yz = *ad.yz(nz_);
+            }
+        }
+        return q;
+    };
+    auto cell = [&](const Ld &c, PsiSetH<T> &q, int k) {
         const T ex = ex_k, ey = ey_k;
-        T ex_kp = zero, ey_kp = zero, ez = zero, ez_jp = zero, ez_ip = zero, ex_jp = zero, ey_ip = zero;
-        if (lm & 1) { ex_kp = a.Ex[base + kp * NxNy]; ex_jp = a.Ex[id + ojp]; }
-        if (lm & 2) { ey_kp = a.Ey[base + kp * NxNy]; ey_ip = a.Ey[id + oip]; }
-        if (lm & 4) { ez = a.Ez[id]; ez_jp = a.Ez[id + ojp]; ez_ip = a.Ez[id + oip]; }
-        ex_k = ex_kp;
-        ey_k = ey_kp;
-        const T hx = (lm & 8) ? a.Hx[id] : zero;
-        const T hy = (lm & 16) ? a.Hy[id] : zero;
-        const T hz = (lm & 32) ? a.Hz[id] : zero;
-        h_cell(a, i, j, k, id, hx, hy, hz, ey_kp - ey, ex_kp - ex, ez_jp - ez, ez_ip - ez, ex_jp - ex, ey_ip - ey, ikx,
-               iky, nx_, ny_);
+        ex_k = c.ex_kp;
+        ey_k = c.ey_kp;
+        h_cell(a, i, j, k, base + k * NxNy, c.hx, c.hy, c.hz, c.ey_kp - ey, c.ex_kp - ex, c.ez_jp - c.ez,
+               c.ez_ip - c.ez, c.ex_jp - ex, c.ey_ip - ey, ikx, iky, uload(a.pf.ikH[2], k), nx_, ny_, side,
+               8 | 16 | 32, PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+    };
+    Ld cur = load(k0);
+    for (int k = k0; k < k1 - 1; k++) {
+        PsiSetH<T> q = load_psi(k);
+        const Ld nxt = load(k + 1);
+        cell(cur, q, k);
+        cur = nxt;
     }
+    PsiSetH<T> q = load_psi(k1 - 1);
+    cell(cur, q, k1 - 1);
 }
 
 // 2D H pass (Nz == 1): rows ascending; E(j+1) of row j is E(j) of row j+1.
@@ -678,6 +921,7 @@ __device__ __forceinline__ void h_tile2d(const HArgs<S, T> &a, int w) {
     const int oip = (i == a.Nx - 1) ? -(a.Nx - 1) : 1;
     const T ikx = a.pf.ikH[0][i];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
+    const T ikz = a.pf.ikH[2][0];
     const int lm = a.lm;
     const T zero = 0;
     T ex_j = (lm & 1) ? a.Ex[i + a.Nx * j0] : zero;
@@ -695,7 +939,7 @@ __device__ __forceinline__ void h_tile2d(const HArgs<S, T> &a, int w) {
         const T iky = a.pf.ikH[1][j];
         const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
         h_cell(a, i, j, 0, id, hx, hy, hz, zero, zero, ez_jp - ez_j, ez_ip - ez_j, ex_jp - ex_j, ey_ip - ey, ikx, iky,
-               nx_, ny_);
+               ikz, nx_, ny_, side_all(a), lm, GlobalPsiH<S, T>{a, i, j, 0});
         ex_j = ex_jp;
         ez_j = ez_jp;
     }
@@ -731,7 +975,7 @@ __global__ __launch_bounds__(256) void k_update_H_col(const HArgs<S, T> a) {
     const int kp = (k == a.Nz - 1) ? 0 : k + 1;
     const T zero = 0;
     h_cell(a, 0, 0, k, k, a.Hx[k], a.Hy[k], a.Hz[k], a.Ey[kp] - a.Ey[k], a.Ex[kp] - a.Ex[k], zero, zero, zero,
-           zero, a.pf.ikH[0][0], a.pf.ikH[1][0], -1, -1);
+           zero, a.pf.ikH[0][0], a.pf.ikH[1][0], a.pf.ikH[2][k], -1, -1, side_all(a), a.lm, GlobalPsiH<S, T>{a, 0, 0, k});
 }
 
 // ================================================================ sensors
