@@ -512,8 +512,8 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
         v.m = a.mat[id];
         return v;
     };
-    // The psi of plane k are requested before the loads of plane k+1, so
-    // that waiting for them leaves the next plane in flight.
+    // The psi of a plane are requested with the plane's fields, one plane
+    // ahead of their use.
     auto load_psi = [&](int k) {
         PsiSetE<T> q{};
         const PsiAddrE<S, T> ad{a, i, j, k};
@@ -544,13 +544,14 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
                7, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
     };
     Ld cur = load(k0);
+    PsiSetE<T> q = load_psi(k0);
     for (int k = k0; k < k1 - 1; k++) {
-        PsiSetE<T> q = load_psi(k);
+        PsiSetE<T> qn = load_psi(k + 1);
         const Ld nxt = load(k + 1);
         cell(cur, q, k);
         cur = nxt;
+        q = qn;
     }
-    PsiSetE<T> q = load_psi(k1 - 1);
     cell(cur, q, k1 - 1);
 }
 
@@ -899,13 +900,14 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
                8 | 16 | 32, PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
     };
     Ld cur = load(k0);
+    PsiSetH<T> q = load_psi(k0);
     for (int k = k0; k < k1 - 1; k++) {
-        PsiSetH<T> q = load_psi(k);
+        PsiSetH<T> qn = load_psi(k + 1);
         const Ld nxt = load(k + 1);
         cell(cur, q, k);
         cur = nxt;
+        q = qn;
     }
-    PsiSetH<T> q = load_psi(k1 - 1);
     cell(cur, q, k1 - 1);
 }
 
