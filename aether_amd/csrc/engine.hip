@@ -118,6 +118,22 @@ int upload_T(aefdtd_grid *g, void *dst, const double *src, size_t n) {
     return AEFDTD_OK;
 }
 
+// Dense host cell array (x fastest, Nx per row) <-> pitched device cell array.
+hipError_t cells_to_device(aefdtd_grid *g, void *dst, const void *src, size_t es) {
+    const size_t rows = (size_t)g->Ny * g->Nz;
+    hipError_t e = hipMemcpy2DAsync(dst, g->px * es, src, g->Nx * es, g->Nx * es, rows, hipMemcpyHostToDevice,
+                                    g->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(g->stream);
+}
+hipError_t cells_to_host(aefdtd_grid *g, void *dst, const void *src, size_t es) {
+    const size_t rows = (size_t)g->Ny * g->Nz;
+    hipError_t e = hipMemcpy2DAsync(dst, g->Nx * es, src, g->px * es, g->Nx * es, rows, hipMemcpyDeviceToHost,
+                                    g->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(g->stream);
+}
+
 void pml_coeff_calc(aefdtd_grid *g) {          // core/fdtd_pml.cpp:305-343
     int N[3] = {g->Nx, g->Ny, g->Nz};
     for (int a = 0; a < 3; a++) {
@@ -248,16 +264,16 @@ int upload_materials_T(aefdtd_grid *g) {
 // neighbouring lanes touch neighbouring slots.
 int upload_grid(aefdtd_grid *g) {
     if (g->Nmat > 256) return fail(g, AEFDTD_EINVAL, "more than 256 material indices");
-    size_t N = g->ncell();
+    const size_t N = g->ncell(), NA = g->nalloc();
     std::vector<uint8_t> m8(N);
     for (size_t c = 0; c < N; c++) m8[c] = (uint8_t)g->matsgrid[c];
     if (!g->d_mat) {
         void *p;
-        int rc = dalloc(g, &p, N);
+        int rc = dalloc(g, &p, NA);
         if (rc) return rc;
         g->d_mat = (uint8_t *)p;
     }
-    HIPOK(g, hipMemcpyAsync(g->d_mat, m8.data(), N, hipMemcpyHostToDevice, g->stream));
+    HIPOK(g, cells_to_device(g, g->d_mat, m8.data(), 1));
     int rc = with_ctype(g, [&]<typename T>() { return upload_materials_T<T>(g); });
     if (rc) return rc;
     long nd = 0;
@@ -270,17 +286,18 @@ int upload_grid(aefdtd_grid *g) {
     dfree(g->d_corr);
     g->n_disp = 0;
     if (any) {
-        slot.assign(N, -1);
+        slot.assign(NA, -1);
         std::vector<int32_t> cells;
         for (size_t c = 0; c < N; c++)
             if (g->mats[g->matsgrid[c]].rc) {
-                slot[c] = (int32_t)nd++;
-                cells.push_back((int32_t)c);
+                const size_t d = g->dcell(c);
+                slot[d] = (int32_t)nd++;
+                cells.push_back((int32_t)d);
             }
         void *p;
-        if ((rc = dalloc(g, &p, N * sizeof(int32_t)))) return rc;
+        if ((rc = dalloc(g, &p, NA * sizeof(int32_t)))) return rc;
         g->d_slot = (int32_t *)p;
-        HIPOK(g, hipMemcpyAsync(g->d_slot, slot.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        HIPOK(g, hipMemcpyAsync(g->d_slot, slot.data(), NA * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
         if ((rc = dalloc(g, &p, cells.size() * sizeof(int32_t)))) return rc;
         g->d_cells = (int32_t *)p;
         HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
@@ -470,7 +487,7 @@ template <typename S, typename T>
 EArgs<S, T> e_args(aefdtd_grid *g) {
     EArgs<S, T> a;
     std::memset(&a, 0, sizeof(a));
-    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz; a.P = g->px;
     a.nmat = (int)g->Nmat;
     a.Ex = fld<S, T>(g, 0); a.Ey = fld<S, T>(g, 1); a.Ez = fld<S, T>(g, 2);
     a.Hx = fld<S, T>(g, 3); a.Hy = fld<S, T>(g, 4); a.Hz = fld<S, T>(g, 5);
@@ -507,7 +524,7 @@ template <typename S, typename T>
 HArgs<S, T> h_args(aefdtd_grid *g) {
     HArgs<S, T> a;
     std::memset(&a, 0, sizeof(a));
-    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz;
+    a.Nx = g->Nx; a.Ny = g->Ny; a.Nz = g->Nz; a.P = g->px;
     a.Ex = fld<S, T>(g, 0); a.Ey = fld<S, T>(g, 1); a.Ez = fld<S, T>(g, 2);
     a.Hx = fld<S, T>(g, 3); a.Hy = fld<S, T>(g, 4); a.Hz = fld<S, T>(g, 5);
     a.pf = prof_of<T>(g);
@@ -638,7 +655,7 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     int rc0 = ensure_osc_tables(g, g->tstep + 1);
     if (rc0) return rc0;
     std::vector<std::pair<void *, size_t>> state;
-    for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->ncell() * g->ssize()});
+    for (int c = 0; c < 6; c++) state.push_back({g->F[c], g->nalloc() * g->ssize()});
     for (int s = 0; s < 12; s++)
         if (g->d_psipml[s]) state.push_back({g->d_psipml[s], g->psipml_n[s] * g->csize()});
     if (g->d_psi) state.push_back({g->d_psi, (size_t)3 * g->np_max * g->n_disp * sizeof(double2)});
@@ -766,7 +783,8 @@ int feed_T(aefdtd_grid *g) {
         prof_begin(g, 2, &ev);
         hipLaunchKernelGGL((k_sample<S, T>), grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc,
                            g->stage_fill, g->stage_P, fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2),
-                           fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, (T *)g->d_stage);
+                           fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, g->px,
+                           (T *)g->d_stage);
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
         g->stage_fill++;
@@ -779,7 +797,7 @@ int feed_T(aefdtd_grid *g) {
         int t = g->tstep - 1;
         if (t < 0 || t >= g->Nt) continue;
         hipLaunchKernelGGL((k_plane_sum<S, T>), dim3(PS_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0), fld<S, T>(g, 1),
-                           fld<S, T>(g, 2), g->Nx, g->Ny, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
+                           fld<S, T>(g, 2), g->Nx, g->Ny, g->px, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
                            g->Dy, g->Dz, s->zpos, (const double *)s->d_tap, t, g->Dt, s->d_out);
         HIPOK(g, hipGetLastError());
         s->n = std::max(s->n, t + 1);
@@ -889,7 +907,7 @@ int ensure_tb_alt(aefdtd_grid *g, int pol) {
     g->tb_alt_pol = -1;
     const size_t before = g->dev_bytes;
     for (int c = 0; c < 3; c++) {
-        int rc = dalloc(g, &g->tb_alt[c], g->ncell() * g->ssize());
+        int rc = dalloc(g, &g->tb_alt[c], g->nalloc() * g->ssize());
         if (rc) return rc;
     }
     for (int c = 0; c < 4; c++) {
@@ -1046,7 +1064,7 @@ int batch_build(Batch *b) {
                 SampB<S, T> q;
                 q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off};
                 for (int c = 0; c < 6; c++) q.F[c] = fld<S, T>(g, c);
-                q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.member = r;
+                q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.P = g->px; q.member = r;
                 q.Ptot = g->stage_P;
                 q.stage = (T *)g->d_stage;
                 samp.push_back(q);
@@ -1341,7 +1359,10 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
     g->xs_s = pml[0] + g->pad[0]; g->xs_e = g->xs_s + Nx;
     g->ys_s = pml[2] + g->pad[2]; g->ys_e = g->ys_s + Ny;
     g->zs_s = pml[4] + g->pad[4]; g->zs_e = g->zs_s + Nz;
-    if ((size_t)g->Nx * g->Ny * g->Nz >= ((size_t)1 << 31)) {     // kernels index cells with 32-bit offsets
+    // row pitch of the device cell arrays: rows of 3D grids start on 64-cell
+    // (256-byte or wider) boundaries
+    g->px = (g->Nz > 1 && g->Nx > 1) ? (g->Nx + 63) / 64 * 64 : g->Nx;
+    if (g->nalloc() >= ((size_t)1 << 31)) {     // kernels index cells with 32-bit offsets
         delete g;
         return cfail(AEFDTD_EINVAL, "grid too large (2^31 cells or more)");
     }
@@ -1349,8 +1370,8 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
     g->dtm = Dt / kMu0;
     g->dtdex = Dt / (Dx * kE0); g->dtdey = Dt / (Dy * kE0); g->dtdez = Dt / (Dz * kE0);
     g->dtdmx = Dt / (Dx * kMu0); g->dtdmy = Dt / (Dy * kMu0); g->dtdmz = Dt / (Dz * kMu0);
-    size_t N = g->ncell();
-    g->matsgrid.assign(N, 0);
+    size_t N = g->nalloc();
+    g->matsgrid.assign(g->ncell(), 0);
     g->mats.assign(1, HostMat());
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
         delete g;
@@ -1523,7 +1544,7 @@ int aefdtd_update_H(aefdtd_grid *g) {
 
 int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;
-    size_t N = g->ncell();
+    size_t N = g->nalloc();
     for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream));
     for (int s = 0; s < 12; s++)
         if (g->d_psipml[s]) HIPOK(g, hipMemsetAsync(g->d_psipml[s], 0, g->psipml_n[s] * g->csize(), g->stream));
@@ -1538,14 +1559,14 @@ int aefdtd_read_field(aefdtd_grid *g, int f, double *out) {
     size_t N = g->ncell();
     HIPOK(g, hipStreamSynchronize(g->stream));
     if (g->precision == AEFDTD_FP64) {
-        HIPOK(g, hipMemcpy(out, g->F[f], N * 8, hipMemcpyDeviceToHost));
+        HIPOK(g, cells_to_host(g, out, g->F[f], 8));
     } else if (g->precision == AEFDTD_FP32) {
         std::vector<float> t(N);
-        HIPOK(g, hipMemcpy(t.data(), g->F[f], N * 4, hipMemcpyDeviceToHost));
+        HIPOK(g, cells_to_host(g, t.data(), g->F[f], 4));
         for (size_t i = 0; i < N; i++) out[i] = t[i];
     } else {
         std::vector<_Float16> t(N);
-        HIPOK(g, hipMemcpy(t.data(), g->F[f], N * 2, hipMemcpyDeviceToHost));
+        HIPOK(g, cells_to_host(g, t.data(), g->F[f], 2));
         const double inv = std::ldexp(1.0, -g->scale_exp[f]);
         for (size_t i = 0; i < N; i++) out[i] = (double)(float)t[i] * inv;
     }
@@ -1561,12 +1582,17 @@ int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
         std::vector<_Float16> t(N);
         const double sc = std::ldexp(1.0, g->scale_exp[f]);
         for (size_t i = 0; i < N; i++) t[i] = (_Float16)(float)(in[i] * sc);
-        HIPOK(g, hipMemcpy(g->F[f], t.data(), N * 2, hipMemcpyHostToDevice));
+        HIPOK(g, cells_to_device(g, g->F[f], t.data(), 2));
         return AEFDTD_OK;
     }
-    int rc = upload_T(g, g->F[f], in, g->ncell());
-    if (rc) return rc;
-    HIPOK(g, hipStreamSynchronize(g->stream));
+    if (is_fp64(g)) {
+        HIPOK(g, cells_to_device(g, g->F[f], in, 8));
+    } else {
+        const size_t N = g->ncell();
+        std::vector<float> t(N);
+        for (size_t i = 0; i < N; i++) t[i] = (float)in[i];
+        HIPOK(g, cells_to_device(g, g->F[f], t.data(), 4));
+    }
     return AEFDTD_OK;
 }
 

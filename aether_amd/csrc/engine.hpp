@@ -166,6 +166,10 @@ struct aefdtd_grid {
     int scale_exp[6] = {0, 0, 0, 0, 0, 0};
 
     size_t ncell() const { return (size_t)Nx * Ny * Nz; }
+    // device cell arrays: row pitch px >= Nx (3D grids pad rows to 64 cells)
+    int px = 0;
+    size_t nalloc() const { return (size_t)px * Ny * Nz; }
+    size_t dcell(size_t c) const { return c % Nx + (size_t)px * (c / Nx); }   // dense -> device index
     size_t ssize() const { return precision == AEFDTD_FP64 ? 8 : (precision == AEFDTD_FP32 ? 4 : 2); }   // field storage
     size_t csize() const { return precision == AEFDTD_FP64 ? 8 : 4; }   // arithmetic, CPML psi, tables
 };

@@ -66,9 +66,14 @@ struct Prof {
     const T *ikE[3], *ikH[3], *bE[3], *cE[3], *bH[3], *cH[3];
 };
 
+// Cell arrays (fields, material ids, dispersive slots) are stored x fastest
+// with a row pitch P >= Nx: cell (i, j, k) at i + P (j + Ny k).  3D grids pad
+// P to a multiple of 64 cells so that every row starts on a 256-byte boundary
+// and a wavefront's 64-cell row segment is a whole number of cache lines;
+// the pad cells are never read.  The CPML slab arrays are dense in Nx.
 template <typename S, typename T>
 struct EArgs {
-    int Nx, Ny, Nz, nmat;
+    int Nx, Ny, Nz, nmat, P;
     int nbx, nby, nbz, remap;
     Fld<S, T> Ex, Ey, Ez, Hx, Hy, Hz;
     const uint8_t *mat;
@@ -91,7 +96,7 @@ struct EArgs {
 
 template <typename S, typename T>
 struct HArgs {
-    int Nx, Ny, Nz;
+    int Nx, Ny, Nz, P;
     int nbx, nby, nbz, remap;
     Fld<S, T> Hx, Hy, Hz, Ex, Ey, Ez;
     Prof<T> pf;
@@ -477,10 +482,10 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
     if (i >= a.Nx || j >= a.Ny) return;
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.Nz);
-    const int NxNy = a.Nx * a.Ny;
-    const int base = i + a.Nx * j;
+    const int NxNy = a.P * a.Ny;                                 // plane stride
+    const int base = i + a.P * j;
     const int oim = (i == 0) ? a.Nx - 1 : -1;                    // (i-1) with wrap
-    const int ojm = (j == 0) ? (a.Ny - 1) * a.Nx : -a.Nx;        // (j-1) with wrap
+    const int ojm = (j == 0) ? (a.Ny - 1) * a.P : -a.P;          // (j-1) with wrap
     const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
     T hx_km = a.Hx[base + km0 * NxNy];
     T hy_km = a.Hy[base + km0 * NxNy];
@@ -571,8 +576,8 @@ __device__ __forceinline__ void e_tile2d(const EArgs<S, T> &a, const MatL<T> *sm
     const int lm = a.lm;
     const T zero = 0;
     const int jm0 = (j0 == 0) ? a.Ny - 1 : j0 - 1;
-    T hx_jm = (lm & 8) ? a.Hx[i + a.Nx * jm0] : zero;
-    T hz_jm = (lm & 32) ? a.Hz[i + a.Nx * jm0] : zero;
+    T hx_jm = (lm & 8) ? a.Hx[i + a.P * jm0] : zero;
+    T hz_jm = (lm & 32) ? a.Hz[i + a.P * jm0] : zero;
     const T ikx = a.pf.ikE[0][i];
     const int nx_ = slabE(i, a.Nx, a.pml[0], a.pml[1]);
     const T ikz = a.pf.ikE[2][0];
@@ -593,11 +598,11 @@ __device__ __forceinline__ void e_tile2d(const EArgs<S, T> &a, const MatL<T> *sm
         v.m = a.mat[id];
         return v;
     };
-    Ld cur = load(i + a.Nx * j0);
+    Ld cur = load(i + a.P * j0);
     for (int j = j0; j < j1; j++) {
-        const int id = i + a.Nx * j;
+        const int id = i + a.P * j;
         Ld nxt = cur;
-        if (j + 1 < j1) nxt = load(id + a.Nx);
+        if (j + 1 < j1) nxt = load(id + a.P);
         const T iky = a.pf.ikE[1][j];
         const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
         e_cell(a, sm[cur.m], cur.m, i, j, 0, id, cur.ex, cur.ey, cur.ez, cur.hz - hz_jm, zero, zero, cur.hz - cur.hz_im,
@@ -839,10 +844,10 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     if (i >= a.Nx || j >= a.Ny) return;
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.Nz);
-    const int NxNy = a.Nx * a.Ny;
-    const int base = i + a.Nx * j;
+    const int NxNy = a.P * a.Ny;
+    const int base = i + a.P * j;
     const int oip = (i == a.Nx - 1) ? -(a.Nx - 1) : 1;
-    const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.Nx : a.Nx;
+    const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.P : a.P;
     const T ikx = a.pf.ikH[0][i], iky = a.pf.ikH[1][j];
     const int nx_ = slabH(i, a.Nx, a.pml[0], a.pml[1]);
     const int ny_ = slabH(j, a.Ny, a.pml[2], a.pml[3]);
@@ -926,11 +931,11 @@ __device__ __forceinline__ void h_tile2d(const HArgs<S, T> &a, int w) {
     const T ikz = a.pf.ikH[2][0];
     const int lm = a.lm;
     const T zero = 0;
-    T ex_j = (lm & 1) ? a.Ex[i + a.Nx * j0] : zero;
-    T ez_j = (lm & 4) ? a.Ez[i + a.Nx * j0] : zero;
+    T ex_j = (lm & 1) ? a.Ex[i + a.P * j0] : zero;
+    T ez_j = (lm & 4) ? a.Ez[i + a.P * j0] : zero;
     for (int j = j0; j < j1; j++) {
-        const int id = i + a.Nx * j;
-        const int idp = (j == a.Ny - 1) ? i : id + a.Nx;      // (j+1) with wrap
+        const int id = i + a.P * j;
+        const int idp = (j == a.Ny - 1) ? i : id + a.P;      // (j+1) with wrap
         T ex_jp = zero, ez_jp = zero, ey = zero, ey_ip = zero, ez_ip = zero;
         if (lm & 1) ex_jp = a.Ex[idp];
         if (lm & 2) { ey = a.Ey[id]; ey_ip = a.Ey[id + oip]; }
@@ -992,14 +997,14 @@ template <typename S, typename T>
 __device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot, long Ptot, const Fld<S, T> &Ex,
                                              const Fld<S, T> &Ey, const Fld<S, T> &Ez, const Fld<S, T> &Hx,
                                              const Fld<S, T> &Hy, const Fld<S, T> &Hz, int Nx, int Ny, int Nz,
-                                             T *stage) {
+                                             int P, T *stage) {
     const int a = (int)(p % d.span1), b = (int)(p / d.span1);
     int i, j, k;
     if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
     else if (d.type == AEFDTD_NORMAL_Y || d.type == AEFDTD_NORMAL_YM) { i = d.x1 + a; j = d.y1; k = d.z1 + b; }
     else { i = d.x1 + a; j = d.y1 + b; k = d.z1; }
     const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
-    auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + Nx * (y + Ny * z)]; };
+    auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + P * (y + Ny * z)]; };
     T v[6];
     v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
     v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
@@ -1014,11 +1019,11 @@ __device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
                                                 Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
-                                                int Nz, T *stage) {
+                                                int Nz, int P, T *stage) {
     const SensDesc d = sd[blockIdx.y];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
-    sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, stage);
+    sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage);
 }
 
 // One sensor of one grid of a batch: the grid's fields, its stage pool and
@@ -1027,7 +1032,7 @@ template <typename S, typename T>
 struct SampB {
     SensDesc d;
     Fld<S, T> F[6];
-    int Nx, Ny, Nz, member;
+    int Nx, Ny, Nz, P, member;
     long Ptot;
     T *stage;
 };
@@ -1041,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_sample_batch(const SampB<S, T> *__restr
     if (tcur[q.member] + s >= tend[q.member]) return;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= q.d.P) return;
-    sample_point(q.d, p, s, q.Ptot, q.F[0], q.F[1], q.F[2], q.F[3], q.F[4], q.F[5], q.Nx, q.Ny, q.Nz, q.stage);
+    sample_point(q.d, p, s, q.Ptot, q.F[0], q.F[1], q.F[2], q.F[3], q.F[4], q.F[5], q.Nx, q.Ny, q.Nz, q.P, q.stage);
 }
 
 // Running DFT over a block of nfill <= KS staged steps (sensors/sensors.cpp:
@@ -1172,7 +1177,7 @@ __device__ inline double d_pulse(const aefdtd_pulse &p, double amp, double x, do
 constexpr int PS_BLOCKS = 64;
 
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int Ny, int kpl,
+__global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int Ny, int P, int kpl,
                                                    aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
                                                    double Dz, double zpos, const double *tap, int t, double Dt,
                                                    double *out) {
@@ -1191,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, F
             in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
             in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
         }
-        const int id = (int)(q + NxNy * kpl);
+        const int id = i + P * (j + Ny * kpl);
         const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
         for (int c = 0; c < 3; c++) {
             acc[3 + c] += in[c] * tp;

@@ -293,9 +293,12 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
     int d[3], pads[6];
     run_dims(kind, m, d, pads);
     const size_t N = (size_t)d[0] * d[1] * d[2];
+    // cell arrays of 3D grids have rows padded to 64 cells
+    const size_t px = (d[2] > 1 && d[0] > 1) ? (size_t)(d[0] + 63) / 64 * 64 : (size_t)d[0];
+    const size_t NA = px * d[1] * d[2];
     const int p = m->precision;
     const size_t ss = p == AEFDTD_FP64 ? 8 : (p == AEFDTD_FP32 ? 4 : 2), cs = p == AEFDTD_FP64 ? 8 : 4;
-    size_t b = 6 * N * ss + N;                              // fields, material ids
+    size_t b = 6 * NA * ss + NA;                            // fields, material ids
     // dispersive cells of the extended grid: edge planes repeat into pads and PML
     int npmax = 0;
     bool any = false;
@@ -326,7 +329,7 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
                 for (int i = 0; i < m->nx; i++)
                     if (row[i] < (uint32_t)m->n_materials && rc[row[i]]) nd += wjk * mult(0, i);
             }
-        b += 4 * N + nd * (4 + 3 * cs + 3 * (size_t)npmax * 16);
+        b += 4 * NA + nd * (4 + 3 * cs + 3 * (size_t)npmax * 16);
     }
     const size_t cross[3] = {(size_t)d[1] * d[2], (size_t)d[0] * d[2], (size_t)d[0] * d[1]};
     size_t cpml = 0;

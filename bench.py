@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--n", type=int, default=256, help="structure cells per side")
+    ap.add_argument("--prof-steps", type=int, default=100,
+                    help="steps of the profiled segment after the timed region (per-kernel averages)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-steps", type=int, default=100,
@@ -369,7 +371,7 @@ def main():
     if args.form == "serving":
         return serving_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
-    nsteps = args.warmup + args.steps
+    nsteps = args.warmup + args.steps + args.prof_steps
     t_setup = time.perf_counter()
     spec, sensors, spheres = workloads.spheres_single_particle(n=args.n, Nt=nsteps, precision=prec)
     run = capi.Run(capi.engine(), 1, spec, sensors)
@@ -387,12 +389,16 @@ def main():
     else:
         tiles = run.autotune(3) if not args.no_tune else None
     run.synchronize()
-    run.profile(True)
+    # timed region: the K steps alone, with no per-kernel events in the stream
     t0 = time.perf_counter()
     run.step(args.steps)
     run.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
+    # per-kernel averages from a further profiled segment (events around each launch)
+    run.profile(True)
+    run.step(args.prof_steps)
+    run.synchronize()
     e_ms, e_n = run.profile_read("E")
     h_ms, h_n = run.profile_read("H")
     s_ms, s_n = run.profile_read("sensor")

@@ -163,19 +163,20 @@ def _tiny_custom(n=8, Nt=20, precision=capi.FP32):
 
 def test_estimate_bytes_closed_form():
     """Fields, material ids and CPML profiles of a grid without PML, poles or
-    sensors, plus the 2% + 8 MiB margin of the estimate."""
+    sensors, plus the 2% + 8 MiB margin of the estimate.  The cell arrays of a
+    3D grid have their 7-cell rows padded to 64 cells."""
     be = capi.engine()
     g = np.zeros((6, 5, 7), dtype=np.uint32)
     for prec, ss, cs in ((capi.FP64, 8, 8), (capi.FP32, 4, 4), (capi.FP16, 2, 4)):
         spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(2.0)], Nt=10, precision=prec)
-        N = 7 * 5 * 6
-        b = 6 * N * ss + N + 6 * (7 + 5 + 6) * cs
+        NA = 64 * 5 * 6
+        b = 6 * NA * ss + NA + 6 * (7 + 5 + 6) * cs
         assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
     # a dispersive material adds the slot map and the per-cell pole state
     spec = capi.ModeSpec(matsgrid=np.ones((6, 5, 7), dtype=np.uint32), materials=[capi.Material(1.0), workloads.GOLD],
                          Nt=10, precision=capi.FP32)
-    N = 7 * 5 * 6
-    b = 6 * N * 4 + N + 6 * 18 * 4 + 4 * N + N * (4 + 3 * 4 + 3 * 3 * 16)
+    N, NA = 7 * 5 * 6, 64 * 5 * 6
+    b = 6 * NA * 4 + NA + 6 * 18 * 4 + 4 * NA + N * (4 + 3 * 4 + 3 * 3 * 16)
     assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
 
 
