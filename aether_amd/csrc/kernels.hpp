@@ -44,11 +44,25 @@ struct Fld {
         if constexpr (std::is_same_v<S, T>) return p[i];
         else return (T)p[i] * ld;
     }
+    // streaming load (not kept in the caches): for values a pass reads once
+    __device__ __forceinline__ T nt(int i) const {
+        if constexpr (std::is_same_v<S, T>) return __builtin_nontemporal_load(p + i);
+        else return (T)__builtin_nontemporal_load(p + i) * ld;
+    }
     __device__ __forceinline__ void set(int i, T v) const {
         if constexpr (std::is_same_v<S, T>) p[i] = v;
         else p[i] = (S)(v * st);
     }
+    // streaming store (not kept in the caches): for passes whose output is
+    // next read a whole pass later
+    __device__ __forceinline__ void set_nt(int i, T v) const {
+        if constexpr (std::is_same_v<S, T>) __builtin_nontemporal_store(v, p + i);
+        else __builtin_nontemporal_store((S)(v * st), p + i);
+    }
 };
+
+// Store-mask bit asking for streaming stores.
+constexpr int STORE_NT = 64;
 
 // Per-material constants (FDTD_Material, core/fdtd_material.h:46-47).
 template <typename T>
@@ -392,6 +406,12 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
                                        T ikx, T iky, T ikz, int nx_, int ny_, int ostep, int side, int smask, P ps) {
     const V3<T> e = e_cell_vals(a, M, i, j, k, id, ex0, ey0, ez0, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky,
                                 ikz, nx_, ny_, ostep, side, ps);
+    if (smask & STORE_NT) {
+        a.Ex.set_nt(id, e.x);
+        a.Ey.set_nt(id, e.y);
+        a.Ez.set_nt(id, e.z);
+        return;
+    }
     if (smask & 1) a.Ex.set(id, e.x);
     if (smask & 2) a.Ey.set(id, e.y);
     if (smask & 4) a.Ez.set(id, e.z);
@@ -511,10 +531,10 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
         v.hz = a.Hz[id];
         v.hz_jm = a.Hz[id + ojm];
         v.hz_im = a.Hz[id + oim];
-        v.ex = a.Ex[id];
-        v.ey = a.Ey[id];
-        v.ez = a.Ez[id];
-        v.m = a.mat[id];
+        v.ex = a.Ex.nt(id);          // read once per pass: streamed
+        v.ey = a.Ey.nt(id);
+        v.ez = a.Ez.nt(id);
+        v.m = __builtin_nontemporal_load(a.mat + id);
         return v;
     };
     // The psi of a plane are requested with the plane's fields, one plane
@@ -546,7 +566,7 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
         hy_km = c.hy;
         e_cell(a, sm[c.m], c.m, i, j, k, base + k * NxNy, c.ex, c.ey, c.ez, c.hz - c.hz_jm, dHy_z, dHx_z,
                c.hz - c.hz_im, c.hy - c.hy_im, c.hx - c.hx_jm, ikx, iky, uload(a.pf.ikE[2], k), nx_, ny_, ostep, side,
-               7, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+               STORE_NT, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
     };
     Ld cur = load(k0);
     PsiSetE<T> q = load_psi(k0);
@@ -827,6 +847,12 @@ __device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k
                                        T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, T ikz, int nx_,
                                        int ny_, int side, int smask, P ps) {
     h_cell_vals(a, i, j, k, hx, hy, hz, dEy_z, dEx_z, dEz_y, dEz_x, dEx_y, dEy_x, ikx, iky, ikz, nx_, ny_, side, ps);
+    if (smask & STORE_NT) {
+        a.Hx.set_nt(id, hx);
+        a.Hy.set_nt(id, hy);
+        a.Hz.set_nt(id, hz);
+        return;
+    }
     if (smask & 8) a.Hx.set(id, hx);
     if (smask & 16) a.Hy.set(id, hy);
     if (smask & 32) a.Hz.set(id, hz);
@@ -871,9 +897,9 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
         v.ez = a.Ez[id];
         v.ez_jp = a.Ez[id + ojp];
         v.ez_ip = a.Ez[id + oip];
-        v.hx = a.Hx[id];
-        v.hy = a.Hy[id];
-        v.hz = a.Hz[id];
+        v.hx = a.Hx.nt(id);          // read once per pass: streamed
+        v.hy = a.Hy.nt(id);
+        v.hz = a.Hz.nt(id);
         return v;
     };
     auto load_psi = [&](int k) {
@@ -902,7 +928,7 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
         ey_k = c.ey_kp;
         h_cell(a, i, j, k, base + k * NxNy, c.hx, c.hy, c.hz, c.ey_kp - ey, c.ex_kp - ex, c.ez_jp - c.ez,
                c.ez_ip - c.ez, c.ex_jp - ex, c.ey_ip - ey, ikx, iky, uload(a.pf.ikH[2], k), nx_, ny_, side,
-               8 | 16 | 32, PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+               STORE_NT, PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
     };
     Ld cur = load(k0);
     PsiSetH<T> q = load_psi(k0);
