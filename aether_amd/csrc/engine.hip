@@ -595,7 +595,7 @@ int launch_rc(aefdtd_grid *g) {
     r.corr = (T *)g->d_corr;
     hipEvent_t ev;
     prof_begin(g, 4, &ev);
-    hipLaunchKernelGGL((k_rc_ante<S, T>), dim3((unsigned)((g->n_disp + 255) / 256)), dim3(256), 0, g->stream, r);
+    hipLaunchKernelGGL((k_rc_ante<S, T>), dim3((unsigned)((g->n_disp + 255) / 256), 3), dim3(256), 0, g->stream, r);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;

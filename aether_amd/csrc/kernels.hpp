@@ -447,31 +447,42 @@ struct RcArgs {
     T *corr;
 };
 
+// One lane per (dispersive slot, component); the pole states of the first
+// RC_PRELOAD poles are all requested before any is updated, so their load
+// latencies overlap.
+constexpr int RC_PRELOAD = 8;
+
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<S, T> a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
     if (s >= a.ndisp) return;
     const int id = a.cells[s];
     const MatDev<T> &G = a.mt[a.mat[id]];
-    const double e3[3] = {(double)a.Ex[id], (double)a.Ey[id], (double)a.Ez[id]};
-    double sum[3] = {0, 0, 0};
-    for (int p = 0; p < G.np; p++) {
+    const Fld<S, T> &E = c == 0 ? a.Ex : (c == 1 ? a.Ey : a.Ez);
+    const double e = (double)E[id];
+    double2 *q = a.psi + (size_t)c * a.ndisp + s;               // pole p at q[3 p ndisp]
+    const size_t stride = (size_t)3 * a.ndisp;
+    const int np = G.np;
+    double2 v[RC_PRELOAD];
+#pragma unroll
+    for (int p = 0; p < RC_PRELOAD; p++)
+        if (p < np) v[p] = q[p * stride];
+    double sum = 0;
+    for (int p = 0; p < np; p++) {
         const double2 cr = a.poles[2 * (G.pole0 + p)], dc = a.poles[2 * (G.pole0 + p) + 1];
-        double2 *q = a.psi + (size_t)(3 * p) * a.ndisp + s;
-        double acc[3];
+        double2 x = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double2 v = q[(size_t)c * a.ndisp];
-            double2 r;
-            r.x = (cr.x * v.x - cr.y * v.y) + dc.x * e3[c];
-            r.y = (cr.x * v.y + cr.y * v.x) + dc.y * e3[c];
-            q[(size_t)c * a.ndisp] = r;
-            acc[c] = r.x;
-        }
-        sum[0] += acc[0]; sum[1] += acc[1]; sum[2] += acc[2];
+        for (int u = 0; u < RC_PRELOAD; u++)
+            if (u == p) x = v[u];
+        if (p >= RC_PRELOAD) x = q[p * stride];
+        double2 r;
+        r.x = (cr.x * x.x - cr.y * x.y) + dc.x * e;
+        r.y = (cr.x * x.y + cr.y * x.x) + dc.y * e;
+        q[p * stride] = r;
+        sum += r.x;
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++) a.corr[3 * s + c] = (T)(G.C3 * sum[c]);
+    a.corr[3 * s + c] = (T)(G.C3 * sum);
 }
 
 template <typename S, typename T>
