@@ -547,6 +547,11 @@ HArgs<S, T> h_args(aefdtd_grid *g) {
         a.aEx = fld<S, T>(g->tfsf_aux, 0); a.aEy = fld<S, T>(g->tfsf_aux, 1); a.aEz = fld<S, T>(g->tfsf_aux, 2);
     }
     a.lm = load_mask_H(g);
+    if (g->d_strip) {
+        a.nstrip = (int)g->strip_cols.size();
+        for (int q = 0; q < a.nstrip; q++) a.scol[q] = g->strip_cols[q];
+        a.strip = (T *)g->d_strip;
+    }
     return a;
 }
 
@@ -607,6 +612,7 @@ int launch_E(aefdtd_grid *g) {
     if (rc) return rc;
     rc = launch_rc<S, T>(g);
     if (rc) return rc;
+    g->strip_valid = false;             // E changes: the strip is stale until the next H pass
     EArgs<S, T> a = e_args<S, T>(g);
     hipEvent_t ev;
     prof_begin(g, 0, &ev);
@@ -624,6 +630,9 @@ int launch_H(aefdtd_grid *g) {
     dispatch_H<S, T>(g, a, g->tile_h);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
+    // the 3D pass wrote the strip from the fields it left (the column kernel
+    // and the 2D pass do not write it)
+    g->strip_valid = a.nstrip > 0 && g->Nz > 1 && !is_column(g);
     return AEFDTD_OK;
 }
 
@@ -691,6 +700,7 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
         dfree(snap[q]);
     }
     HIPOK(g, hipGetLastError());
+    g->strip_valid = false;             // the timing passes rewrote the strip; the fields were restored
     g->tile_e = best[0];
     g->tile_h = best[1];
     if (out) { out[0] = best[0]; out[1] = best[1]; }
@@ -759,14 +769,29 @@ int prepare_stage(aefdtd_grid *g) {
     if (g->d_stage || g->sensors.empty()) return AEFDTD_OK;
     long Ptot = 0;
     std::vector<SensDesc> sd;
+    // planes normal to x get a strip (columns x1 and x1+1) written by the 3D
+    // H pass; planes beyond MAX_STRIP distinct columns sample the fields
+    const bool strips = g->Nz > 1 && g->Nx > 1;
     for (Sensor *s : g->sensors) {
         s->off = Ptot;
         Ptot += s->P;
-        sd.push_back({s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off});
+        int sp = -1;
+        if (strips && (s->type == AEFDTD_NORMAL_X || s->type == AEFDTD_NORMAL_XM)) {
+            for (size_t q = 0; q < g->strip_cols.size(); q++)
+                if (g->strip_cols[q] == s->x1) sp = (int)q;
+            if (sp < 0 && g->strip_cols.size() < (size_t)MAX_STRIP) {
+                sp = (int)g->strip_cols.size();
+                g->strip_cols.push_back(s->x1);
+            }
+        }
+        sd.push_back({s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off, sp});
         g->stage_maxP = std::max(g->stage_maxP, s->P);
     }
     g->stage_P = Ptot;
     int rc;
+    if (!g->strip_cols.empty() &&
+        (rc = dalloc(g, &g->d_strip, g->strip_cols.size() * 12 * (size_t)g->Ny * g->Nz * sizeof(T))))
+        return rc;
     if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
     if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
     HIPOK(g, hipMemcpy(g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
@@ -784,7 +809,7 @@ int feed_T(aefdtd_grid *g) {
         hipLaunchKernelGGL((k_sample<S, T>), grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc,
                            g->stage_fill, g->stage_P, fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2),
                            fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, g->px,
-                           (T *)g->d_stage);
+                           (T *)g->d_stage, g->strip_valid ? (const T *)g->d_strip : nullptr);
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
         g->stage_fill++;
@@ -1030,6 +1055,7 @@ int batch_build(Batch *b) {
         if (tb_on) pol[r] = tb_polarisation(g);
         EArgs<S, T> a = e_args<S, T>(g);
         HArgs<S, T> h = h_args<S, T>(g);
+        h.nstrip = 0;                       // batch sampling reads the fields
         const int nb = tile_geometry(a, b->tile_e), nh = tile_geometry(h, b->tile_h);
         wgE.push_back(totE);
         wgH.push_back(totH);
@@ -1062,7 +1088,7 @@ int batch_build(Batch *b) {
             sbeg[2 * r] = (int)samp.size();
             for (Sensor *s : g->sensors) {
                 SampB<S, T> q;
-                q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off};
+                q.d = SensDesc{s->type, s->x1, s->y1, s->z1, s->span1, s->span2, s->P, s->off, -1};
                 for (int c = 0; c < 6; c++) q.F[c] = fld<S, T>(g, c);
                 q.Nx = g->Nx; q.Ny = g->Ny; q.Nz = g->Nz; q.P = g->px; q.member = r;
                 q.Ptot = g->stage_P;
@@ -1409,6 +1435,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     for (int s = 0; s < 12; s++) dfree(g->d_psipml[s]);
     dfree(g->d_stage);
     dfree(g->d_sdesc);
+    dfree(g->d_strip);
     dfree(g->d_osc);
     for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
@@ -1544,6 +1571,7 @@ int aefdtd_update_H(aefdtd_grid *g) {
 
 int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;
+    g->strip_valid = false;
     size_t N = g->nalloc();
     for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream));
     for (int s = 0; s < 12; s++)
@@ -1575,6 +1603,7 @@ int aefdtd_read_field(aefdtd_grid *g, int f, double *out) {
 
 int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
     if (!g || !in) return AEFDTD_EINVAL;
+    g->strip_valid = false;
     if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
     HIPOK(g, hipStreamSynchronize(g->stream));
     if (g->precision == AEFDTD_FP16) {

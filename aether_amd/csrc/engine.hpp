@@ -148,8 +148,13 @@ struct aefdtd_grid {
     void *d_stage = nullptr;          // [KS][6][Ptot] T
     void *d_sdesc = nullptr;          // sensor descriptors for the sampling kernel
     long stage_P = 0, stage_maxP = 0;
+    // x columns whose cells the H pass copies to the sensor strip (3D grids with
+    // sensor planes normal to x), and whether the strip holds the current fields
+    std::vector<int> strip_cols;      // x1 of each strip plane (columns x1, x1+1)
+    void *d_strip = nullptr;          // [plane][Nz][Ny][2 columns][6] T
+    bool strip_valid = false;
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
-    int tile_e = 2 | (3 << 2) | (1 << 4), tile_h = 2 | (3 << 2) | (1 << 4);
+    int tile_e = 2 | (2 << 2), tile_h = 2 | (2 << 2);
     // bookkeeping
     bool ready = false;               // device copies up to date
     bool pml_dirty = true;

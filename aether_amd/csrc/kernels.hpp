@@ -108,6 +108,8 @@ struct EArgs {
     const OscDev *osc;
 };
 
+constexpr int MAX_STRIP = 4;
+
 template <typename S, typename T>
 struct HArgs {
     int Nx, Ny, Nz, P;
@@ -123,6 +125,14 @@ struct HArgs {
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
     Fld<S, T> aEx, aEy, aEz;
     int lm;                    // bits 0-2: E components loaded, bits 3-5: H components loaded and stored
+    // sensor strip: for each sensor plane normal to x at column x = scol[p]
+    // (p < nstrip), the H pass also writes E and the new H of the cells of
+    // columns x and x+1 (wrapping) to strip[((p Nz + k) Ny + j) 12 + 6 col + c],
+    // so that the samples of those planes read contiguous memory instead of one
+    // cache line per row and component
+    int nstrip;
+    int scol[MAX_STRIP];
+    T *strip;
 };
 
 // Hardware workgroup id -> logical tile (bx, by, bz).  Workgroups are dealt to
@@ -854,19 +864,38 @@ __device__ __forceinline__ void h_cell_vals(const HArgs<S, T> &a, int i, int j, 
 // The new H of one cell written back for the components in store mask
 // `smask` (bit 3 + c: component c).
 template <typename S, typename T, class P>
-__device__ __forceinline__ void h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
-                                       T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, T ikz, int nx_,
-                                       int ny_, int side, int smask, P ps) {
+__device__ __forceinline__ V3<T> h_cell(const HArgs<S, T> &a, int i, int j, int k, int id, T hx, T hy, T hz, T dEy_z,
+                                        T dEx_z, T dEz_y, T dEz_x, T dEx_y, T dEy_x, T ikx, T iky, T ikz, int nx_,
+                                        int ny_, int side, int smask, P ps) {
     h_cell_vals(a, i, j, k, hx, hy, hz, dEy_z, dEx_z, dEz_y, dEz_x, dEx_y, dEy_x, ikx, iky, ikz, nx_, ny_, side, ps);
     if (smask & STORE_NT) {
         a.Hx.set_nt(id, hx);
         a.Hy.set_nt(id, hy);
         a.Hz.set_nt(id, hz);
-        return;
+    } else {
+        if (smask & 8) a.Hx.set(id, hx);
+        if (smask & 16) a.Hy.set(id, hy);
+        if (smask & 32) a.Hz.set(id, hz);
     }
-    if (smask & 8) a.Hx.set(id, hx);
-    if (smask & 16) a.Hy.set(id, hy);
-    if (smask & 32) a.Hz.set(id, hz);
+    return V3<T>{hx, hy, hz};
+}
+
+// The value a field component reads back after storing v (FP16 storage
+// rounds; FP32/FP64 storage is exact).
+template <typename S, typename T>
+__device__ __forceinline__ T stored(const Fld<S, T> &f, T v) {
+    if constexpr (std::is_same_v<S, T>) return v;
+    else return (T)(S)(v * f.st) * f.ld;
+}
+
+// Six values of one cell into strip slot s = 2 p + col (three 2-wide stores).
+template <typename S, typename T>
+__device__ __forceinline__ void strip_put(const HArgs<S, T> &a, int s, int j, int k, const T v[6]) {
+    using T2 = std::conditional_t<std::is_same_v<T, double>, double2, float2>;
+    T2 *sp = reinterpret_cast<T2 *>(a.strip + (((size_t)(s >> 1) * a.Nz + k) * a.Ny + j) * 12 + 6 * (s & 1));
+    sp[0] = T2{v[0], v[1]};
+    sp[1] = T2{v[2], v[3]};
+    sp[2] = T2{v[4], v[5]};
 }
 
 // 3D H pass over tile w of the grid: the mirror of e_tile (every component
@@ -892,6 +921,16 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
     T bx_ = 0, cx_ = 0, by_ = 0, cy_ = 0;
     if ((side & SIDE_X) && nx_ >= 0) { bx_ = a.pf.bH[0][i]; cx_ = a.pf.cH[0][i]; }
     if ((side & SIDE_Y) && ny_ >= 0) { by_ = a.pf.bH[1][j]; cy_ = a.pf.cH[1][j]; }
+    // strip slots (2 p + col) of the lane's column; a column can serve two planes
+    int sq0 = -1, sq1 = -1;
+    for (int q = 0; q < a.nstrip; q++) {
+        const int x0 = a.scol[q], x1 = (x0 + 1 == a.Nx) ? 0 : x0 + 1;
+        const int s = (i == x0) ? 2 * q : (i == x1 ? 2 * q + 1 : -1);
+        if (s >= 0) {
+            if (sq0 < 0) sq0 = s;
+            else sq1 = s;
+        }
+    }
     T ex_k = a.Ex[base + k0 * NxNy];
     T ey_k = a.Ey[base + k0 * NxNy];
     struct Ld {
@@ -937,9 +976,15 @@ __device__ __forceinline__ void h_tile(const HArgs<S, T> &a, int w) {
         const T ex = ex_k, ey = ey_k;
         ex_k = c.ex_kp;
         ey_k = c.ey_kp;
-        h_cell(a, i, j, k, base + k * NxNy, c.hx, c.hy, c.hz, c.ey_kp - ey, c.ex_kp - ex, c.ez_jp - c.ez,
-               c.ez_ip - c.ez, c.ex_jp - ex, c.ey_ip - ey, ikx, iky, uload(a.pf.ikH[2], k), nx_, ny_, side,
-               STORE_NT, PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+        const V3<T> h = h_cell(a, i, j, k, base + k * NxNy, c.hx, c.hy, c.hz, c.ey_kp - ey, c.ex_kp - ex,
+                               c.ez_jp - c.ez, c.ez_ip - c.ez, c.ex_jp - ex, c.ey_ip - ey, ikx, iky,
+                               uload(a.pf.ikH[2], k), nx_, ny_, side, STORE_NT,
+                               PrefPsiH<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+        if (sq0 >= 0) {
+            const T v[6] = {ex, ey, c.ez, stored(a.Hx, h.x), stored(a.Hy, h.y), stored(a.Hz, h.z)};
+            strip_put(a, sq0, j, k, v);
+            if (sq1 >= 0) strip_put(a, sq1, j, k, v);
+        }
     };
     Ld cur = load(k0);
     PsiSetH<T> q = load_psi(k0);
@@ -1026,6 +1071,7 @@ __global__ __launch_bounds__(256) void k_update_H_col(const HArgs<S, T> a) {
 struct SensDesc {
     int type, x1, y1, z1, span1, span2;
     long P, off;
+    int sp;          // planes normal to x: strip plane index (-1: sample the fields)
 };
 
 // local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) at plane cell p of a
@@ -1034,7 +1080,7 @@ template <typename S, typename T>
 __device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot, long Ptot, const Fld<S, T> &Ex,
                                              const Fld<S, T> &Ey, const Fld<S, T> &Ez, const Fld<S, T> &Hx,
                                              const Fld<S, T> &Hy, const Fld<S, T> &Hz, int Nx, int Ny, int Nz,
-                                             int P, T *stage) {
+                                             int P, T *stage, const T *strip = nullptr) {
     const int a = (int)(p % d.span1), b = (int)(p / d.span1);
     int i, j, k;
     if (d.type == AEFDTD_NORMAL_X || d.type == AEFDTD_NORMAL_XM) { i = d.x1; j = d.y1 + a; k = d.z1 + b; }
@@ -1043,6 +1089,21 @@ __device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot
     const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
     auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + P * (y + Ny * z)]; };
     T v[6];
+    if (strip && d.sp >= 0) {
+        // the same sums from the strip the H pass wrote (columns i and ip)
+        auto st = [&](int col, int c, int y, int z) {
+            return strip[(((size_t)d.sp * Nz + z) * Ny + y) * 12 + 6 * col + c];
+        };
+        v[0] = (st(0, 0, j, k) + st(0, 0, jp, k) + st(0, 0, j, kp) + st(0, 0, jp, kp)) / (T)4.0;
+        v[1] = (st(0, 1, j, k) + st(1, 1, j, k) + st(0, 1, j, kp) + st(1, 1, j, kp)) / (T)4.0;
+        v[2] = (st(0, 2, j, k) + st(1, 2, j, k) + st(0, 2, jp, k) + st(1, 2, jp, k)) / (T)4.0;
+        v[3] = (st(0, 3, j, k) + st(1, 3, j, k)) / (T)2.0;
+        v[4] = (st(0, 4, j, k) + st(0, 4, jp, k)) / (T)2.0;
+        v[5] = (st(0, 5, j, k) + st(0, 5, j, kp)) / (T)2.0;
+#pragma unroll
+        for (int c = 0; c < 6; c++) stage[((size_t)slot * 6 + c) * Ptot + d.off + p] = v[c];
+        return;
+    }
     v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
     v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
     v[2] = (at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0;
@@ -1056,11 +1117,11 @@ __device__ __forceinline__ void sample_point(const SensDesc &d, long p, int slot
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
                                                 Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
-                                                int Nz, int P, T *stage) {
+                                                int Nz, int P, T *stage, const T *strip) {
     const SensDesc d = sd[blockIdx.y];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
-    sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage);
+    sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage, strip);
 }
 
 // One sensor of one grid of a batch: the grid's fields, its stage pool and
