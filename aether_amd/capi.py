@@ -124,6 +124,7 @@ def _sig(lib, p):
         "run_destroy": (i, [vp]),
         "synchronize": (i, [vp]),
         "profile_enable": (i, [vp, i]),
+        "profile_kinds": (i, [vp, i]),
         "profile_read": (i, [vp, C.c_char_p, _dp, C.POINTER(C.c_long)]),
         "device_bytes": (i, [vp, C.POINTER(C.c_size_t)]),
         "stats": (i, [vp, C.POINTER(C.c_long)]),
@@ -507,7 +508,8 @@ class Run:
         self.be.fn("layout")(self.grid(), lay)
         return list(lay)
 
-    def profile(self, on):
+    def profile(self, on, kinds=31):
+        self.be.fn("profile_kinds")(self.grid(), int(kinds))
         self.be.fn("profile_enable")(self.grid(), int(on))
 
     def autotune(self, reps=3):

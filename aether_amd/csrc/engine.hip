@@ -322,11 +322,16 @@ int ensure_ready(aefdtd_grid *g) {
     return AEFDTD_OK;
 }
 
+// Events used only to time device work: no system-scope fence (and hence no
+// cache writeback and invalidation) when they are recorded, so that timing a
+// pass does not slow the passes around it.
+hipError_t timing_event(hipEvent_t *e) { return hipEventCreateWithFlags(e, hipEventDisableSystemFence); }
+
 void prof_begin(aefdtd_grid *g, int kind, hipEvent_t *b) {
     *b = nullptr;
-    if (!g->prof_on) return;
+    if (!g->prof_on || !((g->prof_mask >> kind) & 1)) return;
     hipEvent_t a;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(b) != hipSuccess) { *b = nullptr; return; }
+    if (timing_event(&a) != hipSuccess || timing_event(b) != hipSuccess) { *b = nullptr; return; }
     (void)hipEventRecord(a, g->stream);
     g->events.push_back({kind, a, *b});
 }
@@ -678,8 +683,8 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
         HIPOK(g, hipMemcpyAsync(snap[q], state[q].first, state[q].second, hipMemcpyDeviceToDevice, g->stream));
     }
     hipEvent_t ea, eb;
-    HIPOK(g, hipEventCreate(&ea));
-    HIPOK(g, hipEventCreate(&eb));
+    HIPOK(g, timing_event(&ea));
+    HIPOK(g, timing_event(&eb));
     int best[2] = {g->tile_e, g->tile_h};
     for (int pass = 0; pass < 2; pass++) {
         float bt = 1e30f;
@@ -1193,7 +1198,7 @@ int batch_create(Batch **out, int precision) {
     b->precision = precision;
     b->tb_code = precision == AEFDTD_FP16 ? -1 : (1 | (1 << 2));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        timing_event(&b->ev0) != hipSuccess || timing_event(&b->ev1) != hipSuccess) {
         set_global_error("batch stream/event creation failed");
         batch_destroy(b);
         return AEFDTD_EDEVICE;
@@ -1852,6 +1857,12 @@ int aefdtd_profile_enable(aefdtd_grid *g, int on) {
     return AEFDTD_OK;
 }
 
+int aefdtd_profile_kinds(aefdtd_grid *g, int mask) {
+    if (!g || mask < 0 || mask > 31) return AEFDTD_EINVAL;
+    g->prof_mask = mask;
+    return AEFDTD_OK;
+}
+
 int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches) {
     if (!g || !kernel) return AEFDTD_EINVAL;
     prof_collect(g);
@@ -1875,8 +1886,8 @@ int aefdtd_stream_bandwidth(size_t bytes, int reps, double out[3]) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = AEFDTD_OK;
     if (hipMalloc(&a, n * sizeof(float4)) != hipSuccess || hipMalloc(&b, n * sizeof(float4)) != hipSuccess ||
-        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess) {
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || timing_event(&e0) != hipSuccess ||
+        timing_event(&e1) != hipSuccess) {
         rc = AEFDTD_EDEVICE;
     } else {
         (void)hipMemsetAsync(a, 0, n * sizeof(float4), st);

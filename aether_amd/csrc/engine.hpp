@@ -162,6 +162,7 @@ struct aefdtd_grid {
     std::string err;
     // profiling
     bool prof_on = false;
+    int prof_mask = 31;               // kinds bracketed with events (bit k: kind k)
     struct Ev { int kind; hipEvent_t a, b; };
     std::vector<Ev> events;
     double prof_ms[5] = {0, 0, 0, 0, 0};

@@ -402,6 +402,9 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int n_sensors,
  * pre-pass of the E update). */
 int aefdtd_profile_enable(aefdtd_grid *g, int on);
 int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, long *launches);
+/* Which kernels profiling brackets with events (bit 0 E, 1 H, 2 sensor, 3 dft,
+ * 4 rc; default all): fewer events keep a timed region closer to unprofiled. */
+int aefdtd_profile_kinds(aefdtd_grid *g, int mask);
 /* Tile autotuner: times every compiled tile shape of the E and H passes on
  * this grid (reps launches each), keeps the fastest and restores the grid's
  * state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |

@@ -25,7 +25,7 @@ def test_engine_exports_every_declared_symbol():
 def test_oracle_mirrors_the_grid_abi():
     import oracle
     lib = C.CDLL(oracle.ORACLE_SO)
-    engine_only = {"aefdtd_profile_enable", "aefdtd_profile_read", "aefdtd_device_bytes", "aefdtd_stats",
+    engine_only = {"aefdtd_profile_enable", "aefdtd_profile_read", "aefdtd_profile_kinds", "aefdtd_device_bytes", "aefdtd_stats",
                    "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
                    "aefdtd_set_storage_scale", "aefdtd_stream_bandwidth"}
     missing = [s for s in declared_symbols()
