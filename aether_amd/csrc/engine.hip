@@ -917,12 +917,43 @@ int tb_polarisation(const aefdtd_grid *g) {
     return pol;
 }
 
-template <typename S, typename T, int POL, int LY, int TS>
+template <typename S, typename T, int POL, int LY, int TS, bool PML>
 void launch_tb_k(int nwg, size_t lds, hipStream_t st, const void *ea, const void *ha, const int *mem, const int *wg0,
-                 int nm, const int *tc, const int *te, const void *samp, const int *sbeg, const void *alt, int s0) {
-    hipLaunchKernelGGL((k_tb2d_batch<S, T, POL, LY, TS>), dim3(nwg), dim3(64, TB_WAVES, 1), lds, st,
+                 int nm, const int *tc, const int *te, const void *samp, const int *sbeg, const void *alt,
+                 const int4 *rect, int s0) {
+    hipLaunchKernelGGL((k_tb2d_batch<S, T, POL, LY, TS, PML>), dim3(nwg), dim3(64, TB_WAVES, 1), lds, st,
                        (const EArgs<S, T> *)ea, (const HArgs<S, T> *)ha, mem, wg0, nm, tc, te,
-                       (const SampB<S, T> *)samp, sbeg, (const TbAlt<S, T> *)alt, s0, (s0 / TS) & 1);
+                       (const SampB<S, T> *)samp, sbeg, (const TbAlt<S, T> *)alt, rect, s0, (s0 / TS) & 1);
+}
+
+// The interior tiles of a time-skewed grid: [bx0, bx1) x [by0, by1) of its
+// tile grid, the tiles none of whose cells (halo included) lies in a CPML
+// slab or on a PEC wall; an empty rectangle when there are none.  E cells of
+// the slabs lie in 1..pm-1 and N-pp+1..N-1 (wall at 0), H cells in 0..pm-1
+// and N-pp..N-2 (wall at N-1), on axes with any CPML.
+int4 tb_interior(const aefdtd_grid *g, int IX, int IY, int LY, int TS) {
+    auto range = [&](int N, int pm, int pp, int step, int width, int &b0, int &b1) {
+        const int nt = (N + step - 1) / step;
+        b0 = 0;
+        b1 = nt;
+        if (!(pm || pp)) return;
+        const int lo = std::max(pm, 1), hi = N - std::max(pp, 1) - 1;
+        b0 = nt;
+        b1 = 0;
+        for (int t = 0; t < nt; t++) {
+            const int c0 = t * step - TS, c1 = c0 + width - 1;
+            if (c0 >= lo && c1 <= hi) {
+                b0 = std::min(b0, t);
+                b1 = std::max(b1, t + 1);
+            }
+        }
+        if (b0 >= b1) b0 = b1 = 0;
+    };
+    int bx0, bx1, by0, by1;
+    range(g->Nx, g->pml[0], g->pml[1], IX, TBX, bx0, bx1);
+    range(g->Ny, g->pml[2], g->pml[3], IY, LY, by0, by1);
+    if (bx0 >= bx1 || by0 >= by1) return make_int4(0, 0, 0, 0);
+    return make_int4(bx0, bx1, by0, by1);
 }
 
 // The second live-state set of a time-skewed grid (engine-owned, kept until
@@ -949,9 +980,9 @@ int ensure_tb_alt(aefdtd_grid *g, int pol) {
     return AEFDTD_OK;
 }
 
-template <typename S, typename T, int POL, int LY, int TS>
+template <typename S, typename T, int POL, int LY, int TS, bool PML>
 hipError_t tb_attr(size_t lds) {
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2d_batch<S, T, POL, LY, TS>),
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2d_batch<S, T, POL, LY, TS, PML>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
@@ -992,9 +1023,13 @@ struct aefdtd::Batch {
     // time-skewed 2D TE/TM members: tile code (small/large tile height | depth
     // index << 2), -1 = off; member lists and workgroup prefixes per polarisation
     int tb_code = 1 | (1 << 2);
-    int *d_mem[2] = {nullptr, nullptr}, *d_wgT[2] = {nullptr, nullptr}, *d_sbeg = nullptr;
+    // [polarisation][tile class: 0 interior, 1 boundary]
+    int *d_mem[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}, *d_wgT[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int *d_sbeg = nullptr;
     void *d_alt = nullptr;                 // TbAlt per member
-    int nmem[2] = {0, 0}, nwgT[2] = {0, 0}, nstep = 0;  // nstep: members stepped by the per-step passes
+    int4 *d_rect = nullptr;                // interior tile rectangle per member
+    int nmem[2][2] = {{0, 0}, {0, 0}}, nwgT[2][2] = {{0, 0}, {0, 0}};
+    int nstep = 0;                         // members stepped by the per-step passes
     int nsamp_step = 0;                    // leading sample entries that belong to those members
     size_t ldsT = 0;
     std::vector<int> tcur;                 // host mirror of d_tcur
@@ -1013,9 +1048,12 @@ void batch_free_tables(Batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
     for (void **p : {&b->d_ea, &b->d_ha, &b->d_samp, &b->d_dft, (void **)&b->d_wgE, (void **)&b->d_wgH,
-                     (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_mem[0], (void **)&b->d_mem[1],
-                     (void **)&b->d_wgT[0], (void **)&b->d_wgT[1], (void **)&b->d_sbeg, &b->d_alt})
+                     (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_sbeg, &b->d_alt, (void **)&b->d_rect})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (int p = 0; p < 2; p++)
+        for (int c = 0; c < 2; c++)
+            for (int **q : {&b->d_mem[p][c], &b->d_wgT[p][c]})
+                if (*q) { (void)hipFree(*q); *q = nullptr; }
 }
 
 template <typename V>
@@ -1042,11 +1080,12 @@ int batch_build(Batch *b) {
     const int IX = TBX - 2 * TS - 1, IY = ly - 2 * TS - 1;
     std::vector<EArgs<S, T>> ea;
     std::vector<HArgs<S, T>> ha;
-    std::vector<int> wgE, wgH, tend(n), pol(n, -1), mem[2], wgT[2], sbeg(2 * n, 0);
+    std::vector<int> wgE, wgH, tend(n), pol(n, -1), mem[2][2], wgT[2][2], sbeg(2 * n, 0);
+    std::vector<int4> rect(n, make_int4(0, 0, 0, 0));
     std::vector<SampB<S, T>> samp;
     std::vector<DftB<T>> dft;
     std::vector<TbAlt<S, T>> alt(n);
-    int totE = 0, totH = 0, totT[2] = {0, 0}, maxmat = 1;
+    int totE = 0, totH = 0, totT[2][2] = {{0, 0}, {0, 0}}, maxmat = 1;
     b->maxP = 0;
     for (int r = 0; r < n; r++) {
         aefdtd_grid *g = b->members[r];
@@ -1075,9 +1114,16 @@ int batch_build(Batch *b) {
             }
             for (int c = 0; c < 3; c++) alt[r].f[c] = (S *)g->tb_alt[c];
             for (int c = 0; c < 4; c++) alt[r].p[c] = (T *)g->tb_alt[3 + c];
-            mem[p].push_back(r);
-            wgT[p].push_back(totT[p]);
-            totT[p] += ((g->Nx + IX - 1) / IX) * ((g->Ny + IY - 1) / IY);
+            rect[r] = tb_interior(g, IX, IY, ly, TS);
+            const int nall = ((g->Nx + IX - 1) / IX) * ((g->Ny + IY - 1) / IY);
+            const int nin = (rect[r].y - rect[r].x) * (rect[r].w - rect[r].z);
+            const int cnt[2] = {nin, nall - nin};
+            for (int c = 0; c < 2; c++)
+                if (cnt[c] > 0) {
+                    mem[p][c].push_back(r);
+                    wgT[p][c].push_back(totT[p][c]);
+                    totT[p][c] += cnt[c];
+                }
         }
         maxmat = std::max(maxmat, a.nmat);
         ea.push_back(a);
@@ -1108,11 +1154,12 @@ int batch_build(Batch *b) {
         }
     wgE.push_back(totE);
     wgH.push_back(totH);
-    for (int p = 0; p < 2; p++) {
-        wgT[p].push_back(totT[p]);
-        b->nmem[p] = (int)mem[p].size();
-        b->nwgT[p] = totT[p];
-    }
+    for (int p = 0; p < 2; p++)
+        for (int c = 0; c < 2; c++) {
+            wgT[p][c].push_back(totT[p][c]);
+            b->nmem[p][c] = (int)mem[p][c].size();
+            b->nwgT[p][c] = totT[p][c];
+        }
     b->nwgE = totE;
     b->nwgH = totH;
     b->nsamp = (int)samp.size();
@@ -1124,30 +1171,40 @@ int batch_build(Batch *b) {
         (rc = batch_upload(b, (void **)&b->d_wgE, wgE)) || (rc = batch_upload(b, (void **)&b->d_wgH, wgH)) ||
         (rc = batch_upload(b, (void **)&b->d_tcur, b->tcur)) || (rc = batch_upload(b, (void **)&b->d_tend, tend)) ||
         (rc = batch_upload(b, &b->d_samp, samp)) || (rc = batch_upload(b, &b->d_dft, dft)) ||
-        (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)) || (rc = batch_upload(b, &b->d_alt, alt)))
+        (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)) || (rc = batch_upload(b, &b->d_alt, alt)) ||
+        (rc = batch_upload(b, (void **)&b->d_rect, rect)))
         return rc;
     for (int p = 0; p < 2; p++)
-        if (b->nmem[p]) {
-            if ((rc = batch_upload(b, (void **)&b->d_mem[p], mem[p])) ||
-                (rc = batch_upload(b, (void **)&b->d_wgT[p], wgT[p])))
-                return rc;
-            hipError_t ae = hipSuccess;
-            if constexpr (std::is_same_v<S, T>)
-                with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() { ae = tb_attr<S, T, P, L, TSK>(b->ldsT); });
-            BHIPOK(b, ae);
-        }
+        for (int c = 0; c < 2; c++)
+            if (b->nmem[p][c]) {
+                if ((rc = batch_upload(b, (void **)&b->d_mem[p][c], mem[p][c])) ||
+                    (rc = batch_upload(b, (void **)&b->d_wgT[p][c], wgT[p][c])))
+                    return rc;
+                hipError_t ae = hipSuccess;
+                if constexpr (std::is_same_v<S, T>)
+                    with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() {
+                        ae = c ? tb_attr<S, T, P, L, TSK, true>(b->ldsT) : tb_attr<S, T, P, L, TSK, false>(b->ldsT);
+                    });
+                BHIPOK(b, ae);
+            }
     // one chunk: the time-skewed blocks, KS x (E, H, samples) of the other
     // members, the DFT flush, the step advance
     BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     if constexpr (std::is_same_v<S, T>) {
         for (int s0 = 0; s0 < KS; s0 += TS)
             for (int p = 0; p < 2; p++)
-                if (b->nmem[p])
-                    with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() {
-                        launch_tb_k<S, T, P, L, TSK>(b->nwgT[p], b->ldsT, b->stream, b->d_ea, b->d_ha, b->d_mem[p],
-                                                     b->d_wgT[p], b->nmem[p], b->d_tcur, b->d_tend, b->d_samp,
-                                                     b->d_sbeg, b->d_alt, s0);
-                    });
+                for (int c = 0; c < 2; c++)
+                    if (b->nmem[p][c])
+                        with_tb<T>(p, b->tb_code, [&]<int P, int L, int TSK>() {
+                            auto go = [&]<bool PML>() {
+                                launch_tb_k<S, T, P, L, TSK, PML>(b->nwgT[p][c], b->ldsT, b->stream, b->d_ea, b->d_ha,
+                                                                  b->d_mem[p][c], b->d_wgT[p][c], b->nmem[p][c],
+                                                                  b->d_tcur, b->d_tend, b->d_samp, b->d_sbeg,
+                                                                  b->d_alt, b->d_rect, s0);
+                            };
+                            if (c) go.template operator()<true>();
+                            else go.template operator()<false>();
+                        });
     }
     const int ebi = b->tile_e & 3, ekz = (b->tile_e >> 2) & 3, hbi = b->tile_h & 3, hkz = (b->tile_h >> 2) & 3;
     for (int s = 0; s < KS && totE > 0; s++) {

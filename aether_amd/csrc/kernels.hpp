@@ -1372,21 +1372,21 @@ struct TbProf {
     T bE, cE, bH, cH;
 };
 
-template <int POL, typename T>
+template <int POL, bool PML, typename T>
 __device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const TbCoef<T> &M, int i, int j, int t, T &e0, T &e1,
                                           T dHa, T dHb, T ikx, T iky, int nx_, int ny_, const TbProf<T> &PX,
                                           const TbProf<T> &PY, T &pa, T &pb, int om) {
     // terms along the one-cell z axis are products with an exact zero and are left out
     if (POL == 0) {        // Ez from dHy_x = dHa, dHx_y = dHb
         T ez = M.C1 * e0 + M.C2x * ikx * dHa - M.C2y * iky * dHb;
-        if (x.px) {
+        if (PML && x.px) {
             if (nx_ >= 0) {
                 pa = PX.bE * pa + PX.cE * dHa;
                 ez += M.C4 * pa;
             }
             if (i == 0) ez = 0;
         }
-        if (x.py) {
+        if (PML && x.py) {
             if (ny_ >= 0) {
                 pb = PY.bE * pb + PY.cE * dHb;
                 ez -= M.C4 * pb;
@@ -1400,14 +1400,14 @@ __device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const TbCoef<T> &M,
     } else {               // Ex, Ey from dHz_y = dHa, dHz_x = dHb
         T ex = M.C1 * e0 + M.C2y * iky * dHa;
         T ey = M.C1 * e1 - M.C2x * ikx * dHb;
-        if (x.py) {
+        if (PML && x.py) {
             if (ny_ >= 0) {
                 pa = PY.bE * pa + PY.cE * dHa;
                 ex += M.C4 * pa;
             }
             if (j == 0) ex = 0;
         }
-        if (x.px) {
+        if (PML && x.px) {
             if (nx_ >= 0) {
                 pb = PX.bE * pb + PX.cE * dHb;
                 ey -= M.C4 * pb;
@@ -1425,21 +1425,21 @@ __device__ __forceinline__ void tb_e_cell(const TbCtx<T> &x, const TbCoef<T> &M,
     }
 }
 
-template <int POL, typename T>
+template <int POL, bool PML, typename T>
 __device__ __forceinline__ void tb_h_cell(const TbCtx<T> &x, int i, int j, T &h0, T &h1, T dEa, T dEb, T ikx, T iky,
                                           int nx_, int ny_, const TbProf<T> &PX, const TbProf<T> &PY, T &pc, T &pd) {
     if (POL == 0) {        // Hx, Hy from dEz_y = dEa, dEz_x = dEb
         T hx = h0, hy = h1;
         hx += -(x.dtdmy * iky * dEa);
         hy += x.dtdmx * ikx * dEb;
-        if (x.py) {
+        if (PML && x.py) {
             if (ny_ >= 0) {
                 pc = PY.bH * pc + PY.cH * dEa;
                 hx -= x.dtm * pc;
             }
             if (j == x.Ny - 1) hx = 0;
         }
-        if (x.px) {
+        if (PML && x.px) {
             if (nx_ >= 0) {
                 pd = PX.bH * pd + PX.cH * dEb;
                 hy += x.dtm * pd;
@@ -1451,14 +1451,14 @@ __device__ __forceinline__ void tb_h_cell(const TbCtx<T> &x, int i, int j, T &h0
     } else {               // Hz from dEx_y = dEa, dEy_x = dEb
         T hz = h0;
         hz += x.dtdmy * iky * dEa - x.dtdmx * ikx * dEb;
-        if (x.px) {
+        if (PML && x.px) {
             if (nx_ >= 0) {
                 pc = PX.bH * pc + PX.cH * dEb;
                 hz -= x.dtm * pc;
             }
             if (i == x.Nx - 1) hz = 0;
         }
-        if (x.py) {
+        if (PML && x.py) {
             if (ny_ >= 0) {
                 pd = PY.bH * pd + PY.cH * dEa;
                 hz += x.dtm * pd;
@@ -1480,8 +1480,39 @@ struct TbAlt {
     T *p[4];
 };
 
-// POL 0 = TM (LDS planes: Ez, Hx, Hy), 1 = TE (Ex, Ey, Hz).
-template <typename S, typename T, int POL, int LY, int TS>
+// Tiles of a time-skewed grid fall in two classes, launched as two kernels:
+// interior tiles (rect = [bx0, bx1) x [by0, by1) of the tile grid), none of
+// whose cells, halo included, lies in a CPML slab or on a PEC wall, and the
+// frame of boundary tiles around them.  The interior kernel carries no CPML
+// state at all, which roughly halves its registers (two workgroups per CU
+// instead of one) for the bulk of the tiles.
+__device__ __forceinline__ void tb_tile(bool pml, int4 rc, int ntx, int w, int &bx, int &by) {
+    const int nix = rc.y - rc.x;
+    if (!pml) {
+        bx = rc.x + w % nix;
+        by = rc.z + w / nix;
+        return;
+    }
+    const int per = ntx - nix;                   // boundary tiles per row of the rect's rows
+    if (w < rc.z * ntx) {
+        by = w / ntx;
+        bx = w % ntx;
+        return;
+    }
+    w -= rc.z * ntx;
+    if (w < (rc.w - rc.z) * per) {
+        by = rc.z + w / per;
+        const int c = w % per;
+        bx = c < rc.x ? c : rc.y + (c - rc.x);
+        return;
+    }
+    w -= (rc.w - rc.z) * per;
+    by = rc.w + w / ntx;
+    bx = w % ntx;
+}
+
+// POL 0 = TM (LDS planes: Ez, Hx, Hy), 1 = TE (Ex, Ey, Hz).  PML: boundary tiles.
+template <typename S, typename T, int POL, int LY, int TS, bool PML>
 __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> *__restrict__ ead,
                                                              const HArgs<S, T> *__restrict__ had,
                                                              const int *__restrict__ members,
@@ -1490,7 +1521,8 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                                                              const int *__restrict__ tend,
                                                              const SampB<S, T> *__restrict__ samp,
                                                              const int *__restrict__ sbeg,
-                                                             const TbAlt<S, T> *__restrict__ altd, int s0, int odd) {
+                                                             const TbAlt<S, T> *__restrict__ altd,
+                                                             const int4 *__restrict__ rect, int s0, int odd) {
     constexpr int CX = TBX / 64, RY = LY / TB_WAVES;
     constexpr int IX = TBX - 2 * TS - 1, IY = LY - 2 * TS - 1;
     extern __shared__ char lds_raw[];
@@ -1504,10 +1536,13 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
     const int w = blockIdx.x - wg0[q];
     const int Nx = ea.Nx, Ny = ea.Ny;
     const int ntx = (Nx + IX - 1) / IX, nty = (Ny + IY - 1) / IY;
-    if (w >= ntx * nty) return;
+    const int4 rc = rect[r];
+    const int nin = (rc.y - rc.x) * (rc.w - rc.z);
+    if (w >= (PML ? ntx * nty - nin : nin)) return;
     T *F = reinterpret_cast<T *>(lds_raw);
     T *F0 = F, *F1 = F + TBX * LY, *F2 = F + 2 * TBX * LY;      // TM: Ez Hx Hy; TE: Ex Ey Hz
-    const int bx = w % ntx, by = w / ntx;
+    int bx, by;
+    tb_tile(PML, rc, ntx, w, bx, by);
     const int ox = bx * IX, oy = by * IY;           // inner origin (global)
     const int tx = threadIdx.x, wv = threadIdx.y;
     const int wxs = ea.pml[0] + ea.pml[1];
@@ -1551,7 +1586,7 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         ikxE[cx] = ea.pf.ikE[0][g];
         ikxH[cx] = ha.pf.ikH[0][g];
         const T z = 0;
-        PX[cx] = x.px ? TbProf<T>{x.bEx[g], x.cEx[g], x.bHx[g], x.cHx[g]} : TbProf<T>{z, z, z, z};
+        PX[cx] = PML && x.px ? TbProf<T>{x.bEx[g], x.cEx[g], x.bHx[g], x.cHx[g]} : TbProf<T>{z, z, z, z};
     }
 #pragma unroll
     for (int ry = 0; ry < RY; ry++) {
@@ -1563,7 +1598,7 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         ikyE[ry] = ea.pf.ikE[1][g];
         ikyH[ry] = ha.pf.ikH[1][g];
         const T z = 0;
-        PY[ry] = x.py ? TbProf<T>{x.bEy[g], x.cEy[g], x.bHy[g], x.cHy[g]} : TbProf<T>{z, z, z, z};
+        PY[ry] = PML && x.py ? TbProf<T>{x.bEy[g], x.cEy[g], x.bHy[g], x.cHy[g]} : TbProf<T>{z, z, z, z};
     }
     // load: fields into LDS, material ids and psi into registers
     TbCoef<T> cf[CX * RY];
@@ -1580,7 +1615,9 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             const MatDev<T> &G = ea.mt[ea.mat[id]];
             cf[c] = TbCoef<T>{G.C1, G.C2x, G.C2y, G.C4};
             const T z = 0;
-            if (POL == 0) {
+            if (!PML) {
+                pa[c] = pb[c] = pc[c] = pd[c] = z;
+            } else if (POL == 0) {
                 pa[c] = nxE[cx] >= 0 ? psrc[0][nxE[cx] + wxs * gj[ry]] : z;
                 pb[c] = nyE[ry] >= 0 ? psrc[1][gi[cx] + Nx * nyE[ry]] : z;
                 pc[c] = nyH[ry] >= 0 ? psrc[2][gi[cx] + Nx * nyH[ry]] : z;
@@ -1650,12 +1687,12 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                 if (li < 1 || lj < 1) continue;
                 if (POL == 0) {
                     T e0 = F0[l], e1 = zero;
-                    tb_e_cell<0>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - 1], F1[l] - F1[l - TBX],
+                    tb_e_cell<0, PML>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - 1], F1[l] - F1[l - TBX],
                                  ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], PX[cx], PY[ry], pa[c], pb[c], hit[c] & 0xff);
                     F0[l] = e0;
                 } else {
                     T e0 = F0[l], e1 = F1[l];
-                    tb_e_cell<1>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - TBX], F2[l] - F2[l - 1],
+                    tb_e_cell<1, PML>(x, cf[c], gi[cx], gj[ry], t, e0, e1, F2[l] - F2[l - TBX], F2[l] - F2[l - 1],
                                  ikxE[cx], ikyE[ry], nxE[cx], nyE[ry], PX[cx], PY[ry], pa[c], pb[c], hit[c] & 0xff);
                     F0[l] = e0;
                     F1[l] = e1;
@@ -1672,13 +1709,13 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
                 if (POL == 0) {
                     T h0 = F1[l], h1 = F2[l];
                     const T ez = F0[l];
-                    tb_h_cell<0>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - ez, F0[l + 1] - ez, ikxH[cx], ikyH[ry],
+                    tb_h_cell<0, PML>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - ez, F0[l + 1] - ez, ikxH[cx], ikyH[ry],
                                  nxH[cx], nyH[ry], PX[cx], PY[ry], pc[c], pd[c]);
                     F1[l] = h0;
                     F2[l] = h1;
                 } else {
                     T h0 = F2[l], h1 = zero;
-                    tb_h_cell<1>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - F0[l], F1[l + 1] - F1[l], ikxH[cx],
+                    tb_h_cell<1, PML>(x, gi[cx], gj[ry], h0, h1, F0[l + TBX] - F0[l], F1[l + 1] - F1[l], ikxH[cx],
                                  ikyH[ry], nxH[cx], nyH[ry], PX[cx], PY[ry], pc[c], pd[c]);
                     F2[l] = h0;
                 }
@@ -1746,6 +1783,7 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
             dst[0].set(id, F0[l]);
             dst[1].set(id, F1[l]);
             dst[2].set(id, F2[l]);
+            if (!PML) continue;
             if (POL == 0) {
                 if (nxE[cx] >= 0) pdst[0][nxE[cx] + wxs * gj[ry]] = pa[c];
                 if (nyE[ry] >= 0) pdst[1][gi[cx] + Nx * nyE[ry]] = pb[c];
