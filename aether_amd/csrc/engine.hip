@@ -42,7 +42,7 @@ int fail(aefdtd_grid *g, int code, const std::string &msg) {
     return code;
 }
 
-std::string g_create_err;
+thread_local std::string g_create_err;   // the calling thread's last creation error
 std::mutex g_create_mu;
 
 }  // namespace
@@ -116,6 +116,18 @@ int upload_T(aefdtd_grid *g, void *dst, const double *src, size_t n) {
         HIPOK(g, hipStreamSynchronize(g->stream));
     }
     return AEFDTD_OK;
+}
+
+// Host <-> device copies and fills ordered on the grid's stream (never the
+// legacy null stream, which would tie the grid to every other stream of the
+// process, including one another thread is capturing into a graph).
+hipError_t copy_sync(aefdtd_grid *g, void *dst, const void *src, size_t n, hipMemcpyKind k) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, k, g->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g->stream);
+}
+hipError_t fill_sync(aefdtd_grid *g, void *dst, int v, size_t n) {
+    hipError_t e = hipMemsetAsync(dst, v, n, g->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g->stream);
 }
 
 // Dense host cell array (x fastest, Nx per row) <-> pitched device cell array.
@@ -443,7 +455,7 @@ int ensure_osc_tables(aefdtd_grid *g, int upto) {
         int rc = dalloc(g, &p, n * sizeof(double));
         if (rc) return rc;
         HIPOK(g, hipStreamSynchronize(g->stream));
-        HIPOK(g, hipMemcpy(p, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        HIPOK(g, copy_sync(g, p, v.data(), n * sizeof(double), hipMemcpyHostToDevice));
         if (o.val) {
             g->dev_bytes -= (size_t)o.nval * sizeof(double);
             void *old = (void *)o.val;
@@ -453,7 +465,7 @@ int ensure_osc_tables(aefdtd_grid *g, int upto) {
         o.nval = n;
         moved = true;
     }
-    if (moved) HIPOK(g, hipMemcpy(g->d_osc, g->osc.data(), g->osc.size() * sizeof(OscDev), hipMemcpyHostToDevice));
+    if (moved) HIPOK(g, copy_sync(g, g->d_osc, g->osc.data(), g->osc.size() * sizeof(OscDev), hipMemcpyHostToDevice));
     return AEFDTD_OK;
 }
 
@@ -744,7 +756,7 @@ int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
     HIPOK(g, hipStreamSynchronize(g->stream));
     dfree_t(s.d_coef);
     s.d_coef = (double2 *)p;
-    HIPOK(g, hipMemcpy(s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPOK(g, copy_sync(g, s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
     s.ncoef = n;
     return AEFDTD_OK;
 }
@@ -799,7 +811,7 @@ int prepare_stage(aefdtd_grid *g) {
         return rc;
     if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
     if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
-    HIPOK(g, hipMemcpy(g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
+    HIPOK(g, copy_sync(g, g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
     return AEFDTD_OK;
 }
 
@@ -1022,7 +1034,7 @@ struct aefdtd::Batch {
     size_t ldsE = 0;
     // time-skewed 2D TE/TM members: tile code (small/large tile height | depth
     // index << 2), -1 = off; member lists and workgroup prefixes per polarisation
-    int tb_code = 1 | (1 << 2);
+    int tb_code = 1 | (2 << 2);
     // [polarisation][tile class: 0 interior, 1 boundary]
     int *d_mem[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}, *d_wgT[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int *d_sbeg = nullptr;
@@ -1060,7 +1072,8 @@ template <typename V>
 int batch_upload(Batch *b, void **dst, const std::vector<V> &v) {
     if (v.empty()) return AEFDTD_OK;
     BHIPOK(b, hipMalloc(dst, v.size() * sizeof(V)));
-    BHIPOK(b, hipMemcpy(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+    BHIPOK(b, hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, b->stream));
+    BHIPOK(b, hipStreamSynchronize(b->stream));
     return AEFDTD_OK;
 }
 
@@ -1189,7 +1202,9 @@ int batch_build(Batch *b) {
             }
     // one chunk: the time-skewed blocks, KS x (E, H, samples) of the other
     // members, the DFT flush, the step advance
-    BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    // relaxed: other threads (the request server's builders) keep making
+    // allocation and copy calls on their own streams while this one captures
+    BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     if constexpr (std::is_same_v<S, T>) {
         for (int s0 = 0; s0 < KS; s0 += TS)
             for (int p = 0; p < 2; p++)
@@ -1312,6 +1327,8 @@ int batch_remove(Batch *b, aefdtd_grid *g) {
 }
 
 int batch_members(const Batch *b) { return (int)b->members.size(); }
+
+hipStream_t batch_stream(const Batch *b) { return b->stream; }
 
 int batch_step(Batch *b, int chunks) {
     if (!b || chunks < 0) return AEFDTD_EINVAL;
@@ -1817,7 +1834,7 @@ int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
     hipLaunchKernelGGL(k_poynting, dim3(s.Nl), dim3(256), 0, g->stream, (const double2 *)s.d_acc, s.P, s.Nl, f, s.type, d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
-    if (e == hipSuccess) e = hipMemcpy(rsp, d, s.Nl * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = copy_sync(g, rsp, d, s.Nl * sizeof(double), hipMemcpyDeviceToHost);
     void *v = d;
     g->dev_bytes -= s.Nl * sizeof(double);
     dfree(v);
@@ -1834,7 +1851,7 @@ int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]) {
     dims[0] = s.span1; dims[1] = s.span2; dims[2] = s.Nl;
     if (!out) return AEFDTD_OK;
     HIPOK(g, hipStreamSynchronize(g->stream));
-    HIPOK(g, hipMemcpy(out, s.d_acc, (size_t)6 * s.Nl * s.P * sizeof(double2), hipMemcpyDeviceToHost));
+    HIPOK(g, copy_sync(g, out, s.d_acc, (size_t)6 * s.Nl * s.P * sizeof(double2), hipMemcpyDeviceToHost));
     return AEFDTD_OK;
 }
 
@@ -1857,8 +1874,8 @@ int aefdtd_add_plane_sum(aefdtd_grid *g, int k, const aefdtd_pulse *inc, double 
     s->d_tap = (double *)p;
     if ((rc = dalloc(g, &p, (size_t)g->Nt * PS_BLOCKS * 6 * sizeof(double)))) { dfree_t(s->d_tap); delete s; return rc; }
     s->d_out = (double *)p;
-    HIPOK(g, hipMemcpy(s->d_tap, tap, g->Nt * sizeof(double), hipMemcpyHostToDevice));
-    HIPOK(g, hipMemset(s->d_out, 0, (size_t)g->Nt * PS_BLOCKS * 6 * sizeof(double)));
+    HIPOK(g, copy_sync(g, s->d_tap, tap, g->Nt * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(g, fill_sync(g, s->d_out, 0, (size_t)g->Nt * PS_BLOCKS * 6 * sizeof(double)));
     g->sums.push_back(s);
     *id = (int)g->sums.size() - 1;
     return AEFDTD_OK;
@@ -1871,7 +1888,7 @@ int aefdtd_read_plane_sum(aefdtd_grid *g, int id, int n, double *out) {
     int m = std::min(n, s.cap);
     HIPOK(g, hipStreamSynchronize(g->stream));
     std::vector<double> part((size_t)m * PS_BLOCKS * 6);
-    HIPOK(g, hipMemcpy(part.data(), s.d_out, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPOK(g, copy_sync(g, part.data(), s.d_out, part.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int t = 0; t < m; t++)
         for (int c = 0; c < 6; c++) {
             double v = 0;

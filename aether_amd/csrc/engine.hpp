@@ -76,6 +76,7 @@ int batch_remove(Batch *b, aefdtd_grid *g);
 int batch_members(const Batch *b);
 int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
 int batch_synchronize(Batch *b);
+hipStream_t batch_stream(const Batch *b);
 int batch_set_tiles(Batch *b, int tile_e, int tile_h);
 // Time-skewed passes of 2D TE/TM members: code = tile height index (0 small,
 // 1 large) | depth index (2, 4, 8 steps) << 2; -1 steps them one step at a time.

@@ -1,7 +1,11 @@
-// server.cpp -- the request server of the serving form: a worker thread that
-// admits FDTD requests under a device-memory budget and keeps every admitted
-// run resident, stepping all of them together in rounds of 64 steps until
-// their spectra are out (continuous batching).  The reference has no such
+// server.cpp -- the request server of the serving form.  Builder threads
+// admit requests in FIFO order while their estimated bytes fit a device-memory
+// budget and build their runs (host set-up and uploads on each run's own
+// stream); a worker thread keeps every admitted run resident and steps all of
+// them together in rounds of 64 steps until their spectra are out (continuous
+// batching).  Rounds are pipelined one deep: round R is queued on every stream
+// before the worker waits for round R-1, so the device always has a round of
+// work while the host admits and retires runs.  The reference has no such
 // layer: it runs one mode driver per call (modes/lua_fdtd.cpp:296-320).
 #include "runs.hpp"
 
@@ -22,6 +26,11 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kRound = 64;     // steps per round (one batch chunk)
+constexpr int kBuilders = 3;   // threads admitting and building runs
+// Runs stepped on their own streams in one round, in admission order, until
+// their cells reach this many (at least one run): enough concurrent work to
+// fill the device with small grids, without large grids contending for it.
+constexpr size_t kActiveCells = 48u << 20;
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
@@ -42,6 +51,9 @@ struct Request {
     size_t est = 0, bytes = 0;
     aefdtd_run *run = nullptr;
     aefdtd_batch *batch = nullptr;   // set when stepped by a batch
+    hipEvent_t ev[2] = {nullptr, nullptr};   // end of the run's rounds (even, odd) on its stream
+    bool ev_set[2] = {false, false};
+    long fin_round = -1;             // round that queued the run's last steps
     std::vector<double> out;
     Clock::time_point t_submit, t_admit, t_done;
     bool cancel = false;
@@ -88,6 +100,7 @@ struct aefdtd_server {
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Request *> queue;
+    std::vector<Request *> built;         // admitted and built, not yet stepped
     std::vector<Request *> resident;
     std::map<long, std::unique_ptr<Request>> all;
     long next_id = 1;
@@ -95,7 +108,10 @@ struct aefdtd_server {
     size_t used = 0, peak = 0;
     long rounds = 0, n_done = 0, n_failed = 0;
     aefdtd_batch *batch[3] = {nullptr, nullptr, nullptr};   // per precision
+    hipEvent_t bev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    bool bev_set[3][2] = {{false, false}, {false, false}, {false, false}};
     std::thread worker;
+    std::vector<std::thread> builders;
     std::string err;
 };
 
@@ -120,27 +136,39 @@ void finish_request(aefdtd_server *S, Request *q, int state, int rc, const std::
     S->cv.notify_all();
 }
 
-// Builds the run of a request (worker thread, lock not held).
-int admit(aefdtd_server *S, Request *q) {
+void free_events(Request *q) {
+    for (hipEvent_t &e : q->ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+}
+
+// Builds the run of a request (builder thread, lock not held).
+int build(Request *q) {
     int rc = aefdtd_run_create(&q->run, q->kind, &q->m, (int)q->sens.size(), q->sens.data());
     if (rc) {
         q->err = aefdtd_last_error(nullptr);
         return rc;
     }
     q->bytes = run_bytes(q->run);
-    bool batchable = q->kind == 2 && !q->run->g->n_disp;
-    if (batchable) {
-        int p = q->m.precision;
-        if (!S->batch[p] && (rc = aefdtd_batch_create(&S->batch[p], p))) {
-            q->err = "batch creation failed";
-            return rc;
-        }
-        if ((rc = aefdtd_batch_add(S->batch[p], q->run)) == AEFDTD_OK) q->batch = S->batch[p];
-        else q->err = aefdtd_batch_error(S->batch[p]);
-        // a run the batch refuses still runs on its own
-        rc = AEFDTD_OK;
+    return AEFDTD_OK;
+}
+
+// Puts a built run into the batch of its precision when the batched passes
+// can step it (worker thread: the batches are the worker's).  A run the batch
+// refuses still runs on its own.
+int join_batch(aefdtd_server *S, Request *q) {
+    if (q->kind != 2 || q->run->g->n_disp) return AEFDTD_OK;
+    const int p = q->m.precision;
+    int rc;
+    if (!S->batch[p] && (rc = aefdtd_batch_create(&S->batch[p], p))) {
+        q->err = "batch creation failed";
+        return rc;
     }
-    return rc;
+    if (aefdtd_batch_add(S->batch[p], q->run) == AEFDTD_OK) q->batch = S->batch[p];
+    else q->err = aefdtd_batch_error(S->batch[p]);
+    return AEFDTD_OK;
 }
 
 // Spectra of a finished run, then its device memory is released (worker thread).
@@ -168,6 +196,7 @@ int retire(Request *q) {
     }
     aefdtd_run_destroy(q->run);
     q->run = nullptr;
+    free_events(q);
     return rc;
 }
 
@@ -176,12 +205,28 @@ void drop(Request *q) {
     q->batch = nullptr;
     aefdtd_run_destroy(q->run);
     q->run = nullptr;
+    free_events(q);
 }
 
-void worker_loop(aefdtd_server *S) {
+// Marks the end of round R on a stream (event slot R mod 2).
+void mark(hipEvent_t &e, bool &set, hipStream_t st) {
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        e = nullptr;
+        return;
+    }
+    set = hipEventRecord(e, st) == hipSuccess;
+}
+
+void wait_mark(hipEvent_t e, bool &set) {
+    if (set) (void)hipEventSynchronize(e);
+    set = false;
+}
+
+// Admission: FIFO under the budget; builds each admitted run outside the lock.
+void builder_loop(aefdtd_server *S) {
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        // ---- cancelled requests leave the queue wherever they are
+        // cancelled requests leave the queue wherever they are
         for (auto it = S->queue.begin(); it != S->queue.end();) {
             if ((*it)->cancel) {
                 finish_request(S, *it, AEFDTD_REQ_CANCELLED, AEFDTD_ECANCELLED, "cancelled");
@@ -190,69 +235,115 @@ void worker_loop(aefdtd_server *S) {
                 ++it;
             }
         }
-        // ---- admission, FIFO, under the budget
-        while (!S->queue.empty()) {
-            Request *q = S->queue.front();
+        if (S->stop) break;
+        if (S->queue.empty()) {
+            S->cv.wait(lk);
+            continue;
+        }
+        Request *q = S->queue.front();
+        if (q->est > S->budget) {
+            S->queue.pop_front();
+            finish_request(S, q, AEFDTD_REQ_FAILED, AEFDTD_ENOMEM, "request exceeds the device-memory budget");
+            continue;
+        }
+        if (S->used + q->est > S->budget) {      // wait for a retirement
+            S->cv.wait(lk);
+            continue;
+        }
+        S->queue.pop_front();
+        S->used += q->est;                        // held while the run is built
+        lk.unlock();
+        int rc = build(q);
+        lk.lock();
+        S->used -= q->est;
+        if (rc) {
+            if (q->run) drop(q);
+            finish_request(S, q, AEFDTD_REQ_FAILED, rc, q->err.empty() ? "admission failed" : q->err);
+            continue;
+        }
+        S->used += std::max(q->bytes, q->est);
+        S->peak = std::max(S->peak, S->used);
+        q->state = AEFDTD_REQ_RUNNING;
+        q->t_admit = Clock::now();
+        S->built.push_back(q);
+        S->cv.notify_all();
+    }
+}
+
+void worker_loop(aefdtd_server *S) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        // ---- built runs join the resident set (cancelled ones are released)
+        std::vector<Request *> fresh;
+        fresh.swap(S->built);
+        for (Request *q : fresh) {
             if (q->cancel) {
-                S->queue.pop_front();
+                lk.unlock();
+                drop(q);
+                lk.lock();
+                S->used -= std::max(q->bytes, q->est);
                 finish_request(S, q, AEFDTD_REQ_CANCELLED, AEFDTD_ECANCELLED, "cancelled");
                 continue;
             }
-            if (q->est > S->budget) {
-                S->queue.pop_front();
-                finish_request(S, q, AEFDTD_REQ_FAILED, AEFDTD_ENOMEM, "request exceeds the device-memory budget");
-                continue;
-            }
-            if (S->used + q->est > S->budget) break;
-            S->queue.pop_front();
-            S->used += q->est;          // held while the run is built
             lk.unlock();
-            int rc = admit(S, q);
+            int rc = join_batch(S, q);
             lk.lock();
-            S->used -= q->est;
             if (rc) {
-                if (q->run) drop(q);
+                lk.unlock();
+                drop(q);
+                lk.lock();
+                S->used -= std::max(q->bytes, q->est);
                 finish_request(S, q, AEFDTD_REQ_FAILED, rc, q->err.empty() ? "admission failed" : q->err);
                 continue;
             }
-            S->used += std::max(q->bytes, q->est);
-            S->peak = std::max(S->peak, S->used);
-            q->state = AEFDTD_REQ_RUNNING;
-            q->t_admit = Clock::now();
             S->resident.push_back(q);
         }
         if (S->resident.empty()) {
-            if (S->stop) break;
-            S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (S->stop && S->built.empty()) break;
+            S->cv.wait(lk, [&] { return S->stop || !S->built.empty(); });
             continue;
         }
         std::vector<Request *> res = S->resident;
+        const long R = S->rounds;
         lk.unlock();
-        // ---- one round: every resident run advances up to 64 steps
+        // ---- queue round R: every resident run with steps left advances up to 64
         std::vector<std::pair<Request *, int>> failed;
-        for (aefdtd_batch *b : S->batch)
-            if (b && aefdtd_batch_running(b) > 0) {
-                int rc = aefdtd_batch_step(b, 1);
-                if (rc)
-                    for (Request *q : res)
-                        if (q->batch == b) failed.push_back({q, rc});
+        const int cur = (int)(R & 1), prev = cur ^ 1;
+        for (int p = 0; p < 3; p++) {
+            aefdtd_batch *b = S->batch[p];
+            if (!b || aefdtd_batch_running(b) == 0) continue;
+            int rc = aefdtd_batch_step(b, 1);
+            if (rc) {
+                for (Request *q : res)
+                    if (q->batch == b) failed.push_back({q, rc});
+            } else {
+                mark(S->bev[p][cur], S->bev_set[p][cur], batch_stream(b->b));
             }
+        }
+        size_t active = 0;
         for (Request *q : res) {
-            if (q->batch) continue;
+            if (q->batch || q->fin_round >= 0) continue;
+            if (active >= kActiveCells) break;
+            active += aefdtd_run_grid(q->run)->ncell();
             int rc = aefdtd_run_step(q->run, kRound);
             if (rc) failed.push_back({q, rc});
+            else mark(q->ev[cur], q->ev_set[cur], aefdtd_run_grid(q->run)->stream);
         }
-        for (aefdtd_batch *b : S->batch)
-            if (b) aefdtd_batch_synchronize(b);
-        for (Request *q : res)
-            if (!q->batch && q->run) aefdtd_synchronize(aefdtd_run_grid(q->run));
-        // ---- retire finished, failed and cancelled runs
+        for (Request *q : res) {
+            int t = 0, nt = 0;
+            aefdtd_run_progress(q->run, &t, &nt);
+            if (q->fin_round < 0 && t >= nt) q->fin_round = R;
+        }
+        // ---- wait for round R-1 (round R stays queued behind it)
+        for (int p = 0; p < 3; p++) wait_mark(S->bev[p][prev], S->bev_set[p][prev]);
+        for (Request *q : res) wait_mark(q->ev[prev], q->ev_set[prev]);
+        // ---- retire runs whose last steps were in an earlier round, failed and cancelled runs
         std::vector<std::pair<Request *, int>> ended;   // request, final state
         for (Request *q : res) {
             bool f = std::any_of(failed.begin(), failed.end(), [&](auto &p) { return p.first == q; });
-            int t = 0, nt = 0;
-            aefdtd_run_progress(q->run, &t, &nt);
-            if (f || q->cancel || t >= nt) ended.push_back({q, f ? AEFDTD_REQ_FAILED : (q->cancel ? AEFDTD_REQ_CANCELLED : AEFDTD_REQ_DONE)});
+            const bool done = q->fin_round >= 0 && q->fin_round < R;
+            if (f || q->cancel || done)
+                ended.push_back({q, f ? AEFDTD_REQ_FAILED : (q->cancel ? AEFDTD_REQ_CANCELLED : AEFDTD_REQ_DONE)});
         }
         std::vector<int> rcs;
         for (auto &e : ended) {
@@ -374,6 +465,7 @@ int aefdtd_server_create(aefdtd_server **out, size_t budget) {
     auto *S = new aefdtd_server();
     S->budget = budget;
     S->worker = std::thread(worker_loop, S);
+    for (int t = 0; t < kBuilders; t++) S->builders.emplace_back(builder_loop, S);
     *out = S;
     return AEFDTD_OK;
 }
@@ -386,8 +478,15 @@ int aefdtd_server_destroy(aefdtd_server *S) {
         S->stop = true;
     }
     S->cv.notify_all();
+    for (std::thread &t : S->builders) t.join();
     S->worker.join();
+    // a run built after the worker left is released here
+    for (Request *q : S->built) drop(q);
+    S->built.clear();
     for (aefdtd_batch *b : S->batch) aefdtd_batch_destroy(b);
+    for (auto &pair : S->bev)
+        for (hipEvent_t e : pair)
+            if (e) (void)hipEventDestroy(e);
     delete S;
     return AEFDTD_OK;
 }
