@@ -155,7 +155,7 @@ struct aefdtd_grid {
     void *d_strip = nullptr;          // [plane][Nz][Ny][2 columns][6] T
     bool strip_valid = false;
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
-    int tile_e = 2 | (2 << 2), tile_h = 2 | (2 << 2);
+    int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
     // bookkeeping
     bool ready = false;               // device copies up to date
     bool pml_dirty = true;

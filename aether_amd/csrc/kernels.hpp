@@ -15,8 +15,8 @@
 // field.  Each lane marches KZ planes in z and carries the previous plane's H
 // (E pass) / next plane's E (H pass) in registers.  Workgroups are launched
 // as a 1D grid; with `remap` set, the hardware workgroup id is folded so that
-// each of the 8 XCDs owns one contiguous range of tiles (x fastest, then y,
-// then z), which keeps the halo rows of neighbouring tiles in one L2.
+// each of the 8 XCDs takes chunks of a few consecutive tile rows (x fastest,
+// then y, then z), which keeps most halo rows of neighbouring tiles in one L2.
 // The per-material coefficients are staged once per workgroup in LDS.
 #pragma once
 
@@ -136,13 +136,22 @@ struct HArgs {
 };
 
 // Hardware workgroup id -> logical tile (bx, by, bz).  Workgroups are dealt to
-// the 8 XCDs round-robin; the fold gives XCD x the x-th contiguous range.
+// the 8 XCDs round-robin.  With the fold, the tiles are cut into chunks of
+// XCD_ROWS rows of tiles (x fastest) and chunk c goes to XCD c mod 8, so that a
+// tile's y-1 halo row was mostly loaded by the same XCD's L2 a moment before,
+// while the eight XCDs still stream neighbouring chunks at the same time (one
+// contiguous range per XCD keeps the halos too, but measured slower).  Tiles
+// past the last full round of 8 chunks keep the round-robin order.
+constexpr int XCD_ROWS = 4;
+
 __device__ __forceinline__ void tile_of(int w, int nbx, int nby, int nbz, int remap, int &bx, int &by, int &bz) {
     if (remap) {
-        const int B = nbx * nby * nbz;
-        const int q = B / 8, r = B % 8;
-        const int x = w % 8, loc = w / 8;
-        w = (x < r) ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+        const int CH = nbx * XCD_ROWS;
+        const int full = nbx * nby * nbz / (8 * CH);
+        if (w < full * 8 * CH) {
+            const int x = w % 8, i = w / 8;
+            w = ((i / CH) * 8 + x) * CH + i % CH;
+        }
     }
     bx = w % nbx;
     const int t = w / nbx;
