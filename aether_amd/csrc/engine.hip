@@ -373,37 +373,46 @@ void prof_collect(aefdtd_grid *g) {
 constexpr int kBY[4] = {1, 2, 4, 8};
 constexpr int kKZ[4] = {1, 2, 4, 8};
 
+// A 3D launch of nb tile workgroups; `aux` (a 1 x 1 x Nz column, or nullptr)
+// is updated by extra workgroups of the same launch.
 template <typename S, typename T, int BY, int KZ>
-void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st) {
-    if (a.Nz == 1)
+void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st, const EArgs<S, T> *aux) {
+    if (a.Nz == 1) {
         hipLaunchKernelGGL((k_update_E2<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
-    else
-        hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), a.nmat * sizeof(MatL<T>), st, a);
+    } else {
+        const int naux = aux ? (aux->Nz + BX * BY - 1) / (BX * BY) : 0;
+        const int nmat = aux ? std::max(a.nmat, aux->nmat) : a.nmat;
+        hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks + naux), dim3(BX, BY, 1), nmat * sizeof(MatL<T>), st,
+                           a, aux ? *aux : a, nblocks);
+    }
 }
 template <typename S, typename T, int BY, int KZ>
-void launch_H_cfg(const HArgs<S, T> &a, int nblocks, hipStream_t st) {
-    if (a.Nz == 1)
+void launch_H_cfg(const HArgs<S, T> &a, int nblocks, hipStream_t st, const HArgs<S, T> *aux) {
+    if (a.Nz == 1) {
         hipLaunchKernelGGL((k_update_H2<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_update_H<S, T, BY, KZ>), dim3(nblocks), dim3(BX, BY, 1), 0, st, a);
+    } else {
+        const int naux = aux ? (aux->Nz + BX * BY - 1) / (BX * BY) : 0;
+        hipLaunchKernelGGL((k_update_H<S, T, BY, KZ>), dim3(nblocks + naux), dim3(BX, BY, 1), 0, st, a, aux ? *aux : a,
+                           nblocks);
+    }
 }
 
 template <typename S, typename T, int BY>
-void launch_E_by(const EArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
+void launch_E_by(const EArgs<S, T> &a, int kzi, int nb, hipStream_t st, const EArgs<S, T> *aux) {
     switch (kzi) {
-        case 0: launch_E_cfg<S, T, BY, 1>(a, nb, st); break;
-        case 1: launch_E_cfg<S, T, BY, 2>(a, nb, st); break;
-        case 2: launch_E_cfg<S, T, BY, 4>(a, nb, st); break;
-        default: launch_E_cfg<S, T, BY, 8>(a, nb, st); break;
+        case 0: launch_E_cfg<S, T, BY, 1>(a, nb, st, aux); break;
+        case 1: launch_E_cfg<S, T, BY, 2>(a, nb, st, aux); break;
+        case 2: launch_E_cfg<S, T, BY, 4>(a, nb, st, aux); break;
+        default: launch_E_cfg<S, T, BY, 8>(a, nb, st, aux); break;
     }
 }
 template <typename S, typename T, int BY>
-void launch_H_by(const HArgs<S, T> &a, int kzi, int nb, hipStream_t st) {
+void launch_H_by(const HArgs<S, T> &a, int kzi, int nb, hipStream_t st, const HArgs<S, T> *aux) {
     switch (kzi) {
-        case 0: launch_H_cfg<S, T, BY, 1>(a, nb, st); break;
-        case 1: launch_H_cfg<S, T, BY, 2>(a, nb, st); break;
-        case 2: launch_H_cfg<S, T, BY, 4>(a, nb, st); break;
-        default: launch_H_cfg<S, T, BY, 8>(a, nb, st); break;
+        case 0: launch_H_cfg<S, T, BY, 1>(a, nb, st, aux); break;
+        case 1: launch_H_cfg<S, T, BY, 2>(a, nb, st, aux); break;
+        case 2: launch_H_cfg<S, T, BY, 4>(a, nb, st, aux); break;
+        default: launch_H_cfg<S, T, BY, 8>(a, nb, st, aux); break;
     }
 }
 
@@ -573,7 +582,7 @@ HArgs<S, T> h_args(aefdtd_grid *g) {
 }
 
 template <typename S, typename T>
-void dispatch_E(aefdtd_grid *g, EArgs<S, T> &a, int cfg) {
+void dispatch_E(aefdtd_grid *g, EArgs<S, T> &a, int cfg, const EArgs<S, T> *aux = nullptr) {
     if (is_column(g)) {
         hipLaunchKernelGGL((k_update_E_col<S, T>), dim3((g->Nz + 255) / 256), dim3(256), a.nmat * sizeof(MatL<T>),
                            g->stream, a);
@@ -581,25 +590,25 @@ void dispatch_E(aefdtd_grid *g, EArgs<S, T> &a, int cfg) {
     }
     const int nb = tile_geometry(a, cfg);
     switch (cfg & 3) {
-        case 0: launch_E_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 1: launch_E_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 2: launch_E_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        default: launch_E_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 0: launch_E_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        case 1: launch_E_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        case 2: launch_E_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        default: launch_E_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
     }
 }
 
 template <typename S, typename T>
-void dispatch_H(aefdtd_grid *g, HArgs<S, T> &a, int cfg) {
+void dispatch_H(aefdtd_grid *g, HArgs<S, T> &a, int cfg, const HArgs<S, T> *aux = nullptr) {
     if (is_column(g)) {
         hipLaunchKernelGGL((k_update_H_col<S, T>), dim3((g->Nz + 255) / 256), dim3(256), 0, g->stream, a);
         return;
     }
     const int nb = tile_geometry(a, cfg);
     switch (cfg & 3) {
-        case 0: launch_H_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 1: launch_H_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        case 2: launch_H_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream); break;
-        default: launch_H_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream); break;
+        case 0: launch_H_by<S, T, 1>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        case 1: launch_H_by<S, T, 2>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        case 2: launch_H_by<S, T, 4>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
+        default: launch_H_by<S, T, 8>(a, (cfg >> 2) & 3, nb, g->stream, aux); break;
     }
 }
 
@@ -623,28 +632,33 @@ int launch_rc(aefdtd_grid *g) {
     return AEFDTD_OK;
 }
 
+// E update of g (and of its TFSF column aux in the same launch, when given).
 template <typename S, typename T>
-int launch_E(aefdtd_grid *g) {
+int launch_E(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
     int rc = ensure_osc_tables(g, g->tstep + 1);
     if (rc) return rc;
+    if (aux && (rc = ensure_osc_tables(aux, aux->tstep + 1))) return rc;
+    if (aux && (rc = launch_rc<S, T>(aux))) return rc;
     rc = launch_rc<S, T>(g);
     if (rc) return rc;
     g->strip_valid = false;             // E changes: the strip is stale until the next H pass
-    EArgs<S, T> a = e_args<S, T>(g);
+    EArgs<S, T> a = e_args<S, T>(g), x{};
+    if (aux) x = e_args<S, T>(aux);
     hipEvent_t ev;
     prof_begin(g, 0, &ev);
-    dispatch_E<S, T>(g, a, g->tile_e);
+    dispatch_E<S, T>(g, a, g->tile_e, aux ? &x : nullptr);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;
 }
 
 template <typename S, typename T>
-int launch_H(aefdtd_grid *g) {
-    HArgs<S, T> a = h_args<S, T>(g);
+int launch_H(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
+    HArgs<S, T> a = h_args<S, T>(g), x{};
+    if (aux) x = h_args<S, T>(aux);
     hipEvent_t ev;
     prof_begin(g, 1, &ev);
-    dispatch_H<S, T>(g, a, g->tile_h);
+    dispatch_H<S, T>(g, a, g->tile_h, aux ? &x : nullptr);
     prof_end(g, ev);
     HIPOK(g, hipGetLastError());
     // the 3D pass wrote the strip from the fields it left (the column kernel
@@ -1647,6 +1661,55 @@ int aefdtd_update_H(aefdtd_grid *g) {
     g->tstep += 1;
     return AEFDTD_OK;
 }
+
+}  // extern "C"
+
+namespace aefdtd {
+
+// A grid and its TFSF column stepped together: the column's E and H updates
+// ride in the grid's E and H launches (two launches per step instead of four).
+// The column must be g's attached TFSF grid on g's stream; otherwise the two
+// are stepped one after the other as through the C ABI.
+bool pairable(const aefdtd_grid *g, const aefdtd_grid *aux) {
+    return aux && g->tfsf_aux == aux && is_column(aux) && !is_column(g) && g->Nz > 1 && aux->stream == g->stream &&
+           aux->precision == g->precision;
+}
+
+int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux) {
+    if (!g || !aux) return AEFDTD_EINVAL;
+    if (!pairable(g, aux)) {
+        int rc = aefdtd_update_E(aux);
+        return rc ? rc : aefdtd_update_E(g);
+    }
+    for (aefdtd_grid *q : {aux, g}) {
+        int rc = ensure_ready(q);
+        if (rc) return rc;
+        if (q->tstep == 0 || q->pml_dirty) {
+            if ((rc = upload_profiles(q))) return rc;
+            q->pml_dirty = false;
+        }
+    }
+    return with_types(g, [&]<typename S, typename T>() { return launch_E<S, T>(g, aux); });
+}
+
+int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux) {
+    if (!g || !aux) return AEFDTD_EINVAL;
+    if (!pairable(g, aux)) {
+        int rc = aefdtd_update_H(aux);
+        return rc ? rc : aefdtd_update_H(g);
+    }
+    int rc = ensure_ready(aux);
+    if (rc || (rc = ensure_ready(g))) return rc;
+    rc = with_types(g, [&]<typename S, typename T>() { return launch_H<S, T>(g, aux); });
+    if (rc) return rc;
+    aux->tstep += 1;
+    g->tstep += 1;
+    return AEFDTD_OK;
+}
+
+}  // namespace aefdtd
+
+extern "C" {
 
 int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;

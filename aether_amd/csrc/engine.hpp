@@ -76,6 +76,9 @@ int batch_remove(Batch *b, aefdtd_grid *g);
 int batch_members(const Batch *b);
 int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
 int batch_synchronize(Batch *b);
+// A grid and its attached TFSF column stepped with one launch per half step.
+int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux);
+int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux);
 hipStream_t batch_stream(const Batch *b);
 int batch_set_tiles(Batch *b, int tile_e, int tile_h);
 // Time-skewed passes of 2D TE/TM members: code = tile height index (0 small,

@@ -672,10 +672,22 @@ __global__ __launch_bounds__(BX *BY) void k_update_E2(const EArgs<S, T> a) {
     e_tile2d<S, T, BY, KY>(a, sm, blockIdx.x, a.ostep);
 }
 
+template <typename S, typename T>
+__device__ __forceinline__ void e_col_cell(const EArgs<S, T> &a, const MatL<T> *sm, int k);
+
+// 3D E pass; workgroups from nb on update the 1 x 1 x Nz column `aux` instead
+// (the TFSF source grid, stepped in the same launch: it neither reads nor
+// writes anything the 3D pass touches).
 template <typename S, typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a) {
-    extern __shared__ char lds_raw[];          // nmat entries, sized at launch
+__global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a, const EArgs<S, T> aux, int nb) {
+    extern __shared__ char lds_raw[];          // max(nmat) entries, sized at launch
     MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    if ((int)blockIdx.x >= nb) {
+        stage_mats(aux, sm);
+        const int k = ((int)blockIdx.x - nb) * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+        if (k < aux.Nz) e_col_cell(aux, sm, k);
+        return;
+    }
     stage_mats(a, sm);
     e_tile<S, T, BY, KZ>(a, sm, blockIdx.x, a.ostep);
 }
@@ -717,12 +729,7 @@ __global__ __launch_bounds__(BX *BY) void k_update_E_batch(const EArgs<S, T> *__
 // E pass of a 1 x 1 x Nz column (the auxiliary grid of the TFSF source):
 // one lane per cell.
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
-    extern __shared__ char lds_raw[];
-    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
-    stage_mats(a, sm);
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.Nz) return;
+__device__ __forceinline__ void e_col_cell(const EArgs<S, T> &a, const MatL<T> *sm, int k) {
     const int km = (k == 0) ? a.Nz - 1 : k - 1;
     const T hx = a.Hx[k], hy = a.Hy[k];
     const int m = a.mat[k];
@@ -730,6 +737,15 @@ __global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
     e_cell(a, sm[m], m, 0, 0, k, k, a.Ex[k], a.Ey[k], a.Ez[k], zero, hy - a.Hy[km], hx - a.Hx[km], zero, zero,
            zero, a.pf.ikE[0][0], a.pf.ikE[1][0], a.pf.ikE[2][k], -1, -1, a.ostep, side_all(a), a.lm,
            GlobalPsiE<S, T>{a, 0, 0, k});
+}
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_update_E_col(const EArgs<S, T> a) {
+    extern __shared__ char lds_raw[];
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    stage_mats(a, sm);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.Nz) e_col_cell(a, sm, k);
 }
 
 // ---------------------------------------------------------------- H cell
@@ -1048,8 +1064,17 @@ __global__ __launch_bounds__(BX *BY) void k_update_H2(const HArgs<S, T> a) {
     h_tile2d<S, T, BY, KY>(a, blockIdx.x);
 }
 
+template <typename S, typename T>
+__device__ __forceinline__ void h_col_cell(const HArgs<S, T> &a, int k);
+
+// 3D H pass; workgroups from nb on update the column `aux` (see k_update_E).
 template <typename S, typename T, int BY, int KZ>
-__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a) {
+__global__ __launch_bounds__(BX *BY) void k_update_H(const HArgs<S, T> a, const HArgs<S, T> aux, int nb) {
+    if ((int)blockIdx.x >= nb) {
+        const int k = ((int)blockIdx.x - nb) * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+        if (k < aux.Nz) h_col_cell(aux, k);
+        return;
+    }
     h_tile<S, T, BY, KZ>(a, blockIdx.x);
 }
 
@@ -1067,13 +1092,17 @@ __global__ __launch_bounds__(BX *BY) void k_update_H_batch(const HArgs<S, T> *__
 }
 
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_update_H_col(const HArgs<S, T> a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.Nz) return;
+__device__ __forceinline__ void h_col_cell(const HArgs<S, T> &a, int k) {
     const int kp = (k == a.Nz - 1) ? 0 : k + 1;
     const T zero = 0;
     h_cell(a, 0, 0, k, k, a.Hx[k], a.Hy[k], a.Hz[k], a.Ey[kp] - a.Ey[k], a.Ex[kp] - a.Ex[k], zero, zero, zero,
            zero, a.pf.ikH[0][0], a.pf.ikH[1][0], a.pf.ikH[2][k], -1, -1, side_all(a), a.lm, GlobalPsiH<S, T>{a, 0, 0, k});
+}
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_update_H_col(const HArgs<S, T> a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.Nz) h_col_cell(a, k);
 }
 
 // ================================================================ sensors

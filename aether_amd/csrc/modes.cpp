@@ -435,9 +435,10 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
     for (int s = 0; s < n && r->t < r->Nt; s++) {
         int rc;
         if (r->kind == 1) {
-            // single_particle.cpp:256-381: aux E, main E (+TFSF-E), aux H, main H (+TFSF-H), sensors
-            if ((rc = aefdtd_update_E(r->a)) || (rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->a)) ||
-                (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
+            // single_particle.cpp:256-381: aux E, main E (+TFSF-E), aux H, main H (+TFSF-H), sensors;
+            // each aux column update rides in the main grid's launch of the same half step
+            if ((rc = aefdtd::update_E_pair(r->g, r->a)) || (rc = aefdtd::update_H_pair(r->g, r->a)) ||
+                (rc = aefdtd_feed_sensors(r->g)))
                 return rfail(r, rc);
         } else {
             // normal_incidence.cpp:325-415: E (+injection), H (+injection), sensors + R/T sums;
