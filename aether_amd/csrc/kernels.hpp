@@ -1186,40 +1186,45 @@ __global__ __launch_bounds__(256) void k_sample_batch(const SampB<S, T> *__restr
 }
 
 // Running DFT over a block of nfill <= KS staged steps (sensors/sensors.cpp:
-// 456-479): acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p], FP64.
+// 456-479): acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p].  The
+// accumulators are FP64; the block's KS-term sums are formed in the field
+// arithmetic type T (FP64 for FP64 runs, so FP64 runs keep the reference's
+// double sums; FP32 for FP32/FP16 runs, whose samples are FP32 already).
 // One lane per (cell, component) keeps its KS samples in registers; the
 // coefficients of DFT_LCH wavelengths at a time are staged in LDS and read as
 // broadcasts, so each accumulator is read and written once per KS steps.
 template <typename T>
 __device__ __forceinline__ void dft_block(const T *stage, long Ptot, long off, long P, int Nl, const double2 *coef,
                                           int s0, int nfill, double2 *acc, int c) {
-    __shared__ double2 W[KS * DFT_LCH];
+    using T2 = std::conditional_t<std::is_same_v<T, double>, double2, float2>;
+    __shared__ T2 W[KS * DFT_LCH];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int h = c >= 3 ? 1 : 0;
     const bool live = p < P;
-    double x[KS];
+    T x[KS];
 #pragma unroll
-    for (int s = 0; s < KS; s++) x[s] = (live && s < nfill) ? (double)stage[((size_t)s * 6 + c) * Ptot + off + p] : 0.0;
+    for (int s = 0; s < KS; s++) x[s] = (live && s < nfill) ? stage[((size_t)s * 6 + c) * Ptot + off + p] : (T)0;
     for (int l0 = 0; l0 < Nl; l0 += DFT_LCH) {
         const int nl = min(DFT_LCH, Nl - l0);
         __syncthreads();
         for (int q = threadIdx.x; q < KS * DFT_LCH; q += blockDim.x) {
             const int s = q / DFT_LCH, l = q % DFT_LCH;
-            W[q] = (s < nfill && l < nl) ? coef[((size_t)(s0 + s) * 2 + h) * Nl + l0 + l] : make_double2(0.0, 0.0);
+            const double2 w = (s < nfill && l < nl) ? coef[((size_t)(s0 + s) * 2 + h) * Nl + l0 + l] : make_double2(0.0, 0.0);
+            W[q] = T2{(T)w.x, (T)w.y};
         }
         __syncthreads();
         if (live) {
             for (int l = 0; l < nl; l++) {
-                double re = 0, im = 0;
+                T re = 0, im = 0;
 #pragma unroll
                 for (int s = 0; s < KS; s++) {
-                    const double2 w = W[s * DFT_LCH + l];
+                    const T2 w = W[s * DFT_LCH + l];
                     re += w.x * x[s];
                     im += w.y * x[s];
                 }
                 double2 &A = acc[((size_t)c * Nl + l0 + l) * P + p];
-                A.x += re;
-                A.y += im;
+                A.x += (double)re;
+                A.y += (double)im;
             }
         }
     }
