@@ -122,6 +122,7 @@ def _sig(lib, p):
         "run_error": (C.c_char_p, [vp]),
         "run_finish": (i, [vp, C.POINTER(CNormalResult), _dp, _dp]),
         "run_destroy": (i, [vp]),
+        "run_write": (i, [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), _dp, C.POINTER(CNormalResult)]),
         "synchronize": (i, [vp]),
         "profile_enable": (i, [vp, i]),
         "profile_kinds": (i, [vp, i]),
@@ -554,7 +555,20 @@ class Run:
         out["sensors"] = sout[:nsout]
         if fields is not None:
             out["fields"] = fields
+        self._finished = (res, sout)
         return out
+
+    def write(self, directory, prefix="", names=None):
+        """The result files of the finished run in the reference's formats
+        (aefdtd_run_write); names: one per sensor request."""
+        res, sout = self._finished
+        arr = None
+        if names is not None:
+            arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        rc = self.be.fn("run_write")(self.h, str(directory).encode(), prefix.encode(), arr, _dptr(sout),
+                                     C.byref(res) if res is not None else None)
+        if rc != OK:
+            raise FDTDError(rc, (self.be.fn("run_error")(self.h) or b"").decode())
 
     def close(self):
         if self.h:

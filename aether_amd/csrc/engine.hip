@@ -1882,6 +1882,17 @@ int aefdtd_feed_sensors(aefdtd_grid *g) {
     return with_types(g, [&]<typename S, typename T>() { return feed_T<S, T>(g); });
 }
 
+}  // extern "C"
+
+int aefdtd::sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2) {
+    if (!g || id < 0 || id >= (int)g->sensors.size()) return AEFDTD_EINVAL;
+    *span1 = g->sensors[id]->span1;
+    *span2 = g->sensors[id]->span2;
+    return AEFDTD_OK;
+}
+
+extern "C" {
+
 int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
     if (!g || !rsp) return AEFDTD_EINVAL;
     if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");

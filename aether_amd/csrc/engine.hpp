@@ -76,6 +76,8 @@ int batch_remove(Batch *b, aefdtd_grid *g);
 int batch_members(const Batch *b);
 int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
 int batch_synchronize(Batch *b);
+// Plane extent (cells) of sensor id after its link to the grid.
+int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
 // A grid and its attached TFSF column stepped with one launch per half step.
 int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux);
 int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux);

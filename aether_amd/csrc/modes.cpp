@@ -14,7 +14,12 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <complex>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <string>
 #include <vector>
 
 using namespace aefdtd;
@@ -480,6 +485,81 @@ int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res, double *sensor_o
 int aefdtd_run_destroy(aefdtd_run *r) {
     if (r && r->batch) aefdtd_batch_remove(r->batch, r);
     delete r;
+    return AEFDTD_OK;
+}
+
+// Result files as the reference writes them; the same stream formatting
+// (std::ostream, setprecision(15) or the default 6 digits) on the same values.
+int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, const char *const *names,
+                     const double *sensor_out, const aefdtd_normal_result *res) {
+    if (!r || !directory || !r->g) return AEFDTD_EINVAL;
+    namespace fs = std::filesystem;
+    const fs::path dir(directory);
+    auto open = [&](const fs::path &p, std::ofstream &f) {
+        f.open(p, std::ios::out | std::ios::trunc);
+        if (!f) r->err = "cannot write " + p.string();
+        return (bool)f;
+    };
+    const aefdtd_grid *g = r->g;
+    size_t off = 0;
+    for (size_t q = 0; q < r->reqs.size() && sensor_out; q++) {
+        const aefdtd_sensor_req &s = r->reqs[q];
+        const int Nl = s.Nl;
+        std::vector<double> L;
+        lambda_grid(Nl, s.lambda_min, s.lambda_max, L);
+        const std::string name = names && names[q] ? names[q] : "sensor" + std::to_string(q);
+        const double *v = sensor_out + off;
+        if (s.box) {
+            // Box_Spect_Poynting::treat (sensors/s_spec_power.cpp:174-232)
+            std::ofstream f, fm;
+            if (!open(dir / (name + "_bspdft"), f) || !open(dir / (name + ".m"), fm)) return AEFDTD_EINVAL;
+            for (int l = 0; l < Nl; l++) {
+                f << std::setprecision(15) << L[l] << " " << v[l];
+                for (int face = 0; face < 6; face++)
+                    if (!s.disable[face]) f << std::setprecision(15) << " " << v[(size_t)(1 + face) * Nl + l];
+                f << std::endl;
+            }
+            const fs::path core(name);
+            fm << "function [L,tot]=" << core.stem() << "\n";
+            fm << "   data=dlmread('" << core.stem() << "_bspdft');\n";
+            fm << "   L=data(:,1);\n";
+            fm << "   tot=data(:,2);\n";
+            fm << "end";
+            off += 7 * (size_t)Nl;
+        } else {
+            // Spect_Poynting::treat (sensors/s_spec_power.cpp:44-118)
+            int span1 = 0, span2 = 0;
+            TRY(aefdtd::sensor_spans(g, r->ids[q][0], &span1, &span2));
+            double area = span1 * span2;
+            const int t = s.orientation;
+            if (t == AEFDTD_NORMAL_X || t == AEFDTD_NORMAL_XM) area *= g->Dy * g->Dz;
+            else if (t == AEFDTD_NORMAL_Y || t == AEFDTD_NORMAL_YM) area *= g->Dx * g->Dz;
+            else area *= g->Dx * g->Dy;
+            std::ofstream f;
+            if (!open(dir / (name + "_pspdft"), f)) return AEFDTD_EINVAL;
+            for (int l = 0; l < Nl; l++) f << std::setprecision(15) << L[l] << " " << v[l] << " " << v[l] / area << std::endl;
+            off += Nl;
+        }
+    }
+    if (r->kind == 0 && res && res->lambda) {
+        // FDTD_normal_incidence's spectra (modes/normal_incidence.cpp:490-557)
+        const std::string pre = prefix ? prefix : "";
+        std::ofstream f1, f2;
+        if (!open(dir / (pre + "spectdata_norm"), f1) || !open(dir / (pre + "spectdata_norm2"), f2))
+            return AEFDTD_EINVAL;
+        using C = std::complex<double>;
+        for (int l = 0; l < r->Nl; l++) {
+            f1 << res->lambda[l] << " " << res->R[l] << " " << res->T[l] << std::endl;
+            const C rx(res->refx[2 * l], res->refx[2 * l + 1]), ry(res->refy[2 * l], res->refy[2 * l + 1]);
+            const C tx(res->transx[2 * l], res->transx[2 * l + 1]), ty(res->transy[2 * l], res->transy[2 * l + 1]);
+            const double prx = std::norm(rx), pry = std::norm(ry);
+            const double ptx = std::norm(tx) * res->index_sub / res->index_sup;
+            const double pty = std::norm(ty) * res->index_sub / res->index_sup;
+            f2 << res->lambda[l] << " " << std::abs(rx) << " " << std::arg(rx) << " " << prx << " " << std::abs(ry)
+               << " " << std::arg(ry) << " " << pry << " " << std::abs(tx) << " " << std::arg(tx) << " " << ptx << " "
+               << std::abs(ty) << " " << std::arg(ty) << " " << pty << std::endl;
+        }
+    }
     return AEFDTD_OK;
 }
 

@@ -302,6 +302,22 @@ const char *aefdtd_run_error(const aefdtd_run *r);
 int aefdtd_run_finish(aefdtd_run *r, aefdtd_normal_result *res,
                       double *sensor_out, double *fields_out);
 int aefdtd_run_destroy(aefdtd_run *r);
+/* The result files of a finished run, in the reference's formats, into
+ * `directory` (call after aefdtd_run_finish, before aefdtd_run_destroy, with
+ * what finish returned):
+ *   planar sensor q:  <names[q]>_pspdft  "lambda flux flux/area", 15 significant
+ *                     digits (Spect_Poynting::treat, sensors/s_spec_power.cpp:44-118);
+ *   box sensor q:     <names[q]>_bspdft  "lambda total xm xp ym yp zm zp" (enabled
+ *                     faces only), 15 digits, and the Octave reader <names[q]>.m
+ *                     (Box_Spect_Poynting::treat, s_spec_power.cpp:174-232);
+ *   normal incidence: <prefix>spectdata_norm "lambda R T" and <prefix>spectdata_norm2
+ *                     "lambda |rx| arg(rx) Prx |ry| arg(ry) Pry |tx| arg(tx) Ptx |ty|
+ *                     arg(ty) Pty", 6 significant digits
+ *                     (modes/normal_incidence.cpp:490-570).
+ * names: one per sensor request (NULL: "sensor<q>"); sensor_out and res may be
+ * NULL to skip the sensor or the normal-incidence files. */
+int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, const char *const *names,
+                     const double *sensor_out, const aefdtd_normal_result *res);
 /* Waits for all work queued on the grid's stream. */
 int aefdtd_synchronize(aefdtd_grid *g);
 
