@@ -61,7 +61,8 @@ class CMode(C.Structure):
                 ("matsgrid", C.POINTER(C.c_uint32)),
                 ("n_materials", C.c_int), ("materials", C.POINTER(CMaterial)),
                 ("precision", C.c_int),
-                ("n_sources", C.c_int), ("sources", C.POINTER(CSourceReq))]
+                ("n_sources", C.c_int), ("sources", C.POINTER(CSourceReq)),
+                ("time_type", C.c_int), ("cc_step", C.c_int), ("cc_coeff", C.c_double)]
 
 
 class CSensorReq(C.Structure):
@@ -315,6 +316,9 @@ class ModeSpec:
     disable_fields: tuple = ()
     precision: int = FP64
     sources: list = field(default_factory=list)   # SourceReq, custom mode
+    time_type: int = 0                        # 1: auto_tsteps with the energy criterion
+    cc_step: int = 100
+    cc_coeff: float = 1e-3
 
     def to_c(self, keep):
         mg = np.ascontiguousarray(self.matsgrid, dtype=np.uint32)
@@ -346,6 +350,7 @@ class ModeSpec:
         keep.append(srcs)
         c.n_sources = len(self.sources)
         c.sources = srcs
+        c.time_type, c.cc_step, c.cc_coeff = self.time_type, self.cc_step, self.cc_coeff
         return c
 
 

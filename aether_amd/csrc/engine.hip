@@ -1529,6 +1529,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree(g->d_stage);
     dfree(g->d_sdesc);
     dfree(g->d_strip);
+    dfree_t(g->d_energy);
     dfree(g->d_osc);
     for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
@@ -1883,6 +1884,35 @@ int aefdtd_feed_sensors(aefdtd_grid *g) {
 }
 
 }  // extern "C"
+
+int aefdtd::energy_feed(aefdtd_grid *g) {
+    if (!g) return AEFDTD_EINVAL;
+    if (!g->d_energy) {
+        void *p;
+        int rc = dalloc(g, &p, (EN_BLOCKS + 2) * sizeof(double));
+        if (rc) return rc;
+        g->d_energy = (double *)p;
+        HIPOK(g, fill_sync(g, g->d_energy, 0, (EN_BLOCKS + 2) * sizeof(double)));
+    }
+    with_types(g, [&]<typename S, typename T>() {
+        hipLaunchKernelGGL((k_energy_part<S, T>), dim3(EN_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0),
+                           fld<S, T>(g, 1), fld<S, T>(g, 2), g->Nx, g->Ny * g->Nz, g->px, g->d_energy);
+        return AEFDTD_OK;
+    });
+    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(256), 0, g->stream, (const double *)g->d_energy,
+                       g->d_energy + EN_BLOCKS);
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+int aefdtd::energy_read(aefdtd_grid *g, double *last, double *max) {
+    if (!g || !g->d_energy) return AEFDTD_EINVAL;
+    double em[2];
+    HIPOK(g, copy_sync(g, em, g->d_energy + EN_BLOCKS, sizeof(em), hipMemcpyDeviceToHost));
+    *last = em[0];
+    *max = em[1];
+    return AEFDTD_OK;
+}
 
 int aefdtd::sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2) {
     if (!g || id < 0 || id >= (int)g->sensors.size()) return AEFDTD_EINVAL;

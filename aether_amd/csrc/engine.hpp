@@ -76,6 +76,10 @@ int batch_remove(Batch *b, aefdtd_grid *g);
 int batch_members(const Batch *b);
 int batch_step(Batch *b, int chunks);          // advances every member by up to chunks x 64 steps
 int batch_synchronize(Batch *b);
+// Completion energy of the current E (sum of E^2 over the grid): energy_feed
+// queues it and updates the running maximum; energy_read waits and reads both.
+int energy_feed(aefdtd_grid *g);
+int energy_read(aefdtd_grid *g, double *last, double *max);
 // Plane extent (cells) of sensor id after its link to the grid.
 int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
 // A grid and its attached TFSF column stepped with one launch per half step.
@@ -159,6 +163,7 @@ struct aefdtd_grid {
     std::vector<int> strip_cols;      // x1 of each strip plane (columns x1, x1+1)
     void *d_strip = nullptr;          // [plane][Nz][Ny][2 columns][6] T
     bool strip_valid = false;
+    double *d_energy = nullptr;       // completion energy: [EN_BLOCKS partials][last, max]
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
     int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
     // bookkeeping

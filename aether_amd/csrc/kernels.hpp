@@ -1841,6 +1841,51 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         }
 }
 
+// CompletionSensor energy (sensors/s_completion.cpp:160-180): the sum of
+// Ex^2 + Ey^2 + Ez^2 over every cell of the grid in FP64, as fixed per-block
+// partials reduced in a fixed order (reproducible run to run); em[0] is the
+// last sum, em[1] the running maximum.
+constexpr int EN_BLOCKS = 512;
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_energy_part(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int rows, int P,
+                                                     double *part) {
+    // one wavefront per grid row (j, k), lanes along x; rows are strided over the launch
+    __shared__ double sh[256];
+    double acc = 0;
+    const int lane = threadIdx.x & 63;
+    for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < rows; r += EN_BLOCKS * 4) {
+        const int base = P * r;
+        for (int i = lane; i < Nx; i += 64) {
+            const double ex = Ex[base + i], ey = Ey[base + i], ez = Ez[base + i];
+            acc += ex * ex + ey * ey + ez * ez;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_fin(const double *part, double *em) {
+    __shared__ double sh[256];
+    double acc = 0;
+    for (int b = threadIdx.x; b < EN_BLOCKS; b += 256) acc += part[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        em[0] = sh[0];
+        em[1] = fmax(em[1], sh[0]);
+    }
+}
+
 // Streaming copy used to measure the HBM bandwidth a kernel can reach on
 // this device (16-byte accesses, grid-stride).
 typedef float f32x4 __attribute__((ext_vector_type(4)));

@@ -171,7 +171,7 @@ int setup_normal(aefdtd_run *r, const aefdtd_mode *m) {
 // Post-processing of FDTD_normal_incidence (normal_incidence.cpp:442-570).
 int finish_normal(aefdtd_run *r, aefdtd_normal_result *res) {
     aefdtd_grid *g = r->g;
-    const int Nt = r->Nt, Nrun = r->t, Nl = r->Nl;
+    const int Nt = r->Nt, Nrun = r->nrun >= 0 ? r->nrun : r->t, Nl = r->Nl;
     const double Dz = g->Dz, Dt = r->Dt;
     const int zs_s = g->zs_s, zs_e = g->zs_e;
     std::vector<double> Lambda, w(Nl);
@@ -419,6 +419,13 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
     r->lambda_max = m->lambda_max;
     r->reqs.assign(sensors, sensors + std::max(0, n_sensors));
     if (kind == 2 && m->n_sources > 0) r->srcs.assign(m->sources, m->sources + m->n_sources);
+    if (m->time_type < 0 || m->time_type > 1 || (m->time_type == 1 && m->cc_step < 1)) {
+        delete r;
+        return AEFDTD_EINVAL;
+    }
+    r->time_type = m->time_type;
+    r->cc_step = m->cc_step;
+    r->cc_coeff = m->cc_coeff;
     int rc = kind == 0 ? setup_normal(r, m) : (kind == 1 ? setup_single(r, m) : setup_custom(r, m));
     if (rc) {
         // keep the failure text reachable through aefdtd_last_error(NULL)
@@ -437,7 +444,7 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
         r->err = "run is stepped by a batch";
         return AEFDTD_ESTATE;
     }
-    for (int s = 0; s < n && r->t < r->Nt; s++) {
+    for (int s = 0; s < n && r->t < r->Nt && !r->done; s++) {
         int rc;
         if (r->kind == 1) {
             // single_particle.cpp:256-381: aux E, main E (+TFSF-E), aux H, main H (+TFSF-H), sensors;
@@ -452,6 +459,25 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
             if ((rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
                 return rfail(r, rc);
         }
+        if (r->time_type == 1) {
+            // the energy CompletionSensor is fed with the other sensors; the drivers check it
+            // when t % cc_step == 0 (single_particle.cpp:396-401, normal_incidence.cpp:417-423,
+            // default_fdtd.cpp:193-195): complete once its step count reaches twice the
+            // longest grid axis and the last energy is below cc_coeff of the largest
+            // (s_completion.cpp:39-41,122-128)
+            if ((rc = aefdtd::energy_feed(r->g))) return rfail(r, rc);
+            if (r->t % r->cc_step == 0) {
+                double last = 0, mx = 0;
+                if ((rc = aefdtd::energy_read(r->g, &last, &mx))) return rfail(r, rc);
+                const int steps = r->t + 1, nmax = std::max(r->g->Nx, std::max(r->g->Ny, r->g->Nz));
+                if (steps >= 2 * nmax && last / mx < r->cc_coeff) {
+                    r->done = true;
+                    if (r->kind == 0) r->nrun = r->t;      // normal incidence: Nt = t, then break
+                    r->t++;
+                    break;
+                }
+            }
+        }
         r->t++;
         if (g_cancel) return AEFDTD_ECANCELLED;
     }
@@ -461,7 +487,7 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
 int aefdtd_run_progress(const aefdtd_run *r, int *t, int *Nt) {
     if (!r) return AEFDTD_EINVAL;
     if (t) *t = r->t;
-    if (Nt) *Nt = r->Nt;
+    if (Nt) *Nt = r->done ? r->t : r->Nt;     // a run stopped by its completion check has ended
     return AEFDTD_OK;
 }
 
@@ -571,7 +597,7 @@ int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdt
     aefdtd_run *r = nullptr;
     TRY(aefdtd_run_create(&r, 0, m, n_sensors, sensors));
     int rc = AEFDTD_OK;
-    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
     // a cancelled run keeps the steps done so far (Nt=t, normal_incidence.cpp:431-435)
     bool cancelled = rc == AEFDTD_ECANCELLED;
     if (cancelled) { r->t -= 1; rc = AEFDTD_OK; }
@@ -608,6 +634,10 @@ const char *aefdtd_batch_error(const aefdtd_batch *b) {
 int aefdtd_batch_add(aefdtd_batch *b, aefdtd_run *r) {
     if (!b || !r) return AEFDTD_EINVAL;
     b->err.clear();
+    if (r->time_type != 0) {
+        b->err = "runs that stop on their completion check are stepped on their own";
+        return AEFDTD_ESTATE;
+    }
     if (r->kind != 2) {
         b->err = "only custom-mode runs are batched";
         return AEFDTD_ESTATE;
@@ -749,7 +779,7 @@ int aefdtd_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_r
     aefdtd_run *r = nullptr;
     TRY(aefdtd_run_create(&r, 2, m, n_sensors, sensors));
     int rc = AEFDTD_OK;
-    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
     bool cancelled = rc == AEFDTD_ECANCELLED;
     if (cancelled) rc = AEFDTD_OK;
     if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);
@@ -765,7 +795,7 @@ int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd
     aefdtd_run *r = nullptr;
     TRY(aefdtd_run_create(&r, 1, m, n_sensors, sensors));
     int rc = AEFDTD_OK;
-    while (r->t < r->Nt && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
     bool cancelled = rc == AEFDTD_ECANCELLED;
     if (cancelled) rc = AEFDTD_OK;
     if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);

@@ -21,6 +21,11 @@ struct aefdtd_run {
     double index_sup = 1, index_sub = 1;
     std::string err;
     struct aefdtd_batch *batch = nullptr;   // set while a member of a batch
+    // auto_tsteps with the energy criterion (time_type 1); done once it stopped
+    int time_type = 0, cc_step = 1;
+    double cc_coeff = 0;
+    bool done = false;
+    int nrun = -1;                    // normal incidence: the step count its spectra use
     ~aefdtd_run() {
         aefdtd_destroy(g);
         aefdtd_destroy(a);

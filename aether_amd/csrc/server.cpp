@@ -159,7 +159,7 @@ int build(Request *q) {
 // can step it (worker thread: the batches are the worker's).  A run the batch
 // refuses still runs on its own.
 int join_batch(aefdtd_server *S, Request *q) {
-    if (q->kind != 2 || q->run->g->n_disp) return AEFDTD_OK;
+    if (q->kind != 2 || q->run->g->n_disp || q->m.time_type) return AEFDTD_OK;
     const int p = q->m.precision;
     int rc;
     if (!S->batch[p] && (rc = aefdtd_batch_create(&S->batch[p], p))) {

@@ -220,6 +220,14 @@ typedef struct aefdtd_mode {
     int     n_materials; const aefdtd_material *materials;
     int     precision;           /* AEFDTD_FP64 / AEFDTD_FP32 / AEFDTD_FP16 */
     int     n_sources;  const struct aefdtd_source_req *sources;   /* custom mode */
+    /* auto_tsteps (FDTD_Mode::set_auto_tsteps(Nt, cc_step, cc_coeff),
+     * modes/lua_fdtd.cpp:271-278): time_type 1 stops a run early with the energy
+     * CompletionSensor (sensors/s_completion.cpp:23-27,39-130,160-180), checked
+     * every cc_step steps; 0 runs all Nt steps.  The spectral (FT) criterion
+     * of the reference is not provided. */
+    int     time_type;
+    int     cc_step;
+    double  cc_coeff;
 } aefdtd_mode;
 
 /* A source request (Source_generator, core/sources.h:29-59, already

@@ -41,3 +41,19 @@ def load_ref():
     lib.aeref_nearest_integer.argtypes = [C.c_double]
     lib.aeref_nearest_integer.restype = C.c_int
     return lib
+
+
+def last_steps():
+    """Steps the last oracle driver call ran (an energy-completion stop included)."""
+    return C.CDLL(ORACLE_SO).aeoracle_last_steps()
+
+
+def energy_trace():
+    """Per-step energy sums of the last oracle driver call that had the energy
+    completion check on (numpy array, one per step run)."""
+    import numpy as np
+    lib = C.CDLL(ORACLE_SO)
+    n = lib.aeoracle_energy_trace(None, 0)
+    out = np.zeros(n)
+    lib.aeoracle_energy_trace(out.ctypes.data_as(C.POINTER(C.c_double)), n)
+    return out

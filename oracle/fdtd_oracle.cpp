@@ -1500,12 +1500,56 @@ void copy_fields(aeoracle_grid *g, double *out) {
     for (int c = 0; c < 6; c++) std::memcpy(out + c * N, g->F[c].data(), N * sizeof(double));
 }
 
+// CompletionSensor in energy mode (sensors/s_completion.cpp:20-24,39-41,122-128,
+// 160-180): every feed sums Ex^2+Ey^2+Ez^2 over the whole grid in k, j, i order
+// and keeps the running maximum; the drivers call the check when t % cc_step
+// == 0, and it completes once the sensor's step count (t + 1 after the feed of
+// step t) reaches twice the longest axis and last/max < cc_coeff.
+std::vector<double> g_energy_trace;   // every feed's sum, for the tests
+
+struct EnergyCheck {
+    const aefdtd_mode *m;
+    EnergyCheck(const aefdtd_mode *m_) : m(m_) { g_energy_trace.clear(); }
+    double last = 0, max = 0;
+    bool on() const { return m->time_type == 1; }
+    bool feed_and_check(const aeoracle_grid *g, int t) {
+        last = 0;
+        const size_t N = g->F[0].size();
+        for (size_t c = 0; c < N; c++) {
+            const double ex = g->F[0][c], ey = g->F[1][c], ez = g->F[2][c];
+            last += ex * ex + ey * ey + ez * ez;
+        }
+        max = std::max(max, last);
+        g_energy_trace.push_back(last);
+        if (t % m->cc_step) return false;
+        if (t + 1 < 2 * std::max(g->Nx, std::max(g->Ny, g->Nz))) return false;
+        return last / max < m->cc_coeff;
+    }
+};
+
+int check_time_type(const aefdtd_mode *m) {
+    if (m->time_type == 0) return AEFDTD_OK;
+    if (m->time_type != 1 || m->cc_step < 1) return AEFDTD_EINVAL;
+    return AEFDTD_OK;
+}
+
+int g_last_steps = 0;
+
 }  // namespace
+
+int aeoracle_last_steps(void) { return g_last_steps; }
+
+int aeoracle_energy_trace(double *out, int n) {
+    const int m = (int)g_energy_trace.size();
+    for (int i = 0; out && i < std::min(n, m); i++) out[i] = g_energy_trace[i];
+    return m;
+}
 
 // FDTD_normal_incidence, modes/normal_incidence.cpp:161-607
 int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
                                   aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
     g_cancel = false;
+    if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
     int pad_zm = std::max(5, m->pad[4]), pad_zp = std::max(5, m->pad[5]);
@@ -1555,14 +1599,17 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
     int idR, idT;
     aeoracle_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &idR);
     aeoracle_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT);
+    EnergyCheck E(m);
     int t;
     for (t = 0; t < Nt; t++) {
         g->update_E();
         g->update_H();
         aeoracle_feed_sensors(g);
+        if (E.on() && E.feed_and_check(g, t)) break;     // Nt = t (normal_incidence.cpp:417-423)
         if (g_cancel) break;
     }
     int Nrun = t;
+    g_last_steps = (t < Nt && E.on()) ? t + 1 : t;
     std::vector<double> R(6 * (size_t)Nrun), T(6 * (size_t)Nrun);
     aeoracle_read_plane_sum(g, idR, Nrun, R.data());
     aeoracle_read_plane_sum(g, idT, Nrun, T.data());
@@ -1680,6 +1727,7 @@ int aeoracle_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd
 int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors, double *sensor_out,
                         double *fields_out) {
     g_cancel = false;
+    if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
     double Dt = std::min(std::min(Dx, Dy), Dz) / (std::sqrt(3.0) * c_light) * 0.99 * m->time_mod;
@@ -1698,12 +1746,16 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
         if ((rc = aeoracle_add_oscillator(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, q.lambda_max, &id)))
             return rc;
     }
-    for (int t = 0; t < Nt; t++) {
+    EnergyCheck E(m);
+    int t;
+    for (t = 0; t < Nt; t++) {
         g->update_E();       // + sources' inject_E
         g->update_H();       // + sources' inject_H (step count)
         aeoracle_feed_sensors(g);
+        if (E.on() && E.feed_and_check(g, t)) { t++; break; }     // default_fdtd.cpp:193-195
         if (g_cancel) break;
     }
+    g_last_steps = t;
     collect_requests(g, n_sensors, sensors, S, sensor_out);
     copy_fields(g, fields_out);
     return AEFDTD_OK;
@@ -1712,6 +1764,7 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
                                  double *sensor_out, double *fields_out) {
     g_cancel = false;
+    if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
     int pml_x = std::max(m->pml[0], m->pml[1]), pml_y = std::max(m->pml[2], m->pml[3]);
@@ -1766,14 +1819,18 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefd
     if ((rc = aeoracle_set_tfsf(g, a, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e, periodic_x, periodic_y))) return rc;
     SensorSet S;
     if ((rc = add_requests(g, n_sensors, sensors, S))) return rc;
-    for (int t = 0; t < Nt; t++) {
+    EnergyCheck E(m);
+    int t;
+    for (t = 0; t < Nt; t++) {
         a->update_E();
         g->update_E();
         a->update_H();
         g->update_H();
         aeoracle_feed_sensors(g);
+        if (E.on() && E.feed_and_check(g, t)) { t++; break; }     // single_particle.cpp:399-401
         if (g_cancel) break;
     }
+    g_last_steps = t;
     collect_requests(g, n_sensors, sensors, S, sensor_out);
     copy_fields(g, fields_out);
     return AEFDTD_OK;

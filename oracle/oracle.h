@@ -92,6 +92,12 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors,
                                  const aefdtd_sensor_req *sensors,
                                  double *sensor_out, double *fields_out);
+/* Steps the last driver call ran (early stop by the energy completion check
+ * included). */
+int aeoracle_last_steps(void);
+/* The energy sums of the last driver call with the energy completion check,
+ * one per step; copies up to n into out and returns how many there are. */
+int aeoracle_energy_trace(double *out, int n);
 /* Steps a configured grid n full steps (update_E, update_H) with no driver
  * actions: the timed CPU-baseline loop. */
 int aeoracle_step(aeoracle_grid *g, int n);

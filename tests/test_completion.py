@@ -1,0 +1,206 @@
+"""Automatic run length (Aether's CompletionSensor in energy mode).
+
+Reference behaviour (sensors/s_completion.cpp:39-41,122-128,160-180 and the
+drivers modes/default_fdtd.cpp:193-195, modes/single_particle.cpp:399-401,
+modes/normal_incidence.cpp:417-423): every step's sensor feed sums
+Ex^2+Ey^2+Ez^2 over the whole grid and keeps the running maximum; when
+t % cc_step == 0 the driver asks for completion, which holds once the
+sensor's step count (t + 1) is at least twice the longest grid axis and the
+last sum is below cc_coeff times the maximum.  The custom and single-particle
+drivers then stop with t + 1 steps run; normal incidence sets Nt = t, so its
+R/T transforms use the first t samples.
+
+CPU tests pin the oracle: the stop step against the rule applied to the
+oracle's own per-step energies, those energies against the fields of fixed
+runs of the same length, and the results of a stopped run against a
+fixed-length run.  GPU tests hold the engine to the oracle: the same stop step
+and FP64 results at the usual bars."""
+import numpy as np
+import pytest
+
+import oracle
+from aether_amd import capi, workloads
+from aether_amd.capi import Material
+
+CC_STEP = 10
+
+
+def custom_spec(precision=capi.FP64, Nt=4000, cc_coeff=1e-3):
+    """A 14x12x10 dielectric block lit by an x-normal oscillator plane, CPML on
+    every side (36x34x32 cells); the energy peaks near step 790."""
+    n = (14, 12, 10)
+    mg = np.zeros(n[::-1], dtype=np.uint32)
+    mg[3:7, 4:8, 5:9] = 1
+    src = capi.SourceReq(orientation=capi.NORMAL_X, x1=0, x2=14, y1=0, y2=12, z1=2, z2=3,
+                         lambda_min=400e-9, lambda_max=800e-9)
+    sens = [capi.SensorReq(box=True, x1=2, x2=12, y1=2, y2=10, z1=3, z2=8, Nl=7)]
+    spec = capi.ModeSpec(matsgrid=mg, materials=[Material(1.0), Material(4.0)], Nt=Nt, Dx=5e-9, Dy=5e-9,
+                         Dz=5e-9, pml=(6,) * 6, pad=(5,) * 6, tapering=0, sources=[src], precision=precision,
+                         time_type=1, cc_step=CC_STEP, cc_coeff=cc_coeff)
+    return spec, sens
+
+
+def normal_spec(precision=capi.FP64):
+    spec = workloads.slab_normal_incidence(n=(8, 6, 40), slab=(12, 24), Nt=4000, Nl=12, pml_z=8)
+    spec.tapering = 0
+    spec.precision = precision
+    spec.time_type, spec.cc_step, spec.cc_coeff = 1, CC_STEP, 1e-3
+    return spec
+
+
+def single_spec(precision=capi.FP64):
+    spec, sens, _ = workloads.spheres_single_particle(n=12, D=12e-9, count=2, rmin=18e-9, rmax=36e-9, seed=5,
+                                                      Nt=3000, Nl=10, pml=5, precision=precision)
+    spec.time_type, spec.cc_step, spec.cc_coeff = 1, CC_STEP, 1e-2
+    return spec, sens
+
+
+def expected_steps(trace, dims, cc_step, coeff, Nt):
+    """The stop rule applied to a per-step energy trace: the number of steps
+    the custom/single-particle drivers run."""
+    mx = 0.0
+    for t, e in enumerate(trace):
+        mx = max(mx, e)
+        if t % cc_step == 0 and t + 1 >= 2 * max(dims) and e / mx < coeff:
+            return t + 1
+    return Nt
+
+
+def energy(fields):
+    return float(np.sum(fields[0] ** 2 + fields[1] ** 2 + fields[2] ** 2))
+
+
+def fixed(spec, **kw):
+    return capi.ModeSpec(**{**spec.__dict__, "time_type": 0, **kw})
+
+
+# ---------------------------------------------------------------- CPU: the oracle
+
+def test_oracle_custom_stop_rule(oracle_be):
+    spec, sens = custom_spec()
+    r = capi.run_custom(oracle_be, spec, sens)
+    steps, tr = oracle.last_steps(), oracle.energy_trace()
+    dims = capi.custom_dims(spec)
+    assert len(tr) == steps
+    assert 2 * max(dims) < steps < spec.Nt          # stopped early, after the minimum length
+    assert steps == expected_steps(tr, dims, CC_STEP, spec.cc_coeff, spec.Nt)
+    assert (steps - 1) % CC_STEP == 0
+    # the energies are the fields' sums of squares: fixed runs of two lengths
+    for n in (steps, steps - 7):
+        f = capi.run_custom(oracle_be, fixed(spec, Nt=n), sens, want_fields=True)
+        assert abs(energy(f["fields"]) - tr[n - 1]) <= 1e-12 * tr[n - 1]
+        if n == steps:
+            # stopping is the same as a fixed run of that many steps (no tapering)
+            assert np.array_equal(f["sensors"], r["sensors"])
+
+
+def test_oracle_normal_incidence_sets_nt_to_t(oracle_be):
+    spec = normal_spec()
+    r = capi.run_normal_incidence(oracle_be, spec)
+    steps, tr = oracle.last_steps(), oracle.energy_trace()
+    Nz = spec.matsgrid.shape[0] + 2 * 8 + 2 * max(5, spec.pad[4])
+    assert steps == expected_steps(tr, (8, 6, Nz), CC_STEP, spec.cc_coeff, spec.Nt) < spec.Nt
+    assert r["Nt_run"] == steps - 1
+    f = capi.run_normal_incidence(oracle_be, fixed(spec, Nt=steps - 1))
+    for k in ("R", "T", "refx", "refy", "transx", "transy"):
+        assert np.array_equal(f[k], r[k]), k
+
+
+def test_oracle_fixed_runs_keep_their_length(oracle_be):
+    spec, sens = custom_spec(Nt=300)
+    capi.run_custom(oracle_be, fixed(spec), sens)
+    assert oracle.last_steps() == 300
+
+
+def test_oracle_refuses_fourier_completion(oracle_be):
+    spec, sens = custom_spec(Nt=20)
+    spec.time_type = 2
+    with pytest.raises(capi.FDTDError):
+        capi.run_custom(oracle_be, spec, sens)
+
+
+# ---------------------------------------------------------------- GPU: the engine
+
+def run_engine(kind, spec, sens=(), chunk=64):
+    run = capi.Run(capi.engine(), kind, spec, sens)
+    t, nt = run.progress()
+    while t < nt:
+        run.step(chunk)
+        t, nt = run.progress()
+    out = run.finish(want_fields=kind != capi.KIND_NORMAL)
+    run.close()
+    return t, out
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.gpu
+def test_engine_custom_stops_with_oracle_fp64(oracle_be):
+    print("energy completion: custom FP64", flush=True)
+    spec, sens = custom_spec()
+    steps, g = run_engine(capi.KIND_CUSTOM, spec, sens)
+    r = capi.run_custom(oracle_be, spec, sens, want_fields=True)
+    assert steps == oracle.last_steps() < spec.Nt
+    assert rel_l2(g["fields"], r["fields"]) <= 1e-10
+    assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
+    # the one-shot driver stops at the same step
+    d = capi.run_custom(capi.engine(), spec, sens)
+    assert np.array_equal(d["sensors"], g["sensors"])
+
+
+@pytest.mark.gpu
+def test_engine_single_particle_stops_with_oracle_fp64(oracle_be):
+    print("energy completion: single particle FP64", flush=True)
+    spec, sens = single_spec()
+    steps, g = run_engine(capi.KIND_SINGLE, spec, sens)
+    r = capi.run_single_particle(oracle_be, spec, sens)
+    assert steps == oracle.last_steps() < spec.Nt
+    assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
+
+
+@pytest.mark.gpu
+def test_engine_normal_incidence_stops_with_oracle_fp64(oracle_be):
+    print("energy completion: normal incidence FP64", flush=True)
+    spec = normal_spec()
+    steps, g = run_engine(capi.KIND_NORMAL, spec)
+    r = capi.run_normal_incidence(oracle_be, spec)
+    assert steps == oracle.last_steps() and g["Nt_run"] == r["Nt_run"] == steps - 1
+    for k in ("R", "T"):
+        assert np.max(np.abs(g[k] - r[k])) <= 1e-9, k
+
+
+@pytest.mark.gpu
+def test_engine_fp32_stop_step(oracle_be):
+    """FP32 fields change the energy sums in their last bits: the stop lands on
+    the oracle's check or the next one."""
+    print("energy completion: custom FP32", flush=True)
+    spec, sens = custom_spec(precision=capi.FP32)
+    steps, g = run_engine(capi.KIND_CUSTOM, spec, sens)
+    capi.run_custom(oracle_be, fixed(spec, precision=capi.FP64, time_type=1), sens)
+    ref = oracle.last_steps()
+    print("fp32 stop", steps, "oracle", ref)
+    assert abs(steps - ref) <= CC_STEP
+
+
+@pytest.mark.gpu
+def test_batch_and_server_with_completion():
+    """Batches step fixed-length runs only; the server runs a completion
+    request on its own and gives the direct run's result."""
+    print("energy completion: batch refusal, server", flush=True)
+    spec, sens = custom_spec(precision=capi.FP32, cc_coeff=1e-2)
+    b = capi.Batch(capi.engine(), capi.FP32)
+    run = capi.Run(capi.engine(), capi.KIND_CUSTOM, spec, sens)
+    with pytest.raises(capi.FDTDError):
+        b.add(run)
+    run.close()
+    b.close()
+    steps, direct = run_engine(capi.KIND_CUSTOM, spec, sens)
+    srv = capi.Server(capi.engine(), 64 << 30)
+    rid = srv.submit(capi.KIND_CUSTOM, spec, sens)
+    assert srv.wait(rid, 120.0) == capi.REQ_DONE, srv.error(rid)
+    served = srv.result(rid)[0]
+    srv.close()
+    assert steps < spec.Nt
+    assert np.array_equal(served, direct["sensors"])
