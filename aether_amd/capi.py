@@ -9,6 +9,7 @@ carry the ``aeoracle_`` prefix instead of ``aefdtd_``.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 
@@ -76,6 +77,20 @@ class CNormalResult(C.Structure):
     _fields_ = [("lambda_", _dp), ("R", _dp), ("T", _dp),
                 ("refx", _dp), ("refy", _dp), ("transx", _dp), ("transy", _dp),
                 ("index_sup", C.c_double), ("index_sub", C.c_double), ("Nt_run", C.c_int)]
+
+
+OP_BLOCK, OP_SPHERE, OP_CYLINDER, OP_CONE, OP_ELLIPSOID, OP_LAYER = range(6)
+
+
+class CStructureOp(C.Structure):
+    _fields_ = [("type", C.c_int), ("mat_index", C.c_int), ("p", C.c_double * 7)]
+
+
+class CStructure(C.Structure):
+    _fields_ = [("n_ops", C.c_int), ("ops", C.POINTER(CStructureOp)),
+                ("default_material", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("flip", C.c_int * 3), ("loop", C.c_int * 3)]
 
 
 def _sig(lib, p):
@@ -158,6 +173,7 @@ def _sig(lib, p):
         "server_stats": (i, [vp, _dp]),
         "estimate_bytes": (i, [i, C.POINTER(CMode), i, C.POINTER(CSensorReq), C.POINTER(C.c_size_t)]),
         "stream_bandwidth": (i, [C.c_size_t, i, _dp]),
+        "discretize": (i, [C.POINTER(CStructure), i, i, i, d, d, d, C.POINTER(C.c_uint32)]),
         # oracle only
         "step": (i, [vp, i]),
         "set_threads": (None, [i]),
@@ -352,6 +368,92 @@ class ModeSpec:
         c.sources = srcs
         c.time_type, c.cc_step, c.cc_coeff = self.time_type, self.cc_step, self.cc_coeff
         return c
+
+
+class Structure:
+    """A structure as a structure script builds it: the operation list made by
+    the script's add_* calls (structure/lua_structure.cpp:292-547), the default
+    material, the nominal size lx, ly, lz and the flip / loop modifiers
+    (structure/structure.cpp:326-352).  Method names and argument order follow
+    the script functions; lengths in metres."""
+
+    def __init__(self, lx=300e-9, ly=300e-9, lz=300e-9):   # Structure::Structure defaults (structure.cpp:52-66)
+        self.ops = []
+        self.default = 0
+        self.lx, self.ly, self.lz = lx, ly, lz
+        self.flips = (0, 0, 0)
+        self.loops = (0, 0, 0)
+
+    def _add(self, kind, params, index):
+        p = list(map(float, params)) + [0.0] * (7 - len(params))
+        self.ops.append((kind, int(index), p))
+        return self
+
+    def add_block(self, x1, x2, y1, y2, z1, z2, index):
+        return self._add(OP_BLOCK, (x1, x2, y1, y2, z1, z2), index)
+
+    def add_sphere(self, x, y, z, r, index):
+        return self._add(OP_SPHERE, (x, y, z, r), index)
+
+    def add_cylinder(self, x, y, z, dx, dy, dz, r, index):
+        return self._add(OP_CYLINDER, (x, y, z, dx, dy, dz, r), index)
+
+    def add_cone(self, x, y, z, dx, dy, dz, r, index):
+        return self._add(OP_CONE, (x, y, z, dx, dy, dz, r), index)
+
+    def add_ellipsoid(self, x, y, z, rx, ry, rz, index):
+        return self._add(OP_ELLIPSOID, (x, y, z, rx, ry, rz), index)
+
+    def add_layer(self, axis, z1, z2, index):
+        """axis "X"/"x", "Y"/"y", anything else z, as add_layer (lua_structure.cpp:420-440)."""
+        a = 0 if axis in ("X", "x") else (1 if axis in ("Y", "y") else 2)
+        return self._add(OP_LAYER, (a, z1, z2), index)
+
+    def default_material(self, index):
+        self.default = int(index)
+        return self
+
+    def flip(self, x, y, z):
+        self.flips = (int(x != 0), int(y != 0), int(z != 0))
+        return self
+
+    def loop(self, x, y, z):
+        self.loops = (int(x != 0), int(y != 0), int(z != 0))
+        return self
+
+    def to_c(self, keep):
+        arr = (CStructureOp * max(1, len(self.ops)))()
+        for n, (kind, index, p) in enumerate(self.ops):
+            arr[n].type, arr[n].mat_index = kind, index
+            for q in range(7):
+                arr[n].p[q] = p[q]
+        keep.append(arr)
+        c = CStructure()
+        c.n_ops, c.ops = len(self.ops), arr
+        c.default_material = self.default
+        c.lx, c.ly, c.lz = self.lx, self.ly, self.lz
+        for d in range(3):
+            c.flip[d], c.loop[d] = self.flips[d], self.loops[d]
+        return c
+
+    def cells(self, Dx, Dy, Dz):
+        """FD_Mode::compute_discretization (finite_differences/core/lua_fd.cpp:69-82):
+        max(1, nearest_integer(l / D)) per axis, rounding halves away from zero."""
+        def ni(v):
+            return int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
+        return tuple(max(1, ni(l / D)) for l, D in ((self.lx, Dx), (self.ly, Dy), (self.lz, Dz)))
+
+
+def discretize(be: Backend, structure: Structure, N, D):
+    """Structure::discretize over an N = (Nx, Ny, Nz) grid of cell sizes D:
+    returns the (Nz, Ny, Nx) uint32 material grid."""
+    keep = []
+    cs = structure.to_c(keep)
+    out = np.zeros(tuple(N)[::-1], dtype=np.uint32)
+    rc = be.fn("discretize")(C.byref(cs), int(N[0]), int(N[1]), int(N[2]), float(D[0]), float(D[1]), float(D[2]),
+                             out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    be.check(rc, None)
+    return out
 
 
 def mode_finalize(be: Backend, spec: ModeSpec, kind, sensors=()):

@@ -259,6 +259,52 @@ typedef struct aefdtd_sensor_req {
  * without PML gets no pad.  Updates m->pml and m->pad in place. */
 int aefdtd_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors);
 
+/* ---------------------------------------------------------------------------
+ * Structures (src/structure).  A structure is the operation list that a
+ * structure script builds through the Lua functions of
+ * structure/lua_structure.cpp, plus its default material, nominal size,
+ * flip and loop (periodic) modifiers.  Parameters are as the script passes
+ * them, in metres:
+ *   BLOCK     p = x1 x2 y1 y2 z1 z2            add_block (lua_structure.cpp:292-310)
+ *   SPHERE    p = x y z r                      add_sphere (:531-547)
+ *   CYLINDER  p = x y z dx dy dz r             add_cylinder (:376-397): axis from
+ *                                              (x,y,z) to (x+dx, y+dy, z+dz)
+ *   CONE      p = x y z dx dy dz r             add_cone (:334-354): radius r at
+ *                                              (x,y,z), 0 at the tip
+ *   ELLIPSOID p = x y z rx ry rz               add_ellipsoid (:399-418)
+ *   LAYER     p = axis z1 z2                   add_layer (:420-440): axis 0 "X",
+ *                                              1 "Y", 2 "Z"
+ * Coatings, meshes, sinusoidal layers, vector blocks/triangles and Lua
+ * defined operations are not provided.
+ * ------------------------------------------------------------------------- */
+enum { AEFDTD_OP_BLOCK = 0, AEFDTD_OP_SPHERE = 1, AEFDTD_OP_CYLINDER = 2, AEFDTD_OP_CONE = 3,
+       AEFDTD_OP_ELLIPSOID = 4, AEFDTD_OP_LAYER = 5 };
+
+typedef struct aefdtd_structure_op {
+    int    type;        /* AEFDTD_OP_* */
+    int    mat_index;
+    double p[7];
+} aefdtd_structure_op;
+
+typedef struct aefdtd_structure {
+    int n_ops;  const aefdtd_structure_op *ops;   /* script order: later ones win */
+    int    default_material;                      /* default_material(index) */
+    double lx, ly, lz;                            /* the script's lx, ly, lz */
+    int    flip[3];                               /* flip(x, y, z) */
+    int    loop[3];                               /* loop(x, y, z) */
+} aefdtd_structure;
+
+/* Structure::discretize (structure/structure.cpp:94-113, with
+ * Structure::index, :223-262): cell (i,j,k) of the Nx x Ny x Nz grid takes
+ * the material of the last operation containing the point (i Dx, j Dy, k Dz)
+ * (flipped, then folded into [0, l) on looped axes and tested against the
+ * operation and its +-l images), else the default material.  Evaluated on
+ * the device; matsgrid (Nx*Ny*Nz values, x fastest) is a host buffer.
+ * Results are identical to the reference's: the point tests use the same
+ * FP64 operations in the same order, without fused multiply-adds. */
+int aefdtd_discretize(const aefdtd_structure *s, int Nx, int Ny, int Nz, double Dx, double Dy, double Dz,
+                      uint32_t *matsgrid);
+
 /* Results of the normal-incidence driver (normal_incidence.cpp:442-570):
  * arrays of Nl. R,T are the `spectdata_norm` columns; refx..transy the
  * complex amplitudes written to `spectdata_norm2` (as re,im pairs). */

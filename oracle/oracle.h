@@ -95,6 +95,9 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors,
 /* Steps the last driver call ran (early stop by the energy completion check
  * included). */
 int aeoracle_last_steps(void);
+/* Structure::discretize, serially on the CPU (structure_oracle.cpp). */
+int aeoracle_discretize(const aefdtd_structure *s, int Nx, int Ny, int Nz, double Dx, double Dy, double Dz,
+                        uint32_t *matsgrid);
 /* The energy sums of the last driver call with the energy completion check,
  * one per step; copies up to n into out and returns how many there are. */
 int aeoracle_energy_trace(double *out, int n);
