@@ -131,19 +131,24 @@ hipError_t fill_sync(aefdtd_grid *g, void *dst, int v, size_t n) {
 }
 
 // Dense host cell array (x fastest, Nx per row) <-> pitched device cell array.
+// A strided copy from pageable host memory goes row by row (tens of thousands
+// of small transfers on a 3D grid), so the rows are staged in a pitched host
+// buffer and moved as one contiguous copy; the row padding is written as zero.
 hipError_t cells_to_device(aefdtd_grid *g, void *dst, const void *src, size_t es) {
-    const size_t rows = (size_t)g->Ny * g->Nz;
-    hipError_t e = hipMemcpy2DAsync(dst, g->px * es, src, g->Nx * es, g->Nx * es, rows, hipMemcpyHostToDevice,
-                                    g->stream);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(g->stream);
+    const size_t rows = (size_t)g->Ny * g->Nz, w = (size_t)g->Nx * es, pw = (size_t)g->px * es;
+    if (pw == w) return copy_sync(g, dst, src, rows * w, hipMemcpyHostToDevice);
+    std::vector<char> st(rows * pw, 0);
+    for (size_t r = 0; r < rows; r++) std::memcpy(st.data() + r * pw, (const char *)src + r * w, w);
+    return copy_sync(g, dst, st.data(), rows * pw, hipMemcpyHostToDevice);
 }
 hipError_t cells_to_host(aefdtd_grid *g, void *dst, const void *src, size_t es) {
-    const size_t rows = (size_t)g->Ny * g->Nz;
-    hipError_t e = hipMemcpy2DAsync(dst, g->Nx * es, src, g->px * es, g->Nx * es, rows, hipMemcpyDeviceToHost,
-                                    g->stream);
+    const size_t rows = (size_t)g->Ny * g->Nz, w = (size_t)g->Nx * es, pw = (size_t)g->px * es;
+    if (pw == w) return copy_sync(g, dst, src, rows * w, hipMemcpyDeviceToHost);
+    std::vector<char> st(rows * pw);
+    hipError_t e = copy_sync(g, st.data(), src, rows * pw, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return e;
-    return hipStreamSynchronize(g->stream);
+    for (size_t r = 0; r < rows; r++) std::memcpy((char *)dst + r * w, st.data() + r * pw, w);
+    return hipSuccess;
 }
 
 void pml_coeff_calc(aefdtd_grid *g) {          // core/fdtd_pml.cpp:305-343
@@ -276,43 +281,52 @@ int upload_materials_T(aefdtd_grid *g) {
 // neighbouring lanes touch neighbouring slots.
 int upload_grid(aefdtd_grid *g) {
     if (g->Nmat > 256) return fail(g, AEFDTD_EINVAL, "more than 256 material indices");
-    const size_t N = g->ncell(), NA = g->nalloc();
-    std::vector<uint8_t> m8(N);
-    for (size_t c = 0; c < N; c++) m8[c] = (uint8_t)g->matsgrid[c];
+    const size_t NA = g->nalloc(), rows = (size_t)g->Ny * g->Nz;
+    // one pass over the cells: uint8 ids in the pitched layout, and the device
+    // indices of the dispersive cells in x-fastest order
+    bool disp[256] = {};
+    bool any = false;
+    for (unsigned m = 0; m < g->Nmat; m++) {
+        disp[m] = g->mats[m].rc;
+        any = any || disp[m];
+    }
+    std::vector<uint8_t> m8(NA, 0);
+    std::vector<int32_t> cells;
+    for (size_t r = 0; r < rows; r++) {
+        const uint32_t *src = g->matsgrid.data() + r * g->Nx;
+        uint8_t *dst = m8.data() + r * g->px;
+        for (int i = 0; i < g->Nx; i++) {
+            dst[i] = (uint8_t)src[i];
+            if (disp[src[i]]) cells.push_back((int32_t)(r * g->px + i));
+        }
+    }
     if (!g->d_mat) {
         void *p;
         int rc = dalloc(g, &p, NA);
         if (rc) return rc;
         g->d_mat = (uint8_t *)p;
     }
-    HIPOK(g, cells_to_device(g, g->d_mat, m8.data(), 1));
+    HIPOK(g, copy_sync(g, g->d_mat, m8.data(), NA, hipMemcpyHostToDevice));
     int rc = with_ctype(g, [&]<typename T>() { return upload_materials_T<T>(g); });
     if (rc) return rc;
-    long nd = 0;
-    std::vector<int32_t> slot;
-    bool any = false;
-    for (unsigned m = 0; m < g->Nmat; m++) any = any || g->mats[m].rc;
     dfree_t(g->d_slot);
     dfree_t(g->d_psi);
     dfree_t(g->d_cells);
     dfree(g->d_corr);
     g->n_disp = 0;
-    if (any) {
-        slot.assign(NA, -1);
-        std::vector<int32_t> cells;
-        for (size_t c = 0; c < N; c++)
-            if (g->mats[g->matsgrid[c]].rc) {
-                const size_t d = g->dcell(c);
-                slot[d] = (int32_t)nd++;
-                cells.push_back((int32_t)d);
-            }
+    if (any && !cells.empty()) {
+        const long nd = (long)cells.size();
         void *p;
         if ((rc = dalloc(g, &p, NA * sizeof(int32_t)))) return rc;
         g->d_slot = (int32_t *)p;
-        HIPOK(g, hipMemcpyAsync(g->d_slot, slot.data(), NA * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        HIPOK(g, hipMemsetAsync(g->d_slot, 0xFF, NA * sizeof(int32_t), g->stream));     // -1: no slot
         if ((rc = dalloc(g, &p, cells.size() * sizeof(int32_t)))) return rc;
         g->d_cells = (int32_t *)p;
-        HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                g->stream));
+        hipLaunchKernelGGL(k_slot_scatter, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, g->stream,
+                           (const int32_t *)g->d_cells, nd, g->d_slot);
+        HIPOK(g, hipGetLastError());
         if ((rc = dalloc(g, &g->d_corr, (size_t)3 * nd * g->csize()))) return rc;
         size_t np = (size_t)3 * g->np_max * nd;
         if ((rc = dalloc(g, &p, np * sizeof(double2)))) return rc;

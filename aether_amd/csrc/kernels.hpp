@@ -1841,24 +1841,39 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         }
 }
 
+// Slot table of the dispersive cells: slot[cells[i]] = i (the table is -1
+// elsewhere); cells holds device indices, all below the table's length.
+__global__ __launch_bounds__(256) void k_slot_scatter(const int32_t *__restrict__ cells, long n,
+                                                      int32_t *__restrict__ slot) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) slot[cells[i]] = (int32_t)i;
+}
+
 // CompletionSensor energy (sensors/s_completion.cpp:160-180): the sum of
 // Ex^2 + Ey^2 + Ez^2 over every cell of the grid in FP64, as fixed per-block
 // partials reduced in a fixed order (reproducible run to run); em[0] is the
 // last sum, em[1] the running maximum.
-constexpr int EN_BLOCKS = 512;
+constexpr int EN_BLOCKS = 2048;
 
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_energy_part(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int rows, int P,
                                                      double *part) {
-    // one wavefront per grid row (j, k), lanes along x; rows are strided over the launch
+    // wavefronts take groups of four grid rows (j, k), lanes along x, the four
+    // rows' loads issued together; groups are strided over the launch
     __shared__ double sh[256];
     double acc = 0;
     const int lane = threadIdx.x & 63;
-    for (int r = blockIdx.x * 4 + (threadIdx.x >> 6); r < rows; r += EN_BLOCKS * 4) {
-        const int base = P * r;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (int r0 = wave * 4; r0 < rows; r0 += EN_BLOCKS * 16) {
         for (int i = lane; i < Nx; i += 64) {
-            const double ex = Ex[base + i], ey = Ey[base + i], ez = Ez[base + i];
-            acc += ex * ex + ey * ey + ez * ez;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (r0 + q < rows) {
+                    const int c = P * (r0 + q) + i;
+                    const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+                    acc += ex * ex + ey * ey + ez * ez;
+                }
+            }
         }
     }
     sh[threadIdx.x] = acc;

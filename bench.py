@@ -317,6 +317,71 @@ def serving_form(args):
     print(json.dumps(rec))
 
 
+def setup_form(args):
+    """The setup form: what a run costs before and around its time loop.
+    (1) Structure::discretize of the headline structure (256^3 cells, 40
+    spheres) through aefdtd_discretize (allocation, kernel, copy of the uint32
+    grid to the host; median of 5 after one warm call) against the serial loop
+    of the oracle (the reference's algorithm, one thread, one call); (2) run
+    creation (aefdtd_run_create of the single-particle driver: grid, material
+    upload, CPML, aux column, sensors) at 64^3, 128^3 and 256^3 structures, warm
+    process, median of 3; (3) the energy completion check (auto_tsteps, checked
+    every 500 steps as the reference's default) on the headline 300^3 run: ms per
+    step with and without it over 200 steps after 100 warm-up steps."""
+    import statistics
+    import numpy as np
+    from aether_amd import capi, workloads
+    be = capi.engine()
+    n, D = 256, 2e-9
+    sph = workloads.random_spheres(2, 40, (n * D,) * 3, 8e-9, 20e-9)
+    st = capi.Structure(n * D, n * D, n * D)
+    for (x, y, z, r, m) in sph:
+        st.add_sphere(x, y, z, r, m)
+    g = capi.discretize(be, st, (n, n, n), (D, D, D))
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        g = capi.discretize(be, st, (n, n, n), (D, D, D))
+        ts.append(time.perf_counter() - t0)
+    disc = {"engine_ms": round(1e3 * statistics.median(ts), 2)}
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        o = capi.discretize(oracle.load(), st, (n, n, n), (D, D, D))
+        disc["oracle_serial_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+        disc["identical"] = bool(np.array_equal(o, g))
+    print(f"setup form: discretize {disc}", file=sys.stderr, flush=True)
+    create = {}
+    for side in (64, 128, 256):
+        spec, sensors, _ = workloads.spheres_single_particle(n=side, count=max(2, 40 * side ** 3 // 256 ** 3),
+                                                             Nt=1000, precision=capi.FP32)
+        cs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = capi.Run(be, capi.KIND_SINGLE, spec, sensors)
+            r.synchronize()
+            cs.append(time.perf_counter() - t0)
+            r.close()
+        create[str(side)] = round(1e3 * statistics.median(cs), 1)
+    print(f"setup form: run creation ms {create}", file=sys.stderr, flush=True)
+    spec, sensors, _ = workloads.spheres_single_particle(n=256, Nt=400, precision=capi.FP32)
+    per = {}
+    for tt in (0, 1):
+        sp = capi.ModeSpec(**{**spec.__dict__, "time_type": tt, "cc_step": 500, "cc_coeff": 0.0})
+        r = capi.Run(be, capi.KIND_SINGLE, sp, sensors)
+        r.step(100)
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.step(200)
+        r.synchronize()
+        per[tt] = (time.perf_counter() - t0) / 200 * 1e3
+        r.close()
+    rec = {"form": "setup", "discretize_256_40_spheres": disc, "run_create_ms": create,
+           "energy_check": {"ms_per_step_fixed": round(per[0], 4), "ms_per_step_energy": round(per[1], 4),
+                            "overhead": round(per[1] / per[0] - 1.0, 4)}}
+    print(json.dumps(rec))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,7 +397,7 @@ def main():
                     help="CPU baseline sample: (2 + S)-step run minus 2-step run of the oracle")
     ap.add_argument("--tiles", type=int, nargs=2, default=None, metavar=("E", "H"),
                     help="fixed tile configurations (0..31) instead of the autotuner's")
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving", "setup"],
                     help="plain: the headline single-particle run; precision: the 512x512x384 multilayer "
                          "stack in FP64, FP32 and FP16 storage with spectral errors against FP64; "
                          "batching: 64 2D runs batched in FP32 against one-at-a-time FP64 and FP32; "
@@ -370,6 +435,8 @@ def main():
         return batching_form(args)
     if args.form == "serving":
         return serving_form(args)
+    if args.form == "setup":
+        return setup_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
     nsteps = args.warmup + args.steps + args.prof_steps
     t_setup = time.perf_counter()
