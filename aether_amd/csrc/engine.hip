@@ -396,6 +396,13 @@ void launch_E_cfg(const EArgs<S, T> &a, int nblocks, hipStream_t st, const EArgs
     } else {
         const int naux = aux ? (aux->Nz + BX * BY - 1) / (BX * BY) : 0;
         const int nmat = aux ? std::max(a.nmat, aux->nmat) : a.nmat;
+        if constexpr (std::is_same_v<S, T>) {
+            if (a.en_part) {
+                hipLaunchKernelGGL((k_update_E<S, T, BY, KZ, true>), dim3(nblocks + naux), dim3(BX, BY, 1),
+                                   nmat * sizeof(MatL<T>), st, a, aux ? *aux : a, nblocks);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_update_E<S, T, BY, KZ>), dim3(nblocks + naux), dim3(BX, BY, 1), nmat * sizeof(MatL<T>), st,
                            a, aux ? *aux : a, nblocks);
     }
@@ -658,6 +665,23 @@ int launch_E(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
     g->strip_valid = false;             // E changes: the strip is stale until the next H pass
     EArgs<S, T> a = e_args<S, T>(g), x{};
     if (aux) x = e_args<S, T>(aux);
+    g->en_parts = 0;
+    if constexpr (std::is_same_v<S, T>) {
+        if (g->en_fuse && g->Nz > 1 && !is_column(g)) {
+            EArgs<S, T> t = a;
+            const int nb = tile_geometry(t, g->tile_e);
+            if (nb > g->en_part_cap) {
+                dfree_t(g->d_en_part);
+                g->en_part_cap = 0;
+                void *p;
+                if ((rc = dalloc(g, &p, (size_t)nb * sizeof(double)))) return rc;
+                g->d_en_part = (double *)p;
+                g->en_part_cap = nb;
+            }
+            a.en_part = g->d_en_part;
+            g->en_parts = nb;
+        }
+    }
     hipEvent_t ev;
     prof_begin(g, 0, &ev);
     dispatch_E<S, T>(g, a, g->tile_e, aux ? &x : nullptr);
@@ -1544,6 +1568,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree(g->d_sdesc);
     dfree(g->d_strip);
     dfree_t(g->d_energy);
+    dfree_t(g->d_en_part);
     dfree(g->d_osc);
     for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
@@ -1908,13 +1933,23 @@ int aefdtd::energy_feed(aefdtd_grid *g) {
         g->d_energy = (double *)p;
         HIPOK(g, fill_sync(g, g->d_energy, 0, (EN_BLOCKS + 2) * sizeof(double)));
     }
-    with_types(g, [&]<typename S, typename T>() {
-        hipLaunchKernelGGL((k_energy_part<S, T>), dim3(EN_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0),
-                           fld<S, T>(g, 1), fld<S, T>(g, 2), g->Nx, g->Ny * g->Nz, g->px, g->d_energy);
-        return AEFDTD_OK;
-    });
-    hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(256), 0, g->stream, (const double *)g->d_energy,
-                       g->d_energy + EN_BLOCKS);
+    if (g->en_parts > 0) {
+        // the E pass of this step left one partial per tile workgroup: summed
+        // in 256 chunks, then those in one workgroup
+        hipLaunchKernelGGL(k_energy_sum, dim3(256), dim3(256), 0, g->stream, (const double *)g->d_en_part,
+                           (long)g->en_parts, g->d_energy);
+        hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(256), 0, g->stream, (const double *)g->d_energy, 256,
+                           g->d_energy + EN_BLOCKS);
+        g->en_parts = 0;
+    } else {
+        with_types(g, [&]<typename S, typename T>() {
+            hipLaunchKernelGGL((k_energy_part<S, T>), dim3(EN_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0),
+                               fld<S, T>(g, 1), fld<S, T>(g, 2), g->Nx, g->Ny * g->Nz, g->px, g->d_energy);
+            return AEFDTD_OK;
+        });
+        hipLaunchKernelGGL(k_energy_fin, dim3(1), dim3(256), 0, g->stream, (const double *)g->d_energy, EN_BLOCKS,
+                           g->d_energy + EN_BLOCKS);
+    }
     HIPOK(g, hipGetLastError());
     return AEFDTD_OK;
 }

@@ -164,6 +164,10 @@ struct aefdtd_grid {
     void *d_strip = nullptr;          // [plane][Nz][Ny][2 columns][6] T
     bool strip_valid = false;
     double *d_energy = nullptr;       // completion energy: [EN_BLOCKS partials][last, max]
+    bool en_fuse = false;             // the 3D E pass also sums the energy (FP32/FP64 runs with the check)
+    double *d_en_part = nullptr;      // its per-workgroup partials
+    long en_part_cap = 0;
+    int en_parts = 0;                 // partials the last E launch wrote (0: none)
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
     int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
     // bookkeeping

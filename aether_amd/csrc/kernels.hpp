@@ -104,6 +104,7 @@ struct EArgs {
     int tfsf, tb0, tb1, tb2, tb3, tb4, tb5, tper_x, tper_y;
     Fld<S, T> aHx, aHy, aHz;   // aux 1D grid H columns
     int lm;                    // bits 0-2: E components loaded and stored, bits 3-5: H components loaded
+    double *en_part;           // non-null: per-workgroup sums of Ex^2+Ey^2+Ez^2 after the update
     int nosc, ostep;           // oscillators and the step their value is taken at
     const OscDev *osc;
 };
@@ -420,7 +421,7 @@ __device__ __forceinline__ V3<T> e_cell_vals(const EArgs<S, T> &a, const MatL<T>
 // Everything after the loads of one E cell, with the grid's CPML state; the
 // components in store mask `smask` (bit c: component c) are written back.
 template <typename S, typename T, class P>
-__device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
+__device__ __forceinline__ V3<T> e_cell(const EArgs<S, T> &a, const MatL<T> &M, int m, int i, int j, int k, int id,
                                        T ex0, T ey0, T ez0, T dHz_y, T dHy_z, T dHx_z, T dHz_x, T dHy_x, T dHx_y,
                                        T ikx, T iky, T ikz, int nx_, int ny_, int ostep, int side, int smask, P ps) {
     const V3<T> e = e_cell_vals(a, M, i, j, k, id, ex0, ey0, ez0, dHz_y, dHy_z, dHx_z, dHz_x, dHy_x, dHx_y, ikx, iky,
@@ -429,11 +430,12 @@ __device__ __forceinline__ void e_cell(const EArgs<S, T> &a, const MatL<T> &M, i
         a.Ex.set_nt(id, e.x);
         a.Ey.set_nt(id, e.y);
         a.Ez.set_nt(id, e.z);
-        return;
+        return e;
     }
     if (smask & 1) a.Ex.set(id, e.x);
     if (smask & 2) a.Ey.set(id, e.y);
     if (smask & 4) a.Ez.set(id, e.z);
+    return e;
 }
 
 // CPML slab index of an E cell along an axis (near 1..pm-1, far N-pp+1..N-1).
@@ -523,16 +525,21 @@ __device__ __forceinline__ void stage_mats(const EArgs<S, T> &a, MatL<T> *sm) {
 // the next plane's loads stay in flight (software pipeline: plane k+1 is
 // requested before plane k is computed and stored; the last plane of the
 // tile is peeled off so that the loop body never skips its loads).
-template <typename S, typename T, int BY, int KZ>
-__device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, int w, int ostep) {
+// With EN the lane also returns the sum of Ex^2+Ey^2+Ez^2 of the values it
+// stored (all three are stored in 3D), for the energy completion check; the
+// lane sums its KZ cells in the field type (one register more than without
+// the sum, which keeps the kernel at its occupancy), the workgroup in FP64.
+template <typename S, typename T, int BY, int KZ, bool EN = false>
+__device__ __forceinline__ double e_tile(const EArgs<S, T> &a, const MatL<T> *sm, int w, int ostep) {
     int bx, by, bz;
     tile_of(w, a.nbx, a.nby, a.nbz, a.remap, bx, by, bz);
     const int i = bx * BX + threadIdx.x;
     const int j = by * BY + threadIdx.y;
-    if (i >= a.Nx || j >= a.Ny) return;
+    if (i >= a.Nx || j >= a.Ny) return 0.0;
     const int k0 = bz * KZ;
     const int k1 = min(k0 + KZ, a.Nz);
     const int NxNy = a.P * a.Ny;                                 // plane stride
+    T en = 0;
     const int base = i + a.P * j;
     const int oim = (i == 0) ? a.Nx - 1 : -1;                    // (i-1) with wrap
     const int ojm = (j == 0) ? (a.Ny - 1) * a.P : -a.P;          // (j-1) with wrap
@@ -594,9 +601,10 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
         const T dHx_z = c.hx - hx_km;
         hx_km = c.hx;
         hy_km = c.hy;
-        e_cell(a, sm[c.m], c.m, i, j, k, base + k * NxNy, c.ex, c.ey, c.ez, c.hz - c.hz_jm, dHy_z, dHx_z,
-               c.hz - c.hz_im, c.hy - c.hy_im, c.hx - c.hx_jm, ikx, iky, uload(a.pf.ikE[2], k), nx_, ny_, ostep, side,
-               STORE_NT, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+        const V3<T> e = e_cell(a, sm[c.m], c.m, i, j, k, base + k * NxNy, c.ex, c.ey, c.ez, c.hz - c.hz_jm, dHy_z,
+                               dHx_z, c.hz - c.hz_im, c.hy - c.hy_im, c.hx - c.hx_jm, ikx, iky, uload(a.pf.ikE[2], k),
+                               nx_, ny_, ostep, side, STORE_NT, PrefPsiE<S, T>{a, i, j, k, q, bx_, cx_, by_, cy_});
+        if constexpr (EN) en += e.x * e.x + e.y * e.y + e.z * e.z;
     };
     Ld cur = load(k0);
     PsiSetE<T> q = load_psi(k0);
@@ -608,6 +616,7 @@ __device__ __forceinline__ void e_tile(const EArgs<S, T> &a, const MatL<T> *sm, 
         q = qn;
     }
     cell(cur, q, k1 - 1);
+    return (double)en;
 }
 
 // 2D E pass (Nz == 1) over tile w: each lane marches KY consecutive rows in y
@@ -678,7 +687,10 @@ __device__ __forceinline__ void e_col_cell(const EArgs<S, T> &a, const MatL<T> *
 // 3D E pass; workgroups from nb on update the 1 x 1 x Nz column `aux` instead
 // (the TFSF source grid, stepped in the same launch: it neither reads nor
 // writes anything the 3D pass touches).
-template <typename S, typename T, int BY, int KZ>
+// With EN, tile workgroup w also writes en_part[w], the sum of
+// Ex^2+Ey^2+Ez^2 over its cells after the update (lanes in a fixed order, so
+// the partials are the same on every run).
+template <typename S, typename T, int BY, int KZ, bool EN = false>
 __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a, const EArgs<S, T> aux, int nb) {
     extern __shared__ char lds_raw[];          // max(nmat) entries, sized at launch
     MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
@@ -689,7 +701,18 @@ __global__ __launch_bounds__(BX *BY) void k_update_E(const EArgs<S, T> a, const 
         return;
     }
     stage_mats(a, sm);
-    e_tile<S, T, BY, KZ>(a, sm, blockIdx.x, a.ostep);
+    double en = e_tile<S, T, BY, KZ, EN>(a, sm, blockIdx.x, a.ostep);
+    if constexpr (EN) {
+        __shared__ double red[BY];
+        for (int o = 32; o > 0; o >>= 1) en += __shfl_xor(en, o, 64);
+        if (threadIdx.x == 0) red[threadIdx.y] = en;
+        __syncthreads();
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            double t = 0;
+            for (int y = 0; y < BY; y++) t += red[y];
+            a.en_part[blockIdx.x] = t;
+        }
+    }
 }
 
 // Workgroup -> grid of a batch: grid r owns workgroups [wg0[r], wg0[r+1]),
@@ -1885,10 +1908,27 @@ __global__ __launch_bounds__(256) void k_energy_part(Fld<S, T> Ex, Fld<S, T> Ey,
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ __launch_bounds__(256) void k_energy_fin(const double *part, double *em) {
+// Sum of in[0..n) in gridDim.x contiguous chunks, one per workgroup, into
+// out[0..gridDim.x) (fixed order: the same sums on every run).
+__global__ __launch_bounds__(256) void k_energy_sum(const double *__restrict__ in, long n, double *__restrict__ out) {
+    __shared__ double sh[256];
+    const long chunk = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * chunk, hi = min(n, lo + chunk);
+    double acc = 0;
+    for (long b = lo + threadIdx.x; b < hi; b += 256) acc += in[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_fin(const double *part, int n, double *em) {
     __shared__ double sh[256];
     double acc = 0;
-    for (int b = threadIdx.x; b < EN_BLOCKS; b += 256) acc += part[b];
+    for (int b = threadIdx.x; b < n; b += 256) acc += part[b];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {

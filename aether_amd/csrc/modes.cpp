@@ -434,6 +434,7 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
         delete r;
         return rc;
     }
+    r->g->en_fuse = r->time_type == 1;      // the E pass then also sums the energy (3D, FP32/FP64)
     *out = r;
     return AEFDTD_OK;
 }
