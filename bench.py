@@ -19,7 +19,7 @@ FP64 restatement of Aether's CPU solver, threaded as Aether) timed on this
 host on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64]
-       [--no-cpu] [--cpu-steps S]
+       [--no-cpu] [--cpu-steps S] [--form plain|precision|batching|serving|setup|slab]
 """
 import argparse
 import json
@@ -317,6 +317,38 @@ def serving_form(args):
     print(json.dumps(rec))
 
 
+def slab_form(args):
+    """The 64^3 slab configuration in full (normal incidence, 64x64x90 grid,
+    2000 steps, 50 wavelengths, FP64): the engine's driver end to end (run
+    creation, stepping, R/T transforms) against the oracle's FP64 driver on the
+    host (all host threads of the oracle's pool), and their largest R/T
+    difference."""
+    import numpy as np
+    from aether_amd import capi, workloads
+    be = capi.engine()
+    spec = workloads.slab_normal_incidence()
+    capi.run_normal_incidence(be, spec)                 # warm
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        g = capi.run_normal_incidence(be, spec)
+        ts.append(time.perf_counter() - t0)
+    cells = 64 * 64 * 90 * spec.Nt
+    rec = {"form": "slab", "workload": "64^3 slab (eps 2.25, z cells 22-42), 8-cell CPML +-z, 2000 steps, "
+                                       "50 wavelengths, FP64, normal incidence",
+           "gpu_wall_s": round(min(ts), 4), "gpu_Mcell_updates_per_s": round(cells / min(ts) / 1e6, 1)}
+    if not args.no_cpu:
+        import oracle
+        ob = oracle.load()
+        t0 = time.perf_counter()
+        r = capi.run_normal_incidence(ob, spec)
+        dt = time.perf_counter() - t0
+        rec.update(cpu_wall_s=round(dt, 3), cpu_Mcell_updates_per_s=round(cells / dt / 1e6, 1),
+                   cpu_threads=int(ob.fn("get_threads")()),
+                   max_abs_diff_RT=float(max(np.max(np.abs(g[k] - r[k])) for k in ("R", "T"))))
+    print(json.dumps(rec))
+
+
 def setup_form(args):
     """The setup form: what a run costs before and around its time loop.
     (1) Structure::discretize of the headline structure (256^3 cells, 40
@@ -397,11 +429,13 @@ def main():
                     help="CPU baseline sample: (2 + S)-step run minus 2-step run of the oracle")
     ap.add_argument("--tiles", type=int, nargs=2, default=None, metavar=("E", "H"),
                     help="fixed tile configurations (0..31) instead of the autotuner's")
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving", "setup"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving", "setup", "slab"],
                     help="plain: the headline single-particle run; precision: the 512x512x384 multilayer "
                          "stack in FP64, FP32 and FP16 storage with spectral errors against FP64; "
                          "batching: 64 2D runs batched in FP32 against one-at-a-time FP64 and FP32; "
-                         "serving: a Poisson request stream through the request server")
+                         "serving: a Poisson request stream through the request server; "
+                         "setup: discretiser, run creation and the energy completion check; "
+                         "slab: the 64^3 slab configuration in full, FP64, engine against the oracle")
     ap.add_argument("--runs", type=int, default=64, help="runs of the batching form")
     ap.add_argument("--requests", type=int, default=24, help="requests of the serving form")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="device-memory budget of the serving form")
@@ -437,6 +471,8 @@ def main():
         return serving_form(args)
     if args.form == "setup":
         return setup_form(args)
+    if args.form == "slab":
+        return slab_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
     nsteps = args.warmup + args.steps + args.prof_steps
     t_setup = time.perf_counter()

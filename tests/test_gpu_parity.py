@@ -138,6 +138,19 @@ def test_normal_incidence_small_fp64(engine_be, oracle_be):
     assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
 
 
+def test_config1_slab_in_full_fp64(engine_be, oracle_be):
+    """The 64^3 slab configuration run in full: 64x64x90 grid (8-cell CPML and
+    5-cell pads on +-z), 2000 steps, 50 wavelengths, FP64, against the oracle."""
+    print("config 1: 64^3 slab, 2000 steps", flush=True)
+    spec = workloads.slab_normal_incidence()
+    g = capi.run_normal_incidence(engine_be, spec, want_fields=True)
+    r = capi.run_normal_incidence(oracle_be, spec, want_fields=True)
+    assert g["fields"].shape == (6, 90, 64, 64) and g["Nt_run"] == r["Nt_run"] == 2000
+    assert rel_l2(g["fields"], r["fields"]) <= FIELD_TOL
+    for k in ("R", "T", "refx", "refy", "transx", "transy"):
+        assert np.max(np.abs(g[k] - r[k])) <= 1e-9, k
+
+
 def test_single_particle_small_fp64(engine_be, oracle_be):
     spec, sens, _ = workloads.spheres_single_particle(n=20, D=2e-9, count=3, rmin=3e-9, rmax=6e-9, seed=5,
                                                       Nt=150, Nl=10, pml=5, precision=capi.FP64)
