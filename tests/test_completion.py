@@ -172,6 +172,21 @@ def test_engine_normal_incidence_stops_with_oracle_fp64(oracle_be):
 
 
 @pytest.mark.gpu
+def test_engine_2d_custom_stops_with_oracle_fp64(oracle_be):
+    """A 2D TM run (the separate energy pass, not the fused 3D one)."""
+    print("energy completion: 2D custom FP64", flush=True)
+    rng = workloads.MT19937_64(7)
+    spec, sens, _ = workloads.sweep_run(rng, 7, side=60, steps=6000, Nl=8, precision=capi.FP64)
+    spec = capi.mode_finalize(oracle_be, spec, capi.KIND_CUSTOM, sens)
+    spec.time_type, spec.cc_step, spec.cc_coeff = 1, CC_STEP, 1e-3
+    steps, g = run_engine(capi.KIND_CUSTOM, spec, sens)
+    r = capi.run_custom(oracle_be, spec, sens, want_fields=True)
+    assert steps == oracle.last_steps() < spec.Nt
+    assert rel_l2(g["fields"], r["fields"]) <= 1e-10
+    assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
+
+
+@pytest.mark.gpu
 def test_engine_fp32_stop_step(oracle_be):
     """FP32 fields change the energy sums in their last bits: the stop lands on
     the oracle's check or the next one."""
