@@ -63,7 +63,9 @@ class CMode(C.Structure):
                 ("n_materials", C.c_int), ("materials", C.POINTER(CMaterial)),
                 ("precision", C.c_int),
                 ("n_sources", C.c_int), ("sources", C.POINTER(CSourceReq)),
-                ("time_type", C.c_int), ("cc_step", C.c_int), ("cc_coeff", C.c_double)]
+                ("time_type", C.c_int), ("cc_step", C.c_int), ("cc_coeff", C.c_double),
+                ("cc_lmin", C.c_double), ("cc_lmax", C.c_double), ("cc_quant", C.c_int),
+                ("cc_layout", C.c_char * 4), ("cc_seed", C.c_uint)]
 
 
 class CSensorReq(C.Structure):
@@ -332,9 +334,14 @@ class ModeSpec:
     disable_fields: tuple = ()
     precision: int = FP64
     sources: list = field(default_factory=list)   # SourceReq, custom mode
-    time_type: int = 0                        # 1: auto_tsteps with the energy criterion
+    time_type: int = 0                        # 1: auto_tsteps, energy criterion; 2: spectral criterion
     cc_step: int = 100
     cc_coeff: float = 1e-3
+    cc_lmin: float = 370e-9                   # spectral criterion (FDTD_Mode defaults, lua_fdtd.cpp:255-257)
+    cc_lmax: float = 850e-9
+    cc_quant: int = 500
+    cc_layout: str = "nnb"
+    cc_seed: int = 0
 
     def to_c(self, keep):
         mg = np.ascontiguousarray(self.matsgrid, dtype=np.uint32)
@@ -367,6 +374,9 @@ class ModeSpec:
         c.n_sources = len(self.sources)
         c.sources = srcs
         c.time_type, c.cc_step, c.cc_coeff = self.time_type, self.cc_step, self.cc_coeff
+        c.cc_lmin, c.cc_lmax, c.cc_quant = self.cc_lmin, self.cc_lmax, self.cc_quant
+        c.cc_layout = self.cc_layout.encode()[:3]
+        c.cc_seed = int(self.cc_seed) & 0xFFFFFFFF
         return c
 
 

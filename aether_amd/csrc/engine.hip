@@ -1963,6 +1963,57 @@ int aefdtd::energy_read(aefdtd_grid *g, double *last, double *max) {
     return AEFDTD_OK;
 }
 
+int aefdtd::ft_setup(aefdtd_grid *g, FTState &f, const std::vector<int> &pts, const std::vector<double> &w,
+                     int navg) {
+    ft_free(f);
+    const int np = (int)w.size();
+    if (np < 1 || navg < 1 || pts.size() != 3 * w.size()) return fail(g, AEFDTD_EINVAL, "bad completion points");
+    for (int p = 0; p < np; p++)
+        if (pts[3 * p] < 0 || pts[3 * p] >= g->Nx || pts[3 * p + 1] < 0 || pts[3 * p + 1] >= g->Ny ||
+            pts[3 * p + 2] < 0 || pts[3 * p + 2] >= g->Nz)
+            return fail(g, AEFDTD_EINVAL, "completion point outside the grid");
+    void *p0, *p1, *p2, *p3;
+    int rc;
+    if ((rc = dalloc(g, &p0, pts.size() * sizeof(int))) || (rc = dalloc(g, &p1, w.size() * sizeof(double))) ||
+        (rc = dalloc(g, &p2, (size_t)6 * np * sizeof(double))) ||
+        (rc = dalloc(g, &p3, (size_t)6 * np * navg * sizeof(double))))
+        return rc;
+    f.d_pts = (int *)p0; f.d_w = (double *)p1; f.d_acc = (double *)p2; f.d_ring = (double *)p3;
+    f.np = np;
+    f.navg = navg;
+    HIPOK(g, copy_sync(g, f.d_pts, pts.data(), pts.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPOK(g, copy_sync(g, f.d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPOK(g, fill_sync(g, f.d_acc, 0, (size_t)6 * np * sizeof(double)));
+    HIPOK(g, fill_sync(g, f.d_ring, 0, (size_t)6 * np * navg * sizeof(double)));
+    return AEFDTD_OK;
+}
+
+int aefdtd::ft_feed(aefdtd_grid *g, FTState &f, int step, double Dt) {
+    if (!f.np) return AEFDTD_OK;
+    with_types(g, [&]<typename S, typename T>() {
+        hipLaunchKernelGGL((k_ft_feed<S, T>), dim3((f.np + 255) / 256), dim3(256), 0, g->stream, (const int *)f.d_pts,
+                           (const double *)f.d_w, f.np, step, Dt, f.navg, fld<S, T>(g, 0), fld<S, T>(g, 1),
+                           fld<S, T>(g, 2), g->Nx, g->Ny, g->Nz, g->px, f.d_acc, f.d_ring);
+        return AEFDTD_OK;
+    });
+    HIPOK(g, hipGetLastError());
+    return AEFDTD_OK;
+}
+
+int aefdtd::ft_read(aefdtd_grid *g, const FTState &f, std::vector<double> &ring) {
+    ring.resize((size_t)6 * f.np * f.navg);
+    HIPOK(g, copy_sync(g, ring.data(), f.d_ring, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return AEFDTD_OK;
+}
+
+void aefdtd::ft_free(FTState &f) {
+    dfree_t(f.d_pts);
+    dfree_t(f.d_w);
+    dfree_t(f.d_acc);
+    dfree_t(f.d_ring);
+    f.np = f.navg = 0;
+}
+
 int aefdtd::sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2) {
     if (!g || id < 0 || id >= (int)g->sensors.size()) return AEFDTD_EINVAL;
     *span1 = g->sensors[id]->span1;

@@ -80,6 +80,24 @@ int batch_synchronize(Batch *b);
 // queues it and updates the running maximum; energy_read waits and reads both.
 int energy_feed(aefdtd_grid *g);
 int energy_read(aefdtd_grid *g, double *last, double *max);
+
+// Device state of the spectral completion criterion: np watched cells, their
+// running DFT of local_E (FP64 complex, 3 components) and the last navg values
+// of its real and imaginary parts ([6][np][navg], Ex Ey Ez real then imaginary).
+struct FTState {
+    int np = 0, navg = 0;
+    int *d_pts = nullptr;             // [3 * np] i, j, k
+    double *d_w = nullptr;            // [np] angular frequencies
+    double *d_acc = nullptr;          // [6 * np]
+    double *d_ring = nullptr;         // [6 * np * navg]
+};
+int ft_setup(aefdtd_grid *g, FTState &f, const std::vector<int> &pts, const std::vector<double> &w, int navg);
+int ft_feed(aefdtd_grid *g, FTState &f, int step, double Dt);
+int ft_read(aefdtd_grid *g, const FTState &f, std::vector<double> &ring);
+void ft_free(FTState &f);
+int ft_place(const aefdtd_grid *g, double Dt, double lmin, double lmax, int Np, const char layout[4], unsigned seed,
+             std::vector<int> &pts, std::vector<double> &w, int &navg);
+bool ft_complete(const std::vector<double> &ring, int np, int navg, double coeff, int steps, int nmax);
 // Plane extent (cells) of sensor id after its link to the grid.
 int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
 // A grid and its attached TFSF column stepped with one launch per half step.

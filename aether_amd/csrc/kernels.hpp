@@ -1864,6 +1864,38 @@ __global__ __launch_bounds__(64 * TB_WAVES) void k_tb2d_batch(const EArgs<S, T> 
         }
 }
 
+// Spectral completion criterion, one lane per watched cell
+// (CompletionSensor::deep_feed, sensors/s_completion.cpp:132-158): the running
+// DFT acc += local_E * exp(i w step Dt) in FP64 (local_E as local_Ex..Ez,
+// core/fdtd_utils.cpp:457-479), and its real and imaginary parts stored at
+// slot step % navg of the ring.
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_ft_feed(const int *__restrict__ pts, const double *__restrict__ w, int np,
+                                                 int step, double Dt, int navg, Fld<S, T> Ex, Fld<S, T> Ey,
+                                                 Fld<S, T> Ez, int Nx, int Ny, int Nz, int P,
+                                                 double *__restrict__ acc, double *__restrict__ ring) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= np) return;
+    const int i = pts[3 * p], j = pts[3 * p + 1], k = pts[3 * p + 2];
+    const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
+    auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + P * (y + Ny * z)]; };
+    const double e[3] = {
+        (double)((at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0),
+        (double)((at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0),
+        (double)((at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0)};
+    double sn, cs;
+    sincos(w[p] * step * Dt, &sn, &cs);
+    const int off = step % navg;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double re = acc[(2 * c) * np + p] + e[c] * cs, im = acc[(2 * c + 1) * np + p] + e[c] * sn;
+        acc[(2 * c) * np + p] = re;
+        acc[(2 * c + 1) * np + p] = im;
+        ring[((size_t)c * np + p) * navg + off] = re;
+        ring[((size_t)(3 + c) * np + p) * navg + off] = im;
+    }
+}
+
 // Slot table of the dispersive cells: slot[cells[i]] = i (the table is -1
 // elsewhere); cells holds device indices, all below the table's length.
 __global__ __launch_bounds__(256) void k_slot_scatter(const int32_t *__restrict__ cells, long n,

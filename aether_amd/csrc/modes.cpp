@@ -419,7 +419,8 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
     r->lambda_max = m->lambda_max;
     r->reqs.assign(sensors, sensors + std::max(0, n_sensors));
     if (kind == 2 && m->n_sources > 0) r->srcs.assign(m->sources, m->sources + m->n_sources);
-    if (m->time_type < 0 || m->time_type > 1 || (m->time_type == 1 && m->cc_step < 1)) {
+    if (m->time_type < 0 || m->time_type > 2 || (m->time_type != 0 && m->cc_step < 1) ||
+        (m->time_type == 2 && (m->cc_quant < 1 || !(m->cc_lmin > 0) || !(m->cc_lmax > 0)))) {
         delete r;
         return AEFDTD_EINVAL;
     }
@@ -435,6 +436,19 @@ int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m, int n_se
         return rc;
     }
     r->g->en_fuse = r->time_type == 1;      // the E pass then also sums the energy (3D, FP32/FP64)
+    if (r->time_type == 2) {
+        // the sensor is linked to the main grid after its setup (single_particle.cpp:216-229)
+        std::vector<int> pts;
+        std::vector<double> w;
+        int navg = 0;
+        rc = aefdtd::ft_place(r->g, r->Dt, m->cc_lmin, m->cc_lmax, m->cc_quant, m->cc_layout, m->cc_seed, pts, w, navg);
+        if (!rc) rc = aefdtd::ft_setup(r->g, r->ft, pts, w, navg);
+        if (rc) {
+            set_global_error(r->g->err.empty() ? "bad spectral completion parameters" : r->g->err);
+            delete r;
+            return rc;
+        }
+    }
     *out = r;
     return AEFDTD_OK;
 }
@@ -460,18 +474,28 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
             if ((rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
                 return rfail(r, rc);
         }
-        if (r->time_type == 1) {
-            // the energy CompletionSensor is fed with the other sensors; the drivers check it
+        if (r->time_type != 0) {
+            // the CompletionSensor is fed with the other sensors; the drivers check it
             // when t % cc_step == 0 (single_particle.cpp:396-401, normal_incidence.cpp:417-423,
             // default_fdtd.cpp:193-195): complete once its step count reaches twice the
-            // longest grid axis and the last energy is below cc_coeff of the largest
-            // (s_completion.cpp:39-41,122-128)
-            if ((rc = aefdtd::energy_feed(r->g))) return rfail(r, rc);
+            // longest grid axis and, for the energy criterion, the last energy is below
+            // cc_coeff of the largest (s_completion.cpp:39-41,122-128), for the spectral
+            // one, 90% of the watched cells have settled (:43-120)
+            if (r->time_type == 1) rc = aefdtd::energy_feed(r->g);
+            else rc = aefdtd::ft_feed(r->g, r->ft, r->t, r->Dt);
+            if (rc) return rfail(r, rc);
             if (r->t % r->cc_step == 0) {
-                double last = 0, mx = 0;
-                if ((rc = aefdtd::energy_read(r->g, &last, &mx))) return rfail(r, rc);
                 const int steps = r->t + 1, nmax = std::max(r->g->Nx, std::max(r->g->Ny, r->g->Nz));
-                if (steps >= 2 * nmax && last / mx < r->cc_coeff) {
+                bool complete = false;
+                if (r->time_type == 1) {
+                    double last = 0, mx = 0;
+                    if ((rc = aefdtd::energy_read(r->g, &last, &mx))) return rfail(r, rc);
+                    complete = steps >= 2 * nmax && last / mx < r->cc_coeff;
+                } else if (steps >= 2 * nmax) {
+                    if ((rc = aefdtd::ft_read(r->g, r->ft, r->ft_ring))) return rfail(r, rc);
+                    complete = aefdtd::ft_complete(r->ft_ring, r->ft.np, r->ft.navg, r->cc_coeff, steps, nmax);
+                }
+                if (complete) {
                     r->done = true;
                     if (r->kind == 0) r->nrun = r->t;      // normal incidence: Nt = t, then break
                     r->t++;

@@ -21,12 +21,15 @@ struct aefdtd_run {
     double index_sup = 1, index_sub = 1;
     std::string err;
     struct aefdtd_batch *batch = nullptr;   // set while a member of a batch
-    // auto_tsteps with the energy criterion (time_type 1); done once it stopped
+    // auto_tsteps: energy (time_type 1) or spectral (2) criterion; done once it stopped
     int time_type = 0, cc_step = 1;
     double cc_coeff = 0;
     bool done = false;
     int nrun = -1;                    // normal incidence: the step count its spectra use
+    aefdtd::FTState ft;               // the spectral criterion's watched cells
+    std::vector<double> ft_ring;
     ~aefdtd_run() {
+        aefdtd::ft_free(ft);
         aefdtd_destroy(g);
         aefdtd_destroy(a);
     }

@@ -372,6 +372,7 @@ class FDTDMode:
         self.D = [5e-9, 5e-9, 5e-9]
         self.Nt, self.tapering, self.time_mod = 5000, 0, 1.0
         self.time_type, self.cc_step, self.cc_coeff = 0, 500, 1e-3
+        self.cc_lmin, self.cc_lmax, self.cc_quant, self.cc_layout = 370e-9, 850e-9, 500, "nnb"
         self.Nl, self.lambda_min, self.lambda_max = 481, 370e-9, 850e-9
         self.polarization, self.polar_angle = "TE", 0.0
         self.prefix, self.directory = "", ""
@@ -412,7 +413,9 @@ class FDTDMode:
                              alpha=tuple(self.alpha), Nl=self.Nl, lambda_min=self.lambda_min,
                              lambda_max=self.lambda_max, polarization=pol, polar_angle=self.polar_angle,
                              disable_fields=tuple(self.disable), precision=self.precision, sources=sources,
-                             time_type=self.time_type, cc_step=self.cc_step, cc_coeff=self.cc_coeff)
+                             time_type=self.time_type, cc_step=self.cc_step, cc_coeff=self.cc_coeff,
+                             cc_lmin=self.cc_lmin, cc_lmax=self.cc_lmax, cc_quant=self.cc_quant,
+                             cc_layout=self.cc_layout, cc_seed=self.front.seed)
         spec = capi.mode_finalize(self.front.be, spec, self.kind, sensors)
         return spec, sensors
 
@@ -507,12 +510,20 @@ def _spectrum(self, lmin, lmax, Nl=None):
 
 
 def _auto_tsteps(self, Nt, step, *rest):
+    """FDTD_mode_set_auto_tsteps (lua_fdtd.cpp:440-466): (Nt, step, coeff) selects
+    the energy criterion; (Nt, step, lmin, lmax, coeff, quant [, layout]) the
+    spectral one, whose cells are drawn from the generator seeded by rng_seed."""
     self.Nt, self.cc_step = _int(Nt, "auto_tsteps"), _int(step, "auto_tsteps")
     if len(rest) == 1:
         self.cc_coeff = _num(rest[0], "auto_tsteps")
         self.time_type = 1
+    elif len(rest) in (4, 5):
+        self.cc_lmin, self.cc_lmax = _num(rest[0], "auto_tsteps"), _num(rest[1], "auto_tsteps")
+        self.cc_coeff, self.cc_quant = _num(rest[2], "auto_tsteps"), _int(rest[3], "auto_tsteps")
+        self.cc_layout = tostring(rest[4]) if len(rest) == 5 else "nnb"
+        self.time_type = 2
     else:
-        raise LuaError("auto_tsteps: the spectral completion criterion is not provided (energy form only)")
+        raise LuaError("auto_tsteps: 3, 6 or 7 arguments expected")
 
 
 def _padding(self, *p):
@@ -579,6 +590,7 @@ class Front:
         self.results = []
         self.caller_dir = "."
         self.log = []
+        self.seed = 0               # rng_seed(n); the reference seeds its generator with the time
 
     def run_file(self, path):
         self.caller_dir = os.path.dirname(os.path.abspath(path))
@@ -620,9 +632,12 @@ class Front:
         def fail_test(*a):
             raise TestFailure("fail_test() called by the script")
 
+        def rng_seed(n):
+            front.seed = _int(n, "rng_seed") & 0xFFFFFFFF
+
         it.G.update(MODE=mode, Material=material, Structure=structure, create_sensor=lambda t: SensorGen(tostring(t)),
                     create_source=lambda t: SourceGen(tostring(t)), nearest_integer=nearest_integer,
-                    fail_test=fail_test)
+                    fail_test=fail_test, rng_seed=rng_seed)
         # const_material returns a material script path in the reference; here the
         # material itself, which FDTD_Mode:material accepts the same way
         it.G["const_material"] = const_material

@@ -220,14 +220,23 @@ typedef struct aefdtd_mode {
     int     n_materials; const aefdtd_material *materials;
     int     precision;           /* AEFDTD_FP64 / AEFDTD_FP32 / AEFDTD_FP16 */
     int     n_sources;  const struct aefdtd_source_req *sources;   /* custom mode */
-    /* auto_tsteps (FDTD_Mode::set_auto_tsteps(Nt, cc_step, cc_coeff),
-     * modes/lua_fdtd.cpp:271-278): time_type 1 stops a run early with the energy
-     * CompletionSensor (sensors/s_completion.cpp:23-27,39-130,160-180), checked
-     * every cc_step steps; 0 runs all Nt steps.  The spectral (FT) criterion
-     * of the reference is not provided. */
+    /* auto_tsteps (FDTD_Mode::set_auto_tsteps, modes/lua_fdtd.cpp:271-294):
+     * time_type 0 runs all Nt steps; 1 stops early with the energy
+     * CompletionSensor, 2 with its spectral (FT) criterion
+     * (sensors/s_completion.cpp:26-36,39-130,132-163,232-398), checked every
+     * cc_step steps.  The FT criterion watches cc_quant points placed on the
+     * structure region's faces by the 3-letter cc_layout (per axis n none,
+     * d low face, u high face, b both; "v" the volume; unknown or "nnn" become
+     * "nnb"), at wavelengths drawn in [cc_lmin, cc_lmax]; placement and
+     * wavelengths are drawn from std::mt19937 seeded with cc_seed as the
+     * reference draws them from its generator after rng_seed(cc_seed). */
     int     time_type;
     int     cc_step;
     double  cc_coeff;
+    double  cc_lmin, cc_lmax;
+    int     cc_quant;
+    char    cc_layout[4];
+    unsigned int cc_seed;
 } aefdtd_mode;
 
 /* A source request (Source_generator, core/sources.h:29-59, already

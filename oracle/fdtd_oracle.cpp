@@ -33,6 +33,7 @@
 #include <functional>
 #include <mutex>
 #include <numbers>
+#include <random>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1500,36 +1501,175 @@ void copy_fields(aeoracle_grid *g, double *out) {
     for (int c = 0; c < 6; c++) std::memcpy(out + c * N, g->F[c].data(), N * sizeof(double));
 }
 
-// CompletionSensor in energy mode (sensors/s_completion.cpp:20-24,39-41,122-128,
-// 160-180): every feed sums Ex^2+Ey^2+Ez^2 over the whole grid in k, j, i order
-// and keeps the running maximum; the drivers call the check when t % cc_step
-// == 0, and it completes once the sensor's step count (t + 1 after the feed of
-// step t) reaches twice the longest axis and last/max < cc_coeff.
-std::vector<double> g_energy_trace;   // every feed's sum, for the tests
+// CompletionSensor (sensors/s_completion.cpp), fed after the other sensors
+// every step; the drivers check it when t % cc_step == 0 and it never
+// completes before its step count (t + 1 after the feed of step t) reaches
+// twice the longest axis (:39-41).
+//  Energy mode (:122-128,160-180): every feed sums Ex^2+Ey^2+Ez^2 over the
+//  whole grid in k, j, i order and keeps the running maximum; complete when
+//  last/max < cc_coeff.
+//  FT mode (:26-36,43-120,132-158,232-398): cc_quant cells placed on the
+//  structure region's faces by cc_layout (or in its volume), at wavelengths
+//  drawn in [cc_lmin, cc_lmax] from a std::mt19937 (randp, common/mathUT.cpp:
+//  65-68); each feed adds local_E * exp(i w step Dt) to a running DFT per cell
+//  and keeps its last navg values; complete when 90% of the cells' largest
+//  averaged series vary by at most cc_coeff of their average over the ring
+//  (the six averages carry over from one cell to the next, as written).
+std::vector<double> g_energy_trace;   // every energy feed's sum, for the tests
+std::vector<int> g_ft_pts;            // the last FT run's cells (i, j, k), wavelengths, navg, ring, checks
+std::vector<double> g_ft_lambda, g_ft_ring;
+std::vector<int> g_ft_checks;         // per check: sensor step count, settled cells
+int g_ft_navg = 0;
 
 struct EnergyCheck {
     const aefdtd_mode *m;
-    EnergyCheck(const aefdtd_mode *m_) : m(m_) { g_energy_trace.clear(); }
+    const aeoracle_grid *g;
     double last = 0, max = 0;
-    bool on() const { return m->time_type == 1; }
-    bool feed_and_check(const aeoracle_grid *g, int t) {
-        last = 0;
-        const size_t N = g->F[0].size();
-        for (size_t c = 0; c < N; c++) {
-            const double ex = g->F[0][c], ey = g->F[1][c], ez = g->F[2][c];
-            last += ex * ex + ey * ey + ez * ez;
+    // FT mode
+    int np = 0, navg = 0;
+    std::vector<int> I, J, K;
+    std::vector<double> w;
+    std::vector<std::complex<double>> acc;   // [3 * np]
+    std::vector<double> ring;                // [6][np][navg]
+    double Dt = 0;
+
+    EnergyCheck(const aefdtd_mode *m_, const aeoracle_grid *g_, double Dt_) : m(m_), g(g_), Dt(Dt_) {
+        g_energy_trace.clear();
+        if (m->time_type == 2) place();
+    }
+    bool on() const { return m->time_type != 0; }
+
+    static double randp(std::mt19937 &mte, double A, double B) {
+        return A + (B - A) * mte() / (mte.max() - mte.min());
+    }
+
+    void place() {   // CompletionSensor::initialize (s_completion.cpp:232-398)
+        std::mt19937 mte(m->cc_seed);
+        std::string layout(m->cc_layout, strnlen(m->cc_layout, 4));
+        bool known = layout == "v";
+        if (layout.size() == 3) {
+            known = true;
+            for (char c : layout) known = known && (c == 'n' || c == 'd' || c == 'u' || c == 'b');
         }
-        max = std::max(max, last);
-        g_energy_trace.push_back(last);
+        if (layout == "nnn" || !known) layout = "nnb";
+        np = m->cc_quant;
+        I.assign(np, 0); J.assign(np, 0); K.assign(np, 0);
+        auto rx = [&]() { return static_cast<int>(randp(mte, g->xs_s, g->xs_e - 1) + 0.5); };
+        auto ry = [&]() { return static_cast<int>(randp(mte, g->ys_s, g->ys_e - 1) + 0.5); };
+        auto rz = [&]() { return static_cast<int>(randp(mte, g->zs_s, g->zs_e - 1) + 0.5); };
+        if (layout == "v") {
+            for (int i = 0; i < np; i++) { I[i] = rx(); J[i] = ry(); K[i] = rz(); }
+        } else {
+            int Np_xd = 0, Np_xu = 0, Np_yd = 0, Np_yu = 0, Np_zd = 0, Np_zu = 0;
+            const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz;
+            if (layout[0] == 'd') Np_xd = Ny * Nz;
+            if (layout[0] == 'u') Np_xu = Ny * Nz;
+            if (layout[0] == 'b') Np_xd = Np_xu = Ny * Nz;
+            if (layout[1] == 'd') Np_yd = Nx * Nz;
+            if (layout[1] == 'u') Np_yu = Nx * Nz;
+            if (layout[1] == 'b') Np_yd = Np_yu = Nx * Nz;
+            if (layout[2] == 'd') Np_zd = Nx * Ny;
+            if (layout[2] == 'u') Np_zu = Nx * Ny;
+            if (layout[2] == 'b') Np_zd = Np_zu = Nx * Ny;
+            const double tot = Np_xd + Np_xu + Np_yd + Np_yu + Np_zd + Np_zu;
+            int left = np;
+            for (int *q : {&Np_xd, &Np_xu, &Np_yd, &Np_yu, &Np_zd, &Np_zu}) {
+                int a = static_cast<int>(np * *q / tot + 0.5);
+                a = a < left ? a : left;
+                left -= a;
+                *q = a;
+            }
+            int o = 0;
+            for (int i = o; i < o + Np_xd; i++) { I[i] = g->xs_s; J[i] = ry(); K[i] = rz(); }
+            o += Np_xd;
+            for (int i = o; i < o + Np_xu; i++) { I[i] = g->xs_e - 1; J[i] = ry(); K[i] = rz(); }
+            o += Np_xu;
+            for (int i = o; i < o + Np_yd; i++) { I[i] = rx(); J[i] = g->ys_s; K[i] = rz(); }
+            o += Np_yd;
+            for (int i = o; i < o + Np_yu; i++) { I[i] = rx(); J[i] = g->ys_e - 1; K[i] = rz(); }
+            o += Np_yu;
+            for (int i = o; i < o + Np_zd; i++) { I[i] = rx(); J[i] = ry(); K[i] = g->zs_s; }
+            o += Np_zd;
+            for (int i = o; i < o + Np_zu; i++) { I[i] = rx(); J[i] = ry(); K[i] = g->zs_e - 1; }
+        }
+        std::vector<double> lam(np);
+        w.assign(np, 0);
+        for (int i = 0; i < np; i++) {
+            lam[i] = randp(mte, m->cc_lmin, m->cc_lmax);
+            w[i] = 2.0 * Pi * c_light / lam[i];             // m_to_rad_Hz (common/phys_tools.cpp:159)
+        }
+        const double T = std::max(m->cc_lmin, m->cc_lmax) / c_light;
+        navg = static_cast<int>(T / Dt);
+        acc.assign(3 * (size_t)np, 0.0);
+        ring.assign(6 * (size_t)np * navg, 0.0);
+        g_ft_pts.clear();
+        for (int i = 0; i < np; i++) { g_ft_pts.push_back(I[i]); g_ft_pts.push_back(J[i]); g_ft_pts.push_back(K[i]); }
+        g_ft_lambda = lam;
+        g_ft_navg = navg;
+        g_ft_checks.clear();
+    }
+
+    double local(int c, int i, int j, int k) const {   // FDTD::local_Ex..Ez (core/fdtd_utils.cpp:457-479)
+        const int ip = i + 1 == g->Nx ? 0 : i + 1, jp = j + 1 == g->Ny ? 0 : j + 1, kp = k + 1 == g->Nz ? 0 : k + 1;
+        auto F = [&](int x, int y, int z) { return g->F[c][(size_t)x + (size_t)g->Nx * ((size_t)y + (size_t)g->Ny * z)]; };
+        if (c == 0) return (F(i, j, k) + F(i, jp, k) + F(i, j, kp) + F(i, jp, kp)) / 4.0;
+        if (c == 1) return (F(i, j, k) + F(ip, j, k) + F(i, j, kp) + F(ip, j, kp)) / 4.0;
+        return (F(i, j, k) + F(ip, j, k) + F(i, jp, k) + F(ip, jp, k)) / 4.0;
+    }
+
+    bool feed_and_check(const aeoracle_grid *, int t) {
+        const int steps = t + 1, nmax = std::max(g->Nx, std::max(g->Ny, g->Nz));
+        if (m->time_type == 1) {
+            last = 0;
+            const size_t N = g->F[0].size();
+            for (size_t c = 0; c < N; c++) {
+                const double ex = g->F[0][c], ey = g->F[1][c], ez = g->F[2][c];
+                last += ex * ex + ey * ey + ez * ez;
+            }
+            max = std::max(max, last);
+            g_energy_trace.push_back(last);
+            if (t % m->cc_step) return false;
+            if (steps < 2 * nmax) return false;
+            return last / max < m->cc_coeff;
+        }
+        const std::complex<double> Im(0.0, 1.0);
+        const int off = t % navg;
+        for (int i = 0; i < np; i++) {
+            const std::complex<double> coeff = std::exp(w[i] * t * Dt * Im);
+            for (int c = 0; c < 3; c++) {
+                acc[3 * i + c] += local(c, I[i], J[i], K[i]) * coeff;
+                ring[((size_t)c * np + i) * navg + off] = acc[3 * i + c].real();
+                ring[((size_t)(3 + c) * np + i) * navg + off] = acc[3 * i + c].imag();
+            }
+        }
+        g_ft_ring = ring;
         if (t % m->cc_step) return false;
-        if (t + 1 < 2 * std::max(g->Nx, std::max(g->Ny, g->Nz))) return false;
-        return last / max < m->cc_coeff;
+        if (steps < 2 * nmax) return false;
+        double ar[6] = {0, 0, 0, 0, 0, 0};
+        int nok = 0;
+        for (int i = 0; i < np; i++) {
+            for (int j = 0; j < navg; j++)
+                for (int c = 0; c < 6; c++) ar[c] += ring[((size_t)c * np + i) * navg + j];
+            for (int c = 0; c < 6; c++) ar[c] = std::abs(ar[c] / (navg + 0.0));
+            const double amax = std::max({ar[0], ar[1], ar[2], ar[3], ar[4], ar[5]});
+            int tg = 0;
+            for (int c = 5; c >= 0; c--)
+                if (amax == ar[c]) tg = c;                    // the first series with the largest average
+            const double *row = &ring[((size_t)tg * np + i) * navg];
+            double tmin = row[0], tmax = row[0];
+            for (int k = 0; k < navg; k++) { tmin = std::min(tmin, row[k]); tmax = std::max(tmax, row[k]); }
+            if (std::abs(tmax - tmin) <= amax * m->cc_coeff) nok++;
+        }
+        g_ft_checks.push_back(steps);
+        g_ft_checks.push_back(nok);
+        return nok / (np + 0.0) >= 0.9;
     }
 };
 
 int check_time_type(const aefdtd_mode *m) {
     if (m->time_type == 0) return AEFDTD_OK;
-    if (m->time_type != 1 || m->cc_step < 1) return AEFDTD_EINVAL;
+    if (m->time_type < 0 || m->time_type > 2 || m->cc_step < 1) return AEFDTD_EINVAL;
+    if (m->time_type == 2 && (m->cc_quant < 1 || !(m->cc_lmin > 0) || !(m->cc_lmax > 0))) return AEFDTD_EINVAL;
     return AEFDTD_OK;
 }
 
@@ -1538,6 +1678,21 @@ int g_last_steps = 0;
 }  // namespace
 
 int aeoracle_last_steps(void) { return g_last_steps; }
+
+int aeoracle_ft_info(int *navg, int *np, int *nchecks) {
+    *navg = g_ft_navg;
+    *np = (int)g_ft_lambda.size();
+    *nchecks = (int)g_ft_checks.size() / 2;
+    return AEFDTD_OK;
+}
+
+int aeoracle_ft_read(int *pts, double *lambda, double *ring, int *checks) {
+    if (pts) std::copy(g_ft_pts.begin(), g_ft_pts.end(), pts);
+    if (lambda) std::copy(g_ft_lambda.begin(), g_ft_lambda.end(), lambda);
+    if (ring) std::copy(g_ft_ring.begin(), g_ft_ring.end(), ring);
+    if (checks) std::copy(g_ft_checks.begin(), g_ft_checks.end(), checks);
+    return AEFDTD_OK;
+}
 
 int aeoracle_energy_trace(double *out, int n) {
     const int m = (int)g_energy_trace.size();
@@ -1599,7 +1754,7 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
     int idR, idT;
     aeoracle_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &idR);
     aeoracle_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT);
-    EnergyCheck E(m);
+    EnergyCheck E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         g->update_E();
@@ -1746,7 +1901,7 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
         if ((rc = aeoracle_add_oscillator(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, q.lambda_max, &id)))
             return rc;
     }
-    EnergyCheck E(m);
+    EnergyCheck E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         g->update_E();       // + sources' inject_E
@@ -1819,7 +1974,7 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefd
     if ((rc = aeoracle_set_tfsf(g, a, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e, periodic_x, periodic_y))) return rc;
     SensorSet S;
     if ((rc = add_requests(g, n_sensors, sensors, S))) return rc;
-    EnergyCheck E(m);
+    EnergyCheck E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         a->update_E();

@@ -101,6 +101,12 @@ int aeoracle_discretize(const aefdtd_structure *s, int Nx, int Ny, int Nz, doubl
 /* The energy sums of the last driver call with the energy completion check,
  * one per step; copies up to n into out and returns how many there are. */
 int aeoracle_energy_trace(double *out, int n);
+/* The spectral completion criterion of the last driver call that used it: the
+ * ring length, cell count and number of checks; then the cells (i, j, k), their
+ * wavelengths, the last ring [6][np][navg] and per check (step count, settled
+ * cells). */
+int aeoracle_ft_info(int *navg, int *np, int *nchecks);
+int aeoracle_ft_read(int *pts, double *lambda, double *ring, int *checks);
 /* Steps a configured grid n full steps (update_E, update_H) with no driver
  * actions: the timed CPU-baseline loop. */
 int aeoracle_step(aeoracle_grid *g, int n);

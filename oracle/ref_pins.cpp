@@ -39,6 +39,9 @@ int aeref_eval(int kind, const double *p, double eps_inf, double w, double *out)
 }
 
 double aeref_s_curve(double x, double xmin, double xmax) { return s_curve(x, xmin, xmax); }
+// The global generator of mathUT.cpp: seedp (rng_seed) and randp(A, B).
+void aeref_seedp(int s) { seedp(s); }
+double aeref_randp(double A, double B) { return randp(A, B); }
 int aeref_nearest_integer(double x) { return nearest_integer(x); }
 
 }

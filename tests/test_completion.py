@@ -112,11 +112,152 @@ def test_oracle_fixed_runs_keep_their_length(oracle_be):
     assert oracle.last_steps() == 300
 
 
-def test_oracle_refuses_fourier_completion(oracle_be):
+def test_oracle_refuses_unknown_criteria(oracle_be):
     spec, sens = custom_spec(Nt=20)
-    spec.time_type = 2
-    with pytest.raises(capi.FDTDError):
-        capi.run_custom(oracle_be, spec, sens)
+    for bad in (dict(time_type=3), dict(time_type=2, cc_quant=0), dict(time_type=2, cc_lmin=0.0),
+                dict(time_type=1, cc_step=0)):
+        with pytest.raises(capi.FDTDError):
+            capi.run_custom(oracle_be, capi.ModeSpec(**{**spec.__dict__, **bad}), sens)
+
+
+# ---------------------------------------------------------------- CPU: the spectral criterion
+
+def ft_spec(coeff=1e-2, Nt=6000, quant=60, seed=5, layout="nnb"):
+    spec = normal_spec()
+    spec.Nt = Nt
+    spec.time_type, spec.cc_step, spec.cc_coeff = 2, CC_STEP, coeff
+    spec.cc_quant, spec.cc_lmin, spec.cc_lmax, spec.cc_seed, spec.cc_layout = quant, 400e-9, 800e-9, seed, layout
+    return spec
+
+
+def ft_read():
+    import ctypes as C
+    lib = C.CDLL(oracle.ORACLE_SO)
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    lib.aeoracle_ft_info(C.byref(a), C.byref(b), C.byref(c))
+    navg, n, nch = a.value, b.value, c.value
+    pts = np.zeros(3 * n, dtype=np.int32)
+    lam = np.zeros(n)
+    ring = np.zeros(6 * n * navg)
+    checks = np.zeros(2 * nch, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    lib.aeoracle_ft_read(pts.ctypes.data_as(ip), lam.ctypes.data_as(dp), ring.ctypes.data_as(dp),
+                         checks.ctypes.data_as(ip))
+    return pts.reshape(n, 3), lam, ring.reshape(6, n, navg), checks.reshape(nch, 2)
+
+
+def settled(ring, coeff):
+    """The FT check restated from s_completion.cpp:43-120, averages carried over between cells."""
+    n = ring.shape[1]
+    avg = np.zeros(6)
+    nok = 0
+    for p in range(n):
+        for c in range(6):
+            acc = avg[c]
+            for v in ring[c, p]:
+                acc += v
+            avg[c] = abs(acc / ring.shape[2])
+        amax = avg.max()
+        tg = int(np.argmax(avg == amax))
+        row = ring[tg, p]
+        nok += abs(row.max() - row.min()) <= amax * coeff
+    return nok
+
+
+def test_oracle_ft_check_rule(oracle_be):
+    spec = ft_spec()
+    r = capi.run_normal_incidence(oracle_be, spec)
+    steps = oracle.last_steps()
+    pts, lam, ring, checks = ft_read()
+    nz = spec.matsgrid.shape[0] + 2 * 8 + 2 * max(5, spec.pad[4])
+    assert steps < spec.Nt and r["Nt_run"] == steps - 1 and (steps - 1) % CC_STEP == 0
+    # checks from the first step count >= 2 max(N), every cc_step; the last one completes
+    assert checks[0, 0] >= 2 * nz > checks[0, 0] - CC_STEP and checks[-1, 0] == steps
+    assert np.all(np.diff(checks[:, 0]) == CC_STEP)
+    assert checks[-1, 1] >= 0.9 * len(lam) > checks[-2, 1]
+    assert settled(ring, spec.cc_coeff) == checks[-1, 1]
+    # nnb: the cells sit on the structure's lowest and highest z planes
+    zs_s = 8 + max(5, spec.pad[4])
+    assert set(pts[:, 2]) == {zs_s, zs_s + spec.matsgrid.shape[0] - 1}
+    assert np.all((lam >= 400e-9) & (lam <= 800e-9))
+
+
+def test_oracle_ft_feed_is_a_running_dft(oracle_be):
+    """The ring holds Re/Im of sum_s local_E(s) exp(i w s Dt): checked against the
+    fields of fixed runs of every length on a small grid."""
+    spec = ft_spec(coeff=1e-12, Nt=25, quant=6, layout="v")
+    spec.matsgrid = spec.matsgrid[:, :2, :3].copy()
+    capi.run_normal_incidence(oracle_be, spec)
+    pts, lam, ring, _ = ft_read()
+    navg = ring.shape[2]
+    Dt = 5e-9 / (np.sqrt(3.0) * 299792458.0) * 0.99 * spec.time_mod
+    w = 2.0 * np.pi * 299792458.0 / lam
+    acc = np.zeros((6, len(lam)))
+    for s in range(spec.Nt):
+        f = capi.run_normal_incidence(oracle_be, capi.ModeSpec(**{**spec.__dict__, "time_type": 0, "Nt": s + 1}),
+                                      want_fields=True)["fields"]
+        nz, ny, nx = f.shape[1:]
+        for p, (i, j, k) in enumerate(pts):
+            ip, jp, kp = (i + 1) % nx, (j + 1) % ny, (k + 1) % nz
+            loc = [(f[0, k, j, i] + f[0, k, jp, i] + f[0, kp, j, i] + f[0, kp, jp, i]) / 4,
+                   (f[1, k, j, i] + f[1, k, j, ip] + f[1, kp, j, i] + f[1, kp, j, ip]) / 4,
+                   (f[2, k, j, i] + f[2, k, j, ip] + f[2, k, jp, i] + f[2, k, jp, ip]) / 4]
+            ph = np.exp(1j * w[p] * s * Dt)
+            for c in range(3):
+                acc[c, p] += (loc[c] * ph).real
+                acc[3 + c, p] += (loc[c] * ph).imag
+    last = (spec.Nt - 1) % navg
+    assert np.max(np.abs(ring[:, :, last] - acc)) <= 1e-12 * np.max(np.abs(acc))
+    assert np.max(np.abs(acc)) > 0
+
+
+@pytest.mark.skipif(oracle.load_ref() is None, reason="oracle/_ref not built")
+def test_oracle_ft_placement_follows_the_reference_generator(oracle_be):
+    """Cells and wavelengths against Aether's own compiled randp/seedp
+    (common/mathUT.cpp) driven through CompletionSensor::initialize's order."""
+    import ctypes as C
+    ref = C.CDLL(oracle.REF_SO)
+    ref.aeref_randp.argtypes = [C.c_double, C.c_double]
+    ref.aeref_randp.restype = C.c_double
+    ref.aeref_seedp.argtypes = [C.c_int]
+    for layout, seed in (("nnb", 5), ("bdn", 11), ("v", 3)):
+        spec = ft_spec(Nt=2, quant=37, seed=seed, layout=layout)
+        capi.run_normal_incidence(oracle_be, spec)
+        pts, lam, _, _ = ft_read()
+        nz, ny, nx = spec.matsgrid.shape
+        zpad = 8 + max(5, spec.pad[4])
+        Nx, Ny, Nz = nx, ny, nz + 2 * zpad
+        xs, ys, zs = (0, nx), (0, ny), (zpad, zpad + nz)
+        ref.aeref_seedp(seed)
+        rp = lambda a, b: int(ref.aeref_randp(a, b) + 0.5)
+        want = []
+        if layout == "v":
+            for _ in range(37):
+                want.append((rp(xs[0], xs[1] - 1), rp(ys[0], ys[1] - 1), rp(zs[0], zs[1] - 1)))
+        else:
+            area = {0: Ny * Nz, 1: Nx * Nz, 2: Nx * Ny}
+            n = []
+            for a in range(3):
+                n += [area[a] if layout[a] in "db" else 0, area[a] if layout[a] in "ub" else 0]
+            tot = float(sum(n))
+            left = 37
+            for f in range(6):
+                k = min(int(37 * n[f] / tot + 0.5), left)
+                left -= k
+                n[f] = k
+            lo, hi = (xs[0], ys[0], zs[0]), (xs[1] - 1, ys[1] - 1, zs[1] - 1)
+            for f in range(6):
+                a = f // 2
+                for _ in range(n[f]):
+                    c = []
+                    for b in range(3):
+                        c.append((hi[b] if f % 2 else lo[b]) if b == a else rp(lo[b], hi[b]))
+                    want.append(tuple(c))
+            want += [(0, 0, 0)] * (37 - len(want))
+        lam_ref = [ref.aeref_randp(400e-9, 800e-9) for _ in range(37)]
+        assert [tuple(p) for p in pts] == want, layout
+        assert list(lam) == lam_ref, layout
 
 
 # ---------------------------------------------------------------- GPU: the engine
@@ -197,6 +338,35 @@ def test_engine_fp32_stop_step(oracle_be):
     ref = oracle.last_steps()
     print("fp32 stop", steps, "oracle", ref)
     assert abs(steps - ref) <= CC_STEP
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["normal", "single", "custom"])
+def test_engine_ft_criterion_stops_with_oracle_fp64(oracle_be, kind):
+    """The spectral criterion in the three drivers, FP64: the same stop step as
+    the oracle and results at the FP64 bars."""
+    print(f"spectral completion: {kind} FP64", flush=True)
+    ft = dict(time_type=2, cc_step=CC_STEP, cc_quant=60, cc_lmin=400e-9, cc_lmax=800e-9, cc_seed=5)
+    if kind == "normal":
+        spec = ft_spec()
+        steps, g = run_engine(capi.KIND_NORMAL, spec)
+        r = capi.run_normal_incidence(oracle_be, spec)
+        assert steps == oracle.last_steps() < spec.Nt and g["Nt_run"] == r["Nt_run"]
+        for k in ("R", "T"):
+            assert np.max(np.abs(g[k] - r[k])) <= 1e-9, k
+        return
+    if kind == "single":
+        spec, sens = single_spec()
+        spec = capi.ModeSpec(**{**spec.__dict__, **ft, "cc_coeff": 1e-2})
+        steps, g = run_engine(capi.KIND_SINGLE, spec, sens)
+        r = capi.run_single_particle(oracle_be, spec, sens)
+    else:
+        spec, sens = custom_spec()
+        spec = capi.ModeSpec(**{**spec.__dict__, **ft, "cc_coeff": 1e-2, "cc_layout": "bbb"})
+        steps, g = run_engine(capi.KIND_CUSTOM, spec, sens)
+        r = capi.run_custom(oracle_be, spec, sens)
+    assert steps == oracle.last_steps() < spec.Nt
+    assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
 
 
 @pytest.mark.gpu

@@ -214,6 +214,18 @@ def test_script_normal_incidence_fp64(tmp_path):
 
 
 @pytest.mark.gpu
+def test_script_spectral_auto_tsteps_fp64(tmp_path):
+    print("script front: slab_auto_ft.lua", flush=True)
+    out, pairs = _runs("slab_auto_ft.lua", tmp_path)
+    for (m, g), (_, (spec, sens)) in pairs:
+        assert (spec.time_type, spec.cc_quant, spec.cc_layout, spec.cc_seed) == (2, 60, "nnb", 5)
+        r = capi.run_normal_incidence(oracle.load(), spec, sens)
+        assert g["steps"] == oracle.last_steps() < spec.Nt and g["Nt_run"] == r["Nt_run"]
+        for k in ("R", "T"):
+            assert np.max(np.abs(g[k] - r[k])) <= 1e-9, k
+
+
+@pytest.mark.gpu
 def test_script_auto_tsteps_fp64(tmp_path):
     print("script front: slab_auto.lua", flush=True)
     out, pairs = _runs("slab_auto.lua", tmp_path)
