@@ -370,6 +370,27 @@ def test_engine_ft_criterion_stops_with_oracle_fp64(oracle_be, kind):
 
 
 @pytest.mark.gpu
+def test_engine_ft_criterion_fp32_and_served(oracle_be):
+    """FP32 fields: the spectral check lands on the oracle's check or within a
+    few; the server steps such a run on its own and returns the direct run's
+    result."""
+    print("spectral completion: normal FP32, served", flush=True)
+    spec = ft_spec()
+    capi.run_normal_incidence(oracle_be, spec)
+    ref = oracle.last_steps()
+    s32 = capi.ModeSpec(**{**spec.__dict__, "precision": capi.FP32})
+    steps, direct = run_engine(capi.KIND_NORMAL, s32)
+    print("fp32 spectral stop", steps, "oracle", ref)
+    assert abs(steps - ref) <= 5 * CC_STEP and steps < spec.Nt
+    srv = capi.Server(capi.engine(), 64 << 30)
+    rid = srv.submit(capi.KIND_NORMAL, s32, [])
+    assert srv.wait(rid, 120.0) == capi.REQ_DONE, srv.error(rid)
+    out = srv.result(rid)[0]
+    srv.close()
+    assert np.array_equal(out[-2 * spec.Nl:], np.concatenate([direct["R"], direct["T"]]))
+
+
+@pytest.mark.gpu
 def test_batch_and_server_with_completion():
     """Batches step fixed-length runs only; the server runs a completion
     request on its own and gives the direct run's result."""
