@@ -83,6 +83,25 @@ def test_engine_refuses_without_gpu_or_bad_args():
     assert b"dimension" in lib.fn("last_error")(None)
 
 
+def test_discretize_and_run_checks_come_before_the_device():
+    """Malformed structures and completion parameters are refused with EINVAL
+    and a message before any device call (so also on a host without a GPU)."""
+    be = capi.engine()
+    for st, n, D in ((capi.Structure().add_cylinder(0, 0, 0, 0, 0, 0, 1e-9, 1), (4, 4, 4), (1e-9,) * 3),
+                     (capi.Structure(0, 1, 1).loop(1, 0, 0), (4, 4, 4), (1e-9,) * 3),
+                     (capi.Structure().add_sphere(0, 0, 0, -1, 1), (4, 4, 4), (1e-9,) * 3),
+                     (capi.Structure(), (0, 4, 4), (1e-9,) * 3),
+                     (capi.Structure(), (4, 4, 4), (0.0, 1e-9, 1e-9))):
+        with pytest.raises(capi.FDTDError) as e:
+            capi.discretize(be, st, n, D)
+        assert e.value.code == capi.EINVAL and "discretize" in str(e.value)
+    spec, sens = _tiny_custom()
+    for bad in (dict(time_type=3), dict(time_type=2, cc_quant=0), dict(time_type=1, cc_step=0)):
+        with pytest.raises(capi.FDTDError) as e:
+            capi.Run(be, capi.KIND_CUSTOM, capi.ModeSpec(**{**spec.__dict__, **bad}), sens)
+        assert e.value.code == capi.EINVAL
+
+
 def _finalized(be, spec, kind, sensors):
     return capi.mode_finalize(be, spec, kind, sensors)
 
