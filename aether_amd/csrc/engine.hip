@@ -281,6 +281,7 @@ int upload_materials_T(aefdtd_grid *g) {
 // neighbouring lanes touch neighbouring slots.
 int upload_grid(aefdtd_grid *g) {
     if (g->Nmat > 256) return fail(g, AEFDTD_EINVAL, "more than 256 material indices");
+    g->rc_ahead = false;                // the pole states are rebuilt below
     const size_t NA = g->nalloc(), rows = (size_t)g->Ny * g->Nz;
     // one pass over the cells: uint8 ids in the pitched layout, and the device
     // indices of the dispersive cells in x-fastest order
@@ -634,8 +635,7 @@ void dispatch_H(aefdtd_grid *g, HArgs<S, T> &a, int cfg, const HArgs<S, T> *aux 
 }
 
 template <typename S, typename T>
-int launch_rc(aefdtd_grid *g) {
-    if (g->n_disp == 0) return AEFDTD_OK;
+RcArgs<S, T> rc_args(aefdtd_grid *g) {
     RcArgs<S, T> r;
     r.ndisp = (int)g->n_disp;
     r.cells = g->d_cells;
@@ -645,6 +645,13 @@ int launch_rc(aefdtd_grid *g) {
     r.psi = g->d_psi;
     r.Ex = fld<S, T>(g, 0); r.Ey = fld<S, T>(g, 1); r.Ez = fld<S, T>(g, 2);
     r.corr = (T *)g->d_corr;
+    return r;
+}
+
+template <typename S, typename T>
+int launch_rc(aefdtd_grid *g) {
+    if (g->n_disp == 0) return AEFDTD_OK;
+    const RcArgs<S, T> r = rc_args<S, T>(g);
     hipEvent_t ev;
     prof_begin(g, 4, &ev);
     hipLaunchKernelGGL((k_rc_ante<S, T>), dim3((unsigned)((g->n_disp + 255) / 256), 3), dim3(256), 0, g->stream, r);
@@ -660,8 +667,8 @@ int launch_E(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
     if (rc) return rc;
     if (aux && (rc = ensure_osc_tables(aux, aux->tstep + 1))) return rc;
     if (aux && (rc = launch_rc<S, T>(aux))) return rc;
-    rc = launch_rc<S, T>(g);
-    if (rc) return rc;
+    if (!g->rc_ahead && (rc = launch_rc<S, T>(g))) return rc;      // else the sensor feed ran it
+    g->rc_ahead = false;
     g->strip_valid = false;             // E changes: the strip is stale until the next H pass
     EArgs<S, T> a = e_args<S, T>(g), x{};
     if (aux) x = e_args<S, T>(aux);
@@ -872,13 +879,25 @@ int feed_T(aefdtd_grid *g) {
     if (!g->sensors.empty()) {
         int rc;
         if ((rc = prepare_stage<T>(g))) return rc;
-        dim3 grid((unsigned)((g->stage_maxP + 255) / 256), (unsigned)g->sensors.size(), 1);
+        const int nsx = (int)((g->stage_maxP + 255) / 256);
         hipEvent_t ev;
         prof_begin(g, 2, &ev);
-        hipLaunchKernelGGL((k_sample<S, T>), grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc,
-                           g->stage_fill, g->stage_P, fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2),
-                           fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, g->px,
-                           (T *)g->d_stage, g->strip_valid ? (const T *)g->d_strip : nullptr);
+        if (g->n_disp > 0 && !g->rc_ahead) {
+            // the next E pass's RC pre-pass rides in this launch (fields are those of the H pass)
+            const int nrx = (int)((g->n_disp + 255) / 256), ns = nsx * (int)g->sensors.size();
+            hipLaunchKernelGGL((k_sample_rc<S, T>), dim3((unsigned)(ns + 3 * nrx)), dim3(256), 0, g->stream,
+                               (const SensDesc *)g->d_sdesc, g->stage_fill, g->stage_P, fld<S, T>(g, 0),
+                               fld<S, T>(g, 1), fld<S, T>(g, 2), fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5),
+                               g->Nx, g->Ny, g->Nz, g->px, (T *)g->d_stage,
+                               g->strip_valid ? (const T *)g->d_strip : nullptr, nsx, ns, rc_args<S, T>(g), nrx);
+            g->rc_ahead = true;
+        } else {
+            dim3 grid((unsigned)nsx, (unsigned)g->sensors.size(), 1);
+            hipLaunchKernelGGL((k_sample<S, T>), grid, dim3(256), 0, g->stream, (const SensDesc *)g->d_sdesc,
+                               g->stage_fill, g->stage_P, fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2),
+                               fld<S, T>(g, 3), fld<S, T>(g, 4), fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, g->px,
+                               (T *)g->d_stage, g->strip_valid ? (const T *)g->d_strip : nullptr);
+        }
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
         g->stage_fill++;
@@ -1754,6 +1773,7 @@ extern "C" {
 int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;
     g->strip_valid = false;
+    g->rc_ahead = false;
     size_t N = g->nalloc();
     for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream));
     for (int s = 0; s < 12; s++)
@@ -1786,6 +1806,7 @@ int aefdtd_read_field(aefdtd_grid *g, int f, double *out) {
 int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
     if (!g || !in) return AEFDTD_EINVAL;
     g->strip_valid = false;
+    g->rc_ahead = false;
     if (f < 0 || f > 5) return fail(g, AEFDTD_EINVAL, "bad field id");
     HIPOK(g, hipStreamSynchronize(g->stream));
     if (g->precision == AEFDTD_FP16) {

@@ -190,6 +190,7 @@ struct aefdtd_grid {
     int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
     // bookkeeping
     bool ready = false;               // device copies up to date
+    bool rc_ahead = false;            // the last sensor feed also ran the next E pass's RC pre-pass
     bool pml_dirty = true;
     size_t dev_bytes = 0;
     std::string err;

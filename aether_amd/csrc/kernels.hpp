@@ -474,10 +474,7 @@ struct RcArgs {
 constexpr int RC_PRELOAD = 8;
 
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<S, T> a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (s >= a.ndisp) return;
+__device__ __forceinline__ void rc_lane(const RcArgs<S, T> &a, int s, int c) {
     const int id = a.cells[s];
     const MatDev<T> &G = a.mt[a.mat[id]];
     const Fld<S, T> &E = c == 0 ? a.Ex : (c == 1 ? a.Ey : a.Ez);
@@ -504,6 +501,12 @@ __global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<S, T> a) {
         sum += r.x;
     }
     a.corr[3 * s + c] = (T)(G.C3 * sum);
+}
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_rc_ante(const RcArgs<S, T> a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < a.ndisp) rc_lane(a, s, blockIdx.y);
 }
 
 template <typename S, typename T>
@@ -1183,6 +1186,28 @@ __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, lo
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.P) return;
     sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage, strip);
+}
+
+// The sensor samples of step n and the RC pre-pass of step n+1 in one launch:
+// both only read the fields the H pass of step n left (the RC pass writes its
+// pole states and increments, which nothing else touches), so they share the
+// launch instead of two: workgroups [0, ns) sample (sensor blockIdx / nsx),
+// the rest run the RC pre-pass (component (blockIdx - ns) / nrx).
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_sample_rc(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
+                                                   Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx,
+                                                   int Ny, int Nz, int P, T *stage, const T *strip, int nsx, int ns,
+                                                   const RcArgs<S, T> r, int nrx) {
+    const int b = blockIdx.x;
+    if (b < ns) {
+        const SensDesc d = sd[b / nsx];
+        const long p = (long)(b % nsx) * 256 + threadIdx.x;
+        if (p < d.P) sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage, strip);
+        return;
+    }
+    const int q = b - ns;
+    const int s = (q % nrx) * 256 + threadIdx.x;
+    if (s < r.ndisp) rc_lane(r, s, q / nrx);
 }
 
 // One sensor of one grid of a batch: the grid's fields, its stage pool and

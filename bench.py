@@ -541,7 +541,7 @@ def main():
                      "kernel": "k_update_E", "bytes_per_launch": e_bytes, "avg_launch_ms": round(e_avg * 1e3, 5)},
         "kernels_ms_avg": {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5),
                            "sensor": round(s_ms / max(1, s_n), 5), "dft": round(d_ms / max(1, d_n), 5),
-                           "rc": round(r_ms / max(1, r_n), 5)},
+                           "rc": round(r_ms / max(1, r_n), 5) if r_n else "in the sensor launch"},
         "step_hbm_fraction": round(step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
         "stream_copy_GBps": {"plain": round(float(bw[0]), 1), "nontemporal": round(float(bw[1]), 1),
