@@ -906,14 +906,24 @@ int feed_T(aefdtd_grid *g) {
             if ((rc = flush_dft<T>(g))) return rc;
         }
     }
-    for (PlaneSum *s : g->sums) {
-        int t = g->tstep - 1;
-        if (t < 0 || t >= g->Nt) continue;
-        hipLaunchKernelGGL((k_plane_sum<S, T>), dim3(PS_BLOCKS), dim3(256), 0, g->stream, fld<S, T>(g, 0), fld<S, T>(g, 1),
-                           fld<S, T>(g, 2), g->Nx, g->Ny, g->px, s->k, s->inc, (int)s->has_inc, (int)s->subtract, g->Dx,
-                           g->Dy, g->Dz, s->zpos, (const double *)s->d_tap, t, g->Dt, s->d_out);
-        HIPOK(g, hipGetLastError());
-        s->n = std::max(s->n, t + 1);
+    const int t = g->tstep - 1;
+    if (t >= 0 && t < g->Nt) {
+        // two sums per launch (blockIdx.y)
+        for (size_t q = 0; q < g->sums.size(); q += 2) {
+            PlaneSumArgs a[2];
+            const int n = std::min<int>(2, (int)(g->sums.size() - q));
+            for (int u = 0; u < n; u++) {
+                const PlaneSum *s = g->sums[q + u];
+                a[u] = PlaneSumArgs{s->k, s->inc, (int)s->has_inc, (int)s->subtract, s->zpos,
+                                    (const double *)s->d_tap, s->d_out};
+            }
+            if (n == 1) a[1] = a[0];
+            hipLaunchKernelGGL((k_plane_sum<S, T>), dim3(PS_BLOCKS, n), dim3(256), 0, g->stream, fld<S, T>(g, 0),
+                               fld<S, T>(g, 1), fld<S, T>(g, 2), g->Nx, g->Ny, g->px, a[0], a[1], g->Dx, g->Dy, g->Dz,
+                               t, g->Dt);
+            HIPOK(g, hipGetLastError());
+            for (int u = 0; u < n; u++) g->sums[q + u]->n = std::max(g->sums[q + u]->n, t + 1);
+        }
     }
     return AEFDTD_OK;
 }

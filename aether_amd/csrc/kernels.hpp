@@ -1365,31 +1365,44 @@ __device__ inline double d_pulse(const aefdtd_pulse &p, double amp, double x, do
 // when they are read back, so the result does not depend on scheduling.
 constexpr int PS_BLOCKS = 64;
 
+// One plane sum of a launch: plane kpl, incident pulse (if has_inc), whether
+// the incident field is subtracted, its z, the tapering table and the output.
+struct PlaneSumArgs {
+    int kpl;
+    aefdtd_pulse inc;
+    int has_inc, subtract;
+    double zpos;
+    const double *tap;
+    double *out;
+};
+
+// The plane sums of a grid for step t, blockIdx.y selecting the sum (the R
+// and T sums of a normal-incidence run share one launch).
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int Ny, int P, int kpl,
-                                                   aefdtd_pulse inc, int has_inc, int subtract, double Dx, double Dy,
-                                                   double Dz, double zpos, const double *tap, int t, double Dt,
-                                                   double *out) {
+__global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez, int Nx, int Ny, int P,
+                                                   PlaneSumArgs s0, PlaneSumArgs s1, double Dx, double Dy, double Dz,
+                                                   int t, double Dt) {
+    const PlaneSumArgs &ps = blockIdx.y == 0 ? s0 : s1;
     __shared__ double sh[6][256];
     double acc[6] = {0, 0, 0, 0, 0, 0};
     const double tb = (t + 0.5) * Dt;
-    const double tp = tap[t];
+    const double tp = ps.tap[t];
     const long NxNy = (long)Nx * Ny;
     const long stride = (long)blockDim.x * gridDim.x;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < NxNy; q += stride) {
         const int i = (int)(q % Nx), j = (int)(q / Nx);
         const double x = i * Dx, y = j * Dy;
         double in[3] = {0, 0, 0};
-        if (has_inc) {
-            in[0] = d_pulse(inc, inc.Ex0, x + Dx / 2.0, y, zpos, tb);
-            in[1] = d_pulse(inc, inc.Ey0, x, y + Dy / 2.0, zpos, tb);
-            in[2] = d_pulse(inc, inc.Ez0, x, y, zpos + Dz / 2.0, tb);
+        if (ps.has_inc) {
+            in[0] = d_pulse(ps.inc, ps.inc.Ex0, x + Dx / 2.0, y, ps.zpos, tb);
+            in[1] = d_pulse(ps.inc, ps.inc.Ey0, x, y + Dy / 2.0, ps.zpos, tb);
+            in[2] = d_pulse(ps.inc, ps.inc.Ez0, x, y, ps.zpos + Dz / 2.0, tb);
         }
-        const int id = i + P * (j + Ny * kpl);
+        const int id = i + P * (j + Ny * ps.kpl);
         const double f[3] = {(double)Ex[id], (double)Ey[id], (double)Ez[id]};
         for (int c = 0; c < 3; c++) {
             acc[3 + c] += in[c] * tp;
-            acc[c] += (subtract ? (f[c] - in[c]) : f[c]) * tp;
+            acc[c] += (ps.subtract ? (f[c] - in[c]) : f[c]) * tp;
         }
     }
     for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = acc[c];
@@ -1399,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_plane_sum(Fld<S, T> Ex, Fld<S, T> Ey, F
             for (int c = 0; c < 6; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 6) out[((size_t)t * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 6) ps.out[((size_t)t * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // ================================================================ 2D time skewing
