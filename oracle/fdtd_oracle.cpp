@@ -1521,7 +1521,7 @@ std::vector<double> g_ft_lambda, g_ft_ring;
 std::vector<int> g_ft_checks;         // per check: sensor step count, settled cells
 int g_ft_navg = 0;
 
-struct EnergyCheck {
+struct Completion {
     const aefdtd_mode *m;
     const aeoracle_grid *g;
     double last = 0, max = 0;
@@ -1533,7 +1533,7 @@ struct EnergyCheck {
     std::vector<double> ring;                // [6][np][navg]
     double Dt = 0;
 
-    EnergyCheck(const aefdtd_mode *m_, const aeoracle_grid *g_, double Dt_) : m(m_), g(g_), Dt(Dt_) {
+    Completion(const aefdtd_mode *m_, const aeoracle_grid *g_, double Dt_) : m(m_), g(g_), Dt(Dt_) {
         g_energy_trace.clear();
         if (m->time_type == 2) place();
     }
@@ -1754,7 +1754,7 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
     int idR, idT;
     aeoracle_add_plane_sum(g, zs_e - 2, &chp, (zs_e - 2) * Dz, 1, tap.data(), &idR);
     aeoracle_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT);
-    EnergyCheck E(m, g, Dt);
+    Completion E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         g->update_E();
@@ -1901,7 +1901,7 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
         if ((rc = aeoracle_add_oscillator(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, q.lambda_max, &id)))
             return rc;
     }
-    EnergyCheck E(m, g, Dt);
+    Completion E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         g->update_E();       // + sources' inject_E
@@ -1974,7 +1974,7 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefd
     if ((rc = aeoracle_set_tfsf(g, a, g->xs_s, g->xs_e, g->ys_s, g->ys_e, g->zs_s, g->zs_e, periodic_x, periodic_y))) return rc;
     SensorSet S;
     if ((rc = add_requests(g, n_sensors, sensors, S))) return rc;
-    EnergyCheck E(m, g, Dt);
+    Completion E(m, g, Dt);
     int t;
     for (t = 0; t < Nt; t++) {
         a->update_E();
