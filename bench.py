@@ -349,6 +349,57 @@ def slab_form(args):
     print(json.dumps(rec))
 
 
+SCRIPT_STRUCTURE = """lx = 256e-9
+ly = 256e-9
+lz = 256e-9
+default_material(0)
+for i = 1, 3 do
+    add_sphere(64e-9 * i, 128e-9, 100e-9 + 20e-9 * i, 30e-9, 1)
+end
+"""
+SCRIPT_MODE = """st = Structure("structure.lua")
+gold = Material()
+gold:epsilon_infty(1.1)
+gold:add_drude(1.37e16, 1.07e14)
+box = create_sensor("box_spectral_poynting")
+box:location(0, 256e-9, 0, 256e-9, 0, 256e-9)
+box:spectrum(400e-9, 800e-9, 64)
+box:name("box")
+f = MODE("fdtd_single_particle")
+f:structure(st)
+f:Dxyz(2e-9)
+f:N_tsteps(200)
+f:polarization("TM")
+for _, face in ipairs({"xm", "xp", "ym", "yp", "zm", "zp"}) do f["pml_" .. face](f, 12, 25, 1, 0.2) end
+f:material(0, Material(1.33))
+f:material(1, gold)
+f:register_sensor(box)
+f:compute()
+"""
+
+
+def script_front_cost(be):
+    """A single-particle script (128^3 cells of 2 nm, three Drude spheres, box
+    sensor, 200 steps, FP32) through the script front end: the whole run, and
+    its parts before the time loop (interpretation, device discretisation,
+    sizing) and the run itself."""
+    import tempfile
+    from aether_amd import capi
+    from aether_amd.script import Front
+    with tempfile.TemporaryDirectory() as d:
+        for name, text in (("structure.lua", SCRIPT_STRUCTURE), ("mode.lua", SCRIPT_MODE)):
+            with open(os.path.join(d, name), "w") as fh:
+                fh.write(text)
+        Front(be, out_dir=d, precision=capi.FP32).run_file(os.path.join(d, "mode.lua"))      # warm
+        t0 = time.perf_counter()
+        dry = Front(be, out_dir=d, precision=capi.FP32, dry=True)
+        dry.run_file(os.path.join(d, "mode.lua"))
+        t1 = time.perf_counter()
+        Front(be, out_dir=d, precision=capi.FP32).run_file(os.path.join(d, "mode.lua"))
+        t2 = time.perf_counter()
+    return {"inputs_ms": round(1e3 * (t1 - t0), 1), "whole_script_ms": round(1e3 * (t2 - t1), 1)}
+
+
 def setup_form(args):
     """The setup form: what a run costs before and around its time loop.
     (1) Structure::discretize of the headline structure (256^3 cells, 40
@@ -408,7 +459,9 @@ def setup_form(args):
         r.synchronize()
         per[tt] = (time.perf_counter() - t0) / 200 * 1e3
         r.close()
-    rec = {"form": "setup", "discretize_256_40_spheres": disc, "run_create_ms": create,
+    script = script_front_cost(be)
+    print(f"setup form: script front end {script}", file=sys.stderr, flush=True)
+    rec = {"form": "setup", "discretize_256_40_spheres": disc, "run_create_ms": create, "script_front_end": script,
            "energy_check": {"ms_per_step_fixed": round(per[0], 4), "ms_per_step_energy": round(per[1], 4),
                             "overhead": round(per[1] / per[0] - 1.0, 4)}}
     print(json.dumps(rec))
