@@ -1,7 +1,7 @@
 """A small interpreter for the subset of Lua 5.4 that Aether's scripts use.
 
 Aether runs its simulation scripts, structure scripts and material library
-files (``mat_lib/*.lua``) through liblua 5.4 (SURVEY: Lua 5.4.4, not vendored
+files (``mat_lib/*.lua``) through liblua 5.4 (Lua 5.4.4 in its build; not vendored
 in the reference and not installed here).  This module restates the language
 from the Lua 5.4 reference manual for what those scripts need: literals
 (decimal and hexadecimal numbers with the integer/float split, short and long
