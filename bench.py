@@ -459,9 +459,25 @@ def setup_form(args):
         r.synchronize()
         per[tt] = (time.perf_counter() - t0) / 200 * 1e3
         r.close()
+    # result files (64 wavelengths, box sensor) and the sizing of FDTD_Mode::finalize
+    import tempfile
+    spec_w, sens_w, _ = workloads.spheres_single_particle(n=64, count=4, Nt=20, precision=capi.FP32)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        capi.mode_finalize(be, spec_w, capi.KIND_SINGLE, sens_w)
+    finalize_us = (time.perf_counter() - t0) / 100 * 1e6
+    r = capi.Run(be, capi.KIND_SINGLE, spec_w, sens_w)
+    r.step(spec_w.Nt)
+    r.finish()
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        r.write(d, "", ["box"])
+        write_ms = (time.perf_counter() - t0) * 1e3
+    r.close()
     script = script_front_cost(be)
     print(f"setup form: script front end {script}", file=sys.stderr, flush=True)
     rec = {"form": "setup", "discretize_256_40_spheres": disc, "run_create_ms": create, "script_front_end": script,
+           "mode_finalize_us": round(finalize_us, 1), "write_box_64_lambda_ms": round(write_ms, 2),
            "energy_check": {"ms_per_step_fixed": round(per[0], 4), "ms_per_step_energy": round(per[1], 4),
                             "overhead": round(per[1] / per[0] - 1.0, 4)}}
     print(json.dumps(rec))
