@@ -308,6 +308,12 @@ int setup_custom(aefdtd_run *r, const aefdtd_mode *m) {
     TRY(aefdtd_create(&r->g, m->nx, m->ny, m->nz, m->Nt, Dx, Dy, Dz, Dt, AEFDTD_MODE_CUSTOM, m->pml, m->pad,
                       m->precision));
     aefdtd_grid *g = r->g;
+    if (m->precision == AEFDTD_FP16 && m->n_sources > 0) {
+        // oscillators inject amplitude 50 (sc_oscillator.cpp:37-38) and the fields near them
+        // grow past the unit-wave scale: store E unscaled and H at 2^8 (about 377 x E)
+        const int ex[6] = {0, 0, 0, 8, 8, 8};
+        TRY(aefdtd_set_storage_scale(g, ex));
+    }
     for (int f = 0; f < 6; f++) TRY(aefdtd_set_pml(g, f, m->kappa[f], m->sigma[f], m->alpha[f]));
     TRY(aefdtd_set_tapering(g, m->tapering));
     TRY(aefdtd_set_matsgrid(g, m->matsgrid, m->nx, m->ny, m->nz));

@@ -493,7 +493,9 @@ int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
 /* Power-of-two storage scales of the six field components (stored value =
  * field * 2^exps[c]); FP16 storage only (defaults E: 2^8, H: 2^16, so that
- * both sit near the middle of the FP16 range). Before the first update. */
+ * unit-amplitude plane waves sit near the middle of the FP16 range; the custom
+ * driver uses E: 2^0, H: 2^8 when it has oscillator sources, whose amplitude
+ * is 50). Before the first update. */
 int aefdtd_set_storage_scale(aefdtd_grid *g, const int exps[6]);
 /* Reference point for the roofline, no counterpart in the reference: the
  * best bandwidth of a streaming device copy of `bytes` (read + write counted)

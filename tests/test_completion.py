@@ -328,6 +328,34 @@ def test_engine_2d_custom_stops_with_oracle_fp64(oracle_be):
 
 
 @pytest.mark.gpu
+def test_engine_fp16_storage_energy_stop(oracle_be):
+    """FP16 field storage takes the separate energy pass (the fused sum needs
+    the stored values in the arithmetic type): the stop lands within a few
+    checks of the FP64 oracle's."""
+    print("energy completion: custom FP16 storage", flush=True)
+    spec, sens = custom_spec(precision=capi.FP16)
+    steps, _ = run_engine(capi.KIND_CUSTOM, spec, sens)
+    capi.run_custom(oracle_be, fixed(spec, precision=capi.FP64, time_type=1), sens)
+    ref = oracle.last_steps()
+    print("fp16 stop", steps, "oracle", ref)
+    assert abs(steps - ref) <= 5 * CC_STEP and steps < spec.Nt
+
+
+@pytest.mark.gpu
+def test_engine_2d_ft_criterion_fp64(oracle_be):
+    print("spectral completion: 2D custom FP64", flush=True)
+    rng = workloads.MT19937_64(7)
+    spec, sens, _ = workloads.sweep_run(rng, 7, side=60, steps=6000, Nl=8, precision=capi.FP64)
+    spec = capi.mode_finalize(oracle_be, spec, capi.KIND_CUSTOM, sens)
+    spec = capi.ModeSpec(**{**spec.__dict__, "time_type": 2, "cc_step": CC_STEP, "cc_coeff": 1e-1, "cc_quant": 40,
+                            "cc_lmin": 500e-9, "cc_lmax": 1000e-9, "cc_seed": 9, "cc_layout": "bbn"})
+    steps, g = run_engine(capi.KIND_CUSTOM, spec, sens)
+    r = capi.run_custom(oracle_be, spec, sens)
+    assert steps == oracle.last_steps() < spec.Nt
+    assert np.max(np.abs(g["sensors"] - r["sensors"])) <= 1e-9 * np.max(np.abs(r["sensors"]))
+
+
+@pytest.mark.gpu
 def test_engine_fp32_stop_step(oracle_be):
     """FP32 fields change the energy sums in their last bits: the stop lands on
     the oracle's check or the next one."""
