@@ -114,6 +114,26 @@ def test_custom_3d_run_fp64(engine_be, oracle_be):
     assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
 
 
+def test_custom_3d_fp16_storage(engine_be, oracle_be):
+    """FP16 field storage with an oscillator source (amplitude 50): finite fields
+    and the planar spectrum within FP16's precision of the FP64 oracle."""
+    print("custom 3D FP16 storage", flush=True)
+    n = (20, 18, 16)
+    mg = np.zeros(n[::-1], dtype=np.uint32)
+    mg[6:10, 5:12, 7:14] = 1
+    src = capi.SourceReq(orientation=capi.NORMAL_X, x1=0, x2=20, y1=0, y2=18, z1=2, z2=3,
+                         lambda_min=400e-9, lambda_max=800e-9)
+    sens = [capi.SensorReq(box=False, orientation=capi.NORMAL_Z, x1=0, x2=20, y1=0, y2=18, z1=15, z2=16, Nl=9)]
+    spec = capi.ModeSpec(matsgrid=mg, materials=[Material(1.0), Material(4.0)], Nt=1200, Dx=5e-9, Dy=5e-9,
+                         Dz=5e-9, pml=(6,) * 6, pad=(5,) * 6, tapering=40, sources=[src], precision=capi.FP16)
+    g = engine_alone(spec, sens)
+    r = capi.run_custom(oracle_be, capi.ModeSpec(**{**spec.__dict__, "precision": capi.FP64}), sens)
+    assert np.all(np.isfinite(g["fields"]))
+    err = spec_err(g["sensors"], r["sensors"])
+    print("fp16 custom spectral error", err)
+    assert err <= 1e-2
+
+
 def _batch_runs(precision, specs):
     be = capi.engine()
     b = capi.Batch(be, precision)
