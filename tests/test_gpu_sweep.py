@@ -194,6 +194,30 @@ def test_batch_fp32_alone_equals_mixed(engine_be, oracle_be):
     b.close()
 
 
+def test_batch_fp16_storage(engine_be, oracle_be):
+    """FP16 storage in a batch (oscillator sources, per-step passes): each run as
+    alone, finite, and within FP16's precision of the FP64 oracle."""
+    print("batch FP16 storage", flush=True)
+    cases = [small_run(31, 40, 300, capi.FP16, pol="TM"), small_run(32, 48, 200, capi.FP16, pol="TE")]
+    specs = [(finalized(s, q), q) for s, q, _ in cases]
+    b, runs = _batch_runs(capi.FP16, specs)
+    while b.running():
+        b.step(1)
+    b.synchronize()
+    for (spec, sens), run in zip(specs, runs):
+        b.remove(run)
+        mixed = run.finish(want_fields=True)
+        run.close()
+        assert np.all(np.isfinite(mixed["fields"]))
+        alone = engine_alone(spec, sens)
+        assert spec_err(mixed["sensors"], alone["sensors"]) <= 1e-6
+        r = capi.run_custom(oracle_be, capi.ModeSpec(**{**spec.__dict__, "precision": capi.FP64}), sens)
+        err = spec_err(mixed["sensors"], r["sensors"])
+        print("batched fp16 spectral error vs fp64 oracle", err)
+        assert err <= 1e-2
+    b.close()
+
+
 def test_batch_autotune_and_3d_member(engine_be, oracle_be):
     """The online tuner changes tile shapes between chunks without changing
     results; a 3D custom run can share the batch with 2D runs."""
