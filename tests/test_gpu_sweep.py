@@ -114,6 +114,24 @@ def test_custom_3d_run_fp64(engine_be, oracle_be):
     assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
 
 
+def test_custom_3d_dispersive_fp64(engine_be, oracle_be):
+    """3D custom mode with a Drude + critical-point block, an oscillator and a box
+    sensor: the RC pre-pass runs inside the sensor launch from the second step on."""
+    print("custom 3D dispersive FP64", flush=True)
+    n = (20, 18, 16)
+    mg = np.zeros(n[::-1], dtype=np.uint32)
+    mg[6:10, 5:12, 7:14] = 1
+    src = capi.SourceReq(orientation=capi.NORMAL_Y, x1=0, x2=20, y1=2, y2=3, z1=0, z2=16,
+                         lambda_min=400e-9, lambda_max=800e-9)
+    sens = [capi.SensorReq(box=True, x1=3, x2=17, y1=4, y2=15, z1=4, z2=15, Nl=9)]
+    spec = capi.ModeSpec(matsgrid=mg, materials=[Material(1.0), workloads.GOLD], Nt=260, Dx=5e-9, Dy=5e-9,
+                         Dz=5e-9, pml=(6,) * 6, pad=(5,) * 6, tapering=40, sources=[src], precision=capi.FP64)
+    g = engine_alone(spec, sens)
+    r = capi.run_custom(oracle_be, spec, sens, want_fields=True)
+    assert rel_l2(g["fields"], r["fields"]) <= FIELD_TOL
+    assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
+
+
 def test_custom_3d_fp16_storage(engine_be, oracle_be):
     """FP16 field storage with an oscillator source (amplitude 50): finite fields
     and the planar spectrum within FP16's precision of the FP64 oracle."""
