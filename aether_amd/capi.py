@@ -127,14 +127,15 @@ def _sig(lib, p):
         "pulse_init": (i, [C.POINTER(CPulse), d, d, d, d, d, d, d, d, d]),
         "pulse_eval": (d, [C.POINTER(CPulse), i, d, d, d, d]),
         "run_normal_incidence": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq),
-                                     C.POINTER(CNormalResult), _dp, _dp]),
-        "run_single_particle": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
-        "run_custom": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp]),
+                                     C.POINTER(CNormalResult), _dp, _dp, _ip]),
+        "run_single_particle": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp, _ip]),
+        "run_custom": (i, [C.POINTER(CMode), i, C.POINTER(CSensorReq), _dp, _dp, _ip]),
         "add_oscillator": (i, [vp, i, i, i, i, i, i, i, d, d, _ip]),
         "mode_finalize": (i, [C.POINTER(CMode), i, i, C.POINTER(CSensorReq)]),
         # engine only
         "run_create": (i, [C.POINTER(vp), i, C.POINTER(CMode), i, C.POINTER(CSensorReq)]),
         "run_step": (i, [vp, i]),
+        "run_cancel": (i, [vp]),
         "run_progress": (i, [vp, _ip, _ip]),
         "run_grid": (vp, [vp]),
         "run_error": (C.c_char_p, [vp]),
@@ -150,7 +151,6 @@ def _sig(lib, p):
         "autotune": (i, [vp, i, _ip]),
         "set_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
-        "cancel": (None, []),
         "batch_create": (i, [C.POINTER(vp), i]),
         "batch_destroy": (i, [vp]),
         "batch_error": (C.c_char_p, [vp]),
@@ -479,8 +479,14 @@ def mode_finalize(be: Backend, spec: ModeSpec, kind, sensors=()):
     return out
 
 
-def run_custom(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
-    """Runs the custom-mode driver (spec already finalized); sensor outputs and fields."""
+def _flag_ptr(end_flag):
+    """A ctypes c_int (the caller's end_computation flag) as an int pointer, or None."""
+    return None if end_flag is None else C.cast(C.byref(end_flag), _ip)
+
+
+def run_custom(be: Backend, spec: ModeSpec, sensors=(), want_fields=False, end_flag=None):
+    """Runs the custom-mode driver (spec already finalized); sensor outputs and fields.
+    end_flag: optional ctypes.c_int polled after every step (non-zero ends the run)."""
     keep = []
     cm = spec.to_c(keep)
     sreq = (CSensorReq * max(1, len(sensors)))(*[q.to_c() for q in sensors])
@@ -490,10 +496,10 @@ def run_custom(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
     if want_fields:
         fields = np.zeros((6,) + custom_dims(spec)[::-1])
     rc = be.fn("run_custom")(C.byref(cm), len(sensors), sreq, _dptr(sout),
-                             _dptr(fields) if fields is not None else None)
+                             _dptr(fields) if fields is not None else None, _flag_ptr(end_flag))
     if rc not in (OK, ECANCELLED):
         be.check(rc, None)
-    out = {"sensors": sout[:nsout]}
+    out = {"sensors": sout[:nsout], "cancelled": rc == ECANCELLED}
     if fields is not None:
         out["fields"] = fields
     return out
@@ -512,8 +518,9 @@ def _dptr(a):
     return a.ctypes.data_as(_dp)
 
 
-def run_normal_incidence(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
-    """Runs the normal-incidence driver; returns dict of spectra (and fields)."""
+def run_normal_incidence(be: Backend, spec: ModeSpec, sensors=(), want_fields=False, end_flag=None):
+    """Runs the normal-incidence driver; returns dict of spectra (and fields).
+    end_flag as run_custom."""
     keep = []
     cm = spec.to_c(keep)
     Nl = spec.Nl
@@ -530,13 +537,14 @@ def run_normal_incidence(be: Backend, spec: ModeSpec, sensors=(), want_fields=Fa
     if want_fields:
         fields = _normal_fields_buffer(spec)
     rc = be.fn("run_normal_incidence")(C.byref(cm), len(sensors), sreq, C.byref(res), _dptr(sout),
-                                        _dptr(fields) if fields is not None else None)
+                                        _dptr(fields) if fields is not None else None, _flag_ptr(end_flag))
     if rc not in (OK, ECANCELLED):
         be.check(rc, None)
     for k, v in cplx.items():
         out[k] = v[0::2] + 1j * v[1::2]
     out["index_sup"], out["index_sub"], out["Nt_run"] = res.index_sup, res.index_sub, res.Nt_run
     out["sensors"] = sout[:nsout]
+    out["cancelled"] = rc == ECANCELLED
     if fields is not None:
         out["fields"] = fields
     return out
@@ -567,7 +575,8 @@ def single_particle_dims(spec):
     return Nx, Ny, Nz
 
 
-def run_single_particle(be: Backend, spec: ModeSpec, sensors=(), want_fields=False):
+def run_single_particle(be: Backend, spec: ModeSpec, sensors=(), want_fields=False, end_flag=None):
+    """Runs the single-particle driver; end_flag as run_custom."""
     keep = []
     cm = spec.to_c(keep)
     sreq = (CSensorReq * max(1, len(sensors)))(*[s.to_c() for s in sensors])
@@ -578,10 +587,10 @@ def run_single_particle(be: Backend, spec: ModeSpec, sensors=(), want_fields=Fal
         Nx, Ny, Nz = single_particle_dims(spec)
         fields = np.zeros((6, Nz, Ny, Nx))
     rc = be.fn("run_single_particle")(C.byref(cm), len(sensors), sreq, _dptr(sout),
-                                       _dptr(fields) if fields is not None else None)
+                                       _dptr(fields) if fields is not None else None, _flag_ptr(end_flag))
     if rc not in (OK, ECANCELLED):
         be.check(rc, None)
-    out = {"sensors": sout[:nsout]}
+    out = {"sensors": sout[:nsout], "cancelled": rc == ECANCELLED}
     if fields is not None:
         out["fields"] = fields
     return out
@@ -601,9 +610,16 @@ class Run:
         self.h = h
 
     def step(self, n):
+        """Advances up to n steps; False once the run was cancelled."""
         rc = self.be.fn("run_step")(self.h, n)
+        if rc == ECANCELLED:
+            return False
         if rc != OK:
             raise FDTDError(rc, (self.be.fn("run_error")(self.h) or b"").decode())
+        return True
+
+    def cancel(self):
+        self.be.check(self.be.fn("run_cancel")(self.h), None)
 
     def progress(self):
         t, nt = C.c_int(), C.c_int()

@@ -1170,7 +1170,9 @@ int batch_build(Batch *b) {
         return AEFDTD_OK;
     }
     const bool tb_on = b->tb_code >= 0 && std::is_same_v<S, T>;
-    const int ly = tb_ly<T>(b->tb_code & 3), TS = kTS[(b->tb_code >> 2) & 3];
+    // tile height, depth and inner tile of the time-skewed passes (codes are
+    // validated by tb_code_valid; -1 = off, then none of these is used)
+    const int ly = tb_on ? tb_ly<T>(b->tb_code & 3) : 0, TS = tb_on ? kTS[(b->tb_code >> 2) & 3] : 0;
     const int IX = TBX - 2 * TS - 1, IY = ly - 2 * TS - 1;
     std::vector<EArgs<S, T>> ea;
     std::vector<HArgs<S, T>> ha;
@@ -1287,7 +1289,7 @@ int batch_build(Batch *b) {
     // allocation and copy calls on their own streams while this one captures
     BHIPOK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
     if constexpr (std::is_same_v<S, T>) {
-        for (int s0 = 0; s0 < KS; s0 += TS)
+        for (int s0 = 0; tb_on && s0 < KS; s0 += TS)
             for (int p = 0; p < 2; p++)
                 for (int c = 0; c < 2; c++)
                     if (b->nmem[p][c])

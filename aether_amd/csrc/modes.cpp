@@ -26,8 +26,6 @@ using namespace aefdtd;
 
 namespace {
 
-std::atomic<bool> g_cancel{false};
-
 void lambda_grid(int Nl, double lmin, double lmax, std::vector<double> &L) {   // normal_incidence.cpp:235
     L.resize(Nl);
     for (int l = 0; l < Nl; l++) L[l] = lmin + (lmax - lmin) * l / (Nl - 1.0);
@@ -42,6 +40,26 @@ namespace {
         int rc_ = (x);           \
         if (rc_) return rc_;     \
     } while (0)
+
+int rfail(aefdtd_run *r, int rc);
+
+// A driver in one call: create, step to the end (or the caller's cancel
+// flag), post-process.  A cancelled run is post-processed with the steps it
+// made and returns AEFDTD_ECANCELLED.
+int run_one_call(int kind, const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
+                 aefdtd_normal_result *res, double *sensor_out, double *fields_out, const int *end_computation) {
+    aefdtd_run *r = nullptr;
+    TRY(aefdtd_run_create(&r, kind, m, n_sensors, sensors));
+    r->end_flag = end_computation;
+    int rc = AEFDTD_OK;
+    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
+    if (rc == AEFDTD_ECANCELLED) rc = AEFDTD_OK;
+    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, res, sensor_out, fields_out);
+    const bool cancelled = r->cancelled;
+    aefdtd_run_destroy(r);
+    if (rc) return rc;
+    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
 
 int rfail(aefdtd_run *r, int rc) {
     if (r && r->g && !r->g->err.empty()) r->err = r->g->err;
@@ -362,8 +380,6 @@ void aefdtd::run_dims(int kind, const aefdtd_mode *m, int dims[3], int pads[6]) 
 
 extern "C" {
 
-void aefdtd_cancel(void) { g_cancel = true; }
-
 int aefdtd_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors) {
     if (!m || kind < 0 || kind > 2 || (n_sensors > 0 && !sensors) || (m->n_sources > 0 && !m->sources))
         return AEFDTD_EINVAL;
@@ -510,8 +526,21 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
             }
         }
         r->t++;
-        if (g_cancel) return AEFDTD_ECANCELLED;
+        if (r->cancel.load(std::memory_order_relaxed) ||
+            (r->end_flag && __atomic_load_n(r->end_flag, __ATOMIC_RELAXED))) {
+            // the run ends after this step; normal incidence keeps Nt = t steps for its spectra
+            r->done = true;
+            r->cancelled = true;
+            if (r->kind == 0) r->nrun = r->t - 1;
+            return AEFDTD_ECANCELLED;
+        }
     }
+    return r->cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+}
+
+int aefdtd_run_cancel(aefdtd_run *r) {
+    if (!r) return AEFDTD_EINVAL;
+    r->cancel.store(true, std::memory_order_relaxed);
     return AEFDTD_OK;
 }
 
@@ -622,20 +651,10 @@ int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, c
 
 // FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) in one call.
 int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                                aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
+                                aefdtd_normal_result *res, double *sensor_out, double *fields_out,
+                                const int *end_computation) {
     if (!res) return AEFDTD_EINVAL;
-    g_cancel = false;
-    aefdtd_run *r = nullptr;
-    TRY(aefdtd_run_create(&r, 0, m, n_sensors, sensors));
-    int rc = AEFDTD_OK;
-    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
-    // a cancelled run keeps the steps done so far (Nt=t, normal_incidence.cpp:431-435)
-    bool cancelled = rc == AEFDTD_ECANCELLED;
-    if (cancelled) { r->t -= 1; rc = AEFDTD_OK; }
-    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, res, sensor_out, fields_out);
-    aefdtd_run_destroy(r);
-    if (rc) return rc;
-    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+    return run_one_call(0, m, n_sensors, sensors, res, sensor_out, fields_out, end_computation);
 }
 
 // ---- batches of custom-mode runs
@@ -805,34 +824,14 @@ int aefdtd_batch_profile(aefdtd_batch *b, int on, double *ms, long *chunks) {
 
 // mode_default_fdtd (modes/default_fdtd.cpp:24-210) in one call.
 int aefdtd_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors, double *sensor_out,
-                      double *fields_out) {
-    g_cancel = false;
-    aefdtd_run *r = nullptr;
-    TRY(aefdtd_run_create(&r, 2, m, n_sensors, sensors));
-    int rc = AEFDTD_OK;
-    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
-    bool cancelled = rc == AEFDTD_ECANCELLED;
-    if (cancelled) rc = AEFDTD_OK;
-    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);
-    aefdtd_run_destroy(r);
-    if (rc) return rc;
-    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+                      double *fields_out, const int *end_computation) {
+    return run_one_call(2, m, n_sensors, sensors, nullptr, sensor_out, fields_out, end_computation);
 }
 
 // FDTD_single_particle (modes/single_particle.cpp:29-415) in one call.
 int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                               double *sensor_out, double *fields_out) {
-    g_cancel = false;
-    aefdtd_run *r = nullptr;
-    TRY(aefdtd_run_create(&r, 1, m, n_sensors, sensors));
-    int rc = AEFDTD_OK;
-    while (r->t < r->Nt && !r->done && rc == AEFDTD_OK) rc = aefdtd_run_step(r, 1);
-    bool cancelled = rc == AEFDTD_ECANCELLED;
-    if (cancelled) rc = AEFDTD_OK;
-    if (rc == AEFDTD_OK) rc = aefdtd_run_finish(r, nullptr, sensor_out, fields_out);
-    aefdtd_run_destroy(r);
-    if (rc) return rc;
-    return cancelled ? AEFDTD_ECANCELLED : AEFDTD_OK;
+                               double *sensor_out, double *fields_out, const int *end_computation) {
+    return run_one_call(1, m, n_sensors, sensors, nullptr, sensor_out, fields_out, end_computation);
 }
 
 }  // extern "C"

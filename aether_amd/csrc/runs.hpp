@@ -4,6 +4,7 @@
 
 #include "engine.hpp"
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,12 @@ struct aefdtd_run {
     double cc_coeff = 0;
     bool done = false;
     int nrun = -1;                    // normal incidence: the step count its spectra use
+    // cancellation, polled after every step as the drivers poll end_computation
+    // (normal_incidence.cpp:431-435, single_particle.cpp:403, default_fdtd.cpp:201):
+    // this run's own flag (aefdtd_run_cancel) and the caller's flag of a one-call driver
+    std::atomic<bool> cancel{false};
+    const int *end_flag = nullptr;
+    bool cancelled = false;
     aefdtd::FTState ft;               // the spectral criterion's watched cells
     std::vector<double> ft_ring;
     ~aefdtd_run() {

@@ -40,7 +40,7 @@ extern "C" {
 #define AEFDTD_ENOMEM      2   /* device allocation failed / over HBM budget  */
 #define AEFDTD_EDEVICE     3   /* HIP runtime error, or no usable gfx950 GPU  */
 #define AEFDTD_ESTATE      4   /* call out of order (e.g. step before setup)  */
-#define AEFDTD_ECANCELLED  5   /* run stopped by aefdtd_cancel                */
+#define AEFDTD_ECANCELLED  5   /* run stopped by its cancel flag              */
 
 /* ---- grid modes (the `smod` string of FDTD::FDTD, fdtd_core.cpp:198-204) */
 #define AEFDTD_MODE_NORM    0  /* "NORM"   : normal-incidence slab driver     */
@@ -326,29 +326,34 @@ typedef struct aefdtd_normal_result {
 
 /* FDTD_normal_incidence (modes/normal_incidence.cpp:161-607) on the device.
  * sensor_out: per sensor Nl doubles of rsp_result (box: total), may be NULL.
- * fields_out: if non-NULL, 6 x (full grid) doubles of the final fields. */
+ * fields_out: if non-NULL, 6 x (full grid) doubles of the final fields.
+ * end_computation (may be NULL): the driver's `std::atomic<bool>
+ * *end_computation` (lua_fdtd.h:91-108), here an int the caller may set
+ * non-zero from another thread; it is read after every step, and a set flag
+ * ends the run after that step (Nt = t, normal_incidence.cpp:431-435), which
+ * is then post-processed as usual; the call returns AEFDTD_ECANCELLED. */
 int aefdtd_run_normal_incidence(const aefdtd_mode *m, int n_sensors,
                                 const aefdtd_sensor_req *sensors,
                                 aefdtd_normal_result *res, double *sensor_out,
-                                double *fields_out);
+                                double *fields_out, const int *end_computation);
 
 /* FDTD_single_particle (modes/single_particle.cpp:29-415) on the device:
  * TFSF box fed by the 1D auxiliary grid. sensor_out as above; for a box
- * sensor it holds Nl totals followed by 6*Nl per-face values (xm..zp). */
+ * sensor it holds Nl totals followed by 6*Nl per-face values (xm..zp).
+ * end_computation as above (single_particle.cpp:403: the sensors keep the
+ * steps made, including the one during which the flag was set). */
 int aefdtd_run_single_particle(const aefdtd_mode *m, int n_sensors,
                                const aefdtd_sensor_req *sensors,
-                               double *sensor_out, double *fields_out);
+                               double *sensor_out, double *fields_out,
+                               const int *end_computation);
 
 /* mode_default_fdtd (modes/default_fdtd.cpp:24-210), the custom mode, on the
  * device: m->sources (oscillators) and the sensor requests, loop update_E,
  * sources' inject_E, update_H, sensors' feed, sources' inject_H for Nt steps
- * (fixed-time runs).  sensor_out as above. */
+ * (fixed-time runs).  sensor_out and end_computation as above
+ * (default_fdtd.cpp:201). */
 int aefdtd_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                      double *sensor_out, double *fields_out);
-
-/* Cancels a running mode driver between steps (the end_computation poll of
- * normal_incidence.cpp:431-435). Safe to call from another host thread. */
-void aefdtd_cancel(void);
+                      double *sensor_out, double *fields_out, const int *end_computation);
 
 /* The same drivers split into create / step / finish, so that a caller can
  * interleave several runs (serving) or time the stepping alone. kind 0 =
@@ -359,6 +364,12 @@ typedef struct aefdtd_run aefdtd_run;
 int aefdtd_run_create(aefdtd_run **out, int kind, const aefdtd_mode *m,
                       int n_sensors, const aefdtd_sensor_req *sensors);
 int aefdtd_run_step(aefdtd_run *r, int n);
+/* Cancels this run (and no other): the step in progress, or the next one if
+ * none is, is its last (the end_computation poll of normal_incidence.cpp:
+ * 431-435 on this run only); aefdtd_run_step then returns AEFDTD_ECANCELLED
+ * and aefdtd_run_finish post-processes the steps made.  Safe to call from
+ * another host thread. */
+int aefdtd_run_cancel(aefdtd_run *r);
 int aefdtd_run_progress(const aefdtd_run *r, int *t, int *Nt);
 aefdtd_grid *aefdtd_run_grid(aefdtd_run *r);
 const char *aefdtd_run_error(const aefdtd_run *r);

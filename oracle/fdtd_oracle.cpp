@@ -38,6 +38,9 @@
 #include <thread>
 #include <vector>
 
+// The caller's end_computation flag, polled once per step by the drivers.
+static bool cancelled(const int *flag) { return flag && __atomic_load_n(flag, __ATOMIC_RELAXED); }
+
 namespace {
 
 using cplx = std::complex<double>;
@@ -48,7 +51,6 @@ constexpr double c_light = 299792458.0;      // phys_constants.h:21
 constexpr double Pi = std::numbers::pi_v<double>;  // mathUT.h:34
 
 std::string g_create_error;
-std::atomic<bool> g_cancel{false};
 int g_threads = 0;
 
 int nearest_integer(double x) {               // mathUT.cpp:1169-1173
@@ -1702,8 +1704,8 @@ int aeoracle_energy_trace(double *out, int n) {
 
 // FDTD_normal_incidence, modes/normal_incidence.cpp:161-607
 int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                                  aefdtd_normal_result *res, double *sensor_out, double *fields_out) {
-    g_cancel = false;
+                                  aefdtd_normal_result *res, double *sensor_out, double *fields_out,
+                                  const int *end_computation) {
     if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
@@ -1756,12 +1758,13 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
     aeoracle_add_plane_sum(g, zs_s, &chp, zs_s * Dz, 0, tap.data(), &idT);
     Completion E(m, g, Dt);
     int t;
+    bool stopped = false;
     for (t = 0; t < Nt; t++) {
         g->update_E();
         g->update_H();
         aeoracle_feed_sensors(g);
         if (E.on() && E.feed_and_check(g, t)) break;     // Nt = t (normal_incidence.cpp:417-423)
-        if (g_cancel) break;
+        if (cancelled(end_computation)) { stopped = true; break; }    // Nt = t (:431-435)
     }
     int Nrun = t;
     g_last_steps = (t < Nt && E.on()) ? t + 1 : t;
@@ -1828,7 +1831,7 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
     collect_requests(g, n_sensors, sensors, S, sensor_out);
     copy_fields(g, fields_out);
     (void)Nx; (void)Ny;
-    return AEFDTD_OK;
+    return stopped ? AEFDTD_ECANCELLED : AEFDTD_OK;
 }
 
 // FDTD_single_particle, modes/single_particle.cpp:29-415
@@ -1880,8 +1883,7 @@ int aeoracle_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd
 
 // mode_default_fdtd, modes/default_fdtd.cpp:24-210 (fixed step count).
 int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors, double *sensor_out,
-                        double *fields_out) {
-    g_cancel = false;
+                        double *fields_out, const int *end_computation) {
     if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
@@ -1903,22 +1905,22 @@ int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor
     }
     Completion E(m, g, Dt);
     int t;
+    bool stopped = false;
     for (t = 0; t < Nt; t++) {
         g->update_E();       // + sources' inject_E
         g->update_H();       // + sources' inject_H (step count)
         aeoracle_feed_sensors(g);
         if (E.on() && E.feed_and_check(g, t)) { t++; break; }     // default_fdtd.cpp:193-195
-        if (g_cancel) break;
+        if (cancelled(end_computation)) { t++; stopped = true; break; }   // :201
     }
     g_last_steps = t;
     collect_requests(g, n_sensors, sensors, S, sensor_out);
     copy_fields(g, fields_out);
-    return AEFDTD_OK;
+    return stopped ? AEFDTD_ECANCELLED : AEFDTD_OK;
 }
 
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                                 double *sensor_out, double *fields_out) {
-    g_cancel = false;
+                                 double *sensor_out, double *fields_out, const int *end_computation) {
     if (check_time_type(m)) return AEFDTD_EINVAL;
     int Nt = m->Nt;
     double Dx = m->Dx, Dy = m->Dy, Dz = m->Dz;
@@ -1976,6 +1978,7 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefd
     if ((rc = add_requests(g, n_sensors, sensors, S))) return rc;
     Completion E(m, g, Dt);
     int t;
+    bool stopped = false;
     for (t = 0; t < Nt; t++) {
         a->update_E();
         g->update_E();
@@ -1983,12 +1986,12 @@ int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors, const aefd
         g->update_H();
         aeoracle_feed_sensors(g);
         if (E.on() && E.feed_and_check(g, t)) { t++; break; }     // single_particle.cpp:399-401
-        if (g_cancel) break;
+        if (cancelled(end_computation)) { t++; stopped = true; break; }   // :403
     }
     g_last_steps = t;
     collect_requests(g, n_sensors, sensors, S, sensor_out);
     copy_fields(g, fields_out);
-    return AEFDTD_OK;
+    return stopped ? AEFDTD_ECANCELLED : AEFDTD_OK;
 }
 
 }  // extern "C"

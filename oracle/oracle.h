@@ -85,13 +85,13 @@ int aeoracle_nearest_integer(double x);
 int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors,
                                   const aefdtd_sensor_req *sensors,
                                   aefdtd_normal_result *res, double *sensor_out,
-                                  double *fields_out);
+                                  double *fields_out, const int *end_computation);
 int aeoracle_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_sensor_req *sensors);
 int aeoracle_run_custom(const aefdtd_mode *m, int n_sensors, const aefdtd_sensor_req *sensors,
-                        double *sensor_out, double *fields_out);
+                        double *sensor_out, double *fields_out, const int *end_computation);
 int aeoracle_run_single_particle(const aefdtd_mode *m, int n_sensors,
                                  const aefdtd_sensor_req *sensors,
-                                 double *sensor_out, double *fields_out);
+                                 double *sensor_out, double *fields_out, const int *end_computation);
 /* Steps the last driver call ran (early stop by the energy completion check
  * included). */
 int aeoracle_last_steps(void);

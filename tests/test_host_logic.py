@@ -26,7 +26,7 @@ def test_oracle_mirrors_the_grid_abi():
     import oracle
     lib = C.CDLL(oracle.ORACLE_SO)
     engine_only = {"aefdtd_profile_enable", "aefdtd_profile_read", "aefdtd_profile_kinds", "aefdtd_device_bytes", "aefdtd_stats",
-                   "aefdtd_cancel", "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
+                   "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_synchronize",
                    "aefdtd_set_storage_scale", "aefdtd_stream_bandwidth"}
     missing = [s for s in declared_symbols()
                if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_", "aefdtd_server_"))
@@ -242,3 +242,24 @@ def test_server_rejects_bad_submissions():
     with pytest.raises(capi.FDTDError):
         srv.wait(12345, 0.0)
     srv.close()
+
+
+def test_oracle_cancel_flag_ends_after_the_step():
+    """The oracle's drivers poll the caller's end_computation flag after every
+    step (normal_incidence.cpp:431-435, single_particle.cpp:403): a flag set
+    before the run starts ends it after one step, with the sensors of a
+    one-step run (no tapering, so nothing else depends on Nt)."""
+    import oracle
+    be = oracle.load()
+    spec, sens, _ = workloads.spheres_single_particle(n=12, D=2e-9, count=1, rmin=3e-9, rmax=4e-9, seed=5, Nt=40,
+                                                      Nl=6, pml=4, precision=capi.FP64)
+    flag = C.c_int(1)
+    got = capi.run_single_particle(be, spec, sens, end_flag=flag)
+    assert got["cancelled"] and oracle.last_steps() == 1
+    spec1, _, _ = workloads.spheres_single_particle(n=12, D=2e-9, count=1, rmin=3e-9, rmax=4e-9, seed=5, Nt=1,
+                                                    Nl=6, pml=4, precision=capi.FP64)
+    one = capi.run_single_particle(be, spec1, sens)
+    assert np.array_equal(got["sensors"], one["sensors"])
+    flag.value = 0
+    full = capi.run_single_particle(be, spec, sens, end_flag=flag)
+    assert not full["cancelled"] and oracle.last_steps() == 40
