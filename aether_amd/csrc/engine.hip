@@ -823,15 +823,26 @@ int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
 template <typename T>
 int flush_dft(aefdtd_grid *g) {
     if (g->stage_fill == 0) return AEFDTD_OK;
-    int s0 = g->sensor_step - g->stage_fill;
-    for (Sensor *s : g->sensors) {
-        int rc = ensure_coefs(g, *s, g->sensor_step);
-        if (rc) return rc;
-        dim3 grid((unsigned)((s->P + 255) / 256), 6, 1);
+    const int s0 = g->sensor_step - g->stage_fill;
+    // one launch per DFT_MAX_SENS sensors: every plane, component and wavelength
+    for (size_t q0 = 0; q0 < g->sensors.size(); q0 += DFT_MAX_SENS) {
+        const size_t n = std::min<size_t>(DFT_MAX_SENS, g->sensors.size() - q0);
+        DftSet set{};
+        long maxP = 0;
+        for (size_t q = 0; q < n; q++) {
+            Sensor *s = g->sensors[q0 + q];
+            int rc = ensure_coefs(g, *s, g->sensor_step);
+            if (rc) return rc;
+            set.d[q] = DftDesc{s->off, s->P, s->Nl, (const double2 *)s->d_coef, s->d_acc};
+            maxP = std::max(maxP, s->P);
+        }
+        dim3 grid((unsigned)((maxP + 255) / 256), (unsigned)(6 * n), 1);
         hipEvent_t ev;
         prof_begin(g, 3, &ev);
-        hipLaunchKernelGGL(k_dft<T>, grid, dim3(256), 0, g->stream, (const T *)g->d_stage, g->stage_P, s->off, s->P,
-                           s->Nl, (const double2 *)s->d_coef, s0, g->stage_fill, s->d_acc);
+        // T is double exactly for FP64 grids
+        constexpr int KD = stage_depth(sizeof(T) == 8 ? AEFDTD_FP64 : AEFDTD_FP32);
+        hipLaunchKernelGGL((k_dft<T, KD>), grid, dim3(256), 0, g->stream, (const T *)g->d_stage, g->stage_P, set, s0,
+                           g->stage_fill);
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
     }
@@ -868,9 +879,13 @@ int prepare_stage(aefdtd_grid *g) {
     if (!g->strip_cols.empty() &&
         (rc = dalloc(g, &g->d_strip, g->strip_cols.size() * 12 * (size_t)g->Ny * g->Nz * sizeof(T))))
         return rc;
-    if ((rc = dalloc(g, &g->d_stage, (size_t)KS * 6 * Ptot * sizeof(T)))) return rc;
+    if ((rc = dalloc(g, &g->d_stage, (size_t)stage_depth(g->precision) * 6 * Ptot * sizeof(T)))) return rc;
     if ((rc = dalloc(g, &g->d_sdesc, sd.size() * sizeof(SensDesc)))) return rc;
     HIPOK(g, copy_sync(g, g->d_sdesc, sd.data(), sd.size() * sizeof(SensDesc), hipMemcpyHostToDevice));
+    // the DFT coefficient tables for the whole run now, so that no flush waits
+    // for host work and a stream synchronisation
+    for (Sensor *s : g->sensors)
+        if ((rc = ensure_coefs(g, *s, std::max(g->Nt, 1)))) return rc;
     return AEFDTD_OK;
 }
 
@@ -902,7 +917,7 @@ int feed_T(aefdtd_grid *g) {
         HIPOK(g, hipGetLastError());
         g->stage_fill++;
         g->sensor_step++;
-        if (g->stage_fill == KS) {
+        if (g->stage_fill == stage_depth(g->precision)) {
             if ((rc = flush_dft<T>(g))) return rc;
         }
     }
@@ -1385,9 +1400,12 @@ int batch_add(Batch *b, aefdtd_grid *g) {
     if (g->tfsf_aux || g->inj_k >= 0 || !g->sums.empty()) return bad("TFSF, plane injection and plane sums are not batched");
     if (is_column(g)) return bad("1x1xN column grids are not batched");
     if (g->tstep >= g->Nt) return bad("grid has no steps left");
-    if (g->tstep % KS != 0 || g->stage_fill != 0) return bad("a grid joins a batch after a multiple of 64 steps");
+    if (g->tstep % KS != 0) return bad("a grid joins a batch after a multiple of 64 steps");
     int rc = ensure_ready(g);
     if (rc) { b->err = g->err; return rc; }
+    // the batch stages 64 steps per chunk from an empty stage: flush what the
+    // grid staged on its own (its stage is deeper than a chunk)
+    if ((rc = with_ctype(g, [&]<typename T>() { return flush_dft<T>(g); }))) { b->err = g->err; return rc; }
     if (g->n_disp) return bad("grids with dispersive (RC) materials are not batched");
     BHIPOK(b, hipStreamSynchronize(g->stream));
     BHIPOK(b, hipStreamSynchronize(b->stream));

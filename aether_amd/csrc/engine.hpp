@@ -114,6 +114,11 @@ int batch_skewed_members(const Batch *b);
 // starts, 0 stops, -1 only reads.
 void batch_profile(Batch *b, int on, double *ms, long *chunks);
 
+// DFT staging depth of a grid stepped on its own (steps between accumulator
+// flushes): the lane of the flush keeps this many samples in registers, 128
+// FP32 values or 64 FP64 ones.  Batches flush every 64-step chunk.
+constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 64 : 128; }
+
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct PlaneSum;    // normal-incidence R/T plane sum (engine.hip)
 

@@ -28,7 +28,7 @@ namespace aefdtd {
 namespace dev {
 
 constexpr int BX = 64;
-constexpr int KS = 64;             // DFT staging depth (steps per accumulate)
+constexpr int KS = 64;             // steps per batch chunk (and its DFT staging depth)
 constexpr int DFT_LCH = 32;        // wavelengths per LDS chunk in the DFT pass
 constexpr int MAX_MATS = 256;
 
@@ -1233,55 +1233,89 @@ __global__ __launch_bounds__(256) void k_sample_batch(const SampB<S, T> *__restr
     sample_point(q.d, p, s, q.Ptot, q.F[0], q.F[1], q.F[2], q.F[3], q.F[4], q.F[5], q.Nx, q.Ny, q.Nz, q.P, q.stage);
 }
 
-// Running DFT over a block of nfill <= KS staged steps (sensors/sensors.cpp:
+// Running DFT over a block of nfill <= KD staged steps (sensors/sensors.cpp:
 // 456-479): acc[c][l][p] += sum_s coef[s0+s][E|H][l] * stage[s][c][p].  The
-// accumulators are FP64; the block's KS-term sums are formed in the field
+// accumulators are FP64; the block's KD-term sums are formed in the field
 // arithmetic type T (FP64 for FP64 runs, so FP64 runs keep the reference's
 // double sums; FP32 for FP32/FP16 runs, whose samples are FP32 already).
-// One lane per (cell, component) keeps its KS samples in registers; the
+// One lane per (cell, component) keeps its KD samples in registers; the
 // coefficients of DFT_LCH wavelengths at a time are staged in LDS and read as
-// broadcasts, so each accumulator is read and written once per KS steps.
-template <typename T>
+// broadcasts, so each accumulator is read and written once per KD steps.  The
+// next wavelength's accumulator is requested before the current one's sums
+// are formed, so its latency hides behind the 2 KD multiply-adds.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_nt(const double2 *p) {
+    const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(double2 v, double2 *p) {
+    const dvec2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<dvec2 *>(p));
+}
+
+template <typename T, int KD>
 __device__ __forceinline__ void dft_block(const T *stage, long Ptot, long off, long P, int Nl, const double2 *coef,
                                           int s0, int nfill, double2 *acc, int c) {
     using T2 = std::conditional_t<std::is_same_v<T, double>, double2, float2>;
-    __shared__ T2 W[KS * DFT_LCH];
+    __shared__ T2 W[KD * DFT_LCH];
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int h = c >= 3 ? 1 : 0;
     const bool live = p < P;
-    T x[KS];
+    T x[KD];
 #pragma unroll
-    for (int s = 0; s < KS; s++) x[s] = (live && s < nfill) ? stage[((size_t)s * 6 + c) * Ptot + off + p] : (T)0;
+    for (int s = 0; s < KD; s++)
+        x[s] = (live && s < nfill) ? __builtin_nontemporal_load(stage + ((size_t)s * 6 + c) * Ptot + off + p) : (T)0;
+    double2 *a = acc + (size_t)c * Nl * P + p;
     for (int l0 = 0; l0 < Nl; l0 += DFT_LCH) {
         const int nl = min(DFT_LCH, Nl - l0);
         __syncthreads();
-        for (int q = threadIdx.x; q < KS * DFT_LCH; q += blockDim.x) {
+        for (int q = threadIdx.x; q < KD * DFT_LCH; q += blockDim.x) {
             const int s = q / DFT_LCH, l = q % DFT_LCH;
             const double2 w = (s < nfill && l < nl) ? coef[((size_t)(s0 + s) * 2 + h) * Nl + l0 + l] : make_double2(0.0, 0.0);
             W[q] = T2{(T)w.x, (T)w.y};
         }
         __syncthreads();
         if (live) {
+            double2 A = ld_nt(a + (size_t)l0 * P);
             for (int l = 0; l < nl; l++) {
+                double2 An = A;
+                if (l + 1 < nl) An = ld_nt(a + (size_t)(l0 + l + 1) * P);
                 T re = 0, im = 0;
 #pragma unroll
-                for (int s = 0; s < KS; s++) {
+                for (int s = 0; s < KD; s++) {
                     const T2 w = W[s * DFT_LCH + l];
                     re += w.x * x[s];
                     im += w.y * x[s];
                 }
-                double2 &A = acc[((size_t)c * Nl + l0 + l) * P + p];
                 A.x += (double)re;
                 A.y += (double)im;
+                st_nt(A, a + (size_t)(l0 + l) * P);
+                A = An;
             }
         }
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, long off, long P, int Nl,
-                                             const double2 *coef, int s0, int nfill, double2 *acc) {
-    dft_block(stage, Ptot, off, P, Nl, coef, s0, nfill, acc, blockIdx.y);
+// One sensor of a grid's DFT flush: its plane samples at stage offset `off`
+// (P cells), its wavelengths, coefficient table and accumulators.
+struct DftDesc {
+    long off, P;
+    int Nl;
+    const double2 *coef;
+    double2 *acc;
+};
+constexpr int DFT_MAX_SENS = 16;
+struct DftSet {
+    DftDesc d[DFT_MAX_SENS];
+};
+
+// The DFT flush of up to DFT_MAX_SENS sensors of one grid in one launch:
+// blockIdx.y = 6 sensor + component, blockIdx.x = block of 256 plane cells.
+template <typename T, int KD>
+__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, const DftSet set, int s0, int nfill) {
+    const DftDesc &d = set.d[blockIdx.y / 6];
+    if ((long)blockIdx.x * blockDim.x >= d.P) return;
+    dft_block<T, KD>(stage, Ptot, d.off, d.P, d.Nl, d.coef, s0, nfill, d.acc, blockIdx.y % 6);
 }
 
 // One sensor of one grid of a batch for the DFT flush at the end of a chunk.
@@ -1303,7 +1337,7 @@ __global__ __launch_bounds__(256) void k_dft_batch(const DftB<T> *__restrict__ e
     const int t0 = tcur[q.member];
     const int nfill = min(KS, tend[q.member] - t0);
     if (nfill <= 0 || (long)blockIdx.x * blockDim.x >= q.P) return;
-    dft_block(q.stage, q.Ptot, q.off, q.P, q.Nl, q.coef, t0, nfill, q.acc, blockIdx.y);
+    dft_block<T, KS>(q.stage, q.Ptot, q.off, q.P, q.Nl, q.coef, t0, nfill, q.acc, blockIdx.y);
 }
 
 // End of a batch chunk: every member advances KS steps, stopping at its end.

@@ -452,7 +452,13 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
             Ptot += P[f];
         }
     }
-    b += (size_t)kRound * 6 * Ptot * cs;                    // DFT stage
+    b += (size_t)std::max(kRound, stage_depth(p)) * 6 * Ptot * cs;   // DFT stage
+    // sensor strips of planes normal to x on 3D grids (two columns x 6 values per (y, z))
+    if (d[2] > 1 && d[0] > 1) {
+        int nx_planes = 0;
+        for (int i = 0; i < ns; i++) nx_planes += sens[i].box ? 2 : (sens[i].orientation <= 1 ? 1 : 0);
+        b += (size_t)std::min(nx_planes, 4) * 12 * d[1] * d[2] * cs;
+    }
     if (kind == 2) b += (size_t)std::max(0, m->n_sources) * std::max(m->Nt, 1) * 8;
     if (kind == 0) b += 2 * (size_t)std::max(m->Nt, 1) * 64 * 6 * 8 + 2 * (size_t)m->Nt * 8;   // R/T plane sums
     if (kind == 1) b += (size_t)d[2] * (6 * ss + 1 + 8 * cs) + (size_t)d[2] * 4 * 4;           // auxiliary column

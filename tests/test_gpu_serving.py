@@ -64,7 +64,7 @@ def test_server_budget_admission_and_estimate():
     """With room for one request at a time the server serves them in FIFO
     order, one resident at a time, and every run's bytes stay within its
     admission estimate."""
-    reqs = [_custom(11, 40, 200), _custom(12, 44, 150), _single(13, 16, 120)]
+    reqs = [_custom(11, 64, 200), _custom(12, 70, 150), _single(13, 16, 120)]
     be = capi.engine()
     est = [capi.estimate_bytes(be, k, sp, q) for k, sp, q in reqs]
     budget = max(est)
