@@ -151,6 +151,7 @@ def _sig(lib, p):
         "autotune": (i, [vp, i, _ip]),
         "set_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
+        "set_step_tiles": (i, [vp, i]),
         "batch_create": (i, [C.POINTER(vp), i]),
         "batch_destroy": (i, [vp]),
         "batch_error": (C.c_char_p, [vp]),

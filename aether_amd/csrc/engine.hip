@@ -712,6 +712,105 @@ int launch_H(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
     return AEFDTD_OK;
 }
 
+// ---- one-pass step (k_step + k_step_edge) of 3D grids
+constexpr int kStepTY[4] = {4, 8, 16, 32};
+constexpr int kStepKZ[4] = {4, 8, 16, 32};
+
+// Tile plan of step code cfg on grid g (TY index | (RY - 1) << 2 | KZ index
+// << 4 | XCD fold << 6); ok = false where the one-pass kernel does not apply
+// (2D grids, columns, rows wider than a 1024-lane workgroup).
+struct StepPlan {
+    int TY = 0, RY = 0, KZ = 0, WX = 0, nty = 0, ntz = 0, remap = 0;
+    bool ok = false;
+};
+
+StepPlan step_plan(const aefdtd_grid *g, int cfg) {
+    StepPlan p;
+    if (cfg < 0 || cfg > 127 || g->Nz < 2 || is_column(g)) return p;
+    p.TY = kStepTY[cfg & 3];
+    p.RY = ((cfg >> 2) & 3) + 1;
+    p.KZ = kStepKZ[(cfg >> 4) & 3];
+    p.remap = (cfg >> 6) & 1;
+    p.WX = (g->Nx + 63) / 64;
+    if (p.WX * p.RY > 16 || p.RY > p.TY) return p;
+    p.nty = (g->Ny + p.TY - 1) / p.TY;
+    p.ntz = (g->Nz + p.KZ - 1) / p.KZ;
+    p.ok = true;
+    return p;
+}
+
+// The two launches of one step: the E + interior-H pass and the edge-H pass
+// (with the TFSF column's H when xh is given).
+template <typename S, typename T>
+void launch_step_pair(aefdtd_grid *g, const StepPlan &p, const EArgs<S, T> &a, const HArgs<S, T> &h,
+                      const HArgs<S, T> *xh, bool en) {
+    const StepGeom sg{p.WX, p.RY, p.TY, p.KZ, p.nty, p.ntz, p.remap};
+    const int nt = p.nty * p.ntz;
+    const size_t lds = (size_t)a.nmat * sizeof(MatL<T>);
+    const dim3 blk(64 * p.WX * p.RY);
+    hipEvent_t ev;
+    prof_begin(g, 0, &ev);
+    bool done = false;
+    if constexpr (std::is_same_v<S, T>) {
+        if (en) {
+            hipLaunchKernelGGL((k_step<S, T, true>), dim3(nt), blk, lds, g->stream, a, h, sg);
+            done = true;
+        }
+    }
+    if (!done) hipLaunchKernelGGL((k_step<S, T, false>), dim3(nt), blk, lds, g->stream, a, h, sg);
+    prof_end(g, ev);
+    const EdgeGeom eg{p.TY, p.KZ, p.nty, p.ntz, (long)p.ntz * g->Ny + (long)p.nty * (g->Nz - p.ntz)};
+    const long nb = (eg.nrows + 3) / 4;
+    const long naux = xh ? (xh->Nz + 255) / 256 : 0;
+    prof_begin(g, 1, &ev);
+    hipLaunchKernelGGL((k_step_edge<S, T>), dim3((unsigned)(nb + naux), (unsigned)p.WX), dim3(64, 4), 0, g->stream,
+                       h, eg, nb, xh ? *xh : h);
+    prof_end(g, ev);
+}
+
+// One full step of g (and of its TFSF column aux) by the one-pass kernel:
+// the column's E update first (the main grid's H reads its new E), then the
+// RC pre-pass (unless the last sensor feed ran it), the two launches.
+template <typename S, typename T>
+int step_T(aefdtd_grid *g, aefdtd_grid *aux, const StepPlan &p) {
+    int rc = ensure_osc_tables(g, g->tstep + 1);
+    if (rc) return rc;
+    if (aux) {
+        if ((rc = ensure_osc_tables(aux, aux->tstep + 1)) || (rc = launch_rc<S, T>(aux))) return rc;
+        const EArgs<S, T> x = e_args<S, T>(aux);
+        hipLaunchKernelGGL((k_update_E_col<S, T>), dim3((aux->Nz + 255) / 256), dim3(256), x.nmat * sizeof(MatL<T>),
+                           g->stream, x);
+    }
+    if (!g->rc_ahead && (rc = launch_rc<S, T>(g))) return rc;
+    g->rc_ahead = false;
+    EArgs<S, T> a = e_args<S, T>(g);
+    const HArgs<S, T> h = h_args<S, T>(g);
+    HArgs<S, T> xh{};
+    if (aux) xh = h_args<S, T>(aux);
+    g->en_parts = 0;
+    bool en = false;
+    if constexpr (std::is_same_v<S, T>) {
+        if (g->en_fuse) {
+            const int nt = p.nty * p.ntz;
+            if (nt > g->en_part_cap) {
+                dfree_t(g->d_en_part);
+                g->en_part_cap = 0;
+                void *q;
+                if ((rc = dalloc(g, &q, (size_t)nt * sizeof(double)))) return rc;
+                g->d_en_part = (double *)q;
+                g->en_part_cap = nt;
+            }
+            a.en_part = g->d_en_part;
+            g->en_parts = nt;
+            en = true;
+        }
+    }
+    launch_step_pair<S, T>(g, p, a, h, aux ? &xh : nullptr, en);
+    HIPOK(g, hipGetLastError());
+    g->strip_valid = h.nstrip > 0;
+    return AEFDTD_OK;
+}
+
 // Times one tile configuration of the E or H pass: reps launches between events.
 template <typename S, typename T>
 float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hipEvent_t eb) {
@@ -724,6 +823,22 @@ float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hi
             HArgs<S, T> a = h_args<S, T>(g);
             dispatch_H<S, T>(g, a, cfg);
         }
+    }
+    (void)hipEventRecord(eb, g->stream);
+    (void)hipEventSynchronize(eb);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ea, eb);
+    return ms;
+}
+
+// Times one tile plan of the one-pass step: reps step launches between events.
+template <typename S, typename T>
+float time_step(aefdtd_grid *g, const StepPlan &p, int reps, hipEvent_t ea, hipEvent_t eb) {
+    (void)hipEventRecord(ea, g->stream);
+    for (int r = 0; r < reps; r++) {
+        const EArgs<S, T> a = e_args<S, T>(g);
+        const HArgs<S, T> h = h_args<S, T>(g);
+        launch_step_pair<S, T>(g, p, a, h, nullptr, false);
     }
     (void)hipEventRecord(eb, g->stream);
     (void)hipEventSynchronize(eb);
@@ -757,6 +872,7 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     HIPOK(g, timing_event(&ea));
     HIPOK(g, timing_event(&eb));
     int best[2] = {g->tile_e, g->tile_h};
+    float bsum = 0;
     for (int pass = 0; pass < 2; pass++) {
         float bt = 1e30f;
         for (int cfg = 0; cfg < 32; cfg++) {
@@ -764,7 +880,21 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
             float ms = time_cfg<S, T>(g, pass == 0, cfg, reps, ea, eb);
             if (ms < bt) { bt = ms; best[pass] = cfg; }
         }
+        bsum += bt;
     }
+    // the one-pass step against the best pair of passes: rows per tile 8/16,
+    // 2/3 rows in flight, 4/8 planes, XCD fold on
+    int best_step = -1;
+    for (int ty = 1; ty <= 2; ty++)
+        for (int ry = 2; ry <= 3; ry++)
+            for (int kz = 0; kz <= 1; kz++) {
+                const int code = ty | ((ry - 1) << 2) | (kz << 4) | (1 << 6);
+                const StepPlan p = step_plan(g, code);
+                if (!p.ok) continue;
+                time_step<S, T>(g, p, 1, ea, eb);
+                const float ms = time_step<S, T>(g, p, reps, ea, eb);
+                if (ms < bsum) { bsum = ms; best_step = code; }
+            }
     (void)hipEventDestroy(ea);
     (void)hipEventDestroy(eb);
     for (size_t q = 0; q < state.size(); q++) {
@@ -779,6 +909,7 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     g->strip_valid = false;             // the timing passes rewrote the strip; the fields were restored
     g->tile_e = best[0];
     g->tile_h = best[1];
+    g->step_cfg = best_step;
     if (out) { out[0] = best[0]; out[1] = best[1]; }
     return AEFDTD_OK;
 }
@@ -1796,7 +1927,51 @@ int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux) {
     return AEFDTD_OK;
 }
 
+int step_pair(aefdtd_grid *g, aefdtd_grid *aux) {
+    if (!g) return AEFDTD_EINVAL;
+    const StepPlan p = step_plan(g, g->step_cfg);
+    // the one-pass kernel needs the TFSF column stepped with the grid
+    if (!p.ok || (aux ? !pairable(g, aux) : g->tfsf_aux != nullptr)) {
+        int rc;
+        if (aux) {
+            if ((rc = update_E_pair(g, aux))) return rc;
+            return update_H_pair(g, aux);
+        }
+        if ((rc = aefdtd_update_E(g))) return rc;
+        return aefdtd_update_H(g);
+    }
+    for (aefdtd_grid *q : {aux, g}) {
+        if (!q) continue;
+        int rc = ensure_ready(q);
+        if (rc) return rc;
+        if (q->tstep == 0 || q->pml_dirty) {     // pml_coeff_calc at tstep==0 (core/fdtd_core.cpp:1165)
+            if ((rc = upload_profiles(q))) return rc;
+            q->pml_dirty = false;
+        }
+    }
+    int rc = with_types(g, [&]<typename S, typename T>() { return step_T<S, T>(g, aux, p); });
+    if (rc) return rc;
+    if (aux) aux->tstep += 1;
+    g->tstep += 1;
+    return AEFDTD_OK;
+}
+
 }  // namespace aefdtd
+
+extern "C" int aefdtd_step(aefdtd_grid *g, int n) {
+    if (!g || n < 0) return AEFDTD_EINVAL;
+    for (int s = 0; s < n; s++) {
+        int rc = aefdtd::step_pair(g, nullptr);
+        if (rc) return rc;
+    }
+    return AEFDTD_OK;
+}
+
+extern "C" int aefdtd_set_step_tiles(aefdtd_grid *g, int cfg) {
+    if (!g || cfg < -1 || cfg > 127) return AEFDTD_EINVAL;
+    g->step_cfg = cfg;
+    return AEFDTD_OK;
+}
 
 extern "C" {
 

@@ -103,6 +103,9 @@ int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
 // A grid and its attached TFSF column stepped with one launch per half step.
 int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux);
 int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux);
+// One full step (E then H) of a grid and its TFSF column (aux may be null):
+// the one-pass kernel where the grid allows it, else the two passes.
+int step_pair(aefdtd_grid *g, aefdtd_grid *aux);
 hipStream_t batch_stream(const Batch *b);
 int batch_set_tiles(Batch *b, int tile_e, int tile_h);
 // Time-skewed passes of 2D TE/TM members: code = tile height index (0 small,
@@ -193,6 +196,10 @@ struct aefdtd_grid {
     int en_parts = 0;                 // partials the last E launch wrote (0: none)
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
     int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
+    // one-pass step of 3D grids (k_step + k_step_edge): tile code (TY index |
+    // (RY - 1) << 2 | KZ index << 4 | XCD fold << 6), -1 = the E and H passes
+    // (the default; the autotuner picks the one-pass step where it is faster)
+    int step_cfg = -1;
     // bookkeeping
     bool ready = false;               // device copies up to date
     bool rc_ahead = false;            // the last sensor feed also ran the next E pass's RC pre-pass

@@ -2045,6 +2045,306 @@ __global__ __launch_bounds__(256) void k_energy_fin(const double *part, int n, d
     }
 }
 
+// ================================================================ one-pass step
+// A full time step of a 3D grid in one pass, in place: E of plane k, then H
+// of plane k-1.  A workgroup owns whole x rows (WX wavefronts of 64 lanes
+// cover Nx, so the x wrap stays inside it), TY rows in y (lane row ry of RY
+// takes rows ty0 + ry, ty0 + ry + RY, ...) and KZ planes, and marches z:
+//
+//   iteration k:  A) E of plane k, every row of the tile (stencil + TFSF + RC
+//                    increment + CPML + PEC + injection + sources, as e_cell)
+//                 -- workgroup barrier --
+//                 B) H of plane k-1, every row but the tile's last (as h_cell)
+//
+// Phase B reads E of planes k-1 and k and the old H of plane k-1, all touched
+// by the same workgroup one or two phases before, so they come from the
+// caches: HBM sees each field read once and written once per step (the two
+// passes read and write E and H, then read both again).  The new E is stored
+// normally (it is re-read at once), H with streaming stores.
+//
+// The H of the tile's last row and last plane (its "edge" cells) needs E of
+// the next tiles' first row / plane, which other workgroups produce at their
+// own pace; those cells are left to k_step_edge, run after this pass.  The
+// edge cells are exactly the H values other tiles read here (the j-1 row and
+// k-1 plane of a tile's E update), so no tile reads an H value another has
+// already advanced.  Inside a workgroup, H of plane k is written in phase B
+// of iteration k+1, after the barrier that every wavefront reaches only once
+// it has finished phase A of iteration k+1 (which reads the last old H of
+// plane k: the z neighbour of plane k+1).  Every cell sees the same inputs and
+// the same operations as in the two-pass path, so results are bit-identical
+// to k_update_E + k_update_H.
+
+struct StepGeom {
+    int WX, RY, TY, KZ, nty, ntz, remap;
+};
+
+// Rows of the edge pass: z-edge planes (the last plane of each z tile, every
+// row) then the y-edge rows (the last row of each y tile) of the other planes.
+struct EdgeGeom {
+    int TY, KZ, nty, ntz;
+    long nrows;
+};
+
+template <typename S, typename T, bool EN>
+__global__ __launch_bounds__(1024) void k_step(const EArgs<S, T> a, const HArgs<S, T> h, const StepGeom g) {
+    extern __shared__ char lds_raw[];
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    stage_mats(a, sm);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int xw = w % g.WX, ry = w / g.WX;
+    int bxd, by, bz;
+    tile_of(blockIdx.x, 1, g.nty, g.ntz, g.remap, bxd, by, bz);
+    const int ty0 = by * g.TY, ty1 = min(ty0 + g.TY, a.Ny);
+    const int k0 = bz * g.KZ, k1 = min(k0 + g.KZ, a.Nz);
+    const int i = 64 * xw + lane;
+    const bool iv = i < a.Nx;
+    const int il = iv ? i : a.Nx - 1;                           // loads of idle lanes stay in bounds
+    const int NxNy = a.P * a.Ny;
+    const int oim = (il == 0) ? a.Nx - 1 : -1;                  // (i-1) with wrap
+    const int oip = (il == a.Nx - 1) ? -(a.Nx - 1) : 1;         // (i+1) with wrap
+    const int sideE = side_block(a, 0, 0, a.Nx, ty0, ty1, k0, k1);
+    const int sideH = side_block(h, 1, 0, a.Nx, ty0, ty1, k0, k1);
+    const T ikxE = a.pf.ikE[0][il], ikxH = h.pf.ikH[0][il];
+    const int nxE = slabE(il, a.Nx, a.pml[0], a.pml[1]), nxH = slabH(il, a.Nx, h.pml[0], h.pml[1]);
+    // the lane's x CPML profile values (loop invariant)
+    T bxE = 0, cxE = 0, bxH = 0, cxH = 0;
+    if ((sideE & SIDE_X) && nxE >= 0) { bxE = a.pf.bE[0][il]; cxE = a.pf.cE[0][il]; }
+    if ((sideH & SIDE_X) && nxH >= 0) { bxH = h.pf.bH[0][il]; cxH = h.pf.cH[0][il]; }
+    // strip slots of the lane's column (sensor planes normal to x)
+    int sq0 = -1, sq1 = -1;
+    for (int q = 0; q < h.nstrip; q++) {
+        const int x0 = h.scol[q], x1 = (x0 + 1 == a.Nx) ? 0 : x0 + 1;
+        const int s = (il == x0) ? 2 * q : (il == x1 ? 2 * q + 1 : -1);
+        if (s >= 0) {
+            if (sq0 < 0) sq0 = s;
+            else sq1 = s;
+        }
+    }
+    // a row's loads, with the CPML psi of its cells (requested with the
+    // fields, one row ahead of their use)
+    struct LdE {
+        T hx, hy, hz, hy_im, hz_im, hx_jm, hz_jm, hx_km, hy_km, ex, ey, ez;
+        int m;
+        PsiSetE<T> q;
+    };
+    struct LdH {
+        T hx, hy, hz, ex, ey, ez, ey_ip, ez_ip, ex_jp, ez_jp, ex_kp, ey_kp;
+        PsiSetH<T> q;
+    };
+    auto loadE = [&](int j, int k) {
+        const int id = il + a.P * j + k * NxNy;
+        const int ojm = (j == 0) ? (a.Ny - 1) * a.P : -a.P;
+        const int okm = (k == 0) ? (a.Nz - 1) * NxNy : -NxNy;
+        LdE v;
+        v.hx = a.Hx[id];
+        v.hy = a.Hy[id];
+        v.hz = a.Hz[id];
+        v.hy_im = a.Hy[id + oim];
+        v.hz_im = a.Hz[id + oim];
+        v.hx_jm = a.Hx[id + ojm];
+        v.hz_jm = a.Hz[id + ojm];
+        v.hx_km = a.Hx[id + okm];
+        v.hy_km = a.Hy[id + okm];
+        v.ex = a.Ex.nt(id);                                     // read once per step: streamed
+        v.ey = a.Ey.nt(id);
+        v.ez = a.Ez.nt(id);
+        v.m = __builtin_nontemporal_load(a.mat + id);
+        v.q = PsiSetE<T>{};
+        const PsiAddrE<S, T> ad{a, il, j, k};
+        if ((sideE & SIDE_X) && nxE >= 0) {
+            if (a.en & 2) v.q.yx = *ad.yx(nxE);
+            if (a.en & 4) v.q.zx = *ad.zx(nxE);
+        }
+        if (sideE & SIDE_Y) {
+            const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+            if (ny_ >= 0) {
+                if (a.en & 1) v.q.xy = *ad.xy(ny_);
+                if (a.en & 4) v.q.zy = *ad.zy(ny_);
+            }
+        }
+        if (sideE & SIDE_Z) {
+            const int nz_ = slabEz(a, k);
+            if (nz_ >= 0) {
+                if (a.en & 1) v.q.xz = *ad.xz(nz_);
+                if (a.en & 2) v.q.yz = *ad.yz(nz_);
+            }
+        }
+        return v;
+    };
+    auto loadH = [&](int j, int k) {
+        const int id = il + a.P * j + k * NxNy;
+        const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.P : a.P;
+        const int okp = (k == a.Nz - 1) ? -(a.Nz - 1) * NxNy : NxNy;
+        LdH v;
+        v.hx = h.Hx[id];
+        v.hy = h.Hy[id];
+        v.hz = h.Hz[id];
+        v.ex = h.Ex[id];
+        v.ey = h.Ey[id];
+        v.ez = h.Ez[id];
+        v.ey_ip = h.Ey[id + oip];
+        v.ez_ip = h.Ez[id + oip];
+        v.ex_jp = h.Ex[id + ojp];
+        v.ez_jp = h.Ez[id + ojp];
+        v.ex_kp = h.Ex[id + okp];
+        v.ey_kp = h.Ey[id + okp];
+        v.q = PsiSetH<T>{};
+        const PsiAddrH<S, T> ad{h, il, j, k};
+        if ((sideH & SIDE_X) && nxH >= 0) {
+            if (h.en & 16) v.q.yx = *ad.yx(nxH);
+            if (h.en & 32) v.q.zx = *ad.zx(nxH);
+        }
+        if (sideH & SIDE_Y) {
+            const int ny_ = slabH(j, h.Ny, h.pml[2], h.pml[3]);
+            if (ny_ >= 0) {
+                if (h.en & 8) v.q.xy = *ad.xy(ny_);
+                if (h.en & 32) v.q.zy = *ad.zy(ny_);
+            }
+        }
+        if (sideH & SIDE_Z) {
+            const int nz_ = slabH(k, h.Nz, h.pml[4], h.pml[5]);
+            if (nz_ >= 0) {
+                if (h.en & 8) v.q.xz = *ad.xz(nz_);
+                if (h.en & 16) v.q.yz = *ad.yz(nz_);
+            }
+        }
+        return v;
+    };
+    T en = 0;
+    const int jA0 = ty0 + ry;
+    for (int k = k0; k < k1; k++) {
+        // ---- A: E of plane k (rows jA0, jA0 + RY, ...), the next row's loads in flight
+        const T ikzE = uload(a.pf.ikE[2], k);
+        if (jA0 < ty1) {
+            LdE cur = loadE(jA0, k);
+            for (int j = jA0; j < ty1; j += g.RY) {
+                const int jn = j + g.RY;
+                LdE nxt = cur;
+                if (jn < ty1) nxt = loadE(jn, k);
+                if (iv) {
+                    const int id = il + a.P * j + k * NxNy;
+                    T byE = 0, cyE = 0;
+                    const int nyE = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+                    if ((sideE & SIDE_Y) && nyE >= 0) { byE = uload(a.pf.bE[1], j); cyE = uload(a.pf.cE[1], j); }
+                    const V3<T> e = e_cell_vals(a, sm[cur.m], il, j, k, id, cur.ex, cur.ey, cur.ez, cur.hz - cur.hz_jm,
+                                                cur.hy - cur.hy_km, cur.hx - cur.hx_km, cur.hz - cur.hz_im,
+                                                cur.hy - cur.hy_im, cur.hx - cur.hx_jm, ikxE, uload(a.pf.ikE[1], j),
+                                                ikzE, nxE, nyE, a.ostep, sideE,
+                                                PrefPsiE<S, T>{a, il, j, k, cur.q, bxE, cxE, byE, cyE});
+                    a.Ex.set(id, e.x);
+                    a.Ey.set(id, e.y);
+                    a.Ez.set(id, e.z);
+                    if constexpr (EN) en += e.x * e.x + e.y * e.y + e.z * e.z;
+                }
+                cur = nxt;
+            }
+        }
+        __syncthreads();
+        // ---- B: H of plane k-1, every row but the tile's last
+        if (k > k0 && jA0 < ty1 - 1) {
+            const int kh = k - 1;
+            const T ikzH = uload(h.pf.ikH[2], kh);
+            LdH cur = loadH(jA0, kh);
+            for (int j = jA0; j < ty1 - 1; j += g.RY) {
+                const int jn = j + g.RY;
+                LdH nxt = cur;
+                if (jn < ty1 - 1) nxt = loadH(jn, kh);
+                if (iv) {
+                    const int id = il + a.P * j + kh * NxNy;
+                    T byH = 0, cyH = 0;
+                    const int nyH = slabH(j, a.Ny, h.pml[2], h.pml[3]);
+                    if ((sideH & SIDE_Y) && nyH >= 0) { byH = uload(h.pf.bH[1], j); cyH = uload(h.pf.cH[1], j); }
+                    T hx = cur.hx, hy = cur.hy, hz = cur.hz;
+                    h_cell_vals(h, il, j, kh, hx, hy, hz, cur.ey_kp - cur.ey, cur.ex_kp - cur.ex, cur.ez_jp - cur.ez,
+                                cur.ez_ip - cur.ez, cur.ex_jp - cur.ex, cur.ey_ip - cur.ey, ikxH, uload(h.pf.ikH[1], j),
+                                ikzH, nxH, nyH, sideH, PrefPsiH<S, T>{h, il, j, kh, cur.q, bxH, cxH, byH, cyH});
+                    h.Hx.set_nt(id, hx);
+                    h.Hy.set_nt(id, hy);
+                    h.Hz.set_nt(id, hz);
+                    if (sq0 >= 0) {
+                        const T v[6] = {cur.ex, cur.ey, cur.ez, stored(h.Hx, hx), stored(h.Hy, hy), stored(h.Hz, hz)};
+                        strip_put(h, sq0, j, kh, v);
+                        if (sq1 >= 0) strip_put(h, sq1, j, kh, v);
+                    }
+                }
+                cur = nxt;
+            }
+        }
+    }
+    if constexpr (EN) {
+        __shared__ double red[16];
+        double s = (double)en;
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) red[w] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0;
+            for (int q = 0; q < g.WX * g.RY; q++) t += red[q];
+            a.en_part[blockIdx.x] = t;
+        }
+    }
+}
+
+// H of one cell from the fields in memory (the edge cells of k_step, after
+// it has written every E): the arithmetic of h_tile.
+template <typename S, typename T>
+__device__ __forceinline__ void h_point(const HArgs<S, T> &h, int i, int j, int k) {
+    const int NxNy = h.P * h.Ny;
+    const int id = i + h.P * j + k * NxNy;
+    const int oip = (i == h.Nx - 1) ? -(h.Nx - 1) : 1;
+    const int ojp = (j == h.Ny - 1) ? -(h.Ny - 1) * h.P : h.P;
+    const int idkp = i + h.P * j + ((k == h.Nz - 1) ? 0 : k + 1) * NxNy;
+    const T ex = h.Ex[id], ey = h.Ey[id], ez = h.Ez[id];
+    const T ex_kp = h.Ex[idkp], ey_kp = h.Ey[idkp];
+    const T ex_jp = h.Ex[id + ojp], ez_jp = h.Ez[id + ojp];
+    const T ey_ip = h.Ey[id + oip], ez_ip = h.Ez[id + oip];
+    T hx = h.Hx[id], hy = h.Hy[id], hz = h.Hz[id];
+    const int nx_ = slabH(i, h.Nx, h.pml[0], h.pml[1]), ny_ = slabH(j, h.Ny, h.pml[2], h.pml[3]);
+    h_cell_vals(h, i, j, k, hx, hy, hz, ey_kp - ey, ex_kp - ex, ez_jp - ez, ez_ip - ez, ex_jp - ex, ey_ip - ey,
+                h.pf.ikH[0][i], h.pf.ikH[1][j], h.pf.ikH[2][k], nx_, ny_, side_all(h), GlobalPsiH<S, T>{h, i, j, k});
+    h.Hx.set(id, hx);
+    h.Hy.set(id, hy);
+    h.Hz.set(id, hz);
+    for (int q = 0; q < h.nstrip; q++) {
+        const int x0 = h.scol[q], x1 = (x0 + 1 == h.Nx) ? 0 : x0 + 1;
+        const int s = (i == x0) ? 2 * q : (i == x1 ? 2 * q + 1 : -1);
+        if (s >= 0) {
+            const T v[6] = {ex, ey, ez, stored(h.Hx, hx), stored(h.Hy, hy), stored(h.Hz, hz)};
+            strip_put(h, s, j, k, v);
+        }
+    }
+}
+
+// The edge cells of k_step: row R of the edge set (EdgeGeom), 64 cells of it
+// per wavefront (blockIdx.y: segment of the row); workgroups from nb on
+// update the H of the TFSF column `aux` (its E update ran before k_step).
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_step_edge(const HArgs<S, T> h, const EdgeGeom e, long nb,
+                                                   const HArgs<S, T> aux) {
+    if ((long)blockIdx.x >= nb) {
+        const int k = (int)(((long)blockIdx.x - nb) * 256 + threadIdx.x + 64 * threadIdx.y);
+        if (blockIdx.y == 0 && k < aux.Nz) h_col_cell(aux, k);
+        return;
+    }
+    const long R = (long)blockIdx.x * 4 + threadIdx.y;
+    const int i = (int)blockIdx.y * 64 + threadIdx.x;
+    if (R >= e.nrows || i >= h.Nx) return;
+    int j, k;
+    const long nz_rows = (long)e.ntz * h.Ny;
+    if (R < nz_rows) {
+        const int q = (int)(R / h.Ny);
+        j = (int)(R % h.Ny);
+        k = min((q + 1) * e.KZ, h.Nz) - 1;
+    } else {
+        const long Rp = R - nz_rows;
+        const int nzi = h.Nz - e.ntz;                 // planes that are not the last of a z tile
+        const int p = (int)(Rp / nzi), m = (int)(Rp % nzi);
+        j = min((p + 1) * e.TY, h.Ny) - 1;
+        k = (m / (e.KZ - 1)) * e.KZ + m % (e.KZ - 1);
+    }
+    h_point(h, i, j, k);
+}
+
 // Streaming copy used to measure the HBM bandwidth a kernel can reach on
 // this device (16-byte accesses, grid-stride).
 typedef float f32x4 __attribute__((ext_vector_type(4)));

@@ -485,16 +485,13 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
         int rc;
         if (r->kind == 1) {
             // single_particle.cpp:256-381: aux E, main E (+TFSF-E), aux H, main H (+TFSF-H), sensors;
-            // each aux column update rides in the main grid's launch of the same half step
-            if ((rc = aefdtd::update_E_pair(r->g, r->a)) || (rc = aefdtd::update_H_pair(r->g, r->a)) ||
-                (rc = aefdtd_feed_sensors(r->g)))
-                return rfail(r, rc);
+            // one step of the grid and its column (the one-pass kernel on 3D grids)
+            if ((rc = aefdtd::step_pair(r->g, r->a)) || (rc = aefdtd_feed_sensors(r->g))) return rfail(r, rc);
         } else {
             // normal_incidence.cpp:325-415: E (+injection), H (+injection), sensors + R/T sums;
             // default_fdtd.cpp:148-165: E (+sources' inject_E), H, sensors (inject_H of an
             // oscillator only counts its step, which update_H does)
-            if ((rc = aefdtd_update_E(r->g)) || (rc = aefdtd_update_H(r->g)) || (rc = aefdtd_feed_sensors(r->g)))
-                return rfail(r, rc);
+            if ((rc = aefdtd::step_pair(r->g, nullptr)) || (rc = aefdtd_feed_sensors(r->g))) return rfail(r, rc);
         }
         if (r->time_type != 0) {
             // the CompletionSensor is fed with the other sensors; the drivers check it

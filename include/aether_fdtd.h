@@ -124,6 +124,15 @@ int aefdtd_bootstrap(aefdtd_grid *g);
  * where Aether's drivers perform them. update_H advances tstep. */
 int aefdtd_update_E(aefdtd_grid *g);
 int aefdtd_update_H(aefdtd_grid *g);
+/* n full steps: update_E then update_H, as the drivers' loops call them
+ * (normal_incidence.cpp:330-357, default_fdtd.cpp:148-160), with the
+ * registered per-step actions.  On a 3D grid up to 1024 cells wide (and not
+ * fed by a TFSF column, which the drivers step with it) one step is one pass
+ * over the grid that updates E and then H of the planes behind it, with the
+ * same per-cell arithmetic as the two calls (bit-identical results); other
+ * grids make the two calls.  No counterpart in the reference, whose callers
+ * always make the two calls. */
+int aefdtd_step(aefdtd_grid *g, int n);
 /* FDTD::reset_fields -- core/fdtd_utils.cpp:855-881. */
 int aefdtd_reset_fields(aefdtd_grid *g);
 
@@ -496,12 +505,17 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
  * 4 rc; default all): fewer events keep a timed region closer to unprofiled. */
 int aefdtd_profile_kinds(aefdtd_grid *g, int mask);
 /* Tile autotuner: times every compiled tile shape of the E and H passes on
- * this grid (reps launches each), keeps the fastest and restores the grid's
- * state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
+ * this grid (reps launches each), then a few tiles of the one-pass step
+ * against the best pair, keeps the fastest and restores the grid's state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
  * XCD-remap<<4; BY in 1,2,4,8 rows per workgroup, KZ in 1,2,4,8 planes per
  * lane, or rows per lane on grids one cell thick in z).  No counterpart in the reference (a CPU thread-count choice). */
 int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
+/* Tile of the one-pass step (aefdtd_step and the drivers): code = TY index
+ * (4, 8, 16, 32 rows) | (lane rows - 1) << 2 (1..4 rows in flight) | KZ index
+ * (4, 8, 16, 32 planes) << 4 | XCD fold << 6; -1 makes every step the two
+ * passes.  No counterpart in the reference. */
+int aefdtd_set_step_tiles(aefdtd_grid *g, int code);
 /* Power-of-two storage scales of the six field components (stored value =
  * field * 2^exps[c]); FP16 storage only (defaults E: 2^8, H: 2^16, so that
  * unit-amplitude plane waves sit near the middle of the FP16 range; the custom
