@@ -121,6 +121,7 @@ def _sig(lib, p):
         "add_spect_poynting": (i, [vp, i, i, i, i, i, i, i, i, _dp, _ip]),
         "feed_sensors": (i, [vp]),
         "sensor_poynting": (i, [vp, i, _dp]),
+        "sensor_poynting_noalloc": (i, [vp, i, _dp]),
         "sensor_spectra": (i, [vp, i, _dp, _ip]),
         "add_plane_sum": (i, [vp, i, C.POINTER(CPulse), d, i, _dp, _ip]),
         "read_plane_sum": (i, [vp, i, i, _dp]),

@@ -1673,6 +1673,7 @@ int aefdtd_create(aefdtd_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, 
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return cfail(AEFDTD_EDEVICE, std::string("engine is built for gfx950, device is ") + prop.gcnArchName);
     auto *g = new aefdtd_grid();
+    g->device = dev;
     g->precision = precision;
     if (precision == AEFDTD_FP16)
         for (int c = 0; c < 6; c++) g->scale_exp[c] = c < 3 ? 8 : 16;
@@ -2268,6 +2269,78 @@ int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp) {
     void *v = d;
     g->dev_bytes -= s.Nl * sizeof(double);
     dfree(v);
+    if (e != hipSuccess) return fail(g, AEFDTD_EDEVICE, std::string("poynting: ") + hipGetErrorString(e));
+    return AEFDTD_OK;
+}
+
+}  // extern "C"
+
+// ---- the sensor power without allocations
+// k_poynting writes one value per wavelength to a staging buffer before the
+// copy to the caller's host array.  The allocating entry point takes that
+// buffer from hipMalloc on every call; the one below uses `scratch`, a
+// file-scope device array of AEFDTD_NOALLOC_MAX_NL doubles.  Each GPU holds
+// its own copy at its own address, found once per device with
+// hipGetSymbolAddress; host code uses only that address.  Every call on a GPU
+// shares that GPU's array (k_poynting overwrites the Nl values it reads back,
+// so nothing needs clearing): calls that overlap on one GPU race.
+namespace {
+
+static __device__ __align__(256) double scratch[AEFDTD_NOALLOC_MAX_NL];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (dev), or null.
+double *scratch_on(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        double *p = nullptr;
+        if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int aefdtd_sensor_poynting_noalloc(aefdtd_grid *g, int id, double *rsp) {
+    if (!g || !rsp) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->sensors.size()) return fail(g, AEFDTD_EINVAL, "bad sensor id");
+    Sensor &s = *g->sensors[id];
+    if (s.Nl > AEFDTD_NOALLOC_MAX_NL)
+        return fail(g, AEFDTD_EINVAL, "sensor has " + std::to_string(s.Nl) + " wavelengths; the allocation-free "
+                                      "entry point takes at most " + std::to_string(AEFDTD_NOALLOC_MAX_NL));
+    // the pending DFT flush must not need a larger coefficient table (that would allocate)
+    if (g->stage_fill > 0)
+        for (const Sensor *q : g->sensors)
+            if (g->sensor_step > q->ncoef) return fail(g, AEFDTD_ESTATE, "the DFT flush would allocate coefficients");
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(g, AEFDTD_EDEVICE, "hipGetDevice failed");
+    if (cur != g->device && hipSetDevice(g->device) != hipSuccess) return fail(g, AEFDTD_EDEVICE, "hipSetDevice failed");
+    double *d = scratch_on(g->device);
+    int rc = d ? with_ctype(g, [&]<typename T>() { return flush_dft<T>(g); }) : AEFDTD_EDEVICE;
+    hipError_t e = hipSuccess;
+    if (rc == AEFDTD_OK) {
+        double f;
+        if (s.type == AEFDTD_NORMAL_X || s.type == AEFDTD_NORMAL_XM) f = g->Dy * g->Dz;
+        else if (s.type == AEFDTD_NORMAL_Y || s.type == AEFDTD_NORMAL_YM) f = g->Dx * g->Dz;
+        else f = g->Dx * g->Dy;
+        hipLaunchKernelGGL(k_poynting, dim3(s.Nl), dim3(256), 0, g->stream, (const double2 *)s.d_acc, s.P, s.Nl, f,
+                           s.type, d);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = copy_sync(g, rsp, d, s.Nl * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    if (cur != g->device) (void)hipSetDevice(cur);
+    if (!d) return fail(g, AEFDTD_EDEVICE, "hipGetSymbolAddress of the scratch array failed");
+    if (rc) return rc;
     if (e != hipSuccess) return fail(g, AEFDTD_EDEVICE, std::string("poynting: ") + hipGetErrorString(e));
     return AEFDTD_OK;
 }

@@ -149,6 +149,7 @@ struct aefdtd_grid {
     std::vector<double> kE[3], kH[3], bE[3], bH[3], cE[3], cH[3];
 
     // ---- device state
+    int device = 0;                   // the HIP device current at creation
     hipStream_t stream = nullptr;
     bool own_stream = true;
     void *F[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // T* fields

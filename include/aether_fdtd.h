@@ -183,6 +183,17 @@ int aefdtd_add_spect_poynting(aefdtd_grid *g, int orientation,
 int aefdtd_feed_sensors(aefdtd_grid *g);
 /* Spect_Poynting::treat -- rsp_result[l] for l < Nl (s_spec_power.cpp:44-119).*/
 int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp);
+/* aefdtd_sensor_poynting without device allocations.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-wavelength staging buffer is a static device array of
+ * AEFDTD_NOALLOC_MAX_NL doubles per GPU instead of a hipMalloc per call; a
+ * sensor with more wavelengths is refused (AEFDTD_EINVAL), and so is a call
+ * whose pending DFT flush would need a larger coefficient table (AEFDTD_ESTATE:
+ * stepping a grid past the Nt it was created with).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU race (serialise them). */
+#define AEFDTD_NOALLOC_MAX_NL 4096
+int aefdtd_sensor_poynting_noalloc(aefdtd_grid *g, int id, double *rsp);
 /* Raw DFT accumulators sp_Ex..sp_Hz of one sensor: out[c][l][j][i] complex
  * as (re,im), c in Ex,Ey,Ez,Hx,Hy,Hz; span1 x span2 written to dims[0..1]. */
 int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]);
