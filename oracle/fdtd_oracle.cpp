@@ -41,6 +41,14 @@
 // The caller's end_computation flag, polled once per step by the drivers.
 static bool cancelled(const int *flag) { return flag && __atomic_load_n(flag, __ATOMIC_RELAXED); }
 
+// Planted mistakes (aeoracle_set_slip), so that the tests can show that each
+// pin of the oracle fails when the function it pins is wrong: 0 = none,
+// 1 = TFSF-E z faces read the auxiliary H one plane off, 2 = TFSF-H lower y
+// face with the sign of the Hz correction flipped, 3 = no PEC wall for Ex on y,
+// 4 = the far-side x extension copies the second-last structure plane,
+// 5 = normal incidence without the -Dz/2 phase calibration.
+static int g_slip = 0;
+
 namespace {
 
 using cplx = std::complex<double>;
@@ -511,7 +519,8 @@ void aeoracle_grid::app_pml_Ex(int i1, int i2) {
                 }
             }
         for (int i = i1; i < i2; i++)
-            for (int k = 0; k < Nz; k++) Ex(i, 0, k) = 0;
+            for (int k = 0; k < Nz; k++)
+                if (g_slip != 3) Ex(i, 0, k) = 0;
     }
     if (pml[4] || pml[5]) {
         auto P = [&](int i, int j, int n) -> double & { return PsiEx[1][(size_t)i + (size_t)psz[0] * ((size_t)j + (size_t)psz[1] * n)]; };
@@ -768,7 +777,7 @@ void aeoracle_grid::tfsf_E() {
     int i1 = tb[0], i2 = tb[1], j1 = tb[2], j2 = tb[3], k1 = tb[4], k2 = tb[5];
     for (int i = i1; i < i2; i++)
         for (int j = j1; j < j2; j++) {
-            Ex(i, j, k1) += mats[mg(i, j, k1)].C2z * aH(1, k1 - 1);
+            Ex(i, j, k1) += mats[mg(i, j, k1)].C2z * aH(1, g_slip == 1 ? k1 : k1 - 1);
             Ex(i, j, k2) -= mats[mg(i, j, k2)].C2z * aH(1, k2 - 1);
             Ey(i, j, k1) -= mats[mg(i, j, k1)].C2z * aH(0, k1 - 1);
             Ey(i, j, k2) += mats[mg(i, j, k2)].C2z * aH(0, k2 - 1);
@@ -808,7 +817,7 @@ void aeoracle_grid::tfsf_H() {
             for (int k = k1; k < k2; k++) {
                 Hx(i, j1 - 1, k) += dtdmy * aE(2, k);
                 Hx(i, j2 - 1, k) -= dtdmy * aE(2, k);
-                Hz(i, j1 - 1, k) -= dtdmy * aE(0, k);
+                Hz(i, j1 - 1, k) -= (g_slip == 2 ? -dtdmy : dtdmy) * aE(0, k);
                 Hz(i, j2 - 1, k) += dtdmy * aE(0, k);
             }
     if (!tf_per_x)
@@ -1056,6 +1065,14 @@ int threads_now() {
 extern "C" {
 
 void aeoracle_set_threads(int n) { g_threads = n; }
+void aeoracle_set_slip(int code) { g_slip = code; }
+
+// The full-grid material indices (x fastest), after the extension of set_matsgrid.
+int aeoracle_read_matsgrid(const aeoracle_grid *g, uint32_t *out) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    std::copy(g->matsgrid.begin(), g->matsgrid.end(), out);
+    return AEFDTD_OK;
+}
 int aeoracle_get_threads(void) { return threads_now(); }
 
 int aeoracle_create(aeoracle_grid **out, int Nx, int Ny, int Nz, int Nt, double Dx, double Dy,
@@ -1146,7 +1163,7 @@ int aeoracle_set_matsgrid(aeoracle_grid *g, const uint32_t *ids, int nx, int ny,
             for (int k = G.zs_s; k < G.zs_e; k++) M(i, j, k) = M(G.xs_s, j, k);
     for (int i = G.xs_e; i < G.Nx; i++)
         for (int j = G.ys_s; j < G.ys_e; j++)
-            for (int k = G.zs_s; k < G.zs_e; k++) M(i, j, k) = M(G.xs_e - 1, j, k);
+            for (int k = G.zs_s; k < G.zs_e; k++) M(i, j, k) = M(g_slip == 4 ? G.xs_e - 2 : G.xs_e - 1, j, k);
     for (int i = 0; i < G.Nx; i++)
         for (int j = 0; j < G.ys_s; j++)
             for (int k = G.zs_s; k < G.zs_e; k++) M(i, j, k) = M(i, G.ys_s, k);
@@ -1793,8 +1810,10 @@ int aeoracle_run_normal_incidence(const aefdtd_mode *m, int n_sensors, const aef
         Nsup++; k--;
     }
     double hsup = (Nsup - 1) * Dz, hsub = Nsub * Dz, hstruc = (g->Nz_s - Nsup - Nsub) * Dz;
-    hsup -= Dz / 2.0;
-    hsub -= Dz / 2.0;
+    if (g_slip != 5) {
+        hsup -= Dz / 2.0;
+        hsub -= Dz / 2.0;
+    }
     // post-run DFT (normal_incidence.cpp:455-488); Nt is the number of steps run
     int NtE = Nrun;
     for (int l = 0; l < Nl; l++) {

@@ -25,6 +25,11 @@ typedef struct aeoracle_grid aeoracle_grid;
 
 void aeoracle_set_threads(int n);
 int  aeoracle_get_threads(void);
+/* Plants one known mistake in the oracle (code as in fdtd_oracle.cpp; 0 =
+ * none), so that a test can show its pin fails on it. */
+void aeoracle_set_slip(int code);
+/* The full-grid material indices after set_matsgrid's extension (x fastest). */
+int aeoracle_read_matsgrid(const aeoracle_grid *g, uint32_t *out);
 
 int aeoracle_create(aeoracle_grid **out, int Nx, int Ny, int Nz, int Nt,
                     double Dx, double Dy, double Dz, double Dt, int mode,

@@ -421,3 +421,150 @@ def test_oscillator_orientation_and_step_origin(oracle_be):
     assert ex[0, 0, 0] == ref[0] and ex[0, 0, 1] == 0 and ex[0, 1, 0] == 0
     assert ey[0, 0, 1] == ref[0] and ey[0, 1, 1] == ref[0] and ey[0, 0, 0] == 0
     assert not np.any(ez)
+
+
+# ---------------------------------------------------------------- pins by mathematics
+# Each test below also plants a known mistake in the oracle (aeoracle_set_slip)
+# and shows that the same check then fails.
+
+def _slip(code):
+    import oracle
+    C.CDLL(oracle.ORACLE_SO).aeoracle_set_slip(code)
+
+
+def _tfsf_leak(oracle_be, eps, pol, slip=0):
+    """Single-particle driver on a uniform medium (no scatterer): the largest
+    field at least two cells outside the TFSF box over the largest inside it."""
+    n, pml = 16, 5
+    spec, sens, _ = workloads.spheres_single_particle(n=n, D=2e-9, count=0, rmin=3e-9, rmax=6e-9, seed=5, Nt=300,
+                                                      Nl=6, pml=pml, eps_bg=eps, precision=capi.FP64)
+    spec.materials = [Material(eps)]
+    spec.polarization = pol
+    _slip(slip)
+    try:
+        F = capi.run_single_particle(oracle_be, spec, sens, want_fields=True)["fields"]
+    finally:
+        _slip(0)
+    lo, hi = pml + 10, pml + 10 + n                    # the TFSF box [lo, hi) on every axis (10-cell pads)
+    inside = np.abs(F[:, lo:hi, lo:hi, lo:hi]).max()
+    far = np.ones(F.shape[1:], bool)
+    far[lo - 2:hi + 2, lo - 2:hi + 2, lo - 2:hi + 2] = False
+    return np.abs(F[:, far]).max() / inside
+
+
+@pytest.mark.parametrize("eps", [1.0, 1.77])
+@pytest.mark.parametrize("pol", [0, 1], ids=["y_polarised", "x_polarised"])
+def test_tfsf_leaves_no_field_outside_the_box(oracle_be, eps, pol):
+    """In a uniform medium the TFSF corrections (single_particle.cpp:277-378) must
+    cancel the incident wave exactly outside the box: the scattered field there is
+    round-off.  Fails for a face correction read one plane off (slip 1) or with a
+    flipped sign (slip 2)."""
+    assert _tfsf_leak(oracle_be, eps, pol) < 1e-12
+    if pol == 1:
+        assert _tfsf_leak(oracle_be, eps, pol, slip=1) > 1e-3
+        assert _tfsf_leak(oracle_be, eps, pol, slip=2) > 1e-3
+
+
+def _pec_pair(oracle_be, slip=0, steps=10):
+    """A grid with transparent CPML on all six faces (kappa 1, sigma 0: psi stays
+    exactly 0) against the same grid without CPML whose PEC walls are zeroed by
+    hand: tangential E at index 0 and tangential H at index N-1 of every axis
+    with CPML (fdtd_pml.cpp:504-929)."""
+    n, pml = (7, 6, 8), (3, 2, 4, 2, 3, 5)
+    D = (5e-9,) * 3
+    Dt = min(D) / (np.sqrt(3.0) * 299792458.0) * 0.99
+    rng = np.random.default_rng(7)
+    mg = rng.integers(0, 3, size=(n[2], n[1], n[0])).astype(np.uint32)
+    mats = [Material(1.0), Material(2.3), workloads.GOLD]
+    a = capi.Grid(oracle_be, n, steps, D, Dt, capi.MODE_CUSTOM, pml, (0,) * 6, capi.FP64)
+    for f in range(6):
+        a.set_pml(f, 1.0, 0.0, 0.2)
+    pad_b = tuple(pml[f] + max(5, 0) for f in range(6))          # the same full grid, no CPML
+    b = capi.Grid(oracle_be, n, steps, D, Dt, capi.MODE_CUSTOM, (0,) * 6, pad_b, capi.FP64)
+    assert (a.Nx, a.Ny, a.Nz) == (b.Nx, b.Ny, b.Nz)
+    for g in (a, b):
+        g.set_matsgrid(mg)
+        for q, m in enumerate(mats):
+            g.set_material(q, m)
+        g.bootstrap()
+    for f in range(6):
+        v = rng.standard_normal((a.Nz, a.Ny, a.Nx)) * (1.0 if f < 3 else 1 / 376.73)
+        a.write(f, v)
+        b.write(f, v)
+    # walls: (field, axis) of tangential E at index 0 and tangential H at N-1
+    e_walls = [(1, 0), (2, 0), (0, 1), (2, 1), (0, 2), (1, 2)]
+    h_walls = [(4, 0), (5, 0), (3, 1), (5, 1), (3, 2), (4, 2)]
+
+    def zero(g, walls, at_end):
+        for f, ax in walls:
+            v = g.read(f)
+            idx = [slice(None)] * 3
+            idx[2 - ax] = -1 if at_end else 0                    # arrays are (z, y, x)
+            v[tuple(idx)] = 0.0
+            g.write(f, v)
+
+    _slip(slip)
+    try:
+        for _ in range(steps):
+            a.update_E(); a.update_H()
+            b.update_E(); zero(b, e_walls, False)
+            b.update_H(); zero(b, h_walls, True)
+    finally:
+        _slip(0)
+    return max(np.max(np.abs(a.read(f) - b.read(f))) for f in range(6))
+
+
+def test_pec_walls_of_transparent_cpml(oracle_be):
+    assert _pec_pair(oracle_be) == 0.0
+    assert _pec_pair(oracle_be, slip=3) > 1e-3
+
+
+def test_matsgrid_extension_is_edge_replication(oracle_be):
+    """set_matsgrid (fdtd_utils.cpp:572-796, non-oblique modes): every pad and
+    CPML cell takes the material of the nearest structure cell, i.e. the
+    structure padded by edge replication on every axis.  Fails for a far-side
+    extension that copies the wrong plane (slip 4)."""
+    import oracle
+    n, pml, pad = (5, 4, 6), (3, 0, 2, 4, 1, 3), (0, 2, 1, 0, 4, 0)
+    D = (5e-9,) * 3
+    rng = np.random.default_rng(3)
+    mg = rng.integers(0, 9, size=(n[2], n[1], n[0])).astype(np.uint32)
+    lib = C.CDLL(oracle.ORACLE_SO)
+    lib.aeoracle_read_matsgrid.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    for slip in (0, 4):
+        g = capi.Grid(oracle_be, n, 5, D, 1e-17, capi.MODE_CUSTOM, pml, pad, capi.FP64)
+        _slip(slip)
+        try:
+            g.set_matsgrid(mg)
+        finally:
+            _slip(0)
+        out = np.zeros((g.Nz, g.Ny, g.Nx), dtype=np.uint32)
+        lib.aeoracle_read_matsgrid(g.h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        # FDTD::FDTD pads PML faces to at least 5 cells (fdtd_core.cpp:212-243)
+        p = [max(5, pad[f]) if (pml[2 * (f // 2)] or pml[2 * (f // 2) + 1]) else pad[f] for f in range(6)]
+        lo = [pml[2 * a] + p[2 * a] for a in range(3)]
+        hi = [pml[2 * a + 1] + p[2 * a + 1] for a in range(3)]
+        want = np.pad(mg, ((lo[2], hi[2]), (lo[1], hi[1]), (lo[0], hi[0])), mode="edge")
+        assert out.shape == want.shape
+        assert np.array_equal(out, want) == (slip == 0), slip
+        g.close()
+
+
+def test_slab_complex_amplitudes_match_airy_phases(oracle_be):
+    """spectdata_norm2's complex r and t (normal_incidence.cpp:448-570): with the
+    fields referred to the slab faces (the -Dz/2 calibration) their phases are
+    Airy's to within the discretisation error (r: 7e-3 rad, t: 2e-3 rad over
+    450-700 nm at 5 nm cells); without the calibration r is off by k Dz (slip 5)."""
+    spec = workloads.slab_normal_incidence(n=(4, 4, 64))
+    for slip, ok in ((0, True), (5, False)):
+        _slip(slip)
+        try:
+            res = capi.run_normal_incidence(oracle_be, spec)
+        finally:
+            _slip(0)
+        r, t = airy(res["lambda"], 1.5, 20 * 5e-9)
+        band = (res["lambda"] > 450e-9) & (res["lambda"] < 700e-9)
+        dr = np.max(np.abs(np.angle(res["refx"][band] / r[band])))
+        dt = np.max(np.abs(np.angle(res["transx"][band] / t[band])))
+        assert dt < 1e-2
+        assert (dr < 2e-2) == ok, (slip, dr)
