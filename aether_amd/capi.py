@@ -22,6 +22,7 @@ OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED = 0, 1, 2, 3, 4, 5
 MODE_NORM, MODE_CUSTOM = 0, 5
 FP64, FP32, FP16 = 0, 1, 2
 NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM, NORMAL_UNDEF = range(7)
+SENSOR_POYNTING, SENSOR_FIELDBLOCK, SENSOR_FIELDMAP, SENSOR_FIELDMAP2 = range(4)
 KIND_NORMAL, KIND_SINGLE, KIND_CUSTOM = 0, 1, 2
 EX, EY, EZ, HX, HY, HZ = range(6)
 
@@ -72,7 +73,8 @@ class CSensorReq(C.Structure):
     _fields_ = [("box", C.c_int), ("orientation", C.c_int),
                 ("x1", C.c_int), ("x2", C.c_int), ("y1", C.c_int), ("y2", C.c_int),
                 ("z1", C.c_int), ("z2", C.c_int), ("disable", C.c_int * 6),
-                ("Nl", C.c_int), ("lambda_min", C.c_double), ("lambda_max", C.c_double)]
+                ("Nl", C.c_int), ("lambda_min", C.c_double), ("lambda_max", C.c_double),
+                ("kind", C.c_int)]
 
 
 class CNormalResult(C.Structure):
@@ -120,6 +122,10 @@ def _sig(lib, p):
         "set_tfsf": (i, [vp, vp, i, i, i, i, i, i, i, i]),
         "add_spect_poynting": (i, [vp, i, i, i, i, i, i, i, i, _dp, _ip]),
         "feed_sensors": (i, [vp]),
+        "add_field_block": (i, [vp, i, i, i, i, i, i, d, _ip]),
+        "add_field_map": (i, [vp, i, i, i, i, i, i, i, d, i, i, _ip]),
+        "field_sensor_read": (i, [vp, i, _dp, _ip]),
+        "field_sensor_mats": (i, [vp, i, C.POINTER(C.c_uint32)]),
         "sensor_poynting": (i, [vp, i, _dp]),
         "sensor_poynting_noalloc": (i, [vp, i, _dp]),
         "sensor_spectra": (i, [vp, i, _dp, _ip]),
@@ -275,6 +281,7 @@ class SensorReq:
     Nl: int = 64
     lambda_min: float = 400e-9
     lambda_max: float = 800e-9
+    kind: int = SENSOR_POYNTING
 
     def to_c(self):
         s = CSensorReq()
@@ -284,9 +291,25 @@ class SensorReq:
         for f in range(6):
             s.disable[f] = int(self.disable[f])
         s.Nl, s.lambda_min, s.lambda_max = self.Nl, self.lambda_min, self.lambda_max
+        s.kind = self.kind
         return s
 
+    def spans(self):
+        """In-plane spans of a field map (FieldMap::initialize, s_fieldmap.cpp:433-447)."""
+        d = (max(0, self.x2 - self.x1), max(0, self.y2 - self.y1), max(0, self.z2 - self.z1))
+        n = self.orientation // 2
+        return {0: (d[1], d[2]), 1: (d[0], d[2])}.get(n, (d[0], d[1]))
+
     def out_len(self):
+        """Doubles this request adds to a driver's sensor_out."""
+        if self.kind == SENSOR_FIELDBLOCK:
+            return 12 * max(0, self.x2 - self.x1) * max(0, self.y2 - self.y1) * max(0, self.z2 - self.z1)
+        if self.kind == SENSOR_FIELDMAP:
+            s1, s2 = self.spans()
+            return 6 * s1 * s2
+        if self.kind == SENSOR_FIELDMAP2:
+            s1, s2 = self.spans()
+            return 12 * self.Nl * s1 * s2
         return 7 * self.Nl if self.box else self.Nl
 
 
@@ -960,6 +983,34 @@ class Grid:
 
     def feed_sensors(self):
         self._c(self.be.fn("feed_sensors")(self.h))
+
+    def add_field_block(self, box, wavelength):
+        i = C.c_int()
+        self._c(self.be.fn("add_field_block")(self.h, *box, wavelength, C.byref(i)))
+        return i.value
+
+    def add_field_map(self, orientation, box, wavelength, cumulative=False, magnetic=False):
+        i = C.c_int()
+        self._c(self.be.fn("add_field_map")(self.h, orientation, *box, wavelength, int(cumulative),
+                                            int(magnetic), C.byref(i)))
+        return i.value
+
+    def field_sensor(self, fid):
+        """Accumulators of a field sensor as complex (ncomp, s3, s2, s1)."""
+        dims = (C.c_int * 4)()
+        self._c(self.be.fn("field_sensor_read")(self.h, fid, None, dims))
+        nc, s1, s2, s3 = dims
+        out = np.zeros(nc * s1 * s2 * s3 * 2)
+        self._c(self.be.fn("field_sensor_read")(self.h, fid, _dptr(out), dims))
+        return (out[0::2] + 1j * out[1::2]).reshape(nc, s3, s2, s1)
+
+    def field_sensor_mats(self, fid):
+        dims = (C.c_int * 4)()
+        self._c(self.be.fn("field_sensor_read")(self.h, fid, None, dims))
+        _, s1, s2, s3 = dims
+        out = np.zeros(s1 * s2 * s3, dtype=np.uint32)
+        self._c(self.be.fn("field_sensor_mats")(self.h, fid, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out.reshape(s3, s2, s1)
 
     def sensor_poynting(self, sid, Nl):
         out = np.zeros(Nl)

@@ -22,6 +22,16 @@ struct Sensor {                   // Spect_Poynting with interpolated samples
     int ncoef = 0;
 };
 
+// FieldBlock / FieldMap (sensors/s_fieldmap.cpp:89-230, 321-470): the device
+// descriptor, the wavelength and the sensor's own feed count (Sensor::step).
+struct FieldSensor {
+    dev::FieldDesc d;
+    double lambda = 0;
+    int step = 0;
+    int ncomp = 0;
+    int type = 0;                 // orientation of a FieldMap (folded to X/Y/Z)
+};
+
 struct PlaneSum {
     int k;
     bool has_inc, subtract;
@@ -1052,6 +1062,20 @@ int feed_T(aefdtd_grid *g) {
             if ((rc = flush_dft<T>(g))) return rc;
         }
     }
+    // single-wavelength field sensors: one accumulation per feed
+    // (FieldBlock::threaded_computation, FieldMap::FT_compute)
+    for (FieldSensor *s : g->fsensors) {
+        const double w = 2.0 * kPi * kC / s->lambda;
+        const std::complex<double> I(0.0, 1.0);
+        const std::complex<double> cE = std::exp(w * s->step * g->Dt * I);
+        const std::complex<double> cH = s->d.block ? std::exp(w * (s->step + 0.5) * g->Dt * I) : cE;
+        hipLaunchKernelGGL((k_field_feed<S, T>), dim3((unsigned)((s->d.n + 255) / 256)), dim3(256), 0, g->stream, s->d,
+                           fld<S, T>(g, 0), fld<S, T>(g, 1), fld<S, T>(g, 2), fld<S, T>(g, 3), fld<S, T>(g, 4),
+                           fld<S, T>(g, 5), g->Nx, g->Ny, g->Nz, g->px, make_double2(cE.real(), cE.imag()),
+                           make_double2(cH.real(), cH.imag()));
+        HIPOK(g, hipGetLastError());
+        s->step++;
+    }
     const int t = g->tstep - 1;
     if (t >= 0 && t < g->Nt) {
         // two sums per launch (blockIdx.y)
@@ -1136,8 +1160,9 @@ constexpr int tb_ly(int lyi) {
 constexpr int kTS[3] = {2, 4, 8};
 
 bool tb_code_valid(int precision, int code) {
-    if (code < 0) return true;
-    const int lyi = code & 3, tsi = (code >> 2) & 3;
+    if (code == -1) return true;
+    if (code < 0 || code > 11) return false;
+    const int lyi = code & 3, tsi = code >> 2;
     if (lyi > 1 || tsi > 2) return false;
     const int ly = precision == AEFDTD_FP64 ? (lyi ? 32 : 16) : (lyi ? 48 : 32);
     return ly - 2 * kTS[tsi] - 1 >= 4;
@@ -1761,6 +1786,10 @@ int aefdtd_destroy(aefdtd_grid *g) {
         dfree_t(s->d_coef);
         delete s;
     }
+    for (FieldSensor *s : g->fsensors) {
+        dfree_t(s->d.acc);
+        delete s;
+    }
     for (PlaneSum *s : g->sums) {
         dfree_t(s->d_tap);
         dfree_t(s->d_out);
@@ -2142,6 +2171,93 @@ int aefdtd_add_spect_poynting(aefdtd_grid *g, int type, int x1, int x2, int y1, 
     return AEFDTD_OK;
 }
 
+namespace {
+
+// Sensor::link's offset and clamp (sensors.cpp:196-230) of a region given in
+// structure cells; false when it is empty.
+bool link_region(const aefdtd_grid *g, int &x1, int &x2, int &y1, int &y2, int &z1, int &z2) {
+    x1 = std::clamp(x1 + g->xs_s, 0, g->Nx); x2 = std::clamp(x2 + g->xs_s, 0, g->Nx);
+    y1 = std::clamp(y1 + g->ys_s, 0, g->Ny); y2 = std::clamp(y2 + g->ys_s, 0, g->Ny);
+    z1 = std::clamp(z1 + g->zs_s, 0, g->Nz); z2 = std::clamp(z2 + g->zs_s, 0, g->Nz);
+    return x2 > x1 && y2 > y1 && z2 > z1;
+}
+
+int add_field_sensor(aefdtd_grid *g, FieldSensor *s, int *id) {
+    void *p;
+    const size_t bytes = (size_t)s->ncomp * s->d.n * sizeof(double2);
+    int rc = dalloc(g, &p, bytes);
+    if (rc) { delete s; return rc; }
+    s->d.acc = (double2 *)p;
+    HIPOK(g, hipMemsetAsync(s->d.acc, 0, bytes, g->stream));
+    g->fsensors.push_back(s);
+    *id = (int)g->fsensors.size() - 1;
+    return AEFDTD_OK;
+}
+
+}  // namespace
+
+int aefdtd_add_field_block(aefdtd_grid *g, int x1, int x2, int y1, int y2, int z1, int z2, double lambda, int *id) {
+    if (!g || !id) return AEFDTD_EINVAL;
+    if (!(lambda > 0)) return fail(g, AEFDTD_EINVAL, "bad wavelength");
+    if (!link_region(g, x1, x2, y1, y2, z1, z2)) return fail(g, AEFDTD_EINVAL, "empty sensor");
+    auto *s = new FieldSensor();
+    s->lambda = lambda;
+    s->ncomp = 6;
+    s->d = dev::FieldDesc{1, 0, 0, 0, x1, y1, z1, x2 - x1, y2 - y1, z2 - z1,
+                          (long)(x2 - x1) * (y2 - y1) * (z2 - z1), nullptr};
+    s->step = g->sensor_step;
+    return add_field_sensor(g, s, id);
+}
+
+int aefdtd_add_field_map(aefdtd_grid *g, int orientation, int x1, int x2, int y1, int y2, int z1, int z2,
+                         double lambda, int cumulative, int magnetic, int *id) {
+    if (!g || !id) return AEFDTD_EINVAL;
+    if (orientation < 0 || orientation > 5) return fail(g, AEFDTD_EINVAL, "bad orientation");
+    if (!(lambda > 0)) return fail(g, AEFDTD_EINVAL, "bad wavelength");
+    if (!link_region(g, x1, x2, y1, y2, z1, z2)) return fail(g, AEFDTD_EINVAL, "empty sensor");
+    // FieldMap::initialize folds the - orientations (s_fieldmap.cpp:433-440)
+    const int normal = orientation / 2;
+    auto *s = new FieldSensor();
+    s->lambda = lambda;
+    s->ncomp = 3;
+    s->type = 2 * normal;
+    const int ex[3] = {x2 - x1, y2 - y1, z2 - z1};
+    const int s1 = normal == 0 ? ex[1] : ex[0], s2 = normal == 2 ? ex[1] : ex[2];
+    s->d = dev::FieldDesc{0, normal, cumulative ? 1 : 0, magnetic ? 1 : 0, x1, y1, z1, s1, s2, ex[normal],
+                          (long)s1 * s2, nullptr};
+    s->step = g->sensor_step;
+    return add_field_sensor(g, s, id);
+}
+
+int aefdtd_field_sensor_read(aefdtd_grid *g, int id, double *out, int dims[4]) {
+    if (!g || !dims) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->fsensors.size()) return fail(g, AEFDTD_EINVAL, "bad field sensor id");
+    const FieldSensor &s = *g->fsensors[id];
+    dims[0] = s.ncomp;
+    dims[1] = s.d.s1;
+    dims[2] = s.d.s2;
+    dims[3] = s.d.block ? s.d.s3 : 1;
+    if (!out) return AEFDTD_OK;
+    HIPOK(g, copy_sync(g, out, s.d.acc, (size_t)s.ncomp * s.d.n * sizeof(double2), hipMemcpyDeviceToHost));
+    return AEFDTD_OK;
+}
+
+int aefdtd_field_sensor_mats(aefdtd_grid *g, int id, uint32_t *out) {
+    if (!g || !out) return AEFDTD_EINVAL;
+    if (id < 0 || id >= (int)g->fsensors.size()) return fail(g, AEFDTD_EINVAL, "bad field sensor id");
+    if (g->matsgrid.empty()) return fail(g, AEFDTD_ESTATE, "no material grid");
+    const dev::FieldDesc &d = g->fsensors[id]->d;
+    auto mg = [&](int i, int j, int k) { return g->matsgrid[(size_t)i + (size_t)g->Nx * ((size_t)j + (size_t)g->Ny * k)]; };
+    for (long q = 0; q < d.n; q++) {
+        const int a = (int)(q % d.s1), b = (int)((q / d.s1) % d.s2), c = d.block ? (int)(q / ((long)d.s1 * d.s2)) : 0;
+        if (d.block) out[q] = mg(d.x1 + a, d.y1 + b, d.z1 + c);
+        else if (d.normal == 0) out[q] = mg(d.x1, d.y1 + a, d.z1 + b);
+        else if (d.normal == 1) out[q] = mg(d.x1 + a, d.y1, d.z1 + b);
+        else out[q] = mg(d.x1 + a, d.y1 + b, d.z1);
+    }
+    return AEFDTD_OK;
+}
+
 int aefdtd_feed_sensors(aefdtd_grid *g) {
     if (!g) return AEFDTD_EINVAL;
     int rc = ensure_ready(g);
@@ -2245,6 +2361,23 @@ int aefdtd::sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2) {
     if (!g || id < 0 || id >= (int)g->sensors.size()) return AEFDTD_EINVAL;
     *span1 = g->sensors[id]->span1;
     *span2 = g->sensors[id]->span2;
+    return AEFDTD_OK;
+}
+
+int aefdtd::sensor_box(const aefdtd_grid *g, int id, int box[6]) {
+    if (!g || id < 0 || id >= (int)g->sensors.size()) return AEFDTD_EINVAL;
+    const Sensor &s = *g->sensors[id];
+    const int b[6] = {s.x1, s.x2, s.y1, s.y2, s.z1, s.z2};
+    for (int q = 0; q < 6; q++) box[q] = b[q];
+    return AEFDTD_OK;
+}
+
+int aefdtd::field_sensor_geom(const aefdtd_grid *g, int id, int out[8], double *lambda) {
+    if (!g || id < 0 || id >= (int)g->fsensors.size()) return AEFDTD_EINVAL;
+    const dev::FieldDesc &d = g->fsensors[id]->d;
+    const int v[8] = {d.block, d.normal, d.x1, d.y1, d.z1, d.s1, d.s2, d.s3};
+    for (int q = 0; q < 8; q++) out[q] = v[q];
+    if (lambda) *lambda = g->fsensors[id]->lambda;
     return AEFDTD_OK;
 }
 

@@ -100,6 +100,10 @@ int ft_place(const aefdtd_grid *g, double Dt, double lmin, double lmax, int Np, 
 bool ft_complete(const std::vector<double> &ring, int np, int navg, double coeff, int steps, int nmax);
 // Plane extent (cells) of sensor id after its link to the grid.
 int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
+// Linked (full-grid) location x1, x2, y1, y2, z1, z2 of sensor id.
+int sensor_box(const aefdtd_grid *g, int id, int box[6]);
+// Field sensor id: block, normal axis, x1, y1, z1 (full grid), s1, s2, s3, and its wavelength.
+int field_sensor_geom(const aefdtd_grid *g, int id, int out[8], double *lambda);
 // A grid and its attached TFSF column stepped with one launch per half step.
 int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux);
 int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux);
@@ -123,6 +127,7 @@ void batch_profile(Batch *b, int on, double *ms, long *chunks);
 constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 64 : 128; }
 
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
+struct FieldSensor; // single-wavelength FieldBlock / FieldMap (engine.hip)
 struct PlaneSum;    // normal-incidence R/T plane sum (engine.hip)
 
 }  // namespace aefdtd
@@ -179,6 +184,7 @@ struct aefdtd_grid {
     bool tf_per_x = false, tf_per_y = false;
     // sensors
     std::vector<aefdtd::Sensor *> sensors;
+    std::vector<aefdtd::FieldSensor *> fsensors;
     std::vector<aefdtd::PlaneSum *> sums;
     int sensor_step = 0;              // Sensor::step shared by every registered sensor
     int stage_fill = 0;

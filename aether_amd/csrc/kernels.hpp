@@ -1210,6 +1210,79 @@ __global__ __launch_bounds__(256) void k_sample_rc(const SensDesc *sd, int slot,
     if (s < r.ndisp) rc_lane(r, s, q / nrx);
 }
 
+// ---- single-wavelength field sensors (sensors/s_fieldmap.cpp)
+// FieldBlock (:89-230): over a box, E and H interpolated to the cell
+// (local_E*/local_H*) times e^{i w step Dt} (E) and e^{i w (step+1/2) Dt} (H),
+// added to FP64 complex accumulators every step.  FieldMap (:321-470): over a
+// plane normal to x, y or z, E (or H with `mag`) times e^{i w step Dt}; with
+// `cum` each plane cell first sums the values along the normal over the
+// sensor's depth (in FP64).  No tapering: the reference applies none.
+struct FieldDesc {
+    int block;                  // 1 FieldBlock, 0 FieldMap
+    int normal;                 // FieldMap: 0 x, 1 y, 2 z
+    int cum, mag;
+    int x1, y1, z1;             // first cell (full grid)
+    int s1, s2, s3;             // block: spans along x, y, z; map: in-plane spans and the depth
+    long n;                     // accumulator cells: s1 s2 s3 (block) or s1 s2 (map)
+    double2 *acc;               // [6 or 3][n]
+};
+
+// local_Ex..local_Hz (core/fdtd_utils.cpp:457-507) of cell (i, j, k), the
+// arithmetic of sample_point.
+template <typename S, typename T>
+__device__ __forceinline__ void local_fields(const Fld<S, T> &Ex, const Fld<S, T> &Ey, const Fld<S, T> &Ez,
+                                             const Fld<S, T> &Hx, const Fld<S, T> &Hy, const Fld<S, T> &Hz, int Nx,
+                                             int Ny, int Nz, int P, int i, int j, int k, T v[6]) {
+    const int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
+    auto at = [&](const Fld<S, T> &f, int x, int y, int z) { return f[x + P * (y + Ny * z)]; };
+    v[0] = (at(Ex, i, j, k) + at(Ex, i, jp, k) + at(Ex, i, j, kp) + at(Ex, i, jp, kp)) / (T)4.0;
+    v[1] = (at(Ey, i, j, k) + at(Ey, ip, j, k) + at(Ey, i, j, kp) + at(Ey, ip, j, kp)) / (T)4.0;
+    v[2] = (at(Ez, i, j, k) + at(Ez, ip, j, k) + at(Ez, i, jp, k) + at(Ez, ip, jp, k)) / (T)4.0;
+    v[3] = (at(Hx, i, j, k) + at(Hx, ip, j, k)) / (T)2.0;
+    v[4] = (at(Hy, i, j, k) + at(Hy, i, jp, k)) / (T)2.0;
+    v[5] = (at(Hz, i, j, k) + at(Hz, i, j, kp)) / (T)2.0;
+}
+
+// One feed of a field sensor: lane q updates accumulator cell q.
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_field_feed(const FieldDesc d, Fld<S, T> Ex, Fld<S, T> Ey, Fld<S, T> Ez,
+                                                    Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx, int Ny,
+                                                    int Nz, int P, double2 cE, double2 cH) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.n) return;
+    auto add = [&](int c, double v, double2 w) {
+        double2 &A = d.acc[(size_t)c * d.n + q];
+        A.x += v * w.x;
+        A.y += v * w.y;
+    };
+    T v[6];
+    if (d.block) {
+        const int i = (int)(q % d.s1), j = (int)((q / d.s1) % d.s2), k = (int)(q / ((long)d.s1 * d.s2));
+        local_fields(Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, d.x1 + i, d.y1 + j, d.z1 + k, v);
+        for (int c = 0; c < 6; c++) add(c, (double)v[c], c < 3 ? cE : cH);
+        return;
+    }
+    const int a = (int)(q % d.s1), b = (int)(q / d.s1);
+    // plane cell (a, b) -> grid offsets: normal x: (., a, b), y: (a, ., b), z: (a, b, .)
+    int ox = 0, oy = 0, oz = 0;
+    if (d.normal == 0) { oy = a; oz = b; }
+    else if (d.normal == 1) { ox = a; oz = b; }
+    else { ox = a; oy = b; }
+    const int o = d.mag ? 3 : 0;
+    if (!d.cum) {
+        local_fields(Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, d.x1 + ox, d.y1 + oy, d.z1 + oz, v);
+        for (int c = 0; c < 3; c++) add(c, (double)v[o + c], cE);
+        return;
+    }
+    double sum[3] = {0, 0, 0};
+    for (int t = 0; t < d.s3; t++) {
+        const int dx = d.normal == 0 ? t : 0, dy = d.normal == 1 ? t : 0, dz = d.normal == 2 ? t : 0;
+        local_fields(Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, d.x1 + ox + dx, d.y1 + oy + dy, d.z1 + oz + dz, v);
+        for (int c = 0; c < 3; c++) sum[c] += (double)v[o + c];
+    }
+    for (int c = 0; c < 3; c++) add(c, sum[c], cE);
+}
+
 // One sensor of one grid of a batch: the grid's fields, its stage pool and
 // the batch member index (for the step window).
 template <typename S, typename T>

@@ -19,6 +19,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -68,15 +69,62 @@ int rfail(aefdtd_run *r, int rc) {
     return rc;
 }
 
+// In-plane extents (span1, span2) of a planar sensor of orientation t over
+// the box x1..z2 (Sensor::link's spans, sensors.cpp:196-230).
+void plane_spans(int t, int dx, int dy, int dz, int &s1, int &s2) {
+    if (t == AEFDTD_NORMAL_X || t == AEFDTD_NORMAL_XM) { s1 = dy; s2 = dz; }
+    else if (t == AEFDTD_NORMAL_Y || t == AEFDTD_NORMAL_YM) { s1 = dx; s2 = dz; }
+    else { s1 = dx; s2 = dy; }
+}
+
+}  // namespace
+
+// Doubles a request contributes to sensor_out (the AEFDTD_SENSOR_* table of
+// aether_fdtd.h).
+size_t aefdtd::request_out_len(const aefdtd_sensor_req &q) {
+    const size_t dx = std::max(0, q.x2 - q.x1), dy = std::max(0, q.y2 - q.y1), dz = std::max(0, q.z2 - q.z1);
+    int s1 = 0, s2 = 0;
+    switch (q.kind) {
+        case AEFDTD_SENSOR_FIELDBLOCK: return 6 * dx * dy * dz * 2;
+        case AEFDTD_SENSOR_FIELDMAP:
+            plane_spans(2 * (q.orientation / 2), (int)dx, (int)dy, (int)dz, s1, s2);
+            return 3 * (size_t)s1 * s2 * 2;
+        case AEFDTD_SENSOR_FIELDMAP2:
+            plane_spans(q.orientation, (int)dx, (int)dy, (int)dz, s1, s2);
+            return 6 * (size_t)std::max(q.Nl, 0) * s1 * s2 * 2;
+        default: return q.box ? 7 * (size_t)std::max(q.Nl, 0) : (size_t)std::max(q.Nl, 0);
+    }
+}
+
+namespace {
+
 // Box_Spect_Poynting (sensors/s_spec_power.cpp:125-232) becomes six planar
-// sensors; a planar request one sensor.
+// sensors; a planar request or a FieldMap2 one sensor; FieldBlock and
+// FieldMap their field sensors (ids kept per request).
 int add_requests(aefdtd_run *r) {
     for (const aefdtd_sensor_req &q : r->reqs) {
-        if (q.Nl < 1) return AEFDTD_EINVAL;
+        if (q.kind < AEFDTD_SENSOR_POYNTING || q.kind > AEFDTD_SENSOR_FIELDMAP2) return AEFDTD_EINVAL;
+        if ((q.kind == AEFDTD_SENSOR_POYNTING || q.kind == AEFDTD_SENSOR_FIELDMAP2) && q.Nl < 1) return AEFDTD_EINVAL;
+        std::vector<int> v;
+        if (q.kind == AEFDTD_SENSOR_FIELDBLOCK || q.kind == AEFDTD_SENSOR_FIELDMAP) {
+            int id = -1;
+            if (q.kind == AEFDTD_SENSOR_FIELDBLOCK)
+                TRY(aefdtd_add_field_block(r->g, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, &id));
+            else
+                TRY(aefdtd_add_field_map(r->g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, 0, 0, &id));
+            int dims[4];
+            TRY(aefdtd_field_sensor_read(r->g, id, nullptr, dims));
+            if ((size_t)dims[0] * dims[1] * dims[2] * dims[3] * 2 != aefdtd::request_out_len(q)) {
+                r->g->err = "field sensor reaches outside the grid";
+                return AEFDTD_EINVAL;
+            }
+            v.push_back(id);
+            r->ids.push_back(v);
+            continue;
+        }
         std::vector<double> L;
         lambda_grid(q.Nl, q.lambda_min, q.lambda_max, L);
-        std::vector<int> v;
-        if (q.box) {
+        if (q.box && q.kind == AEFDTD_SENSOR_POYNTING) {
             const int t[6] = {AEFDTD_NORMAL_XM, AEFDTD_NORMAL_X, AEFDTD_NORMAL_YM, AEFDTD_NORMAL_Y, AEFDTD_NORMAL_ZM, AEFDTD_NORMAL_Z};
             const int b[6][6] = {{q.x1, q.x1 + 1, q.y1, q.y2, q.z1, q.z2}, {q.x2 - 1, q.x2, q.y1, q.y2, q.z1, q.z2},
                                  {q.x1, q.x2, q.y1, q.y1 + 1, q.z1, q.z2}, {q.x1, q.x2, q.y2 - 1, q.y2, q.z1, q.z2},
@@ -90,6 +138,14 @@ int add_requests(aefdtd_run *r) {
         } else {
             int id = -1;
             TRY(aefdtd_add_spect_poynting(r->g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.Nl, L.data(), &id));
+            if (q.kind == AEFDTD_SENSOR_FIELDMAP2) {
+                int s1 = 0, s2 = 0;
+                TRY(aefdtd::sensor_spans(r->g, id, &s1, &s2));
+                if (6 * (size_t)q.Nl * s1 * s2 * 2 != aefdtd::request_out_len(q)) {
+                    r->g->err = "field sensor reaches outside the grid";
+                    return AEFDTD_EINVAL;
+                }
+            }
             v.push_back(id);
         }
         r->ids.push_back(v);
@@ -102,6 +158,22 @@ int add_requests(aefdtd_run *r) {
 int collect_requests(aefdtd_run *r, double *out) {
     size_t off = 0;
     for (size_t q = 0; q < r->reqs.size(); q++) {
+        const int kind = r->reqs[q].kind;
+        if (kind != AEFDTD_SENSOR_POYNTING) {
+            // field sensors: their accumulators as they are
+            const size_t n = aefdtd::request_out_len(r->reqs[q]);
+            if (out) {
+                if (kind == AEFDTD_SENSOR_FIELDMAP2) {
+                    int dims[3];
+                    TRY(aefdtd_sensor_spectra(r->g, r->ids[q][0], out + off, dims));
+                } else {
+                    int dims[4];
+                    TRY(aefdtd_field_sensor_read(r->g, r->ids[q][0], out + off, dims));
+                }
+            }
+            off += n;
+            continue;
+        }
         int Nl = r->reqs[q].Nl;
         std::vector<double> tmp(Nl);
         if (r->reqs[q].box) {
@@ -571,6 +643,205 @@ int aefdtd_run_destroy(aefdtd_run *r) {
     return AEFDTD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+template <typename V>
+void put(std::ofstream &f, V v) { f.write(reinterpret_cast<const char *>(&v), sizeof(V)); }
+
+// FieldBlock::treat (sensors/s_fieldmap.cpp:232-287): <name>.afblock, binary.
+bool write_fieldblock(aefdtd_run *r, const std::filesystem::path &dir, const std::string &name, int id,
+                      const double *v) {
+    aefdtd_grid *g = r->g;
+    int geo[8];
+    double lambda = 0;
+    if (aefdtd::field_sensor_geom(g, id, geo, &lambda)) return false;
+    const int x1 = geo[2], y1 = geo[3], z1 = geo[4], s1 = geo[5], s2 = geo[6], s3 = geo[7];
+    const size_t n = (size_t)s1 * s2 * s3;
+    std::vector<uint32_t> mats(n);
+    if (aefdtd_field_sensor_mats(g, id, mats.data())) return false;
+    std::ofstream f(dir / (name + ".afblock"), std::ios::out | std::ios::trunc | std::ios::binary);
+    if (!f) return false;
+    put(f, lambda); put(f, x1); put(f, s1); put(f, y1); put(f, s2); put(f, z1); put(f, s3);
+    put(f, g->Dx); put(f, g->Dy); put(f, g->Dz);
+    for (int i = 0; i < s1; i++)
+        for (int j = 0; j < s2; j++)
+            for (int k = 0; k < s3; k++) {
+                const size_t q = (size_t)i + (size_t)s1 * ((size_t)j + (size_t)s2 * k);
+                for (int c = 0; c < 6; c++) { put(f, v[2 * (c * n + q)]); put(f, v[2 * (c * n + q) + 1]); }
+                put(f, mats[q]);
+            }
+    return (bool)f;
+}
+
+// FieldMap::treat (s_fieldmap.cpp:470-506, PNG excepted): <name>_fieldmap
+// (binary) and, through fmap_script / fmap_raw / fmap_mats_raw
+// (sensors/fieldblock_holder.cpp:83-361), <name>.m, <name>_E{x,y,z}_raw and
+// <name>_mats_raw.
+bool write_fieldmap(aefdtd_run *r, const std::filesystem::path &dir, const std::string &name, int id,
+                    const double *v) {
+    namespace fs = std::filesystem;
+    aefdtd_grid *g = r->g;
+    int geo[8];
+    double lambda = 0;
+    if (aefdtd::field_sensor_geom(g, id, geo, &lambda)) return false;
+    const int normal = geo[1], x1 = geo[2], y1 = geo[3], z1 = geo[4], s1 = geo[5], s2 = geo[6];
+    const size_t n = (size_t)s1 * s2;
+    std::vector<uint32_t> mats(n);
+    if (aefdtd_field_sensor_mats(g, id, mats.data())) return false;
+    auto re = [&](int c, int i, int j) { return v[2 * (c * n + (size_t)i + (size_t)s1 * j)]; };
+    auto im = [&](int c, int i, int j) { return v[2 * (c * n + (size_t)i + (size_t)s1 * j) + 1]; };
+    {
+        std::ofstream f(dir / (name + "_fieldmap"), std::ios::out | std::ios::trunc | std::ios::binary);
+        if (!f) return false;
+        const int type = 2 * normal;
+        put(f, lambda); put(f, type);
+        put(f, normal == 0 ? y1 : x1);
+        put(f, s1);
+        put(f, normal == 2 ? y1 : z1);
+        put(f, s2);
+        put(f, g->Dx); put(f, g->Dy); put(f, g->Dz);
+        for (int i = 0; i < s1; i++)
+            for (int j = 0; j < s2; j++)
+                for (int c = 0; c < 3; c++) { put(f, re(c, i, j)); put(f, im(c, i, j)); }
+        if (!f) return false;
+    }
+    const fs::path base = dir / name;
+    const std::string stem = base.stem().generic_string();
+    const char *raw[3] = {"_Ex_raw", "_Ey_raw", "_Ez_raw"};
+    {
+        fs::path mfile = base;
+        if (mfile.extension() != ".m") mfile.replace_extension(".m");
+        std::ofstream f(mfile, std::ios::out | std::ios::trunc | std::ios::binary);
+        if (!f) return false;
+        const double D1 = 1.0, D2 = 1.0;
+        f << "function out=" << stem << "(varargin)" << '\n' << "" << '\n' << "baseline=1;" << '\n' << "" << '\n'
+          << "x=linspace(0,6,1000);" << '\n' << "" << '\n'
+          << "r=(x<2)+(3.0-x).*(x>=2).*(x<3)+(x-5).*(x>=5).*(x<6)+(x>=6);" << '\n'
+          << "g=(x-1.0).*(x>=1).*(x<2)+(x>=2).*(x<4)+(5.0-x).*(x>=4).*(x<5);" << '\n'
+          << "b=(1.0-x).*(x<1)+(x-3).*(x>=3).*(x<4)+(x>=4);" << '\n' << "" << '\n'
+          << "cmap=[transpose(r) transpose(g) transpose(b)];" << '\n' << "" << '\n'
+          << "Fx=dlmread('" << stem << raw[0] << "')/baseline;" << '\n'
+          << "Fy=dlmread('" << stem << raw[1] << "')/baseline;" << '\n'
+          << "Fz=dlmread('" << stem << raw[2] << "')/baseline;" << '\n'
+          << "mats=dlmread('" << stem << "_mats_raw" << "');" << '\n' << "" << '\n'
+          << "Nx=size(Fx,2);" << '\n' << "Ny=size(Fy,2);" << '\n' << "Nz=size(Fz,2);" << '\n' << "" << '\n'
+          << "Fx_r=Fx(:,1:2:Nx);" << '\n' << "Fx_i=Fx(:,2:2:Nx);" << '\n' << "Fx=Fx_r+i*Fx_i;" << '\n' << "" << '\n'
+          << "Fy_r=Fy(:,1:2:Ny);" << '\n' << "Fy_i=Fy(:,2:2:Ny);" << '\n' << "Fy=Fy_r+i*Fy_i;" << '\n' << "" << '\n'
+          << "Fz_r=Fz(:,1:2:Nz);" << '\n' << "Fz_i=Fz(:,2:2:Nz);" << '\n' << "Fz=Fz_r+i*Fz_i;" << '\n'
+          << "" << '\n' << "N1=size(Fx,2);" << '\n' << "N2=size(Fx,1);" << '\n'
+          << "N1t=ones(N2,1)*(1:N1)*" << D1 << ";" << '\n' << "N2t=transpose((1:N2)*" << D2 << ")*ones(1,N1);" << '\n'
+          << "N1=N1t;" << '\n' << "N2=N2t;" << '\n' << '\n' << "silent=0;" << '\n' << '\n'
+          << "if nargin>0" << '\n' << "    j=1;" << '\n' << "    while j<=nargin" << '\n'
+          << "        if strcmp(varargin{j},'saturate')" << '\n' << "            val=varargin{j+1}/sqrt(3.0);" << '\n'
+          << "            Fx=Fx.*(abs(Fx)<val)+val*exp(i*angle(Fx)).*(abs(Fx)>=val);" << '\n'
+          << "            Fy=Fy.*(abs(Fy)<val)+val*exp(i*angle(Fy)).*(abs(Fy)>=val);" << '\n'
+          << "            Fz=Fz.*(abs(Fz)<val)+val*exp(i*angle(Fz)).*(abs(Fz)>=val);" << '\n'
+          << "            j=j+2;" << '\n' << "            continue;" << '\n' << "        end" << '\n'
+          << "        if strcmp(varargin{j},'silent')" << '\n' << "            silent=1;" << '\n' << "            j=j+1;" << '\n'
+          << "            continue;" << '\n' << "        end" << '\n' << "    end" << '\n' << "end" << '\n' << '\n'
+          << "F_tot=sqrt(abs(Fx).^2+abs(Fy).^2+abs(Fz).^2);" << '\n' << '\n' << "if silent==0" << '\n';
+        const char *lab[3] = {"x", "y", "z"};
+        for (int c = 0; c < 3; c++)
+            f << "    figure(" << c + 1 << ")" << '\n' << "    pcolor(N1,N2,abs(F" << lab[c] << "))" << '\n'
+              << "    colormap(jet(500))" << '\n' << "    colorbar" << '\n' << "    title('|E_" << lab[c] << "|')" << '\n'
+              << "    shading flat" << '\n';
+        f << "" << '\n';
+        for (int c = 0; c < 3; c++)
+            f << "    figure(" << c + 4 << ")" << '\n' << "    pcolor(N1,N2,angle(F" << lab[c] << "))" << '\n'
+              << "    colormap(cmap)" << '\n' << "    shading flat" << '\n' << "    title('arg(E_" << lab[c] << ")')" << '\n';
+        f << "" << '\n' << "    figure(7)" << '\n' << "    pcolor(N1,N2,F_tot)" << '\n' << "    title('|E|')" << '\n'
+          << "    colormap(jet(500))" << '\n' << "    colorbar" << '\n' << "    shading flat" << '\n' << "end" << '\n'
+          << "" << '\n' << "out.Ex=Fx;" << '\n' << "out.Ey=Fy;" << '\n' << "out.Ez=Fz;" << '\n' << "out.mats=mats;" << '\n'
+          << "" << '\n' << "end" << '\n';
+        if (!f) return false;
+    }
+    for (int c = 0; c < 3; c++) {
+        std::ofstream f(base.parent_path() / (stem + raw[c]), std::ios::out | std::ios::trunc);
+        if (!f) return false;
+        std::stringstream strm;
+        for (int j = 0; j < s2; j++) {
+            for (int i = 0; i < s1; i++) strm << re(c, i, j) << " " << im(c, i, j) << " ";
+            strm << std::endl;
+        }
+        f << strm.str();
+    }
+    std::ofstream f(base.parent_path() / (stem + "_mats_raw"), std::ios::out | std::ios::trunc);
+    if (!f) return false;
+    std::stringstream strm;
+    for (int j = 0; j < s2; j++) {
+        for (int i = 0; i < s1; i++) strm << mats[(size_t)i + (size_t)s1 * j] << " ";
+        strm << std::endl;
+    }
+    f << strm.str();
+    return (bool)f;
+}
+
+// FieldMap2::treat (s_fieldmap.cpp:594-672): <name>_fieldmap, text.  The
+// coordinate lines add the structure offset to the linked (already offset)
+// indices, and the z line is always written (its test is `type != NORMAL_Z
+// || type != NORMAL_ZM`), as the reference does.
+bool write_fieldmap2(aefdtd_run *r, const std::filesystem::path &dir, const std::string &name, int id,
+                     const aefdtd_sensor_req &q, const double *v) {
+    aefdtd_grid *g = r->g;
+    int b[6], s1 = 0, s2 = 0;
+    if (aefdtd::sensor_box(g, id, b) || aefdtd::sensor_spans(g, id, &s1, &s2)) return false;
+    std::vector<double> L;
+    lambda_grid(q.Nl, q.lambda_min, q.lambda_max, L);
+    const int type = q.orientation;
+    std::ofstream f(dir / (name + "_fieldmap"), std::ios::out | std::ios::trunc | std::ios::binary);
+    if (!f) return false;
+    f << 0 << " " << type << " 1 1 1 1 1 1 1\n";
+    for (int l = 0; l < q.Nl; l++) {
+        f << L[l];
+        if (l + 1 != q.Nl) f << " ";
+    }
+    f << "\n";
+    if (type != AEFDTD_NORMAL_X && type != AEFDTD_NORMAL_XM) {
+        for (int i = b[0]; i < b[1]; i++) {
+            f << (g->xs_s + i) * g->Dx;
+            if (i + 1 != b[1]) f << " ";
+        }
+        f << "\n";
+    }
+    if (type != AEFDTD_NORMAL_Y && type != AEFDTD_NORMAL_YM) {
+        for (int j = b[2]; j < b[3]; j++) {
+            f << (g->ys_s + j) * g->Dy;
+            if (j + 1 != b[3]) f << " ";
+        }
+        f << "\n";
+    }
+    for (int k = b[4]; k < b[5]; k++) {
+        f << (g->zs_s + k) * g->Dz;
+        if (k + 1 != b[5]) f << " ";
+    }
+    f << "\n";
+    for (int j = 0; j < s2; j++) {
+        for (int i = 0; i < s1; i++) {
+            f << 0;
+            if (i + 1 != s1) f << " ";
+        }
+        f << "\n";
+    }
+    const size_t P = (size_t)s1 * s2;
+    for (int l = 0; l < q.Nl; l++)
+        for (int c = 0; c < 6; c++)
+            for (int j = 0; j < s2; j++) {
+                for (int i = 0; i < s1; i++) {
+                    const size_t o = 2 * (((size_t)c * q.Nl + l) * P + (size_t)i + (size_t)s1 * j);
+                    f << v[o] << " " << v[o + 1];
+                    if (i + 1 != s1) f << " ";
+                }
+                f << "\n";
+            }
+    return (bool)f;
+}
+
+}  // namespace
+
+extern "C" {
+
 // Result files as the reference writes them; the same stream formatting
 // (std::ostream, setprecision(15) or the default 6 digits) on the same values.
 int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, const char *const *names,
@@ -592,6 +863,18 @@ int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, c
         lambda_grid(Nl, s.lambda_min, s.lambda_max, L);
         const std::string name = names && names[q] ? names[q] : "sensor" + std::to_string(q);
         const double *v = sensor_out + off;
+        if (s.kind != AEFDTD_SENSOR_POYNTING) {
+            const int id = r->ids[q][0];
+            bool ok = s.kind == AEFDTD_SENSOR_FIELDBLOCK ? write_fieldblock(r, dir, name, id, v)
+                      : s.kind == AEFDTD_SENSOR_FIELDMAP ? write_fieldmap(r, dir, name, id, v)
+                                                         : write_fieldmap2(r, dir, name, id, s, v);
+            if (!ok) {
+                r->err = "cannot write the files of sensor " + name;
+                return AEFDTD_EINVAL;
+            }
+            off += aefdtd::request_out_len(s);
+            continue;
+        }
         if (s.box) {
             // Box_Spect_Poynting::treat (sensors/s_spec_power.cpp:174-232)
             std::ofstream f, fm;
@@ -627,6 +910,22 @@ int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, c
     if (r->kind == 0 && res && res->lambda) {
         // FDTD_normal_incidence's spectra (modes/normal_incidence.cpp:490-557)
         const std::string pre = prefix ? prefix : "";
+        {
+            // norm_octave_script (normal_incidence.cpp:24-59, called at :601)
+            std::ofstream f;
+            if (!open(dir / (pre + "show_norm.m"), f)) return AEFDTD_EINVAL;
+            const std::string stem = fs::path((dir / pre).generic_string()).stem().generic_string();
+            using std::endl;
+            f << "function out=" << stem << "show_norm(varargin)" << endl << endl;
+            f << "D=load('" << stem + "spectdata_norm" << "');" << endl;
+            f << "L=D(:,1)*1e6;" << endl << "r=D(:,2);" << endl << "t=D(:,3);" << endl << endl;
+            f << "R=r.^2;" << endl << "T=" << res->index_sub << "/" << res->index_sup << "*t.^2;" << endl
+              << "A=1-R-T;" << endl << endl;
+            f << "figure(1)" << endl << "plot(L,r,L,t)" << endl << "axis([min(L) max(L) 0 1])" << endl << endl;
+            f << "figure(2)" << endl << "plot(L,R,L,T,L,A)" << endl << "legend('R','T','A')" << endl
+              << "xlabel('Lambda (microns)')" << endl << "axis([min(L) max(L) 0 1])" << endl << endl;
+            f << "if nargin>0" << endl << "    out=[L,R,T,A];" << endl << "end" << endl << endl << "end" << endl;
+        }
         std::ofstream f1, f2;
         if (!open(dir / (pre + "spectdata_norm"), f1) || !open(dir / (pre + "spectdata_norm2"), f2))
             return AEFDTD_EINVAL;

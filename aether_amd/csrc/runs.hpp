@@ -52,4 +52,6 @@ namespace aefdtd {
 // Full-grid dimensions a mode driver of `kind` builds for mode m (the padding
 // rules of the drivers and of FDTD::FDTD), and the auxiliary grid's cells.
 void run_dims(int kind, const aefdtd_mode *m, int dims[3], int pads[6]);
+// Doubles a sensor request contributes to the drivers' sensor_out.
+size_t request_out_len(const aefdtd_sensor_req &q);
 }  // namespace aefdtd

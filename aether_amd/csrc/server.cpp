@@ -86,7 +86,7 @@ void copy_inputs(Request &q, const aefdtd_mode *m, int ns, const aefdtd_sensor_r
 
 size_t out_len(const Request &q) {
     size_t n = 0;
-    for (const aefdtd_sensor_req &s : q.sens) n += s.box ? 7 * (size_t)s.Nl : (size_t)s.Nl;
+    for (const aefdtd_sensor_req &s : q.sens) n += aefdtd::request_out_len(s);
     if (q.kind == 0) n += 2 * (size_t)q.m.Nl;
     return n;
 }
@@ -439,7 +439,15 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
         const size_t sx = std::max(0, q.x2 - q.x1), sy = std::max(0, q.y2 - q.y1), sz = std::max(0, q.z2 - q.z1);
         size_t P[6] = {0, 0, 0, 0, 0, 0};
         int nf = 0;
-        if (q.box) {
+        if (q.kind == AEFDTD_SENSOR_FIELDBLOCK || q.kind == AEFDTD_SENSOR_FIELDMAP) {
+            // FP64 complex accumulators (6 or 3 components) and a material index per cell
+            const size_t cells = q.kind == AEFDTD_SENSOR_FIELDBLOCK ? sx * sy * sz
+                                 : q.orientation / 2 == 0 ? sy * sz
+                                 : q.orientation / 2 == 1 ? sx * sz : sx * sy;
+            b += cells * ((q.kind == AEFDTD_SENSOR_FIELDBLOCK ? 6 : 3) * 16 + 4);
+            continue;
+        }
+        if (q.box && q.kind == AEFDTD_SENSOR_POYNTING) {
             P[0] = P[1] = sy * sz; P[2] = P[3] = sx * sz; P[4] = P[5] = sx * sy;
             nf = 6;
         } else {
@@ -447,7 +455,7 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
             nf = 1;
         }
         for (int f = 0; f < nf; f++) {
-            if (q.box && q.disable[f]) continue;
+            if (nf == 6 && q.disable[f]) continue;
             b += 6 * (size_t)q.Nl * P[f] * 16 + (size_t)std::max(m->Nt, 1) * 2 * q.Nl * 16;
             Ptot += P[f];
         }
@@ -456,7 +464,11 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
     // sensor strips of planes normal to x on 3D grids (two columns x 6 values per (y, z))
     if (d[2] > 1 && d[0] > 1) {
         int nx_planes = 0;
-        for (int i = 0; i < ns; i++) nx_planes += sens[i].box ? 2 : (sens[i].orientation <= 1 ? 1 : 0);
+        for (int i = 0; i < ns; i++) {
+            const aefdtd_sensor_req &q = sens[i];
+            if (q.kind == AEFDTD_SENSOR_FIELDBLOCK || q.kind == AEFDTD_SENSOR_FIELDMAP) continue;
+            nx_planes += q.box && q.kind == AEFDTD_SENSOR_POYNTING ? 2 : (q.orientation <= 1 ? 1 : 0);
+        }
         b += (size_t)std::min(nx_planes, 4) * 12 * d[1] * d[2] * cs;
     }
     if (kind == 2) b += (size_t)std::max(0, m->n_sources) * std::max(m->Nt, 1) * 8;

@@ -232,8 +232,14 @@ Structure.lua_methods = {"parameter": _structure_parameter, "finalize": lambda s
 
 # ---------------------------------------------------------------- sensors and sources
 
+_SENSOR_KINDS = {"box_spectral_poynting": capi.SENSOR_POYNTING, "planar_spectral_poynting": capi.SENSOR_POYNTING,
+                 "fieldblock": capi.SENSOR_FIELDBLOCK, "fieldmap": capi.SENSOR_FIELDMAP,
+                 "fieldmap2": capi.SENSOR_FIELDMAP2}
+
+
 class SensorGen:
-    """Sensor_generator (sensors/sensors.cpp:20-161) for the spectral Poynting sensors."""
+    """Sensor_generator (sensors/sensors.cpp:20-161) for the spectral Poynting
+    and the field sensors (Sensor_type names, lua_sensors.cpp:126-147)."""
 
     def __init__(self, kind):
         self.kind = kind
@@ -247,7 +253,7 @@ class SensorGen:
 
     def to_req(self, D):
         """Sensor_generator::to_discrete (sensors.cpp:142-161)."""
-        if self.kind not in ("box_spectral_poynting", "planar_spectral_poynting"):
+        if self.kind not in _SENSOR_KINDS:
             raise LuaError(f"sensor type {self.kind} is not provided by the engine")
         g = list(self.g)
         if self.real:
@@ -259,7 +265,8 @@ class SensorGen:
                 g[2 * a], g[2 * a + 1] = lo, hi
         return capi.SensorReq(box=self.kind == "box_spectral_poynting", orientation=self.orientation,
                               x1=g[0], x2=g[1], y1=g[2], y2=g[3], z1=g[4], z2=g[5], disable=tuple(self.disable),
-                              Nl=self.Nl, lambda_min=self.lambda_min, lambda_max=self.lambda_max)
+                              Nl=self.Nl, lambda_min=self.lambda_min, lambda_max=self.lambda_max,
+                              kind=_SENSOR_KINDS[self.kind])
 
 
 def _sensor_location(real):
@@ -297,10 +304,16 @@ def _sensor_spectrum(self, lmin, lmax, Nl):
     self.lambda_min, self.lambda_max, self.Nl = _num(lmin, "spectrum"), _num(lmax, "spectrum"), _int(Nl, "spectrum")
 
 
+def _sensor_wavelength(self, lam):
+    """Sensor_generator::set_wavelength (sensors.cpp:135-140): the analysis
+    wavelength of fieldblock / fieldmap sensors."""
+    self.lambda_min = self.lambda_max = _num(lam, "wavelength")
+
+
 SensorGen.lua_methods = {
     "location": _sensor_location(True), "location_grid": _sensor_location(False), "disable": _sensor_disable,
     "name": lambda self, n: setattr(self, "name", tostring(n)), "orientation": _sensor_orientation,
-    "spectrum": _sensor_spectrum,
+    "spectrum": _sensor_spectrum, "wavelength": _sensor_wavelength,
 }
 
 

@@ -194,6 +194,28 @@ int aefdtd_sensor_poynting(aefdtd_grid *g, int id, double *rsp);
  * shares that array: calls that overlap on one GPU race (serialise them). */
 #define AEFDTD_NOALLOC_MAX_NL 4096
 int aefdtd_sensor_poynting_noalloc(aefdtd_grid *g, int id, double *rsp);
+/* Single-wavelength field sensors (sensors/s_fieldmap.cpp), location in
+ * structure cells, offset and clamped as Sensor::link.  Every feed adds the
+ * interpolated fields (local_E*, local_H*) times the DFT phase to FP64 complex
+ * accumulators, without tapering:
+ *  FieldBlock (:89-230): the box; E at e^{i w step Dt}, H at e^{i w (step+1/2) Dt}.
+ *  FieldMap (:321-470): the first plane of the box normal to `orientation`
+ *  (the - orientations fold to +); E (H with `magnetic`) at e^{i w step Dt};
+ *  with `cumulative`, the values summed along the normal over the box depth.
+ * Each writes its id (ids count field sensors separately from the spectral
+ * ones). */
+int aefdtd_add_field_block(aefdtd_grid *g, int x1, int x2, int y1, int y2, int z1, int z2,
+                           double lambda, int *id);
+int aefdtd_add_field_map(aefdtd_grid *g, int orientation, int x1, int x2, int y1, int y2,
+                         int z1, int z2, double lambda, int cumulative, int magnetic, int *id);
+/* Accumulators of field sensor id: out[c][q] complex as (re,im), c over the
+ * ncomp = dims[0] components (block: Ex..Hz; map: Ex,Ey,Ez or Hx,Hy,Hz), q =
+ * a + s1 (b + s2 c) over dims[1..3] = s1, s2, s3 (block: the spans along x, y,
+ * z; map: the in-plane spans, x then y for a z map, y then z for an x map, x
+ * then z for a y map, and 1).  out may be NULL to read dims only. */
+int aefdtd_field_sensor_read(aefdtd_grid *g, int id, double *out, int dims[4]);
+/* The material indices under those cells (FieldBlock::mats, FieldMap::mats). */
+int aefdtd_field_sensor_mats(aefdtd_grid *g, int id, uint32_t *out);
 /* Raw DFT accumulators sp_Ex..sp_Hz of one sensor: out[c][l][j][i] complex
  * as (re,im), c in Ex,Ey,Ez,Hx,Hy,Hz; span1 x span2 written to dims[0..1]. */
 int aefdtd_sensor_spectra(aefdtd_grid *g, int id, double *out, int dims[3]);
@@ -278,7 +300,24 @@ typedef struct aefdtd_sensor_req {
     int    disable[6];           /* box faces to skip                     */
     int    Nl;
     double lambda_min, lambda_max;
+    int    kind;                 /* AEFDTD_SENSOR_*                       */
 } aefdtd_sensor_req;
+
+/* Sensor request kinds (Sensor_type, sensors/lua_sensors.cpp:126-147) and
+ * what aefdtd_run_finish writes to sensor_out for each, in request order:
+ *   POYNTING  box_spectral_poynting (box=1): Nl totals, then 6 x Nl per face;
+ *             planar_spectral_poynting (box=0): Nl values
+ *   FIELDBLOCK  fieldblock at lambda_min: 6 x s1 s2 s3 complex (re,im), the
+ *             layout of aefdtd_field_sensor_read (s = the box extents)
+ *   FIELDMAP  fieldmap at lambda_min, normal to `orientation`: 3 x s1 s2
+ *             complex, E (as the reference's scripts create it)
+ *   FIELDMAP2 fieldmap2: the spectral accumulators of a planar sensor with Nl
+ *             wavelengths, 6 x Nl x s1 s2 complex (aefdtd_sensor_spectra)
+ * Field sensors must lie inside the grid (their extents are not clamped). */
+#define AEFDTD_SENSOR_POYNTING   0
+#define AEFDTD_SENSOR_FIELDBLOCK 1
+#define AEFDTD_SENSOR_FIELDMAP   2
+#define AEFDTD_SENSOR_FIELDMAP2  3
 
 /* The sizing part of FDTD_Mode::finalize (modes/lua_fdtd.cpp:74-203) for
  * mode type `kind` (0 normal incidence, 1 single particle, 2 custom): normal

@@ -46,7 +46,8 @@ static bool cancelled(const int *flag) { return flag && __atomic_load_n(flag, __
 // 1 = TFSF-E z faces read the auxiliary H one plane off, 2 = TFSF-H lower y
 // face with the sign of the Hz correction flipped, 3 = no PEC wall for Ex on y,
 // 4 = the far-side x extension copies the second-last structure plane,
-// 5 = normal incidence without the -Dz/2 phase calibration.
+// 5 = normal incidence without the -Dz/2 phase calibration, 6 = local_Ex
+// averaging toward j-1.
 static int g_slip = 0;
 
 namespace {
@@ -326,6 +327,19 @@ struct Oscillator {
     double w0 = 0, dw = 0, Dt = 0;
 };
 
+// FieldBlock (sensors/s_fieldmap.cpp:28-230) and FieldMap (:290-470): a
+// single wavelength, one accumulation per feed at the sensor's own step.
+struct FieldSens {
+    bool block = true;
+    int type = 0;                  // FieldMap: NORMAL_X/Y/Z after initialize's folding (:433-440)
+    bool cumulative = false, mag_map = false;
+    int x1, x2, y1, y2, z1, z2;    // linked (full-grid) location
+    int span1 = 0, span2 = 0, span3 = 0;
+    int step = 0;
+    double lambda = 0, Dt = 0;
+    std::vector<cplx> acc[6];      // block: Ex..Hz (span1, span2, span3); map: acc_Ex..Ez (span1, span2)
+};
+
 struct PlaneSum {
     int k;
     bool has_inc, subtract;
@@ -362,6 +376,7 @@ struct aeoracle_grid {
     int tb[6] = {0, 0, 0, 0, 0, 0};
     bool tf_per_x = false, tf_per_y = false;
     std::vector<SpectSensor *> sensors;
+    std::vector<FieldSens *> fsensors;
     std::vector<PlaneSum *> sums;
     std::vector<Oscillator> oscs;
     Pool *pool = nullptr;
@@ -941,7 +956,9 @@ void aeoracle_grid::update_H() {
 double aeoracle_grid::local(int c, int i, int j, int k) const {
     int ip = (i + 1 == Nx) ? 0 : i + 1, jp = (j + 1 == Ny) ? 0 : j + 1, kp = (k + 1 == Nz) ? 0 : k + 1;
     switch (c) {
-        case 0: return (cf(0, i, j, k) + cf(0, i, jp, k) + cf(0, i, j, kp) + cf(0, i, jp, kp)) / 4.0;
+        case 0:
+            if (g_slip == 6) jp = j == 0 ? Ny - 1 : j - 1;
+            return (cf(0, i, j, k) + cf(0, i, jp, k) + cf(0, i, j, kp) + cf(0, i, jp, kp)) / 4.0;
         case 1: return (cf(1, i, j, k) + cf(1, ip, j, k) + cf(1, i, j, kp) + cf(1, ip, j, kp)) / 4.0;
         case 2: return (cf(2, i, j, k) + cf(2, ip, j, k) + cf(2, i, jp, k) + cf(2, ip, jp, k)) / 4.0;
         case 3: return (cf(3, i, j, k) + cf(3, ip, j, k)) / 2.0;
@@ -1124,6 +1141,7 @@ int aeoracle_create(aeoracle_grid **out, int Nx, int Ny, int Nz, int Nt, double 
 int aeoracle_destroy(aeoracle_grid *g) {
     if (!g) return AEFDTD_OK;
     for (auto *s : g->sensors) delete s;
+    for (auto *s : g->fsensors) delete s;
     for (auto *s : g->sums) delete s;
     delete g->pool;
     delete g;
@@ -1265,6 +1283,87 @@ int aeoracle_set_tfsf(aeoracle_grid *g, const aeoracle_grid *aux, int i1, int i2
     return AEFDTD_OK;
 }
 
+static int add_field(aeoracle_grid *g, FieldSens *s, int x1, int x2, int y1, int y2, int z1, int z2, double lambda,
+                     int *id) {
+    // Sensor::link offsets and clamps (sensors.cpp:214-224)
+    s->x1 = std::clamp(x1 + g->xs_s, 0, g->Nx); s->x2 = std::clamp(x2 + g->xs_s, 0, g->Nx);
+    s->y1 = std::clamp(y1 + g->ys_s, 0, g->Ny); s->y2 = std::clamp(y2 + g->ys_s, 0, g->Ny);
+    s->z1 = std::clamp(z1 + g->zs_s, 0, g->Nz); s->z2 = std::clamp(z2 + g->zs_s, 0, g->Nz);
+    if (s->x2 <= s->x1 || s->y2 <= s->y1 || s->z2 <= s->z1 || !(lambda > 0)) {
+        delete s;
+        return set_err(g, AEFDTD_EINVAL, "empty sensor");
+    }
+    s->lambda = lambda;                   // set_spectrum(lambda) (sensors.cpp:254-263)
+    s->Dt = g->Dt;
+    if (s->block) {                       // FieldBlock::initialize (s_fieldmap.cpp:177-191)
+        s->span1 = s->x2 - s->x1; s->span2 = s->y2 - s->y1; s->span3 = s->z2 - s->z1;
+        for (int c = 0; c < 6; c++) s->acc[c].assign((size_t)s->span1 * s->span2 * s->span3, cplx(0, 0));
+    } else {                              // FieldMap::initialize (:433-446)
+        if (s->type == AEFDTD_NORMAL_XM) s->type = AEFDTD_NORMAL_X;
+        else if (s->type == AEFDTD_NORMAL_YM) s->type = AEFDTD_NORMAL_Y;
+        else if (s->type == AEFDTD_NORMAL_ZM) s->type = AEFDTD_NORMAL_Z;
+        if (s->type == AEFDTD_NORMAL_X) { s->span1 = s->y2 - s->y1; s->span2 = s->z2 - s->z1; s->span3 = s->x2 - s->x1; }
+        if (s->type == AEFDTD_NORMAL_Y) { s->span1 = s->x2 - s->x1; s->span2 = s->z2 - s->z1; s->span3 = s->y2 - s->y1; }
+        if (s->type == AEFDTD_NORMAL_Z) { s->span1 = s->x2 - s->x1; s->span2 = s->y2 - s->y1; s->span3 = s->z2 - s->z1; }
+        for (int c = 0; c < 3; c++) s->acc[c].assign((size_t)s->span1 * s->span2, cplx(0, 0));
+    }
+    g->fsensors.push_back(s);
+    *id = (int)g->fsensors.size() - 1;
+    return AEFDTD_OK;
+}
+
+int aeoracle_add_field_block(aeoracle_grid *g, int x1, int x2, int y1, int y2, int z1, int z2, double lambda,
+                             int *id) {
+    auto *s = new FieldSens();
+    s->block = true;
+    return add_field(g, s, x1, x2, y1, y2, z1, z2, lambda, id);
+}
+
+int aeoracle_add_field_map(aeoracle_grid *g, int orientation, int x1, int x2, int y1, int y2, int z1, int z2,
+                           double lambda, int cumulative, int magnetic, int *id) {
+    if (orientation < 0 || orientation > 5) return set_err(g, AEFDTD_EINVAL, "bad orientation");
+    auto *s = new FieldSens();
+    s->block = false;
+    s->type = orientation;
+    s->cumulative = cumulative != 0;
+    s->mag_map = magnetic != 0;
+    return add_field(g, s, x1, x2, y1, y2, z1, z2, lambda, id);
+}
+
+int aeoracle_field_sensor_read(aeoracle_grid *g, int id, double *out, int dims[4]) {
+    if (!g || !dims || id < 0 || id >= (int)g->fsensors.size()) return AEFDTD_EINVAL;
+    const FieldSens &s = *g->fsensors[id];
+    const int nc = s.block ? 6 : 3;
+    dims[0] = nc; dims[1] = s.span1; dims[2] = s.span2; dims[3] = s.block ? s.span3 : 1;
+    if (!out) return AEFDTD_OK;
+    size_t n = s.acc[0].size(), o = 0;
+    for (int c = 0; c < nc; c++)
+        for (size_t q = 0; q < n; q++) { out[o++] = s.acc[c][q].real(); out[o++] = s.acc[c][q].imag(); }
+    return AEFDTD_OK;
+}
+
+// FieldBlock::deep_feed / FieldMap::deep_feed at step 0 (s_fieldmap.cpp:100-110, 331-345)
+int aeoracle_field_sensor_mats(aeoracle_grid *g, int id, uint32_t *out) {
+    if (!g || !out || id < 0 || id >= (int)g->fsensors.size()) return AEFDTD_EINVAL;
+    const FieldSens &s = *g->fsensors[id];
+    if (s.block) {
+        for (int k = 0; k < s.span3; k++)
+            for (int j = 0; j < s.span2; j++)
+                for (int i = 0; i < s.span1; i++)
+                    out[(size_t)i + (size_t)s.span1 * ((size_t)j + (size_t)s.span2 * k)] = g->mg(s.x1 + i, s.y1 + j, s.z1 + k);
+        return AEFDTD_OK;
+    }
+    for (int j = 0; j < s.span2; j++)
+        for (int i = 0; i < s.span1; i++) {
+            int a = 0, b = 0, c = 0;
+            if (s.type == AEFDTD_NORMAL_X) { b = i; c = j; }
+            else if (s.type == AEFDTD_NORMAL_Y) { a = i; c = j; }
+            else { a = i; b = j; }
+            out[(size_t)i + (size_t)s.span1 * j] = g->mg(s.x1 + a, s.y1 + b, s.z1 + c);
+        }
+    return AEFDTD_OK;
+}
+
 int aeoracle_add_spect_poynting(aeoracle_grid *g, int type, int x1, int x2, int y1, int y2, int z1,
                                 int z2, int Nl, const double *lambda, int *id) {
     auto *s = new SpectSensor();
@@ -1314,8 +1413,52 @@ int aeoracle_add_oscillator(aeoracle_grid *g, int type, int x1, int x2, int y1, 
     return AEFDTD_OK;
 }
 
+// FieldBlock::threaded_computation (s_fieldmap.cpp:196-229) and
+// FieldMap::FT_compute (:352-431), then Sensor::feed's step += 1 (sensors.cpp:232-243).
+static void field_feed(const aeoracle_grid &g, FieldSens &s) {
+    const double w = 2.0 * Pi * c_light / s.lambda;
+    const cplx I(0.0, 1.0);
+    if (s.block) {
+        cplx tcoeff = std::exp(w * s.step * s.Dt * I);
+        cplx tcoeff2 = std::exp(w * (s.step + 0.5) * s.Dt * I);
+        for (int i = 0; i < s.span1; i++)
+            for (int j = 0; j < s.span2; j++)
+                for (int k = 0; k < s.span3; k++) {
+                    size_t q = (size_t)i + (size_t)s.span1 * ((size_t)j + (size_t)s.span2 * k);
+                    for (int c = 0; c < 3; c++) s.acc[c][q] += g.local(c, i + s.x1, j + s.y1, k + s.z1) * tcoeff;
+                    for (int c = 3; c < 6; c++) s.acc[c][q] += g.local(c, i + s.x1, j + s.y1, k + s.z1) * tcoeff2;
+                }
+    } else {
+        cplx tf_coeff = std::exp(w * s.step * s.Dt * I);
+        const int o = s.mag_map ? 3 : 0;
+        for (int j = 0; j < s.span2; j++)
+            for (int i = 0; i < s.span1; i++) {
+                int a = 0, b = 0, c = 0;
+                if (s.type == AEFDTD_NORMAL_X) { b = i; c = j; }
+                else if (s.type == AEFDTD_NORMAL_Y) { a = i; c = j; }
+                else { a = i; b = j; }
+                double S[3] = {0, 0, 0};
+                if (s.cumulative) {
+                    for (int k = 0; k < s.span3; k++) {
+                        int aa = a, bb = b, cc = c;
+                        if (s.type == AEFDTD_NORMAL_X) aa = k;
+                        else if (s.type == AEFDTD_NORMAL_Y) bb = k;
+                        else cc = k;
+                        for (int q = 0; q < 3; q++) S[q] += g.local(o + q, s.x1 + aa, s.y1 + bb, s.z1 + cc);
+                    }
+                } else {
+                    for (int q = 0; q < 3; q++) S[q] = g.local(o + q, s.x1 + a, s.y1 + b, s.z1 + c);
+                }
+                size_t p = (size_t)i + (size_t)s.span1 * j;
+                for (int q = 0; q < 3; q++) s.acc[q][p] += tf_coeff * S[q];
+            }
+    }
+    s.step += 1;
+}
+
 int aeoracle_feed_sensors(aeoracle_grid *g) {
     for (auto *s : g->sensors) sensor_feed(*g, *s);
+    for (auto *s : g->fsensors) field_feed(*g, *s);
     for (auto *s : g->sums) plane_sum_feed(*g, *s);
     return AEFDTD_OK;
 }
@@ -1447,10 +1590,23 @@ struct SensorSet {
 int add_requests(aeoracle_grid *g, int n, const aefdtd_sensor_req *req, SensorSet &S) {
     for (int r = 0; r < n; r++) {
         const aefdtd_sensor_req &q = req[r];
+        std::vector<int> ids;
+        if (q.kind < AEFDTD_SENSOR_POYNTING || q.kind > AEFDTD_SENSOR_FIELDMAP2) return AEFDTD_EINVAL;
+        if (q.kind == AEFDTD_SENSOR_FIELDBLOCK || q.kind == AEFDTD_SENSOR_FIELDMAP) {
+            // FieldBlock / FieldMap at lambda_min (sensors.cpp:362-378)
+            int id = -1;
+            int rc = q.kind == AEFDTD_SENSOR_FIELDBLOCK
+                         ? aeoracle_add_field_block(g, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, &id)
+                         : aeoracle_add_field_map(g, q.orientation, q.x1, q.x2, q.y1, q.y2, q.z1, q.z2, q.lambda_min, 0, 0, &id);
+            if (rc) return rc;
+            ids.push_back(id);
+            S.ids.push_back(ids);
+            continue;
+        }
+        if (q.Nl < 1) return AEFDTD_EINVAL;
         std::vector<double> L;
         lambda_grid(q.Nl, q.lambda_min, q.lambda_max, L);
-        std::vector<int> ids;
-        if (q.box) {
+        if (q.box && q.kind == AEFDTD_SENSOR_POYNTING) {
             const int t[6] = {AEFDTD_NORMAL_XM, AEFDTD_NORMAL_X, AEFDTD_NORMAL_YM, AEFDTD_NORMAL_Y, AEFDTD_NORMAL_ZM, AEFDTD_NORMAL_Z};
             const int b[6][6] = {{q.x1, q.x1 + 1, q.y1, q.y2, q.z1, q.z2}, {q.x2 - 1, q.x2, q.y1, q.y2, q.z1, q.z2},
                                  {q.x1, q.x2, q.y1, q.y1 + 1, q.z1, q.z2}, {q.x1, q.x2, q.y2 - 1, q.y2, q.z1, q.z2},
@@ -1479,6 +1635,20 @@ void collect_requests(aeoracle_grid *g, int n, const aefdtd_sensor_req *req, Sen
     size_t off = 0;
     for (int r = 0; r < n; r++) {
         int Nl = req[r].Nl;
+        if (req[r].kind == AEFDTD_SENSOR_FIELDBLOCK || req[r].kind == AEFDTD_SENSOR_FIELDMAP) {
+            int dims[4];
+            aeoracle_field_sensor_read(g, S.ids[r][0], out ? out + off : nullptr, dims);
+            off += (size_t)dims[0] * dims[1] * dims[2] * dims[3] * 2;
+            continue;
+        }
+        if (req[r].kind == AEFDTD_SENSOR_FIELDMAP2) {
+            // FieldMap2: the raw spectral accumulators (s_fieldmap.cpp:624-672)
+            int dims[3];
+            aeoracle_sensor_spectra(g, S.ids[r][0], nullptr, dims);
+            if (out) aeoracle_sensor_spectra(g, S.ids[r][0], out + off, dims);
+            off += 6 * (size_t)Nl * dims[0] * dims[1] * 2;
+            continue;
+        }
         std::vector<double> tmp(Nl);
         if (req[r].box) {
             std::vector<double> tot(Nl, 0.0);

@@ -60,6 +60,12 @@ int aeoracle_add_spect_poynting(aeoracle_grid *g, int orientation,
 int aeoracle_add_oscillator(aeoracle_grid *g, int orientation, int x1, int x2, int y1, int y2,
                             int z1, int z2, double lambda_min, double lambda_max, int *id);
 int aeoracle_feed_sensors(aeoracle_grid *g);
+int aeoracle_add_field_block(aeoracle_grid *g, int x1, int x2, int y1, int y2, int z1, int z2,
+                             double lambda, int *id);
+int aeoracle_add_field_map(aeoracle_grid *g, int orientation, int x1, int x2, int y1, int y2,
+                           int z1, int z2, double lambda, int cumulative, int magnetic, int *id);
+int aeoracle_field_sensor_read(aeoracle_grid *g, int id, double *out, int dims[4]);
+int aeoracle_field_sensor_mats(aeoracle_grid *g, int id, uint32_t *out);
 int aeoracle_sensor_poynting(aeoracle_grid *g, int id, double *rsp);
 int aeoracle_sensor_spectra(aeoracle_grid *g, int id, double *out, int dims[3]);
 int aeoracle_add_plane_sum(aeoracle_grid *g, int k_plane, const aefdtd_pulse *inc,

@@ -66,6 +66,12 @@ def test_normal_incidence_files(engine_be, tmp_path):
             cols += [abs(z), cmath.phase(z), (z.real * z.real + z.imag * z.imag) * ratio]
         lines.append(" ".join(g6(c) for c in cols) + "\n")
     assert read(tmp_path / "slab_spectdata_norm2") == "".join(lines)
+    # norm_octave_script (normal_incidence.cpp:24-59): loads spectdata_norm,
+    # T scaled by n_sub / n_sup in the default 6 digits
+    script = read(tmp_path / "slab_show_norm.m")
+    assert script.startswith("function out=slab_show_norm(varargin)\n\nD=load('slab_spectdata_norm');\n")
+    assert f"T={g6(out['index_sub'])}/{g6(out['index_sup'])}*t.^2;\n" in script
+    assert script.endswith("if nargin>0\n    out=[L,R,T,A];\nend\n\nend\n")
 
 
 def test_box_sensor_files(engine_be, tmp_path):

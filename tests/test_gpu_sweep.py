@@ -299,7 +299,24 @@ def _sensor_variety(seed, side, steps, pol):
     return finalized(spec, sens), sens
 
 
-@pytest.mark.parametrize("code", list(range(-1, 12)))
+# time-skew codes: tile height index (0, 1) | depth index (0: 2, 1: 4, 2: 8 steps) << 2;
+# an 8-step skew needs the taller FP64 tile (16 rows leave no interior)
+SKEW_CODES = [0, 1, 4, 5, 8, 9]
+SKEW_CODES_FP64 = [0, 1, 4, 5, 9]
+
+
+def test_batch_time_skew_rejects_unknown_codes(engine_be):
+    for prec, bad in ((capi.FP32, (2, 3, 6, 7, 10, 11, 12, 16, -2)), (capi.FP64, (8, 2, 13, 17))):
+        b = capi.Batch(capi.engine(), prec)
+        try:
+            for code in bad:
+                with pytest.raises(capi.FDTDError):
+                    b.set_skew(code)
+        finally:
+            b.close()
+
+
+@pytest.mark.parametrize("code", [-1] + SKEW_CODES_FP64)
 def test_batch_time_skew_codes_fp64(engine_be, oracle_be, code):
     """Every time-skew code (tile height, depth 2/4/8) and the per-step passes
     give the oracle's fields and spectra on uneven 2D TM/TE runs, including
@@ -308,11 +325,7 @@ def test_batch_time_skew_codes_fp64(engine_be, oracle_be, code):
              _sensor_variety(53, 20, 77, "TE")]
     be = capi.engine()
     b = capi.Batch(be, capi.FP64)
-    try:
-        b.set_skew(code)
-    except capi.FDTDError:
-        b.close()
-        pytest.skip("code not valid for FP64 tiles")
+    b.set_skew(code)
     runs = []
     for spec, sens in cases:
         r = capi.Run(be, capi.KIND_CUSTOM, spec, sens)
@@ -330,6 +343,41 @@ def test_batch_time_skew_codes_fp64(engine_be, oracle_be, code):
         assert spec_err(g["sensors"], r["sensors"]) <= 1e-9
         run.close()
     b.close()
+
+
+def _batch_results(prec, code, cases):
+    be = capi.engine()
+    b = capi.Batch(be, prec)
+    b.set_skew(code)
+    runs = []
+    for spec, sens in cases:
+        r = capi.Run(be, capi.KIND_CUSTOM, spec, sens)
+        b.add(r)
+        runs.append(r)
+    while b.running():
+        b.step(1)
+    out = []
+    for r in runs:
+        b.remove(r)
+        out.append(r.finish(want_fields=True))
+        r.close()
+    b.close()
+    return out
+
+
+@pytest.mark.parametrize("code", SKEW_CODES)
+def test_batch_time_skew_codes_fp32(engine_be, code):
+    """FP32 storage: every time-skew code against the per-step passes on the
+    uneven TM/TE runs (FP32 tiles are twice as tall as FP64 ones)."""
+    cases = [_sensor_variety(51, 36, 203, "TM"), _sensor_variety(52, 150, 131, "TE"),
+             _sensor_variety(53, 20, 77, "TE")]
+    for spec, _ in cases:
+        spec.precision = capi.FP32
+    ref = _batch_results(capi.FP32, -1, cases)
+    got = _batch_results(capi.FP32, code, cases)
+    for a, c in zip(ref, got):
+        assert rel_l2(c["fields"], a["fields"]) <= 1e-5
+        assert spec_err(c["sensors"], a["sensors"]) <= 1e-5
 
 
 @pytest.mark.parametrize("prec,tol", [(capi.FP64, 1e-12), (capi.FP32, 1e-5)])
