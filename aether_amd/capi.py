@@ -159,6 +159,7 @@ def _sig(lib, p):
         "set_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
         "set_step_tiles": (i, [vp, i]),
+        "get_step_tiles": (i, [vp, _ip]),
         "batch_create": (i, [C.POINTER(vp), i]),
         "batch_destroy": (i, [vp]),
         "batch_error": (C.c_char_p, [vp]),

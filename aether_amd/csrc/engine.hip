@@ -231,6 +231,9 @@ Prof<T> prof_of(const aefdtd_grid *g) {
 // EXY,EZY,HXY,HZY (y slabs), EXZ,EYZ,HXZ,HYZ (z slabs).
 int alloc_pml_state(aefdtd_grid *g) {
     for (int s = 0; s < 12; s++) {
+        if (g->psi_alt[s]) g->dev_bytes -= g->psipml_n[s] * g->csize();
+        dfree(g->psi_alt[s]);           // the one-pass step's second set is made again from these
+        if (g->d_psipml[s]) g->dev_bytes -= g->psipml_n[s] * g->csize();
         dfree(g->d_psipml[s]);
         g->psipml_n[s] = 0;
     }
@@ -722,69 +725,124 @@ int launch_H(aefdtd_grid *g, aefdtd_grid *aux = nullptr) {
     return AEFDTD_OK;
 }
 
-// ---- one-pass step (k_step + k_step_edge) of 3D grids
-constexpr int kStepTY[4] = {4, 8, 16, 32};
-constexpr int kStepKZ[4] = {4, 8, 16, 32};
+// ---- one-pass step (k_step) of 3D grids
+constexpr int kStepNW[3] = {4, 8, 16};                // wavefronts per workgroup (NW - 2 rows owned)
+constexpr int kStepKZ[4] = {8, 16, 32, 64};
+constexpr int kStepPsiE[6] = {0, 1, 4, 5, 8, 9};     // E slabs in d_psipml (EYX EZX EXY EZY EXZ EYZ)
 
-// Tile plan of step code cfg on grid g (TY index | (RY - 1) << 2 | KZ index
-// << 4 | XCD fold << 6); ok = false where the one-pass kernel does not apply
-// (2D grids, columns, rows wider than a 1024-lane workgroup).
+// Tile plan of step code cfg on grid g (NW index | KZ index << 2 | XCD fold
+// << 4; tiles of 64 columns, NW - 2 rows and KZ planes); ok = false where the
+// one-pass kernel does not apply (2D grids, columns) or the code names no tile.
 struct StepPlan {
-    int TY = 0, RY = 0, KZ = 0, WX = 0, nty = 0, ntz = 0, remap = 0;
+    int NW = 0, KZ = 0, ntx = 0, nty = 0, ntz = 0, remap = 0;
     bool ok = false;
 };
 
 StepPlan step_plan(const aefdtd_grid *g, int cfg) {
     StepPlan p;
-    if (cfg < 0 || cfg > 127 || g->Nz < 2 || is_column(g)) return p;
-    p.TY = kStepTY[cfg & 3];
-    p.RY = ((cfg >> 2) & 3) + 1;
-    p.KZ = kStepKZ[(cfg >> 4) & 3];
-    p.remap = (cfg >> 6) & 1;
-    p.WX = (g->Nx + 63) / 64;
-    if (p.WX * p.RY > 16 || p.RY > p.TY) return p;
-    p.nty = (g->Ny + p.TY - 1) / p.TY;
+    if (cfg < 0 || cfg > 31 || (cfg & 3) == 3 || g->Nz < 2 || is_column(g)) return p;
+    p.NW = kStepNW[cfg & 3];
+    p.KZ = kStepKZ[(cfg >> 2) & 3];
+    p.remap = (cfg >> 4) & 1;
+    p.ntx = (g->Nx + BX - 1) / BX;
+    p.nty = (g->Ny + p.NW - 3) / (p.NW - 2);
     p.ntz = (g->Nz + p.KZ - 1) / p.KZ;
     p.ok = true;
     return p;
 }
 
-// The two launches of one step: the E + interior-H pass and the edge-H pass
-// (with the TFSF column's H when xh is given).
+// The second set of fields and E CPML psi the one-pass step writes into,
+// allocated at the first one-pass step and filled with the current state (so
+// that entries no step updates, such as the psi of a disabled component, hold
+// the same values in both sets).
+int ensure_step_set(aefdtd_grid *g) {
+    const size_t nf = g->nalloc() * g->ssize();
+    for (int c = 0; c < 6; c++) {
+        if (g->F_alt[c]) continue;
+        int rc = dalloc(g, &g->F_alt[c], nf);
+        if (rc) return rc;
+        HIPOK(g, hipMemcpyAsync(g->F_alt[c], g->F[c], nf, hipMemcpyDeviceToDevice, g->stream));
+    }
+    for (int s : kStepPsiE) {
+        if (g->psi_alt[s] || !g->d_psipml[s]) continue;
+        const size_t n = g->psipml_n[s] * g->csize();
+        int rc = dalloc(g, &g->psi_alt[s], n);
+        if (rc) return rc;
+        HIPOK(g, hipMemcpyAsync(g->psi_alt[s], g->d_psipml[s], n, hipMemcpyDeviceToDevice, g->stream));
+    }
+    return AEFDTD_OK;
+}
+
+void free_step_set(aefdtd_grid *g) {
+    for (void *&p : g->F_alt) {
+        if (p) g->dev_bytes -= g->nalloc() * g->ssize();
+        dfree(p);
+    }
+    for (int s = 0; s < 12; s++) {
+        if (g->psi_alt[s]) g->dev_bytes -= g->psipml_n[s] * g->csize();
+        dfree(g->psi_alt[s]);
+    }
+}
+
+// After a one-pass launch the second set holds the current state.
+void swap_step_set(aefdtd_grid *g) {
+    for (int c = 0; c < 6; c++) std::swap(g->F[c], g->F_alt[c]);
+    for (int s : kStepPsiE) std::swap(g->d_psipml[s], g->psi_alt[s]);
+}
+
 template <typename S, typename T>
-void launch_step_pair(aefdtd_grid *g, const StepPlan &p, const EArgs<S, T> &a, const HArgs<S, T> &h,
-                      const HArgs<S, T> *xh, bool en) {
-    const StepGeom sg{p.WX, p.RY, p.TY, p.KZ, p.nty, p.ntz, p.remap};
-    const int nt = p.nty * p.ntz;
-    const size_t lds = (size_t)a.nmat * sizeof(MatL<T>);
-    const dim3 blk(64 * p.WX * p.RY);
-    hipEvent_t ev;
-    prof_begin(g, 0, &ev);
-    bool done = false;
+StepOut<S, T> step_out(const aefdtd_grid *g) {
+    StepOut<S, T> o;
+    Fld<S, T> *f[6] = {&o.Ex, &o.Ey, &o.Ez, &o.Hx, &o.Hy, &o.Hz};
+    for (int c = 0; c < 6; c++) {
+        *f[c] = fld<S, T>(g, c);
+        f[c]->p = (S *)g->F_alt[c];
+    }
+    o.pEyx = (T *)g->psi_alt[0]; o.pEzx = (T *)g->psi_alt[1];
+    o.pExy = (T *)g->psi_alt[4]; o.pEzy = (T *)g->psi_alt[5];
+    o.pExz = (T *)g->psi_alt[8]; o.pEyz = (T *)g->psi_alt[9];
+    return o;
+}
+
+template <typename S, typename T, int NW>
+void launch_step_nw(const StepPlan &p, const EArgs<S, T> &a, const HArgs<S, T> &h, const StepOut<S, T> &o, bool en,
+                    hipStream_t st) {
+    const StepGeom sg{p.ntx, p.nty, p.ntz, p.KZ, p.remap};
+    const dim3 grid((unsigned)(p.ntx * p.nty * p.ntz)), blk(BX, NW);
+    // LDS: rows [2][NW-1][2][64] and halo column [2][2][64] of T, then the materials
+    const size_t lds = (size_t)(4 * (NW - 1) + 4) * BX * sizeof(T) + (size_t)a.nmat * sizeof(MatL<T>);
     if constexpr (std::is_same_v<S, T>) {
         if (en) {
-            hipLaunchKernelGGL((k_step<S, T, true>), dim3(nt), blk, lds, g->stream, a, h, sg);
-            done = true;
+            hipLaunchKernelGGL((k_step<S, T, NW, true>), grid, blk, lds, st, a, h, o, sg);
+            return;
         }
     }
-    if (!done) hipLaunchKernelGGL((k_step<S, T, false>), dim3(nt), blk, lds, g->stream, a, h, sg);
+    hipLaunchKernelGGL((k_step<S, T, NW, false>), grid, blk, lds, st, a, h, o, sg);
+}
+
+// One one-pass launch from the current set into the second, then the swap.
+template <typename S, typename T>
+void launch_step(aefdtd_grid *g, const StepPlan &p, const EArgs<S, T> &a, const HArgs<S, T> &h, bool en) {
+    const StepOut<S, T> o = step_out<S, T>(g);
+    hipEvent_t ev;
+    prof_begin(g, 0, &ev);
+    switch (p.NW) {
+        case 4: launch_step_nw<S, T, 4>(p, a, h, o, en, g->stream); break;
+        case 8: launch_step_nw<S, T, 8>(p, a, h, o, en, g->stream); break;
+        default: launch_step_nw<S, T, 16>(p, a, h, o, en, g->stream); break;
+    }
     prof_end(g, ev);
-    const EdgeGeom eg{p.TY, p.KZ, p.nty, p.ntz, (long)p.ntz * g->Ny + (long)p.nty * (g->Nz - p.ntz)};
-    const long nb = (eg.nrows + 3) / 4;
-    const long naux = xh ? (xh->Nz + 255) / 256 : 0;
-    prof_begin(g, 1, &ev);
-    hipLaunchKernelGGL((k_step_edge<S, T>), dim3((unsigned)(nb + naux), (unsigned)p.WX), dim3(64, 4), 0, g->stream,
-                       h, eg, nb, xh ? *xh : h);
-    prof_end(g, ev);
+    swap_step_set(g);
 }
 
 // One full step of g (and of its TFSF column aux) by the one-pass kernel:
-// the column's E update first (the main grid's H reads its new E), then the
-// RC pre-pass (unless the last sensor feed ran it), the two launches.
+// the column's E update first (the main grid's H update reads its new E),
+// the RC pre-pass (unless the last sensor feed ran it), the one-pass launch,
+// then the column's H update (the main grid's E update read its old H).
 template <typename S, typename T>
 int step_T(aefdtd_grid *g, aefdtd_grid *aux, const StepPlan &p) {
     int rc = ensure_osc_tables(g, g->tstep + 1);
-    if (rc) return rc;
+    if (rc || (rc = ensure_step_set(g))) return rc;
     if (aux) {
         if ((rc = ensure_osc_tables(aux, aux->tstep + 1)) || (rc = launch_rc<S, T>(aux))) return rc;
         const EArgs<S, T> x = e_args<S, T>(aux);
@@ -795,13 +853,11 @@ int step_T(aefdtd_grid *g, aefdtd_grid *aux, const StepPlan &p) {
     g->rc_ahead = false;
     EArgs<S, T> a = e_args<S, T>(g);
     const HArgs<S, T> h = h_args<S, T>(g);
-    HArgs<S, T> xh{};
-    if (aux) xh = h_args<S, T>(aux);
     g->en_parts = 0;
     bool en = false;
     if constexpr (std::is_same_v<S, T>) {
         if (g->en_fuse) {
-            const int nt = p.nty * p.ntz;
+            const int nt = p.ntx * p.nty * p.ntz;
             if (nt > g->en_part_cap) {
                 dfree_t(g->d_en_part);
                 g->en_part_cap = 0;
@@ -815,7 +871,11 @@ int step_T(aefdtd_grid *g, aefdtd_grid *aux, const StepPlan &p) {
             en = true;
         }
     }
-    launch_step_pair<S, T>(g, p, a, h, aux ? &xh : nullptr, en);
+    launch_step<S, T>(g, p, a, h, en);
+    if (aux) {
+        const HArgs<S, T> x = h_args<S, T>(aux);
+        hipLaunchKernelGGL((k_update_H_col<S, T>), dim3((aux->Nz + 255) / 256), dim3(256), 0, g->stream, x);
+    }
     HIPOK(g, hipGetLastError());
     g->strip_valid = h.nstrip > 0;
     return AEFDTD_OK;
@@ -841,20 +901,21 @@ float time_cfg(aefdtd_grid *g, bool e_pass, int cfg, int reps, hipEvent_t ea, hi
     return ms;
 }
 
-// Times one tile plan of the one-pass step: reps step launches between events.
+// Times one tile plan of the one-pass step: 2 reps launches between events
+// (an even number, so that the current set is where it was), per step.
 template <typename S, typename T>
 float time_step(aefdtd_grid *g, const StepPlan &p, int reps, hipEvent_t ea, hipEvent_t eb) {
     (void)hipEventRecord(ea, g->stream);
-    for (int r = 0; r < reps; r++) {
+    for (int r = 0; r < 2 * reps; r++) {
         const EArgs<S, T> a = e_args<S, T>(g);
         const HArgs<S, T> h = h_args<S, T>(g);
-        launch_step_pair<S, T>(g, p, a, h, nullptr, false);
+        launch_step<S, T>(g, p, a, h, false);
     }
     (void)hipEventRecord(eb, g->stream);
     (void)hipEventSynchronize(eb);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ea, eb);
-    return ms;
+    return ms / 2;
 }
 
 // Project autotuner: runs every compiled (BY, KZ, remap) tile of the E and H
@@ -892,19 +953,22 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
         }
         bsum += bt;
     }
-    // the one-pass step against the best pair of passes: rows per tile 8/16,
-    // 2/3 rows in flight, 4/8 planes, XCD fold on
+    // every tile of the one-pass step (3D grids) against the best pair of passes
     int best_step = -1;
-    for (int ty = 1; ty <= 2; ty++)
-        for (int ry = 2; ry <= 3; ry++)
-            for (int kz = 0; kz <= 1; kz++) {
-                const int code = ty | ((ry - 1) << 2) | (kz << 4) | (1 << 6);
-                const StepPlan p = step_plan(g, code);
-                if (!p.ok) continue;
-                time_step<S, T>(g, p, 1, ea, eb);
-                const float ms = time_step<S, T>(g, p, reps, ea, eb);
-                if (ms < bsum) { bsum = ms; best_step = code; }
-            }
+    if (step_plan(g, 0).ok) {
+        int rc = ensure_step_set(g);
+        if (rc) {
+            for (void *&p : snap) dfree(p);
+            return rc;
+        }
+        for (int code = 0; code < 32; code++) {
+            const StepPlan p = step_plan(g, code);
+            if (!p.ok) continue;
+            time_step<S, T>(g, p, 1, ea, eb);
+            const float ms = time_step<S, T>(g, p, reps, ea, eb);
+            if (ms < bsum) { bsum = ms; best_step = code; }
+        }
+    }
     (void)hipEventDestroy(ea);
     (void)hipEventDestroy(eb);
     for (size_t q = 0; q < state.size(); q++) {
@@ -920,6 +984,7 @@ int autotune_T(aefdtd_grid *g, int reps, int out[2]) {
     g->tile_e = best[0];
     g->tile_h = best[1];
     g->step_cfg = best_step;
+    if (best_step < 0) free_step_set(g);         // the second set only serves the one-pass step
     if (out) { out[0] = best[0]; out[1] = best[1]; }
     return AEFDTD_OK;
 }
@@ -1761,6 +1826,8 @@ int aefdtd_destroy(aefdtd_grid *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     prof_collect(g);
     for (int c = 0; c < 6; c++) dfree(g->F[c]);
+    for (void *&p : g->F_alt) dfree(p);
+    for (void *&p : g->psi_alt) dfree(p);
     dfree_t(g->d_mat);
     dfree(g->d_mtab);
     dfree_t(g->d_poles);
@@ -1997,8 +2064,14 @@ extern "C" int aefdtd_step(aefdtd_grid *g, int n) {
     return AEFDTD_OK;
 }
 
+extern "C" int aefdtd_get_step_tiles(aefdtd_grid *g, int *cfg) {
+    if (!g || !cfg) return AEFDTD_EINVAL;
+    *cfg = g->step_cfg;
+    return AEFDTD_OK;
+}
+
 extern "C" int aefdtd_set_step_tiles(aefdtd_grid *g, int cfg) {
-    if (!g || cfg < -1 || cfg > 127) return AEFDTD_EINVAL;
+    if (!g || cfg < -1 || cfg > 31 || (cfg >= 0 && (cfg & 3) == 3)) return AEFDTD_EINVAL;
     g->step_cfg = cfg;
     return AEFDTD_OK;
 }
@@ -2009,6 +2082,8 @@ int aefdtd_reset_fields(aefdtd_grid *g) {   // core/fdtd_utils.cpp:855-881
     if (!g) return AEFDTD_EINVAL;
     g->strip_valid = false;
     g->rc_ahead = false;
+    HIPOK(g, hipStreamSynchronize(g->stream));
+    free_step_set(g);                   // made again from the zeroed state at the next one-pass step
     size_t N = g->nalloc();
     for (int c = 0; c < 6; c++) HIPOK(g, hipMemsetAsync(g->F[c], 0, N * g->ssize(), g->stream));
     for (int s = 0; s < 12; s++)

@@ -203,10 +203,14 @@ struct aefdtd_grid {
     int en_parts = 0;                 // partials the last E launch wrote (0: none)
     // tile configuration of the 3D passes (BY index | KZ index << 2 | remap << 4)
     int tile_e = 1 | (2 << 2) | (1 << 4), tile_h = 1 | (2 << 2) | (1 << 4);
-    // one-pass step of 3D grids (k_step + k_step_edge): tile code (TY index |
-    // (RY - 1) << 2 | KZ index << 4 | XCD fold << 6), -1 = the E and H passes
-    // (the default; the autotuner picks the one-pass step where it is faster)
+    // one-pass step of 3D grids (k_step): tile code (BY index | KZ index << 2 |
+    // XCD fold << 4), -1 = the E and H passes (the default; the autotuner picks
+    // the one-pass step where it is faster)
     int step_cfg = -1;
+    // the set the one-pass step writes (fields, E CPML psi at the d_psipml
+    // slots 0 1 4 5 8 9); swapped with F / d_psipml after each such step
+    void *F_alt[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *psi_alt[12] = {};
     // bookkeeping
     bool ready = false;               // device copies up to date
     bool rc_ahead = false;            // the last sensor feed also ran the next E pass's RC pre-pass

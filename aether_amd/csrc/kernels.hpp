@@ -2119,96 +2119,155 @@ __global__ __launch_bounds__(256) void k_energy_fin(const double *part, int n, d
 }
 
 // ================================================================ one-pass step
-// A full time step of a 3D grid in one pass, in place: E of plane k, then H
-// of plane k-1.  A workgroup owns whole x rows (WX wavefronts of 64 lanes
-// cover Nx, so the x wrap stays inside it), TY rows in y (lane row ry of RY
-// takes rows ty0 + ry, ty0 + ry + RY, ...) and KZ planes, and marches z:
+// A full time step of a 3D grid in one pass (E, then H from the new E) that
+// reads the fields of step n from one set of arrays and writes those of step
+// n+1 to a second set, which the host then takes as the current one.  No
+// workgroup ever reads a value that another has already advanced, so the
+// tiles need no ordering.
 //
-//   iteration k:  A) E of plane k, every row of the tile (stencil + TFSF + RC
-//                    increment + CPML + PEC + injection + sources, as e_cell)
-//                 -- workgroup barrier --
-//                 B) H of plane k-1, every row but the tile's last (as h_cell)
+// A workgroup of NW wavefronts owns a tile of 64 columns (aligned to the
+// 64-cell row pitch, so that every row access is whole cache lines), R = NW-2
+// rows and planes k0 .. k1-1.  Wavefronts 0..R-1 take its rows, wavefront R
+// the row after them (the +y halo) and wavefront NW-1 the column after them
+// (the +x halo, one lane per row); every wavefront marches planes k0 .. k1,
+// the last being the +z halo.  The H update of an owned cell needs E^{n+1} at
+// i+1, j+1 and k+1, so the tile computes E^{n+1} of its halo cells as well
+// (their owners compute them too, from the same inputs with the same
+// operations, so both agree bit for bit) and discards it.  Per plane k:
 //
-// Phase B reads E of planes k-1 and k and the old H of plane k-1, all touched
-// by the same workgroup one or two phases before, so they come from the
-// caches: HBM sees each field read once and written once per step (the two
-// passes read and write E and H, then read both again).  The new E is stored
-// normally (it is re-read at once), H with streaming stores.
+//   E^{n+1}(k) of every cell of the tile and its halo: E^n, H^n of the cell,
+//     H^n at i-1 and j-1 from memory, H^n at k-1 carried from plane k-1
+//   Ex, Ez of E^{n+1}(k) of each row, Ey, Ez of the halo column, to LDS
+//   H^{n+1}(k-1) of the owned cells: E^{n+1}(k-1) and E^{n+1}(k) of the cell
+//     from registers, i+1 by a lane shuffle (the last owned lane: the halo
+//     column in LDS), j+1 from the LDS copy of plane k-1
+//   one workgroup barrier (LDS is double buffered by plane parity)
 //
-// The H of the tile's last row and last plane (its "edge" cells) needs E of
-// the next tiles' first row / plane, which other workgroups produce at their
-// own pace; those cells are left to k_step_edge, run after this pass.  The
-// edge cells are exactly the H values other tiles read here (the j-1 row and
-// k-1 plane of a tile's E update), so no tile reads an H value another has
-// already advanced.  Inside a workgroup, H of plane k is written in phase B
-// of iteration k+1, after the barrier that every wavefront reaches only once
-// it has finished phase A of iteration k+1 (which reads the last old H of
-// plane k: the z neighbour of plane k+1).  Every cell sees the same inputs and
-// the same operations as in the two-pass path, so results are bit-identical
-// to k_update_E + k_update_H.
+// HBM sees each field read once and written once per step, 12 values and the
+// material id per cell against 18 values and the id of the E and H passes;
+// the halo column, row and plane are read again by their tiles, mostly from
+// the caches.  E's CPML psi are also read from one set and written to the
+// other (a halo cell's psi belongs to its owner); H's are updated in place,
+// since only an H cell's owner updates it.  Every cell sees the inputs and
+// operations of k_update_E + k_update_H, so the results are bit-identical to
+// the two passes.
 
+// Fields and E CPML psi of step n+1 (the second set; same layouts).
+template <typename S, typename T>
+struct StepOut {
+    Fld<S, T> Ex, Ey, Ez, Hx, Hy, Hz;
+    T *pEyx, *pEzx, *pExy, *pEzy, *pExz, *pEyz;
+};
+
+// Tiles of the one-pass step: ntx x nty x ntz, KZ planes per tile, XCD fold.
 struct StepGeom {
-    int WX, RY, TY, KZ, nty, ntz, remap;
+    int ntx, nty, ntz, KZ, remap;
 };
 
-// Rows of the edge pass: z-edge planes (the last plane of each z tile, every
-// row) then the y-edge rows (the last row of each y tile) of the other planes.
-struct EdgeGeom {
-    int TY, KZ, nty, ntz;
-    long nrows;
+// A CPML psi of step n held in a register; its update goes to the step n+1
+// set, and only where the tile owns the cell (g null: a halo cell).
+template <typename T>
+struct PsiNext {
+    T &r;
+    T *g;
+    __device__ __forceinline__ operator T() const { return r; }
+    __device__ __forceinline__ void operator=(T v) const {
+        r = v;
+        if (g) *g = v;
+    }
 };
 
-template <typename S, typename T, bool EN>
-__global__ __launch_bounds__(1024) void k_step(const EArgs<S, T> a, const HArgs<S, T> h, const StepGeom g) {
+// Boundary data of an E cell in the one-pass step: psi loaded ahead from the
+// step n set, updates written to the step n+1 set at the same offset.
+template <typename S, typename T>
+struct NextPsiE {
+    const EArgs<S, T> &a;
+    const StepOut<S, T> &o;
+    int i, j, k;
+    bool own;
+    PsiSetE<T> &q;
+    T bx_, cx_, by_, cy_;
+    __device__ __forceinline__ PsiAddrE<S, T> ad() const { return PsiAddrE<S, T>{a, i, j, k}; }
+    __device__ __forceinline__ PsiNext<T> Exy(int n) const { return {q.xy, own ? o.pExy + (ad().xy(n) - a.pExy) : nullptr}; }
+    __device__ __forceinline__ PsiNext<T> Exz(int n) const { return {q.xz, own ? o.pExz + (ad().xz(n) - a.pExz) : nullptr}; }
+    __device__ __forceinline__ PsiNext<T> Eyx(int n) const { return {q.yx, own ? o.pEyx + (ad().yx(n) - a.pEyx) : nullptr}; }
+    __device__ __forceinline__ PsiNext<T> Eyz(int n) const { return {q.yz, own ? o.pEyz + (ad().yz(n) - a.pEyz) : nullptr}; }
+    __device__ __forceinline__ PsiNext<T> Ezx(int n) const { return {q.zx, own ? o.pEzx + (ad().zx(n) - a.pEzx) : nullptr}; }
+    __device__ __forceinline__ PsiNext<T> Ezy(int n) const { return {q.zy, own ? o.pEzy + (ad().zy(n) - a.pEzy) : nullptr}; }
+    __device__ __forceinline__ T bx() const { return bx_; }
+    __device__ __forceinline__ T cx() const { return cx_; }
+    __device__ __forceinline__ T by() const { return by_; }
+    __device__ __forceinline__ T cy() const { return cy_; }
+};
+
+template <typename S, typename T, int NW, bool EN>
+__global__ __launch_bounds__(BX *NW) void k_step(const EArgs<S, T> a, const HArgs<S, T> h, const StepOut<S, T> o,
+                                                 const StepGeom g) {
+    constexpr int R = NW - 2;                                  // rows a tile owns
     extern __shared__ char lds_raw[];
-    MatL<T> *sm = reinterpret_cast<MatL<T> *>(lds_raw);
+    T *xz = reinterpret_cast<T *>(lds_raw);                    // [2 planes][R+1 rows][Ex, Ez][64]
+    T *yzc = xz + 2 * (R + 1) * 2 * BX;                        // [2 planes][Ey, Ez][64]: the halo column
+    MatL<T> *sm = reinterpret_cast<MatL<T> *>(yzc + 2 * 2 * BX);
     stage_mats(a, sm);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int xw = w % g.WX, ry = w / g.WX;
-    int bxd, by, bz;
-    tile_of(blockIdx.x, 1, g.nty, g.ntz, g.remap, bxd, by, bz);
-    const int ty0 = by * g.TY, ty1 = min(ty0 + g.TY, a.Ny);
-    const int k0 = bz * g.KZ, k1 = min(k0 + g.KZ, a.Nz);
-    const int i = 64 * xw + lane;
-    const bool iv = i < a.Nx;
-    const int il = iv ? i : a.Nx - 1;                           // loads of idle lanes stay in bounds
+    const int lane = threadIdx.x, w = threadIdx.y;
+    int bx, by, bz;
+    tile_of(blockIdx.x, g.ntx, g.nty, g.ntz, g.remap, bx, by, bz);
+    const int x0 = bx * BX, y0 = by * R, k0 = bz * g.KZ;
+    const int xe = min(x0 + BX, a.Nx), ye = min(y0 + R, a.Ny), ke = min(k0 + g.KZ, a.Nz);
+    const int xh = (xe == a.Nx) ? 0 : xe, yh = (ye == a.Ny) ? 0 : ye;   // halo column and row (wrapping)
+    const int nr = ye - y0, lastx = xe - 1 - x0;              // owned rows, last owned lane
+    // the lane's cell: owned, halo or idle (an idle lane computes on an
+    // in-bounds cell and stores nothing)
+    const bool colw = w == NW - 1;                             // the halo-column wavefront
+    int i, j;
+    bool own;
+    if (colw) {
+        i = xh;
+        j = lane < nr ? y0 + lane : y0;
+        own = false;
+    } else {
+        i = lane <= lastx ? x0 + lane : x0;
+        j = w < nr ? y0 + w : (w == nr ? yh : y0);
+        own = lane <= lastx && w < nr;
+    }
     const int NxNy = a.P * a.Ny;
-    const int oim = (il == 0) ? a.Nx - 1 : -1;                  // (i-1) with wrap
-    const int oip = (il == a.Nx - 1) ? -(a.Nx - 1) : 1;         // (i+1) with wrap
-    const int sideE = side_block(a, 0, 0, a.Nx, ty0, ty1, k0, k1);
-    const int sideH = side_block(h, 1, 0, a.Nx, ty0, ty1, k0, k1);
-    const T ikxE = a.pf.ikE[0][il], ikxH = h.pf.ikH[0][il];
-    const int nxE = slabE(il, a.Nx, a.pml[0], a.pml[1]), nxH = slabH(il, a.Nx, h.pml[0], h.pml[1]);
-    // the lane's x CPML profile values (loop invariant)
-    T bxE = 0, cxE = 0, bxH = 0, cxH = 0;
-    if ((sideE & SIDE_X) && nxE >= 0) { bxE = a.pf.bE[0][il]; cxE = a.pf.cE[0][il]; }
-    if ((sideH & SIDE_X) && nxH >= 0) { bxH = h.pf.bH[0][il]; cxH = h.pf.cH[0][il]; }
-    // strip slots of the lane's column (sensor planes normal to x)
+    const int oim = (i == 0) ? a.Nx - 1 : -1;                  // (i-1) with wrap
+    const int ojm = (j == 0) ? (a.Ny - 1) * a.P : -a.P;        // (j-1) with wrap
+    // side masks: E over the tile and its halo (every treatment where a halo
+    // wraps around the grid), H over the owned cells
+    const bool wraps = xe == a.Nx || ye == a.Ny || ke == a.Nz;
+    const int sideE = wraps ? side_all(a) : side_block(a, 0, x0, xe + 1, y0, ye + 1, k0, ke + 1);
+    const int sideH = side_block(h, 1, x0, xe, y0, ye, k0, ke);
+    const T ikxE = a.pf.ikE[0][i], ikyE = a.pf.ikE[1][j], ikxH = h.pf.ikH[0][i], ikyH = h.pf.ikH[1][j];
+    const int nxE = slabE(i, a.Nx, a.pml[0], a.pml[1]), nyE = slabE(j, a.Ny, a.pml[2], a.pml[3]);
+    const int nxH = slabH(i, a.Nx, h.pml[0], h.pml[1]), nyH = slabH(j, a.Ny, h.pml[2], h.pml[3]);
+    T bxE = 0, cxE = 0, byE = 0, cyE = 0, bxH = 0, cxH = 0, byH = 0, cyH = 0;
+    if ((sideE & SIDE_X) && nxE >= 0) { bxE = a.pf.bE[0][i]; cxE = a.pf.cE[0][i]; }
+    if ((sideE & SIDE_Y) && nyE >= 0) { byE = a.pf.bE[1][j]; cyE = a.pf.cE[1][j]; }
+    if ((sideH & SIDE_X) && nxH >= 0) { bxH = h.pf.bH[0][i]; cxH = h.pf.cH[0][i]; }
+    if ((sideH & SIDE_Y) && nyH >= 0) { byH = h.pf.bH[1][j]; cyH = h.pf.cH[1][j]; }
+    // strip slots (sensor planes normal to x) of an owned lane's column
     int sq0 = -1, sq1 = -1;
-    for (int q = 0; q < h.nstrip; q++) {
-        const int x0 = h.scol[q], x1 = (x0 + 1 == a.Nx) ? 0 : x0 + 1;
-        const int s = (il == x0) ? 2 * q : (il == x1 ? 2 * q + 1 : -1);
+    for (int q = 0; own && q < h.nstrip; q++) {
+        const int c0 = h.scol[q], c1 = (c0 + 1 == a.Nx) ? 0 : c0 + 1;
+        const int s = (i == c0) ? 2 * q : (i == c1 ? 2 * q + 1 : -1);
         if (s >= 0) {
             if (sq0 < 0) sq0 = s;
             else sq1 = s;
         }
     }
-    // a row's loads, with the CPML psi of its cells (requested with the
-    // fields, one row ahead of their use)
-    struct LdE {
-        T hx, hy, hz, hy_im, hz_im, hx_jm, hz_jm, hx_km, hy_km, ex, ey, ez;
+    // one plane's loads: the E update's, and the CPML psi of the H update of
+    // the plane before (requested one plane ahead of their use)
+    struct Ld {
+        T hx, hy, hz, hy_im, hz_im, hx_jm, hz_jm, ex, ey, ez;
         int m;
-        PsiSetE<T> q;
+        PsiSetE<T> qe;
+        PsiSetH<T> qh;
     };
-    struct LdH {
-        T hx, hy, hz, ex, ey, ez, ey_ip, ez_ip, ex_jp, ez_jp, ex_kp, ey_kp;
-        PsiSetH<T> q;
-    };
-    auto loadE = [&](int j, int k) {
-        const int id = il + a.P * j + k * NxNy;
-        const int ojm = (j == 0) ? (a.Ny - 1) * a.P : -a.P;
-        const int okm = (k == 0) ? (a.Nz - 1) * NxNy : -NxNy;
-        LdE v;
+    auto load = [&](int k) {
+        const int kk = (k == a.Nz) ? 0 : k;
+        const int id = i + a.P * j + kk * NxNy;
+        Ld v;
         v.hx = a.Hx[id];
         v.hy = a.Hy[id];
         v.hz = a.Hz[id];
@@ -2216,206 +2275,137 @@ __global__ __launch_bounds__(1024) void k_step(const EArgs<S, T> a, const HArgs<
         v.hz_im = a.Hz[id + oim];
         v.hx_jm = a.Hx[id + ojm];
         v.hz_jm = a.Hz[id + ojm];
-        v.hx_km = a.Hx[id + okm];
-        v.hy_km = a.Hy[id + okm];
-        v.ex = a.Ex.nt(id);                                     // read once per step: streamed
+        v.ex = a.Ex.nt(id);
         v.ey = a.Ey.nt(id);
         v.ez = a.Ez.nt(id);
         v.m = __builtin_nontemporal_load(a.mat + id);
-        v.q = PsiSetE<T>{};
-        const PsiAddrE<S, T> ad{a, il, j, k};
+        v.qe = PsiSetE<T>{};
+        const PsiAddrE<S, T> ae{a, i, j, kk};
         if ((sideE & SIDE_X) && nxE >= 0) {
-            if (a.en & 2) v.q.yx = *ad.yx(nxE);
-            if (a.en & 4) v.q.zx = *ad.zx(nxE);
+            if (a.en & 2) v.qe.yx = *ae.yx(nxE);
+            if (a.en & 4) v.qe.zx = *ae.zx(nxE);
         }
-        if (sideE & SIDE_Y) {
-            const int ny_ = slabE(j, a.Ny, a.pml[2], a.pml[3]);
-            if (ny_ >= 0) {
-                if (a.en & 1) v.q.xy = *ad.xy(ny_);
-                if (a.en & 4) v.q.zy = *ad.zy(ny_);
-            }
+        if ((sideE & SIDE_Y) && nyE >= 0) {
+            if (a.en & 1) v.qe.xy = *ae.xy(nyE);
+            if (a.en & 4) v.qe.zy = *ae.zy(nyE);
         }
         if (sideE & SIDE_Z) {
-            const int nz_ = slabEz(a, k);
+            const int nz_ = slabEz(a, kk);
             if (nz_ >= 0) {
-                if (a.en & 1) v.q.xz = *ad.xz(nz_);
-                if (a.en & 2) v.q.yz = *ad.yz(nz_);
+                if (a.en & 1) v.qe.xz = *ae.xz(nz_);
+                if (a.en & 2) v.qe.yz = *ae.yz(nz_);
+            }
+        }
+        v.qh = PsiSetH<T>{};
+        if (k > k0) {
+            const PsiAddrH<S, T> ah{h, i, j, k - 1};
+            if ((sideH & SIDE_X) && nxH >= 0) {
+                if (h.en & 16) v.qh.yx = *ah.yx(nxH);
+                if (h.en & 32) v.qh.zx = *ah.zx(nxH);
+            }
+            if ((sideH & SIDE_Y) && nyH >= 0) {
+                if (h.en & 8) v.qh.xy = *ah.xy(nyH);
+                if (h.en & 32) v.qh.zy = *ah.zy(nyH);
+            }
+            if (sideH & SIDE_Z) {
+                const int nz_ = slabH(k - 1, h.Nz, h.pml[4], h.pml[5]);
+                if (nz_ >= 0) {
+                    if (h.en & 8) v.qh.xz = *ah.xz(nz_);
+                    if (h.en & 16) v.qh.yz = *ah.yz(nz_);
+                }
             }
         }
         return v;
     };
-    auto loadH = [&](int j, int k) {
-        const int id = il + a.P * j + k * NxNy;
-        const int ojp = (j == a.Ny - 1) ? -(a.Ny - 1) * a.P : a.P;
-        const int okp = (k == a.Nz - 1) ? -(a.Nz - 1) * NxNy : NxNy;
-        LdH v;
-        v.hx = h.Hx[id];
-        v.hy = h.Hy[id];
-        v.hz = h.Hz[id];
-        v.ex = h.Ex[id];
-        v.ey = h.Ey[id];
-        v.ez = h.Ez[id];
-        v.ey_ip = h.Ey[id + oip];
-        v.ez_ip = h.Ez[id + oip];
-        v.ex_jp = h.Ex[id + ojp];
-        v.ez_jp = h.Ez[id + ojp];
-        v.ex_kp = h.Ex[id + okp];
-        v.ey_kp = h.Ey[id + okp];
-        v.q = PsiSetH<T>{};
-        const PsiAddrH<S, T> ad{h, il, j, k};
-        if ((sideH & SIDE_X) && nxH >= 0) {
-            if (h.en & 16) v.q.yx = *ad.yx(nxH);
-            if (h.en & 32) v.q.zx = *ad.zx(nxH);
-        }
-        if (sideH & SIDE_Y) {
-            const int ny_ = slabH(j, h.Ny, h.pml[2], h.pml[3]);
-            if (ny_ >= 0) {
-                if (h.en & 8) v.q.xy = *ad.xy(ny_);
-                if (h.en & 32) v.q.zy = *ad.zy(ny_);
-            }
-        }
-        if (sideH & SIDE_Z) {
-            const int nz_ = slabH(k, h.Nz, h.pml[4], h.pml[5]);
-            if (nz_ >= 0) {
-                if (h.en & 8) v.q.xz = *ad.xz(nz_);
-                if (h.en & 16) v.q.yz = *ad.yz(nz_);
-            }
-        }
-        return v;
-    };
+    // H^n at k0-1 (z neighbour of the first plane), then carried
+    const int km0 = (k0 == 0) ? a.Nz - 1 : k0 - 1;
+    T hkx = a.Hx[i + a.P * j + km0 * NxNy], hky = a.Hy[i + a.P * j + km0 * NxNy];
+    // E^{n+1} and H^n of the cell at the plane before
+    T pex = 0, pey = 0, pez = 0, phx = 0, phy = 0, phz = 0;
     T en = 0;
-    const int jA0 = ty0 + ry;
-    for (int k = k0; k < k1; k++) {
-        // ---- A: E of plane k (rows jA0, jA0 + RY, ...), the next row's loads in flight
-        const T ikzE = uload(a.pf.ikE[2], k);
-        if (jA0 < ty1) {
-            LdE cur = loadE(jA0, k);
-            for (int j = jA0; j < ty1; j += g.RY) {
-                const int jn = j + g.RY;
-                LdE nxt = cur;
-                if (jn < ty1) nxt = loadE(jn, k);
-                if (iv) {
-                    const int id = il + a.P * j + k * NxNy;
-                    T byE = 0, cyE = 0;
-                    const int nyE = slabE(j, a.Ny, a.pml[2], a.pml[3]);
-                    if ((sideE & SIDE_Y) && nyE >= 0) { byE = uload(a.pf.bE[1], j); cyE = uload(a.pf.cE[1], j); }
-                    const V3<T> e = e_cell_vals(a, sm[cur.m], il, j, k, id, cur.ex, cur.ey, cur.ez, cur.hz - cur.hz_jm,
-                                                cur.hy - cur.hy_km, cur.hx - cur.hx_km, cur.hz - cur.hz_im,
-                                                cur.hy - cur.hy_im, cur.hx - cur.hx_jm, ikxE, uload(a.pf.ikE[1], j),
-                                                ikzE, nxE, nyE, a.ostep, sideE,
-                                                PrefPsiE<S, T>{a, il, j, k, cur.q, bxE, cxE, byE, cyE});
-                    a.Ex.set(id, e.x);
-                    a.Ey.set(id, e.y);
-                    a.Ez.set(id, e.z);
-                    if constexpr (EN) en += e.x * e.x + e.y * e.y + e.z * e.z;
+    auto body = [&](Ld &c, int k) {
+        const int kk = (k == a.Nz) ? 0 : k;
+        const int id = i + a.P * j + kk * NxNy;
+        const bool ownE = own && k < ke;
+        V3<T> e = e_cell_vals(a, sm[c.m], i, j, kk, id, c.ex, c.ey, c.ez, c.hz - c.hz_jm, c.hy - hky, c.hx - hkx,
+                              c.hz - c.hz_im, c.hy - c.hy_im, c.hx - c.hx_jm, ikxE, ikyE, uload(a.pf.ikE[2], kk), nxE,
+                              nyE, a.ostep, sideE, NextPsiE<S, T>{a, o, i, j, kk, ownE, c.qe, bxE, cxE, byE, cyE});
+        hkx = c.hx;
+        hky = c.hy;
+        if (ownE) {
+            o.Ex.set_nt(id, e.x);
+            o.Ey.set_nt(id, e.y);
+            o.Ez.set_nt(id, e.z);
+            if constexpr (EN) en += e.x * e.x + e.y * e.y + e.z * e.z;
+        }
+        // the H update reads E as stored
+        e.x = stored(o.Ex, e.x);
+        e.y = stored(o.Ey, e.y);
+        e.z = stored(o.Ez, e.z);
+        if (colw) {
+            T *col = yzc + (k & 1) * 2 * BX;
+            col[lane] = e.y;
+            col[BX + lane] = e.z;
+        } else if (w <= R) {
+            T *row = xz + ((k & 1) * (R + 1) + w) * 2 * BX;
+            row[lane] = e.x;
+            row[BX + lane] = e.z;
+        }
+        if (k > k0) {
+            T ey_ip = __shfl_down(pey, 1, 64), ez_ip = __shfl_down(pez, 1, 64);
+            if (own) {
+                const int kh = k - 1;
+                if (lane == lastx) {
+                    const T *col = yzc + ((k - 1) & 1) * 2 * BX;
+                    ey_ip = col[w];
+                    ez_ip = col[BX + w];
                 }
-                cur = nxt;
+                const T *up = xz + (((k - 1) & 1) * (R + 1) + w + 1) * 2 * BX;
+                const T ex_jp = up[lane], ez_jp = up[BX + lane];
+                T hx = phx, hy = phy, hz = phz;
+                h_cell_vals(h, i, j, kh, hx, hy, hz, e.y - pey, e.x - pex, ez_jp - pez, ez_ip - pez, ex_jp - pex,
+                            ey_ip - pey, ikxH, ikyH, uload(h.pf.ikH[2], kh), nxH, nyH, sideH,
+                            PrefPsiH<S, T>{h, i, j, kh, c.qh, bxH, cxH, byH, cyH});
+                const int idh = i + a.P * j + kh * NxNy;
+                o.Hx.set_nt(idh, hx);
+                o.Hy.set_nt(idh, hy);
+                o.Hz.set_nt(idh, hz);
+                if (sq0 >= 0) {
+                    const T v[6] = {pex, pey, pez, stored(o.Hx, hx), stored(o.Hy, hy), stored(o.Hz, hz)};
+                    strip_put(h, sq0, j, kh, v);
+                    if (sq1 >= 0) strip_put(h, sq1, j, kh, v);
+                }
             }
         }
+        pex = e.x;
+        pey = e.y;
+        pez = e.z;
+        phx = c.hx;
+        phy = c.hy;
+        phz = c.hz;
         __syncthreads();
-        // ---- B: H of plane k-1, every row but the tile's last
-        if (k > k0 && jA0 < ty1 - 1) {
-            const int kh = k - 1;
-            const T ikzH = uload(h.pf.ikH[2], kh);
-            LdH cur = loadH(jA0, kh);
-            for (int j = jA0; j < ty1 - 1; j += g.RY) {
-                const int jn = j + g.RY;
-                LdH nxt = cur;
-                if (jn < ty1 - 1) nxt = loadH(jn, kh);
-                if (iv) {
-                    const int id = il + a.P * j + kh * NxNy;
-                    T byH = 0, cyH = 0;
-                    const int nyH = slabH(j, a.Ny, h.pml[2], h.pml[3]);
-                    if ((sideH & SIDE_Y) && nyH >= 0) { byH = uload(h.pf.bH[1], j); cyH = uload(h.pf.cH[1], j); }
-                    T hx = cur.hx, hy = cur.hy, hz = cur.hz;
-                    h_cell_vals(h, il, j, kh, hx, hy, hz, cur.ey_kp - cur.ey, cur.ex_kp - cur.ex, cur.ez_jp - cur.ez,
-                                cur.ez_ip - cur.ez, cur.ex_jp - cur.ex, cur.ey_ip - cur.ey, ikxH, uload(h.pf.ikH[1], j),
-                                ikzH, nxH, nyH, sideH, PrefPsiH<S, T>{h, il, j, kh, cur.q, bxH, cxH, byH, cyH});
-                    h.Hx.set_nt(id, hx);
-                    h.Hy.set_nt(id, hy);
-                    h.Hz.set_nt(id, hz);
-                    if (sq0 >= 0) {
-                        const T v[6] = {cur.ex, cur.ey, cur.ez, stored(h.Hx, hx), stored(h.Hy, hy), stored(h.Hz, hz)};
-                        strip_put(h, sq0, j, kh, v);
-                        if (sq1 >= 0) strip_put(h, sq1, j, kh, v);
-                    }
-                }
-                cur = nxt;
-            }
-        }
+    };
+    // software pipeline: plane k+1 requested before plane k is computed; the
+    // last (halo) plane peeled off
+    Ld cur = load(k0);
+    for (int k = k0; k < ke; k++) {
+        Ld nxt = load(k + 1);
+        body(cur, k);
+        cur = nxt;
     }
+    body(cur, ke);
     if constexpr (EN) {
-        __shared__ double red[16];
+        __shared__ double red[NW];
         double s = (double)en;
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
         if (lane == 0) red[w] = s;
         __syncthreads();
-        if (tid == 0) {
+        if (lane == 0 && w == 0) {
             double t = 0;
-            for (int q = 0; q < g.WX * g.RY; q++) t += red[q];
+            for (int y = 0; y < NW; y++) t += red[y];
             a.en_part[blockIdx.x] = t;
         }
     }
-}
-
-// H of one cell from the fields in memory (the edge cells of k_step, after
-// it has written every E): the arithmetic of h_tile.
-template <typename S, typename T>
-__device__ __forceinline__ void h_point(const HArgs<S, T> &h, int i, int j, int k) {
-    const int NxNy = h.P * h.Ny;
-    const int id = i + h.P * j + k * NxNy;
-    const int oip = (i == h.Nx - 1) ? -(h.Nx - 1) : 1;
-    const int ojp = (j == h.Ny - 1) ? -(h.Ny - 1) * h.P : h.P;
-    const int idkp = i + h.P * j + ((k == h.Nz - 1) ? 0 : k + 1) * NxNy;
-    const T ex = h.Ex[id], ey = h.Ey[id], ez = h.Ez[id];
-    const T ex_kp = h.Ex[idkp], ey_kp = h.Ey[idkp];
-    const T ex_jp = h.Ex[id + ojp], ez_jp = h.Ez[id + ojp];
-    const T ey_ip = h.Ey[id + oip], ez_ip = h.Ez[id + oip];
-    T hx = h.Hx[id], hy = h.Hy[id], hz = h.Hz[id];
-    const int nx_ = slabH(i, h.Nx, h.pml[0], h.pml[1]), ny_ = slabH(j, h.Ny, h.pml[2], h.pml[3]);
-    h_cell_vals(h, i, j, k, hx, hy, hz, ey_kp - ey, ex_kp - ex, ez_jp - ez, ez_ip - ez, ex_jp - ex, ey_ip - ey,
-                h.pf.ikH[0][i], h.pf.ikH[1][j], h.pf.ikH[2][k], nx_, ny_, side_all(h), GlobalPsiH<S, T>{h, i, j, k});
-    h.Hx.set(id, hx);
-    h.Hy.set(id, hy);
-    h.Hz.set(id, hz);
-    for (int q = 0; q < h.nstrip; q++) {
-        const int x0 = h.scol[q], x1 = (x0 + 1 == h.Nx) ? 0 : x0 + 1;
-        const int s = (i == x0) ? 2 * q : (i == x1 ? 2 * q + 1 : -1);
-        if (s >= 0) {
-            const T v[6] = {ex, ey, ez, stored(h.Hx, hx), stored(h.Hy, hy), stored(h.Hz, hz)};
-            strip_put(h, s, j, k, v);
-        }
-    }
-}
-
-// The edge cells of k_step: row R of the edge set (EdgeGeom), 64 cells of it
-// per wavefront (blockIdx.y: segment of the row); workgroups from nb on
-// update the H of the TFSF column `aux` (its E update ran before k_step).
-template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_step_edge(const HArgs<S, T> h, const EdgeGeom e, long nb,
-                                                   const HArgs<S, T> aux) {
-    if ((long)blockIdx.x >= nb) {
-        const int k = (int)(((long)blockIdx.x - nb) * 256 + threadIdx.x + 64 * threadIdx.y);
-        if (blockIdx.y == 0 && k < aux.Nz) h_col_cell(aux, k);
-        return;
-    }
-    const long R = (long)blockIdx.x * 4 + threadIdx.y;
-    const int i = (int)blockIdx.y * 64 + threadIdx.x;
-    if (R >= e.nrows || i >= h.Nx) return;
-    int j, k;
-    const long nz_rows = (long)e.ntz * h.Ny;
-    if (R < nz_rows) {
-        const int q = (int)(R / h.Ny);
-        j = (int)(R % h.Ny);
-        k = min((q + 1) * e.KZ, h.Nz) - 1;
-    } else {
-        const long Rp = R - nz_rows;
-        const int nzi = h.Nz - e.ntz;                 // planes that are not the last of a z tile
-        const int p = (int)(Rp / nzi), m = (int)(Rp % nzi);
-        j = min((p + 1) * e.TY, h.Ny) - 1;
-        k = (m / (e.KZ - 1)) * e.KZ + m % (e.KZ - 1);
-    }
-    h_point(h, i, j, k);
 }
 
 // Streaming copy used to measure the HBM bandwidth a kernel can reach on

@@ -95,6 +95,15 @@ def e_pass_bytes(lay, pml, stats):
     return b
 
 
+def one_pass_bytes(e_bytes, h_bytes, stats):
+    """Algorithmic HBM bytes of one one-pass step launch (k_step): the two
+    passes' bytes less the second reads of E and H (each field is read once
+    and written once, the material id read once; CPML psi and the dispersive
+    increments as in the passes)."""
+    ncell, _, _, s = stats
+    return e_bytes + h_bytes - ncell * 6 * s
+
+
 def rc_pass_bytes(stats):
     """RC pre-pass (k_rc_ante): per dispersive cell read cell index, material id
     and E (3), read+write 3 complex FP64 accumulators per pole, write 3 values."""
@@ -541,6 +550,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="CPU baseline sample: (2 + S)-step run minus 2-step run of the oracle")
+    ap.add_argument("--step", type=int, default=None,
+                    help="one-pass step tile code (-1: the E and H passes); with --tiles or alone it skips the tuner")
     ap.add_argument("--tiles", type=int, nargs=2, default=None, metavar=("E", "H"),
                     help="fixed tile configurations (0..31) instead of the autotuner's")
     ap.add_argument("--form", default="plain", choices=["plain", "precision", "batching", "serving", "setup", "slab"],
@@ -618,7 +629,7 @@ def main():
         # the same workload with FP64 storage and arithmetic (the reference's precision)
         f64 = headline_run(args, capi.FP64, KS, 2 * KS, KS)
         f64.pop("_info")
-        rec["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "roofline", "kernels_ms_avg", "tiles")}
+        rec["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "roofline", "kernels_ms_avg", "tiles", "step_tiles")}
         rec["fp64"]["steps"] = 2 * KS
     if not args.no_cpu:
         threads, cpu_note = host_cpu_share()
@@ -649,11 +660,17 @@ def headline_run(args, prec, W, K, P):
     bw = np.zeros(3)
     capi.engine().fn("stream_bandwidth")(1 << 31, 3, bw.ctypes.data_as(capi._dp))
     run.step(W)
-    if args.tiles is not None:
-        run.set_tiles(*args.tiles)
-        tiles = list(args.tiles)
+    be = capi.engine()
+    if args.tiles is not None or args.step is not None:
+        if args.tiles is not None:
+            run.set_tiles(*args.tiles)
+        be.check(be.fn("set_step_tiles")(run.grid(), -1 if args.step is None else args.step), run.grid())
+        tiles = list(args.tiles) if args.tiles is not None else None
     else:
         tiles = run.autotune(3) if not args.no_tune else None
+    code = capi.C.c_int(-1)
+    be.fn("get_step_tiles")(run.grid(), capi.C.byref(code))
+    step_code = code.value
     run.synchronize()
     run.profile(True, kinds=3)
     t0 = time.perf_counter()
@@ -674,31 +691,41 @@ def headline_run(args, prec, W, K, P):
     h_bytes = h_pass_bytes(lay, spec.pml, stats)
     e_avg = e_ms / max(1, e_n) / 1e3
     h_avg = h_ms / max(1, h_n) / 1e3
-    # the roofline names the longer of the two passes
-    if h_avg > e_avg:
+    one_pass = step_code >= 0 and h_n == 0
+    if one_pass:
+        # the one-pass step (its launches are profiled as "E"): each field read
+        # and written once, the material id read once
+        kname, kbytes, kavg = "k_step", one_pass_bytes(e_bytes, h_bytes, stats), e_avg
+        pass_bytes = kbytes
+    elif h_avg > e_avg:
+        # the roofline names the longer of the two passes
         kname, kbytes, kavg = "k_update_H", h_bytes, h_avg
+        pass_bytes = e_bytes + h_bytes
     else:
         kname, kbytes, kavg = "k_update_E", e_bytes, e_avg
+        pass_bytes = e_bytes + h_bytes
     achieved = kbytes / kavg / 1e9
-    step_bytes = e_bytes + h_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats)
+    step_bytes = pass_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats)
     rec = {
         "value": round(ncell * K / wall / 1e6, 1),
         "ms_per_step": round(wall / K * 1e3, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(prec, args.n, kname),
                      "kernel": kname, "bytes_per_launch": kbytes, "avg_launch_ms": round(kavg * 1e3, 5)},
-        "kernels_ms_avg": {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5),
+        "kernels_ms_avg": {**({"step": round(e_avg * 1e3, 5)} if one_pass else
+                              {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5)}),
                            "sensor": round(s_ms / max(1, s_n), 5),
                            "dft_per_flush": round(d_ms * KS / max(1, P), 5),
                            "dft_launches_per_flush": round(d_n * KS / max(1, P), 2),
                            "rc": round(r_ms / max(1, r_n), 5) if r_n else "in the sensor launch"},
         "step_hbm_fraction": round(step_bytes / (wall / K) / 1e9 / HBM_PEAK_GBS, 4),
-        "e_pass_GBps": round(e_bytes / e_avg / 1e9, 1),
-        "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1),
+        **({"step_pass_GBps": round(kbytes / kavg / 1e9, 1)} if one_pass else
+           {"e_pass_GBps": round(e_bytes / e_avg / 1e9, 1), "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1)}),
         "stream_copy_GBps": {"plain": round(float(bw[0]), 1), "nontemporal": round(float(bw[1]), 1),
                              "note": "best 2 GiB device copy (read+write), the achievable-bandwidth reference"},
         "setup_s": round(setup_s, 2),
         "tiles": tiles,
+        "step_tiles": step_code,
         "_info": {"workload": f"single-particle {args.n}^3 structure -> {lay[0]}x{lay[1]}x{lay[2]} grid, "
                               f"{len(spheres)} Drude+2CP spheres, 12-cell CPML x6, 64-lambda box DFT",
                   "cells": ncell, "ndisp": stats[1], "poles": stats[2], "spec": spec, "sensors": sensors,

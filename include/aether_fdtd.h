@@ -555,17 +555,23 @@ int aefdtd_profile_read(aefdtd_grid *g, const char *kernel, double *total_ms, lo
  * 4 rc; default all): fewer events keep a timed region closer to unprofiled. */
 int aefdtd_profile_kinds(aefdtd_grid *g, int mask);
 /* Tile autotuner: times every compiled tile shape of the E and H passes on
- * this grid (reps launches each), then a few tiles of the one-pass step
- * against the best pair, keeps the fastest and restores the grid's state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
+ * this grid (reps launches each), then every tile of the one-pass step
+ * (3D grids) against the best pair, keeps the fastest and restores the grid's state. out (may be NULL) receives the chosen codes (BY index | KZ index<<2 |
  * XCD-remap<<4; BY in 1,2,4,8 rows per workgroup, KZ in 1,2,4,8 planes per
  * lane, or rows per lane on grids one cell thick in z).  No counterpart in the reference (a CPU thread-count choice). */
 int aefdtd_autotune(aefdtd_grid *g, int reps, int out[2]);
 int aefdtd_set_tiles(aefdtd_grid *g, int tile_e, int tile_h);
-/* Tile of the one-pass step (aefdtd_step and the drivers): code = TY index
- * (4, 8, 16, 32 rows) | (lane rows - 1) << 2 (1..4 rows in flight) | KZ index
- * (4, 8, 16, 32 planes) << 4 | XCD fold << 6; -1 makes every step the two
- * passes.  No counterpart in the reference. */
+/* Tile of the one-pass step of 3D grids (aefdtd_step and the drivers; the
+ * fields of step n are read from one set of arrays and those of step n+1
+ * written to a second set, which holds the state afterwards): code = NW index
+ * (4, 8, 16 wavefronts per workgroup; a tile owns 64 columns and NW-2 rows) |
+ * KZ index (8, 16, 32, 64 planes) << 2 | XCD fold << 4; NW index 3 is
+ * refused; -1 makes every step the two passes.  The second set is
+ * allocated at the first one-pass step (six field arrays and the E CPML psi).
+ * No counterpart in the reference. */
 int aefdtd_set_step_tiles(aefdtd_grid *g, int code);
+/* The step tile code in use (-1: the two passes), e.g. the autotuner's choice. */
+int aefdtd_get_step_tiles(aefdtd_grid *g, int *code);
 /* Power-of-two storage scales of the six field components (stored value =
  * field * 2^exps[c]); FP16 storage only (defaults E: 2^8, H: 2^16, so that
  * unit-amplitude plane waves sit near the middle of the FP16 range; the custom
