@@ -20,6 +20,8 @@ struct Sensor {                   // Spect_Poynting with interpolated samples
     double2 *d_acc = nullptr;     // [6][Nl][P]
     double2 *d_coef = nullptr;    // [ncoef steps][2][Nl]: E then H coefficient
     int ncoef = 0;
+    float *d_coefmx = nullptr;    // FP32-arithmetic grids: [ncoef + 2 steps][2][LP] for the matrix-core flush
+    int LP = 0;                   // 2 Nl rounded up to 32
 };
 
 // FieldBlock / FieldMap (sensors/s_fieldmap.cpp:89-230, 321-470): the device
@@ -1022,6 +1024,27 @@ int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
     dfree_t(s.d_coef);
     s.d_coef = (double2 *)p;
     HIPOK(g, copy_sync(g, s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (g->precision != AEFDTD_FP64) {
+        // the same coefficients in FP32, (re, im) interleaved per wavelength and
+        // each step's row padded to a multiple of 32 columns, with zero rows
+        // after the last step (the flush reads up to 2 MX_U + 1 rows past the
+        // steps it sums)
+        const int LP = (2 * s.Nl + 31) / 32 * 32;
+        std::vector<float> m((size_t)(n + 2 * MX_U + 2) * 2 * LP, 0.0f);
+        for (int st = 0; st < n; st++)
+            for (int e = 0; e < 2; e++)
+                for (int l = 0; l < s.Nl; l++) {
+                    const double2 v = h[((size_t)st * 2 + e) * s.Nl + l];
+                    m[((size_t)st * 2 + e) * LP + 2 * l] = (float)v.x;
+                    m[((size_t)st * 2 + e) * LP + 2 * l + 1] = (float)v.y;
+                }
+        void *q;
+        if ((rc = dalloc(g, &q, m.size() * sizeof(float)))) return rc;
+        dfree_t(s.d_coefmx);
+        s.d_coefmx = (float *)q;
+        s.LP = LP;
+        HIPOK(g, copy_sync(g, s.d_coefmx, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     s.ncoef = n;
     return AEFDTD_OK;
 }
@@ -1033,6 +1056,26 @@ int flush_dft(aefdtd_grid *g) {
     // one launch per DFT_MAX_SENS sensors: every plane, component and wavelength
     for (size_t q0 = 0; q0 < g->sensors.size(); q0 += DFT_MAX_SENS) {
         const size_t n = std::min<size_t>(DFT_MAX_SENS, g->sensors.size() - q0);
+        if constexpr (sizeof(T) == 4) {
+            // FP32 arithmetic: the matrix-core flush (4 wavefronts of 32 cells per workgroup)
+            DftSetMx set{};
+            long maxP = 0;
+            for (size_t q = 0; q < n; q++) {
+                Sensor *s = g->sensors[q0 + q];
+                int rc = ensure_coefs(g, *s, g->sensor_step);
+                if (rc) return rc;
+                set.d[q] = DftDescMx{s->off, s->P, s->Nl, s->LP, s->d_coefmx, s->d_acc};
+                maxP = std::max(maxP, s->P);
+            }
+            hipEvent_t ev;
+            prof_begin(g, 3, &ev);
+            hipLaunchKernelGGL(k_dft_mx, dim3((unsigned)((maxP + 127) / 128), (unsigned)(6 * n)), dim3(256), 0,
+                               g->stream, (const float *)g->d_stage, g->stage_P, stage_depth(g->precision), set, s0,
+                               g->stage_fill);
+            prof_end(g, ev);
+            HIPOK(g, hipGetLastError());
+            continue;
+        }
         DftSet set{};
         long maxP = 0;
         for (size_t q = 0; q < n; q++) {
@@ -1045,8 +1088,8 @@ int flush_dft(aefdtd_grid *g) {
         dim3 grid((unsigned)((maxP + 255) / 256), (unsigned)(6 * n), 1);
         hipEvent_t ev;
         prof_begin(g, 3, &ev);
-        // T is double exactly for FP64 grids
-        constexpr int KD = stage_depth(sizeof(T) == 8 ? AEFDTD_FP64 : AEFDTD_FP32);
+        // T is double here (FP64 grids)
+        constexpr int KD = stage_depth(AEFDTD_FP64);
         hipLaunchKernelGGL((k_dft<T, KD>), grid, dim3(256), 0, g->stream, (const T *)g->d_stage, g->stage_P, set, s0,
                            g->stage_fill);
         prof_end(g, ev);
@@ -1851,6 +1894,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     for (Sensor *s : g->sensors) {
         dfree_t(s->d_acc);
         dfree_t(s->d_coef);
+        dfree_t(s->d_coefmx);
         delete s;
     }
     for (FieldSensor *s : g->fsensors) {

@@ -122,9 +122,11 @@ int batch_skewed_members(const Batch *b);
 void batch_profile(Batch *b, int on, double *ms, long *chunks);
 
 // DFT staging depth of a grid stepped on its own (steps between accumulator
-// flushes): the lane of the flush keeps this many samples in registers, 128
-// FP32 values or 64 FP64 ones.  Batches flush every 64-step chunk.
-constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 64 : 128; }
+// flushes): FP64 grids' flush lane keeps its 64 samples in registers; the
+// matrix-core flush of FP32-arithmetic grids streams its samples, so its
+// depth only sets how often the FP64 accumulators are read and written.
+// Batches flush every 64-step chunk.
+constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 64 : 256; }
 
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct FieldSensor; // single-wavelength FieldBlock / FieldMap (engine.hip)

@@ -1391,6 +1391,105 @@ __global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, const Df
     dft_block<T, KD>(stage, Ptot, d.off, d.P, d.Nl, d.coef, s0, nfill, d.acc, blockIdx.y % 6);
 }
 
+// The DFT flush of FP32-arithmetic grids on the matrix cores.  For one
+// component c of a sensor plane, the block's sums are a product
+//   D[l'][p] = sum_s W[s][l'] X[s][p],   l' = 2 l + (0 re, 1 im),
+// of the coefficient table W (steps x 2 Nl, in FP32) and the staged samples X
+// (steps x cells), formed with v_mfma_f32_32x32x2f32: A = W^T (32 rows l'),
+// B = X (32 cells), 2 steps per product.  A wavefront owns 32 cells of one
+// component and up to MXB blocks of 32 l' (D in 16 registers per block); each
+// product step reads one sample per lane from the stage (rows s and s+1, 32
+// consecutive cells each) and one coefficient per block (from L2, shared by
+// every wavefront of the flush).  The sums are then added to the FP64
+// accumulators, lane = cell, so that each accumulator row is read and written
+// as 32 consecutive double2.  Output layout of a 32x32 block: lane L holds
+// column L % 32 and rows 8 (r / 4) + 4 (L / 32) + r % 4 in its register r.
+// The matrix cores take the multiply-adds off the vector ALU (whose FP32
+// multiply-adds, unpacked, would bound the flush), so the flush runs at the
+// speed of its HBM traffic: the accumulators read and written once per block
+// and the samples read once.
+constexpr int MXB = 4;                     // l' blocks per pass of a wavefront
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct DftDescMx {
+    long off, P;
+    int Nl, LP;                            // LP: 2 Nl rounded up to 32
+    const float *W;                        // [step][E|H][LP], zero past 2 Nl, 2 MX_U + 2 rows past the last step
+    double2 *acc;                          // [6][Nl][P]
+};
+struct DftSetMx {
+    DftDescMx d[DFT_MAX_SENS];
+};
+
+// Product steps are requested MX_U at a time, one batch ahead of the one
+// being multiplied (samples beyond nfill read a clamped row and count zero;
+// coefficient rows are padded past the table's last step).
+constexpr int MX_U = 8;
+
+__global__ __launch_bounds__(256) void k_dft_mx(const float *stage, long Ptot, int KD, const DftSetMx set, int s0,
+                                                int nfill) {
+    const DftDescMx &d = set.d[blockIdx.y / 6];
+    const int c = blockIdx.y % 6, h = c >= 3 ? 1 : 0;
+    const int lane = threadIdx.x & 63;
+    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    if (p0 >= d.P) return;
+    const long p = p0 + (lane & 31);
+    const bool live = p < d.P;
+    const int kk = lane >> 5;                               // step of the pair this lane supplies
+    const float *xs = stage + (size_t)c * Ptot + d.off + (live ? p : p0);
+    const int nb = (2 * d.Nl + 31) / 32;
+    for (int b0 = 0; b0 < nb; b0 += MXB) {
+        f32x16 D[MXB];
+#pragma unroll
+        for (int q = 0; q < MXB; q++)
+            for (int r = 0; r < 16; r++) D[q][r] = 0.0f;
+        const float *wr = d.W + ((size_t)(s0 + kk) * 2 + h) * d.LP + b0 * 32 + (lane & 31);
+        float xa[MX_U], wa[MX_U][MXB], xn[MX_U], wn[MX_U][MXB];
+        auto load = [&](int s, float (&x)[MX_U], float (&w)[MX_U][MXB]) {
+#pragma unroll
+            for (int u = 0; u < MX_U; u++) {
+                const int row = s + 2 * u + kk;
+                const float v = __builtin_nontemporal_load(xs + (size_t)min(row, KD - 1) * 6 * Ptot);
+                x[u] = row < nfill ? v : 0.0f;
+                const float *wu = wr + (size_t)(s + 2 * u) * 2 * d.LP;
+#pragma unroll
+                for (int q = 0; q < MXB; q++) w[u][q] = wu[q * 32];
+            }
+        };
+        load(0, xa, wa);
+        for (int s = 0; s < nfill; s += 2 * MX_U) {
+            load(s + 2 * MX_U, xn, wn);
+#pragma unroll
+            for (int u = 0; u < MX_U; u++)
+#pragma unroll
+                for (int q = 0; q < MXB; q++)
+                    if (b0 + q < nb) D[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[u][q], xa[u], D[q], 0, 0, 0);
+#pragma unroll
+            for (int u = 0; u < MX_U; u++) {
+                xa[u] = xn[u];
+#pragma unroll
+                for (int q = 0; q < MXB; q++) wa[u][q] = wn[u][q];
+            }
+        }
+        if (!live) continue;
+#pragma unroll
+        for (int q = 0; q < MXB; q++) {
+            if (b0 + q >= nb) break;
+#pragma unroll
+            for (int r = 0; r < 16; r += 2) {
+                const int l = ((b0 + q) * 32 + 8 * (r >> 2) + 4 * kk + (r & 3)) >> 1;
+                if (l < d.Nl) {
+                    double2 *a = d.acc + ((size_t)c * d.Nl + l) * d.P + p;
+                    double2 v = ld_nt(a);
+                    v.x += (double)D[q][r];
+                    v.y += (double)D[q][r + 1];
+                    st_nt(v, a);
+                }
+            }
+        }
+    }
+}
+
 // One sensor of one grid of a batch for the DFT flush at the end of a chunk.
 template <typename T>
 struct DftB {

@@ -31,7 +31,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
-KS = 64                 # DFT staging depth of the engine (steps per accumulator flush)
+KS = 64                 # steps per batch chunk (and the batches' DFT flush)
+
+
+def stage_depth(prec):
+    """DFT staging depth of a grid stepped on its own (steps per accumulator
+    flush): 64 for FP64 grids, 256 for FP32-arithmetic grids."""
+    from aether_amd import capi
+    return 64 if prec == capi.FP64 else 256
 
 
 def host_cpu_share():
@@ -62,10 +69,10 @@ def host_cpu_share():
     return threads, note
 
 
-def dft_bytes_per_step(spec, sensors, lay, stats):
+def dft_bytes_per_step(spec, sensors, lay, stats, KD):
     """Algorithmic bytes per step of the sensor path: every plane cell's 6
     staged samples written once and read once, and the FP64 complex
-    accumulators (16 B) of 6 components x Nl read and written once per 64-step
+    accumulators (16 B) of 6 components x Nl read and written once per KD-step
     block."""
     s = stats[3]
     tot = 0
@@ -73,7 +80,7 @@ def dft_bytes_per_step(spec, sensors, lay, stats):
         nx, ny, nz = q.x2 - q.x1, q.y2 - q.y1, q.z2 - q.z1
         planes = [ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny] if q.box else [max(1, nx) * max(1, ny) * max(1, nz)]
         for P in planes:
-            tot += P * 6 * 2 * s + P * 6 * q.Nl * 32 / KS
+            tot += P * 6 * 2 * s + P * 6 * q.Nl * 32 / KD
     return tot
 
 
@@ -599,11 +606,12 @@ def main():
     if args.form == "slab":
         return slab_form(args)
     prec = capi.FP32 if args.precision == "fp32" else capi.FP64
-    # whole 64-step DFT staging blocks: the warm-up ends and the timed region
-    # starts with an empty stage, and the timed region holds K/64 flushes
-    K = -(-max(1, args.steps) // KS) * KS
-    W = -(-max(0, args.warmup) // KS) * KS
-    P = -(-max(1, args.prof_steps) // KS) * KS
+    # whole DFT staging blocks: the warm-up ends and the timed region starts
+    # with an empty stage, and the timed region holds K/KD flushes
+    KD = stage_depth(prec)
+    K = -(-max(1, args.steps) // KD) * KD
+    W = -(-max(0, args.warmup) // KD) * KD
+    P = -(-max(1, args.prof_steps) // KD) * KD
     main_rec = headline_run(args, prec, W, K, P)
     info = main_rec.pop("_info")
     rec = {
@@ -621,16 +629,17 @@ def main():
         "data": "synthetic (seeded random sphere packing, no files)",
         "config": {"workload": info["workload"], "cells": info["cells"], "dispersive_cells": info["ndisp"],
                    "poles": info["poles"], "parallelism": "single GPU",
-                   "timed_window": f"{K} steps = {K // KS} whole 64-step DFT blocks (requested {args.steps}), "
+                   "timed_window": f"{K} steps = {K // KD} whole {KD}-step DFT blocks (requested {args.steps}), "
                                    f"after {W} warm-up steps (requested {args.warmup})"},
     }
     rec.update({k: v for k, v in main_rec.items() if k not in ("value", "ms_per_step")})
     if prec == capi.FP32 and not args.no_fp64:
         # the same workload with FP64 storage and arithmetic (the reference's precision)
-        f64 = headline_run(args, capi.FP64, KS, 2 * KS, KS)
+        KD64 = stage_depth(capi.FP64)
+        f64 = headline_run(args, capi.FP64, KD64, 2 * KD64, KD64)
         f64.pop("_info")
         rec["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "roofline", "kernels_ms_avg", "tiles", "step_tiles")}
-        rec["fp64"]["steps"] = 2 * KS
+        rec["fp64"]["steps"] = 2 * KD64
     if not args.no_cpu:
         threads, cpu_note = host_cpu_share()
         lo, hi = 2, 2 + args.cpu_steps
@@ -705,7 +714,8 @@ def headline_run(args, prec, W, K, P):
         kname, kbytes, kavg = "k_update_E", e_bytes, e_avg
         pass_bytes = e_bytes + h_bytes
     achieved = kbytes / kavg / 1e9
-    step_bytes = pass_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats)
+    KD = stage_depth(prec)
+    step_bytes = pass_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats, KD)
     rec = {
         "value": round(ncell * K / wall / 1e6, 1),
         "ms_per_step": round(wall / K * 1e3, 4),
@@ -715,8 +725,9 @@ def headline_run(args, prec, W, K, P):
         "kernels_ms_avg": {**({"step": round(e_avg * 1e3, 5)} if one_pass else
                               {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5)}),
                            "sensor": round(s_ms / max(1, s_n), 5),
-                           "dft_per_flush": round(d_ms * KS / max(1, P), 5),
-                           "dft_launches_per_flush": round(d_n * KS / max(1, P), 2),
+                           "dft_per_flush": round(d_ms / max(1, d_n), 5),
+                           "dft_us_per_step": round(d_ms * 1e3 / max(1, P), 3),
+                           "dft_steps_per_flush": KD,
                            "rc": round(r_ms / max(1, r_n), 5) if r_n else "in the sensor launch"},
         "step_hbm_fraction": round(step_bytes / (wall / K) / 1e9 / HBM_PEAK_GBS, 4),
         **({"step_pass_GBps": round(kbytes / kavg / 1e9, 1)} if one_pass else

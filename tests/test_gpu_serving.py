@@ -64,7 +64,7 @@ def test_server_budget_admission_and_estimate():
     """With room for one request at a time the server serves them in FIFO
     order, one resident at a time, and every run's bytes stay within its
     admission estimate."""
-    reqs = [_custom(11, 64, 200), _custom(12, 70, 150), _single(13, 16, 120)]
+    reqs = [_single(13, 16, 120), _single(14, 15, 100), _custom(11, 64, 200)]
     be = capi.engine()
     est = [capi.estimate_bytes(be, k, sp, q) for k, sp, q in reqs]
     budget = max(est)
@@ -104,3 +104,51 @@ def test_server_cancel_releases_memory():
     r2 = srv.submit(k2, sp2, q2)
     assert srv.wait(r2, 60.0) == capi.REQ_DONE
     srv.close()
+
+
+def _single_real(side, steps, precision):
+    """A serving-stream 3D request at its own size class: structure side
+    `side` cells of 2 nm, gold-like spheres (Drude + 2 CP), 12-cell CPML on
+    all faces, box sensor at 64 wavelengths."""
+    D = 2e-9
+    sph = workloads.random_spheres(31, max(1, round(40 * (side / 256) ** 3)), (side * D,) * 3, 8e-9, 20e-9)
+    g = workloads.voxelize_spheres((side,) * 3, (D,) * 3, sph)
+    spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(1.77), workloads.GOLD], Nt=steps, Dx=D, Dy=D, Dz=D,
+                         pml=(12,) * 6, Nl=64, lambda_min=400e-9, lambda_max=800e-9, polarization=1,
+                         precision=precision)
+    sens = [capi.SensorReq(box=True, x1=0, x2=side, y1=0, y2=side, z1=0, z2=side, Nl=64,
+                           lambda_min=400e-9, lambda_max=800e-9)]
+    return capi.KIND_SINGLE, capi.mode_finalize(capi.engine(), spec, capi.KIND_SINGLE, sens), sens
+
+
+def test_server_request_sizes_against_oracle(oracle_be):
+    """A 96^3 single-particle request and a 512^2 2D request (the smallest of
+    the serving stream's size classes), each in FP64 and FP32, served together:
+    the FP64 results against the CPU oracle at the FP64 bars (spectra within
+    1e-9 of their peak), the FP32 results within 1e-4 of their peak of the
+    FP64 ones."""
+    be = capi.engine()
+    reqs = {"3d64": _single_real(96, 260, capi.FP64), "3d32": _single_real(96, 260, capi.FP32),
+            "2d64": _custom(41, 512, workloads.sweep_steps(512), capi.FP64),
+            "2d32": _custom(41, 512, workloads.sweep_steps(512), capi.FP32)}
+    assert capi.single_particle_dims(reqs["3d64"][1])[0] >= 96
+    srv = capi.Server(be, 200 << 30)
+    ids = {k: srv.submit(*v) for k, v in reqs.items()}
+    out = {}
+    for k, rid in ids.items():
+        assert srv.wait(rid, 300.0) == capi.REQ_DONE, srv.error(rid)
+        out[k] = srv.result(rid)[0]
+    srv.close()
+    print("served; oracle runs (16 threads)", flush=True)
+    oracle_be.fn("set_threads")(16)
+    ref3 = capi.run_single_particle(oracle_be, reqs["3d64"][1], reqs["3d64"][2])["sensors"]
+    ref2 = capi.run_custom(oracle_be, reqs["2d64"][1], reqs["2d64"][2])["sensors"]
+    oracle_be.fn("set_threads")(0)
+    for name, ref in (("3d", ref3), ("2d", ref2)):
+        peak = np.max(np.abs(ref))
+        e64 = np.max(np.abs(out[name + "64"] - ref)) / peak
+        e32 = np.max(np.abs(out[name + "32"] - out[name + "64"])) / peak
+        print(f"{name}: FP64 vs oracle {e64:.3e}, FP32 vs FP64 {e32:.3e} (of peak {peak:.3e})", flush=True)
+        assert peak > 0
+        assert e64 <= 1e-9, (name, e64)
+        assert e32 <= 1e-4, (name, e32)
