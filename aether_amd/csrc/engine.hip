@@ -1026,11 +1026,9 @@ int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
     HIPOK(g, copy_sync(g, s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
     if (g->precision != AEFDTD_FP64) {
         // the same coefficients in FP32, (re, im) interleaved per wavelength and
-        // each step's row padded to a multiple of 32 columns, with zero rows
-        // after the last step (the flush reads up to 2 MX_U + 1 rows past the
-        // steps it sums)
+        // each step's row padded with zeros to a multiple of 32 columns
         const int LP = (2 * s.Nl + 31) / 32 * 32;
-        std::vector<float> m((size_t)(n + 2 * MX_U + 2) * 2 * LP, 0.0f);
+        std::vector<float> m((size_t)n * 2 * LP, 0.0f);
         for (int st = 0; st < n; st++)
             for (int e = 0; e < 2; e++)
                 for (int l = 0; l < s.Nl; l++) {
@@ -1064,7 +1062,7 @@ int flush_dft(aefdtd_grid *g) {
                 Sensor *s = g->sensors[q0 + q];
                 int rc = ensure_coefs(g, *s, g->sensor_step);
                 if (rc) return rc;
-                set.d[q] = DftDescMx{s->off, s->P, s->Nl, s->LP, s->d_coefmx, s->d_acc};
+                set.d[q] = DftDescMx{s->off, s->P, s->Nl, s->LP, s->ncoef, s->d_coefmx, s->d_acc};
                 maxP = std::max(maxP, s->P);
             }
             hipEvent_t ev;

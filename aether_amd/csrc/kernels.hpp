@@ -1414,7 +1414,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 struct DftDescMx {
     long off, P;
     int Nl, LP;                            // LP: 2 Nl rounded up to 32
-    const float *W;                        // [step][E|H][LP], zero past 2 Nl, 2 MX_U + 2 rows past the last step
+    int rows;                              // steps in W
+    const float *W;                        // [rows][E|H][LP], zero past 2 Nl
     double2 *acc;                          // [6][Nl][P]
 };
 struct DftSetMx {
@@ -1422,8 +1423,9 @@ struct DftSetMx {
 };
 
 // Product steps are requested MX_U at a time, one batch ahead of the one
-// being multiplied (samples beyond nfill read a clamped row and count zero;
-// coefficient rows are padded past the table's last step).
+// being multiplied; steps beyond nfill (the prefetch past the block's end)
+// read clamped rows of the stage and of the coefficient table, and their
+// samples count zero.
 constexpr int MX_U = 8;
 
 __global__ __launch_bounds__(256) void k_dft_mx(const float *stage, long Ptot, int KD, const DftSetMx set, int s0,
@@ -1443,7 +1445,7 @@ __global__ __launch_bounds__(256) void k_dft_mx(const float *stage, long Ptot, i
 #pragma unroll
         for (int q = 0; q < MXB; q++)
             for (int r = 0; r < 16; r++) D[q][r] = 0.0f;
-        const float *wr = d.W + ((size_t)(s0 + kk) * 2 + h) * d.LP + b0 * 32 + (lane & 31);
+        const float *wr = d.W + (size_t)h * d.LP + (lane & 31);
         float xa[MX_U], wa[MX_U][MXB], xn[MX_U], wn[MX_U][MXB];
         auto load = [&](int s, float (&x)[MX_U], float (&w)[MX_U][MXB]) {
 #pragma unroll
@@ -1451,9 +1453,9 @@ __global__ __launch_bounds__(256) void k_dft_mx(const float *stage, long Ptot, i
                 const int row = s + 2 * u + kk;
                 const float v = __builtin_nontemporal_load(xs + (size_t)min(row, KD - 1) * 6 * Ptot);
                 x[u] = row < nfill ? v : 0.0f;
-                const float *wu = wr + (size_t)(s + 2 * u) * 2 * d.LP;
+                const float *wu = wr + (size_t)min(s0 + row, d.rows - 1) * 2 * d.LP;
 #pragma unroll
-                for (int q = 0; q < MXB; q++) w[u][q] = wu[q * 32];
+                for (int q = 0; q < MXB; q++) w[u][q] = wu[min(b0 + q, nb - 1) * 32];   // blocks past nb: unused
             }
         };
         load(0, xa, wa);

@@ -458,7 +458,7 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
             if (nf == 6 && q.disable[f]) continue;
             // accumulators, the FP64 coefficient table and (FP32 arithmetic) its FP32 copy
             b += 6 * (size_t)q.Nl * P[f] * 16 + (size_t)std::max(m->Nt, 1) * 2 * q.Nl * 16;
-            if (p != AEFDTD_FP64) b += (size_t)(std::max(m->Nt, 1) + 18) * 2 * ((2 * q.Nl + 31) / 32 * 32) * 4;
+            if (p != AEFDTD_FP64) b += (size_t)std::max(m->Nt, 1) * 2 * ((2 * q.Nl + 31) / 32 * 32) * 4;
             Ptot += P[f];
         }
     }
