@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_SO = os.path.join(HERE, "libaether_fdtd.so")
 
-OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED = 0, 1, 2, 3, 4, 5
+OK, EINVAL, ENOMEM, EDEVICE, ESTATE, ECANCELLED, ERANGE = 0, 1, 2, 3, 4, 5, 6
 MODE_NORM, MODE_CUSTOM = 0, 5
 FP64, FP32, FP16 = 0, 1, 2
 NORMAL_X, NORMAL_XM, NORMAL_Y, NORMAL_YM, NORMAL_Z, NORMAL_ZM, NORMAL_UNDEF = range(7)
@@ -157,6 +157,7 @@ def _sig(lib, p):
         "stats": (i, [vp, C.POINTER(C.c_long)]),
         "autotune": (i, [vp, i, _ip]),
         "set_storage_scale": (i, [vp, _ip]),
+        "get_storage_scale": (i, [vp, _ip]),
         "set_tiles": (i, [vp, i, i]),
         "set_step_tiles": (i, [vp, i]),
         "get_step_tiles": (i, [vp, _ip]),

@@ -1696,16 +1696,34 @@ hipStream_t batch_stream(const Batch *b) { return b->stream; }
 int batch_step(Batch *b, int chunks) {
     if (!b || chunks < 0) return AEFDTD_EINVAL;
     if (b->members.empty() || chunks == 0) return AEFDTD_OK;
-    if (b->dirty) {
-        int rc = with_types_p(b->precision, [&]<typename S, typename T>() { return batch_build<S, T>(b); });
-        if (rc) return rc;
-    }
     if (b->prof_on) BHIPOK(b, hipEventRecord(b->ev0, b->stream));
     for (int c = 0; c < chunks; c++) {
+        if (b->dirty) {
+            int rc = with_types_p(b->precision, [&]<typename S, typename T>() { return batch_build<S, T>(b); });
+            if (rc) return rc;
+        }
         BHIPOK(b, hipGraphLaunch(b->exec, b->stream));
         for (size_t r = 0; r < b->members.size(); r++) {
             aefdtd_grid *g = b->members[r];
             b->tcur[r] = std::min(b->tcur[r] + KS, g->Nt);
+        }
+        if (b->precision == AEFDTD_FP16) {
+            // FP16 range control after every chunk (every 64 steps, as a run
+            // stepped on its own checks): a changed exponent is baked into the
+            // chunk's tables, so the next chunk is built again
+            BHIPOK(b, hipStreamSynchronize(b->stream));
+            for (size_t r = 0; r < b->members.size(); r++) {
+                aefdtd_grid *g = b->members[r];
+                g->tstep = b->tcur[r];
+                if (g->tstep % KS != 0) continue;         // a run alone checks at multiples of 64 steps
+                bool ch = false;
+                int rc = fp16_range(g, &ch);
+                if (rc) { b->err = g->err; return rc; }
+                if (ch) {
+                    BHIPOK(b, hipStreamSynchronize(g->stream));
+                    b->dirty = true;
+                }
+            }
         }
     }
     if (b->prof_on) {
@@ -1883,6 +1901,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree(g->d_strip);
     dfree_t(g->d_energy);
     dfree_t(g->d_en_part);
+    dfree_t(g->d_absmax);
     dfree(g->d_osc);
     for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
@@ -2051,6 +2070,46 @@ int update_E_pair(aefdtd_grid *g, aefdtd_grid *aux) {
     return with_types(g, [&]<typename S, typename T>() { return launch_E<S, T>(g, aux); });
 }
 
+int fp16_range(aefdtd_grid *g, bool *changed) {
+    if (changed) *changed = false;
+    if (!g || g->precision != AEFDTD_FP16) return AEFDTD_OK;
+    int rc = ensure_ready(g);
+    if (rc) return rc;
+    if (!g->d_absmax) {
+        void *p;
+        if ((rc = dalloc(g, &p, 6 * sizeof(unsigned)))) return rc;
+        g->d_absmax = (unsigned *)p;
+    }
+    FieldPtrs fp;
+    for (int c = 0; c < 6; c++) fp.f[c] = g->F[c];
+    const size_t n = g->nalloc();
+    const unsigned nb = (unsigned)std::min<size_t>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, g->stream, g->d_absmax, 6);
+    hipLaunchKernelGGL((k_absmax<_Float16>), dim3(nb, 6), dim3(256), 0, g->stream, fp, n, g->d_absmax);
+    HIPOK(g, hipGetLastError());
+    unsigned bits[6];
+    HIPOK(g, copy_sync(g, bits, g->d_absmax, sizeof(bits), hipMemcpyDeviceToHost));
+    static const char *names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+    for (int c = 0; c < 6; c++) {
+        if (!g->enable[c]) continue;
+        if (bits[c] >= 0x7f800000u)
+            return fail(g, AEFDTD_ERANGE, std::string("FP16 storage overflow: ") + names[c] + " holds a non-finite "
+                                          "value at step " + std::to_string(g->tstep) + " (storage exponent " +
+                                          std::to_string(g->scale_exp[c]) + ")");
+        float m;
+        std::memcpy(&m, &bits[c], sizeof(m));
+        if (m > 8192.0f) {
+            const int d = 12 - (int)std::ceil(std::log2((double)m));
+            hipLaunchKernelGGL((k_rescale<_Float16>), dim3(nb), dim3(256), 0, g->stream, (_Float16 *)g->F[c], n,
+                               (float)std::ldexp(1.0, d));
+            HIPOK(g, hipGetLastError());
+            g->scale_exp[c] += d;
+            if (changed) *changed = true;
+        }
+    }
+    return AEFDTD_OK;
+}
+
 int update_H_pair(aefdtd_grid *g, aefdtd_grid *aux) {
     if (!g || !aux) return AEFDTD_EINVAL;
     if (!pairable(g, aux)) {
@@ -2177,6 +2236,12 @@ int aefdtd_write_field(aefdtd_grid *g, int f, const double *in) {
         for (size_t i = 0; i < N; i++) t[i] = (float)in[i];
         HIPOK(g, cells_to_device(g, g->F[f], t.data(), 4));
     }
+    return AEFDTD_OK;
+}
+
+int aefdtd_get_storage_scale(aefdtd_grid *g, int exps[6]) {
+    if (!g || !exps) return AEFDTD_EINVAL;
+    for (int c = 0; c < 6; c++) exps[c] = g->scale_exp[c];
     return AEFDTD_OK;
 }
 

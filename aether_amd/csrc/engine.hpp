@@ -98,6 +98,11 @@ void ft_free(FTState &f);
 int ft_place(const aefdtd_grid *g, double Dt, double lmin, double lmax, int Np, const char layout[4], unsigned seed,
              std::vector<int> &pts, std::vector<double> &w, int &navg);
 bool ft_complete(const std::vector<double> &ring, int np, int navg, double coeff, int steps, int nmax);
+// FP16 storage range control of a grid (no-op for FP32/FP64): the largest
+// stored value of each enabled component; above 2^13 the component's storage
+// exponent is lowered so that it is at most 2^12 and its values are rescaled
+// (*changed set); a stored infinity or NaN fails with AEFDTD_ERANGE.
+int fp16_range(aefdtd_grid *g, bool *changed);
 // Plane extent (cells) of sensor id after its link to the grid.
 int sensor_spans(const aefdtd_grid *g, int id, int *span1, int *span2);
 // Linked (full-grid) location x1, x2, y1, y2, z1, z2 of sensor id.
@@ -198,6 +203,7 @@ struct aefdtd_grid {
     std::vector<int> strip_cols;      // x1 of each strip plane (columns x1, x1+1)
     void *d_strip = nullptr;          // [plane][Nz][Ny][2 columns][6] T
     bool strip_valid = false;
+    unsigned *d_absmax = nullptr;     // FP16 range control: largest |stored| per component (float bits)
     double *d_energy = nullptr;       // completion energy: [EN_BLOCKS partials][last, max]
     bool en_fuse = false;             // the 3D E pass also sums the energy (FP32/FP64 runs with the check)
     double *d_en_part = nullptr;      // its per-workgroup partials

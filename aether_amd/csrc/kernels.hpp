@@ -2509,6 +2509,38 @@ __global__ __launch_bounds__(BX *NW) void k_step(const EArgs<S, T> a, const HArg
     }
 }
 
+// ================================================================ FP16 storage range
+// Largest |stored value| of each field component, as the bit pattern of a
+// float (non-negative floats order as their bits; infinity and NaN above every
+// finite value), so that one atomicMax per workgroup and component suffices.
+struct FieldPtrs {
+    const void *f[6];
+};
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_absmax(const FieldPtrs fp, size_t n, unsigned *out) {
+    const S *f = static_cast<const S *>(fp.f[blockIdx.y]);
+    unsigned m = 0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf((float)f[q])));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+    __shared__ unsigned red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out + blockIdx.y, max(max(red[0], red[1]), max(red[2], red[3])));
+}
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// Stored values of one component times 2^d (exact for normal FP16 values).
+template <typename S>
+__global__ __launch_bounds__(256) void k_rescale(S *f, size_t n, float fac) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        f[q] = (S)((float)f[q] * fac);
+}
+
 // Streaming copy used to measure the HBM bandwidth a kernel can reach on
 // this device (16-byte accesses, grid-stride).
 typedef float f32x4 __attribute__((ext_vector_type(4)));

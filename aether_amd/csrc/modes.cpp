@@ -595,6 +595,13 @@ int aefdtd_run_step(aefdtd_run *r, int n) {
             }
         }
         r->t++;
+        if (r->g->precision == AEFDTD_FP16 && r->t % 64 == 0) {
+            // FP16 storage range control every 64 steps (a batch checks after each
+            // 64-step chunk, at the same steps)
+            bool ch = false;
+            if ((rc = aefdtd::fp16_range(r->g, &ch)) || (r->a && (rc = aefdtd::fp16_range(r->a, &ch))))
+                return rfail(r, rc);
+        }
         if (r->cancel.load(std::memory_order_relaxed) ||
             (r->end_flag && __atomic_load_n(r->end_flag, __ATOMIC_RELAXED))) {
             // the run ends after this step; normal incidence keeps Nt = t steps for its spectra

@@ -41,6 +41,7 @@ extern "C" {
 #define AEFDTD_EDEVICE     3   /* HIP runtime error, or no usable gfx950 GPU  */
 #define AEFDTD_ESTATE      4   /* call out of order (e.g. step before setup)  */
 #define AEFDTD_ECANCELLED  5   /* run stopped by its cancel flag              */
+#define AEFDTD_ERANGE      6   /* a field left the FP16 storage range         */
 
 /* ---- grid modes (the `smod` string of FDTD::FDTD, fdtd_core.cpp:198-204) */
 #define AEFDTD_MODE_NORM    0  /* "NORM"   : normal-incidence slab driver     */
@@ -578,6 +579,13 @@ int aefdtd_get_step_tiles(aefdtd_grid *g, int *code);
  * driver uses E: 2^0, H: 2^8 when it has oscillator sources, whose amplitude
  * is 50). Before the first update. */
 int aefdtd_set_storage_scale(aefdtd_grid *g, const int exps[6]);
+/* The storage exponents in use.  Runs with FP16 storage (the drivers, run
+ * objects, the server and batches) check every 64 steps the largest stored
+ * value of each component: above 2^13 the component's exponent is lowered so
+ * that it is at most 2^12 (its stored values rescaled, exactly), and a stored
+ * infinity or NaN ends the run with AEFDTD_ERANGE.  No counterpart in the
+ * reference (which stores FP64). */
+int aefdtd_get_storage_scale(aefdtd_grid *g, int exps[6]);
 /* Reference point for the roofline, no counterpart in the reference: the
  * best bandwidth of a streaming device copy of `bytes` (read + write counted)
  * over grid sizes 1024..8192 x 256 lanes, 16-byte accesses; out[0] GB/s with
