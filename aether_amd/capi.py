@@ -158,6 +158,7 @@ def _sig(lib, p):
         "autotune": (i, [vp, i, _ip]),
         "set_storage_scale": (i, [vp, _ip]),
         "get_storage_scale": (i, [vp, _ip]),
+        "sensor_out_len": (C.c_size_t, [C.POINTER(CSensorReq)]),
         "set_tiles": (i, [vp, i, i]),
         "set_step_tiles": (i, [vp, i]),
         "get_step_tiles": (i, [vp, _ip]),

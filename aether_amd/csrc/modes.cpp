@@ -96,6 +96,8 @@ size_t aefdtd::request_out_len(const aefdtd_sensor_req &q) {
     }
 }
 
+extern "C" size_t aefdtd_sensor_out_len(const aefdtd_sensor_req *q) { return q ? aefdtd::request_out_len(*q) : 0; }
+
 namespace {
 
 // Box_Spect_Poynting (sensors/s_spec_power.cpp:125-232) becomes six planar

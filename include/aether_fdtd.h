@@ -315,6 +315,10 @@ typedef struct aefdtd_sensor_req {
  *   FIELDMAP2 fieldmap2: the spectral accumulators of a planar sensor with Nl
  *             wavelengths, 6 x Nl x s1 s2 complex (aefdtd_sensor_spectra)
  * Field sensors must lie inside the grid (their extents are not clamped). */
+/* Doubles aefdtd_run_finish and the one-call drivers write to sensor_out for
+ * request q (the sizes above; field sensors after their extents are linked to
+ * the grid, as given). */
+size_t aefdtd_sensor_out_len(const aefdtd_sensor_req *q);
 #define AEFDTD_SENSOR_POYNTING   0
 #define AEFDTD_SENSOR_FIELDBLOCK 1
 #define AEFDTD_SENSOR_FIELDMAP   2

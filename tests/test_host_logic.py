@@ -28,7 +28,8 @@ def test_oracle_mirrors_the_grid_abi():
     engine_only = {"aefdtd_profile_enable", "aefdtd_profile_read", "aefdtd_profile_kinds", "aefdtd_device_bytes", "aefdtd_stats",
                    "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_set_step_tiles", "aefdtd_get_step_tiles", "aefdtd_synchronize",
                    "aefdtd_sensor_poynting_noalloc",
-                   "aefdtd_set_storage_scale", "aefdtd_get_storage_scale", "aefdtd_stream_bandwidth"}
+                   "aefdtd_set_storage_scale", "aefdtd_get_storage_scale", "aefdtd_stream_bandwidth",
+                   "aefdtd_sensor_out_len"}
     missing = [s for s in declared_symbols()
                if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_", "aefdtd_server_"))
                and s != "aefdtd_estimate_bytes" or s in
@@ -264,3 +265,20 @@ def test_oracle_cancel_flag_ends_after_the_step():
     flag.value = 0
     full = capi.run_single_particle(be, spec, sens, end_flag=flag)
     assert not full["cancelled"] and oracle.last_steps() == 40
+
+
+def test_sensor_out_len_matches_the_python_view():
+    """aefdtd_sensor_out_len (host code, no device call) agrees with
+    SensorReq.out_len for every request kind."""
+    from aether_amd import capi
+    be = capi.engine()
+    reqs = [capi.SensorReq(box=True, x1=0, x2=10, y1=0, y2=8, z1=0, z2=6, Nl=13),
+            capi.SensorReq(box=False, orientation=capi.NORMAL_Y, x1=0, x2=10, y1=3, y2=4, z1=0, z2=6, Nl=7),
+            capi.SensorReq(kind=capi.SENSOR_FIELDBLOCK, x1=1, x2=4, y1=2, y2=7, z1=0, z2=3, Nl=1),
+            capi.SensorReq(kind=capi.SENSOR_FIELDMAP, orientation=capi.NORMAL_X, x1=2, x2=3, y1=0, y2=9, z1=1, z2=5,
+                           Nl=1),
+            capi.SensorReq(kind=capi.SENSOR_FIELDMAP2, orientation=capi.NORMAL_Z, x1=0, x2=6, y1=0, y2=5, z1=2, z2=3,
+                           Nl=4)]
+    for q in reqs:
+        c = q.to_c()
+        assert be.fn("sensor_out_len")(C.byref(c)) == q.out_len(), q.kind
