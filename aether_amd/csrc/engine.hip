@@ -1400,6 +1400,11 @@ struct aefdtd::Batch {
     int4 *d_rect = nullptr;                // interior tile rectangle per member
     int nmem[2][2] = {{0, 0}, {0, 0}}, nwgT[2][2] = {{0, 0}, {0, 0}};
     int nstep = 0;                         // members stepped by the per-step passes
+    // dispersive members' RC pre-pass (RcArgs per entry, workgroup prefix,
+    // member index, skipped step) and members with a plane injection (InjB)
+    void *d_rc = nullptr, *d_inj = nullptr;
+    int *d_wgR = nullptr, *d_rmem = nullptr, *d_rskip = nullptr;
+    int nrc = 0, nwgR = 0, ninj = 0;
     int nsamp_step = 0;                    // leading sample entries that belong to those members
     size_t ldsT = 0;
     std::vector<int> tcur;                 // host mirror of d_tcur
@@ -1418,12 +1423,29 @@ void batch_free_tables(Batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
     for (void **p : {&b->d_ea, &b->d_ha, &b->d_samp, &b->d_dft, (void **)&b->d_wgE, (void **)&b->d_wgH,
-                     (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_sbeg, &b->d_alt, (void **)&b->d_rect})
+                     (void **)&b->d_tcur, (void **)&b->d_tend, (void **)&b->d_sbeg, &b->d_alt, (void **)&b->d_rect,
+                     &b->d_rc, &b->d_inj, (void **)&b->d_wgR, (void **)&b->d_rmem, (void **)&b->d_rskip})
         if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (int p = 0; p < 2; p++)
         for (int c = 0; c < 2; c++)
             for (int **q : {&b->d_mem[p][c], &b->d_wgT[p][c]})
                 if (*q) { (void)hipFree(*q); *q = nullptr; }
+}
+
+// The plane injection values of g on the device (batches read the step's
+// values there), [step][dEx, dEy, dHx, dHy].
+int ensure_inj_table(aefdtd_grid *g) {
+    if (g->d_inj || g->inj[0].empty()) return AEFDTD_OK;
+    const size_t n = g->inj[0].size();
+    std::vector<double> t(4 * n);
+    for (size_t s = 0; s < n; s++)
+        for (int c = 0; c < 4; c++) t[4 * s + c] = g->inj[c][s];
+    void *p;
+    int rc = dalloc(g, &p, t.size() * sizeof(double));
+    if (rc) return rc;
+    g->d_inj = (double *)p;
+    HIPOK(g, copy_sync(g, g->d_inj, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    return AEFDTD_OK;
 }
 
 template <typename V>
@@ -1458,7 +1480,10 @@ int batch_build(Batch *b) {
     std::vector<SampB<S, T>> samp;
     std::vector<DftB<T>> dft;
     std::vector<TbAlt<S, T>> alt(n);
-    int totE = 0, totH = 0, totT[2][2] = {{0, 0}, {0, 0}}, maxmat = 1;
+    std::vector<RcArgs<S, T>> rca;
+    std::vector<int> wgR, rmem, rskip;
+    std::vector<InjB> inj;
+    int totE = 0, totH = 0, totR = 0, totT[2][2] = {{0, 0}, {0, 0}}, maxmat = 1;
     b->maxP = 0;
     for (int r = 0; r < n; r++) {
         aefdtd_grid *g = b->members[r];
@@ -1466,6 +1491,28 @@ int batch_build(Batch *b) {
         if ((rc = ensure_ready(g)) || (rc = ensure_osc_tables(g, g->Nt)) || (rc = prepare_stage<T>(g))) {
             b->err = g->err;
             return rc;
+        }
+        if (g->pml_dirty) {
+            if ((rc = upload_profiles(g))) { b->err = g->err; return rc; }
+            g->pml_dirty = false;
+        }
+        // a TFSF grid's source column must be stepped in the same launches
+        if (g->tfsf_aux && std::find(b->members.begin(), b->members.end(), g->tfsf_aux) == b->members.end()) {
+            b->err = "a TFSF grid's source column is not in the batch";
+            return AEFDTD_ESTATE;
+        }
+        if (g->n_disp) {
+            rca.push_back(rc_args<S, T>(g));
+            wgR.push_back(totR);
+            totR += (int)((g->n_disp + 255) / 256);
+            rmem.push_back(r);
+            // the last sensor feed on its own ran the pre-pass of this step
+            rskip.push_back(g->rc_ahead ? g->tstep : -1);
+            g->rc_ahead = false;
+        }
+        if (g->inj_k >= 0) {
+            if ((rc = ensure_inj_table(g))) { b->err = g->err; return rc; }
+            inj.push_back(InjB{r, g->inj_k, (int)g->inj[0].size(), g->d_inj});
         }
         for (Sensor *s : g->sensors)
             if ((rc = ensure_coefs(g, *s, g->Nt))) { b->err = g->err; return rc; }
@@ -1547,6 +1594,15 @@ int batch_build(Batch *b) {
         (rc = batch_upload(b, (void **)&b->d_sbeg, sbeg)) || (rc = batch_upload(b, &b->d_alt, alt)) ||
         (rc = batch_upload(b, (void **)&b->d_rect, rect)))
         return rc;
+    wgR.push_back(totR);
+    b->nrc = (int)rca.size();
+    b->nwgR = totR;
+    b->ninj = (int)inj.size();
+    if (b->nrc && ((rc = batch_upload(b, &b->d_rc, rca)) || (rc = batch_upload(b, (void **)&b->d_wgR, wgR)) ||
+                   (rc = batch_upload(b, (void **)&b->d_rmem, rmem)) ||
+                   (rc = batch_upload(b, (void **)&b->d_rskip, rskip))))
+        return rc;
+    if (b->ninj && (rc = batch_upload(b, &b->d_inj, inj))) return rc;
     for (int p = 0; p < 2; p++)
         for (int c = 0; c < 2; c++)
             if (b->nmem[p][c]) {
@@ -1583,6 +1639,14 @@ int batch_build(Batch *b) {
     }
     const int ebi = b->tile_e & 3, ekz = (b->tile_e >> 2) & 3, hbi = b->tile_h & 3, hkz = (b->tile_h >> 2) & 3;
     for (int s = 0; s < KS && totE > 0; s++) {
+        if (b->ninj)
+            hipLaunchKernelGGL((k_batch_inj<S, T>), dim3((unsigned)((b->ninj + 63) / 64)), dim3(64), 0, b->stream,
+                               (EArgs<S, T> *)b->d_ea, (HArgs<S, T> *)b->d_ha, (const InjB *)b->d_inj, b->ninj,
+                               b->d_tcur, b->d_tend, s);
+        if (b->nrc)
+            hipLaunchKernelGGL((k_rc_batch<S, T>), dim3((unsigned)b->nwgR, 3), dim3(256), 0, b->stream,
+                               (const RcArgs<S, T> *)b->d_rc, b->d_wgR, b->nrc, b->d_rmem, b->d_rskip, b->d_tcur,
+                               b->d_tend, s);
         switch (ebi) {
             case 0: launch_Eb_by<S, T, 1>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
             case 1: launch_Eb_by<S, T, 2>(ekz, totE, b->ldsE, b->stream, b->d_ea, b->d_wgE, n, b->d_tcur, b->d_tend, s); break;
@@ -1659,8 +1723,7 @@ int batch_add(Batch *b, aefdtd_grid *g) {
     };
     if (g->precision != b->precision) return bad("grid precision differs from the batch's");
     if (std::find(b->members.begin(), b->members.end(), g) != b->members.end()) return bad("grid already in the batch");
-    if (g->tfsf_aux || g->inj_k >= 0 || !g->sums.empty()) return bad("TFSF, plane injection and plane sums are not batched");
-    if (is_column(g)) return bad("1x1xN column grids are not batched");
+    if (!g->sums.empty()) return bad("plane sums (normal incidence) are not batched");
     if (g->tstep >= g->Nt) return bad("grid has no steps left");
     if (g->tstep % KS != 0) return bad("a grid joins a batch after a multiple of 64 steps");
     int rc = ensure_ready(g);
@@ -1668,9 +1731,9 @@ int batch_add(Batch *b, aefdtd_grid *g) {
     // the batch stages 64 steps per chunk from an empty stage: flush what the
     // grid staged on its own (its stage is deeper than a chunk)
     if ((rc = with_ctype(g, [&]<typename T>() { return flush_dft<T>(g); }))) { b->err = g->err; return rc; }
-    if (g->n_disp) return bad("grids with dispersive (RC) materials are not batched");
     BHIPOK(b, hipStreamSynchronize(g->stream));
     BHIPOK(b, hipStreamSynchronize(b->stream));
+    g->strip_valid = false;             // batch sampling reads the fields, not the strip
     b->members.push_back(g);
     b->dirty = true;
     return AEFDTD_OK;
@@ -1902,6 +1965,7 @@ int aefdtd_destroy(aefdtd_grid *g) {
     dfree_t(g->d_energy);
     dfree_t(g->d_en_part);
     dfree_t(g->d_absmax);
+    dfree_t(g->d_inj);
     dfree(g->d_osc);
     for (void *&p : g->tb_alt) dfree(p);
     for (OscDev &o : g->osc) {
@@ -2261,6 +2325,11 @@ int aefdtd_set_plane_injection(aefdtd_grid *g, int k, int n, const double *dEx, 
     if (k < 0 || k >= g->Nz || n < 0 || (n > 0 && (!dEx || !dEy || !dHx || !dHy)))
         return fail(g, AEFDTD_EINVAL, "bad injection plane");
     g->inj_k = k;
+    if (g->d_inj) {
+        HIPOK(g, hipStreamSynchronize(g->stream));
+        g->dev_bytes -= g->inj[0].size() * 4 * sizeof(double);
+        dfree_t(g->d_inj);
+    }
     g->inj[0].assign(dEx, dEx + n); g->inj[1].assign(dEy, dEy + n);
     g->inj[2].assign(dHx, dHx + n); g->inj[3].assign(dHy, dHy + n);
     return AEFDTD_OK;

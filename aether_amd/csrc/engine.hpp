@@ -180,6 +180,7 @@ struct aefdtd_grid {
     // driver actions
     int inj_k = -1;
     std::vector<double> inj[4];
+    double *d_inj = nullptr;          // the same values on the device, [step][dEx, dEy, dHx, dHy] (batches)
     void *tb_alt[7] = {};             // second live-state set of the time-skewed batch passes
     int tb_alt_pol = -1;              // (3 field components, 4 CPML psi), for this polarisation
     size_t tb_alt_bytes = 0;

@@ -1514,6 +1514,55 @@ __global__ __launch_bounds__(256) void k_dft_batch(const DftB<T> *__restrict__ e
     dft_block<T, KS>(q.stage, q.Ptot, q.off, q.P, q.Nl, q.coef, t0, nfill, q.acc, blockIdx.y);
 }
 
+// Plane injection of batch members (the TFSF source column of a
+// single-particle run): before the passes of step s of a chunk, member r's E
+// and H arguments take the injected values of its step tcur + s from its
+// table ([step][dEx, dEy, dHx, dHy], as aefdtd_set_plane_injection gave them),
+// or no injection past the table's end (as e_args / h_args do for a grid
+// stepped on its own).
+struct InjB {
+    int member, k, n;
+    const double *tab;
+};
+
+template <typename S, typename T>
+__global__ void k_batch_inj(EArgs<S, T> *ea, HArgs<S, T> *ha, const InjB *__restrict__ q, int nq,
+                            const int *__restrict__ tcur, const int *__restrict__ tend, int s) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nq) return;
+    const InjB &b = q[e];
+    const int t = tcur[b.member] + s;
+    if (t >= tend[b.member]) return;
+    EArgs<S, T> &a = ea[b.member];
+    HArgs<S, T> &h = ha[b.member];
+    if (t < b.n) {
+        a.inj_k = b.k;
+        h.inj_k = b.k;
+        a.injx = (T)b.tab[4 * (size_t)t];
+        a.injy = (T)b.tab[4 * (size_t)t + 1];
+        h.injx = (T)b.tab[4 * (size_t)t + 2];
+        h.injy = (T)b.tab[4 * (size_t)t + 3];
+    } else {
+        a.inj_k = -1;
+        h.inj_k = -1;
+    }
+}
+
+// RC pre-pass of the dispersive batch members at step s of a chunk: entry r
+// of the table (member mem[r]) owns workgroups [wg0[r], wg0[r+1]), one lane per
+// dispersive slot, blockIdx.y the component.  skip[r]: a step whose pre-pass
+// the member's last sensor feed on its own already ran (-1: none).
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_rc_batch(const RcArgs<S, T> *__restrict__ rc, const int *__restrict__ wg0,
+                                                  int n, const int *__restrict__ mem, const int *__restrict__ skip,
+                                                  const int *__restrict__ tcur, const int *__restrict__ tend, int s) {
+    const int r = batch_member(wg0, n);
+    const int t = tcur[mem[r]] + s;
+    if (t >= tend[mem[r]] || t == skip[r]) return;
+    const int slot = (blockIdx.x - wg0[r]) * blockDim.x + threadIdx.x;
+    if (slot < rc[r].ndisp) rc_lane(rc[r], slot, blockIdx.y);
+}
+
 // End of a batch chunk: every member advances KS steps, stopping at its end.
 __global__ void k_batch_advance(int *tcur, const int *tend, int n) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;

@@ -993,15 +993,20 @@ int aefdtd_batch_add(aefdtd_batch *b, aefdtd_run *r) {
         b->err = "runs that stop on their completion check are stepped on their own";
         return AEFDTD_ESTATE;
     }
-    if (r->kind != 2) {
-        b->err = "only custom-mode runs are batched";
+    if (r->kind == 0) {
+        b->err = "normal-incidence runs (plane sums) are stepped on their own";
         return AEFDTD_ESTATE;
     }
     if (r->batch) {
         b->err = "run is already in a batch";
         return AEFDTD_ESTATE;
     }
-    TRY(batch_add(b->b, r->g));
+    // a single-particle run brings its TFSF source column, stepped in the same launches
+    if (r->a) TRY(batch_add(b->b, r->a));
+    if (int rc = batch_add(b->b, r->g)) {
+        if (r->a) (void)batch_remove(b->b, r->a);
+        return rc;
+    }
     b->runs.push_back(r);
     r->batch = b;
     return AEFDTD_OK;
@@ -1016,6 +1021,7 @@ int aefdtd_batch_remove(aefdtd_batch *b, aefdtd_run *r) {
         return AEFDTD_EINVAL;
     }
     TRY(batch_remove(b->b, r->g));
+    if (r->a) TRY(batch_remove(b->b, r->a));
     b->runs.erase(it);
     r->batch = nullptr;
     return AEFDTD_OK;

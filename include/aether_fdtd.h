@@ -459,14 +459,18 @@ int aefdtd_run_write(aefdtd_run *r, const char *directory, const char *prefix, c
 /* Waits for all work queued on the grid's stream. */
 int aefdtd_synchronize(aefdtd_grid *g);
 
-/* ---- batches: many independent custom-mode runs stepped together -------- */
+/* ---- batches: many independent runs stepped together -------------------- */
 /* No counterpart in the reference, which steps one FDTD object at a time (its
  * only multi-grid loop is the lockstep pair of modes/oblique_biphase.cpp:
  * 251-292).  Each member advances exactly as aefdtd_run_step would advance
- * it alone; the batch issues one launch per half step for all members and
- * replays chunks of 64 steps from a captured HIP graph.
- *  - Members are kind-2 (custom) runs of the batch's precision without
- *    dispersive materials, added at step 0 (or any multiple of 64).
+ * it alone; the batch issues one launch per half step for all members (after
+ * one for the RC pre-pass of the dispersive ones) and replays chunks of 64
+ * steps from a captured HIP graph.
+ *  - Members are custom-mode (kind 2) and single-particle (kind 1) runs of
+ *    the batch's precision, dispersive materials included, added at step 0
+ *    (or any multiple of 64); a single-particle run's TFSF source column
+ *    joins with it.  Normal-incidence runs (plane sums) and runs with a
+ *    completion check are refused.
  *  - batch_step(b, c) advances every member by up to 64*c steps (never past
  *    its Nt); runs that reach Nt idle until removed.  A member must not be
  *    stepped on its own while in the batch; remove it, then finish it.

@@ -270,10 +270,10 @@ def test_batch_refuses_what_it_cannot_step(engine_be):
     r64 = capi.Run(be, capi.KIND_CUSTOM, finalized(spec, sens), sens)
     with pytest.raises(capi.FDTDError, match="precision"):
         b.add(r64)
-    sp, ss, _ = workloads.spheres_single_particle(n=12, D=2e-9, count=1, rmin=3e-9, rmax=4e-9, seed=5, Nt=10,
-                                                  Nl=4, pml=4, precision=capi.FP32)
-    rs = capi.Run(be, capi.KIND_SINGLE, sp, ss)
-    with pytest.raises(capi.FDTDError, match="custom"):
+    sp = workloads.slab_normal_incidence(n=(4, 4, 40), slab=(10, 20), Nt=10, Nl=4)
+    sp.precision = capi.FP32
+    rs = capi.Run(be, capi.KIND_NORMAL, sp)
+    with pytest.raises(capi.FDTDError, match="normal-incidence"):
         b.add(rs)
     spec, sens, _ = small_run(42, 30, 100, capi.FP32)
     r = capi.Run(be, capi.KIND_CUSTOM, finalized(spec, sens), sens)
