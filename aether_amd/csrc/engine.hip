@@ -1068,8 +1068,7 @@ int flush_dft(aefdtd_grid *g) {
             hipEvent_t ev;
             prof_begin(g, 3, &ev);
             hipLaunchKernelGGL(k_dft_mx, dim3((unsigned)((maxP + 127) / 128), (unsigned)(6 * n)), dim3(256), 0,
-                               g->stream, (const float *)g->d_stage, g->stage_P, stage_depth(g->precision), set, s0,
-                               g->stage_fill);
+                               g->stream, (const float *)g->d_stage, g->stage_P, set, s0, g->stage_fill);
             prof_end(g, ev);
             HIPOK(g, hipGetLastError());
             continue;

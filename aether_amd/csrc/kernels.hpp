@@ -1395,21 +1395,27 @@ __global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, const Df
 // component c of a sensor plane, the block's sums are a product
 //   D[l'][p] = sum_s W[s][l'] X[s][p],   l' = 2 l + (0 re, 1 im),
 // of the coefficient table W (steps x 2 Nl, in FP32) and the staged samples X
-// (steps x cells), formed with v_mfma_f32_32x32x2f32: A = W^T (32 rows l'),
-// B = X (32 cells), 2 steps per product.  A wavefront owns 32 cells of one
-// component and up to MXB blocks of 32 l' (D in 16 registers per block); each
-// product step reads one sample per lane from the stage (rows s and s+1, 32
-// consecutive cells each) and one coefficient per block (from L2, shared by
-// every wavefront of the flush).  The sums are then added to the FP64
-// accumulators, lane = cell, so that each accumulator row is read and written
-// as 32 consecutive double2.  Output layout of a 32x32 block: lane L holds
-// column L % 32 and rows 8 (r / 4) + 4 (L / 32) + r % 4 in its register r.
-// The matrix cores take the multiply-adds off the vector ALU (whose FP32
-// multiply-adds, unpacked, would bound the flush), so the flush runs at the
-// speed of its HBM traffic: the accumulators read and written once per block
-// and the samples read once.
+// (steps x cells).  The product runs on v_mfma_f32_32x32x16_bf16 with both
+// operands split into two bf16 parts, w = wh + wl and x = xh + xl (wh, xh
+// rounded to bf16, wl, xl the rounded remainders), as
+//   w x ~ wh xh + wh xl + wl xh,
+// three bf16 products whose dropped term wl xl and the rounding of the
+// remainders leave about 2^-17 of each term (about FP32's product error over a
+// block of steps); at 16 times the rate of the FP32 matrix op, the three cost
+// a fifth of one FP32 product, which would otherwise take as long as the
+// flush's HBM traffic.  A = W^T (32 rows l'), B = X (32 cells), 16 steps per
+// product: lane L supplies steps 8 (L / 32) + e, e = 0..7, of row or cell
+// L % 32 in both operands.  A wavefront owns 32 cells of one component and up
+// to MXB blocks of 32 l' (D in 16 registers per block); the four wavefronts
+// of a workgroup (128 cells) share each batch's coefficients, split once into
+// LDS.  The sums are then added to the FP64 accumulators, lane = cell, so that
+// each accumulator row is read and written as 32 consecutive double2.  Output
+// layout of a 32x32 block: lane L holds column L % 32 and rows
+// 8 (r / 4) + 4 (L / 32) + r % 4 in its register r.
 constexpr int MXB = 4;                     // l' blocks per pass of a wavefront
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 struct DftDescMx {
     long off, P;
@@ -1422,70 +1428,128 @@ struct DftSetMx {
     DftDescMx d[DFT_MAX_SENS];
 };
 
-// Product steps are requested MX_U at a time, one batch ahead of the one
-// being multiplied; steps beyond nfill (the prefetch past the block's end)
-// read clamped rows of the stage and of the coefficient table, and their
-// samples count zero.
-constexpr int MX_U = 8;
+__device__ __forceinline__ void bf16_split(const f32x8 &v, bf16x8 &hi, bf16x8 &lo) {
+    hi = __builtin_convertvector(v, bf16x8);
+    lo = __builtin_convertvector(v - __builtin_convertvector(hi, f32x8), bf16x8);
+}
 
-__global__ __launch_bounds__(256) void k_dft_mx(const float *stage, long Ptot, int KD, const DftSetMx set, int s0,
-                                                int nfill) {
+// Steps go in batches of MX_K, one product each.  A batch's coefficients are
+// loaded one batch ahead (thread t: row l' = 32 b0 + t % 128, steps
+// 8 (t / 128) + e) and split into LDS at the end of the batch before; each
+// lane requests its samples two batches ahead, after the coefficients, so that
+// waiting for the coefficients never waits for those samples.  Steps past
+// nfill read the block's last staged row and a zero coefficient.
+constexpr int MX_K = 16;
+
+__global__ __launch_bounds__(256, 2) void k_dft_mx(const float *stage, long Ptot, const DftSetMx set, int s0, int nfill) {
+    __shared__ bf16x8 Lh[2][MXB * 64], Ll[2][MXB * 64];
     const DftDescMx &d = set.d[blockIdx.y / 6];
+    if ((long)blockIdx.x * 128 >= d.P) return;              // the whole workgroup: no cell of this sensor
     const int c = blockIdx.y % 6, h = c >= 3 ? 1 : 0;
-    const int lane = threadIdx.x & 63;
-    const long p0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-    if (p0 >= d.P) return;
-    const long p = p0 + (lane & 31);
-    const bool live = p < d.P;
-    const int kk = lane >> 5;                               // step of the pair this lane supplies
-    const float *xs = stage + (size_t)c * Ptot + d.off + (live ? p : p0);
-    const int nb = (2 * d.Nl + 31) / 32;
+    const int tid = threadIdx.x, lane = tid & 63, half = lane >> 5;
+    const long p = ((long)blockIdx.x * 4 + (tid >> 6)) * 32 + (lane & 31);
+    const bool live = p < d.P;                              // lanes past the sensor compute and discard
+    const float *xs = stage + (size_t)c * Ptot + d.off + (live ? p : 0);
+    const int nb = (2 * d.Nl + 31) / 32, nbatch = (nfill + MX_K - 1) / MX_K, last = max(nfill - 1, 0);
+    const int wr = tid & 127, wh = tid >> 7, wslot = (wr >> 5) * 64 + wh * 32 + (wr & 31);
     for (int b0 = 0; b0 < nb; b0 += MXB) {
+        const float *wcol = d.W + (size_t)h * d.LP + min(b0 + (wr >> 5), nb - 1) * 32 + (wr & 31);
+        auto loadw = [&](int b, f32x8 &w) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int rel = b * MX_K + 8 * wh + e;
+                w[e] = wcol[(size_t)min(s0 + rel, d.rows - 1) * 2 * d.LP] * (rel < nfill ? 1.0f : 0.0f);
+            }
+        };
+        auto storew = [&](int buf, const f32x8 &w) {
+            bf16x8 hi, lo;
+            bf16_split(w, hi, lo);
+            Lh[buf][wslot] = hi;
+            Ll[buf][wslot] = lo;
+        };
+        auto loadx = [&](int b, f32x8 &x) {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                x[e] = __builtin_nontemporal_load(xs + (size_t)min(b * MX_K + 8 * half + e, last) * 6 * Ptot);
+        };
         f32x16 D[MXB];
 #pragma unroll
         for (int q = 0; q < MXB; q++)
             for (int r = 0; r < 16; r++) D[q][r] = 0.0f;
-        const float *wr = d.W + (size_t)h * d.LP + (lane & 31);
-        float xa[MX_U], wa[MX_U][MXB], xn[MX_U], wn[MX_U][MXB];
-        auto load = [&](int s, float (&x)[MX_U], float (&w)[MX_U][MXB]) {
+        auto batch = [&](int b, const f32x8 &x, f32x8 &xnext, f32x8 &w) {
+            loadw(b + 1, w);
+            loadx(b + 2, xnext);
+            bf16x8 xh, xl, ah[MXB], al[MXB];
+            bf16_split(x, xh, xl);
 #pragma unroll
-            for (int u = 0; u < MX_U; u++) {
-                const int row = s + 2 * u + kk;
-                const float v = __builtin_nontemporal_load(xs + (size_t)min(row, KD - 1) * 6 * Ptot);
-                x[u] = row < nfill ? v : 0.0f;
-                const float *wu = wr + (size_t)min(s0 + row, d.rows - 1) * 2 * d.LP;
-#pragma unroll
-                for (int q = 0; q < MXB; q++) w[u][q] = wu[min(b0 + q, nb - 1) * 32];   // blocks past nb: unused
+            for (int q = 0; q < MXB; q++) {
+                ah[q] = Lh[b & 1][q * 64 + lane];
+                al[q] = Ll[b & 1][q * 64 + lane];
             }
+#pragma unroll
+            for (int q = 0; q < MXB; q++) D[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[q], xh, D[q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < MXB; q++) D[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[q], xl, D[q], 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < MXB; q++) D[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[q], xh, D[q], 0, 0, 0);
+            storew((b + 1) & 1, w);
+            __syncthreads();
         };
-        load(0, xa, wa);
-        for (int s = 0; s < nfill; s += 2 * MX_U) {
-            load(s + 2 * MX_U, xn, wn);
-#pragma unroll
-            for (int u = 0; u < MX_U; u++)
-#pragma unroll
-                for (int q = 0; q < MXB; q++)
-                    if (b0 + q < nb) D[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[u][q], xa[u], D[q], 0, 0, 0);
-#pragma unroll
-            for (int u = 0; u < MX_U; u++) {
-                xa[u] = xn[u];
-#pragma unroll
-                for (int q = 0; q < MXB; q++) wa[u][q] = wn[u][q];
-            }
+        f32x8 x0, x1, x2, w;
+        loadw(0, w);
+        loadx(0, x0);
+        loadx(1, x1);
+        storew(0, w);
+        __syncthreads();
+        for (int b = 0; b < nbatch; b += 3) {                  // three sample buffers in rotation
+            batch(b, x0, x2, w);
+            if (b + 1 < nbatch) batch(b + 1, x1, x0, w);
+            if (b + 2 < nbatch) batch(b + 2, x2, x1, w);
         }
         if (!live) continue;
+        if ((b0 + MXB) * 32 <= 2 * d.Nl) {
+            // every row of the MXB blocks exists: the accumulators of two
+            // blocks requested before the first of them is written
+#pragma unroll
+            for (int q0 = 0; q0 < MXB; q0 += 2) {
+                double2 v[2][8];
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int r = 0; r < 16; r += 2) {
+                        const int l = ((b0 + q0 + q) * 32 + 8 * (r >> 2) + 4 * half + (r & 3)) >> 1;
+                        v[q][r / 2] = ld_nt(d.acc + ((size_t)c * d.Nl + l) * d.P + p);
+                    }
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int r = 0; r < 16; r += 2) {
+                        const int l = ((b0 + q0 + q) * 32 + 8 * (r >> 2) + 4 * half + (r & 3)) >> 1;
+                        v[q][r / 2].x += (double)D[q0 + q][r];
+                        v[q][r / 2].y += (double)D[q0 + q][r + 1];
+                        st_nt(v[q][r / 2], d.acc + ((size_t)c * d.Nl + l) * d.P + p);
+                    }
+            }
+            continue;
+        }
+        // the eight accumulators of a block are all requested before the
+        // first is written (rows l >= Nl read row Nl - 1 and write nothing)
 #pragma unroll
         for (int q = 0; q < MXB; q++) {
             if (b0 + q >= nb) break;
+            double2 v[8];
 #pragma unroll
             for (int r = 0; r < 16; r += 2) {
-                const int l = ((b0 + q) * 32 + 8 * (r >> 2) + 4 * kk + (r & 3)) >> 1;
+                const int l = ((b0 + q) * 32 + 8 * (r >> 2) + 4 * half + (r & 3)) >> 1;
+                v[r / 2] = ld_nt(d.acc + ((size_t)c * d.Nl + min(l, d.Nl - 1)) * d.P + p);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r += 2) {
+                const int l = ((b0 + q) * 32 + 8 * (r >> 2) + 4 * half + (r & 3)) >> 1;
                 if (l < d.Nl) {
-                    double2 *a = d.acc + ((size_t)c * d.Nl + l) * d.P + p;
-                    double2 v = ld_nt(a);
-                    v.x += (double)D[q][r];
-                    v.y += (double)D[q][r + 1];
-                    st_nt(v, a);
+                    v[r / 2].x += (double)D[q][r];
+                    v[r / 2].y += (double)D[q][r + 1];
+                    st_nt(v[r / 2], d.acc + ((size_t)c * d.Nl + l) * d.P + p);
                 }
             }
         }

@@ -84,6 +84,19 @@ def dft_bytes_per_step(spec, sensors, lay, stats, KD):
     return tot
 
 
+def dft_flush_bytes(spec, sensors, KD, stage_bytes):
+    """Algorithmic bytes of one DFT flush of KD staged steps: each plane cell's
+    6 x KD staged samples read once and its 6 x Nl FP64 complex accumulators
+    (16 B) read and written once."""
+    tot = 0
+    for q in sensors:
+        nx, ny, nz = q.x2 - q.x1, q.y2 - q.y1, q.z2 - q.z1
+        planes = [ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny] if q.box else [max(1, nx) * max(1, ny) * max(1, nz)]
+        for P in planes:
+            tot += P * 6 * (KD * stage_bytes + q.Nl * 32)
+    return tot
+
+
 def e_pass_bytes(lay, pml, stats):
     """Algorithmic HBM bytes of one E-pass launch (k_update_E): read E,H (6
     values) and the uint8 material id, write E (3 values); CPML slab cells
@@ -638,7 +651,8 @@ def main():
         KD64 = stage_depth(capi.FP64)
         f64 = headline_run(args, capi.FP64, KD64, 2 * KD64, KD64)
         f64.pop("_info")
-        rec["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "roofline", "kernels_ms_avg", "tiles", "step_tiles")}
+        rec["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "roofline", "kernels_ms_avg", "dft_roofline", "tiles",
+                                             "step_tiles")}
         rec["fp64"]["steps"] = 2 * KD64
     if not args.no_cpu:
         threads, cpu_note = host_cpu_share()
@@ -716,6 +730,8 @@ def headline_run(args, prec, W, K, P):
     achieved = kbytes / kavg / 1e9
     KD = stage_depth(prec)
     step_bytes = pass_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats, KD)
+    flush_bytes = dft_flush_bytes(spec, sensors, KD, 8 if prec == capi.FP64 else 4)
+    flush_s = d_ms / max(1, d_n) / 1e3
     rec = {
         "value": round(ncell * K / wall / 1e6, 1),
         "ms_per_step": round(wall / K * 1e3, 4),
@@ -729,6 +745,10 @@ def headline_run(args, prec, W, K, P):
                            "dft_us_per_step": round(d_ms * 1e3 / max(1, P), 3),
                            "dft_steps_per_flush": KD,
                            "rc": round(r_ms / max(1, r_n), 5) if r_n else "in the sensor launch"},
+        "dft_roofline": {"bound": "hbm", "achieved": round(flush_bytes / flush_s / 1e9, 1) if d_n else None,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(flush_bytes / flush_s / 1e9 / HBM_PEAK_GBS, 4) if d_n else None,
+                         "kernel": "k_dft_mx" if prec != capi.FP64 else "k_dft", "bytes_per_flush": flush_bytes},
         "step_hbm_fraction": round(step_bytes / (wall / K) / 1e9 / HBM_PEAK_GBS, 4),
         **({"step_pass_GBps": round(kbytes / kavg / 1e9, 1)} if one_pass else
            {"e_pass_GBps": round(e_bytes / e_avg / 1e9, 1), "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1)}),

@@ -110,12 +110,13 @@ def test_sensors_beyond_one_lds_chunk(engine_be, oracle_be, Nl):
 
 @pytest.mark.parametrize("Nl,Nt", [(8, 20), (17, 37), (50, 300), (128, 513)])
 def test_fp32_matrix_core_flush_against_oracle(engine_be, oracle_be, Nl, Nt):
-    """The FP32 flush (k_dft_mx: 32 wavelength columns per matrix block, blocks
-    of 4, 256-step staging) on runs whose last flush is partial and whose
-    wavelength count fills 1, 2, 4 and 8 blocks, some partly: FP32 spectra
-    within 1e-4 of their peak of the FP64 oracle.  Runs of fewer steps than the
-    coefficient table's prefetch depth end their single flush at the table's
-    last row (the flush reads clamped rows past it)."""
+    """The FP32 flush (k_dft_mx: split-bf16 matrix products of 16 steps, 32
+    wavelength columns per block, blocks of 4, 256-step staging) on runs whose
+    last flush is partial and whose wavelength count fills 1, 2, 4 and 8
+    blocks, some partly: FP32 spectra within 1e-4 of their peak of the FP64
+    oracle.  Runs of fewer steps than the flush's prefetch depth end their
+    single flush at the table's last row (the flush reads clamped rows past
+    it)."""
     spec, sens, _ = workloads.spheres_single_particle(n=16, D=2e-9, count=2, rmin=3e-9, rmax=5e-9, seed=9,
                                                       Nt=Nt, Nl=Nl, pml=5, precision=capi.FP64)
     r = capi.run_single_particle(oracle_be, spec, sens)["sensors"]

@@ -203,7 +203,10 @@ def test_batch_fp32_alone_equals_mixed(engine_be, oracle_be):
         run.close()
         alone = engine_alone(spec, sens)
         assert rel_l2(mixed["fields"], alone["fields"]) <= 1e-6
-        assert spec_err(mixed["sensors"], alone["sensors"]) <= 1e-6
+        # a grid stepped alone forms its DFT sums on the matrix cores from
+        # split-bf16 operands (about 2^-17 of each term: ~1e-6 of the peak
+        # here); the batch flush multiplies in FP64 on the vector ALU
+        assert spec_err(mixed["sensors"], alone["sensors"]) <= 1e-5
         s64 = capi.ModeSpec(**{**spec.__dict__, "precision": capi.FP64})
         r = capi.run_custom(oracle_be, s64, sens)
         err = spec_err(mixed["sensors"], r["sensors"])
