@@ -1481,6 +1481,7 @@ int batch_build(Batch *b) {
     std::vector<TbAlt<S, T>> alt(n);
     std::vector<RcArgs<S, T>> rca;
     std::vector<int> wgR, rmem, rskip;
+    std::vector<aefdtd_grid *> ahead;   // members whose next pre-pass already ran
     std::vector<InjB> inj;
     int totE = 0, totH = 0, totR = 0, totT[2][2] = {{0, 0}, {0, 0}}, maxmat = 1;
     b->maxP = 0;
@@ -1506,8 +1507,10 @@ int batch_build(Batch *b) {
             totR += (int)((g->n_disp + 255) / 256);
             rmem.push_back(r);
             // the last sensor feed on its own ran the pre-pass of this step
+            // (the pre-pass advances the pole recursions, so the flag is
+            // cleared only once the chunk that skips it exists)
             rskip.push_back(g->rc_ahead ? g->tstep : -1);
-            g->rc_ahead = false;
+            if (g->rc_ahead) ahead.push_back(g);
         }
         if (g->inj_k >= 0) {
             if ((rc = ensure_inj_table(g))) { b->err = g->err; return rc; }
@@ -1671,6 +1674,7 @@ int batch_build(Batch *b) {
     BHIPOK(b, le);
     BHIPOK(b, ce);
     BHIPOK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    for (aefdtd_grid *g : ahead) g->rc_ahead = false;
     b->dirty = false;
     return AEFDTD_OK;
 }
