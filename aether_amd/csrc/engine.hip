@@ -20,7 +20,7 @@ struct Sensor {                   // Spect_Poynting with interpolated samples
     double2 *d_acc = nullptr;     // [6][Nl][P]
     double2 *d_coef = nullptr;    // [ncoef steps][2][Nl]: E then H coefficient
     int ncoef = 0;
-    float *d_coefmx = nullptr;    // FP32-arithmetic grids: [ncoef + 2 steps][2][LP] for the matrix-core flush
+    void *d_coefmx = nullptr;     // [ncoef steps][E|H][LP] for the matrix-core flush: float (FP32 arithmetic) or double
     int LP = 0;                   // 2 Nl rounded up to 32
 };
 
@@ -1024,25 +1024,30 @@ int ensure_coefs(aefdtd_grid *g, Sensor &s, int upto) {
     dfree_t(s.d_coef);
     s.d_coef = (double2 *)p;
     HIPOK(g, copy_sync(g, s.d_coef, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
-    if (g->precision != AEFDTD_FP64) {
-        // the same coefficients in FP32, (re, im) interleaved per wavelength and
-        // each step's row padded with zeros to a multiple of 32 columns
-        const int LP = (2 * s.Nl + 31) / 32 * 32;
-        std::vector<float> m((size_t)n * 2 * LP, 0.0f);
+    // the matrix-core flush's table: the same coefficients in the arithmetic
+    // type, (re, im) interleaved per wavelength and each step's row padded
+    // with zeros to a multiple of 32 columns
+    const int LP = (2 * s.Nl + 31) / 32 * 32;
+    auto table = [&]<typename V>() {
+        std::vector<V> m((size_t)n * 2 * LP, (V)0);
         for (int st = 0; st < n; st++)
             for (int e = 0; e < 2; e++)
                 for (int l = 0; l < s.Nl; l++) {
                     const double2 v = h[((size_t)st * 2 + e) * s.Nl + l];
-                    m[((size_t)st * 2 + e) * LP + 2 * l] = (float)v.x;
-                    m[((size_t)st * 2 + e) * LP + 2 * l + 1] = (float)v.y;
+                    m[((size_t)st * 2 + e) * LP + 2 * l] = (V)v.x;
+                    m[((size_t)st * 2 + e) * LP + 2 * l + 1] = (V)v.y;
                 }
         void *q;
-        if ((rc = dalloc(g, &q, m.size() * sizeof(float)))) return rc;
+        int rq = dalloc(g, &q, m.size() * sizeof(V));
+        if (rq) return rq;
         dfree_t(s.d_coefmx);
-        s.d_coefmx = (float *)q;
+        s.d_coefmx = q;
         s.LP = LP;
-        HIPOK(g, copy_sync(g, s.d_coefmx, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
+        HIPOK(g, copy_sync(g, s.d_coefmx, m.data(), m.size() * sizeof(V), hipMemcpyHostToDevice));
+        return AEFDTD_OK;
+    };
+    if ((rc = g->precision == AEFDTD_FP64 ? table.template operator()<double>() : table.template operator()<float>()))
+        return rc;
     s.ncoef = n;
     return AEFDTD_OK;
 }
@@ -1062,7 +1067,7 @@ int flush_dft(aefdtd_grid *g) {
                 Sensor *s = g->sensors[q0 + q];
                 int rc = ensure_coefs(g, *s, g->sensor_step);
                 if (rc) return rc;
-                set.d[q] = DftDescMx{s->off, s->P, s->Nl, s->LP, s->ncoef, s->d_coefmx, s->d_acc};
+                set.d[q] = DftDescMx{s->off, s->P, s->Nl, s->LP, s->ncoef, (const float *)s->d_coefmx, s->d_acc};
                 maxP = std::max(maxP, s->P);
             }
             hipEvent_t ev;
@@ -1073,22 +1078,20 @@ int flush_dft(aefdtd_grid *g) {
             HIPOK(g, hipGetLastError());
             continue;
         }
-        DftSet set{};
+        // FP64: the FP64 matrix-core flush (8 wavefronts of 16 cells per workgroup)
+        DftSetMx64 set{};
         long maxP = 0;
         for (size_t q = 0; q < n; q++) {
             Sensor *s = g->sensors[q0 + q];
             int rc = ensure_coefs(g, *s, g->sensor_step);
             if (rc) return rc;
-            set.d[q] = DftDesc{s->off, s->P, s->Nl, (const double2 *)s->d_coef, s->d_acc};
+            set.d[q] = DftDescMx64{s->off, s->P, s->Nl, s->LP, s->ncoef, (const double *)s->d_coefmx, s->d_acc};
             maxP = std::max(maxP, s->P);
         }
-        dim3 grid((unsigned)((maxP + 255) / 256), (unsigned)(6 * n), 1);
         hipEvent_t ev;
         prof_begin(g, 3, &ev);
-        // T is double here (FP64 grids)
-        constexpr int KD = stage_depth(AEFDTD_FP64);
-        hipLaunchKernelGGL((k_dft<T, KD>), grid, dim3(256), 0, g->stream, (const T *)g->d_stage, g->stage_P, set, s0,
-                           g->stage_fill);
+        hipLaunchKernelGGL(k_dft_mx64, dim3((unsigned)((maxP + 127) / 128), (unsigned)(6 * n)), dim3(512), 0,
+                           g->stream, (const double *)g->d_stage, g->stage_P, set, s0, g->stage_fill);
         prof_end(g, ev);
         HIPOK(g, hipGetLastError());
     }

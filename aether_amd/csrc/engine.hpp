@@ -127,11 +127,11 @@ int batch_skewed_members(const Batch *b);
 void batch_profile(Batch *b, int on, double *ms, long *chunks);
 
 // DFT staging depth of a grid stepped on its own (steps between accumulator
-// flushes): FP64 grids' flush lane keeps its 64 samples in registers; the
-// matrix-core flush of FP32-arithmetic grids streams its samples, so its
-// depth only sets how often the FP64 accumulators are read and written.
-// Batches flush every 64-step chunk.
-constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 64 : 256; }
+// flushes): the matrix-core flushes stream their samples, so the depth only
+// sets how often the FP64 accumulators are read and written (and the stage's
+// size: 256 x 6 x plane cells x the arithmetic type).  Batches flush every
+// 64-step chunk.
+constexpr int stage_depth(int) { return 256; }
 
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct FieldSensor; // single-wavelength FieldBlock / FieldMap (engine.hip)

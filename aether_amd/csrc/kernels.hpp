@@ -1369,27 +1369,8 @@ __device__ __forceinline__ void dft_block(const T *stage, long Ptot, long off, l
     }
 }
 
-// One sensor of a grid's DFT flush: its plane samples at stage offset `off`
-// (P cells), its wavelengths, coefficient table and accumulators.
-struct DftDesc {
-    long off, P;
-    int Nl;
-    const double2 *coef;
-    double2 *acc;
-};
+// Sensors of one grid flushed by one launch.
 constexpr int DFT_MAX_SENS = 16;
-struct DftSet {
-    DftDesc d[DFT_MAX_SENS];
-};
-
-// The DFT flush of up to DFT_MAX_SENS sensors of one grid in one launch:
-// blockIdx.y = 6 sensor + component, blockIdx.x = block of 256 plane cells.
-template <typename T, int KD>
-__global__ __launch_bounds__(256) void k_dft(const T *stage, long Ptot, const DftSet set, int s0, int nfill) {
-    const DftDesc &d = set.d[blockIdx.y / 6];
-    if ((long)blockIdx.x * blockDim.x >= d.P) return;
-    dft_block<T, KD>(stage, Ptot, d.off, d.P, d.Nl, d.coef, s0, nfill, d.acc, blockIdx.y % 6);
-}
 
 // The DFT flush of FP32-arithmetic grids on the matrix cores.  For one
 // component c of a sensor plane, the block's sums are a product
@@ -1552,6 +1533,125 @@ __global__ __launch_bounds__(256, 2) void k_dft_mx(const float *stage, long Ptot
                     st_nt(v[r / 2], d.acc + ((size_t)c * d.Nl + l) * d.P + p);
                 }
             }
+        }
+    }
+}
+
+// The DFT flush of FP64 grids on the matrix cores: the same product
+//   D[l'][p] = sum_s W[s][l'] X[s][p]
+// in FP64 with v_mfma_f64_16x16x4f64: A = W^T (16 rows l'), B = X (16 cells),
+// 4 steps per product, lane L supplying step L / 16 of row or cell L % 16 in
+// both operands.  The matrix rate of FP64 equals its vector rate on this part;
+// what the matrix form buys is room: the block sums of 16 cells x 128 l' are
+// spread over the wavefront (4 registers per block), so a flush can take 256
+// staged steps instead of the 64 samples a lane could hold, and the FP64
+// accumulators are read and written once per 256 steps.  A workgroup is 8
+// wavefronts (128 cells of one component) sharing each 16-step batch of
+// coefficients through LDS (thread t: l' = 16 b0 + t % 128, steps
+// 4 (t / 128) + e), one batch ahead, with the samples two batches ahead.
+// Output layout of a 16x16 block: lane L holds column L % 16 and rows
+// 4 r + L / 16 in its register r.  Row i of A takes l' = 4 (i % 4) + i / 4,
+// so that lane L's four registers hold l' = 4 (L / 16) + r: the (re, im)
+// pairs of wavelengths 2 (L / 16) and 2 (L / 16) + 1 of the block.  Steps
+// past nfill read the block's last staged row and a zero coefficient.
+constexpr int M64B = 8;                    // l' blocks of 16 per pass (128 l')
+constexpr int M64_K = 16;                  // steps per batch: four products
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+struct DftDescMx64 {
+    long off, P;
+    int Nl, LP;                            // LP: 2 Nl rounded up to 32
+    int rows;                              // steps in W
+    const double *W;                       // [rows][E|H][LP], zero past 2 Nl
+    double2 *acc;                          // [6][Nl][P]
+};
+struct DftSetMx64 {
+    DftDescMx64 d[DFT_MAX_SENS];
+};
+
+__global__ __launch_bounds__(512) void k_dft_mx64(const double *stage, long Ptot, const DftSetMx64 set, int s0,
+                                                  int nfill) {
+    __shared__ double Ws[2][M64_K][M64B * 16];
+    const DftDescMx64 &d = set.d[blockIdx.y / 6];
+    if ((long)blockIdx.x * 128 >= d.P) return;              // the whole workgroup: no cell of this sensor
+    const int c = blockIdx.y % 6, h = c >= 3 ? 1 : 0;
+    const int tid = threadIdx.x, lane = tid & 63, g4 = lane >> 4, col = lane & 15;
+    const int arow = 4 * (col & 3) + (col >> 2);            // l' of the A row this lane supplies
+    const long p = ((long)blockIdx.x * 8 + (tid >> 6)) * 16 + col;
+    const bool live = p < d.P;                              // lanes past the sensor compute and discard
+    const double *xs = stage + (size_t)c * Ptot + d.off + (live ? p : 0);
+    const int nb = (2 * d.Nl + 15) / 16, nbatch = (nfill + M64_K - 1) / M64_K, last = max(nfill - 1, 0);
+    const int wr = tid & 127, wh = tid >> 7;
+    for (int b0 = 0; b0 < nb; b0 += M64B) {
+        const double *wcol = d.W + (size_t)h * d.LP + min(b0 * 16 + wr, d.LP - 1);
+        auto loadw = [&](int b, double (&w)[4]) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int rel = b * M64_K + 4 * wh + e;
+                w[e] = wcol[(size_t)min(s0 + rel, d.rows - 1) * 2 * d.LP] * (rel < nfill ? 1.0 : 0.0);
+            }
+        };
+        auto storew = [&](int buf, const double (&w)[4]) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) Ws[buf][4 * wh + e][wr] = w[e];
+        };
+        auto loadx = [&](int b, double (&x)[4]) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                x[u] = __builtin_nontemporal_load(xs + (size_t)min(b * M64_K + 4 * u + g4, last) * 6 * Ptot);
+        };
+        f64x4 D[M64B];
+#pragma unroll
+        for (int q = 0; q < M64B; q++)
+            for (int r = 0; r < 4; r++) D[q][r] = 0.0;
+        auto batch = [&](int b, const double (&x)[4], double (&xnext)[4], double (&w)[4]) {
+            loadw(b + 1, w);
+            loadx(b + 2, xnext);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int q = 0; q < M64B; q++)
+                    D[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ws[b & 1][4 * u + g4][q * 16 + arow], x[u], D[q],
+                                                               0, 0, 0);
+            storew((b + 1) & 1, w);
+            __syncthreads();
+        };
+        double x0[4], x1[4], x2[4], w[4];
+        loadw(0, w);
+        loadx(0, x0);
+        loadx(1, x1);
+        storew(0, w);
+        __syncthreads();
+        for (int b = 0; b < nbatch; b += 3) {                  // three sample buffers in rotation
+            batch(b, x0, x2, w);
+            if (b + 1 < nbatch) batch(b + 1, x1, x0, w);
+            if (b + 2 < nbatch) batch(b + 2, x2, x1, w);
+        }
+        if (!live) continue;
+        // register r of block q: l' = 16 (b0 + q) + 4 g4 + r, i.e. l = 8 (b0 + q) + 2 g4 + r / 2,
+        // (re, im) = (r even, r odd);
+        // the accumulators of four blocks requested before the first is written
+#pragma unroll
+        for (int q0 = 0; q0 < M64B; q0 += 4) {
+            double2 v[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int l = 8 * (b0 + q0 + q) + 2 * g4 + t;
+                    v[q][t] = ld_nt(d.acc + ((size_t)c * d.Nl + min(l, d.Nl - 1)) * d.P + p);
+                }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int l = 8 * (b0 + q0 + q) + 2 * g4 + t;
+                    if (l < d.Nl) {
+                        v[q][t].x += D[q0 + q][2 * t];
+                        v[q][t].y += D[q0 + q][2 * t + 1];
+                        st_nt(v[q][t], d.acc + ((size_t)c * d.Nl + l) * d.P + p);
+                    }
+                }
         }
     }
 }

@@ -456,9 +456,10 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
         }
         for (int f = 0; f < nf; f++) {
             if (nf == 6 && q.disable[f]) continue;
-            // accumulators, the FP64 coefficient table and (FP32 arithmetic) its FP32 copy
+            // accumulators, the FP64 coefficient table and its copy for the
+            // matrix-core flush (in the arithmetic type)
             b += 6 * (size_t)q.Nl * P[f] * 16 + (size_t)std::max(m->Nt, 1) * 2 * q.Nl * 16;
-            if (p != AEFDTD_FP64) b += (size_t)std::max(m->Nt, 1) * 2 * ((2 * q.Nl + 31) / 32 * 32) * 4;
+            b += (size_t)std::max(m->Nt, 1) * 2 * ((2 * q.Nl + 31) / 32 * 32) * (p == AEFDTD_FP64 ? 8 : 4);
             Ptot += P[f];
         }
     }

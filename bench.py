@@ -36,9 +36,8 @@ KS = 64                 # steps per batch chunk (and the batches' DFT flush)
 
 def stage_depth(prec):
     """DFT staging depth of a grid stepped on its own (steps per accumulator
-    flush): 64 for FP64 grids, 256 for FP32-arithmetic grids."""
-    from aether_amd import capi
-    return 64 if prec == capi.FP64 else 256
+    flush): 256 in every precision."""
+    return 256
 
 
 def host_cpu_share():
@@ -750,7 +749,7 @@ def headline_run(args, prec, W, K, P):
         "dft_roofline": {"bound": "hbm", "achieved": round(flush_bytes / flush_s / 1e9, 1) if d_n else None,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(flush_bytes / flush_s / 1e9 / HBM_PEAK_GBS, 4) if d_n else None,
-                         "kernel": "k_dft_mx" if prec != capi.FP64 else "k_dft", "bytes_per_flush": flush_bytes},
+                         "kernel": "k_dft_mx" if prec != capi.FP64 else "k_dft_mx64", "bytes_per_flush": flush_bytes},
         "step_hbm_fraction": round(step_bytes / (wall / K) / 1e9 / HBM_PEAK_GBS, 4),
         **({"step_pass_GBps": round(kbytes / kavg / 1e9, 1)} if one_pass else
            {"e_pass_GBps": round(e_bytes / e_avg / 1e9, 1), "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1)}),
