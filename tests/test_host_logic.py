@@ -201,6 +201,24 @@ def test_estimate_bytes_closed_form():
     assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
 
 
+def test_estimate_bytes_sensor_terms():
+    """A plane spectral-Poynting sensor adds its FP64 accumulators (6 x Nl x P
+    complex), the FP64 coefficient table (Nt x E|H x Nl complex), the matrix
+    flush's copy of it in the arithmetic type (Nt x E|H x LP, LP = 2 Nl
+    rounded up to 32) and the 256-step stage of 6 samples per plane cell."""
+    be = capi.engine()
+    g = np.zeros((6, 5, 7), dtype=np.uint32)
+    Nt, Nl = 10, 10
+    sens = [capi.SensorReq(box=False, orientation=capi.NORMAL_Z, x1=1, x2=5, y1=1, y2=4, z1=3, z2=3, Nl=Nl)]
+    P, LP = 4 * 3, 32
+    for prec, ss, cs in ((capi.FP64, 8, 8), (capi.FP32, 4, 4), (capi.FP16, 2, 4)):
+        spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(2.0)], Nt=Nt, precision=prec)
+        NA = 64 * 5 * 6
+        b = 6 * NA * ss + NA + 6 * (7 + 5 + 6) * cs
+        b += 6 * Nl * P * 16 + Nt * 2 * Nl * 16 + Nt * 2 * LP * cs + 256 * 6 * P * cs
+        assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec, sens) == b + b // 50 + (8 << 20), prec
+
+
 def test_server_refuses_request_over_budget():
     """A request larger than the whole budget fails at admission with ENOMEM
     (no device work), and the server keeps serving."""
