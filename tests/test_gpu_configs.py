@@ -2,8 +2,8 @@
 staged DFT beyond one 32-wavelength LDS chunk.
 
 - The 256^3 scatterer configuration (300^3 grid, 40 Drude + 2 CP spheres,
-  12-cell CPML on six faces, TFSF, 64-wavelength box sensor) in FP64 for 130
-  steps (two 64-step DFT flushes and a partial one): fields <= 1e-10 relative
+  12-cell CPML on six faces, TFSF, 64-wavelength box sensor) in FP64 for 520
+  steps (two 256-step DFT flushes and a partial one): fields <= 1e-10 relative
   L2, spectra <= 1e-9 of their peak against the oracle.
 - The same configuration in FP32 over 3000 steps: box spectra within 1e-4 of
   the FP64 engine's (which the test above holds to the oracle).
@@ -32,8 +32,8 @@ def rel_l2(a, b):
 
 
 def test_config2_full_size_fp64_against_oracle(engine_be, oracle_be):
-    print("config 2 at size: 300^3 grid, 40 spheres, FP64, 130 steps", flush=True)
-    spec, sens, sph = workloads.spheres_single_particle(Nt=130, precision=capi.FP64)
+    print("config 2 at size: 300^3 grid, 40 spheres, FP64, 520 steps", flush=True)
+    spec, sens, sph = workloads.spheres_single_particle(Nt=520, precision=capi.FP64)
     assert len(sph) == 40 and sens[0].Nl == 64
     g = capi.run_single_particle(engine_be, spec, sens, want_fields=True)
     print("engine done; oracle (16 threads)", flush=True)
