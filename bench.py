@@ -148,8 +148,10 @@ def pmc_traffic(prec, n, kernel="k_update_E"):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE runs, FETCH scaled by the copy-kernel calibration factor) on this
     workload; read from the committed counter summary, None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_single_particle_256_fp32_pmc.txt")
-    if prec == 1 and n == 256 and os.path.exists(path):
+    from aether_amd import capi
+    name = {capi.FP32: "fp32", capi.FP64: "fp64"}.get(prec)
+    path = os.path.join(ROOT, "profiles", f"{kernel}_single_particle_256_{name}_pmc.txt")
+    if name and n == 256 and os.path.exists(path):
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(float(line.split(":")[1]))
