@@ -21,6 +21,19 @@
 //
 // Threading follows the reference: a persistent pool of Nthreads workers and a
 // barrier between phases, with the z->y->x slab rule of fdtd_threads.cpp:56-82.
+//
+// Pins (tests/test_oracle_pins.py, tests/test_field_sensors.py,
+// tests/test_completion.py): pole and RC constants against Aether's own
+// compiled material layer and quadrature, coefficients and CPML grading in
+// closed form, the stencils by the discrete-energy invariant and the
+// dispersion recurrence, CPML by absorption and the transparent-CPML wall
+// test, TFSF by the field outside the box in a uniform medium, the extension
+// by edge replication, the normal-mode drivers against Airy.
+// Parity unpinned (only the reference's text defines them, or no check above
+// reaches them): the kappa_y gating (fdtd_core.cpp:399), the sensor tapering
+// step (sensors.cpp:236-240), the CPML psi slab bounds beyond absorption and
+// the wall test (fdtd_pml.cpp:504-929), and the custom driver's phase sequence
+// (default_fdtd.cpp:148-165) beyond the oscillator and sensor pins.
 
 #include "oracle.h"
 
