@@ -8,6 +8,13 @@
 // bindings in structure/lua_structure.cpp.  Plain double arithmetic in the
 // reference's order; built without -march, so x86-64 has no fused
 // multiply-add to contract into.
+//
+// Pins (tests/test_structure.py): integer lattice-point counts of closed balls
+// and discs, half-open blocks and layers, later-wins and the default material,
+// loop images against a rolled grid, flip against a mirrored grid.  Parity
+// unpinned: the axis normalisation in the cylinder and cone constructors for
+// axes that are not along x, y or z (pinned only through axis-aligned cases,
+// where it is exact).
 #include <algorithm>
 #include <cmath>
 #include <memory>
