@@ -1,5 +1,5 @@
 """The benchmark configurations at their own sizes against the oracle, and the
-staged DFT beyond one 32-wavelength LDS chunk.
+staged DFT flush over more wavelengths than one pass of its matrix blocks.
 
 - The 256^3 scatterer configuration (300^3 grid, 40 Drude + 2 CP spheres,
   12-cell CPML on six faces, TFSF, 64-wavelength box sensor) in FP64 for 520
@@ -7,8 +7,10 @@ staged DFT beyond one 32-wavelength LDS chunk.
   L2, spectra <= 1e-9 of their peak against the oracle.
 - The same configuration in FP32 over 3000 steps: box spectra within 1e-4 of
   the FP64 engine's (which the test above holds to the oracle).
-- Sensors with 40, 64 and 128 wavelengths (two to four LDS chunks) against the
-  oracle.
+- Sensors with 40, 64 and 128 wavelengths against the oracle in FP64: 5, 8
+  and 16 blocks of 16 (re, im) rows of the matrix flush (eight blocks per
+  pass, so 128 wavelengths take two passes), across two 256-step flushes and
+  a partial one.
 - The precision configuration's layer stack: normal incidence is invariant in
   x and y, so the full 384-cell stack (12 layers + Lorentz layer, 16-cell
   CPML, 8000 steps, 128 wavelengths) runs on a 4 x 4 cross-section in FP64
@@ -65,9 +67,9 @@ def test_config2_full_size_fp32_spectra_within_1e4(engine_be):
 
 
 @pytest.mark.parametrize("Nl", [40, 64, 128])
-def test_sensors_beyond_one_lds_chunk(engine_be, oracle_be, Nl):
+def test_sensors_many_wavelengths(engine_be, oracle_be, Nl):
     """Three planar sensors with Nl wavelengths on random fields with a dispersive
-    material, 150 steps (two flushes and a partial block) in FP64."""
+    material, 600 steps (two 256-step flushes and a partial one) in FP64."""
     rng = np.random.default_rng(Nl)
     n = (14, 12, 18)
     D = (5e-9,) * 3
@@ -93,7 +95,7 @@ def test_sensors_beyond_one_lds_chunk(engine_be, oracle_be, Nl):
     ids = [[g.add_spect_poynting(capi.NORMAL_Z, (2, 12, 1, 11, 12, 13), lam),
             g.add_spect_poynting(capi.NORMAL_XM, (4, 5, 0, 12, 2, 16), lam),
             g.add_spect_poynting(capi.NORMAL_Y, (0, 14, 7, 8, 3, 15), lam)] for g in grids]
-    for _ in range(150):
+    for _ in range(600):
         for g in grids:
             g.update_E(); g.update_H(); g.feed_sensors()
     e, o = grids
