@@ -307,13 +307,13 @@ int upload_grid(aefdtd_grid *g) {
         any = any || disp[m];
     }
     std::vector<uint8_t> m8(NA, 0);
-    std::vector<int32_t> cells;
+    std::vector<int2> cells;
     for (size_t r = 0; r < rows; r++) {
         const uint32_t *src = g->matsgrid.data() + r * g->Nx;
         uint8_t *dst = m8.data() + r * g->px;
         for (int i = 0; i < g->Nx; i++) {
             dst[i] = (uint8_t)src[i];
-            if (disp[src[i]]) cells.push_back((int32_t)(r * g->px + i));
+            if (disp[src[i]]) cells.push_back(make_int2((int)(r * g->px + i), (int)src[i]));
         }
     }
     if (!g->d_mat) {
@@ -336,12 +336,12 @@ int upload_grid(aefdtd_grid *g) {
         if ((rc = dalloc(g, &p, NA * sizeof(int32_t)))) return rc;
         g->d_slot = (int32_t *)p;
         HIPOK(g, hipMemsetAsync(g->d_slot, 0xFF, NA * sizeof(int32_t), g->stream));     // -1: no slot
-        if ((rc = dalloc(g, &p, cells.size() * sizeof(int32_t)))) return rc;
-        g->d_cells = (int32_t *)p;
-        HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+        if ((rc = dalloc(g, &p, cells.size() * sizeof(int2)))) return rc;
+        g->d_cells = (int2 *)p;
+        HIPOK(g, hipMemcpyAsync(g->d_cells, cells.data(), cells.size() * sizeof(int2), hipMemcpyHostToDevice,
                                 g->stream));
         hipLaunchKernelGGL(k_slot_scatter, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, g->stream,
-                           (const int32_t *)g->d_cells, nd, g->d_slot);
+                           (const int2 *)g->d_cells, nd, g->d_slot);
         HIPOK(g, hipGetLastError());
         if ((rc = dalloc(g, &g->d_corr, (size_t)3 * nd * g->csize()))) return rc;
         size_t np = (size_t)3 * g->np_max * nd;
@@ -653,8 +653,8 @@ template <typename S, typename T>
 RcArgs<S, T> rc_args(aefdtd_grid *g) {
     RcArgs<S, T> r;
     r.ndisp = (int)g->n_disp;
+    r.np_max = g->np_max;
     r.cells = g->d_cells;
-    r.mat = g->d_mat;
     r.mt = (const MatDev<T> *)g->d_mtab;
     r.poles = g->d_poles;
     r.psi = g->d_psi;

@@ -169,7 +169,7 @@ struct aefdtd_grid {
     void *d_mtab = nullptr;           // per-material coefficient table (MatDev<T>)
     double2 *d_poles = nullptr;       // per pole: Crec, Dchi
     int32_t *d_slot = nullptr;        // dispersive slot per cell, -1 elsewhere
-    int32_t *d_cells = nullptr;       // cell of each dispersive slot
+    int2 *d_cells = nullptr;          // cell and material of each dispersive slot
     void *d_corr = nullptr;           // per slot: RC increment of the current E update (T[3])
     double2 *d_psi = nullptr;         // RC accumulators [3*Npmax][Ndisp]
     long n_disp = 0;

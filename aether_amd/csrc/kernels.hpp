@@ -458,9 +458,8 @@ __device__ __forceinline__ int slabH(int v, int N, int pm, int pp) {
 // of RC_apply_E (:38-50), which the E pass adds after its stencil and TFSF.
 template <typename S, typename T>
 struct RcArgs {
-    int ndisp;
-    const int32_t *cells;
-    const uint8_t *mat;
+    int ndisp, np_max;              // np_max: pole states allocated per slot and component
+    const int2 *cells;              // per slot: its cell and material
     const MatDev<T> *mt;
     const double2 *poles;       // [pole]: Crec at 2p, Dchi at 2p+1
     double2 *psi;               // [3 * pole + component][slot]
@@ -468,24 +467,28 @@ struct RcArgs {
     T *corr;
 };
 
-// One lane per (dispersive slot, component); the pole states of the first
-// RC_PRELOAD poles are all requested before any is updated, so their load
-// latencies overlap.
+// One lane per (dispersive slot, component).  The lane's pole states depend
+// only on the slot, so the first RC_PRELOAD of them (up to the grid's largest
+// pole count) are requested at once with the slot's (cell, material) pair;
+// the cell's field and the material's constants follow, and the pole table
+// after them: two memory latencies in a row instead of the four of reading
+// the cell, then its material id, then the material, then its poles.
 constexpr int RC_PRELOAD = 8;
 
 template <typename S, typename T>
 __device__ __forceinline__ void rc_lane(const RcArgs<S, T> &a, int s, int c) {
-    const int id = a.cells[s];
-    const MatDev<T> &G = a.mt[a.mat[id]];
-    const Fld<S, T> &E = c == 0 ? a.Ex : (c == 1 ? a.Ey : a.Ez);
-    const double e = (double)E[id];
     double2 *q = a.psi + (size_t)c * a.ndisp + s;               // pole p at q[3 p ndisp]
     const size_t stride = (size_t)3 * a.ndisp;
-    const int np = G.np;
     double2 v[RC_PRELOAD];
 #pragma unroll
     for (int p = 0; p < RC_PRELOAD; p++)
-        if (p < np) v[p] = q[p * stride];
+        if (p < a.np_max) v[p] = q[p * stride];
+    const int2 cm = a.cells[s];
+    const int id = cm.x;
+    const MatDev<T> &G = a.mt[cm.y];
+    const Fld<S, T> &E = c == 0 ? a.Ex : (c == 1 ? a.Ey : a.Ez);
+    const double e = (double)E[id];
+    const int np = G.np;
     double sum = 0;
     for (int p = 0; p < np; p++) {
         const double2 cr = a.poles[2 * (G.pole0 + p)], dc = a.poles[2 * (G.pole0 + p) + 1];
@@ -2357,10 +2360,10 @@ __global__ __launch_bounds__(256) void k_ft_feed(const int *__restrict__ pts, co
 
 // Slot table of the dispersive cells: slot[cells[i]] = i (the table is -1
 // elsewhere); cells holds device indices, all below the table's length.
-__global__ __launch_bounds__(256) void k_slot_scatter(const int32_t *__restrict__ cells, long n,
+__global__ __launch_bounds__(256) void k_slot_scatter(const int2 *__restrict__ cells, long n,
                                                       int32_t *__restrict__ slot) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) slot[cells[i]] = (int32_t)i;
+    if (i < n) slot[cells[i].x] = (int32_t)i;
 }
 
 // CompletionSensor energy (sensors/s_completion.cpp:160-180): the sum of

@@ -420,7 +420,7 @@ int aefdtd_estimate_bytes(int kind, const aefdtd_mode *m, int ns, const aefdtd_s
                 for (int i = 0; i < m->nx; i++)
                     if (row[i] < (uint32_t)m->n_materials && rc[row[i]]) nd += wjk * mult(0, i);
             }
-        b += 4 * NA + nd * (4 + 3 * cs + 3 * (size_t)npmax * 16);
+        b += 4 * NA + nd * (8 + 3 * cs + 3 * (size_t)npmax * 16);   // slot map, (cell, material), increments, poles
     }
     const size_t cross[3] = {(size_t)d[1] * d[2], (size_t)d[0] * d[2], (size_t)d[0] * d[1]};
     size_t cpml = 0;

@@ -197,7 +197,7 @@ def test_estimate_bytes_closed_form():
     spec = capi.ModeSpec(matsgrid=np.ones((6, 5, 7), dtype=np.uint32), materials=[capi.Material(1.0), workloads.GOLD],
                          Nt=10, precision=capi.FP32)
     N, NA = 7 * 5 * 6, 64 * 5 * 6
-    b = 6 * NA * 4 + NA + 6 * 18 * 4 + 4 * NA + N * (4 + 3 * 4 + 3 * 3 * 16)
+    b = 6 * NA * 4 + NA + 6 * 18 * 4 + 4 * NA + N * (8 + 3 * 4 + 3 * 3 * 16)
     assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec) == b + b // 50 + (8 << 20)
 
 
