@@ -1194,23 +1194,25 @@ __global__ __launch_bounds__(256) void k_sample(const SensDesc *sd, int slot, lo
 // The sensor samples of step n and the RC pre-pass of step n+1 in one launch:
 // both only read the fields the H pass of step n left (the RC pass writes its
 // pole states and increments, which nothing else touches), so they share the
-// launch instead of two: workgroups [0, ns) sample (sensor blockIdx / nsx),
-// the rest run the RC pre-pass (component (blockIdx - ns) / nrx).
+// launch instead of two: workgroups [0, 3 nrx) run the RC pre-pass
+// (component blockIdx / nrx), the rest sample (sensor (blockIdx - 3 nrx) /
+// nsx).  The RC lanes go first: each waits on a chain of dependent loads, and
+// dispatched first they wait while the sampling workgroups run.
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_sample_rc(const SensDesc *sd, int slot, long Ptot, Fld<S, T> Ex, Fld<S, T> Ey,
                                                    Fld<S, T> Ez, Fld<S, T> Hx, Fld<S, T> Hy, Fld<S, T> Hz, int Nx,
                                                    int Ny, int Nz, int P, T *stage, const T *strip, int nsx, int ns,
                                                    const RcArgs<S, T> r, int nrx) {
     const int b = blockIdx.x;
-    if (b < ns) {
-        const SensDesc d = sd[b / nsx];
-        const long p = (long)(b % nsx) * 256 + threadIdx.x;
-        if (p < d.P) sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage, strip);
+    if (b < 3 * nrx) {
+        const int s = (b % nrx) * 256 + threadIdx.x;
+        if (s < r.ndisp) rc_lane(r, s, b / nrx);
         return;
     }
-    const int q = b - ns;
-    const int s = (q % nrx) * 256 + threadIdx.x;
-    if (s < r.ndisp) rc_lane(r, s, q / nrx);
+    const int q = b - 3 * nrx;
+    const SensDesc d = sd[q / nsx];
+    const long p = (long)(q % nsx) * 256 + threadIdx.x;
+    if (p < d.P) sample_point(d, p, slot, Ptot, Ex, Ey, Ez, Hx, Hy, Hz, Nx, Ny, Nz, P, stage, strip);
 }
 
 // ---- single-wavelength field sensors (sensors/s_fieldmap.cpp)
