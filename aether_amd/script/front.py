@@ -58,15 +58,29 @@ def _int(v, what):
     return i if i is not None else int(n)   # lua_tointeger truncates a float argument through the C API
 
 
+# Aether's executable and installed-resources directories (where its mat_lib
+# lives) for the script being run: Front(resources=...) or, by default, the
+# AETHER_RESOURCES variable (os.pathsep-separated).
+_resources = []
+
+
+def _resource_dirs():
+    env = os.environ.get("AETHER_RESOURCES", "")
+    return _resources + [d for d in env.split(os.pathsep) if d]
+
+
 def locate(path, caller_dir):
-    """PathManager::locate_file: the path as given, else relative to the
-    calling script's directory."""
+    """PathManager::locate_file (common/filehdl.cpp:292-339): an absolute path
+    as given; a relative one from the working directory, else from the calling
+    script's directory, else from the executable and resources directories."""
     if os.path.isabs(path) or os.path.exists(path):
         return path
-    p = os.path.join(caller_dir, path)
-    if os.path.exists(p):
-        return p
-    raise LuaError(f"file not found: {path} (searched {caller_dir})")
+    dirs = [caller_dir] + _resource_dirs()
+    for base in dirs:
+        p = os.path.join(base, path)
+        if os.path.exists(p):
+            return p
+    raise LuaError(f"file not found: {path} (searched the working directory, {', '.join(dirs)})")
 
 
 def _orientation(s):
@@ -595,9 +609,10 @@ class Front:
     """One script run: the global functions of src/base.cpp:203-282 that the
     FDTD path needs, bound to the engine backend `be` (or the oracle)."""
 
-    def __init__(self, be=None, out_dir=".", precision=capi.FP64, dry=False):
+    def __init__(self, be=None, out_dir=".", precision=capi.FP64, dry=False, resources=()):
         self.be = be if be is not None else capi.engine()
         self.dry = dry                      # compute() records the mode without running it
+        self.resources = list(resources)    # Aether's executable / share/Aether directories
         self.out_dir = out_dir
         self.precision = precision
         self.results = []
@@ -611,6 +626,14 @@ class Front:
             return self.run(fh.read(), chunk=os.path.basename(path))
 
     def run(self, source, chunk="script"):
+        global _resources
+        saved, _resources = _resources, self.resources
+        try:
+            return self._run(source, chunk)
+        finally:
+            _resources = saved
+
+    def _run(self, source, chunk):
         it = Interpreter(chunk=chunk, out=self.log)
         front = self
 
@@ -658,9 +681,9 @@ class Front:
         return it
 
 
-def run_script(path, be=None, out_dir=".", precision=capi.FP64):
+def run_script(path, be=None, out_dir=".", precision=capi.FP64, resources=()):
     """Runs an Aether simulation script; returns the Front (results, log)."""
-    f = Front(be, out_dir, precision)
+    f = Front(be, out_dir, precision, resources=resources)
     f.run_file(path)
     return f
 

@@ -108,6 +108,34 @@ def test_reference_parametric_structure_test(oracle_be):
 
 
 @needs_ref
+def test_reference_sample_script_from_its_installation(oracle_be, monkeypatch):
+    """samples/fdtd/scripts/nanorods_grid.lua as Aether runs it when installed:
+    its material file (mat_lib/...) and structure script (samples/...) are
+    found under the resources directory, as PathManager::locate_file
+    (common/filehdl.cpp:292-339) searches the working directory, the script's
+    directory, then the executable's and resources directories.  Dry run:
+    each of the three compute() calls records its normal-incidence mode."""
+    monkeypatch.delenv("AETHER_RESOURCES", raising=False)
+    path = os.path.join(REF, "samples/fdtd/scripts/nanorods_grid.lua")
+    with pytest.raises(LuaError, match="Au_1m_5m_Vial"):
+        Front(be=oracle_be, dry=True).run_file(path)
+    f = Front(be=oracle_be, dry=True, resources=[REF])
+    f.run_file(path)
+    assert len(f.results) == 3
+    for mode, (spec, sensors) in f.results:
+        assert mode.kind == 0 and mode.Nt == 20000 and mode.Nl == 2001 and mode.pml[4:] == [25, 25]
+        gold = mode.materials[2]
+        assert gold.eps_inf == pytest.approx(6.0031) and len(gold.drude) == 1
+        assert spec.matsgrid.shape == (30, 60, 60) and int(spec.matsgrid.max()) == 2
+    assert [m.polarization for m, _ in f.results] == ["TE", "TM", "TE"]
+    # the same through the environment variable
+    monkeypatch.setenv("AETHER_RESOURCES", REF)
+    g = Front(be=oracle_be, dry=True)
+    g.run_file(path)
+    assert len(g.results) == 3
+
+
+@needs_ref
 def test_reference_sample_structures():
     s = Structure(os.path.join(REF, "samples/fdtd/structures/nanorods_grid.lua")).finalize()
     assert (s.lx, s.ly, s.lz) == (300e-9, 300e-9, 150e-9) and s.default == 0
