@@ -83,7 +83,7 @@ class CNormalResult(C.Structure):
                 ("index_sup", C.c_double), ("index_sub", C.c_double), ("Nt_run", C.c_int)]
 
 
-OP_BLOCK, OP_SPHERE, OP_CYLINDER, OP_CONE, OP_ELLIPSOID, OP_LAYER = range(6)
+OP_BLOCK, OP_SPHERE, OP_CYLINDER, OP_CONE, OP_ELLIPSOID, OP_LAYER, OP_COATING = range(7)
 
 
 class CStructureOp(C.Structure):
@@ -446,6 +446,10 @@ class Structure:
         """axis "X"/"x", "Y"/"y", anything else z, as add_layer (lua_structure.cpp:420-440)."""
         a = 0 if axis in ("X", "x") else (1 if axis in ("Y", "y") else 2)
         return self._add(OP_LAYER, (a, z1, z2), index)
+
+    def add_coating(self, theta, phi, thickness, delta, origin, index):
+        """add_coating (lua_structure.cpp:314-331): angles in degrees."""
+        return self._add(OP_COATING, (theta, phi, thickness, delta, int(origin)), index)
 
     def default_material(self, index):
         self.default = int(index)

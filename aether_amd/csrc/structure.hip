@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numbers>
 #include <string>
 #include <vector>
 
@@ -88,11 +89,26 @@ int prepare(const aefdtd_structure_op &q, OpDev &o) {
         o.a[2] = p[2];
         return AEFDTD_OK;
     }
+    case AEFDTD_OP_COATING: {   // Add_Coating ctor (:45-53), angles as Degree() converts them
+        double theta = p[0], phi = p[1];
+        theta *= std::numbers::pi / 180.0;
+        phi *= std::numbers::pi / 180.0;
+        const double v[6] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), -std::cos(theta),
+                             p[2], p[3], p[4]};
+        // the support march ends past `thickness` only with delta > 0, the
+        // march back up to lz only when it climbs (theta below 90 degrees)
+        if (!(p[3] > 0) || !(p[2] >= 0) || !std::isfinite(p[2]) || !(v[2] < 0) || p[4] < 0 ||
+            (double)(int)p[4] != p[4])
+            return AEFDTD_EINVAL;
+        std::copy(v, v + 6, o.a);
+        return AEFDTD_OK;
+    }
     }
     return AEFDTD_EINVAL;
 }
 
-// One operation's index() at (x, y, z): its material, or -1.
+// One operation's index() at (x, y, z): its material, or -1 (every operation
+// but a coating).
 __device__ __forceinline__ int op_index(const OpDev *__restrict__ ops, int l, double x, double y, double z) {
     const OpDev &o = ops[l];
     const double *a = o.a;
@@ -146,6 +162,65 @@ struct StructDev {
     int flip[3], loop[3];
 };
 
+// Structure::index(x, y, z, level) (structure.cpp:228-262) over the
+// operations below `level`, none of them a coating: flips, folding on looped
+// axes, then the operations last first, each with its images.
+__device__ int stack_index(const OpDev *__restrict__ ops, const StructDev &s, int level, double x, double y,
+                           double z) {
+    if (s.flip[0]) x = s.lx - x;
+    if (s.flip[1]) y = s.ly - y;
+    if (s.flip[2]) z = s.lz - z;
+    if (s.loop[0]) x = fold(x, s.lx);
+    if (s.loop[1]) y = fold(y, s.ly);
+    if (s.loop[2]) z = fold(z, s.lz);
+    for (int l = level - 1; l >= 0; l--)
+        for (int a = -s.loop[0]; a <= s.loop[0]; a++)
+            for (int b = -s.loop[1]; b <= s.loop[1]; b++)
+                for (int c = -s.loop[2]; c <= s.loop[2]; c++) {
+                    const int v = op_index(ops, l, x + a * s.lx, y + b * s.ly, z + c * s.lz);
+                    if (v != -1) return v;
+                }
+    return s.def;
+}
+
+// Add_Coating::index (structure_primitives.cpp:55-96) of operation l: the
+// point must be of the origin material under the operations below l; a
+// support of another material within `thickness` along dir (the loop tests
+// one point past it, as the reference's does); nothing but the origin
+// material from the point back up to lz.  The host bounds both marches
+// (cap1, cap2 points) from thickness / delta and the grid's z extent, so
+// that the caps are never what ends them.
+__device__ int coating_index(const OpDev *__restrict__ ops, const StructDev &s, int l, double x, double y, double z) {
+    const OpDev &o = ops[l];
+    const double dx = o.a[0], dy = o.a[1], dz = o.a[2], th = o.a[3], delta = o.a[4];
+    const int origin = (int)o.a[5], cap1 = (int)o.a[6], cap2 = (int)o.a[7];
+    if (stack_index(ops, s, l, x, y, z) != origin) return -1;
+    double px = x, py = y, pz = z;
+    bool support = false;
+    for (int i = 0; i < cap1; i++) {
+        const double ex = px - x, ey = py - y, ez = pz - z;
+        if (!(ex * ex + ey * ey + ez * ez <= th * th && pz > 0)) break;
+        const double a = i * delta;
+        px = x + a * dx;
+        py = y + a * dy;
+        pz = z + a * dz;
+        if (stack_index(ops, s, l, px, py, pz) != origin) {
+            support = true;
+            break;
+        }
+    }
+    if (!support) return -1;
+    pz = z;
+    for (int i = 0; i < cap2 && pz <= s.lz; i++) {
+        const double a = i * delta;
+        px = x - a * dx;
+        py = y - a * dy;
+        pz = z - a * dz;
+        if (stack_index(ops, s, l, px, py, pz) != origin) return -1;
+    }
+    return o.mat;
+}
+
 __global__ __launch_bounds__(256) void k_discretize(const OpDev *__restrict__ ops, StructDev s, int Nx, int Ny,
                                                     int Nz, double Dx, double Dy, double Dz,
                                                     uint32_t *__restrict__ out) {
@@ -161,10 +236,12 @@ __global__ __launch_bounds__(256) void k_discretize(const OpDev *__restrict__ op
     int m = s.def;
     for (int l = s.n_ops - 1; l >= 0; l--) {
         int hit = -1;
+        const bool coat = ops[l].type == AEFDTD_OP_COATING;
         for (int a = -s.loop[0]; a <= s.loop[0] && hit == -1; a++)
             for (int b = -s.loop[1]; b <= s.loop[1] && hit == -1; b++)
                 for (int c = -s.loop[2]; c <= s.loop[2] && hit == -1; c++)
-                    hit = op_index(ops, l, x + a * s.lx, y + b * s.ly, z + c * s.lz);
+                    hit = coat ? coating_index(ops, s, l, x + a * s.lx, y + b * s.ly, z + c * s.lz)
+                               : op_index(ops, l, x + a * s.lx, y + b * s.ly, z + c * s.lz);
         if (hit != -1) {
             m = hit;
             break;
@@ -197,9 +274,26 @@ extern "C" int aefdtd_discretize(const aefdtd_structure *s, int Nx, int Ny, int 
         if (sd.loop[d] && !(L[d] > 0)) return sfail(AEFDTD_EINVAL, "discretize: a looped axis needs a positive length");
     }
     std::vector<OpDev> ops(std::max(1, s->n_ops));
-    for (int l = 0; l < s->n_ops; l++)
+    bool coated = false;
+    for (int l = 0; l < s->n_ops; l++) {
         if (prepare(s->ops[l], ops[l]))
             return sfail(AEFDTD_EINVAL, "discretize: operation " + std::to_string(l) + " is malformed");
+        if (ops[l].type != AEFDTD_OP_COATING) continue;
+        if (coated)
+            return sfail(AEFDTD_EINVAL, "discretize: a coating over another coating is not provided");
+        coated = true;
+        // points the marches test: the support march at most thickness / delta
+        // + 2; the climb from the lowest point a lane can start at (a grid
+        // corner, flipped, or its image one period below) to above lz
+        const double *a = ops[l].a;
+        const double zr = std::max(Nz * Dz, std::fabs(s->lz)), z0 = -(2 * zr + std::fabs(s->lz));
+        const double n1 = a[3] / a[4] + 2, n2 = (s->lz - z0) / (a[4] * -a[2]) + 2;
+        if (!(n1 <= 1e6) || !(n2 <= 1e7))
+            return sfail(AEFDTD_EINVAL, "discretize: coating " + std::to_string(l) +
+                                            " needs too many steps (thickness / delta, or lz / (delta cos theta))");
+        ops[l].a[6] = std::ceil(n1) + 1;
+        ops[l].a[7] = std::ceil(n2) + 1;
+    }
     const size_t N = (size_t)Nx * Ny * Nz;
     hipStream_t st = nullptr;
     OpDev *d_ops = nullptr;

@@ -216,15 +216,23 @@ class Structure:
         def add_layer(axis, z1, z2, index):
             s.add_layer(tostring(axis), _num(z1, "add_layer"), _num(z2, "add_layer"), _int(index, "add_layer"))
 
+        def add_coating(*a):
+            # lua_structure.cpp:314-331: four numbers, then two integers
+            if len(a) < 6:
+                raise LuaError("add_coating: 6 arguments expected")
+            s.add_coating(*[_num(x, "add_coating") for x in a[:4]], _int(a[4], "add_coating"),
+                          _int(a[5], "add_coating"))
+
         G = it.G
         G.update(add_block=op(s.add_block, 6, "add_block"), add_sphere=op(s.add_sphere, 4, "add_sphere"),
                  add_cylinder=op(s.add_cylinder, 7, "add_cylinder"), add_cone=op(s.add_cone, 7, "add_cone"),
                  add_ellipsoid=op(s.add_ellipsoid, 6, "add_ellipsoid"), add_layer=add_layer,
+                 add_coating=add_coating,
                  default_material=lambda i: s.default_material(_int(i, "default_material")),
                  flip=lambda x, y, z: s.flip(_int(x, "flip"), _int(y, "flip"), _int(z, "flip")),
                  loop=lambda x, y, z: s.loop(_int(x, "loop"), _int(y, "loop"), _int(z, "loop")),
                  declare_parameter=declare_parameter, nearest_integer=nearest_integer)
-        for name in ("add_coating", "add_conformal_coating", "add_lua_def", "add_mesh", "add_sin_layer",
+        for name in ("add_conformal_coating", "add_lua_def", "add_mesh", "add_sin_layer",
                      "add_vect_block", "add_vect_tri", "random_packing", "trapz"):
             G[name] = (lambda n: (lambda *a: (_ for _ in ()).throw(
                 LuaError(f"{n}: this structure operation is not provided by the device discretiser"))))(name)

@@ -347,11 +347,21 @@ int aefdtd_mode_finalize(aefdtd_mode *m, int kind, int n_sensors, const aefdtd_s
  *   ELLIPSOID p = x y z rx ry rz               add_ellipsoid (:399-418)
  *   LAYER     p = axis z1 z2                   add_layer (:420-440): axis 0 "X",
  *                                              1 "Y", 2 "Z"
- * Coatings, meshes, sinusoidal layers, vector blocks/triangles and Lua
- * defined operations are not provided.
+ *   COATING   p = theta phi thickness delta origin
+ *                                              add_coating (:314-331,
+ *             structure_primitives.cpp:45-96): a point of material `origin`
+ *             (as the operations before this one make it) takes mat_index when,
+ *             stepping by delta along the deposition direction (theta, phi in
+ *             degrees; dz = -cos theta), a point of another material lies
+ *             within thickness, and stepping the other way up to lz meets only
+ *             `origin`.  Needs delta > 0 and theta below 90 degrees (the
+ *             reference's upward march never ends otherwise); the operations
+ *             before a coating may not include another coating here.
+ * Conformal coatings, meshes, sinusoidal layers, vector blocks/triangles and
+ * Lua defined operations are not provided.
  * ------------------------------------------------------------------------- */
 enum { AEFDTD_OP_BLOCK = 0, AEFDTD_OP_SPHERE = 1, AEFDTD_OP_CYLINDER = 2, AEFDTD_OP_CONE = 3,
-       AEFDTD_OP_ELLIPSOID = 4, AEFDTD_OP_LAYER = 5 };
+       AEFDTD_OP_ELLIPSOID = 4, AEFDTD_OP_LAYER = 5, AEFDTD_OP_COATING = 6 };
 
 typedef struct aefdtd_structure_op {
     int    type;        /* AEFDTD_OP_* */

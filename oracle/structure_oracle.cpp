@@ -11,13 +11,17 @@
 //
 // Pins (tests/test_structure.py): integer lattice-point counts of closed balls
 // and discs, half-open blocks and layers, later-wins and the default material,
-// loop images against a rolled grid, flip against a mirrored grid.  Parity
-// unpinned: the axis normalisation in the cylinder and cone constructors for
-// axes that are not along x, y or z (pinned only through axis-aligned cases,
-// where it is exact).
+// loop images against a rolled grid, flip against a mirrored grid, coatings on
+// a flat substrate in closed form (vertical and oblique deposition, the point
+// past the thickness, an overhang's shadow).  Parity unpinned: the axis
+// normalisation in the cylinder and cone constructors for axes that are not
+// along x, y or z (pinned only through axis-aligned cases, where it is
+// exact), and coatings over curved surfaces and on flipped structures beyond
+// what the engine-equals-oracle tests show.
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <numbers>
 #include <vector>
 
 #include "oracle.h"
@@ -130,12 +134,90 @@ double modulus(double v, double m) {   // common/mathUT.h:605-611
     return v;
 }
 
+// The structure's operation stack with Structure::index(x, y, z,
+// restrict_level) (structure.cpp:228-262): flips, folding on looped axes, then
+// the operations below restrict_level, last first, each with its images.
+struct Stack {
+    const aefdtd_structure *s;
+    std::vector<std::unique_ptr<Op>> ops;
+    int index(double x, double y, double z, int level) const {
+        const int px = s->loop[0] != 0, py = s->loop[1] != 0, pz = s->loop[2] != 0;
+        if (s->flip[0]) x = s->lx - x;
+        if (s->flip[1]) y = s->ly - y;
+        if (s->flip[2]) z = s->lz - z;
+        if (px) x = modulus(x, s->lx);
+        if (py) y = modulus(y, s->ly);
+        if (pz) z = modulus(z, s->lz);
+        for (int l = level - 1; l >= 0; l--)
+            for (int i = -px; i <= px; i++)
+                for (int j = -py; j <= py; j++)
+                    for (int k = -pz; k <= pz; k++) {
+                        const int v = ops[l]->index(x + i * s->lx, y + j * s->ly, z + k * s->lz);
+                        if (v != -1) return v;
+                    }
+        return s->default_material;
+    }
+};
+
+// Add_Coating (structure_primitives.cpp:45-96, angles as Degree() converts
+// them, common/mathUT.cpp:84-88): a point of the origin material (under the
+// operations before this one) with a support of another material within
+// `thickness` along the deposition direction, and nothing but the origin
+// material from it back up to lz.  The loops test their points exactly as
+// the reference's do (the first one point past `thickness`).
+struct Coating : Op {
+    const Stack *parent;
+    int level;
+    V3 dir;
+    double thickness, delta;
+    int origin, mat;
+    Coating(const double *p, int m, const Stack *parent_, int level_)
+        : parent(parent_), level(level_), thickness(p[2]), delta(p[3]), origin((int)p[4]), mat(m) {
+        double theta = p[0], phi = p[1];
+        theta *= std::numbers::pi / 180.0;
+        phi *= std::numbers::pi / 180.0;
+        dir = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta), -std::cos(theta)};
+    }
+    int index(double x, double y, double z) const override {
+        if (parent->index(x, y, z, level) != origin) return -1;
+        const V3 O{x, y, z};
+        int i = 0;
+        V3 disp = O;
+        const double lz = parent->s->lz;
+        bool support = false;
+        auto nsq = [&](const V3 &d) {
+            const double dx = d.x - O.x, dy = d.y - O.y, dz = d.z - O.z;
+            return dx * dx + dy * dy + dz * dz;
+        };
+        while (nsq(disp) <= thickness * thickness && disp.z > 0) {
+            const double a = i * delta;
+            disp = {O.x + a * dir.x, O.y + a * dir.y, O.z + a * dir.z};
+            if (parent->index(disp.x, disp.y, disp.z, level) != origin) {
+                support = true;
+                break;
+            }
+            i++;
+        }
+        if (!support) return -1;
+        i = 0;
+        disp = O;
+        while (disp.z <= lz) {
+            const double a = i * delta;
+            disp = {O.x - a * dir.x, O.y - a * dir.y, O.z - a * dir.z};
+            if (parent->index(disp.x, disp.y, disp.z, level) != origin) return -1;
+            i++;
+        }
+        return mat;
+    }
+};
+
 }  // namespace
 
 extern "C" int aeoracle_discretize(const aefdtd_structure *s, int Nx, int Ny, int Nz, double Dx, double Dy, double Dz,
                                    uint32_t *matsgrid) {
     if (!s || !matsgrid || Nx < 1 || Ny < 1 || Nz < 1) return AEFDTD_EINVAL;
-    std::vector<std::unique_ptr<Op>> ops;
+    Stack st{s, {}};
+    auto &ops = st.ops;
     for (int l = 0; l < s->n_ops; l++) {
         const aefdtd_structure_op &q = s->ops[l];
         switch (q.type) {
@@ -145,26 +227,14 @@ extern "C" int aeoracle_discretize(const aefdtd_structure *s, int Nx, int Ny, in
         case AEFDTD_OP_CONE: ops.emplace_back(new Axial(q.p, q.mat_index, true)); break;
         case AEFDTD_OP_ELLIPSOID: ops.emplace_back(new Ellipsoid(q.p, q.mat_index)); break;
         case AEFDTD_OP_LAYER: ops.emplace_back(new Layer(q.p, q.mat_index)); break;
+        case AEFDTD_OP_COATING:
+            if (!(q.p[3] > 0) || !(std::cos(q.p[0] * (std::numbers::pi / 180.0)) > 0)) return AEFDTD_EINVAL;
+            ops.emplace_back(new Coating(q.p, q.mat_index, &st, l));
+            break;
         default: return AEFDTD_EINVAL;
         }
     }
-    const int px = s->loop[0] != 0, py = s->loop[1] != 0, pz = s->loop[2] != 0;
-    auto index = [&](double x, double y, double z) -> int {   // Structure::index (structure.cpp:228-262)
-        if (s->flip[0]) x = s->lx - x;
-        if (s->flip[1]) y = s->ly - y;
-        if (s->flip[2]) z = s->lz - z;
-        if (px) x = modulus(x, s->lx);
-        if (py) y = modulus(y, s->ly);
-        if (pz) z = modulus(z, s->lz);
-        for (int l = (int)ops.size() - 1; l >= 0; l--)
-            for (int i = -px; i <= px; i++)
-                for (int j = -py; j <= py; j++)
-                    for (int k = -pz; k <= pz; k++) {
-                        const int v = ops[l]->index(x + i * s->lx, y + j * s->ly, z + k * s->lz);
-                        if (v != -1) return v;
-                    }
-        return s->default_material;
-    };
+    auto index = [&](double x, double y, double z) { return st.index(x, y, z, (int)ops.size()); };
     for (int i = 0; i < Nx; i++)      // Structure::discretize (structure.cpp:94-113)
         for (int j = 0; j < Ny; j++)
             for (int k = 0; k < Nz; k++)

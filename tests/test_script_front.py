@@ -145,6 +145,10 @@ def test_reference_sample_structures():
     st.params["h"] = 80e-9
     s = st.finalize()
     assert s.lz == 200e-9 + 80e-9 and s.ops[0][0] == capi.OP_CYLINDER and s.ops[0][2][5] == 80e-9
+    # the coated variant: add_coating(60, 20, 20e-9, 1e-9, 0, 3) after the rods
+    s = Structure(os.path.join(REF, "samples/fdtd/structures/cyl_var_h_var_r_var_p_coated.lua")).finalize()
+    assert s.ops[-1][0] == capi.OP_COATING and s.ops[-1][1] == 3
+    assert s.ops[-1][2][:5] == [60.0, 20.0, 20e-9, 1e-9, 0.0]
 
 
 @needs_ref

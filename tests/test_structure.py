@@ -101,6 +101,61 @@ def test_flip_mirrors(oracle_be):
     assert np.array_equal(flipped, plain[::-1, :, ::-1])
 
 
+def coated_flat(theta, thickness, delta, overhang=False):
+    """A substrate (layer z < 2) under a coating deposited at polar angle
+    theta; with `overhang`, a slab z in [6, 7) over x < 4 that shadows the
+    substrate below it.  Unit-ish cells: 8 x 8 x 40 corners, Dz = 0.25."""
+    st = capi.Structure(8.0, 8.0, 10.0).add_layer("z", -1.0, 2.0, 1)
+    if overhang:
+        st.add_block(0.0, 4.0, -1.0, 9.0, 6.0, 7.0, 3)
+    return st.add_coating(theta, 0, thickness, delta, 0, 2)
+
+
+def expected_coating(n_steps, cos_t, delta, overhang=False):
+    """Closed form of Add_Coating on coated_flat: air corners whose support
+    march (points i delta apart along dir, up to the first one past the
+    thickness, i.e. n_steps of them below the start) reaches the substrate or
+    the slab top, and whose climb meets no slab."""
+    z = np.arange(40) * 0.25
+    x = np.arange(8)
+    g = np.zeros((40, 8, 8), dtype=np.uint32)
+    g[z < 2.0] = 1
+    reach = n_steps * delta * cos_t
+    coat = (z >= 2.0) & (z - reach < 2.0)
+    g[coat] = 2
+    if overhang:
+        under = x < 4
+        slab = (z >= 6.0) & (z < 7.0)
+        g[np.ix_(coat, np.ones(8, bool), under)] = 0                  # shadowed by the slab
+        g[np.ix_(slab, np.ones(8, bool), under)] = 3
+        top = (z >= 7.0) & (z - reach < 7.0)
+        g[np.ix_(top, np.ones(8, bool), under)] = 2
+    return g
+
+
+@pytest.mark.parametrize("theta,thickness,delta,n_steps", [
+    (0, 1.5, 0.5, 4),       # 1.5 / 0.5 steps within the thickness, then one point past it
+    (0, 1.4, 0.5, 3),
+    (45, 1.5, 0.5, 4),      # reach 4 x 0.5 x cos 45 = 1.414: corners up to 3.25
+])
+def test_coating_flat_substrate_closed_form(oracle_be, theta, thickness, delta, n_steps):
+    """add_coating (structure_primitives.cpp:45-96): on a flat substrate the
+    coating is the air within n_steps x delta x cos(theta) of its top, where
+    n_steps counts the support march's points, the first past `thickness`
+    included (a march that stopped at the thickness would give 3 steps in the
+    first case).  Angles in degrees, dir_z = -cos(theta)."""
+    g = grid(oracle_be, coated_flat(theta, thickness, delta), (8, 8, 40), (1.0, 1.0, 0.25))
+    want = expected_coating(n_steps, np.cos(np.radians(theta)), delta)
+    assert np.array_equal(g, want)
+
+
+def test_coating_is_shadowed_by_an_overhang(oracle_be):
+    """The climb back up to lz: substrate below a slab is not coated, the slab's
+    top is."""
+    g = grid(oracle_be, coated_flat(0, 1.5, 0.5, overhang=True), (8, 8, 40), (1.0, 1.0, 0.25))
+    assert np.array_equal(g, expected_coating(4, 1.0, 0.5, overhang=True))
+
+
 def test_benchmark_sphere_voxeliser_matches_oracle(oracle_be):
     n, D = 48, 2e-9
     sph = workloads.random_spheres(9, 12, (n * D,) * 3, 3e-9, 9e-9)
@@ -188,8 +243,48 @@ def test_engine_benchmark_packing(engine_be):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("theta,thickness,delta,n_steps", [(0, 1.5, 0.5, 4), (0, 1.4, 0.5, 3), (45, 1.5, 0.5, 4)])
+def test_engine_coating_closed_form(engine_be, theta, thickness, delta, n_steps):
+    print(f"discretize coating theta={theta} thickness={thickness}", flush=True)
+    g = grid(engine_be, coated_flat(theta, thickness, delta), (8, 8, 40), (1.0, 1.0, 0.25))
+    assert np.array_equal(g, expected_coating(n_steps, np.cos(np.radians(theta)), delta))
+    g = grid(engine_be, coated_flat(0, 1.5, 0.5, overhang=True), (8, 8, 40), (1.0, 1.0, 0.25))
+    assert np.array_equal(g, expected_coating(4, 1.0, 0.5, overhang=True))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("loop,flip", [((0, 0, 0), (0, 0, 0)), ((1, 1, 0), (0, 0, 0)), ((0, 0, 0), (1, 1, 0)),
+                                       ((1, 0, 1), (0, 1, 0)), ((0, 0, 0), (0, 0, 1))])
+def test_engine_coating_matches_oracle(engine_be, oracle_be, loop, flip):
+    """Cylinders standing on a substrate, coated at 60 degrees from the normal
+    and 20 degrees of azimuth with 1 nm steps (shadowed sides and tops), on
+    plain, looped and flipped axes: the engine equals the oracle bit for bit.
+    With z flipped the coating is empty: Add_Coating's own Structure::index
+    calls flip the already flipped points back, as in the reference."""
+    print(f"discretize coating loop={loop} flip={flip}", flush=True)
+    L, N, D = (200e-9, 200e-9, 250e-9), (40, 40, 50), (5e-9, 5e-9, 5e-9)
+    s = capi.Structure(*L).loop(*loop).flip(*flip)
+    s.add_layer("z", -50e-9, 100e-9, 1)
+    s.add_cylinder(60e-9, 60e-9, 100e-9, 0, 0, 60e-9, 30e-9, 2)
+    s.add_cylinder(150e-9, 130e-9, 100e-9, 0, 0, 90e-9, 25e-9, 2)
+    s.add_coating(60, 20, 20e-9, 1e-9, 0, 3)
+    s.add_sphere(100e-9, 170e-9, 190e-9, 15e-9, 4)
+    e = grid(engine_be, s, N, D)
+    o = grid(oracle_be, s, N, D)
+    assert np.array_equal(e, o)
+    assert (int((o == 3).sum()) > 100) == (flip[2] == 0) and int((o == 4).sum()) > 0
+
+
+@pytest.mark.gpu
 def test_engine_rejects_malformed(engine_be):
     with pytest.raises(capi.FDTDError):
         grid(engine_be, capi.Structure().add_cylinder(1, 1, 1, 0, 0, 0, 1, 1), (4, 4, 4))
+    # a coating deposited from 90 degrees or more never ends its climb; one
+    # over another coating is not provided
+    with pytest.raises(capi.FDTDError):
+        grid(engine_be, capi.Structure(4, 4, 4).add_layer("z", 0, 1, 1).add_coating(90, 0, 1, 0.5, 0, 2), (4, 4, 4))
+    with pytest.raises(capi.FDTDError):
+        grid(engine_be, capi.Structure(4, 4, 4).add_coating(0, 0, 1, 0.5, 0, 2).add_coating(0, 0, 1, 0.5, 0, 3),
+             (4, 4, 4))
     with pytest.raises(capi.FDTDError):
         grid(engine_be, capi.Structure(0, 1, 1).loop(1, 0, 0), (4, 4, 4))
