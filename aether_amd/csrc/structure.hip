@@ -12,8 +12,8 @@
 // operations in the reference's order, and FP contraction is off for the
 // whole file, so no multiply-add is fused where the reference rounds twice.
 // Derived operation constants (block corner order, cylinder/cone axis and
-// length, bounding boxes) are computed on the host exactly as the reference's
-// constructors compute them.
+// length, bounding boxes, a coating's deposition direction) are computed on
+// the host exactly as the reference's constructors compute them.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
