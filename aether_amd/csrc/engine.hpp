@@ -129,9 +129,11 @@ void batch_profile(Batch *b, int on, double *ms, long *chunks);
 // DFT staging depth of a grid stepped on its own (steps between accumulator
 // flushes): the matrix-core flushes stream their samples, so the depth only
 // sets how often the FP64 accumulators are read and written (and the stage's
-// size: 256 x 6 x plane cells x the arithmetic type).  Batches flush every
-// 64-step chunk.
-constexpr int stage_depth(int) { return 256; }
+// size: depth x 6 x plane cells x the arithmetic type).  FP32 arithmetic
+// stages 512 steps; the FP64 flush is bound by the FP64 matrix rate, so a
+// deeper stage would only cost memory there.  Batches flush every 64-step
+// chunk.
+constexpr int stage_depth(int precision) { return precision == AEFDTD_FP64 ? 256 : 512; }
 
 struct Sensor;      // planar spectral Poynting sensor (engine.hip)
 struct FieldSensor; // single-wavelength FieldBlock / FieldMap (engine.hip)

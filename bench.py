@@ -36,8 +36,9 @@ KS = 64                 # steps per batch chunk (and the batches' DFT flush)
 
 def stage_depth(prec):
     """DFT staging depth of a grid stepped on its own (steps per accumulator
-    flush): 256 in every precision."""
-    return 256
+    flush): 512 for FP32 arithmetic, 256 for FP64."""
+    from aether_amd import capi
+    return 256 if prec == capi.FP64 else 512
 
 
 def host_cpu_share():

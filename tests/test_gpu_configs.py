@@ -113,7 +113,7 @@ def test_sensors_many_wavelengths(engine_be, oracle_be, Nl):
 @pytest.mark.parametrize("Nl,Nt", [(8, 20), (17, 37), (50, 300), (128, 513)])
 def test_fp32_matrix_core_flush_against_oracle(engine_be, oracle_be, Nl, Nt):
     """The FP32 flush (k_dft_mx: split-bf16 matrix products of 16 steps, 32
-    wavelength columns per block, blocks of 4, 256-step staging) on runs whose
+    wavelength columns per block, blocks of 4, 512-step staging) on runs whose
     last flush is partial and whose wavelength count fills 1, 2, 4 and 8
     blocks, some partly: FP32 spectra within 1e-4 of their peak of the FP64
     oracle.  Runs of fewer steps than the flush's prefetch depth end their

@@ -205,7 +205,8 @@ def test_estimate_bytes_sensor_terms():
     """A plane spectral-Poynting sensor adds its FP64 accumulators (6 x Nl x P
     complex), the FP64 coefficient table (Nt x E|H x Nl complex), the matrix
     flush's copy of it in the arithmetic type (Nt x E|H x LP, LP = 2 Nl
-    rounded up to 32) and the 256-step stage of 6 samples per plane cell."""
+    rounded up to 32) and the stage of 6 samples per plane cell (256 steps
+    deep for FP64 arithmetic, 512 otherwise)."""
     be = capi.engine()
     g = np.zeros((6, 5, 7), dtype=np.uint32)
     Nt, Nl = 10, 10
@@ -215,7 +216,8 @@ def test_estimate_bytes_sensor_terms():
         spec = capi.ModeSpec(matsgrid=g, materials=[capi.Material(2.0)], Nt=Nt, precision=prec)
         NA = 64 * 5 * 6
         b = 6 * NA * ss + NA + 6 * (7 + 5 + 6) * cs
-        b += 6 * Nl * P * 16 + Nt * 2 * Nl * 16 + Nt * 2 * LP * cs + 256 * 6 * P * cs
+        depth = 256 if prec == capi.FP64 else 512
+        b += 6 * Nl * P * 16 + Nt * 2 * Nl * 16 + Nt * 2 * LP * cs + depth * 6 * P * cs
         assert capi.estimate_bytes(be, capi.KIND_CUSTOM, spec, sens) == b + b // 50 + (8 << 20), prec
 
 
