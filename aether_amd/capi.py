@@ -186,6 +186,7 @@ def _sig(lib, p):
         "server_stats": (i, [vp, _dp]),
         "estimate_bytes": (i, [i, C.POINTER(CMode), i, C.POINTER(CSensorReq), C.POINTER(C.c_size_t)]),
         "stream_bandwidth": (i, [C.c_size_t, i, _dp]),
+        "stream_mix_bandwidth": (i, [C.c_size_t, i, _dp]),
         "discretize": (i, [C.POINTER(CStructure), i, i, i, d, d, d, C.POINTER(C.c_uint32)]),
         # oracle only
         "step": (i, [vp, i]),

@@ -2891,6 +2891,51 @@ int aefdtd_stream_bandwidth(size_t bytes, int reps, double out[3]) {
     return rc;
 }
 
+int aefdtd_stream_mix_bandwidth(size_t bytes, int reps, double out[2]) {
+    if (!out || bytes < (1u << 20) || reps < 1) return AEFDTD_EINVAL;
+    const size_t n = bytes / sizeof(float);
+    float *buf[9] = {};
+    float **ptrs = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = AEFDTD_OK;
+    bool ok = hipMalloc(&ptrs, 9 * sizeof(float *)) == hipSuccess;
+    for (int i = 0; i < 9 && ok; i++) ok = hipMalloc(&buf[i], n * sizeof(float)) == hipSuccess;
+    if (!ok || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || timing_event(&e0) != hipSuccess ||
+        timing_event(&e1) != hipSuccess) {
+        rc = AEFDTD_EDEVICE;
+    } else {
+        for (int i = 0; i < 9; i++) (void)hipMemsetAsync(buf[i], 0, n * sizeof(float), st);
+        (void)hipMemcpyAsync(ptrs, buf, 9 * sizeof(float *), hipMemcpyHostToDevice, st);
+        double best = 0, best_ms = 0;
+        const unsigned grids[3] = {2048, 8192, 65536};
+        for (unsigned gdim : grids)
+            for (int r = 0; r < reps + 1; r++) {
+                (void)hipEventRecord(e0, st);
+                hipLaunchKernelGGL(k_stream_mix, dim3(gdim), dim3(256), 0, st, (const float *const *)ptrs,
+                                   (float *const *)(ptrs + 6), n);
+                (void)hipEventRecord(e1, st);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (r == 0 || ms <= 0) continue;            // first launch warms up
+                const double gbs = 9.0 * n * sizeof(float) / (ms * 1e-3) / 1e9;
+                if (gbs > best) { best = gbs; best_ms = ms; }
+            }
+        if (hipGetLastError() != hipSuccess) rc = AEFDTD_EDEVICE;
+        out[0] = best;
+        out[1] = best_ms;
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    for (float *p : buf)
+        if (p) (void)hipFree(p);
+    if (ptrs) (void)hipFree(ptrs);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
 int aefdtd_stats(const aefdtd_grid *g, long out[4]) {
     if (!g || !out) return AEFDTD_EINVAL;
     out[0] = (long)g->ncell();

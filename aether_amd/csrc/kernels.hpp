@@ -2771,5 +2771,21 @@ __global__ __launch_bounds__(256) void k_stream_copy(const f32x4 *__restrict__ a
     }
 }
 
+// Streaming reference with the read:write mix of the E and H passes: six
+// arrays read and three written, 4 bytes per lane, non-temporal accesses.
+__global__ __launch_bounds__(256) void k_stream_mix(const float *const *__restrict__ x, float *const *__restrict__ y,
+                                                    size_t n) {
+    const float *x0 = x[0], *x1 = x[1], *x2 = x[2], *x3 = x[3], *x4 = x[4], *x5 = x[5];
+    float *y0 = y[0], *y1 = y[1], *y2 = y[2];
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const float a0 = __builtin_nontemporal_load(x0 + q), a1 = __builtin_nontemporal_load(x1 + q);
+        const float a2 = __builtin_nontemporal_load(x2 + q), a3 = __builtin_nontemporal_load(x3 + q);
+        const float a4 = __builtin_nontemporal_load(x4 + q), a5 = __builtin_nontemporal_load(x5 + q);
+        __builtin_nontemporal_store(a0 + a3, y0 + q);
+        __builtin_nontemporal_store(a1 + a4, y1 + q);
+        __builtin_nontemporal_store(a2 + a5, y2 + q);
+    }
+}
+
 }  // namespace dev
 }  // namespace aefdtd

@@ -686,6 +686,8 @@ def headline_run(args, prec, W, K, P):
     ncell = stats[0]
     bw = np.zeros(3)
     capi.engine().fn("stream_bandwidth")(1 << 31, 3, bw.ctypes.data_as(capi._dp))
+    bwm = np.zeros(2)
+    capi.engine().fn("stream_mix_bandwidth")(128 << 20, 3, bwm.ctypes.data_as(capi._dp))
     run.step(W)
     be = capi.engine()
     if args.tiles is not None or args.step is not None:
@@ -732,6 +734,7 @@ def headline_run(args, prec, W, K, P):
         kname, kbytes, kavg = "k_update_E", e_bytes, e_avg
         pass_bytes = e_bytes + h_bytes
     achieved = kbytes / kavg / 1e9
+    traffic = pmc_traffic(prec, args.n, kname)
     KD = stage_depth(prec)
     step_bytes = pass_bytes + rc_pass_bytes(stats) + dft_bytes_per_step(spec, sensors, lay, stats, KD)
     flush_bytes = dft_flush_bytes(spec, sensors, KD, 8 if prec == capi.FP64 else 4)
@@ -740,8 +743,11 @@ def headline_run(args, prec, W, K, P):
         "value": round(ncell * K / wall / 1e6, 1),
         "ms_per_step": round(wall / K * 1e3, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(prec, args.n, kname),
-                     "kernel": kname, "bytes_per_launch": kbytes, "avg_launch_ms": round(kavg * 1e3, 5)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kname, "bytes_per_launch": kbytes, "avg_launch_ms": round(kavg * 1e3, 5),
+                     **({"moved_GBps": round(traffic / kavg / 1e9, 1),
+                         "moved_vs_stream_mix": round(traffic / kavg / 1e9 / float(bwm[0]), 4)}
+                        if traffic and bwm[0] > 0 else {})},
         "kernels_ms_avg": {**({"step": round(e_avg * 1e3, 5)} if one_pass else
                               {"E": round(e_avg * 1e3, 5), "H": round(h_avg * 1e3, 5)}),
                            "sensor": round(s_ms / max(1, s_n), 5),
@@ -758,6 +764,9 @@ def headline_run(args, prec, W, K, P):
            {"e_pass_GBps": round(e_bytes / e_avg / 1e9, 1), "h_pass_GBps": round(h_bytes / h_avg / 1e9, 1)}),
         "stream_copy_GBps": {"plain": round(float(bw[0]), 1), "nontemporal": round(float(bw[1]), 1),
                              "note": "best 2 GiB device copy (read+write), the achievable-bandwidth reference"},
+        "stream_mix_GBps": {"value": round(float(bwm[0]), 1),
+                            "note": "best streaming kernel with the E/H passes' mix: six 128 MiB arrays read, "
+                                    "three written, 4 B per lane, non-temporal"},
         "setup_s": round(setup_s, 2),
         "tiles": tiles,
         "step_tiles": step_code,

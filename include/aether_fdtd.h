@@ -610,6 +610,12 @@ int aefdtd_get_storage_scale(aefdtd_grid *g, int exps[6]);
  * plain accesses, out[1] GB/s with non-temporal ones, out[2] ms of the best
  * plain copy. */
 int aefdtd_stream_bandwidth(size_t bytes, int reps, double out[3]);
+/* Reference point for the roofline, no counterpart in the reference: the
+ * best bandwidth of a streaming kernel with the E and H passes' read:write
+ * mix, six arrays of `bytes` read and three written (4-byte lanes,
+ * non-temporal accesses), over grid sizes 2048..65536 x 256 lanes; out[0]
+ * GB/s (all nine arrays counted), out[1] ms of the best launch. */
+int aefdtd_stream_mix_bandwidth(size_t bytes, int reps, double out[2]);
 /* Bytes of device memory held by the handle. */
 int aefdtd_device_bytes(const aefdtd_grid *g, size_t *bytes);
 /* out[0]=cells of the full grid, out[1]=dispersive (RC) cells, out[2]=max

@@ -29,6 +29,7 @@ def test_oracle_mirrors_the_grid_abi():
                    "aefdtd_autotune", "aefdtd_set_tiles", "aefdtd_set_step_tiles", "aefdtd_get_step_tiles", "aefdtd_synchronize",
                    "aefdtd_sensor_poynting_noalloc",
                    "aefdtd_set_storage_scale", "aefdtd_get_storage_scale", "aefdtd_stream_bandwidth",
+                   "aefdtd_stream_mix_bandwidth",
                    "aefdtd_sensor_out_len"}
     missing = [s for s in declared_symbols()
                if s not in engine_only and not s.startswith(("aefdtd_run_", "aefdtd_batch_", "aefdtd_server_"))
