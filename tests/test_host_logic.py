@@ -84,6 +84,12 @@ def test_engine_refuses_without_gpu_or_bad_args():
     rc = lib.fn("create")(C.byref(h), 0, 4, 4, 10, 1e-9, 1e-9, 1e-9, 1e-18, capi.MODE_CUSTOM, p6, p6, capi.FP64)
     assert rc == capi.EINVAL
     assert b"dimension" in lib.fn("last_error")(None)
+    # the bandwidth references check their arguments before any device call
+    out = np.zeros(3)
+    for name in ("stream_bandwidth", "stream_mix_bandwidth"):
+        assert lib.fn(name)(1 << 10, 3, out.ctypes.data_as(capi._dp)) == capi.EINVAL
+        assert lib.fn(name)(1 << 20, 0, out.ctypes.data_as(capi._dp)) == capi.EINVAL
+        assert lib.fn(name)(1 << 20, 3, None) == capi.EINVAL
 
 
 def test_discretize_and_run_checks_come_before_the_device():
